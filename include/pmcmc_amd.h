@@ -1,0 +1,153 @@
+/*
+ * pmcmc_amd.h — C ABI of the MI355X (gfx950) DEER-over-MALA library.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status
+ * (PMCMC_OK == 0) and never throws across the boundary; the message of the
+ * last failure on a context is pmcmc_last_error(ctx).
+ *
+ * Memory: every array argument is a DEVICE pointer (hipMalloc'd memory or a
+ * ROCArray/torch tensor's storage) unless a comment says otherwise.  Matrices
+ * are column-major like Julia's: element (i, j) of an m x n matrix is at
+ * i + j*m.  A trajectory is D x T, column t is the state after step t+1.
+ *
+ * Precision: PMCMC_PREC_F64 buffers are double; PMCMC_PREC_F32 and
+ * PMCMC_PREC_BF16_JAC buffers are float (BF16 is internal to the Jacobian
+ * path of the GEMM targets).
+ *
+ * Threading: a context owns one HIP stream on one device; calls on one
+ * context are serialised by the caller (one context per Julia task / chain),
+ * different contexts are independent.
+ */
+#ifndef PMCMC_AMD_H
+#define PMCMC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PMCMC_OK 0
+#define PMCMC_ERR_ARGUMENT 1  /* Julia ArgumentError */
+#define PMCMC_ERR_DIMENSION 2 /* Julia DimensionMismatch */
+#define PMCMC_ERR_HIP 3       /* HIP runtime error */
+#define PMCMC_ERR_ALLOC 4     /* device allocation failed */
+#define PMCMC_ERR_DEVICE 5    /* a kernel reported an error (bounded wait expired) */
+#define PMCMC_ERR_UNSUPPORTED 6
+
+/* ---- enums -------------------------------------------------------------- */
+#define PMCMC_PREC_F64 0
+#define PMCMC_PREC_F32 1
+#define PMCMC_PREC_BF16_JAC 2
+
+#define PMCMC_JAC_STOCH_DIAG 0 /* Hutchinson diagonal, src/DEER/DEER.jl:398-419 */
+#define PMCMC_JAC_DIAG 1       /* exact diagonal, src/DEER/DEER.jl:456-462 */
+
+typedef struct pmcmc_ctx pmcmc_ctx;
+typedef struct pmcmc_target pmcmc_target;
+typedef struct pmcmc_deer_ws pmcmc_deer_ws;
+
+/* ---- context ------------------------------------------------------------ */
+/* stream: a hipStream_t, or NULL for a stream the library creates.          */
+int pmcmc_ctx_create(int device, void* stream, pmcmc_ctx** out);
+int pmcmc_ctx_destroy(pmcmc_ctx* ctx);
+const char* pmcmc_last_error(const pmcmc_ctx* ctx);
+int pmcmc_ctx_synchronize(pmcmc_ctx* ctx);
+/* Library version string (static storage). */
+const char* pmcmc_version(void);
+
+/* ---- device buffers (for hosts without their own device arrays) --------- */
+int pmcmc_buffer_alloc(pmcmc_ctx* ctx, int64_t nbytes, void** out);
+int pmcmc_buffer_free(pmcmc_ctx* ctx, void* ptr);
+/* host <-> device copies; `host` is a HOST pointer.  Synchronous. */
+int pmcmc_buffer_upload(pmcmc_ctx* ctx, void* dst_device, const void* host, int64_t nbytes);
+int pmcmc_buffer_download(pmcmc_ctx* ctx, void* host, const void* src_device, int64_t nbytes);
+
+/* ---- diagonal affine scan ----------------------------------------------
+ * S[:,t] = A[:,t] .* S[:,t-1] + B[:,t], S[:,0] uses s0.  Single pass with
+ * decoupled look-back.  Replaces DEERScan.solve_affine_scan_diag!
+ * (src/DEER/DEERScan.jl:102-173); S must alias neither A nor B (:124-125).
+ * T == 0 is a no-op.                                                        */
+int pmcmc_scan_diag_f32(pmcmc_ctx* ctx, const float* A, const float* B, const float* s0,
+                        float* S, int64_t D, int64_t T);
+int pmcmc_scan_diag_f64(pmcmc_ctx* ctx, const double* A, const double* B, const double* s0,
+                        double* S, int64_t D, int64_t T);
+
+/* ---- targets -------------------------------------------------------------
+ * Kernels cannot call user closures, so a DensityModel carries one of these
+ * descriptors (the analytic counterpart of its logdensity / grad / hvp
+ * callbacks, src/interface.jl:36-70).  Arrays are copied into the target at
+ * creation; `dtype_prec` is the PMCMC_PREC_* of the buffers passed in.     */
+
+/* N(mean, diag(1/prec)); prec/mean may be NULL (1 and 0: the standard normal). */
+int pmcmc_target_diag_gaussian(pmcmc_ctx* ctx, int64_t D, int dtype_prec, const void* prec,
+                               const void* mean, pmcmc_target** out);
+/* Bayesian logistic regression with prior N(0, prior_scale^2 I);
+ * Xd is N x D column-major, y has N entries in {0,1}.
+ * benchmarks/ParallelMCMCBenchmarks/src/models/bayes_logreg.jl:91-206.      */
+int pmcmc_target_logistic(pmcmc_ctx* ctx, int64_t N, int64_t D, int dtype_prec, const void* Xd,
+                          const void* y, double prior_scale, pmcmc_target** out);
+int pmcmc_target_destroy(pmcmc_target* tgt);
+int64_t pmcmc_target_dim(const pmcmc_target* tgt);
+
+/* ---- batched MALA step + surrogate JVP -----------------------------------
+ * Per column t: the taped MALA step from X[:,t] with noise Xi[:,t] and
+ * uniform U[t], and the JVP of the sigmoid surrogate along Z[:,t].
+ * Replaces MALA.mala_step_batched_fwd_and_jvp! (src/MALA/MALA.jl:556-628),
+ * the fwd_and_jvp_batch(Xbar, Z) plug-in of DEER.TapedRecursion
+ * (src/DEER/DEER.jl:17-38).  accepted[t] = (log U[t] < logalpha[t]).
+ * logalpha may be NULL; it is double for every precision.                  */
+int pmcmc_mala_fwd_and_jvp(pmcmc_ctx* ctx, const pmcmc_target* tgt, int prec, double eps,
+                           const void* X, const void* Xi, const void* U, const void* Z, void* FT,
+                           void* Jt, uint8_t* accepted, double* logalpha, int64_t D, int64_t T);
+
+/* ---- DEER Newton solve ----------------------------------------------------
+ * Replaces DEER.solve (src/DEER/DEER.jl:506-595) on the taped MALA
+ * recursion built by _build_mala_deer_rec (src/interface.jl:360-473):
+ * S starts at s0 in every column (or at `init`), each iteration is one
+ * deer_update! (:375-475), metric = max|S_new-S| / (tol_abs + tol_rel
+ * max|S_new|), stop at metric <= 1 or maxiter.  Probes are generated on the
+ * device from (seed, iteration, probe, t*D+d) instead of the host RNG.     */
+typedef struct pmcmc_deer_params {
+    double eps;
+    double tol_abs;
+    double tol_rel;
+    double damping; /* in (0, 1] */
+    int32_t maxiter;
+    int32_t jacobian; /* PMCMC_JAC_* */
+    int32_t probes;   /* >= 1; used by PMCMC_JAC_STOCH_DIAG */
+    int32_t precision; /* PMCMC_PREC_* */
+    uint64_t seed;    /* probe stream */
+} pmcmc_deer_params;
+
+typedef struct pmcmc_deer_info {
+    int32_t converged;
+    int32_t iters;
+    double metric; /* metric of the last iteration */
+} pmcmc_deer_info;
+
+/* Workspace for chains of shape D x T at one precision (DEERWorkspace,
+ * src/DEER/DEER.jl:47-90).  Reused across solves of the same shape.        */
+int pmcmc_deer_ws_create(pmcmc_ctx* ctx, int64_t D, int64_t T, int precision,
+                         pmcmc_deer_ws** out);
+int pmcmc_deer_ws_destroy(pmcmc_deer_ws* ws);
+
+/* x0: D; Xi: D x T; U: T; init: D x T or NULL; S_out: D x T (may not alias
+ * inputs); accepted_out: T bytes or NULL (hard accept mask of the final
+ * iteration); info: HOST pointer or NULL.  Synchronises the context stream. */
+int pmcmc_deer_solve(pmcmc_ctx* ctx, pmcmc_deer_ws* ws, const pmcmc_target* tgt,
+                     const pmcmc_deer_params* params, const void* x0, const void* Xi,
+                     const void* U, const void* init, void* S_out, uint8_t* accepted_out,
+                     pmcmc_deer_info* info);
+
+/* Trajectory log-densities logp(S[:,t]) (src/interface.jl:517-524),
+ * written as double.                                                        */
+int pmcmc_target_logdensity_batch(pmcmc_ctx* ctx, const pmcmc_target* tgt, int prec,
+                                  const void* S, double* out, int64_t T);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PMCMC_AMD_H */

@@ -1,0 +1,62 @@
+"""Batched taped MALA step + surrogate JVP on the GPU.
+
+Mirrors MALA.mala_step_batched_fwd_and_jvp! (src/MALA/MALA.jl:556-628): for
+every column t, the exact taped MALA step from X[:,t] (noise Xi[:,t], uniform
+U[t]) and the JVP of the sigmoid surrogate along Z[:,t].  The target's
+gradient / HVP / log-density are the GPU target descriptor's closed forms
+instead of user closures.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _device, _lib
+from ._lib import ArgumentError, DimensionMismatch
+from .interface import _precision_of, _target_of
+
+
+def mala_step_batched_fwd_and_jvp(model, X, eps, Xi, U, Z, precision=None, device=None, return_logalpha=False):
+    """Returns (FT, Jt, accepted) as device tensors (accepted is uint8)."""
+    tgt = _target_of(model)
+    prec = _precision_of(eps, precision)
+    D, T = X.shape
+    if tuple(Xi.shape) != (D, T):
+        raise DimensionMismatch("X and Xi must have the same size")
+    if tuple(Z.shape) != (D, T):
+        raise DimensionMismatch("X and Z must have the same size")
+    if len(U) != T:
+        raise DimensionMismatch("u must have length size(X,2)")
+    if D != tgt.dim:
+        raise DimensionMismatch("X rows must equal the model dimension")
+    dev = torch.device("cuda", device if device is not None else torch.cuda.current_device())
+    dt = _device.torch_dtype(prec)
+    X_, Xi_, Z_ = (_device.to_colmajor(M, dt, dev) for M in (X, Xi, Z))
+    U_ = _device.to_vec(U, dt, dev)
+    FT = _device.colmajor_empty(D, T, dt, dev)
+    Jt = _device.colmajor_empty(D, T, dt, dev)
+    acc = torch.empty(T, dtype=torch.uint8, device=dev)
+    la = torch.empty(T, dtype=torch.float64, device=dev)
+    ctx = _device.context(dev.index)
+    ctx.call("pmcmc_mala_fwd_and_jvp", tgt.handle(ctx), prec, float(eps), _device.ptr(X_), _device.ptr(Xi_),
+             _device.ptr(U_), _device.ptr(Z_), _device.ptr(FT), _device.ptr(Jt), _device.ptr(acc),
+             _device.ptr(la), D, T)
+    if return_logalpha:
+        return FT, Jt, acc, la
+    return FT, Jt, acc
+
+
+def logdensity_batch(model, S, precision=None, device=None):
+    """logp of every column of S as float64 (src/interface.jl:517-524)."""
+    tgt = _target_of(model)
+    prec = _precision_of(1.0, precision if precision is not None else
+                         (_lib.PREC_F64 if isinstance(S, torch.Tensor) and S.dtype == torch.float64 else None))
+    if isinstance(S, torch.Tensor) and precision is None:
+        prec = _lib.PREC_F64 if S.dtype == torch.float64 else _lib.PREC_F32
+    D, T = S.shape
+    dev = S.device if isinstance(S, torch.Tensor) else torch.device("cuda", device if device is not None else torch.cuda.current_device())
+    S_ = S if (isinstance(S, torch.Tensor) and _device.is_colmajor(S) and S.dtype == _device.torch_dtype(prec)) else \
+        _device.to_colmajor(S, _device.torch_dtype(prec), dev)
+    out = torch.empty(T, dtype=torch.float64, device=dev)
+    ctx = _device.context(dev.index)
+    ctx.call("pmcmc_target_logdensity_batch", tgt.handle(ctx), prec, _device.ptr(S_), _device.ptr(out), T)
+    return out

@@ -1,0 +1,40 @@
+"""ParallelMCMC on MI355X: DEER's Newton solve over taped MALA trajectories,
+run by hand-written gfx950 HIP kernels behind the C ABI of
+include/pmcmc_amd.h.
+
+Surface mirrored from the reference package (src/ParallelMCMC.jl:32-41):
+``DensityModel``, ``ParallelMALASampler``, ``sample``, ``step`` and the
+``DEER``, ``DEERScan``, ``MALA`` submodules.
+"""
+from . import _lib
+from ._lib import ArgumentError, DimensionMismatch, PmcmcError
+from .interface import (
+    DensityModel,
+    ParallelMALABlockSamples,
+    ParallelMALASampler,
+    ParallelMALAState,
+    ParallelMALATransition,
+    sample,
+    step,
+)
+from .targets import DiagGaussianTarget, LogisticTarget, StdNormalTarget
+from . import DEER, DEERScan, MALA  # noqa: E402
+
+__all__ = [
+    "DensityModel",
+    "ParallelMALASampler",
+    "ParallelMALABlockSamples",
+    "ParallelMALAState",
+    "ParallelMALATransition",
+    "sample",
+    "step",
+    "DEER",
+    "DEERScan",
+    "MALA",
+    "DiagGaussianTarget",
+    "StdNormalTarget",
+    "LogisticTarget",
+    "ArgumentError",
+    "DimensionMismatch",
+    "PmcmcError",
+]

@@ -1,0 +1,126 @@
+"""ctypes binding of libpmcmc_amd.so (the C ABI in include/pmcmc_amd.h).
+
+The library is the only compute path: if it is missing, importing the
+compute entry points raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.environ.get("PMCMC_AMD_LIB", os.path.join(_HERE, "libpmcmc_amd.so"))
+
+PMCMC_OK = 0
+PMCMC_ERR_ARGUMENT = 1
+PMCMC_ERR_DIMENSION = 2
+PMCMC_ERR_HIP = 3
+PMCMC_ERR_ALLOC = 4
+PMCMC_ERR_DEVICE = 5
+PMCMC_ERR_UNSUPPORTED = 6
+
+PREC_F64 = 0
+PREC_F32 = 1
+PREC_BF16_JAC = 2
+
+JAC_STOCH_DIAG = 0
+JAC_DIAG = 1
+
+
+class ArgumentError(ValueError):
+    """Julia's ArgumentError."""
+
+
+class DimensionMismatch(ValueError):
+    """Julia's DimensionMismatch."""
+
+
+class PmcmcError(RuntimeError):
+    pass
+
+
+class DeerParams(C.Structure):
+    _fields_ = [
+        ("eps", C.c_double),
+        ("tol_abs", C.c_double),
+        ("tol_rel", C.c_double),
+        ("damping", C.c_double),
+        ("maxiter", C.c_int32),
+        ("jacobian", C.c_int32),
+        ("probes", C.c_int32),
+        ("precision", C.c_int32),
+        ("seed", C.c_uint64),
+    ]
+
+
+class DeerInfo(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("iters", C.c_int32), ("metric", C.c_double)]
+
+
+_P = C.c_void_p
+_I64 = C.c_int64
+_SIGS = {
+    "pmcmc_version": ([], C.c_char_p),
+    "pmcmc_ctx_create": ([C.c_int, _P, C.POINTER(_P)], C.c_int),
+    "pmcmc_ctx_destroy": ([_P], C.c_int),
+    "pmcmc_last_error": ([_P], C.c_char_p),
+    "pmcmc_ctx_synchronize": ([_P], C.c_int),
+    "pmcmc_buffer_alloc": ([_P, _I64, C.POINTER(_P)], C.c_int),
+    "pmcmc_buffer_free": ([_P, _P], C.c_int),
+    "pmcmc_buffer_upload": ([_P, _P, _P, _I64], C.c_int),
+    "pmcmc_buffer_download": ([_P, _P, _P, _I64], C.c_int),
+    "pmcmc_scan_diag_f32": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
+    "pmcmc_scan_diag_f64": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
+    "pmcmc_target_diag_gaussian": ([_P, _I64, C.c_int, _P, _P, C.POINTER(_P)], C.c_int),
+    "pmcmc_target_logistic": ([_P, _I64, _I64, C.c_int, _P, _P, C.c_double, C.POINTER(_P)], C.c_int),
+    "pmcmc_target_destroy": ([_P], C.c_int),
+    "pmcmc_target_dim": ([_P], _I64),
+    "pmcmc_mala_fwd_and_jvp": ([_P, _P, C.c_int, C.c_double, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64], C.c_int),
+    "pmcmc_deer_ws_create": ([_P, _I64, _I64, C.c_int, C.POINTER(_P)], C.c_int),
+    "pmcmc_deer_ws_destroy": ([_P], C.c_int),
+    "pmcmc_deer_solve": ([_P, _P, _P, C.POINTER(DeerParams), _P, _P, _P, _P, _P, _P, C.POINTER(DeerInfo)], C.c_int),
+    "pmcmc_target_logdensity_batch": ([_P, _P, C.c_int, _P, _P, _I64], C.c_int),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise PmcmcError(
+                    f"{LIB_PATH} not found: build it with `python parallelmcmc.jl_amd/build_lib.py`"
+                )
+            lib = C.CDLL(LIB_PATH)
+            for name, (args, res) in _SIGS.items():
+                fn = getattr(lib, name)
+                fn.argtypes = args
+                fn.restype = res
+            _lib = lib
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(rc, ctx=None, what=""):
+    if rc == PMCMC_OK:
+        return
+    msg = ""
+    if ctx is not None:
+        m = load().pmcmc_last_error(ctx)
+        msg = m.decode() if m else ""
+    text = f"{what}: {msg}" if what else msg
+    if rc == PMCMC_ERR_ARGUMENT:
+        raise ArgumentError(text)
+    if rc == PMCMC_ERR_DIMENSION:
+        raise DimensionMismatch(text)
+    if rc == PMCMC_ERR_UNSUPPORTED:
+        raise NotImplementedError(text)
+    raise PmcmcError(f"status {rc}: {text}")

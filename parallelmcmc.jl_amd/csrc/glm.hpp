@@ -1,0 +1,63 @@
+// Logistic-regression (GLM) target: data layout and the Newton-iteration
+// products computed as GEMMs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "common.hpp"
+
+namespace pmcmc {
+
+struct GlmData {
+    int64_t N = 0, D = 0;
+    int64_t Dp = 0;         // D padded to the MFMA K granularity
+    float* Xrm = nullptr;   // N x Dp row-major (row n contiguous), zero padded
+    float* Xcm = nullptr;   // Dp x N: column d of the design contiguous over n
+    double* Xd64 = nullptr; // N x D column-major copy for the FP64 path
+    float* y = nullptr;
+    double* y64 = nullptr;
+};
+
+struct GlmWork {
+    int64_t D = 0, T = 0;
+    int probes = 0;
+    size_t esz = 0;
+    void* Y = nullptr;   // D x T
+    void* gx = nullptr;  // D x T
+    void* gy = nullptr;  // D x T
+    void* W = nullptr;   // probes x D x T
+    void* hw = nullptr;  // probes x D x T
+    double* lpx = nullptr;
+    double* lpy = nullptr;
+    void* part = nullptr;  // split-N partial sums
+    size_t part_bytes = 0;
+};
+
+int glm_data_create(GlmData* g, hipStream_t s, int64_t N, int64_t D, bool src_f64, const void* Xd,
+                    const void* y, std::string* msg);
+void glm_data_destroy(GlmData* g);
+int glm_work_ensure(GlmWork* w, const GlmData* g, hipStream_t s, int64_t D, int64_t T, int probes,
+                    bool f64, std::string* msg);
+void glm_work_destroy(GlmWork* w);
+
+// Products of one Newton iteration at Xbar = [x0, S_in[:, 0:T-1]]:
+//   gx = grad(Xbar), W_k = Z_k + eps H(Xbar) Z_k, Y = Xbar + eps gx + sqrt(2 eps) Xi,
+//   gy = grad(Y), hw_k = H(Y) W_k, lpx = logp(Xbar), lpy = logp(Y).
+// Early-exits (no-op) once sync->converged is set.
+template <typename T>
+int glm_newton_products(GlmWork* w, const GlmData* g, hipStream_t s, SyncState* sync, int precision,
+                        double eps, double prior_scale, const T* x0, const T* S_in, const T* Xi,
+                        uint64_t seed, int iter, int probes, int64_t D, int64_t Tn,
+                        std::string* msg);
+
+int glm_fwd_and_jvp(const GlmData* g, hipStream_t s, int prec, double eps, double prior_scale,
+                    const void* X, const void* Xi, const void* U, const void* Z, void* FT, void* Jt,
+                    uint8_t* accepted, double* logalpha, int64_t D, int64_t T, std::string* msg);
+
+int glm_logdensity(const GlmData* g, hipStream_t s, int prec, double prior_scale, const void* S,
+                   double* out, int64_t T, std::string* msg);
+
+}  // namespace pmcmc

@@ -1,0 +1,722 @@
+// C ABI of the MI355X DEER-over-MALA library (declared in include/pmcmc_amd.h).
+//
+// Host orchestration of the device-resident Newton loop: the iterate, the
+// tape and every temporary stay in HBM; the host enqueues iterations in
+// growing chunks and reads one convergence word per chunk (the reference
+// syncs two maxima to the host every iteration, src/DEER/DEER.jl:567-571).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/pmcmc_amd.h"
+#include "common.hpp"
+#include "glm.hpp"
+#include "tile_kernel.hpp"
+
+using namespace pmcmc;
+
+struct pmcmc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    // scratch of the standalone scan entry points (grown on demand)
+    SyncState* sync = nullptr;
+    unsigned* flags = nullptr;
+    size_t flags_cap = 0;
+    void* agg = nullptr;
+    size_t agg_cap = 0;
+    SyncState* host_sync = nullptr;  // pinned
+};
+
+enum { TGT_DIAG = 0, TGT_LOGISTIC = 1 };
+
+struct pmcmc_target {
+    pmcmc_ctx* ctx = nullptr;
+    int kind = TGT_DIAG;
+    int64_t D = 0, N = 0;
+    // diagonal Gaussian (both precisions; null = standard normal)
+    float* prec_f = nullptr;
+    float* mean_f = nullptr;
+    double* prec_d = nullptr;
+    double* mean_d = nullptr;
+    // logistic regression
+    GlmData glm;
+    double prior_scale = 1.0;
+};
+
+struct pmcmc_deer_ws {
+    pmcmc_ctx* ctx = nullptr;
+    int64_t D = 0, T = 0;
+    int prec = PMCMC_PREC_F32;
+    size_t esz = 4;
+    void* buf[2] = {nullptr, nullptr};
+    SyncState* sync = nullptr;
+    unsigned* flags = nullptr;
+    void* agg = nullptr;  // agg_a | agg_b | pref
+    int numTiles = 0, TC = 0;
+    // GEMM-target buffers (allocated on first use)
+    GlmWork glmw;
+};
+
+static int fail(pmcmc_ctx* ctx, int code, const char* fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? PMCMC_ERR_ALLOC : PMCMC_ERR_HIP, \
+                        "%s: %s", #call, hipGetErrorString(e_));                            \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+__global__ void reset_sync_kernel(SyncState* s) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        s->ticket = 0u;
+        s->done = 0u;
+        s->err = 0u;
+        s->dmax_bits = 0ull;
+        s->smax_bits = 0ull;
+        s->converged = 0;
+        s->conv_iter = -1;
+        s->last_metric = INFINITY;
+        s->last_iter = -1;
+    }
+}
+
+template <typename T>
+__global__ void init_iterate_kernel(T* S, const T* s0, const T* init, int64_t D, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        S[i] = init ? init[i] : s0[i % D];
+}
+
+template <typename Tin, typename Tout>
+__global__ void convert_kernel(Tout* out, const Tin* in, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (Tout)in[i];
+}
+
+// Standalone batched MALA step + surrogate JVP for diagonal-Hessian targets
+// (src/MALA/MALA.jl:556-628); one group of G lanes per column.
+template <typename T>
+__global__ __launch_bounds__(256) void fwdjvp_diag_kernel(const T* X, const T* Xi, const T* U,
+                                                          const T* Z, const T* prec, const T* mean,
+                                                          T* FT, T* Jt, uint8_t* acc_out,
+                                                          double* la_out, double eps, int64_t D,
+                                                          int64_t Tn, int G) {
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t col0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int64_t ncol_stride = ((int64_t)gridDim.x * blockDim.x) / G;
+    const double sq2e = sqrt(2.0 * eps);
+    const double c_norm = ((double)D / 2.0) * log(4.0 * 3.14159265358979323846 * eps);
+    for (int64_t t = col0; t < Tn; t += ncol_stride) {
+        double s_lpx = 0, s_lpy = 0, s_qyx = 0, s_qxy = 0, d1 = 0, d2 = 0, d3 = 0;
+        for (int64_t d = lane; d < D; d += G) {
+            const size_t gi = (size_t)t * D + d;
+            const double p = prec ? (double)prec[d] : 1.0;
+            const double m = mean ? (double)mean[d] : 0.0;
+            const double x = (double)X[gi];
+            const double gx = -p * (x - m);
+            const double y = x + eps * gx + sq2e * (double)Xi[gi];
+            const double gy = -p * (y - m);
+            const double z = (double)Z[gi];
+            const double w = z + eps * (-p * z);
+            const double r = x - y - eps * gy;
+            const double dr = z - w - eps * (-p * w);
+            s_lpx += -0.5 * p * (x - m) * (x - m);
+            s_lpy += -0.5 * p * (y - m) * (y - m);
+            const double e1 = y - x - eps * gx;
+            s_qyx += e1 * e1;
+            s_qxy += r * r;
+            d1 += gy * w;
+            d2 += gx * z;
+            d3 += r * dr;
+        }
+        s_lpx = group_sum(s_lpx, G);
+        s_lpy = group_sum(s_lpy, G);
+        s_qyx = group_sum(s_qyx, G);
+        s_qxy = group_sum(s_qxy, G);
+        d1 = group_sum(d1, G);
+        d2 = group_sum(d2, G);
+        d3 = group_sum(d3, G);
+        const double la = (s_lpy + (-0.5 * s_qxy / (2.0 * eps) - c_norm)) -
+                          (s_lpx + (-0.5 * s_qyx / (2.0 * eps) - c_norm));
+        const double lu = log((double)U[t]);
+        const double g = 1.0 / (1.0 + exp(-(la - lu)));
+        const bool acc = lu < la;
+        const double dg = g * (1.0 - g) * (d1 - d2 - d3 / (2.0 * eps));
+        if (lane == 0) {
+            if (acc_out) acc_out[t] = acc ? 1 : 0;
+            if (la_out) la_out[t] = la;
+        }
+        for (int64_t d = lane; d < D; d += G) {
+            const size_t gi = (size_t)t * D + d;
+            const double p = prec ? (double)prec[d] : 1.0;
+            const double m = mean ? (double)mean[d] : 0.0;
+            const double x = (double)X[gi];
+            const double y = x + eps * (-p * (x - m)) + sq2e * (double)Xi[gi];
+            const double z = (double)Z[gi];
+            const double w = z + eps * (-p * z);
+            FT[gi] = (T)(acc ? y : x);
+            Jt[gi] = (T)((acc ? w : z) + (y - x) * dg);
+        }
+    }
+}
+
+// Column log-densities of a diagonal Gaussian.
+template <typename T>
+__global__ __launch_bounds__(256) void logp_diag_kernel(const T* S, const T* prec, const T* mean,
+                                                        double* out, int64_t D, int64_t Tn, int G) {
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t col0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int64_t ncol_stride = ((int64_t)gridDim.x * blockDim.x) / G;
+    for (int64_t t = col0; t < Tn; t += ncol_stride) {
+        double s = 0;
+        for (int64_t d = lane; d < D; d += G) {
+            const double p = prec ? (double)prec[d] : 1.0;
+            const double m = mean ? (double)mean[d] : 0.0;
+            const double x = (double)S[(size_t)t * D + d];
+            s += -0.5 * p * (x - m) * (x - m);
+        }
+        s = group_sum(s, G);
+        if (lane == 0) out[t] = s;
+    }
+}
+
+static int group_size(int64_t D) {
+    int G = 1;
+    while (G < D && G < 64) G <<= 1;
+    return G;
+}
+
+static int grid_for(int64_t n, int block = 256, int cap = 8192) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+// Tile shape: TC columns so that TC*D ~ the per-type element budget, with
+// at least ~2 tiles per CU when the problem is large enough.
+template <typename T>
+static void tile_shape(int64_t D, int64_t Tn, bool with_iterate, int* TC, int* P, int* numTiles) {
+    int64_t E = DevTraits<T>::tile_elems;
+    int64_t tc = E / D;
+    if (tc < 1) tc = 1;
+    while (tc > 1 && tile_lds_bytes<T>(D, (int)tc, with_iterate) > 64512) tc >>= 1;
+    // keep >= 512 tiles when that still leaves >= 8 columns per tile
+    while (tc > 8 && (Tn + tc - 1) / tc < 512) tc >>= 1;
+    if (tc > Tn && Tn > 0) tc = Tn;
+    *TC = (int)tc;
+    *P = D >= PMCMC_NT ? 1 : (int)(PMCMC_NT / D);
+    if (*P > tc) *P = (int)tc;
+    if (*P < 1) *P = 1;
+    *numTiles = (int)((Tn + tc - 1) / tc);
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" const char* pmcmc_version(void) { return "pmcmc_amd 0.1.0 (gfx950)"; }
+
+extern "C" int pmcmc_ctx_create(int device, void* stream, pmcmc_ctx** out) {
+    if (!out) return PMCMC_ERR_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return PMCMC_ERR_HIP;
+    pmcmc_ctx* c = new (std::nothrow) pmcmc_ctx();
+    if (!c) return PMCMC_ERR_ALLOC;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return PMCMC_ERR_HIP;
+    }
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return PMCMC_ERR_HIP;
+        }
+        c->own_stream = true;
+    }
+    if (hipMalloc(&c->sync, sizeof(SyncState)) != hipSuccess ||
+        hipMemset(c->sync, 0, sizeof(SyncState)) != hipSuccess ||
+        hipHostMalloc(&c->host_sync, sizeof(SyncState), hipHostMallocDefault) != hipSuccess) {
+        pmcmc_ctx_destroy(c);
+        return PMCMC_ERR_ALLOC;
+    }
+    *out = c;
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_ctx_destroy(pmcmc_ctx* c) {
+    if (!c) return PMCMC_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->sync) hipFree(c->sync);
+    if (c->flags) hipFree(c->flags);
+    if (c->agg) hipFree(c->agg);
+    if (c->host_sync) hipHostFree(c->host_sync);
+    if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return PMCMC_OK;
+}
+
+extern "C" const char* pmcmc_last_error(const pmcmc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int pmcmc_ctx_synchronize(pmcmc_ctx* c) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_buffer_alloc(pmcmc_ctx* c, int64_t nbytes, void** out) {
+    if (!c || !out || nbytes < 0) return fail(c, PMCMC_ERR_ARGUMENT, "bad buffer request");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMalloc(out, nbytes > 0 ? (size_t)nbytes : 1));
+    return PMCMC_OK;
+}
+extern "C" int pmcmc_buffer_free(pmcmc_ctx* c, void* p) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    if (p) HIPCHK(c, hipFree(p));
+    return PMCMC_OK;
+}
+extern "C" int pmcmc_buffer_upload(pmcmc_ctx* c, void* dst, const void* host, int64_t nbytes) {
+    if (!c || nbytes < 0) return fail(c, PMCMC_ERR_ARGUMENT, "bad upload");
+    if (nbytes == 0) return PMCMC_OK;
+    HIPCHK(c, hipMemcpyAsync(dst, host, (size_t)nbytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PMCMC_OK;
+}
+extern "C" int pmcmc_buffer_download(pmcmc_ctx* c, void* host, const void* src, int64_t nbytes) {
+    if (!c || nbytes < 0) return fail(c, PMCMC_ERR_ARGUMENT, "bad download");
+    if (nbytes == 0) return PMCMC_OK;
+    HIPCHK(c, hipMemcpyAsync(host, src, (size_t)nbytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PMCMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// standalone diagonal scan
+// ---------------------------------------------------------------------------
+template <typename T>
+static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, int64_t D,
+                     int64_t Tn) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    if (D < 1 || Tn < 0) return fail(c, PMCMC_ERR_DIMENSION, "scan: need D >= 1 and T >= 0 (D=%lld T=%lld)", (long long)D, (long long)Tn);
+    if (Tn == 0) return PMCMC_OK;
+    if (!A || !B || !s0 || !S) return fail(c, PMCMC_ERR_ARGUMENT, "scan: null pointer");
+    const size_t bytes = (size_t)D * Tn * sizeof(T);
+    auto overlap = [&](const void* p) {
+        const char* a0 = (const char*)p;
+        const char* b0 = (const char*)S;
+        return a0 < b0 + bytes && b0 < a0 + bytes;
+    };
+    if (overlap(A)) return fail(c, PMCMC_ERR_ARGUMENT, "S must not alias A");
+    if (overlap(B)) return fail(c, PMCMC_ERR_ARGUMENT, "S must not alias B");
+    int TC, P, nt;
+    tile_shape<T>(D, Tn, false, &TC, &P, &nt);
+    size_t lds = tile_lds_bytes<T>(D, TC, false);
+    if (lds > 64512) return fail(c, PMCMC_ERR_UNSUPPORTED, "scan: D=%lld too large for one tile", (long long)D);
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((size_t)nt > c->flags_cap) {
+        if (c->flags) HIPCHK(c, hipFree(c->flags));
+        c->flags = nullptr;
+        HIPCHK(c, hipMalloc(&c->flags, (size_t)nt * sizeof(unsigned)));
+        HIPCHK(c, hipMemsetAsync(c->flags, 0, (size_t)nt * sizeof(unsigned), c->stream));
+        c->flags_cap = nt;
+    }
+    const size_t agg_bytes = 3 * (size_t)nt * D * sizeof(T);
+    if (agg_bytes > c->agg_cap) {
+        if (c->agg) HIPCHK(c, hipFree(c->agg));
+        c->agg = nullptr;
+        HIPCHK(c, hipMalloc(&c->agg, agg_bytes));
+        c->agg_cap = agg_bytes;
+    }
+    TileArgs<T> a{};
+    a.A = A;
+    a.B = B;
+    a.s0 = s0;
+    a.S = S;
+    a.D = D;
+    a.T_len = Tn;
+    a.TC = TC;
+    a.numTiles = nt;
+    a.P = P;
+    a.sync = c->sync;
+    a.flags = c->flags;
+    a.agg_a = (T*)c->agg;
+    a.agg_b = a.agg_a + (size_t)nt * D;
+    a.pref = a.agg_b + (size_t)nt * D;
+    a.damping = 1.0;
+    hipLaunchKernelGGL((tile_kernel<T, MODE_SCAN>), dim3(nt), dim3(PMCMC_NT), lds, c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->host_sync, c->sync, sizeof(SyncState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->host_sync->err) return fail(c, PMCMC_ERR_DEVICE, "scan: look-back wait expired");
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_scan_diag_f32(pmcmc_ctx* c, const float* A, const float* B, const float* s0,
+                                   float* S, int64_t D, int64_t T) {
+    return scan_diag<float>(c, A, B, s0, S, D, T);
+}
+extern "C" int pmcmc_scan_diag_f64(pmcmc_ctx* c, const double* A, const double* B,
+                                   const double* s0, double* S, int64_t D, int64_t T) {
+    return scan_diag<double>(c, A, B, s0, S, D, T);
+}
+
+// ---------------------------------------------------------------------------
+// targets
+// ---------------------------------------------------------------------------
+static int copy_vec_both(pmcmc_ctx* c, const void* src, int src_prec, int64_t n, float** f,
+                         double** d) {
+    if (!src) return PMCMC_OK;
+    HIPCHK(c, hipMalloc(f, n * sizeof(float)));
+    HIPCHK(c, hipMalloc(d, n * sizeof(double)));
+    if (src_prec == PMCMC_PREC_F64) {
+        HIPCHK(c, hipMemcpyAsync(*d, src, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0, c->stream, *f, (const double*)src, n);
+    } else {
+        HIPCHK(c, hipMemcpyAsync(*f, src, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(n)), dim3(256), 0, c->stream, *d, (const float*)src, n);
+    }
+    HIPCHK(c, hipGetLastError());
+    return PMCMC_OK;
+}
+
+static bool valid_prec(int p) { return p == PMCMC_PREC_F64 || p == PMCMC_PREC_F32 || p == PMCMC_PREC_BF16_JAC; }
+
+extern "C" int pmcmc_target_diag_gaussian(pmcmc_ctx* c, int64_t D, int dtype_prec, const void* prec,
+                                          const void* mean, pmcmc_target** out) {
+    if (!c || !out) return PMCMC_ERR_ARGUMENT;
+    *out = nullptr;
+    if (D < 1) return fail(c, PMCMC_ERR_DIMENSION, "target dim must be >= 1");
+    if (!valid_prec(dtype_prec)) return fail(c, PMCMC_ERR_ARGUMENT, "bad precision %d", dtype_prec);
+    HIPCHK(c, hipSetDevice(c->device));
+    pmcmc_target* t = new (std::nothrow) pmcmc_target();
+    if (!t) return fail(c, PMCMC_ERR_ALLOC, "host allocation failed");
+    t->ctx = c;
+    t->kind = TGT_DIAG;
+    t->D = D;
+    int rc = copy_vec_both(c, prec, dtype_prec, D, &t->prec_f, &t->prec_d);
+    if (rc == PMCMC_OK) rc = copy_vec_both(c, mean, dtype_prec, D, &t->mean_f, &t->mean_d);
+    if (rc == PMCMC_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, PMCMC_ERR_HIP, "sync failed");
+    if (rc != PMCMC_OK) {
+        pmcmc_target_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_target_logistic(pmcmc_ctx* c, int64_t N, int64_t D, int dtype_prec,
+                                     const void* Xd, const void* y, double prior_scale,
+                                     pmcmc_target** out) {
+    if (!c || !out) return PMCMC_ERR_ARGUMENT;
+    *out = nullptr;
+    if (N < 1 || D < 1) return fail(c, PMCMC_ERR_DIMENSION, "logistic target needs N >= 1 and D >= 1");
+    if (!Xd || !y) return fail(c, PMCMC_ERR_ARGUMENT, "logistic target: null data");
+    if (!(prior_scale > 0)) return fail(c, PMCMC_ERR_ARGUMENT, "prior_scale must be > 0");
+    if (!valid_prec(dtype_prec)) return fail(c, PMCMC_ERR_ARGUMENT, "bad precision %d", dtype_prec);
+    HIPCHK(c, hipSetDevice(c->device));
+    pmcmc_target* t = new (std::nothrow) pmcmc_target();
+    if (!t) return fail(c, PMCMC_ERR_ALLOC, "host allocation failed");
+    t->ctx = c;
+    t->kind = TGT_LOGISTIC;
+    t->D = D;
+    t->N = N;
+    t->prior_scale = prior_scale;
+    std::string msg;
+    int rc = glm_data_create(&t->glm, c->stream, N, D, dtype_prec == PMCMC_PREC_F64, Xd, y, &msg);
+    if (rc != PMCMC_OK) {
+        fail(c, rc, "%s", msg.c_str());
+        pmcmc_target_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_target_destroy(pmcmc_target* t) {
+    if (!t) return PMCMC_OK;
+    if (t->ctx) hipSetDevice(t->ctx->device);
+    if (t->ctx && t->ctx->stream) hipStreamSynchronize(t->ctx->stream);
+    hipFree(t->prec_f);
+    hipFree(t->mean_f);
+    hipFree(t->prec_d);
+    hipFree(t->mean_d);
+    glm_data_destroy(&t->glm);
+    delete t;
+    return PMCMC_OK;
+}
+
+extern "C" int64_t pmcmc_target_dim(const pmcmc_target* t) { return t ? t->D : -1; }
+
+// ---------------------------------------------------------------------------
+// batched MALA step + JVP
+// ---------------------------------------------------------------------------
+extern "C" int pmcmc_mala_fwd_and_jvp(pmcmc_ctx* c, const pmcmc_target* tgt, int prec, double eps,
+                                      const void* X, const void* Xi, const void* U, const void* Z,
+                                      void* FT, void* Jt, uint8_t* accepted, double* logalpha,
+                                      int64_t D, int64_t Tn) {
+    if (!c || !tgt) return fail(c, PMCMC_ERR_ARGUMENT, "null context/target");
+    if (D != tgt->D) return fail(c, PMCMC_ERR_DIMENSION, "X has %lld rows but the target has dim %lld", (long long)D, (long long)tgt->D);
+    if (Tn < 0) return fail(c, PMCMC_ERR_DIMENSION, "T must be >= 0");
+    if (!valid_prec(prec)) return fail(c, PMCMC_ERR_ARGUMENT, "bad precision %d", prec);
+    if (!(eps > 0)) return fail(c, PMCMC_ERR_ARGUMENT, "eps must be > 0");
+    if (Tn == 0) return PMCMC_OK;
+    if (!X || !Xi || !U || !Z || !FT || !Jt) return fail(c, PMCMC_ERR_ARGUMENT, "null buffer");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int G = group_size(D);
+    const int grid = grid_for(Tn * G);
+    if (tgt->kind == TGT_DIAG) {
+        if (prec == PMCMC_PREC_F64)
+            hipLaunchKernelGGL((fwdjvp_diag_kernel<double>), dim3(grid), dim3(256), 0, c->stream,
+                               (const double*)X, (const double*)Xi, (const double*)U, (const double*)Z,
+                               tgt->prec_d, tgt->mean_d, (double*)FT, (double*)Jt, accepted, logalpha,
+                               eps, D, Tn, G);
+        else
+            hipLaunchKernelGGL((fwdjvp_diag_kernel<float>), dim3(grid), dim3(256), 0, c->stream,
+                               (const float*)X, (const float*)Xi, (const float*)U, (const float*)Z,
+                               tgt->prec_f, tgt->mean_f, (float*)FT, (float*)Jt, accepted, logalpha,
+                               eps, D, Tn, G);
+        HIPCHK(c, hipGetLastError());
+    } else {
+        std::string msg;
+        int rc = glm_fwd_and_jvp(&tgt->glm, c->stream, prec, eps, tgt->prior_scale, X, Xi, U, Z, FT,
+                                 Jt, accepted, logalpha, D, Tn, &msg);
+        if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_target_logdensity_batch(pmcmc_ctx* c, const pmcmc_target* tgt, int prec,
+                                             const void* S, double* out, int64_t Tn) {
+    if (!c || !tgt) return fail(c, PMCMC_ERR_ARGUMENT, "null context/target");
+    if (!valid_prec(prec)) return fail(c, PMCMC_ERR_ARGUMENT, "bad precision %d", prec);
+    if (Tn < 0) return fail(c, PMCMC_ERR_DIMENSION, "T must be >= 0");
+    if (Tn == 0) return PMCMC_OK;
+    if (!S || !out) return fail(c, PMCMC_ERR_ARGUMENT, "null buffer");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t D = tgt->D;
+    if (tgt->kind == TGT_DIAG) {
+        const int G = group_size(D);
+        const int grid = grid_for(Tn * G);
+        if (prec == PMCMC_PREC_F64)
+            hipLaunchKernelGGL((logp_diag_kernel<double>), dim3(grid), dim3(256), 0, c->stream,
+                               (const double*)S, tgt->prec_d, tgt->mean_d, out, D, Tn, G);
+        else
+            hipLaunchKernelGGL((logp_diag_kernel<float>), dim3(grid), dim3(256), 0, c->stream,
+                               (const float*)S, tgt->prec_f, tgt->mean_f, out, D, Tn, G);
+        HIPCHK(c, hipGetLastError());
+    } else {
+        std::string msg;
+        int rc = glm_logdensity(&tgt->glm, c->stream, prec, tgt->prior_scale, S, out, Tn, &msg);
+        if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PMCMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// DEER workspace + solve
+// ---------------------------------------------------------------------------
+extern "C" int pmcmc_deer_ws_create(pmcmc_ctx* c, int64_t D, int64_t Tn, int prec,
+                                    pmcmc_deer_ws** out) {
+    if (!c || !out) return PMCMC_ERR_ARGUMENT;
+    *out = nullptr;
+    if (D < 1 || Tn < 1) return fail(c, PMCMC_ERR_DIMENSION, "workspace needs D >= 1 and T >= 1");
+    if (!valid_prec(prec)) return fail(c, PMCMC_ERR_ARGUMENT, "bad precision %d", prec);
+    HIPCHK(c, hipSetDevice(c->device));
+    pmcmc_deer_ws* w = new (std::nothrow) pmcmc_deer_ws();
+    if (!w) return fail(c, PMCMC_ERR_ALLOC, "host allocation failed");
+    w->ctx = c;
+    w->D = D;
+    w->T = Tn;
+    w->prec = prec;
+    w->esz = prec == PMCMC_PREC_F64 ? 8 : 4;
+    int TC, P, nt;
+    if (w->esz == 8)
+        tile_shape<double>(D, Tn, true, &TC, &P, &nt);
+    else
+        tile_shape<float>(D, Tn, true, &TC, &P, &nt);
+    w->TC = TC;
+    w->numTiles = nt;
+    const size_t mat = (size_t)D * Tn * w->esz;
+    bool ok = hipMalloc(&w->buf[0], mat) == hipSuccess && hipMalloc(&w->buf[1], mat) == hipSuccess &&
+              hipMalloc(&w->sync, sizeof(SyncState)) == hipSuccess &&
+              hipMemset(w->sync, 0, sizeof(SyncState)) == hipSuccess &&
+              hipMalloc(&w->flags, (size_t)nt * sizeof(unsigned)) == hipSuccess &&
+              hipMemset(w->flags, 0, (size_t)nt * sizeof(unsigned)) == hipSuccess &&
+              hipMalloc(&w->agg, 3 * (size_t)nt * D * w->esz) == hipSuccess;
+    if (!ok) {
+        pmcmc_deer_ws_destroy(w);
+        return fail(c, PMCMC_ERR_ALLOC, "workspace allocation failed (D=%lld T=%lld)", (long long)D, (long long)Tn);
+    }
+    *out = w;
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_deer_ws_destroy(pmcmc_deer_ws* w) {
+    if (!w) return PMCMC_OK;
+    if (w->ctx) {
+        hipSetDevice(w->ctx->device);
+        hipStreamSynchronize(w->ctx->stream);
+    }
+    hipFree(w->buf[0]);
+    hipFree(w->buf[1]);
+    hipFree(w->sync);
+    hipFree(w->flags);
+    hipFree(w->agg);
+    glm_work_destroy(&w->glmw);
+    delete w;
+    return PMCMC_OK;
+}
+
+template <typename T>
+static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
+                        const pmcmc_deer_params* p, const void* x0, const void* Xi, const void* U,
+                        const void* init, void* S_out, uint8_t* acc_out, pmcmc_deer_info* info) {
+    const int64_t D = w->D, Tn = w->T;
+    const int mode = tgt->kind == TGT_DIAG ? MODE_ELEM : MODE_GEMM;
+    int TC, P, nt;
+    tile_shape<T>(D, Tn, true, &TC, &P, &nt);
+    const size_t lds = tile_lds_bytes<T>(D, TC, true);
+    if (lds > 64512) return fail(c, PMCMC_ERR_UNSUPPORTED, "D=%lld too large for the fused tile kernel", (long long)D);
+    if (mode == MODE_GEMM) {
+        if (p->jacobian != PMCMC_JAC_STOCH_DIAG)
+            return fail(c, PMCMC_ERR_UNSUPPORTED, "logistic target supports jacobian=:stoch_diag only");
+        std::string msg;
+        int rc = glm_work_ensure(&w->glmw, &tgt->glm, c->stream, D, Tn, p->probes, sizeof(T) == 8, &msg);
+        if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+    }
+    T* bufs[2] = {(T*)w->buf[0], (T*)w->buf[1]};
+    hipLaunchKernelGGL(reset_sync_kernel, dim3(1), dim3(64), 0, c->stream, w->sync);
+    hipLaunchKernelGGL((init_iterate_kernel<T>), dim3(grid_for(D * Tn)), dim3(256), 0, c->stream,
+                       bufs[0], (const T*)x0, (const T*)init, D, D * Tn);
+    HIPCHK(c, hipGetLastError());
+
+    TileArgs<T> a{};
+    a.s0 = (const T*)x0;
+    a.D = D;
+    a.T_len = Tn;
+    a.TC = TC;
+    a.numTiles = nt;
+    a.P = P;
+    a.sync = w->sync;
+    a.flags = w->flags;
+    a.agg_a = (T*)w->agg;
+    a.agg_b = a.agg_a + (size_t)nt * D;
+    a.pref = a.agg_b + (size_t)nt * D;
+    a.Xi = (const T*)Xi;
+    a.U = (const T*)U;
+    a.acc_out = acc_out;
+    a.la_out = nullptr;
+    a.eps = p->eps;
+    a.damping = p->damping;
+    a.tol_abs = p->tol_abs;
+    a.tol_rel = p->tol_rel;
+    a.probes = p->probes;
+    a.jacobian = p->jacobian;
+    a.maxiter = p->maxiter;
+    a.seed = p->seed;
+    if (tgt->kind == TGT_DIAG) {
+        a.prec = sizeof(T) == 8 ? (const T*)tgt->prec_d : (const T*)tgt->prec_f;
+        a.mean = sizeof(T) == 8 ? (const T*)tgt->mean_d : (const T*)tgt->mean_f;
+    }
+
+    int k = 0, chunk = 1;
+    bool done = false;
+    while (k < p->maxiter && !done) {
+        const int n = (p->maxiter - k) < chunk ? (p->maxiter - k) : chunk;
+        for (int i = 0; i < n; ++i) {
+            const int it = k + i;
+            a.iter = it;
+            a.S_in = bufs[it & 1];
+            a.S = bufs[(it + 1) & 1];
+            if (mode == MODE_GEMM) {
+                std::string msg;
+                int rc = glm_newton_products(&w->glmw, &tgt->glm, c->stream, w->sync, p->precision,
+                                             p->eps, tgt->prior_scale, (const T*)x0, a.S_in,
+                                             (const T*)Xi, p->seed, it, p->probes, D, Tn, &msg);
+                if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+                a.Y = (const T*)w->glmw.Y;
+                a.gx = (const T*)w->glmw.gx;
+                a.gy = (const T*)w->glmw.gy;
+                a.W = (const T*)w->glmw.W;
+                a.hw = (const T*)w->glmw.hw;
+                a.lpx = w->glmw.lpx;
+                a.lpy = w->glmw.lpy;
+                hipLaunchKernelGGL((tile_kernel<T, MODE_GEMM>), dim3(nt), dim3(PMCMC_NT), lds, c->stream, a);
+            } else {
+                hipLaunchKernelGGL((tile_kernel<T, MODE_ELEM>), dim3(nt), dim3(PMCMC_NT), lds, c->stream, a);
+            }
+            HIPCHK(c, hipGetLastError());
+        }
+        k += n;
+        HIPCHK(c, hipMemcpyAsync(c->host_sync, w->sync, sizeof(SyncState), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->host_sync->err) return fail(c, PMCMC_ERR_DEVICE, "DEER: look-back wait expired");
+        done = c->host_sync->converged != 0;
+        if (chunk < 16) chunk <<= 1;
+    }
+    const SyncState hs = *c->host_sync;
+    const int iters = hs.converged ? hs.conv_iter + 1 : (hs.last_iter + 1);
+    T* result = bufs[iters & 1];
+    if ((void*)result != S_out)
+        HIPCHK(c, hipMemcpyAsync(S_out, result, (size_t)D * Tn * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (info) {
+        info->converged = hs.converged;
+        info->iters = iters;
+        info->metric = hs.last_metric;
+    }
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_deer_solve(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
+                                const pmcmc_deer_params* p, const void* x0, const void* Xi,
+                                const void* U, const void* init, void* S_out, uint8_t* acc_out,
+                                pmcmc_deer_info* info) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    if (!w || !tgt || !p) return fail(c, PMCMC_ERR_ARGUMENT, "null workspace/target/params");
+    // argument validation of src/DEER/DEER.jl:525-528 and :391-392
+    if (p->maxiter < 1) return fail(c, PMCMC_ERR_ARGUMENT, "maxiter must be >= 1");
+    if (!(p->tol_abs >= 0)) return fail(c, PMCMC_ERR_ARGUMENT, "tol_abs must be >= 0");
+    if (!(p->tol_rel >= 0)) return fail(c, PMCMC_ERR_ARGUMENT, "tol_rel must be >= 0");
+    if (!(p->damping > 0 && p->damping <= 1)) return fail(c, PMCMC_ERR_ARGUMENT, "damping must be in (0,1]");
+    if (p->probes < 1) return fail(c, PMCMC_ERR_ARGUMENT, "probes must be >= 1");
+    if (p->probes > PMCMC_MAXP) return fail(c, PMCMC_ERR_UNSUPPORTED, "probes must be <= %d", PMCMC_MAXP);
+    if (p->jacobian != PMCMC_JAC_STOCH_DIAG && p->jacobian != PMCMC_JAC_DIAG)
+        return fail(c, PMCMC_ERR_ARGUMENT, "jacobian must be :diag or :stoch_diag");
+    if (!(p->eps > 0)) return fail(c, PMCMC_ERR_ARGUMENT, "epsilon must be > 0");
+    if (p->precision != w->prec) return fail(c, PMCMC_ERR_ARGUMENT, "params precision %d does not match the workspace (%d)", p->precision, w->prec);
+    if (tgt->D != w->D) return fail(c, PMCMC_ERR_DIMENSION, "target dim %lld != workspace D %lld", (long long)tgt->D, (long long)w->D);
+    if (!x0 || !Xi || !U || !S_out) return fail(c, PMCMC_ERR_ARGUMENT, "null buffer");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (w->prec == PMCMC_PREC_F64)
+        return deer_solve_t<double>(c, w, tgt, p, x0, Xi, U, init, S_out, acc_out, info);
+    return deer_solve_t<float>(c, w, tgt, p, x0, Xi, U, init, S_out, acc_out, info);
+}

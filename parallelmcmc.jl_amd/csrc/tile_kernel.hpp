@@ -1,0 +1,494 @@
+// Single-pass tile kernel: diagonal affine scan with decoupled look-back,
+// optionally fused with the DEER Newton update of a taped MALA recursion.
+//
+// One workgroup owns TC consecutive columns (all D rows) of a column-major
+// D x T problem.  Its input columns are contiguous in HBM, so they are staged
+// into LDS with fully coalesced loads, scanned in LDS, chained to the
+// predecessor tiles through per-tile aggregates (decoupled look-back), and
+// written back with coalesced stores.  HBM traffic is one read of every input
+// and one write of the output: the minimum for the scan.
+//
+// Modes
+//   MODE_SCAN: A, B read from HBM; S written.           (DEERScan.jl:102-173)
+//   MODE_ELEM: A, B built on chip from the iterate S_in, the tape (Xi, U) and
+//              on-chip Rademacher probes, for targets with a diagonal Hessian
+//              (fused MALA.jl:556-628 + DEER.jl:398-419 + scan + damping
+//              DEER.jl:470-473 + convergence maxima DEER.jl:567-571).
+//   MODE_GEMM: as MODE_ELEM but gradients / HVPs / log-densities of the
+//              proposal and current state come precomputed from the GEMM
+//              target kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace pmcmc {
+
+enum { MODE_SCAN = 0, MODE_ELEM = 1, MODE_GEMM = 2 };
+enum { JAC_STOCH = 0, JAC_DIAG = 1 };
+#define PMCMC_MAXP 4
+
+template <typename T>
+struct TileArgs {
+    // scan operands (MODE_SCAN)
+    const T* A;
+    const T* B;
+    // common
+    const T* s0;
+    T* S;  // output
+    int64_t D;
+    int64_t T_len;
+    int TC;        // columns per tile
+    int numTiles;
+    int P;         // parts per row in the in-tile scan
+    SyncState* sync;
+    unsigned* flags;
+    T* agg_a;
+    T* agg_b;
+    T* pref;
+    // DEER (MODE_ELEM / MODE_GEMM)
+    const T* S_in;
+    const T* Xi;
+    const T* U;
+    uint8_t* acc_out;
+    double* la_out;
+    double eps, damping, tol_abs, tol_rel;
+    int iter;
+    int probes;
+    int jacobian;
+    int maxiter;
+    uint64_t seed;
+    // diagonal-Hessian target (nullable: standard normal)
+    const T* prec;
+    const T* mean;
+    // GEMM target products (MODE_GEMM): D x T, probe k of hz/hw at k*D*T
+    const T* Y;
+    const T* gx;
+    const T* gy;
+    const T* W;   // w_k = z_k + eps * H(x) z_k
+    const T* hw;  // H(y) w_k
+    const double* lpx;
+    const double* lpy;
+};
+
+// LDS layout (elements of T):
+//   xs  : (TC+1)*D  iterate columns c0-1 .. c0+TC-1 (Xbar and S_old)
+//   bl  : TC*D      B (MODE_ELEM/GEMM: the tape Xi first, overwritten by B)
+//   al  : TC*D      A, then the output tile
+//   pa,pb: max(NT, D) part aggregates
+//   sin : D         incoming state per row
+//   lka,lkb: D      look-back composition
+//   acc : TC bytes  accept flags
+template <typename T>
+__host__ __device__ inline size_t tile_lds_bytes(int64_t D, int TC, bool with_iterate) {
+    int64_t PA = D > PMCMC_NT ? D : PMCMC_NT;
+    int64_t xs = with_iterate ? ((int64_t)TC + 1) * D : 0;
+    size_t n = (size_t)(xs + 2 * (int64_t)TC * D + 2 * PA + 3 * D);
+    return n * sizeof(T) + (size_t)TC + 64;
+}
+
+template <typename T>
+__device__ inline T wave_max(T v) {
+    for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+// Sum over a group of G lanes (G a power of two <= 64, groups aligned).
+__device__ inline double group_sum(double v, int G) {
+    for (int m = G >> 1; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_tile;
+    __shared__ unsigned s_epoch;
+    __shared__ unsigned s_status;
+    __shared__ double s_red[2][PMCMC_NT / 64];
+
+    const int tid = threadIdx.x;
+    SyncState* sy = a.sync;
+    if (MODE != MODE_SCAN) {
+        // Newton loop already converged: later queued iterations are no-ops.
+        if (*(volatile int*)&sy->converged) return;
+    }
+    if (tid == 0) {
+        s_tile = (int)atomicAdd(&sy->ticket, 1u);
+        s_epoch = *(volatile unsigned*)&sy->epoch;
+    }
+    __syncthreads();
+    const int tile = s_tile;
+    const unsigned epoch = s_epoch;
+    const int64_t D = a.D;
+    const int TC = a.TC;
+    const int64_t c0 = (int64_t)tile * TC;
+    const int tc = (int)((a.T_len - c0) < TC ? (a.T_len - c0) : TC);
+    const int P = a.P;
+    const int64_t PA = D > PMCMC_NT ? D : PMCMC_NT;
+
+    T* xs = reinterpret_cast<T*>(smem);
+    T* bl = xs + (MODE == MODE_SCAN ? (size_t)0 : (size_t)(TC + 1) * D);
+    T* al = bl + (size_t)TC * D;
+    T* pa = al + (size_t)TC * D;
+    T* pb = pa + PA;
+    T* sin_ = pb + PA;
+    T* lka = sin_ + D;
+    T* lkb = lka + D;
+    uint8_t* accl = reinterpret_cast<uint8_t*>(lkb + D);
+
+    const int64_t nel = (int64_t)tc * D;
+    const int64_t gbase = c0 * D;
+
+    // ---- phase 0: stage inputs (coalesced) -------------------------------
+    if (MODE == MODE_SCAN) {
+        for (int64_t i = tid; i < nel; i += PMCMC_NT) {
+            al[i] = a.A[gbase + i];
+            bl[i] = a.B[gbase + i];
+        }
+    } else {
+        const int64_t nx = nel + D;
+        for (int64_t i = tid; i < nx; i += PMCMC_NT) {
+            xs[i] = (c0 == 0 && i < D) ? a.s0[i] : a.S_in[gbase - D + i];
+        }
+        for (int64_t i = tid; i < nel; i += PMCMC_NT) bl[i] = a.Xi[gbase + i];
+    }
+    __syncthreads();
+
+    // ---- phase 1: build A, B per column (MALA step + surrogate JVP) -------
+    if (MODE != MODE_SCAN) {
+        int G = 1;
+        while (G < D && G < 64) G <<= 1;
+        const int lane = tid & (G - 1);
+        const int grp = tid / G;
+        const int ngrp = PMCMC_NT / G;
+        const double eps = a.eps;
+        const double sq2e = sqrt(2.0 * eps);
+        const double c_norm = ((double)D / 2.0) * log(4.0 * 3.14159265358979323846 * eps);
+        const int K = a.probes;
+        uint32_t keys[PMCMC_MAXP];
+        #pragma unroll
+        for (int k = 0; k < PMCMC_MAXP; ++k) keys[k] = probe_key(a.seed, (uint32_t)a.iter, (uint32_t)k);
+        const size_t DT = (size_t)D * (size_t)a.T_len;
+
+        for (int c = grp; c < tc; c += ngrp) {
+            const int64_t t = c0 + c;
+            const T* xcol = xs + (size_t)c * D;
+            T* bcol = bl + (size_t)c * D;
+            T* acol = al + (size_t)c * D;
+            double s_lpx = 0, s_lpy = 0, s_qyx = 0, s_qxy = 0;
+            double d1[PMCMC_MAXP], d2[PMCMC_MAXP], d3[PMCMC_MAXP];
+            #pragma unroll
+            for (int k = 0; k < PMCMC_MAXP; ++k) d1[k] = d2[k] = d3[k] = 0;
+            for (int64_t d = lane; d < D; d += G) {
+                const double x = (double)xcol[d];
+                const size_t gi = (size_t)t * D + d;
+                double gx, y, gy;
+                if (MODE == MODE_ELEM) {
+                    const double p = a.prec ? (double)a.prec[d] : 1.0;
+                    const double m = a.mean ? (double)a.mean[d] : 0.0;
+                    gx = -p * (x - m);
+                    y = x + eps * gx + sq2e * (double)bcol[d];
+                    gy = -p * (y - m);
+                    s_lpx += -0.5 * p * (x - m) * (x - m);
+                    s_lpy += -0.5 * p * (y - m) * (y - m);
+#pragma unroll
+                    for (int k = 0; k < PMCMC_MAXP; ++k) {
+                        if (k >= K) break;
+                        const double z = (double)rademacher(keys[k], gi);
+                        const double w = z + eps * (-p * z);
+                        const double hy = -p * w;
+                        const double r = x - y - eps * gy;
+                        const double dr = z - w - eps * hy;
+                        d1[k] += gy * w;
+                        d2[k] += gx * z;
+                        d3[k] += r * dr;
+                    }
+                } else {
+                    gx = (double)a.gx[gi];
+                    y = (double)a.Y[gi];
+                    gy = (double)a.gy[gi];
+#pragma unroll
+                    for (int k = 0; k < PMCMC_MAXP; ++k) {
+                        if (k >= K) break;
+                        const double z = (double)rademacher(keys[k], gi);
+                        const double w = (double)a.W[k * DT + gi];
+                        const double hy = (double)a.hw[k * DT + gi];
+                        const double r = x - y - eps * gy;
+                        const double dr = z - w - eps * hy;
+                        d1[k] += gy * w;
+                        d2[k] += gx * z;
+                        d3[k] += r * dr;
+                    }
+                }
+                const double e1 = y - x - eps * gx;
+                const double e2 = x - y - eps * gy;
+                s_qyx += e1 * e1;
+                s_qxy += e2 * e2;
+            }
+            s_qyx = group_sum(s_qyx, G);
+            s_qxy = group_sum(s_qxy, G);
+            if (MODE == MODE_ELEM) {
+                s_lpx = group_sum(s_lpx, G);
+                s_lpy = group_sum(s_lpy, G);
+            } else {
+                s_lpx = a.lpx[t];
+                s_lpy = a.lpy[t];
+            }
+#pragma unroll
+            for (int k = 0; k < PMCMC_MAXP; ++k) {
+                if (k >= K) break;
+                d1[k] = group_sum(d1[k], G);
+                d2[k] = group_sum(d2[k], G);
+                d3[k] = group_sum(d3[k], G);
+            }
+            // log acceptance ratio (MALA.jl:589-595), in double
+            const double lq_yx = -0.5 * s_qyx / (2.0 * eps) - c_norm;
+            const double lq_xy = -0.5 * s_qxy / (2.0 * eps) - c_norm;
+            const double la = (s_lpy + lq_xy) - (s_lpx + lq_yx);
+            const double lu = log((double)a.U[t]);
+            const double g = 1.0 / (1.0 + exp(-(la - lu)));
+            const bool acc = lu < la;
+            const double gg = g * (1.0 - g);
+            double dg[PMCMC_MAXP];
+#pragma unroll
+            for (int k = 0; k < PMCMC_MAXP; ++k) dg[k] = gg * (d1[k] - d2[k] - d3[k] / (2.0 * eps));
+            if (lane == 0) {
+                accl[c] = acc ? 1 : 0;
+                if (a.acc_out) a.acc_out[t] = acc ? 1 : 0;
+                if (a.la_out) a.la_out[t] = la;
+            }
+            // second pass: A = mean_k z_k * Jt_k (or the exact diagonal), B = FT - A * x
+            for (int64_t d = lane; d < D; d += G) {
+                const double x = (double)xcol[d];
+                const size_t gi = (size_t)t * D + d;
+                double gx, y, gy, p = 1.0;
+                if (MODE == MODE_ELEM) {
+                    p = a.prec ? (double)a.prec[d] : 1.0;
+                    const double m = a.mean ? (double)a.mean[d] : 0.0;
+                    gx = -p * (x - m);
+                    y = x + eps * gx + sq2e * (double)bcol[d];
+                    gy = -p * (y - m);
+                } else {
+                    gx = (double)a.gx[gi];
+                    y = (double)a.Y[gi];
+                    gy = (double)a.gy[gi];
+                }
+                const double ft = acc ? y : x;
+                double Av = 0;
+                if (a.jacobian == JAC_DIAG && MODE == MODE_ELEM) {
+                    // exact diag of J = a(I + eps Hx) + (1-a)I + g(1-g)(y-x)c^T
+                    const double hx = -p, hy = -p;
+                    const double r = x - y - eps * gy;
+                    const double cc = (1.0 + eps * hx) * gy - gx -
+                                      (r - (1.0 + eps * hx) * (1.0 + eps * hy) * r) / (2.0 * eps);
+                    Av = (acc ? (1.0 + eps * hx) : 1.0) + gg * (y - x) * cc;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < PMCMC_MAXP; ++k) {
+                        if (k >= K) break;
+                        const double z = (double)rademacher(keys[k], gi);
+                        double w;
+                        if (MODE == MODE_ELEM)
+                            w = z + eps * (-p * z);
+                        else
+                            w = (double)a.W[k * DT + gi];
+                        const double jt = (acc ? w : z) + (y - x) * dg[k];
+                        Av += z * jt;
+                    }
+                    if (K > 1) Av *= 1.0 / (double)K;
+                }
+                const T Ar = (T)Av;
+                acol[d] = Ar;
+                bcol[d] = (T)(ft - (double)Ar * x);
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- phase 2: in-tile scan (rows x P parts, then Hillis-Steele over P)
+    const int chunk = (tc + P - 1) / P;
+    const int64_t nitems = D * P;
+    for (int64_t i = tid; i < nitems; i += PMCMC_NT) {
+        const int64_t r = i % D;
+        const int p = (int)(i / D);
+        const int cb = p * chunk;
+        const int ce = (cb + chunk) < tc ? (cb + chunk) : tc;
+        T al_ = (T)1, be_ = (T)0;
+        for (int c = cb; c < ce; ++c) {
+            const T av = al[(size_t)c * D + r];
+            be_ = av * be_ + bl[(size_t)c * D + r];
+            al_ = av * al_;
+        }
+        pa[p * D + r] = al_;
+        pb[p * D + r] = be_;
+    }
+    __syncthreads();
+    for (int off = 1; off < P; off <<= 1) {
+        T na = 0, nb = 0;
+        const bool act = tid < nitems;
+        int64_t r = 0;
+        int p = 0;
+        if (act) {
+            r = tid % D;
+            p = (int)(tid / D);
+            if (p >= off) {
+                const T a1 = pa[p * D + r], b1 = pb[p * D + r];
+                const T a0 = pa[(p - off) * D + r], b0 = pb[(p - off) * D + r];
+                na = a1 * a0;
+                nb = a1 * b0 + b1;
+            }
+        }
+        __syncthreads();
+        if (act && p >= off) {
+            pa[p * D + r] = na;
+            pb[p * D + r] = nb;
+        }
+        __syncthreads();
+    }
+    // tile aggregate of row r: (pa[(P-1)D+r], pb[(P-1)D+r])
+
+    // ---- phase 3: decoupled look-back across tiles ------------------------
+    const size_t tofs = (size_t)tile * D;
+    if (tile == 0) {
+        for (int64_t r = tid; r < D; r += PMCMC_NT) {
+            const T s = a.s0[r];
+            sin_[r] = s;
+            a.pref[r] = pa[(P - 1) * D + r] * s + pb[(P - 1) * D + r];
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) st_release(&a.flags[0], (epoch << 2) | PMCMC_ST_PREFIX);
+    } else {
+        for (int64_t r = tid; r < D; r += PMCMC_NT) {
+            a.agg_a[tofs + r] = pa[(P - 1) * D + r];
+            a.agg_b[tofs + r] = pb[(P - 1) * D + r];
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) st_release(&a.flags[tile], (epoch << 2) | PMCMC_ST_AGG);
+        if (tid < 64) {
+            const int lane = tid;
+            for (int64_t r = lane; r < D; r += 64) {
+                lka[r] = (T)1;
+                lkb[r] = (T)0;
+            }
+            int j = tile - 1;
+            while (true) {
+                unsigned f = 0;
+                if (lane == 0) {
+                    unsigned spins = 0;
+                    while (true) {
+                        f = ld_acquire(&a.flags[j]);
+                        if ((f >> 2) == (epoch & 0x3FFFFFFFu) && (f & 3u)) break;
+                        if (++spins > (1u << 22)) {  // bounded wait: report, never hang
+                            atomicOr(&sy->err, 1u);
+                            f = (epoch << 2) | PMCMC_ST_PREFIX;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                f = __shfl(f, 0, 64);
+                __threadfence();
+                const size_t jofs = (size_t)j * D;
+                if ((f & 3u) == PMCMC_ST_PREFIX) {
+                    for (int64_t r = lane; r < D; r += 64)
+                        sin_[r] = lka[r] * a.pref[jofs + r] + lkb[r];
+                    break;
+                }
+                for (int64_t r = lane; r < D; r += 64) {
+                    const T la_ = lka[r];
+                    lkb[r] = la_ * a.agg_b[jofs + r] + lkb[r];
+                    lka[r] = la_ * a.agg_a[jofs + r];
+                }
+                --j;
+            }
+        }
+        __syncthreads();
+        for (int64_t r = tid; r < D; r += PMCMC_NT)
+            a.pref[tofs + r] = pa[(P - 1) * D + r] * sin_[r] + pb[(P - 1) * D + r];
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) st_release(&a.flags[tile], (epoch << 2) | PMCMC_ST_PREFIX);
+    }
+
+    // ---- phase 4: apply the incoming state, damping, convergence maxima ---
+    double dmax = 0, smax = 0;
+    const T damp = (T)a.damping;
+    for (int64_t i = tid; i < nitems; i += PMCMC_NT) {
+        const int64_t r = i % D;
+        const int p = (int)(i / D);
+        const int cb = p * chunk;
+        const int ce = (cb + chunk) < tc ? (cb + chunk) : tc;
+        T s = sin_[r];
+        if (p > 0) s = pa[(p - 1) * D + r] * s + pb[(p - 1) * D + r];
+        for (int c = cb; c < ce; ++c) {
+            const size_t li = (size_t)c * D + r;
+            s = al[li] * s + bl[li];
+            if (MODE == MODE_SCAN) {
+                al[li] = s;
+            } else {
+                const T old = xs[li + D];
+                const T v = (a.damping == 1.0) ? s : ((T)1 - damp) * old + damp * s;
+                al[li] = v;
+                dmax = fmax(dmax, fabs((double)v - (double)old));
+                smax = fmax(smax, fabs((double)v));
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 5: store the tile (coalesced) ------------------------------
+    for (int64_t i = tid; i < nel; i += PMCMC_NT) a.S[gbase + i] = al[i];
+
+    if (MODE != MODE_SCAN) {
+        dmax = wave_max(dmax);
+        smax = wave_max(smax);
+        if ((tid & 63) == 0) {
+            s_red[0][tid >> 6] = dmax;
+            s_red[1][tid >> 6] = smax;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double m0 = 0, m1 = 0;
+            for (int w = 0; w < PMCMC_NT / 64; ++w) {
+                m0 = fmax(m0, s_red[0][w]);
+                m1 = fmax(m1, s_red[1][w]);
+            }
+            // non-negative doubles order like their bit patterns; NaN sorts above +Inf
+            atomicMax(&sy->dmax_bits, (unsigned long long)__double_as_longlong(m0));
+            atomicMax(&sy->smax_bits, (unsigned long long)__double_as_longlong(m1));
+        }
+    }
+
+    // ---- phase 6: the last workgroup closes the launch ---------------------
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        const unsigned prev = atomicAdd(&sy->done, 1u);
+        if (prev == (unsigned)a.numTiles - 1u) {
+            __threadfence();
+            if (MODE != MODE_SCAN) {
+                const double dm = __longlong_as_double(
+                    (long long)__hip_atomic_load(&sy->dmax_bits, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+                const double sm = __longlong_as_double(
+                    (long long)__hip_atomic_load(&sy->smax_bits, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+                const double metric = dm / (a.tol_abs + a.tol_rel * sm);
+                sy->last_metric = metric;
+                sy->last_iter = a.iter;
+                if (metric <= 1.0) {
+                    sy->conv_iter = a.iter;
+                    sy->converged = 1;
+                }
+                sy->dmax_bits = 0ull;
+                sy->smax_bits = 0ull;
+            }
+            sy->ticket = 0u;
+            sy->done = 0u;
+            sy->epoch = (epoch + 1u) & 0x3FFFFFFFu;
+            __threadfence();
+        }
+    }
+}
+
+}  // namespace pmcmc
