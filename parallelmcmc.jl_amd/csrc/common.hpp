@@ -38,34 +38,21 @@ __host__ __device__ inline float rademacher(uint32_t key, uint64_t idx) {
 }
 
 // ---------------------------------------------------------------------------
-// Cross-workgroup state of one single-pass scan launch (decoupled look-back)
-// and of the Newton loop's device-side convergence test.  Counters are reset
-// by the last workgroup of each launch, so launches need no memset.
+// Device-side state of the Newton loop: convergence maxima of the current
+// iteration, the arrival counter that elects the workgroup which closes it,
+// and the convergence flag that turns later queued iterations into no-ops.
 // ---------------------------------------------------------------------------
 struct SyncState {
-    unsigned int ticket;  // dynamic tile index
-    unsigned int done;    // finished workgroups of the current launch
-    unsigned int epoch;   // launch counter; look-back flags carry it
-    unsigned int err;     // nonzero: a bounded wait expired
+    unsigned int done;             // arrived workgroups of the current iteration
+    unsigned int err;
     unsigned long long dmax_bits;  // max |S_new - S| of this iteration (double bits)
     unsigned long long smax_bits;  // max |S_new| (double bits)
-    int converged;        // set by the iteration that met metric <= 1
-    int conv_iter;        // that iteration (0-based)
+    int converged;                 // set by the iteration that met metric <= 1
+    int conv_iter;                 // that iteration (0-based)
     double last_metric;
     int last_iter;
     int pad;
 };
-
-// flag word: (epoch << 2) | status
-#define PMCMC_ST_AGG 1u
-#define PMCMC_ST_PREFIX 2u
-
-__device__ inline unsigned ld_acquire(const unsigned* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void st_release(unsigned* p, unsigned v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 template <typename T>
 struct DevTraits;

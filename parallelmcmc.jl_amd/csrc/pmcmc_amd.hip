@@ -20,17 +20,31 @@
 
 using namespace pmcmc;
 
+// Levels of the recursive reduce-then-scan: level l scans a D x L_l problem
+// in tiles of TC_l columns; when it has more than one tile, its D x nT_l
+// tile aggregates (agg_a, agg_b) form the level-(l+1) problem, whose output
+// (states) holds the end state of every level-l tile.
+struct ScanLevel {
+    int64_t len = 0;
+    int TC = 0, P = 0, nT = 0;
+    size_t off_a = 0, off_b = 0, off_s = 0;  // byte offsets into the level blob
+};
+
+struct ScanPlan {
+    int64_t D = 0;
+    int nlev = 0;
+    ScanLevel lev[24];
+    size_t bytes = 0;
+};
+
 struct pmcmc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string err;
     // scratch of the standalone scan entry points (grown on demand)
-    SyncState* sync = nullptr;
-    unsigned* flags = nullptr;
-    size_t flags_cap = 0;
-    void* agg = nullptr;
-    size_t agg_cap = 0;
+    void* scan_blob = nullptr;
+    size_t scan_blob_cap = 0;
     SyncState* host_sync = nullptr;  // pinned
 };
 
@@ -57,9 +71,8 @@ struct pmcmc_deer_ws {
     size_t esz = 4;
     void* buf[2] = {nullptr, nullptr};
     SyncState* sync = nullptr;
-    unsigned* flags = nullptr;
-    void* agg = nullptr;  // agg_a | agg_b | pref
-    int numTiles = 0, TC = 0;
+    ScanPlan plan;
+    void* blob = nullptr;  // scan levels
     // GEMM-target buffers (allocated on first use)
     GlmWork glmw;
 };
@@ -89,7 +102,6 @@ static int fail(pmcmc_ctx* ctx, int code, const char* fmt, ...) {
 // ---------------------------------------------------------------------------
 __global__ void reset_sync_kernel(SyncState* s) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        s->ticket = 0u;
         s->done = 0u;
         s->err = 0u;
         s->dmax_bits = 0ull;
@@ -215,22 +227,77 @@ static int grid_for(int64_t n, int block = 256, int cap = 8192) {
     return (int)g;
 }
 
-// Tile shape: TC columns so that TC*D ~ the per-type element budget, with
-// at least ~2 tiles per CU when the problem is large enough.
+// Tile shape of one level: TC columns so that TC*D is about the per-type
+// element budget, shrunk until the LDS footprint fits 63 KB.
 template <typename T>
 static void tile_shape(int64_t D, int64_t Tn, bool with_iterate, int* TC, int* P, int* numTiles) {
     int64_t E = DevTraits<T>::tile_elems;
     int64_t tc = E / D;
     if (tc < 1) tc = 1;
     while (tc > 1 && tile_lds_bytes<T>(D, (int)tc, with_iterate) > 64512) tc >>= 1;
-    // keep >= 512 tiles when that still leaves >= 8 columns per tile
-    while (tc > 8 && (Tn + tc - 1) / tc < 512) tc >>= 1;
     if (tc > Tn && Tn > 0) tc = Tn;
     *TC = (int)tc;
     *P = D >= PMCMC_NT ? 1 : (int)(PMCMC_NT / D);
     if (*P > tc) *P = (int)tc;
     if (*P < 1) *P = 1;
     *numTiles = (int)((Tn + tc - 1) / tc);
+}
+
+template <typename T>
+static bool plan_scan(ScanPlan* p, int64_t D, int64_t Tn, bool deer_top) {
+    p->D = D;
+    p->nlev = 0;
+    p->bytes = 0;
+    int64_t len = Tn;
+    for (int l = 0; l < 24; ++l) {
+        ScanLevel& L = p->lev[l];
+        L.len = len;
+        tile_shape<T>(D, len, deer_top && l == 0, &L.TC, &L.P, &L.nT);
+        p->nlev = l + 1;
+        if (L.nT <= 1) return tile_lds_bytes<T>(D, L.TC, deer_top && l == 0) <= 64512;
+        const size_t m = (size_t)D * L.nT * sizeof(T);
+        L.off_a = p->bytes;
+        L.off_b = p->bytes + m;
+        L.off_s = p->bytes + 2 * m;
+        p->bytes += 3 * m;
+        len = L.nT;
+    }
+    return false;
+}
+
+template <typename T, int MODE, int PASS>
+static void launch_tile(const TileArgs<T>& a, int grid, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((tile_kernel<T, MODE, PASS>), dim3(grid), dim3(PMCMC_NT), lds, s, a);
+}
+
+// Scan levels l.. of a MODE_SCAN problem: S = scan(A, B) from s0.
+template <typename T>
+static void run_scan_levels(const ScanPlan& p, int l, const T* A, const T* B, T* S, const T* s0,
+                            char* blob, const int* skip, hipStream_t s) {
+    const ScanLevel& L = p.lev[l];
+    TileArgs<T> a{};
+    a.A = A;
+    a.B = B;
+    a.s0 = s0;
+    a.S = S;
+    a.D = p.D;
+    a.T_len = L.len;
+    a.TC = L.TC;
+    a.numTiles = L.nT;
+    a.P = L.P;
+    a.skip = skip;
+    a.damping = 1.0;
+    const size_t lds = tile_lds_bytes<T>(p.D, L.TC, false);
+    if (L.nT <= 1) {
+        launch_tile<T, MODE_SCAN, PASS_SINGLE>(a, 1, lds, s);
+        return;
+    }
+    a.agg_a = (T*)(blob + L.off_a);
+    a.agg_b = (T*)(blob + L.off_b);
+    launch_tile<T, MODE_SCAN, PASS_REDUCE>(a, L.nT, lds, s);
+    run_scan_levels<T>(p, l + 1, a.agg_a, a.agg_b, (T*)(blob + L.off_s), s0, blob, skip, s);
+    a.prev = (const T*)(blob + L.off_s);
+    launch_tile<T, MODE_SCAN, PASS_APPLY>(a, L.nT, lds, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -259,9 +326,7 @@ extern "C" int pmcmc_ctx_create(int device, void* stream, pmcmc_ctx** out) {
         }
         c->own_stream = true;
     }
-    if (hipMalloc(&c->sync, sizeof(SyncState)) != hipSuccess ||
-        hipMemset(c->sync, 0, sizeof(SyncState)) != hipSuccess ||
-        hipHostMalloc(&c->host_sync, sizeof(SyncState), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc(&c->host_sync, sizeof(SyncState), hipHostMallocDefault) != hipSuccess) {
         pmcmc_ctx_destroy(c);
         return PMCMC_ERR_ALLOC;
     }
@@ -273,9 +338,7 @@ extern "C" int pmcmc_ctx_destroy(pmcmc_ctx* c) {
     if (!c) return PMCMC_OK;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->sync) hipFree(c->sync);
-    if (c->flags) hipFree(c->flags);
-    if (c->agg) hipFree(c->agg);
+    if (c->scan_blob) hipFree(c->scan_blob);
     if (c->host_sync) hipHostFree(c->host_sync);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -334,46 +397,20 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
     };
     if (overlap(A)) return fail(c, PMCMC_ERR_ARGUMENT, "S must not alias A");
     if (overlap(B)) return fail(c, PMCMC_ERR_ARGUMENT, "S must not alias B");
-    int TC, P, nt;
-    tile_shape<T>(D, Tn, false, &TC, &P, &nt);
-    size_t lds = tile_lds_bytes<T>(D, TC, false);
-    if (lds > 64512) return fail(c, PMCMC_ERR_UNSUPPORTED, "scan: D=%lld too large for one tile", (long long)D);
+    ScanPlan plan;
+    if (!plan_scan<T>(&plan, D, Tn, false))
+        return fail(c, PMCMC_ERR_UNSUPPORTED, "scan: D=%lld too large for one tile", (long long)D);
     HIPCHK(c, hipSetDevice(c->device));
-    if ((size_t)nt > c->flags_cap) {
-        if (c->flags) HIPCHK(c, hipFree(c->flags));
-        c->flags = nullptr;
-        HIPCHK(c, hipMalloc(&c->flags, (size_t)nt * sizeof(unsigned)));
-        HIPCHK(c, hipMemsetAsync(c->flags, 0, (size_t)nt * sizeof(unsigned), c->stream));
-        c->flags_cap = nt;
+    if (plan.bytes > c->scan_blob_cap) {
+        if (c->scan_blob) HIPCHK(c, hipFree(c->scan_blob));
+        c->scan_blob = nullptr;
+        c->scan_blob_cap = 0;
+        HIPCHK(c, hipMalloc(&c->scan_blob, plan.bytes));
+        c->scan_blob_cap = plan.bytes;
     }
-    const size_t agg_bytes = 3 * (size_t)nt * D * sizeof(T);
-    if (agg_bytes > c->agg_cap) {
-        if (c->agg) HIPCHK(c, hipFree(c->agg));
-        c->agg = nullptr;
-        HIPCHK(c, hipMalloc(&c->agg, agg_bytes));
-        c->agg_cap = agg_bytes;
-    }
-    TileArgs<T> a{};
-    a.A = A;
-    a.B = B;
-    a.s0 = s0;
-    a.S = S;
-    a.D = D;
-    a.T_len = Tn;
-    a.TC = TC;
-    a.numTiles = nt;
-    a.P = P;
-    a.sync = c->sync;
-    a.flags = c->flags;
-    a.agg_a = (T*)c->agg;
-    a.agg_b = a.agg_a + (size_t)nt * D;
-    a.pref = a.agg_b + (size_t)nt * D;
-    a.damping = 1.0;
-    hipLaunchKernelGGL((tile_kernel<T, MODE_SCAN>), dim3(nt), dim3(PMCMC_NT), lds, c->stream, a);
+    run_scan_levels<T>(plan, 0, A, B, S, s0, (char*)c->scan_blob, nullptr, c->stream);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(c->host_sync, c->sync, sizeof(SyncState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (c->host_sync->err) return fail(c, PMCMC_ERR_DEVICE, "scan: look-back wait expired");
     return PMCMC_OK;
 }
 
@@ -557,20 +594,17 @@ extern "C" int pmcmc_deer_ws_create(pmcmc_ctx* c, int64_t D, int64_t Tn, int pre
     w->T = Tn;
     w->prec = prec;
     w->esz = prec == PMCMC_PREC_F64 ? 8 : 4;
-    int TC, P, nt;
-    if (w->esz == 8)
-        tile_shape<double>(D, Tn, true, &TC, &P, &nt);
-    else
-        tile_shape<float>(D, Tn, true, &TC, &P, &nt);
-    w->TC = TC;
-    w->numTiles = nt;
+    const bool ok_plan = w->esz == 8 ? plan_scan<double>(&w->plan, D, Tn, true)
+                                     : plan_scan<float>(&w->plan, D, Tn, true);
+    if (!ok_plan) {
+        delete w;
+        return fail(c, PMCMC_ERR_UNSUPPORTED, "D=%lld too large for the fused tile kernel", (long long)D);
+    }
     const size_t mat = (size_t)D * Tn * w->esz;
     bool ok = hipMalloc(&w->buf[0], mat) == hipSuccess && hipMalloc(&w->buf[1], mat) == hipSuccess &&
               hipMalloc(&w->sync, sizeof(SyncState)) == hipSuccess &&
               hipMemset(w->sync, 0, sizeof(SyncState)) == hipSuccess &&
-              hipMalloc(&w->flags, (size_t)nt * sizeof(unsigned)) == hipSuccess &&
-              hipMemset(w->flags, 0, (size_t)nt * sizeof(unsigned)) == hipSuccess &&
-              hipMalloc(&w->agg, 3 * (size_t)nt * D * w->esz) == hipSuccess;
+              hipMalloc(&w->blob, w->plan.bytes > 0 ? w->plan.bytes : 16) == hipSuccess;
     if (!ok) {
         pmcmc_deer_ws_destroy(w);
         return fail(c, PMCMC_ERR_ALLOC, "workspace allocation failed (D=%lld T=%lld)", (long long)D, (long long)Tn);
@@ -588,11 +622,34 @@ extern "C" int pmcmc_deer_ws_destroy(pmcmc_deer_ws* w) {
     hipFree(w->buf[0]);
     hipFree(w->buf[1]);
     hipFree(w->sync);
-    hipFree(w->flags);
-    hipFree(w->agg);
+    hipFree(w->blob);
     glm_work_destroy(&w->glmw);
     delete w;
     return PMCMC_OK;
+}
+
+// One Newton iteration on the device: [GEMM products] -> level-0 reduce ->
+// scan of the tile aggregates -> level-0 apply (damping, maxima, metric).
+template <typename T, int MODE>
+static void launch_deer_iteration(pmcmc_deer_ws* w, TileArgs<T> a, hipStream_t s) {
+    const ScanPlan& p = w->plan;
+    const ScanLevel& L = p.lev[0];
+    char* blob = (char*)w->blob;
+    a.TC = L.TC;
+    a.P = L.P;
+    a.numTiles = L.nT;
+    a.skip = &w->sync->converged;
+    const size_t lds = tile_lds_bytes<T>(p.D, L.TC, true);
+    if (L.nT <= 1) {
+        launch_tile<T, MODE, PASS_SINGLE>(a, 1, lds, s);
+        return;
+    }
+    a.agg_a = (T*)(blob + L.off_a);
+    a.agg_b = (T*)(blob + L.off_b);
+    launch_tile<T, MODE, PASS_REDUCE>(a, L.nT, lds, s);
+    run_scan_levels<T>(p, 1, a.agg_a, a.agg_b, (T*)(blob + L.off_s), a.s0, blob, a.skip, s);
+    a.prev = (const T*)(blob + L.off_s);
+    launch_tile<T, MODE, PASS_APPLY>(a, L.nT, lds, s);
 }
 
 template <typename T>
@@ -601,10 +658,6 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
                         const void* init, void* S_out, uint8_t* acc_out, pmcmc_deer_info* info) {
     const int64_t D = w->D, Tn = w->T;
     const int mode = tgt->kind == TGT_DIAG ? MODE_ELEM : MODE_GEMM;
-    int TC, P, nt;
-    tile_shape<T>(D, Tn, true, &TC, &P, &nt);
-    const size_t lds = tile_lds_bytes<T>(D, TC, true);
-    if (lds > 64512) return fail(c, PMCMC_ERR_UNSUPPORTED, "D=%lld too large for the fused tile kernel", (long long)D);
     if (mode == MODE_GEMM) {
         if (p->jacobian != PMCMC_JAC_STOCH_DIAG)
             return fail(c, PMCMC_ERR_UNSUPPORTED, "logistic target supports jacobian=:stoch_diag only");
@@ -622,14 +675,7 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
     a.s0 = (const T*)x0;
     a.D = D;
     a.T_len = Tn;
-    a.TC = TC;
-    a.numTiles = nt;
-    a.P = P;
     a.sync = w->sync;
-    a.flags = w->flags;
-    a.agg_a = (T*)w->agg;
-    a.agg_b = a.agg_a + (size_t)nt * D;
-    a.pref = a.agg_b + (size_t)nt * D;
     a.Xi = (const T*)Xi;
     a.U = (const T*)U;
     a.acc_out = acc_out;
@@ -640,7 +686,6 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
     a.tol_rel = p->tol_rel;
     a.probes = p->probes;
     a.jacobian = p->jacobian;
-    a.maxiter = p->maxiter;
     a.seed = p->seed;
     if (tgt->kind == TGT_DIAG) {
         a.prec = sizeof(T) == 8 ? (const T*)tgt->prec_d : (const T*)tgt->prec_f;
@@ -669,16 +714,15 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
                 a.hw = (const T*)w->glmw.hw;
                 a.lpx = w->glmw.lpx;
                 a.lpy = w->glmw.lpy;
-                hipLaunchKernelGGL((tile_kernel<T, MODE_GEMM>), dim3(nt), dim3(PMCMC_NT), lds, c->stream, a);
+                launch_deer_iteration<T, MODE_GEMM>(w, a, c->stream);
             } else {
-                hipLaunchKernelGGL((tile_kernel<T, MODE_ELEM>), dim3(nt), dim3(PMCMC_NT), lds, c->stream, a);
+                launch_deer_iteration<T, MODE_ELEM>(w, a, c->stream);
             }
             HIPCHK(c, hipGetLastError());
         }
         k += n;
         HIPCHK(c, hipMemcpyAsync(c->host_sync, w->sync, sizeof(SyncState), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        if (c->host_sync->err) return fail(c, PMCMC_ERR_DEVICE, "DEER: look-back wait expired");
         done = c->host_sync->converged != 0;
         if (chunk < 16) chunk <<= 1;
     }

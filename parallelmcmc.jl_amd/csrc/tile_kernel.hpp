@@ -1,21 +1,32 @@
-// Single-pass tile kernel: diagonal affine scan with decoupled look-back,
-// optionally fused with the DEER Newton update of a taped MALA recursion.
+// Tile kernels of the diagonal affine scan, optionally fused with the DEER
+// Newton update of a taped MALA recursion.
 //
-// One workgroup owns TC consecutive columns (all D rows) of a column-major
-// D x T problem.  Its input columns are contiguous in HBM, so they are staged
-// into LDS with fully coalesced loads, scanned in LDS, chained to the
-// predecessor tiles through per-tile aggregates (decoupled look-back), and
-// written back with coalesced stores.  HBM traffic is one read of every input
-// and one write of the output: the minimum for the scan.
+//   s_t = a_t .* s_{t-1} + b_t   (rows independent; D x T column-major)
+//
+// One workgroup owns TC consecutive columns (all D rows).  Those columns are
+// contiguous in HBM, so they are staged into LDS with fully coalesced loads
+// and scanned in LDS (rows x P parts, then a Hillis-Steele combine of the
+// parts).  The scan across tiles is a reduce-then-scan:
+//   PASS_REDUCE: per tile, the composed map (prod a, combined b) of each row;
+//   the D x nTiles aggregates are scanned recursively by the same kernels
+//   (they form a D x nTiles diagonal scan); then
+//   PASS_APPLY: per tile, the incoming state is the previous tile's end state
+//   and the tile is re-scanned from it and written out.
+//   PASS_SINGLE: a problem that fits one tile, scanned from s0 directly.
+// No workgroup ever waits on another (no spin loops).  PASS_APPLY re-reads
+// what PASS_REDUCE read just before; at these working-set sizes the re-read
+// is served by the 256 MB last-level cache, so HBM sees close to one read of
+// every input and one write of the output.
 //
 // Modes
-//   MODE_SCAN: A, B read from HBM; S written.           (DEERScan.jl:102-173)
+//   MODE_SCAN: A, B read from HBM.                       (DEERScan.jl:102-173)
 //   MODE_ELEM: A, B built on chip from the iterate S_in, the tape (Xi, U) and
 //              on-chip Rademacher probes, for targets with a diagonal Hessian
-//              (fused MALA.jl:556-628 + DEER.jl:398-419 + scan + damping
-//              DEER.jl:470-473 + convergence maxima DEER.jl:567-571).
-//   MODE_GEMM: as MODE_ELEM but gradients / HVPs / log-densities of the
-//              proposal and current state come precomputed from the GEMM
+//              (MALA.jl:556-628 + DEER.jl:398-419 fused with the scan, the
+//              damping DEER.jl:470-473 and the convergence maxima
+//              DEER.jl:567-571).
+//   MODE_GEMM: as MODE_ELEM but the gradients / HVPs / log-densities of the
+//              current and proposed states come precomputed by the GEMM
 //              target kernels.
 #pragma once
 
@@ -24,6 +35,7 @@
 namespace pmcmc {
 
 enum { MODE_SCAN = 0, MODE_ELEM = 1, MODE_GEMM = 2 };
+enum { PASS_REDUCE = 0, PASS_APPLY = 1, PASS_SINGLE = 2 };
 enum { JAC_STOCH = 0, JAC_DIAG = 1 };
 #define PMCMC_MAXP 4
 
@@ -34,18 +46,18 @@ struct TileArgs {
     const T* B;
     // common
     const T* s0;
-    T* S;  // output
+    T* S;           // PASS_APPLY / PASS_SINGLE output
+    T* agg_a;       // PASS_REDUCE output: D x numTiles
+    T* agg_b;
+    const T* prev;  // PASS_APPLY: D x numTiles end states of the tiles (column j = after tile j)
     int64_t D;
     int64_t T_len;
     int TC;        // columns per tile
     int numTiles;
     int P;         // parts per row in the in-tile scan
-    SyncState* sync;
-    unsigned* flags;
-    T* agg_a;
-    T* agg_b;
-    T* pref;
+    const int* skip;  // nonzero => the kernel is a no-op (converged Newton loop)
     // DEER (MODE_ELEM / MODE_GEMM)
+    SyncState* sync;
     const T* S_in;
     const T* Xi;
     const T* U;
@@ -55,12 +67,11 @@ struct TileArgs {
     int iter;
     int probes;
     int jacobian;
-    int maxiter;
     uint64_t seed;
     // diagonal-Hessian target (nullable: standard normal)
     const T* prec;
     const T* mean;
-    // GEMM target products (MODE_GEMM): D x T, probe k of hz/hw at k*D*T
+    // GEMM target products (MODE_GEMM): D x T, probe k of W/hw at k*D*T
     const T* Y;
     const T* gx;
     const T* gy;
@@ -71,18 +82,17 @@ struct TileArgs {
 };
 
 // LDS layout (elements of T):
-//   xs  : (TC+1)*D  iterate columns c0-1 .. c0+TC-1 (Xbar and S_old)
-//   bl  : TC*D      B (MODE_ELEM/GEMM: the tape Xi first, overwritten by B)
+//   xs  : (TC+1)*D  iterate columns c0-1 .. c0+TC-1 (Xbar and S_old); DEER modes only
+//   bl  : TC*D      B (DEER modes: the tape Xi first, overwritten by B)
 //   al  : TC*D      A, then the output tile
 //   pa,pb: max(NT, D) part aggregates
 //   sin : D         incoming state per row
-//   lka,lkb: D      look-back composition
 //   acc : TC bytes  accept flags
 template <typename T>
 __host__ __device__ inline size_t tile_lds_bytes(int64_t D, int TC, bool with_iterate) {
     int64_t PA = D > PMCMC_NT ? D : PMCMC_NT;
     int64_t xs = with_iterate ? ((int64_t)TC + 1) * D : 0;
-    size_t n = (size_t)(xs + 2 * (int64_t)TC * D + 2 * PA + 3 * D);
+    size_t n = (size_t)(xs + 2 * (int64_t)TC * D + 2 * PA + D);
     return n * sizeof(T) + (size_t)TC + 64;
 }
 
@@ -98,49 +108,37 @@ __device__ inline double group_sum(double v, int G) {
     return v;
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, int PASS>
 __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int s_tile;
-    __shared__ unsigned s_epoch;
-    __shared__ unsigned s_status;
     __shared__ double s_red[2][PMCMC_NT / 64];
 
     const int tid = threadIdx.x;
-    SyncState* sy = a.sync;
-    if (MODE != MODE_SCAN) {
-        // Newton loop already converged: later queued iterations are no-ops.
-        if (*(volatile int*)&sy->converged) return;
-    }
-    if (tid == 0) {
-        s_tile = (int)atomicAdd(&sy->ticket, 1u);
-        s_epoch = *(volatile unsigned*)&sy->epoch;
-    }
-    __syncthreads();
-    const int tile = s_tile;
-    const unsigned epoch = s_epoch;
+    // Newton loop already converged: later queued iterations are no-ops.
+    if (a.skip && *(volatile const int*)a.skip) return;
+    const int tile = blockIdx.x;
     const int64_t D = a.D;
     const int TC = a.TC;
     const int64_t c0 = (int64_t)tile * TC;
     const int tc = (int)((a.T_len - c0) < TC ? (a.T_len - c0) : TC);
     const int P = a.P;
     const int64_t PA = D > PMCMC_NT ? D : PMCMC_NT;
+    constexpr bool DEERM = MODE != MODE_SCAN;
+    constexpr bool OUT = PASS != PASS_REDUCE;
 
     T* xs = reinterpret_cast<T*>(smem);
-    T* bl = xs + (MODE == MODE_SCAN ? (size_t)0 : (size_t)(TC + 1) * D);
+    T* bl = xs + (DEERM ? (size_t)(TC + 1) * D : (size_t)0);
     T* al = bl + (size_t)TC * D;
     T* pa = al + (size_t)TC * D;
     T* pb = pa + PA;
     T* sin_ = pb + PA;
-    T* lka = sin_ + D;
-    T* lkb = lka + D;
-    uint8_t* accl = reinterpret_cast<uint8_t*>(lkb + D);
+    uint8_t* accl = reinterpret_cast<uint8_t*>(sin_ + D);
 
     const int64_t nel = (int64_t)tc * D;
     const int64_t gbase = c0 * D;
 
     // ---- phase 0: stage inputs (coalesced) -------------------------------
-    if (MODE == MODE_SCAN) {
+    if (!DEERM) {
         for (int64_t i = tid; i < nel; i += PMCMC_NT) {
             al[i] = a.A[gbase + i];
             bl[i] = a.B[gbase + i];
@@ -152,10 +150,15 @@ __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
         }
         for (int64_t i = tid; i < nel; i += PMCMC_NT) bl[i] = a.Xi[gbase + i];
     }
+    // incoming state of the tile (needed only by the output passes)
+    if (OUT) {
+        for (int64_t r = tid; r < D; r += PMCMC_NT)
+            sin_[r] = (PASS == PASS_SINGLE || tile == 0) ? a.s0[r] : a.prev[(size_t)(tile - 1) * D + r];
+    }
     __syncthreads();
 
     // ---- phase 1: build A, B per column (MALA step + surrogate JVP) -------
-    if (MODE != MODE_SCAN) {
+    if (DEERM) {
         int G = 1;
         while (G < D && G < 64) G <<= 1;
         const int lane = tid & (G - 1);
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
         const double c_norm = ((double)D / 2.0) * log(4.0 * 3.14159265358979323846 * eps);
         const int K = a.probes;
         uint32_t keys[PMCMC_MAXP];
-        #pragma unroll
+#pragma unroll
         for (int k = 0; k < PMCMC_MAXP; ++k) keys[k] = probe_key(a.seed, (uint32_t)a.iter, (uint32_t)k);
         const size_t DT = (size_t)D * (size_t)a.T_len;
 
@@ -177,7 +180,7 @@ __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
             T* acol = al + (size_t)c * D;
             double s_lpx = 0, s_lpy = 0, s_qyx = 0, s_qxy = 0;
             double d1[PMCMC_MAXP], d2[PMCMC_MAXP], d3[PMCMC_MAXP];
-            #pragma unroll
+#pragma unroll
             for (int k = 0; k < PMCMC_MAXP; ++k) d1[k] = d2[k] = d3[k] = 0;
             for (int64_t d = lane; d < D; d += G) {
                 const double x = (double)xcol[d];
@@ -252,7 +255,7 @@ __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
             double dg[PMCMC_MAXP];
 #pragma unroll
             for (int k = 0; k < PMCMC_MAXP; ++k) dg[k] = gg * (d1[k] - d2[k] - d3[k] / (2.0 * eps));
-            if (lane == 0) {
+            if (OUT && lane == 0) {
                 accl[c] = acc ? 1 : 0;
                 if (a.acc_out) a.acc_out[t] = acc ? 1 : 0;
                 if (a.la_out) a.la_out[t] = la;
@@ -345,71 +348,14 @@ __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
         }
         __syncthreads();
     }
-    // tile aggregate of row r: (pa[(P-1)D+r], pb[(P-1)D+r])
 
-    // ---- phase 3: decoupled look-back across tiles ------------------------
-    const size_t tofs = (size_t)tile * D;
-    if (tile == 0) {
+    if (PASS == PASS_REDUCE) {
+        // tile aggregate of row r: (pa[(P-1)D+r], pb[(P-1)D+r]), column `tile`
         for (int64_t r = tid; r < D; r += PMCMC_NT) {
-            const T s = a.s0[r];
-            sin_[r] = s;
-            a.pref[r] = pa[(P - 1) * D + r] * s + pb[(P - 1) * D + r];
+            a.agg_a[(size_t)tile * D + r] = pa[(P - 1) * D + r];
+            a.agg_b[(size_t)tile * D + r] = pb[(P - 1) * D + r];
         }
-        __threadfence();
-        __syncthreads();
-        if (tid == 0) st_release(&a.flags[0], (epoch << 2) | PMCMC_ST_PREFIX);
-    } else {
-        for (int64_t r = tid; r < D; r += PMCMC_NT) {
-            a.agg_a[tofs + r] = pa[(P - 1) * D + r];
-            a.agg_b[tofs + r] = pb[(P - 1) * D + r];
-        }
-        __threadfence();
-        __syncthreads();
-        if (tid == 0) st_release(&a.flags[tile], (epoch << 2) | PMCMC_ST_AGG);
-        if (tid < 64) {
-            const int lane = tid;
-            for (int64_t r = lane; r < D; r += 64) {
-                lka[r] = (T)1;
-                lkb[r] = (T)0;
-            }
-            int j = tile - 1;
-            while (true) {
-                unsigned f = 0;
-                if (lane == 0) {
-                    unsigned spins = 0;
-                    while (true) {
-                        f = ld_acquire(&a.flags[j]);
-                        if ((f >> 2) == (epoch & 0x3FFFFFFFu) && (f & 3u)) break;
-                        if (++spins > (1u << 22)) {  // bounded wait: report, never hang
-                            atomicOr(&sy->err, 1u);
-                            f = (epoch << 2) | PMCMC_ST_PREFIX;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                    }
-                }
-                f = __shfl(f, 0, 64);
-                __threadfence();
-                const size_t jofs = (size_t)j * D;
-                if ((f & 3u) == PMCMC_ST_PREFIX) {
-                    for (int64_t r = lane; r < D; r += 64)
-                        sin_[r] = lka[r] * a.pref[jofs + r] + lkb[r];
-                    break;
-                }
-                for (int64_t r = lane; r < D; r += 64) {
-                    const T la_ = lka[r];
-                    lkb[r] = la_ * a.agg_b[jofs + r] + lkb[r];
-                    lka[r] = la_ * a.agg_a[jofs + r];
-                }
-                --j;
-            }
-        }
-        __syncthreads();
-        for (int64_t r = tid; r < D; r += PMCMC_NT)
-            a.pref[tofs + r] = pa[(P - 1) * D + r] * sin_[r] + pb[(P - 1) * D + r];
-        __threadfence();
-        __syncthreads();
-        if (tid == 0) st_release(&a.flags[tile], (epoch << 2) | PMCMC_ST_PREFIX);
+        return;
     }
 
     // ---- phase 4: apply the incoming state, damping, convergence maxima ---
@@ -425,7 +371,7 @@ __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
         for (int c = cb; c < ce; ++c) {
             const size_t li = (size_t)c * D + r;
             s = al[li] * s + bl[li];
-            if (MODE == MODE_SCAN) {
+            if (!DEERM) {
                 al[li] = s;
             } else {
                 const T old = xs[li + D];
@@ -441,7 +387,8 @@ __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
     // ---- phase 5: store the tile (coalesced) ------------------------------
     for (int64_t i = tid; i < nel; i += PMCMC_NT) a.S[gbase + i] = al[i];
 
-    if (MODE != MODE_SCAN) {
+    if (DEERM) {
+        SyncState* sy = a.sync;
         dmax = wave_max(dmax);
         smax = wave_max(smax);
         if ((tid & 63) == 0) {
@@ -458,21 +405,13 @@ __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
             // non-negative doubles order like their bit patterns; NaN sorts above +Inf
             atomicMax(&sy->dmax_bits, (unsigned long long)__double_as_longlong(m0));
             atomicMax(&sy->smax_bits, (unsigned long long)__double_as_longlong(m1));
-        }
-    }
-
-    // ---- phase 6: the last workgroup closes the launch ---------------------
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-        const unsigned prev = atomicAdd(&sy->done, 1u);
-        if (prev == (unsigned)a.numTiles - 1u) {
-            __threadfence();
-            if (MODE != MODE_SCAN) {
-                const double dm = __longlong_as_double(
-                    (long long)__hip_atomic_load(&sy->dmax_bits, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
-                const double sm = __longlong_as_double(
-                    (long long)__hip_atomic_load(&sy->smax_bits, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+            // the last workgroup closes the iteration: metric and convergence flag.
+            // acq_rel: this block's maxima are visible before its arrival counts,
+            // and the last arrival sees every block's maxima.
+            const unsigned prev = __hip_atomic_fetch_add(&sy->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (prev == (unsigned)a.numTiles - 1u) {
+                const double dm = __longlong_as_double((long long)atomicAdd(&sy->dmax_bits, 0ull));
+                const double sm = __longlong_as_double((long long)atomicAdd(&sy->smax_bits, 0ull));
                 const double metric = dm / (a.tol_abs + a.tol_rel * sm);
                 sy->last_metric = metric;
                 sy->last_iter = a.iter;
@@ -482,11 +421,8 @@ __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
                 }
                 sy->dmax_bits = 0ull;
                 sy->smax_bits = 0ull;
+                sy->done = 0u;
             }
-            sy->ticket = 0u;
-            sy->done = 0u;
-            sy->epoch = (epoch + 1u) & 0x3FFFFFFFu;
-            __threadfence();
         }
     }
 }
