@@ -33,7 +33,7 @@ extern "C" {
 #define PMCMC_ERR_DIMENSION 2 /* Julia DimensionMismatch */
 #define PMCMC_ERR_HIP 3       /* HIP runtime error */
 #define PMCMC_ERR_ALLOC 4     /* device allocation failed */
-#define PMCMC_ERR_DEVICE 5    /* a kernel reported an error (bounded wait expired) */
+#define PMCMC_ERR_DEVICE 5    /* a kernel reported an error */
 #define PMCMC_ERR_UNSUPPORTED 6
 
 /* ---- enums -------------------------------------------------------------- */
@@ -57,6 +57,21 @@ int pmcmc_ctx_synchronize(pmcmc_ctx* ctx);
 /* Library version string (static storage). */
 const char* pmcmc_version(void);
 
+/* ---- measurement --------------------------------------------------------
+ * No counterpart in the reference (its benchmarks time whole calls with
+ * BenchmarkTools).  Per-kernel durations of DEER solves on this context,
+ * from HIP events recorded on the context stream around each launch; only
+ * launches of iterations up to convergence are counted.                     */
+#define PMCMC_KT_GLM_PASS 0    /* fused GEMM->sigmoid->GEMM^T pass (MFMA) */
+#define PMCMC_KT_TILE_REDUCE 1 /* level-0 tile reduce (MALA step + JVP + in-tile scan) */
+#define PMCMC_KT_TILE_APPLY 2  /* level-0 tile apply (or the single-tile kernel) */
+#define PMCMC_KT_GLM_REDUCE 3  /* split-sum + prior terms of a GEMM pass */
+#define PMCMC_KT_SCAN_SPINE 4  /* scan of the tile aggregates (all levels) */
+#define PMCMC_KT_ITERATION 5   /* one whole Newton iteration */
+int pmcmc_ctx_set_profiling(pmcmc_ctx* ctx, int on);
+int pmcmc_ctx_kernel_time(pmcmc_ctx* ctx, int kind, int64_t* launches, double* total_ms);
+int pmcmc_ctx_reset_kernel_times(pmcmc_ctx* ctx);
+
 /* ---- device buffers (for hosts without their own device arrays) --------- */
 int pmcmc_buffer_alloc(pmcmc_ctx* ctx, int64_t nbytes, void** out);
 int pmcmc_buffer_free(pmcmc_ctx* ctx, void* ptr);
@@ -65,8 +80,9 @@ int pmcmc_buffer_upload(pmcmc_ctx* ctx, void* dst_device, const void* host, int6
 int pmcmc_buffer_download(pmcmc_ctx* ctx, void* host, const void* src_device, int64_t nbytes);
 
 /* ---- diagonal affine scan ----------------------------------------------
- * S[:,t] = A[:,t] .* S[:,t-1] + B[:,t], S[:,0] uses s0.  Single pass with
- * decoupled look-back.  Replaces DEERScan.solve_affine_scan_diag!
+ * S[:,t] = A[:,t] .* S[:,t-1] + B[:,t], S[:,0] uses s0.  LDS-tiled
+ * reduce-then-scan (no inter-workgroup waits).  Replaces
+ * DEERScan.solve_affine_scan_diag!
  * (src/DEER/DEERScan.jl:102-173); S must alias neither A nor B (:124-125).
  * T == 0 is a no-op.                                                        */
 int pmcmc_scan_diag_f32(pmcmc_ctx* ctx, const float* A, const float* B, const float* s0,

@@ -8,6 +8,7 @@ instead of user closures.
 """
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from . import _device, _lib
@@ -48,14 +49,23 @@ def mala_step_batched_fwd_and_jvp(model, X, eps, Xi, U, Z, precision=None, devic
 def logdensity_batch(model, S, precision=None, device=None):
     """logp of every column of S as float64 (src/interface.jl:517-524)."""
     tgt = _target_of(model)
-    prec = _precision_of(1.0, precision if precision is not None else
-                         (_lib.PREC_F64 if isinstance(S, torch.Tensor) and S.dtype == torch.float64 else None))
-    if isinstance(S, torch.Tensor) and precision is None:
-        prec = _lib.PREC_F64 if S.dtype == torch.float64 else _lib.PREC_F32
+    if precision is None:
+        is64 = (S.dtype == torch.float64) if isinstance(S, torch.Tensor) else (np.asarray(S).dtype == np.float64)
+        prec = _lib.PREC_F64 if is64 else _lib.PREC_F32
+    else:
+        prec = _precision_of(1.0, precision)
     D, T = S.shape
-    dev = S.device if isinstance(S, torch.Tensor) else torch.device("cuda", device if device is not None else torch.cuda.current_device())
-    S_ = S if (isinstance(S, torch.Tensor) and _device.is_colmajor(S) and S.dtype == _device.torch_dtype(prec)) else \
-        _device.to_colmajor(S, _device.torch_dtype(prec), dev)
+    if D != tgt.dim:
+        raise DimensionMismatch("S rows must equal the model dimension")
+    if isinstance(S, torch.Tensor) and S.device.type == "cuda":
+        dev = S.device
+    else:
+        dev = torch.device("cuda", device if device is not None else torch.cuda.current_device())
+    dt = _device.torch_dtype(prec)
+    if isinstance(S, torch.Tensor) and S.device == dev and S.dtype == dt and _device.is_colmajor(S):
+        S_ = S
+    else:
+        S_ = _device.to_colmajor(S, dt, dev)
     out = torch.empty(T, dtype=torch.float64, device=dev)
     ctx = _device.context(dev.index)
     ctx.call("pmcmc_target_logdensity_batch", tgt.handle(ctx), prec, _device.ptr(S_), _device.ptr(out), T)

@@ -36,6 +36,22 @@ class Context:
         rc = getattr(_lib.load(), name)(self.h, *args)
         _lib.check(rc, self.h, name)
 
+    def set_profiling(self, on=True):
+        _lib.check(_lib.load().pmcmc_ctx_set_profiling(self.h, 1 if on else 0), self.h)
+
+    def reset_kernel_times(self):
+        _lib.check(_lib.load().pmcmc_ctx_reset_kernel_times(self.h), self.h)
+
+    def kernel_times(self):
+        """{name: (launches, total_ms)} of the profiled kernels."""
+        out = {}
+        for k, name in enumerate(_lib.KT_NAMES):
+            n = C.c_int64()
+            ms = C.c_double()
+            _lib.check(_lib.load().pmcmc_ctx_kernel_time(self.h, k, C.byref(n), C.byref(ms)), self.h)
+            out[name] = (int(n.value), float(ms.value))
+        return out
+
     def __del__(self):
         try:
             if getattr(self, "h", None):

@@ -24,6 +24,14 @@ PREC_F64 = 0
 PREC_F32 = 1
 PREC_BF16_JAC = 2
 
+KT_GLM_PASS = 0
+KT_TILE_REDUCE = 1
+KT_TILE_APPLY = 2
+KT_GLM_REDUCE = 3
+KT_SCAN_SPINE = 4
+KT_ITERATION = 5
+KT_NAMES = ("glm_pass", "tile_reduce", "tile_apply", "glm_reduce", "scan_spine", "iteration")
+
 JAC_STOCH_DIAG = 0
 JAC_DIAG = 1
 
@@ -66,6 +74,9 @@ _SIGS = {
     "pmcmc_ctx_destroy": ([_P], C.c_int),
     "pmcmc_last_error": ([_P], C.c_char_p),
     "pmcmc_ctx_synchronize": ([_P], C.c_int),
+    "pmcmc_ctx_set_profiling": ([_P, C.c_int], C.c_int),
+    "pmcmc_ctx_kernel_time": ([_P, C.c_int, C.POINTER(_I64), C.POINTER(C.c_double)], C.c_int),
+    "pmcmc_ctx_reset_kernel_times": ([_P], C.c_int),
     "pmcmc_buffer_alloc": ([_P, _I64, C.POINTER(_P)], C.c_int),
     "pmcmc_buffer_free": ([_P, _P], C.c_int),
     "pmcmc_buffer_upload": ([_P, _P, _P, _I64], C.c_int),

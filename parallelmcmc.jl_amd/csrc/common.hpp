@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #define PMCMC_NT 256  // threads per workgroup of the tile kernels (4 wave64)
 
 namespace pmcmc {
@@ -52,6 +54,76 @@ struct SyncState {
     double last_metric;
     int last_iter;
     int pad;
+};
+
+// ---------------------------------------------------------------------------
+// Optional per-kernel timing with HIP events on the launching stream.  Pairs
+// are tagged with the Newton iteration so that launches queued after
+// convergence (which exit at once) are not counted.
+// ---------------------------------------------------------------------------
+enum { KT_GLM_PASS = 0, KT_TILE_REDUCE = 1, KT_TILE_APPLY = 2, KT_GLM_REDUCE = 3, KT_SCAN_SPINE = 4,
+       KT_ITERATION = 5, KT_COUNT = 6 };
+
+struct Prof {
+    bool on = false;
+    struct Pair {
+        hipEvent_t a, b;
+        int kind, iter;
+    };
+    std::vector<Pair> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[KT_COUNT] = {0};
+    long long n[KT_COUNT] = {0};
+
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    // returns the index of the open pair, or -1
+    int begin(int kind, int iter, hipStream_t s) {
+        if (!on) return -1;
+        Pair p{get(), get(), kind, iter};
+        (void)hipEventRecord(p.a, s);
+        pending.push_back(p);
+        return (int)pending.size() - 1;
+    }
+    void end(int idx, hipStream_t s) {
+        if (idx >= 0) (void)hipEventRecord(pending[idx].b, s);
+    }
+    // after the stream has been synchronised: count pairs of iterations <= last_iter
+    void flush(int last_iter) {
+        for (auto& p : pending) {
+            if (p.iter <= last_iter) {
+                float t = 0.f;
+                if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
+                    ms[p.kind] += t;
+                    n[p.kind] += 1;
+                }
+            }
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void reset() {
+        for (int k = 0; k < KT_COUNT; ++k) {
+            ms[k] = 0;
+            n[k] = 0;
+        }
+    }
+    ~Prof() {
+        for (auto& p : pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
 };
 
 template <typename T>

@@ -1,71 +1,694 @@
-// Logistic-regression target kernels (placeholder until the MFMA path lands).
-#include "glm.hpp"
+// Logistic-regression target: the per-iteration gradient / HVP products of
+// the DEER Newton loop as fused GEMM -> sigmoid -> GEMM^T passes.
+//
+// For a set of columns V (the states x_t and the probe directions z_t of the
+// same t), one pass computes, without materialising any N x T matrix in HBM,
+//   l = Xd x_t, q_k = Xd z_{k,t}               (GEMM-A, N x cols)
+//   r = y - sigmoid(l), u_k = sigmoid'(l) q_k  (elementwise, on chip)
+//   G = Xd^T r, H_k = Xd^T u_k                 (GEMM-B, D x cols)
+//   ll_t = sum_n y log s(l) + (1-y) log s(-l)
+// (benchmarks/ParallelMCMCBenchmarks/src/models/bayes_logreg.jl:118-203 make
+// the same products with 12 separate GEMMs and N x T temporaries).
+// Two passes per Newton iteration: at Xbar (with the probes Z) and at the
+// proposal Y (with W = Z + eps H(Xbar) Z).
+//
+// FP32 path: v_mfma_f32_16x16x4_f32 tiles; the design matrix is streamed
+// through LDS in 32-row chunks and used by both GEMMs; the column block stays
+// in LDS; GEMM-B accumulates in registers across the N range of the
+// workgroup; N is split over blockIdx.y and the splits are summed by
+// glm_reduce_kernel in a fixed order (deterministic).
+// FP64 path: plain kernels (small problems and parity runs).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
 
 #include "../../include/pmcmc_amd.h"
+#include "glm.hpp"
 
 namespace pmcmc {
 
-int glm_data_create(GlmData* g, hipStream_t, int64_t N, int64_t D, bool, const void*, const void*,
-                    std::string* msg) {
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+static int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+static int grid_cap(int64_t n, int block = 256, int cap = 8192) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+__device__ inline float softplusf(float x) { return fmaxf(x, 0.f) + log1pf(expf(-fabsf(x))); }
+__device__ inline double softplusd(double x) { return fmax(x, 0.0) + log1p(exp(-fabs(x))); }
+__device__ inline float sigmoidf_(float x) {
+    if (x >= 0.f) return 1.f / (1.f + expf(-x));
+    const float e = expf(x);
+    return e / (1.f + e);
+}
+__device__ inline double sigmoidd_(double x) {
+    if (x >= 0.0) return 1.0 / (1.0 + exp(-x));
+    const double e = exp(x);
+    return e / (1.0 + e);
+}
+
+// ---------------------------------------------------------------------------
+// data preparation
+// ---------------------------------------------------------------------------
+template <typename Tin>
+__global__ void glm_prep_kernel(const Tin* Xcm, const Tin* y, int64_t N, int64_t D, int64_t Np,
+                                int64_t Dp, float* Xrm, float* y32, double* X64, double* y64) {
+    const int64_t n_all = Np * Dp;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_all;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t n = i / Dp, d = i % Dp;
+        const bool in = n < N && d < D;
+        const double v = in ? (double)Xcm[n + d * N] : 0.0;
+        Xrm[i] = (float)v;
+        if (in) X64[n + d * N] = v;
+        if (d == 0) {
+            const double yv = n < N ? (double)y[n] : 0.0;
+            y32[n] = (float)yv;
+            if (n < N) y64[n] = yv;
+        }
+    }
+}
+
+int glm_data_create(GlmData* g, hipStream_t s, int64_t N, int64_t D, bool src_f64, const void* Xd,
+                    const void* y, std::string* msg) {
     g->N = N;
     g->D = D;
-    *msg = "logistic target: not built yet";
-    return PMCMC_ERR_UNSUPPORTED;
+    g->Dp = round_up(D, 16);
+    const int64_t Np = round_up(N, 32);
+    bool ok = hipMalloc(&g->Xrm, Np * g->Dp * sizeof(float)) == hipSuccess &&
+              hipMalloc(&g->y, Np * sizeof(float)) == hipSuccess &&
+              hipMalloc(&g->Xd64, N * D * sizeof(double)) == hipSuccess &&
+              hipMalloc(&g->y64, N * sizeof(double)) == hipSuccess;
+    if (!ok) {
+        *msg = "logistic target: device allocation failed";
+        return PMCMC_ERR_ALLOC;
+    }
+    const int grid = grid_cap(Np * g->Dp);
+    if (src_f64)
+        hipLaunchKernelGGL((glm_prep_kernel<double>), dim3(grid), dim3(256), 0, s, (const double*)Xd,
+                           (const double*)y, N, D, Np, g->Dp, g->Xrm, g->y, g->Xd64, g->y64);
+    else
+        hipLaunchKernelGGL((glm_prep_kernel<float>), dim3(grid), dim3(256), 0, s, (const float*)Xd,
+                           (const float*)y, N, D, Np, g->Dp, g->Xrm, g->y, g->Xd64, g->y64);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        *msg = "logistic target: preparation kernel failed";
+        return PMCMC_ERR_HIP;
+    }
+    return PMCMC_OK;
 }
 
 void glm_data_destroy(GlmData* g) {
-    hipFree(g->Xrm);
-    hipFree(g->Xcm);
-    hipFree(g->Xd64);
-    hipFree(g->y);
-    hipFree(g->y64);
+    (void)hipFree(g->Xrm);
+    (void)hipFree(g->Xcm);
+    (void)hipFree(g->Xd64);
+    (void)hipFree(g->y);
+    (void)hipFree(g->y64);
     *g = GlmData{};
 }
 
-int glm_work_ensure(GlmWork*, const GlmData*, hipStream_t, int64_t, int64_t, int, bool,
-                    std::string* msg) {
-    *msg = "logistic target: not built yet";
-    return PMCMC_ERR_UNSUPPORTED;
+// ---------------------------------------------------------------------------
+// work buffers
+// ---------------------------------------------------------------------------
+static constexpr int CB = 32;   // t-columns per column block of the MFMA pass
+static constexpr int NB = 32;   // design rows per LDS chunk
+
+static int glm_splits(int64_t N, int64_t T) {
+    // enough workgroups to fill 256 CUs twice, chunks of >= 8 row blocks each
+    const int64_t nblk = (T + CB - 1) / CB;
+    int64_t ns = (512 + nblk - 1) / nblk;
+    const int64_t nchunks = (N + NB - 1) / NB;
+    if (ns > nchunks / 8) ns = nchunks / 8;
+    if (ns < 1) ns = 1;
+    if (ns > 64) ns = 64;
+    return (int)ns;
+}
+
+int glm_work_ensure(GlmWork* w, const GlmData* g, hipStream_t, int64_t D, int64_t T, int probes,
+                    bool f64, std::string* msg) {
+    const size_t esz = f64 ? 8 : 4;
+    if (w->Y && w->D == D && w->T == T && w->probes == probes && w->esz == esz) return PMCMC_OK;
+    glm_work_destroy(w);
+    w->D = D;
+    w->T = T;
+    w->probes = probes;
+    w->esz = esz;
+    const size_t mat = (size_t)D * T * esz;
+    const int ns = f64 ? 1 : glm_splits(g->N, T);
+    // partial sums: [split][set][T][Dp] floats (f32) or, for the f64 path,
+    // the N x T residual matrices [set][T][N] plus [set][T][Dp] sums
+    size_t part = f64 ? ((size_t)(1 + probes) * T * (g->N + g->Dp)) * sizeof(double)
+                      : (size_t)ns * (1 + probes) * T * g->Dp * sizeof(float);
+    part += (size_t)ns * T * sizeof(double) * (f64 ? (size_t)g->N : 1);  // log-lik partials
+    w->part_bytes = part;
+    bool ok = hipMalloc(&w->Y, mat) == hipSuccess && hipMalloc(&w->gx, mat) == hipSuccess &&
+              hipMalloc(&w->gy, mat) == hipSuccess &&
+              hipMalloc(&w->W, mat * probes) == hipSuccess &&
+              hipMalloc(&w->hw, mat * probes) == hipSuccess &&
+              hipMalloc(&w->lpx, T * sizeof(double)) == hipSuccess &&
+              hipMalloc(&w->lpy, T * sizeof(double)) == hipSuccess &&
+              hipMalloc(&w->part, part) == hipSuccess;
+    if (!ok) {
+        glm_work_destroy(w);
+        *msg = "logistic workspace allocation failed";
+        return PMCMC_ERR_ALLOC;
+    }
+    return PMCMC_OK;
 }
 
 void glm_work_destroy(GlmWork* w) {
-    hipFree(w->Y);
-    hipFree(w->gx);
-    hipFree(w->gy);
-    hipFree(w->W);
-    hipFree(w->hw);
-    hipFree(w->lpx);
-    hipFree(w->lpy);
-    hipFree(w->part);
+    (void)hipFree(w->Y);
+    (void)hipFree(w->gx);
+    (void)hipFree(w->gy);
+    (void)hipFree(w->W);
+    (void)hipFree(w->hw);
+    (void)hipFree(w->lpx);
+    (void)hipFree(w->lpy);
+    (void)hipFree(w->part);
     *w = GlmWork{};
 }
 
+// ---------------------------------------------------------------------------
+// column sources of a pass
+// ---------------------------------------------------------------------------
+// Set 0 holds the states: column t is V[:, t], or with `shift` the iterate
+// shifted by one step (x0 for t = 0, V[:, t-1] otherwise) = Xbar.
+// Sets 1..K hold directions: Zsrc[k] if given, else the Rademacher probe k.
 template <typename T>
-int glm_newton_products(GlmWork*, const GlmData*, hipStream_t, SyncState*, int, double, double,
-                        const T*, const T*, const T*, uint64_t, int, int, int64_t, int64_t,
+struct ColSrc {
+    const T* V;
+    const T* x0;  // non-null: shifted
+    const T* Z;   // K x D x T or null
+    uint32_t keys[4];
+    int K;
+    int64_t D, Tn;
+
+    __device__ inline double get(int set, int64_t t, int64_t d) const {
+        if (set == 0) {
+            if (x0) return t == 0 ? (double)x0[d] : (double)V[(size_t)(t - 1) * D + d];
+            return (double)V[(size_t)t * D + d];
+        }
+        const size_t gi = (size_t)t * D + d;
+        if (Z) return (double)Z[(size_t)(set - 1) * D * Tn + gi];
+        return (double)rademacher(keys[set - 1], gi);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// FP32 MFMA fused pass
+// ---------------------------------------------------------------------------
+// grid: (ceil(T/CB), NS); block 256 = 4 waves.
+// LDS: Vs[NC][Dp+2] column block, Xs[NB][Dp+2] design chunk, Rs[NB][NC+16].
+template <int DPT, int K>
+__global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, const float* Xrm,
+                                                            const float* y32, int64_t N,
+                                                            int64_t rows_per_split,
+                                                            float* part, double* llpart,
+                                                            const int* skip) {
+    constexpr int Dp = DPT * 16;
+    constexpr int NC = CB * (1 + K);
+    constexpr int SV = Dp + 2;
+    constexpr int SX = Dp + 2;
+    constexpr int SR = NC + 16;
+    constexpr int NCT = NC / 16;             // column tiles
+    constexpr int CT_PER_WAVE = (NCT + 3) / 4;
+    __shared__ float Vs[NC * SV];
+    __shared__ float Xs[NB * SX];
+    __shared__ float Rs[NB * SR];
+    __shared__ double llred[4][16];
+
+    if (skip && *(volatile const int*)skip) return;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int li = lane & 15;  // row (A) / column (B, C) within a tile
+    const int lk = lane >> 4;  // k index within a 4-step (A, B); row group (C)
+    const int64_t D = src.D, T = src.Tn;
+    const int64_t cb = blockIdx.x;
+    const int split = blockIdx.y;
+
+    // 1. column block into LDS (zero padded)
+    for (int i = tid; i < NC * Dp; i += 256) {
+        const int j = i / Dp, d = i % Dp;
+        const int set = j / CB;
+        const int64_t t = cb * CB + (j % CB);
+        float v = 0.f;
+        if (t < T && d < D) v = (float)src.get(set, t, d);
+        Vs[j * SV + d] = v;
+    }
+
+    v4f acc[CT_PER_WAVE][DPT];
+#pragma unroll
+    for (int c = 0; c < CT_PER_WAVE; ++c)
+#pragma unroll
+        for (int dt = 0; dt < DPT; ++dt) acc[c][dt] = (v4f){0.f, 0.f, 0.f, 0.f};
+    double ll = 0.0;
+
+    const int rt = wave >> 1;  // GEMM-A row tile
+    const int cx = wave & 1;   // GEMM-A x column tile
+    const int64_t n_begin = (int64_t)split * rows_per_split;
+    int64_t n_end = n_begin + rows_per_split;
+    if (n_end > N) n_end = N;
+
+    for (int64_t n0 = n_begin; n0 < n_end; n0 += NB) {
+        __syncthreads();  // previous chunk fully consumed (and Vs written)
+        // 2. design chunk into LDS; rows >= N are zero in Xrm (padded to 32)
+        for (int i = tid; i < NB * Dp; i += 256) {
+            const int r = i / Dp, d = i % Dp;
+            Xs[r * SX + d] = Xrm[(size_t)(n0 + r) * Dp + d];
+        }
+        __syncthreads();
+        // 3. GEMM-A: logits of the x tile and the matching probe tiles
+        v4f cl[1 + K];
+#pragma unroll
+        for (int s = 0; s <= K; ++s) cl[s] = (v4f){0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+        for (int ks = 0; ks < Dp / 4; ++ks) {
+            const float av = Xs[(16 * rt + li) * SX + 4 * ks + lk];
+#pragma unroll
+            for (int s = 0; s <= K; ++s) {
+                const int col = s * CB + 16 * cx + li;
+                const float bv = Vs[col * SV + 4 * ks + lk];
+                cl[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, cl[s], 0, 0, 0);
+            }
+        }
+        // 4. elementwise: residuals and sigmoid' weighted directions into LDS
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * rt + 4 * lk + r;
+            const int64_t n = n0 + row;
+            const bool valid = n < N;
+            const float l = cl[0][r];
+            const float sg = sigmoidf_(l);
+            const float yv = y32[n];
+            if (valid) ll += (double)(-yv * softplusf(-l) - (1.f - yv) * softplusf(l));
+            Rs[row * SR + 16 * cx + li] = valid ? (yv - sg) : 0.f;
+            const float wgt = sg * (1.f - sg);
+#pragma unroll
+            for (int s = 1; s <= K; ++s) Rs[row * SR + s * CB + 16 * cx + li] = valid ? wgt * cl[s][r] : 0.f;
+        }
+        __syncthreads();
+        // 5. GEMM-B: accumulate Xd^T R
+#pragma unroll
+        for (int ks = 0; ks < NB / 4; ++ks) {
+            float xa[DPT];
+#pragma unroll
+            for (int dt = 0; dt < DPT; ++dt) xa[dt] = Xs[(4 * ks + lk) * SX + 16 * dt + li];
+#pragma unroll
+            for (int c = 0; c < CT_PER_WAVE; ++c) {
+                const int ct = wave + 4 * c;
+                if (ct < NCT) {
+                    const float bv = Rs[(4 * ks + lk) * SR + 16 * ct + li];
+#pragma unroll
+                    for (int dt = 0; dt < DPT; ++dt)
+                        acc[c][dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[dt], bv, acc[c][dt], 0, 0, 0);
+                }
+            }
+        }
+    }
+
+    // 6. partial sums out: part[split][set][t][Dp]
+#pragma unroll
+    for (int c = 0; c < CT_PER_WAVE; ++c) {
+        const int ct = wave + 4 * c;
+        if (ct >= NCT) continue;
+        const int j = 16 * ct + li;
+        const int set = j / CB;
+        const int64_t t = cb * CB + (j % CB);
+        if (t >= T) continue;
+        float* dst = part + (((size_t)split * (1 + K) + set) * T + t) * Dp;
+#pragma unroll
+        for (int dt = 0; dt < DPT; ++dt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) dst[16 * dt + 4 * lk + r] = acc[c][dt][r];
+    }
+    // log-likelihood of column 16*cx + li: lanes li, li+16, li+32, li+48 and waves cx, cx+2
+    ll += __shfl_xor(ll, 16, 64);
+    ll += __shfl_xor(ll, 32, 64);
+    if (lk == 0) llred[wave][li] = ll;
+    __syncthreads();
+    if (tid < CB) {
+        const int cxx = tid / 16, l2 = tid % 16;
+        const double v = llred[cxx][l2] + llred[cxx + 2][l2];
+        const int64_t t = cb * CB + tid;
+        if (t < T) llpart[(size_t)split * T + t] = v;
+    }
+}
+
+// Sum the splits, add the prior terms, and form the next pass's columns.
+//   pass 0 (at Xbar): gx = G0 - x/s^2 ; hz_k = -G_k - z_k/s^2 ;
+//                     Y = x + eps gx + sqrt(2 eps) xi ; W_k = z_k + eps hz_k ;
+//                     lpx = ll - |x|^2 / (2 s^2)
+//   pass 1 (at Y):    gy = G0 - y/s^2 ; hw_k = -G_k - w_k/s^2 ; lpy likewise
+// One wave per column.
+template <typename T, typename P>
+__global__ __launch_bounds__(256) void glm_reduce_kernel(ColSrc<T> src, const P* part, const double* llpart,
+                                                         int ns, int64_t Dp, int64_t llstride, int pass,
+                                                         double inv_s2, double eps, const T* Xi, T* gout,
+                                                         T* hout, double* lpout, T* Yout, T* Wout,
+                                                         const int* skip) {
+    if (skip && *(volatile const int*)skip) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t D = src.D, Tn = src.Tn;
+    const int K = src.K;
+    if (t >= Tn) return;
+    const double sq2e = sqrt(2.0 * eps);
+    double nrm = 0.0;
+    for (int64_t d = lane; d < D; d += 64) {
+        const double x = src.get(0, t, d);
+        double G = 0.0;
+        for (int s = 0; s < ns; ++s) G += (double)part[(((size_t)s * (1 + K) + 0) * Tn + t) * Dp + d];
+        const double g = G - x * inv_s2;
+        const size_t gi = (size_t)t * D + d;
+        gout[gi] = (T)g;
+        nrm += x * x;
+        double y = 0.0;
+        if (pass == 0) {
+            y = x + eps * g + sq2e * (double)Xi[gi];
+            Yout[gi] = (T)y;
+        }
+        for (int k = 0; k < K; ++k) {
+            double H = 0.0;
+            for (int s = 0; s < ns; ++s)
+                H += (double)part[(((size_t)s * (1 + K) + 1 + k) * Tn + t) * Dp + d];
+            const double z = src.get(1 + k, t, d);
+            const double h = -H - z * inv_s2;
+            hout[(size_t)k * D * Tn + gi] = (T)h;
+            if (pass == 0) Wout[(size_t)k * D * Tn + gi] = (T)(z + eps * h);
+        }
+    }
+    for (int m = 32; m >= 1; m >>= 1) nrm += __shfl_xor(nrm, m, 64);
+    if (lane == 0) {
+        double ll = 0.0;
+        for (int s = 0; s < ns; ++s) ll += llpart[(size_t)s * llstride + t];
+        lpout[t] = ll - 0.5 * nrm * inv_s2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 plain pass: residual matrices R[set][t][n] then Xd^T R.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void glm_naive_fwd_kernel(ColSrc<T> src, const double* X64, const double* y64, int64_t N,
+                                     double* R, double* llnt, const int* skip) {
+    if (skip && *(volatile const int*)skip) return;
+    const int64_t D = src.D, Tn = src.Tn;
+    const int K = src.K;
+    const int64_t total = N * Tn;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t n = i % N, t = i / N;
+        double l = 0.0;
+        for (int64_t d = 0; d < D; ++d) l += X64[n + d * N] * src.get(0, t, d);
+        const double sg = sigmoidd_(l);
+        const double yv = y64[n];
+        R[((size_t)0 * Tn + t) * N + n] = yv - sg;
+        llnt[(size_t)t * N + n] = -yv * softplusd(-l) - (1.0 - yv) * softplusd(l);
+        for (int k = 0; k < K; ++k) {
+            double q = 0.0;
+            for (int64_t d = 0; d < D; ++d) q += X64[n + d * N] * src.get(1 + k, t, d);
+            R[((size_t)(1 + k) * Tn + t) * N + n] = sg * (1.0 - sg) * q;
+        }
+    }
+}
+
+// sums[set][t][Dp] = Xd^T R[set][t]; llsum[t] = sum_n llnt[t][n]
+template <typename T>
+__global__ void glm_naive_bwd_kernel(int K, int64_t D, int64_t Dp, int64_t Tn, const double* X64,
+                                     int64_t N, const double* R, const double* llnt, double* sums,
+                                     double* llsum, const int* skip) {
+    if (skip && *(volatile const int*)skip) return;
+    const int64_t total = (int64_t)(1 + K) * Tn * D;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t d = i % D;
+        const int64_t st = i / D;  // set * Tn + t
+        const double* r = R + (size_t)st * N;
+        double acc = 0.0;
+        for (int64_t n = 0; n < N; ++n) acc += X64[n + d * N] * r[n];
+        sums[(size_t)st * Dp + d] = acc;
+        if (d == 0 && st < Tn) {
+            double s = 0.0;
+            const double* lr = llnt + (size_t)st * N;
+            for (int64_t n = 0; n < N; ++n) s += lr[n];
+            llsum[st] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one pass (dispatch)
+// ---------------------------------------------------------------------------
+template <int DPT>
+static void launch_mfma(const ColSrc<float>& src, const GlmData* g, int ns, int64_t rows_per_split,
+                        float* part, double* llpart, const int* skip, hipStream_t s) {
+    dim3 grid((unsigned)((src.Tn + CB - 1) / CB), (unsigned)ns);
+    hipLaunchKernelGGL((glm_pass_mfma_kernel<DPT, 1>), grid, dim3(256), 0, s, src, g->Xrm, g->y, g->N,
+                       rows_per_split, part, llpart, skip);
+}
+
+template <typename T>
+static int glm_pass(const GlmData* g, GlmWork* w, hipStream_t s, const ColSrc<T>& src, int pass,
+                    double eps, double prior_scale, const T* Xi, T* gout, T* hout, double* lpout,
+                    T* Yout, T* Wout, const int* skip, Prof* prof, int iter, std::string* msg) {
+    const int64_t Tn = src.Tn;
+    const int K = src.K;
+    const double inv_s2 = 1.0 / (prior_scale * prior_scale);
+    const int rgrid = (int)((Tn + 3) / 4);
+    const bool mfma = sizeof(T) == 4 && K == 1 && g->Dp <= 256;
+    if (mfma) {
+        const int ns = glm_splits(g->N, Tn);
+        const int64_t nch = (g->N + NB - 1) / NB;
+        const int64_t rows_per_split = ((nch + ns - 1) / ns) * NB;
+        float* part = (float*)w->part;
+        double* llpart = (double*)((char*)w->part + (size_t)ns * (1 + K) * Tn * g->Dp * sizeof(float));
+        const ColSrc<float>& fs = reinterpret_cast<const ColSrc<float>&>(src);
+        const int pe = prof ? prof->begin(KT_GLM_PASS, iter, s) : -1;
+        switch (g->Dp / 16) {
+#define PMCMC_DPT(n) \
+    case n: launch_mfma<n>(fs, g, ns, rows_per_split, part, llpart, skip, s); break;
+            PMCMC_DPT(1) PMCMC_DPT(2) PMCMC_DPT(3) PMCMC_DPT(4) PMCMC_DPT(5) PMCMC_DPT(6)
+            PMCMC_DPT(7) PMCMC_DPT(8) PMCMC_DPT(9) PMCMC_DPT(10) PMCMC_DPT(11) PMCMC_DPT(12)
+            PMCMC_DPT(13) PMCMC_DPT(14) PMCMC_DPT(15) PMCMC_DPT(16)
+#undef PMCMC_DPT
+            default:
+                *msg = "logistic target: unsupported D";
+                return PMCMC_ERR_UNSUPPORTED;
+        }
+        if (prof) prof->end(pe, s);
+        const int pr = prof ? prof->begin(KT_GLM_REDUCE, iter, s) : -1;
+        hipLaunchKernelGGL((glm_reduce_kernel<T, float>), dim3(rgrid), dim3(256), 0, s, src, part, llpart, ns,
+                           g->Dp, Tn, pass, inv_s2, eps, Xi, gout, hout, lpout, Yout, Wout, skip);
+        if (prof) prof->end(pr, s);
+    } else {
+        // plain path: residual matrices in the part buffer
+        double* R = (double*)w->part;
+        double* sums = R + (size_t)(1 + K) * Tn * g->N;
+        double* llnt = sums + (size_t)(1 + K) * Tn * g->Dp;
+        double* llsum = llnt + (size_t)Tn * g->N;
+        const size_t need = ((size_t)(1 + K) * Tn * (g->N + g->Dp) + (size_t)Tn * g->N + Tn) * sizeof(double);
+        if (need > w->part_bytes) {
+            (void)hipFree(w->part);
+            w->part = nullptr;
+            if (hipMalloc(&w->part, need) != hipSuccess) {
+                *msg = "logistic workspace allocation failed";
+                return PMCMC_ERR_ALLOC;
+            }
+            w->part_bytes = need;
+            R = (double*)w->part;
+            sums = R + (size_t)(1 + K) * Tn * g->N;
+            llnt = sums + (size_t)(1 + K) * Tn * g->Dp;
+            llsum = llnt + (size_t)Tn * g->N;
+        }
+        hipLaunchKernelGGL((glm_naive_fwd_kernel<T>), dim3(grid_cap(g->N * Tn)), dim3(256), 0, s, src, g->Xd64,
+                           g->y64, g->N, R, llnt, skip);
+        hipLaunchKernelGGL((glm_naive_bwd_kernel<T>), dim3(grid_cap((1 + K) * Tn * src.D)), dim3(256), 0, s, K,
+                           src.D, g->Dp, Tn, g->Xd64, g->N, R, llnt, sums, llsum, skip);
+        hipLaunchKernelGGL((glm_reduce_kernel<T, double>), dim3(rgrid), dim3(256), 0, s, src, sums, llsum, 1,
+                           g->Dp, Tn, pass, inv_s2, eps, Xi, gout, hout, lpout, Yout, Wout, skip);
+    }
+    if (hipGetLastError() != hipSuccess) {
+        *msg = "logistic pass: kernel launch failed";
+        return PMCMC_ERR_HIP;
+    }
+    return PMCMC_OK;
+}
+
+template <typename T>
+int glm_newton_products(GlmWork* w, const GlmData* g, hipStream_t s, SyncState* sync, int precision,
+                        double eps, double prior_scale, const T* x0, const T* S_in, const T* Xi,
+                        uint64_t seed, int iter, int probes, int64_t D, int64_t Tn, Prof* prof,
                         std::string* msg) {
-    *msg = "logistic target: not built yet";
-    return PMCMC_ERR_UNSUPPORTED;
+    (void)precision;
+    const int* skip = sync ? &sync->converged : nullptr;
+    ColSrc<T> a{};
+    a.V = S_in;
+    a.x0 = x0;
+    a.Z = nullptr;
+    a.K = probes;
+    a.D = D;
+    a.Tn = Tn;
+    for (int k = 0; k < probes && k < 4; ++k) a.keys[k] = probe_key(seed, (uint32_t)iter, (uint32_t)k);
+    int rc = glm_pass<T>(g, w, s, a, 0, eps, prior_scale, Xi, (T*)w->gx, (T*)w->hw, w->lpx, (T*)w->Y,
+                         (T*)w->W, skip, prof, iter, msg);
+    if (rc != PMCMC_OK) return rc;
+    ColSrc<T> b{};
+    b.V = (const T*)w->Y;
+    b.x0 = nullptr;
+    b.Z = (const T*)w->W;
+    b.K = probes;
+    b.D = D;
+    b.Tn = Tn;
+    return glm_pass<T>(g, w, s, b, 1, eps, prior_scale, Xi, (T*)w->gy, (T*)w->hw, w->lpy, nullptr, nullptr,
+                       skip, prof, iter, msg);
 }
 template int glm_newton_products<float>(GlmWork*, const GlmData*, hipStream_t, SyncState*, int,
                                         double, double, const float*, const float*, const float*,
-                                        uint64_t, int, int, int64_t, int64_t, std::string*);
+                                        uint64_t, int, int, int64_t, int64_t, Prof*, std::string*);
 template int glm_newton_products<double>(GlmWork*, const GlmData*, hipStream_t, SyncState*, int,
                                          double, double, const double*, const double*,
-                                         const double*, uint64_t, int, int, int64_t, int64_t,
+                                         const double*, uint64_t, int, int, int64_t, int64_t, Prof*,
                                          std::string*);
 
-int glm_fwd_and_jvp(const GlmData*, hipStream_t, int, double, double, const void*, const void*,
-                    const void*, const void*, void*, void*, uint8_t*, double*, int64_t, int64_t,
-                    std::string* msg) {
-    *msg = "logistic target: not built yet";
-    return PMCMC_ERR_UNSUPPORTED;
+// ---------------------------------------------------------------------------
+// standalone batched MALA step + JVP, and log-densities
+// ---------------------------------------------------------------------------
+// One wave per column: logalpha, accept, FT, Jt from the two passes' products.
+template <typename T>
+__global__ __launch_bounds__(256) void glm_finish_kernel(const T* X, const T* Z, const T* U, const T* Y,
+                                                         const T* gx, const T* W, const T* gy,
+                                                         const T* hw, const double* lpx,
+                                                         const double* lpy, double eps, int64_t D,
+                                                         int64_t Tn, T* FT, T* Jt, uint8_t* acc_out,
+                                                         double* la_out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= Tn) return;
+    const double c_norm = ((double)D / 2.0) * log(4.0 * 3.14159265358979323846 * eps);
+    double qyx = 0, qxy = 0, d1 = 0, d2 = 0, d3 = 0;
+    for (int64_t d = lane; d < D; d += 64) {
+        const size_t gi = (size_t)t * D + d;
+        const double x = X[gi], y = Y[gi], g0 = gx[gi], g1 = gy[gi], z = Z[gi], w = W[gi], h = hw[gi];
+        const double e1 = y - x - eps * g0, r = x - y - eps * g1, dr = z - w - eps * h;
+        qyx += e1 * e1;
+        qxy += r * r;
+        d1 += g1 * w;
+        d2 += g0 * z;
+        d3 += r * dr;
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        qyx += __shfl_xor(qyx, m, 64);
+        qxy += __shfl_xor(qxy, m, 64);
+        d1 += __shfl_xor(d1, m, 64);
+        d2 += __shfl_xor(d2, m, 64);
+        d3 += __shfl_xor(d3, m, 64);
+    }
+    const double la = (lpy[t] + (-0.5 * qxy / (2.0 * eps) - c_norm)) - (lpx[t] + (-0.5 * qyx / (2.0 * eps) - c_norm));
+    const double lu = log((double)U[t]);
+    const double g = 1.0 / (1.0 + exp(-(la - lu)));
+    const bool acc = lu < la;
+    const double dg = g * (1.0 - g) * (d1 - d2 - d3 / (2.0 * eps));
+    if (lane == 0) {
+        if (acc_out) acc_out[t] = acc ? 1 : 0;
+        if (la_out) la_out[t] = la;
+    }
+    for (int64_t d = lane; d < D; d += 64) {
+        const size_t gi = (size_t)t * D + d;
+        const double x = X[gi], y = Y[gi], z = Z[gi], w = W[gi];
+        FT[gi] = (T)(acc ? y : x);
+        Jt[gi] = (T)((acc ? w : z) + (y - x) * dg);
+    }
 }
 
-int glm_logdensity(const GlmData*, hipStream_t, int, double, const void*, double*, int64_t,
-                   std::string* msg) {
-    *msg = "logistic target: not built yet";
-    return PMCMC_ERR_UNSUPPORTED;
+template <typename T>
+static int fwd_jvp_t(const GlmData* g, hipStream_t s, double eps, double prior_scale, const T* X,
+                     const T* Xi, const T* U, const T* Z, T* FT, T* Jt, uint8_t* acc, double* la,
+                     int64_t D, int64_t Tn, std::string* msg) {
+    GlmWork w{};
+    int rc = glm_work_ensure(&w, g, s, D, Tn, 1, sizeof(T) == 8, msg);
+    if (rc != PMCMC_OK) return rc;
+    ColSrc<T> a{};
+    a.V = X;
+    a.x0 = nullptr;
+    a.Z = Z;
+    a.K = 1;
+    a.D = D;
+    a.Tn = Tn;
+    rc = glm_pass<T>(g, &w, s, a, 0, eps, prior_scale, Xi, (T*)w.gx, (T*)w.hw, w.lpx, (T*)w.Y, (T*)w.W, nullptr,
+                     nullptr, 0, msg);
+    if (rc == PMCMC_OK) {
+        ColSrc<T> b = a;
+        b.V = (const T*)w.Y;
+        b.Z = (const T*)w.W;
+        rc = glm_pass<T>(g, &w, s, b, 1, eps, prior_scale, Xi, (T*)w.gy, (T*)w.hw, w.lpy, nullptr, nullptr, nullptr,
+                         nullptr, 0, msg);
+    }
+    if (rc == PMCMC_OK) {
+        hipLaunchKernelGGL((glm_finish_kernel<T>), dim3((unsigned)((Tn + 3) / 4)), dim3(256), 0, s, X, Z, U,
+                           (const T*)w.Y, (const T*)w.gx, (const T*)w.W, (const T*)w.gy, (const T*)w.hw,
+                           w.lpx, w.lpy, eps, D, Tn, FT, Jt, acc, la);
+        if (hipGetLastError() != hipSuccess) {
+            *msg = "logistic finish kernel failed";
+            rc = PMCMC_ERR_HIP;
+        }
+    }
+    hipStreamSynchronize(s);
+    glm_work_destroy(&w);
+    return rc;
+}
+
+int glm_fwd_and_jvp(const GlmData* g, hipStream_t s, int prec, double eps, double prior_scale,
+                    const void* X, const void* Xi, const void* U, const void* Z, void* FT, void* Jt,
+                    uint8_t* accepted, double* logalpha, int64_t D, int64_t T, std::string* msg) {
+    if (prec == PMCMC_PREC_F64)
+        return fwd_jvp_t<double>(g, s, eps, prior_scale, (const double*)X, (const double*)Xi,
+                                 (const double*)U, (const double*)Z, (double*)FT, (double*)Jt, accepted,
+                                 logalpha, D, T, msg);
+    return fwd_jvp_t<float>(g, s, eps, prior_scale, (const float*)X, (const float*)Xi, (const float*)U,
+                            (const float*)Z, (float*)FT, (float*)Jt, accepted, logalpha, D, T, msg);
+}
+
+// logp of every column: one workgroup per column, threads over n, in double.
+template <typename T>
+__global__ __launch_bounds__(256) void glm_logp_kernel(const T* S, const double* X64, const double* y64,
+                                                       int64_t N, int64_t D, double inv_s2, double* out) {
+    __shared__ double red[4];
+    const int64_t t = blockIdx.x;
+    const T* x = S + (size_t)t * D;
+    double acc = 0.0;
+    for (int64_t n = threadIdx.x; n < N; n += 256) {
+        double l = 0.0;
+        for (int64_t d = 0; d < D; ++d) l += X64[n + d * N] * (double)x[d];
+        const double yv = y64[n];
+        acc += -yv * softplusd(-l) - (1.0 - yv) * softplusd(l);
+    }
+    double nrm = 0.0;
+    for (int64_t d = threadIdx.x; d < D; d += 256) nrm += (double)x[d] * (double)x[d];
+    acc -= 0.5 * nrm * inv_s2;
+    for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[t] = red[0] + red[1] + red[2] + red[3];
+}
+
+int glm_logdensity(const GlmData* g, hipStream_t s, int prec, double prior_scale, const void* S,
+                   double* out, int64_t T, std::string* msg) {
+    const double inv_s2 = 1.0 / (prior_scale * prior_scale);
+    if (prec == PMCMC_PREC_F64)
+        hipLaunchKernelGGL((glm_logp_kernel<double>), dim3((unsigned)T), dim3(256), 0, s, (const double*)S,
+                           g->Xd64, g->y64, g->N, g->D, inv_s2, out);
+    else
+        hipLaunchKernelGGL((glm_logp_kernel<float>), dim3((unsigned)T), dim3(256), 0, s, (const float*)S,
+                           g->Xd64, g->y64, g->N, g->D, inv_s2, out);
+    if (hipGetLastError() != hipSuccess) {
+        *msg = "logistic log-density kernel failed";
+        return PMCMC_ERR_HIP;
+    }
+    return PMCMC_OK;
 }
 
 }  // namespace pmcmc

@@ -50,7 +50,7 @@ void glm_work_destroy(GlmWork* w);
 template <typename T>
 int glm_newton_products(GlmWork* w, const GlmData* g, hipStream_t s, SyncState* sync, int precision,
                         double eps, double prior_scale, const T* x0, const T* S_in, const T* Xi,
-                        uint64_t seed, int iter, int probes, int64_t D, int64_t Tn,
+                        uint64_t seed, int iter, int probes, int64_t D, int64_t Tn, Prof* prof,
                         std::string* msg);
 
 int glm_fwd_and_jvp(const GlmData* g, hipStream_t s, int prec, double eps, double prior_scale,

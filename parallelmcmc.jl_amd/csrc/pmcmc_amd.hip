@@ -46,6 +46,7 @@ struct pmcmc_ctx {
     void* scan_blob = nullptr;
     size_t scan_blob_cap = 0;
     SyncState* host_sync = nullptr;  // pinned
+    Prof prof;
 };
 
 enum { TGT_DIAG = 0, TGT_LOGISTIC = 1 };
@@ -353,6 +354,25 @@ extern "C" int pmcmc_ctx_synchronize(pmcmc_ctx* c) {
     return PMCMC_OK;
 }
 
+extern "C" int pmcmc_ctx_set_profiling(pmcmc_ctx* c, int on) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    c->prof.on = on != 0;
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_ctx_kernel_time(pmcmc_ctx* c, int kind, int64_t* launches, double* total_ms) {
+    if (!c || kind < 0 || kind >= KT_COUNT) return fail(c, PMCMC_ERR_ARGUMENT, "bad kernel-time kind %d", kind);
+    if (launches) *launches = c->prof.n[kind];
+    if (total_ms) *total_ms = c->prof.ms[kind];
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_ctx_reset_kernel_times(pmcmc_ctx* c) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    c->prof.reset();
+    return PMCMC_OK;
+}
+
 extern "C" int pmcmc_buffer_alloc(pmcmc_ctx* c, int64_t nbytes, void** out) {
     if (!c || !out || nbytes < 0) return fail(c, PMCMC_ERR_ARGUMENT, "bad buffer request");
     HIPCHK(c, hipSetDevice(c->device));
@@ -631,7 +651,7 @@ extern "C" int pmcmc_deer_ws_destroy(pmcmc_deer_ws* w) {
 // One Newton iteration on the device: [GEMM products] -> level-0 reduce ->
 // scan of the tile aggregates -> level-0 apply (damping, maxima, metric).
 template <typename T, int MODE>
-static void launch_deer_iteration(pmcmc_deer_ws* w, TileArgs<T> a, hipStream_t s) {
+static void launch_deer_iteration(pmcmc_deer_ws* w, TileArgs<T> a, hipStream_t s, Prof* prof) {
     const ScanPlan& p = w->plan;
     const ScanLevel& L = p.lev[0];
     char* blob = (char*)w->blob;
@@ -641,15 +661,23 @@ static void launch_deer_iteration(pmcmc_deer_ws* w, TileArgs<T> a, hipStream_t s
     a.skip = &w->sync->converged;
     const size_t lds = tile_lds_bytes<T>(p.D, L.TC, true);
     if (L.nT <= 1) {
+        const int e = prof->begin(KT_TILE_APPLY, a.iter, s);
         launch_tile<T, MODE, PASS_SINGLE>(a, 1, lds, s);
+        prof->end(e, s);
         return;
     }
     a.agg_a = (T*)(blob + L.off_a);
     a.agg_b = (T*)(blob + L.off_b);
+    int e = prof->begin(KT_TILE_REDUCE, a.iter, s);
     launch_tile<T, MODE, PASS_REDUCE>(a, L.nT, lds, s);
+    prof->end(e, s);
+    e = prof->begin(KT_SCAN_SPINE, a.iter, s);
     run_scan_levels<T>(p, 1, a.agg_a, a.agg_b, (T*)(blob + L.off_s), a.s0, blob, a.skip, s);
+    prof->end(e, s);
     a.prev = (const T*)(blob + L.off_s);
+    e = prof->begin(KT_TILE_APPLY, a.iter, s);
     launch_tile<T, MODE, PASS_APPLY>(a, L.nT, lds, s);
+    prof->end(e, s);
 }
 
 template <typename T>
@@ -658,6 +686,8 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
                         const void* init, void* S_out, uint8_t* acc_out, pmcmc_deer_info* info) {
     const int64_t D = w->D, Tn = w->T;
     const int mode = tgt->kind == TGT_DIAG ? MODE_ELEM : MODE_GEMM;
+    if (p->precision == PMCMC_PREC_BF16_JAC)
+        return fail(c, PMCMC_ERR_UNSUPPORTED, "precision bf16_jac is not available yet");
     if (mode == MODE_GEMM) {
         if (p->jacobian != PMCMC_JAC_STOCH_DIAG)
             return fail(c, PMCMC_ERR_UNSUPPORTED, "logistic target supports jacobian=:stoch_diag only");
@@ -701,11 +731,12 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
             a.iter = it;
             a.S_in = bufs[it & 1];
             a.S = bufs[(it + 1) & 1];
+            const int eit = c->prof.begin(KT_ITERATION, it, c->stream);
             if (mode == MODE_GEMM) {
                 std::string msg;
                 int rc = glm_newton_products(&w->glmw, &tgt->glm, c->stream, w->sync, p->precision,
                                              p->eps, tgt->prior_scale, (const T*)x0, a.S_in,
-                                             (const T*)Xi, p->seed, it, p->probes, D, Tn, &msg);
+                                             (const T*)Xi, p->seed, it, p->probes, D, Tn, &c->prof, &msg);
                 if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
                 a.Y = (const T*)w->glmw.Y;
                 a.gx = (const T*)w->glmw.gx;
@@ -714,10 +745,11 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
                 a.hw = (const T*)w->glmw.hw;
                 a.lpx = w->glmw.lpx;
                 a.lpy = w->glmw.lpy;
-                launch_deer_iteration<T, MODE_GEMM>(w, a, c->stream);
+                launch_deer_iteration<T, MODE_GEMM>(w, a, c->stream, &c->prof);
             } else {
-                launch_deer_iteration<T, MODE_ELEM>(w, a, c->stream);
+                launch_deer_iteration<T, MODE_ELEM>(w, a, c->stream, &c->prof);
             }
+            c->prof.end(eit, c->stream);
             HIPCHK(c, hipGetLastError());
         }
         k += n;
@@ -732,6 +764,7 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
     if ((void*)result != S_out)
         HIPCHK(c, hipMemcpyAsync(S_out, result, (size_t)D * Tn * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->prof.flush(iters - 1);
     if (info) {
         info->converged = hs.converged;
         info->iters = iters;

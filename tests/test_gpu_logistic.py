@@ -1,0 +1,104 @@
+"""GPU parity of the logistic-regression (GEMM) target path against the
+oracle: the batched MALA step + JVP through the fused GEMM passes, trajectory
+log-densities, and the full DEER solve (FP64 plain path at the 1e-10 bar,
+FP32 MFMA path at the 1e-5 relative bar) — the logistic model of
+benchmarks/ParallelMCMCBenchmarks/src/models/bayes_logreg.jl and
+test/test-DEER-Turing-Logistic.jl of the reference."""
+import numpy as np
+import pytest
+
+from oracle import deer_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(seed, N, D, prior_scale=1.0):
+    from ParallelMCMC import DensityModel, LogisticTarget
+
+    rng = np.random.default_rng(seed)
+    Xd, y, _ = O.make_logreg_data(rng, N, D)
+    tgt = O.LogReg(Xd, y, prior_scale)
+    model = DensityModel(dim=D, target=LogisticTarget(Xd, y, prior_scale))
+    return rng, tgt, model
+
+
+@pytest.mark.parametrize("N,D,T,prec", [(50, 3, 9, "f64"), (333, 7, 40, "f64"), (1000, 100, 96, "f32"),
+                                        (77, 17, 33, "f32"), (2000, 200, 40, "f32")])
+def test_logistic_fwd_and_jvp(N, D, T, prec):
+    from ParallelMCMC import MALA
+
+    rng, tgt, model = _setup(N + D + T, N, D, prior_scale=0.8)
+    X = 0.3 * rng.standard_normal((D, T))
+    Xi = rng.standard_normal((D, T))
+    Z = np.where(rng.random((D, T)) < 0.5, -1.0, 1.0)
+    U = rng.uniform(0.01, 0.99, T)
+    eps = 0.01
+    dt = np.float64 if prec == "f64" else np.float32
+    X, Xi, Z, U = (a.astype(dt) for a in (X, Xi, Z, U))
+    ref = O.mala_step_batched_fwd_and_jvp(tgt, X.astype(np.float64), eps, Xi.astype(np.float64),
+                                          U.astype(np.float64), Z.astype(np.float64))
+    FT, Jt, acc, la = MALA.mala_step_batched_fwd_and_jvp(model, X, eps, Xi, U, Z, precision=prec, return_logalpha=True)
+    la = la.cpu().numpy()
+    if prec == "f64":
+        assert np.allclose(la, ref.logalpha, rtol=1e-10, atol=1e-10)
+        assert np.array_equal(acc.cpu().numpy().astype(bool), ref.acc)
+        assert np.allclose(FT.cpu().numpy(), ref.FT, rtol=1e-12, atol=1e-12)
+        assert np.allclose(Jt.cpu().numpy(), ref.Jt, rtol=1e-9, atol=1e-9)
+    else:
+        # FP32 GEMMs: logalpha to ~1e-4 absolute; decisions identical away from ties
+        assert np.max(np.abs(la - ref.logalpha)) < 2e-3
+        clear = np.abs(ref.logalpha - np.log(U.astype(np.float64))) > 1e-2
+        assert np.array_equal(acc.cpu().numpy().astype(bool)[clear], ref.acc[clear])
+        assert np.max(np.abs(FT.cpu().numpy() - ref.FT)) / np.max(np.abs(ref.FT)) < 1e-5
+        assert np.max(np.abs(Jt.cpu().numpy() - ref.Jt)) / np.max(np.abs(ref.Jt)) < 1e-3
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_logistic_logdensity_batch(prec):
+    from ParallelMCMC import MALA
+
+    rng, tgt, model = _setup(4, 500, 12, 1.3)
+    S = 0.5 * rng.standard_normal((12, 70))
+    dt = np.float64 if prec == "f64" else np.float32
+    lp = MALA.logdensity_batch(model, S.astype(dt), precision=prec).cpu().numpy()
+    ref = tgt.logp_batch(S.astype(dt).astype(np.float64))
+    assert np.allclose(lp, ref, rtol=1e-12, atol=1e-9)
+
+
+def test_logistic_deer_solve_f64_recovers_sequential():
+    from ParallelMCMC import DEER
+
+    rng, tgt, model = _setup(22, 200, 6)
+    D, T, eps = 6, 80, 0.02
+    x0 = 0.3 * rng.standard_normal(D)
+    Xi, U = O.make_tape(rng, D, T)
+    ref, acc_ref, _ = O.run_mala_sequential_taped(tgt, x0, eps, Xi, U)
+    _, info_o = O.deer_solve(tgt, eps, Xi, U, x0, tol_abs=1e-12, tol_rel=1e-11, maxiter=300,
+                             damping=0.5, seed=3)
+    rec = DEER.TapedRecursion(model, eps, Xi, U)
+    S, info = DEER.solve(rec, x0, tol_abs=1e-12, tol_rel=1e-11, maxiter=300, damping=0.5, seed=3,
+                         return_info=True)
+    assert info.converged
+    assert info.iters <= info_o.iters + 1
+    assert np.max(np.abs(S.cpu().numpy() - ref)) < 1e-10
+    assert np.array_equal(info.accepted.cpu().numpy().astype(bool), acc_ref)
+
+
+def test_logistic_deer_solve_f32_mini_c1():
+    """Logistic workload at reduced size (N=2000, D=100, T=512, eps=5e-3),
+    FP32 MFMA path vs the FP64 sequential oracle on the same float32 tape."""
+    from ParallelMCMC import DEER
+
+    rng, tgt, model = _setup(7, 2000, 100)
+    D, T, eps = 100, 512, 5e-3
+    x0 = (0.1 * rng.standard_normal(D)).astype(np.float32)
+    Xi, U = O.make_tape(rng, D, T, np.float32)
+    ref, acc_ref, la_ref = O.run_mala_sequential_taped(tgt, x0.astype(np.float64), eps, Xi.astype(np.float64),
+                                                       U.astype(np.float64))
+    rec = DEER.TapedRecursion(model, eps, Xi, U, precision="f32")
+    S, info = DEER.solve(rec, x0, tol_abs=1e-6, tol_rel=1e-6, maxiter=500, damping=1.0, seed=11,
+                         return_info=True)
+    assert info.converged, info
+    err = O.trace_error(S.cpu().numpy().astype(np.float64), ref)
+    assert err < 1e-5, err
+    assert np.array_equal(info.accepted.cpu().numpy().astype(bool), acc_ref)
