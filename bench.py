@@ -80,6 +80,46 @@ def cpu_baseline(X, y, x0, Xi0, U0, S_gpu0, acc_gpu0, iters0, budget_s):
     }, trace_err, mism
 
 
+def bench_scan(args):
+    """Standalone diagonal affine scan (DEERScan.solve_affine_scan_diag!): GB/s of
+    the algorithmic 3 D T element bytes, timed with HIP events on the library
+    stream around the whole scan (all levels)."""
+    import torch
+
+    from ParallelMCMC import DEERScan, _device
+
+    dev = torch.device("cuda", 0)
+    dt = torch.float32 if args.scan_dtype == "f32" else torch.float64
+    D_, T_ = args.scan_D, args.scan_T
+    A = _device.colmajor_empty(D_, T_, dt, dev).uniform_(0.99, 1.0)
+    B = _device.colmajor_empty(D_, T_, dt, dev).normal_()
+    s0 = torch.zeros(D_, dtype=dt, device=dev)
+    S = _device.colmajor_empty(D_, T_, dt, dev)
+    ctx = _device.context(0)
+    for _ in range(args.warmup):
+        DEERScan.solve_affine_scan_diag_(S, A, B, s0)
+    ctx.reset_kernel_times()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        DEERScan.solve_affine_scan_diag_(S, A, B, s0)
+    el = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    kt = ctx.kernel_times()
+    n, ms = kt["iteration"]
+    bytes_ = 3 * D_ * T_ * A.element_size()
+    kern_s = ms / n / 1e3
+    print(json.dumps({
+        "metric": "diagonal affine scan throughput", "value": bytes_ / kern_s / 1e9, "unit": "GB/s",
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
+        "dtype": args.scan_dtype, "config": {"workload": f"scan D={D_} T={T_}",
+                                              "kernel_us": kern_s * 1e6,
+                                              "host_us_per_call": 1e6 * el / args.steps},
+        "roofline": {"bound": "hbm", "achieved": bytes_ / kern_s / 1e9, "peak": 8000.0, "unit": "GB/s",
+                     "frac": bytes_ / kern_s / 8e12, "traffic": None},
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,11 +129,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--damping", type=float, default=0.5)
+    ap.add_argument("--workload", default="deer", choices=["deer", "scan"])
+    ap.add_argument("--scan-D", type=int, default=64)
+    ap.add_argument("--scan-T", type=int, default=1 << 20)
+    ap.add_argument("--scan-dtype", default="f32", choices=["f32", "f64"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
               "the multi-GPU sharding is a design note, not a measured configuration", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "scan":
+        bench_scan(args)
+        return
 
     import numpy as np
     import torch

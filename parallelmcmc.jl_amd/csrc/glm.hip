@@ -201,7 +201,10 @@ struct ColSrc {
 // FP32 MFMA fused pass
 // ---------------------------------------------------------------------------
 // grid: (ceil(T/CB), NS); block 256 = 4 waves.
-// LDS: Vs[NC][Dp+2] column block, Xs[NB][Dp+2] design chunk, Rs[NB][NC+16].
+// LDS: Vs[NC][Dp+2] column block, Xs[2][NB][Dp+2] design chunks (double
+// buffered), Rs[NB][NC+16].  The next design chunk is fetched into registers
+// (float4) while the current one is multiplied, then stored into the other
+// LDS buffer: two barriers per chunk, global latency off the critical path.
 template <int DPT, int K>
 __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, const float* Xrm,
                                                             const float* y32, int64_t N,
@@ -215,8 +218,10 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
     constexpr int SR = NC + 16;
     constexpr int NCT = NC / 16;             // column tiles
     constexpr int CT_PER_WAVE = (NCT + 3) / 4;
+    constexpr int F4 = NB * Dp / 4;          // float4 per design chunk
+    constexpr int PF = (F4 + 255) / 256;     // float4 per thread per chunk
     __shared__ float Vs[NC * SV];
-    __shared__ float Xs[NB * SX];
+    __shared__ float Xs[2][NB * SX];
     __shared__ float Rs[NB * SR];
     __shared__ double llred[4][16];
 
@@ -229,7 +234,36 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
     const int64_t D = src.D, T = src.Tn;
     const int64_t cb = blockIdx.x;
     const int split = blockIdx.y;
+    const int64_t n_begin = (int64_t)split * rows_per_split;
+    int64_t n_end = n_begin + rows_per_split;
+    if (n_end > N) n_end = N;
+    const int nchunk = n_end > n_begin ? (int)((n_end - n_begin + NB - 1) / NB) : 0;
 
+    // design chunk c -> registers (rows beyond N are zero in Xrm, padded to 32)
+    float4 pf[PF];
+    auto fetch = [&](int c) {
+        const float4* src4 = reinterpret_cast<const float4*>(Xrm + (size_t)(n_begin + (int64_t)c * NB) * Dp);
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int idx = tid + 256 * i;
+            if (idx < F4) pf[i] = src4[idx];
+        }
+    };
+    auto stash = [&](int b) {
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int idx = tid + 256 * i;
+            if (idx < F4) {
+                const int e = 4 * idx;
+                const int r = e / Dp, d = e % Dp;
+                float2* dst = reinterpret_cast<float2*>(&Xs[b][r * SX + d]);
+                dst[0] = make_float2(pf[i].x, pf[i].y);
+                dst[1] = make_float2(pf[i].z, pf[i].w);
+            }
+        }
+    };
+
+    if (nchunk > 0) fetch(0);
     // 1. column block into LDS (zero padded)
     for (int i = tid; i < NC * Dp; i += 256) {
         const int j = i / Dp, d = i % Dp;
@@ -239,6 +273,8 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
         if (t < T && d < D) v = (float)src.get(set, t, d);
         Vs[j * SV + d] = v;
     }
+    if (nchunk > 0) stash(0);
+    __syncthreads();
 
     v4f acc[CT_PER_WAVE][DPT];
 #pragma unroll
@@ -249,25 +285,19 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
 
     const int rt = wave >> 1;  // GEMM-A row tile
     const int cx = wave & 1;   // GEMM-A x column tile
-    const int64_t n_begin = (int64_t)split * rows_per_split;
-    int64_t n_end = n_begin + rows_per_split;
-    if (n_end > N) n_end = N;
 
-    for (int64_t n0 = n_begin; n0 < n_end; n0 += NB) {
-        __syncthreads();  // previous chunk fully consumed (and Vs written)
-        // 2. design chunk into LDS; rows >= N are zero in Xrm (padded to 32)
-        for (int i = tid; i < NB * Dp; i += 256) {
-            const int r = i / Dp, d = i % Dp;
-            Xs[r * SX + d] = Xrm[(size_t)(n0 + r) * Dp + d];
-        }
-        __syncthreads();
-        // 3. GEMM-A: logits of the x tile and the matching probe tiles
+    for (int c = 0; c < nchunk; ++c) {
+        const int b = c & 1;
+        const int64_t n0 = n_begin + (int64_t)c * NB;
+        if (c + 1 < nchunk) fetch(c + 1);
+        const float* X_ = Xs[b];
+        // GEMM-A: logits of the x tile and the matching probe tiles
         v4f cl[1 + K];
 #pragma unroll
         for (int s = 0; s <= K; ++s) cl[s] = (v4f){0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
+#pragma unroll 7
         for (int ks = 0; ks < Dp / 4; ++ks) {
-            const float av = Xs[(16 * rt + li) * SX + 4 * ks + lk];
+            const float av = X_[(16 * rt + li) * SX + 4 * ks + lk];
 #pragma unroll
             for (int s = 0; s <= K; ++s) {
                 const int col = s * CB + 16 * cx + li;
@@ -275,45 +305,51 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
                 cl[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, cl[s], 0, 0, 0);
             }
         }
-        // 4. elementwise: residuals and sigmoid' weighted directions into LDS
+        // elementwise: residuals and sigmoid'-weighted directions into LDS.
+        // With e = exp(-|l|): softplus(l) = max(l,0) + log(1+e),
+        // sigmoid(l) = (l >= 0 ? 1 : e) / (1+e), log-lik = y l - softplus(l).
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * rt + 4 * lk + r;
             const int64_t n = n0 + row;
             const bool valid = n < N;
             const float l = cl[0][r];
-            const float sg = sigmoidf_(l);
+            const float e = __expf(-fabsf(l));
+            const float inv = 1.f / (1.f + e);
+            const float sg = l >= 0.f ? inv : e * inv;
             const float yv = y32[n];
-            if (valid) ll += (double)(-yv * softplusf(-l) - (1.f - yv) * softplusf(l));
+            if (valid) ll += (double)(yv * l - (fmaxf(l, 0.f) + __logf(1.f + e)));
             Rs[row * SR + 16 * cx + li] = valid ? (yv - sg) : 0.f;
             const float wgt = sg * (1.f - sg);
 #pragma unroll
             for (int s = 1; s <= K; ++s) Rs[row * SR + s * CB + 16 * cx + li] = valid ? wgt * cl[s][r] : 0.f;
         }
         __syncthreads();
-        // 5. GEMM-B: accumulate Xd^T R
+        // GEMM-B: accumulate Xd^T R
 #pragma unroll
         for (int ks = 0; ks < NB / 4; ++ks) {
             float xa[DPT];
 #pragma unroll
-            for (int dt = 0; dt < DPT; ++dt) xa[dt] = Xs[(4 * ks + lk) * SX + 16 * dt + li];
+            for (int dt = 0; dt < DPT; ++dt) xa[dt] = X_[(4 * ks + lk) * SX + 16 * dt + li];
 #pragma unroll
-            for (int c = 0; c < CT_PER_WAVE; ++c) {
-                const int ct = wave + 4 * c;
+            for (int cc = 0; cc < CT_PER_WAVE; ++cc) {
+                const int ct = wave + 4 * cc;
                 if (ct < NCT) {
                     const float bv = Rs[(4 * ks + lk) * SR + 16 * ct + li];
 #pragma unroll
                     for (int dt = 0; dt < DPT; ++dt)
-                        acc[c][dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[dt], bv, acc[c][dt], 0, 0, 0);
+                        acc[cc][dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[dt], bv, acc[cc][dt], 0, 0, 0);
                 }
             }
         }
+        if (c + 1 < nchunk) stash(b ^ 1);
+        __syncthreads();
     }
 
-    // 6. partial sums out: part[split][set][t][Dp]
+    // partial sums out: part[split][set][t][Dp]
 #pragma unroll
-    for (int c = 0; c < CT_PER_WAVE; ++c) {
-        const int ct = wave + 4 * c;
+    for (int cc = 0; cc < CT_PER_WAVE; ++cc) {
+        const int ct = wave + 4 * cc;
         if (ct >= NCT) continue;
         const int j = 16 * ct + li;
         const int set = j / CB;
@@ -323,7 +359,7 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
 #pragma unroll
         for (int dt = 0; dt < DPT; ++dt)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) dst[16 * dt + 4 * lk + r] = acc[c][dt][r];
+            for (int r = 0; r < 4; ++r) dst[16 * dt + 4 * lk + r] = acc[cc][dt][r];
     }
     // log-likelihood of column 16*cx + li: lanes li, li+16, li+32, li+48 and waves cx, cx+2
     ll += __shfl_xor(ll, 16, 64);

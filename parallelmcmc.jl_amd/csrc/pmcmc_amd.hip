@@ -428,9 +428,12 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
         HIPCHK(c, hipMalloc(&c->scan_blob, plan.bytes));
         c->scan_blob_cap = plan.bytes;
     }
+    const int e = c->prof.begin(KT_ITERATION, 0, c->stream);
     run_scan_levels<T>(plan, 0, A, B, S, s0, (char*)c->scan_blob, nullptr, c->stream);
+    c->prof.end(e, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->prof.flush(0);
     return PMCMC_OK;
 }
 

@@ -30,6 +30,8 @@
 //              target kernels.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace pmcmc {
@@ -108,6 +110,40 @@ __device__ inline double group_sum(double v, int G) {
     return v;
 }
 
+// Copy n elements from global memory into LDS: 16-byte vector loads, eight
+// per thread in flight before the LDS writes (the staging loop is otherwise
+// bound by one global-load latency per element).
+template <typename T>
+__device__ inline void stage_lds(T* dst, const T* src, int64_t n, int tid) {
+    typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type V;
+    constexpr int VEC = 16 / sizeof(T);
+    const uintptr_t addr = (uintptr_t)src;
+    int64_t head = (int64_t)(((16 - (addr & 15)) & 15) / sizeof(T));
+    if (head > n) head = n;
+    for (int64_t i = tid; i < head; i += PMCMC_NT) dst[i] = src[i];
+    const int64_t nv = (n - head) / VEC;
+    const V* sv = reinterpret_cast<const V*>(src + head);
+    T* dh = dst + head;
+    for (int64_t base = 0; base < nv; base += 8 * PMCMC_NT) {
+        V r[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int64_t idx = base + tid + (int64_t)k * PMCMC_NT;
+            if (idx < nv) r[k] = sv[idx];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int64_t idx = base + tid + (int64_t)k * PMCMC_NT;
+            if (idx < nv) {
+                const T* rv = reinterpret_cast<const T*>(&r[k]);
+#pragma unroll
+                for (int j = 0; j < VEC; ++j) dh[idx * VEC + j] = rv[j];
+            }
+        }
+    }
+    for (int64_t i = head + nv * VEC + tid; i < n; i += PMCMC_NT) dst[i] = src[i];
+}
+
 template <typename T, int MODE, int PASS>
 __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -139,16 +175,16 @@ __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
 
     // ---- phase 0: stage inputs (coalesced) -------------------------------
     if (!DEERM) {
-        for (int64_t i = tid; i < nel; i += PMCMC_NT) {
-            al[i] = a.A[gbase + i];
-            bl[i] = a.B[gbase + i];
-        }
+        stage_lds<T>(al, a.A + gbase, nel, tid);
+        stage_lds<T>(bl, a.B + gbase, nel, tid);
     } else {
-        const int64_t nx = nel + D;
-        for (int64_t i = tid; i < nx; i += PMCMC_NT) {
-            xs[i] = (c0 == 0 && i < D) ? a.s0[i] : a.S_in[gbase - D + i];
+        if (c0 == 0) {
+            for (int64_t i = tid; i < D; i += PMCMC_NT) xs[i] = a.s0[i];
+            stage_lds<T>(xs + D, a.S_in, nel, tid);
+        } else {
+            stage_lds<T>(xs, a.S_in + gbase - D, nel + D, tid);
         }
-        for (int64_t i = tid; i < nel; i += PMCMC_NT) bl[i] = a.Xi[gbase + i];
+        stage_lds<T>(bl, a.Xi + gbase, nel, tid);
     }
     // incoming state of the tile (needed only by the output passes)
     if (OUT) {
