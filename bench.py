@@ -4,15 +4,21 @@
 Workload (the logistic-regression configuration): Bayesian logistic
 regression, X is 10000 x 100 with entries N(0,1)/sqrt(100), theta* ~ N(0,I),
 y ~ Bernoulli(sigmoid(X theta*)), prior N(0,I); eps = 5e-3, T = 4096,
-jacobian = :stoch_diag (1 probe), the ParallelMALASampler defaults
-tol_abs = 1e-6, tol_rel = 1e-5, damping = 0.5, maxiter = 200.
+jacobian = :stoch_diag (1 probe), the ParallelMALASampler tolerances
+tol_abs = 1e-6, tol_rel = 1e-5, maxiter = 200, and DEER.solve's default
+damping 1.0 (--damping 0.5 gives the sampler's default).
 
 A step is one DEER solve of one T-block (the Newton loop to convergence),
 blocks chained like the sampler's block loop (x0 of block k+1 is the last
 state of block k), every block with its own synthetic tape.  Tapes, data and
-workspaces are resident in HBM before the timed region.
+workspaces are resident in HBM before the timed region.  Both variants run:
+plain FP32 DEER and BF16-Jacobian DEER (probe GEMMs on BF16 MFMA, residual
+path FP32); `value` is the BF16-Jacobian samples/s, the FP32 figure and the
+ratio are in `config`.  The CPU leg checks block 0 of each variant against
+the FP64 sequential taped MALA run (trace error, accept decisions).
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--precision f32]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--precision both|f32|bf16_jac]
+    python bench.py --workload scan --scan-D 64 --scan-T 1048576
 """
 import argparse
 import json
@@ -25,7 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "parallelmcmc.jl_amd"))
 
 METRIC = "MALA samples/s: DEER BF16-Jac vs plain FP32 DEER at ≤1e-5 trace err, 1 GPU"
-PEAK_TFLOPS = {"f32": 157.3, "bf16_jac": 157.3}  # the residual (FP32 MFMA) half bounds the pass
+PEAK_TFLOPS = {"f32": 157.3}  # MI355X dense FP32 MFMA
+PEAK_BF16_TFLOPS = 2500.0     # MI355X dense BF16 MFMA
 N_DATA, D, T, EPS = 10000, 100, 4096, 5e-3
 
 
@@ -68,7 +75,7 @@ def cpu_baseline(X, y, x0, Xi0, U0, S_gpu0, acc_gpu0, iters0, budget_s):
     deer_sps = T / (s_per_iter * iters0)
     trace_err = O.trace_error(S_gpu0, S_ref)
     mism = int(np.sum(acc_gpu0 != acc_ref))
-    return {
+    return ({
         "value": deer_sps,
         "unit": "samples/s",
         "cores": threads,
@@ -77,7 +84,7 @@ def cpu_baseline(X, y, x0, Xi0, U0, S_gpu0, acc_gpu0, iters0, budget_s):
                    f"({s_per_iter:.2f} s/iter, OpenBLAS {threads} threads) x {iters0} iterations (the GPU solve's count); "
                    f"sequential taped MALA over the full block: {T / t_seq:.1f} samples/s"),
         "mala_sequential_samples_per_s": T / t_seq,
-    }, trace_err, mism
+    }, trace_err, mism, S_ref, acc_ref)
 
 
 def bench_scan(args):
@@ -125,10 +132,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="f32", choices=["f32", "bf16_jac"])
+    ap.add_argument("--precision", default="both", choices=["both", "f32", "bf16_jac"])
+    ap.add_argument("--tol-abs", type=float, default=1e-6)
+    ap.add_argument("--tol-rel", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--damping", type=float, default=0.5)
+    ap.add_argument("--damping", type=float, default=1.0)
     ap.add_argument("--workload", default="deer", choices=["deer", "scan"])
     ap.add_argument("--scan-D", type=int, default=64)
     ap.add_argument("--scan-T", type=int, default=1 << 20)
@@ -153,100 +162,115 @@ def main():
     X, y, rng = make_data(1234)
     model = DensityModel(dim=D, target=LogisticTarget(X, y))
     nblk = args.warmup + args.steps
-    prec = args.precision
     tapes = []
     for _ in range(nblk):
         Xi = rng.standard_normal((T, D)).T.astype(np.float32)
         U = rng.random(T).astype(np.float32)
         tapes.append((_device.to_colmajor(Xi, torch.float32, dev), _device.to_vec(U, torch.float32, dev), Xi, U))
     x0 = (0.1 * rng.standard_normal(D)).astype(np.float32)
-    precode = _lib.PREC_F32 if prec == "f32" else _lib.PREC_BF16_JAC
-    ws = DEER.DEERWorkspace(D, T, precode)
-    ctx = ws.ctx
-    kw = dict(tol_abs=1e-6, tol_rel=1e-5, maxiter=200, damping=args.damping, probes=1, jacobian="stoch_diag",
-              return_info=True, workspace=ws, copy_result=False)
+    variants = ["f32", "bf16_jac"] if args.precision == "both" else [args.precision]
 
-    def run_block(b, xs):
-        Xi_d, U_d, _, _ = tapes[b]
-        rec = DEER.TapedRecursion(model, EPS, Xi_d, U_d, precision=precode)
-        S, info = DEER.solve(rec, xs, seed=1000 + b, **kw)
-        return S, info
+    def run_variant(prec):
+        precode = _lib.PREC_F32 if prec == "f32" else _lib.PREC_BF16_JAC
+        ws = DEER.DEERWorkspace(D, T, precode)
+        ctx = ws.ctx
+        kw = dict(tol_abs=args.tol_abs, tol_rel=args.tol_rel, maxiter=200, damping=args.damping, probes=1,
+                  jacobian="stoch_diag", return_info=True, workspace=ws, copy_result=False)
 
-    xs = torch.from_numpy(x0).to(dev)
-    x_first = x0.copy()
-    S0 = acc0 = None
-    iters0 = None
-    for b in range(args.warmup):
-        S, info = run_block(b, xs)
-        if b == 0:
-            S0 = S.cpu().numpy().astype(np.float64)
-            acc0 = info.accepted.cpu().numpy().astype(bool)
-            iters0 = info.iters
-        xs = S[:, T - 1].contiguous().clone()
+        def run_block(b, xs):
+            Xi_d, U_d, _, _ = tapes[b]
+            rec = DEER.TapedRecursion(model, EPS, Xi_d, U_d, precision=precode)
+            return DEER.solve(rec, xs, seed=1000 + b, **kw)
 
-    ctx.reset_kernel_times()
-    ctx.set_profiling(True)
-    torch.cuda.synchronize()
-    iters = []
-    t0 = time.perf_counter()
-    for b in range(args.warmup, nblk):
-        S, info = run_block(b, xs)
-        iters.append(info.iters)
-        if b == 0:  # no warmup: keep block 0 for the accuracy check (device copy)
-            S0_dev, acc0_dev, iters0 = S.t().clone().t(), info.accepted.clone(), info.iters
-        if not info.converged:
-            print(f"warning: block {b} did not converge in {kw['maxiter']} iterations", file=sys.stderr)
-        xs = S[:, T - 1].contiguous().clone()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ctx.set_profiling(False)
-    kt = ctx.kernel_times()
-    if S0 is None:
-        S0 = S0_dev.cpu().numpy().astype(np.float64)
-        acc0 = acc0_dev.cpu().numpy().astype(bool)
+        xs = torch.from_numpy(x0).to(dev)
+        first = None
+        for b in range(args.warmup):
+            S, info = run_block(b, xs)
+            if b == 0:
+                first = (S.cpu().numpy().astype(np.float64), info.accepted.cpu().numpy().astype(bool), info.iters)
+            xs = S[:, T - 1].contiguous().clone()
+        ctx.reset_kernel_times()
+        ctx.set_profiling(True)
+        torch.cuda.synchronize()
+        iters = []
+        keep = None
+        t0 = time.perf_counter()
+        for b in range(args.warmup, nblk):
+            S, info = run_block(b, xs)
+            iters.append(info.iters)
+            if b == 0:  # no warmup: keep block 0 for the accuracy check (device copy)
+                keep = (S.t().clone().t(), info.accepted.clone(), info.iters)
+            if not info.converged:
+                print(f"warning: {prec} block {b} did not converge in {kw['maxiter']} iterations", file=sys.stderr)
+            xs = S[:, T - 1].contiguous().clone()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        ctx.set_profiling(False)
+        kt = ctx.kernel_times()
+        if first is None:
+            first = (keep[0].cpu().numpy().astype(np.float64), keep[1].cpu().numpy().astype(bool), keep[2])
+        return {"prec": prec, "elapsed": elapsed, "sps": args.steps * T / elapsed, "iters": iters, "kt": kt,
+                "first": first}
 
-    value = args.steps * T / elapsed
+    res = {v: run_variant(v) for v in variants}
+    head = res["bf16_jac"] if "bf16_jac" in res else res[variants[0]]
+    kt = head["kt"]
     n_pass, ms_pass = kt["glm_pass"]
     avg_pass_s = (ms_pass / max(n_pass, 1)) / 1e3
-    flops_per_pass = 2 * 2 * N_DATA * D * (2 * T)  # two GEMMs over the x and probe columns
-    achieved = flops_per_pass / avg_pass_s / 1e12 if n_pass else 0.0
+    f_res = 2 * 2 * N_DATA * D * T   # residual columns: two GEMMs, FP32 MFMA
+    f_jac = 2 * 2 * N_DATA * D * T   # probe columns: FP32 or BF16 MFMA
+    if head["prec"] == "bf16_jac":
+        t_roof = f_res / (PEAK_TFLOPS["f32"] * 1e12) + f_jac / (PEAK_BF16_TFLOPS * 1e12)
+    else:
+        t_roof = (f_res + f_jac) / (PEAK_TFLOPS["f32"] * 1e12)
+    achieved = (f_res + f_jac) / avg_pass_s / 1e12 if n_pass else 0.0
+    peak_eff = (f_res + f_jac) / t_roof / 1e12
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["sps"],
         "unit": "samples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": 1e3 * head["elapsed"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if prec == "f32" else "bf16+f32",
+        "dtype": "bf16+f32" if head["prec"] == "bf16_jac" else "f32",
         "data": "synthetic (seeded logistic design, labels and tapes; nothing loaded)",
         "config": {
             "workload": f"logistic regression N={N_DATA} D={D}, DEER-MALA T={T} eps={EPS} :stoch_diag probes=1, "
-                        f"{prec}, tol 1e-6/1e-5, damping {args.damping}",
-            "precision": prec,
-            "newton_iters_per_block": iters,
-            "kernel_ms": {k: (v[1] / v[0] if v[0] else None) for k, v in kt.items()},
-            "kernel_launches": {k: v[0] for k, v in kt.items()},
+                        f"tol {args.tol_abs:g}/{args.tol_rel:g}, damping {args.damping}",
+            "precision": head["prec"],
+            "variants": {v: {"samples_per_s": r["sps"], "ms_per_block": 1e3 * r["elapsed"] / args.steps,
+                             "newton_iters_per_block": r["iters"],
+                             "kernel_ms": {k: (x[1] / x[0] if x[0] else None) for k, x in r["kt"].items()}}
+                         for v, r in res.items()},
         },
         "roofline": {
-            "kernel": "glm_pass_mfma_kernel (fused GEMM -> sigmoid -> GEMM^T)",
+            "kernel": "glm_pass_mfma_kernel (fused GEMM -> sigmoid -> GEMM^T), " + head["prec"],
             "bound": "mfma",
             "achieved": achieved,
-            "peak": PEAK_TFLOPS[prec],
+            "peak": peak_eff,
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_TFLOPS[prec],
+            "frac": achieved / peak_eff,
             "traffic": None,
+            "note": ("peak is the time-weighted roofline of the pass: residual half at 157.3 TFLOP/s FP32 MFMA, "
+                     "probe half at 2500 TFLOP/s BF16 MFMA" if head["prec"] == "bf16_jac"
+                     else "157.3 TFLOP/s FP32 MFMA"),
         },
     }
+    if "f32" in res and "bf16_jac" in res:
+        out["config"]["bf16_jac_over_f32"] = res["bf16_jac"]["sps"] / res["f32"]["sps"]
     if not args.no_cpu_baseline:
         _, _, Xi0, U0 = tapes[0]
-        cb, trace_err, mism = cpu_baseline(X, y, x_first, Xi0, U0, S0, acc0, iters0, args.cpu_budget)
+        S0, acc0, it0 = head["first"]
+        cb, trace_err, mism, S_ref, acc_ref = cpu_baseline(X, y, x0, Xi0, U0, S0, acc0, it0, args.cpu_budget)
         out["cpu_baseline"] = cb
-        out["config"]["trace_err_block0_vs_fp64_sequential"] = trace_err
-        out["config"]["accept_mismatches_block0"] = mism
+        for v, r in res.items():
+            out["config"]["variants"][v]["trace_err_block0_vs_fp64_sequential"] = float(
+                np.max(np.abs(r["first"][0] - S_ref)) / np.max(np.abs(S_ref)))
+            out["config"]["variants"][v]["accept_mismatches_block0"] = int(np.sum(r["first"][1] != acc_ref))
     print(json.dumps(out))
 
 

@@ -28,6 +28,7 @@
 namespace pmcmc {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 static int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
@@ -205,7 +206,7 @@ struct ColSrc {
 // buffered), Rs[NB][NC+16].  The next design chunk is fetched into registers
 // (float4) while the current one is multiplied, then stored into the other
 // LDS buffer: two barriers per chunk, global latency off the critical path.
-template <int DPT, int K>
+template <int DPT, int K, bool JBF>
 __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, const float* Xrm,
                                                             const float* y32, int64_t N,
                                                             int64_t rows_per_split,
@@ -276,11 +277,23 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
     if (nchunk > 0) stash(0);
     __syncthreads();
 
-    v4f acc[CT_PER_WAVE][DPT];
+    // FP32 mode: wave w owns column tile w for every d tile.
+    // BF16-Jacobian mode: wave w owns d tiles w, w+4, ... for every column
+    // tile (x tiles on FP32 MFMA, probe tiles on BF16 MFMA).
+    constexpr int DPW = (DPT + 3) / 4;
+    v4f acc[JBF ? 1 : CT_PER_WAVE][JBF ? 1 : DPT];
+    v4f accx[JBF ? DPW : 1][2], accp[JBF ? DPW : 1][2];
+    if constexpr (!JBF) {
 #pragma unroll
-    for (int c = 0; c < CT_PER_WAVE; ++c)
+        for (int c = 0; c < CT_PER_WAVE; ++c)
 #pragma unroll
-        for (int dt = 0; dt < DPT; ++dt) acc[c][dt] = (v4f){0.f, 0.f, 0.f, 0.f};
+            for (int dt = 0; dt < DPT; ++dt) acc[c][dt] = (v4f){0.f, 0.f, 0.f, 0.f};
+    } else {
+#pragma unroll
+        for (int m = 0; m < DPW; ++m)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) accx[m][c] = accp[m][c] = (v4f){0.f, 0.f, 0.f, 0.f};
+    }
     double ll = 0.0;
 
     const int rt = wave >> 1;  // GEMM-A row tile
@@ -295,14 +308,31 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
         v4f cl[1 + K];
 #pragma unroll
         for (int s = 0; s <= K; ++s) cl[s] = (v4f){0.f, 0.f, 0.f, 0.f};
+        constexpr int SA = JBF ? 0 : K;  // last set on FP32 MFMA
 #pragma unroll 7
         for (int ks = 0; ks < Dp / 4; ++ks) {
             const float av = X_[(16 * rt + li) * SX + 4 * ks + lk];
 #pragma unroll
-            for (int s = 0; s <= K; ++s) {
+            for (int s = 0; s <= SA; ++s) {
                 const int col = s * CB + 16 * cx + li;
                 const float bv = Vs[col * SV + 4 * ks + lk];
                 cl[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, cl[s], 0, 0, 0);
+            }
+        }
+        if constexpr (JBF) {
+            // probe logits q = Xd z on BF16 MFMA (K = 32 per step), operands
+            // rounded from the FP32 tiles; lane (li, lk) holds k = 8 lk .. 8 lk + 7
+#pragma unroll
+            for (int ks = 0; ks < (Dp + 31) / 32; ++ks) {
+                v8bf av, bv;
+                const int col = CB + 16 * cx + li;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int d = 32 * ks + 8 * lk + e;
+                    av[e] = (__bf16)(d < Dp ? X_[(16 * rt + li) * SX + d] : 0.f);
+                    bv[e] = (__bf16)(d < Dp ? Vs[col * SV + d] : 0.f);
+                }
+                cl[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, cl[1], 0, 0, 0);
             }
         }
         // elementwise: residuals and sigmoid'-weighted directions into LDS.
@@ -326,19 +356,49 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
         }
         __syncthreads();
         // GEMM-B: accumulate Xd^T R
+        if constexpr (!JBF) {
 #pragma unroll
-        for (int ks = 0; ks < NB / 4; ++ks) {
-            float xa[DPT];
+            for (int ks = 0; ks < NB / 4; ++ks) {
+                float xa[DPT];
 #pragma unroll
-            for (int dt = 0; dt < DPT; ++dt) xa[dt] = X_[(4 * ks + lk) * SX + 16 * dt + li];
+                for (int dt = 0; dt < DPT; ++dt) xa[dt] = X_[(4 * ks + lk) * SX + 16 * dt + li];
 #pragma unroll
-            for (int cc = 0; cc < CT_PER_WAVE; ++cc) {
-                const int ct = wave + 4 * cc;
-                if (ct < NCT) {
-                    const float bv = Rs[(4 * ks + lk) * SR + 16 * ct + li];
+                for (int cc = 0; cc < CT_PER_WAVE; ++cc) {
+                    const int ct = wave + 4 * cc;
+                    if (ct < NCT) {
+                        const float bv = Rs[(4 * ks + lk) * SR + 16 * ct + li];
 #pragma unroll
-                    for (int dt = 0; dt < DPT; ++dt)
-                        acc[cc][dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[dt], bv, acc[cc][dt], 0, 0, 0);
+                        for (int dt = 0; dt < DPT; ++dt)
+                            acc[cc][dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[dt], bv, acc[cc][dt], 0, 0, 0);
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < DPW; ++m) {
+                const int dt = wave + 4 * m;
+                if (dt < DPT) {
+                    // residual columns: FP32 MFMA over the 32 rows
+#pragma unroll
+                    for (int ks = 0; ks < NB / 4; ++ks) {
+                        const float xa = X_[(4 * ks + lk) * SX + 16 * dt + li];
+#pragma unroll
+                        for (int c = 0; c < 2; ++c) {
+                            const float bv = Rs[(4 * ks + lk) * SR + 16 * c + li];
+                            accx[m][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, bv, accx[m][c], 0, 0, 0);
+                        }
+                    }
+                    // probe columns: one BF16 MFMA over the 32 rows
+                    v8bf xa;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) xa[e] = (__bf16)X_[(8 * lk + e) * SX + 16 * dt + li];
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        v8bf bv;
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) bv[e] = (__bf16)Rs[(8 * lk + e) * SR + CB + 16 * c + li];
+                        accp[m][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa, bv, accp[m][c], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -347,19 +407,38 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
     }
 
     // partial sums out: part[split][set][t][Dp]
+    if constexpr (!JBF) {
 #pragma unroll
-    for (int cc = 0; cc < CT_PER_WAVE; ++cc) {
-        const int ct = wave + 4 * cc;
-        if (ct >= NCT) continue;
-        const int j = 16 * ct + li;
-        const int set = j / CB;
-        const int64_t t = cb * CB + (j % CB);
-        if (t >= T) continue;
-        float* dst = part + (((size_t)split * (1 + K) + set) * T + t) * Dp;
+        for (int cc = 0; cc < CT_PER_WAVE; ++cc) {
+            const int ct = wave + 4 * cc;
+            if (ct >= NCT) continue;
+            const int j = 16 * ct + li;
+            const int set = j / CB;
+            const int64_t t = cb * CB + (j % CB);
+            if (t >= T) continue;
+            float* dst = part + (((size_t)split * (1 + K) + set) * T + t) * Dp;
 #pragma unroll
-        for (int dt = 0; dt < DPT; ++dt)
+            for (int dt = 0; dt < DPT; ++dt)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) dst[16 * dt + 4 * lk + r] = acc[cc][dt][r];
+                for (int r = 0; r < 4; ++r) dst[16 * dt + 4 * lk + r] = acc[cc][dt][r];
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < DPW; ++m) {
+            const int dt = wave + 4 * m;
+            if (dt >= DPT) continue;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int j = 16 * c + li;
+                const int set = j / CB;
+                const int64_t t = cb * CB + (j % CB);
+                if (t >= T) continue;
+                float* dst = part + (((size_t)split * (1 + K) + set) * T + t) * Dp;
+                const v4f v = c < 2 ? accx[m][c & 1] : accp[m][c & 1];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) dst[16 * dt + 4 * lk + r] = v[r];
+            }
+        }
     }
     // log-likelihood of column 16*cx + li: lanes li, li+16, li+32, li+48 and waves cx, cx+2
     ll += __shfl_xor(ll, 16, 64);
@@ -481,16 +560,21 @@ __global__ void glm_naive_bwd_kernel(int K, int64_t D, int64_t Dp, int64_t Tn, c
 // ---------------------------------------------------------------------------
 template <int DPT>
 static void launch_mfma(const ColSrc<float>& src, const GlmData* g, int ns, int64_t rows_per_split,
-                        float* part, double* llpart, const int* skip, hipStream_t s) {
+                        float* part, double* llpart, const int* skip, bool jbf, hipStream_t s) {
     dim3 grid((unsigned)((src.Tn + CB - 1) / CB), (unsigned)ns);
-    hipLaunchKernelGGL((glm_pass_mfma_kernel<DPT, 1>), grid, dim3(256), 0, s, src, g->Xrm, g->y, g->N,
-                       rows_per_split, part, llpart, skip);
+    if (jbf)
+        hipLaunchKernelGGL((glm_pass_mfma_kernel<DPT, 1, true>), grid, dim3(256), 0, s, src, g->Xrm, g->y, g->N,
+                           rows_per_split, part, llpart, skip);
+    else
+        hipLaunchKernelGGL((glm_pass_mfma_kernel<DPT, 1, false>), grid, dim3(256), 0, s, src, g->Xrm, g->y, g->N,
+                           rows_per_split, part, llpart, skip);
 }
 
 template <typename T>
 static int glm_pass(const GlmData* g, GlmWork* w, hipStream_t s, const ColSrc<T>& src, int pass,
                     double eps, double prior_scale, const T* Xi, T* gout, T* hout, double* lpout,
-                    T* Yout, T* Wout, const int* skip, Prof* prof, int iter, std::string* msg) {
+                    T* Yout, T* Wout, const int* skip, Prof* prof, int iter, bool jbf,
+                    std::string* msg) {
     const int64_t Tn = src.Tn;
     const int K = src.K;
     const double inv_s2 = 1.0 / (prior_scale * prior_scale);
@@ -506,7 +590,7 @@ static int glm_pass(const GlmData* g, GlmWork* w, hipStream_t s, const ColSrc<T>
         const int pe = prof ? prof->begin(KT_GLM_PASS, iter, s) : -1;
         switch (g->Dp / 16) {
 #define PMCMC_DPT(n) \
-    case n: launch_mfma<n>(fs, g, ns, rows_per_split, part, llpart, skip, s); break;
+    case n: launch_mfma<n>(fs, g, ns, rows_per_split, part, llpart, skip, jbf, s); break;
             PMCMC_DPT(1) PMCMC_DPT(2) PMCMC_DPT(3) PMCMC_DPT(4) PMCMC_DPT(5) PMCMC_DPT(6)
             PMCMC_DPT(7) PMCMC_DPT(8) PMCMC_DPT(9) PMCMC_DPT(10) PMCMC_DPT(11) PMCMC_DPT(12)
             PMCMC_DPT(13) PMCMC_DPT(14) PMCMC_DPT(15) PMCMC_DPT(16)
@@ -559,7 +643,7 @@ int glm_newton_products(GlmWork* w, const GlmData* g, hipStream_t s, SyncState* 
                         double eps, double prior_scale, const T* x0, const T* S_in, const T* Xi,
                         uint64_t seed, int iter, int probes, int64_t D, int64_t Tn, Prof* prof,
                         std::string* msg) {
-    (void)precision;
+    const bool jbf = precision == PMCMC_PREC_BF16_JAC;
     const int* skip = sync ? &sync->converged : nullptr;
     ColSrc<T> a{};
     a.V = S_in;
@@ -570,7 +654,7 @@ int glm_newton_products(GlmWork* w, const GlmData* g, hipStream_t s, SyncState* 
     a.Tn = Tn;
     for (int k = 0; k < probes && k < 4; ++k) a.keys[k] = probe_key(seed, (uint32_t)iter, (uint32_t)k);
     int rc = glm_pass<T>(g, w, s, a, 0, eps, prior_scale, Xi, (T*)w->gx, (T*)w->hw, w->lpx, (T*)w->Y,
-                         (T*)w->W, skip, prof, iter, msg);
+                         (T*)w->W, skip, prof, iter, jbf, msg);
     if (rc != PMCMC_OK) return rc;
     ColSrc<T> b{};
     b.V = (const T*)w->Y;
@@ -580,7 +664,7 @@ int glm_newton_products(GlmWork* w, const GlmData* g, hipStream_t s, SyncState* 
     b.D = D;
     b.Tn = Tn;
     return glm_pass<T>(g, w, s, b, 1, eps, prior_scale, Xi, (T*)w->gy, (T*)w->hw, w->lpy, nullptr, nullptr,
-                       skip, prof, iter, msg);
+                       skip, prof, iter, jbf, msg);
 }
 template int glm_newton_products<float>(GlmWork*, const GlmData*, hipStream_t, SyncState*, int,
                                         double, double, const float*, const float*, const float*,
@@ -643,7 +727,7 @@ __global__ __launch_bounds__(256) void glm_finish_kernel(const T* X, const T* Z,
 template <typename T>
 static int fwd_jvp_t(const GlmData* g, hipStream_t s, double eps, double prior_scale, const T* X,
                      const T* Xi, const T* U, const T* Z, T* FT, T* Jt, uint8_t* acc, double* la,
-                     int64_t D, int64_t Tn, std::string* msg) {
+                     int64_t D, int64_t Tn, bool jbf, std::string* msg) {
     GlmWork w{};
     int rc = glm_work_ensure(&w, g, s, D, Tn, 1, sizeof(T) == 8, msg);
     if (rc != PMCMC_OK) return rc;
@@ -655,13 +739,13 @@ static int fwd_jvp_t(const GlmData* g, hipStream_t s, double eps, double prior_s
     a.D = D;
     a.Tn = Tn;
     rc = glm_pass<T>(g, &w, s, a, 0, eps, prior_scale, Xi, (T*)w.gx, (T*)w.hw, w.lpx, (T*)w.Y, (T*)w.W, nullptr,
-                     nullptr, 0, msg);
+                     nullptr, 0, jbf, msg);
     if (rc == PMCMC_OK) {
         ColSrc<T> b = a;
         b.V = (const T*)w.Y;
         b.Z = (const T*)w.W;
         rc = glm_pass<T>(g, &w, s, b, 1, eps, prior_scale, Xi, (T*)w.gy, (T*)w.hw, w.lpy, nullptr, nullptr, nullptr,
-                         nullptr, 0, msg);
+                         nullptr, 0, jbf, msg);
     }
     if (rc == PMCMC_OK) {
         hipLaunchKernelGGL((glm_finish_kernel<T>), dim3((unsigned)((Tn + 3) / 4)), dim3(256), 0, s, X, Z, U,
@@ -683,9 +767,10 @@ int glm_fwd_and_jvp(const GlmData* g, hipStream_t s, int prec, double eps, doubl
     if (prec == PMCMC_PREC_F64)
         return fwd_jvp_t<double>(g, s, eps, prior_scale, (const double*)X, (const double*)Xi,
                                  (const double*)U, (const double*)Z, (double*)FT, (double*)Jt, accepted,
-                                 logalpha, D, T, msg);
+                                 logalpha, D, T, false, msg);
     return fwd_jvp_t<float>(g, s, eps, prior_scale, (const float*)X, (const float*)Xi, (const float*)U,
-                            (const float*)Z, (float*)FT, (float*)Jt, accepted, logalpha, D, T, msg);
+                            (const float*)Z, (float*)FT, (float*)Jt, accepted, logalpha, D, T,
+                            prec == PMCMC_PREC_BF16_JAC, msg);
 }
 
 // logp of every column: one workgroup per column, threads over n, in double.

@@ -689,8 +689,6 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
                         const void* init, void* S_out, uint8_t* acc_out, pmcmc_deer_info* info) {
     const int64_t D = w->D, Tn = w->T;
     const int mode = tgt->kind == TGT_DIAG ? MODE_ELEM : MODE_GEMM;
-    if (p->precision == PMCMC_PREC_BF16_JAC)
-        return fail(c, PMCMC_ERR_UNSUPPORTED, "precision bf16_jac is not available yet");
     if (mode == MODE_GEMM) {
         if (p->jacobian != PMCMC_JAC_STOCH_DIAG)
             return fail(c, PMCMC_ERR_UNSUPPORTED, "logistic target supports jacobian=:stoch_diag only");

@@ -102,3 +102,45 @@ def test_logistic_deer_solve_f32_mini_c1():
     err = O.trace_error(S.cpu().numpy().astype(np.float64), ref)
     assert err < 1e-5, err
     assert np.array_equal(info.accepted.cpu().numpy().astype(bool), acc_ref)
+
+
+def test_logistic_bf16_jacobian_keeps_residual_path_fp32():
+    """BF16 Jacobian products change Jt only: FT, decisions and log-alpha are
+    bit-identical to the FP32 path, Jt agrees to BF16 accuracy."""
+    from ParallelMCMC import MALA
+
+    rng, tgt, model = _setup(31, 3000, 100)
+    D, T, eps = 100, 256, 5e-3
+    X = (0.2 * rng.standard_normal((D, T))).astype(np.float32)
+    Xi = rng.standard_normal((D, T)).astype(np.float32)
+    Z = np.where(rng.random((D, T)) < 0.5, -1.0, 1.0).astype(np.float32)
+    U = rng.uniform(0.01, 0.99, T).astype(np.float32)
+    F32 = MALA.mala_step_batched_fwd_and_jvp(model, X, eps, Xi, U, Z, precision="f32", return_logalpha=True)
+    BF = MALA.mala_step_batched_fwd_and_jvp(model, X, eps, Xi, U, Z, precision="bf16_jac", return_logalpha=True)
+    assert np.array_equal(F32[0].cpu().numpy(), BF[0].cpu().numpy())
+    assert np.array_equal(F32[2].cpu().numpy(), BF[2].cpu().numpy())
+    assert np.array_equal(F32[3].cpu().numpy(), BF[3].cpu().numpy())
+    j32, jbf = F32[1].cpu().numpy(), BF[1].cpu().numpy()
+    assert np.max(np.abs(jbf - j32)) / np.max(np.abs(j32)) < 2e-2
+    ref = O.mala_step_batched_fwd_and_jvp(tgt, X.astype(np.float64), eps, Xi.astype(np.float64),
+                                          U.astype(np.float64), Z.astype(np.float64))
+    assert np.max(np.abs(jbf - ref.Jt)) / np.max(np.abs(ref.Jt)) < 2e-2
+
+
+def test_logistic_deer_solve_bf16_jac_mini_c1():
+    """Same fixed point as FP32 (trace error vs the FP64 sequential run
+    <= 1e-5, identical decisions); only the iteration count may change."""
+    from ParallelMCMC import DEER
+
+    rng, tgt, model = _setup(7, 2000, 100)
+    D, T, eps = 100, 512, 5e-3
+    x0 = (0.1 * rng.standard_normal(D)).astype(np.float32)
+    Xi, U = O.make_tape(rng, D, T, np.float32)
+    ref, acc_ref, _ = O.run_mala_sequential_taped(tgt, x0.astype(np.float64), eps, Xi.astype(np.float64),
+                                                  U.astype(np.float64))
+    rec = DEER.TapedRecursion(model, eps, Xi, U, precision="bf16_jac")
+    S, info = DEER.solve(rec, x0, tol_abs=1e-6, tol_rel=1e-6, maxiter=500, damping=1.0, seed=11,
+                         return_info=True)
+    assert info.converged, info
+    assert O.trace_error(S.cpu().numpy().astype(np.float64), ref) < 1e-5
+    assert np.array_equal(info.accepted.cpu().numpy().astype(bool), acc_ref)
