@@ -218,7 +218,8 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
     constexpr int SX = Dp + 2;
     constexpr int SR = NC + 16;
     constexpr int NCT = NC / 16;             // column tiles
-    constexpr int CT_PER_WAVE = (NCT + 3) / 4;
+    static_assert(NCT % 4 == 0, "column tiles must divide over the 4 waves");
+    constexpr int CT_PER_WAVE = NCT / 4;
     constexpr int F4 = NB * Dp / 4;          // float4 per design chunk
     constexpr int PF = (F4 + 255) / 256;     // float4 per thread per chunk
     __shared__ float Vs[NC * SV];
@@ -364,20 +365,22 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
                 for (int dt = 0; dt < DPT; ++dt) xa[dt] = X_[(4 * ks + lk) * SX + 16 * dt + li];
 #pragma unroll
                 for (int cc = 0; cc < CT_PER_WAVE; ++cc) {
+                    // NCT is a multiple of 4: every wave owns CT_PER_WAVE tiles (no branch,
+                    // so the accumulators stay in place across k steps)
                     const int ct = wave + 4 * cc;
-                    if (ct < NCT) {
-                        const float bv = Rs[(4 * ks + lk) * SR + 16 * ct + li];
+                    const float bv = Rs[(4 * ks + lk) * SR + 16 * ct + li];
 #pragma unroll
-                        for (int dt = 0; dt < DPT; ++dt)
-                            acc[cc][dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[dt], bv, acc[cc][dt], 0, 0, 0);
-                    }
+                    for (int dt = 0; dt < DPT; ++dt)
+                        acc[cc][dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[dt], bv, acc[cc][dt], 0, 0, 0);
                 }
             }
         } else {
 #pragma unroll
             for (int m = 0; m < DPW; ++m) {
-                const int dt = wave + 4 * m;
-                if (dt < DPT) {
+                // d tile of this wave, clamped so every wave runs the same
+                // branch-free sequence (a clamped duplicate is not stored)
+                const int dt = min(wave + 4 * m, DPT - 1);
+                {
                     // residual columns: FP32 MFMA over the 32 rows
 #pragma unroll
                     for (int ks = 0; ks < NB / 4; ++ks) {
