@@ -104,6 +104,12 @@ int pmcmc_target_diag_gaussian(pmcmc_ctx* ctx, int64_t D, int dtype_prec, const 
  * benchmarks/ParallelMCMCBenchmarks/src/models/bayes_logreg.jl:91-206.      */
 int pmcmc_target_logistic(pmcmc_ctx* ctx, int64_t N, int64_t D, int dtype_prec, const void* Xd,
                           const void* y, double prior_scale, pmcmc_target** out);
+/* Dense Gaussian N(mean, Lambda^-1); Lambda is D x D symmetric (column-major),
+ * mean may be NULL (zero).  The precision-matrix form of the correlated
+ * Gaussian models of the reference's GPU tests (grad = -Lambda (x - mean),
+ * Hv = -Lambda v), evaluated as MFMA GEMMs.                                  */
+int pmcmc_target_gaussian(pmcmc_ctx* ctx, int64_t D, int dtype_prec, const void* Lambda,
+                          const void* mean, pmcmc_target** out);
 int pmcmc_target_destroy(pmcmc_target* tgt);
 int64_t pmcmc_target_dim(const pmcmc_target* tgt);
 

@@ -17,7 +17,7 @@ from .interface import (
     sample,
     step,
 )
-from .targets import DiagGaussianTarget, LogisticTarget, StdNormalTarget
+from .targets import DiagGaussianTarget, GaussianTarget, LogisticTarget, StdNormalTarget
 from . import DEER, DEERScan, MALA  # noqa: E402
 
 __all__ = [
@@ -34,6 +34,7 @@ __all__ = [
     "DiagGaussianTarget",
     "StdNormalTarget",
     "LogisticTarget",
+    "GaussianTarget",
     "ArgumentError",
     "DimensionMismatch",
     "PmcmcError",

@@ -89,3 +89,24 @@ class LogisticTarget(GPUTarget):
         ctx.call("pmcmc_target_logistic", self.N, self.dim, _lib.PREC_F64, _device.ptr(Xd), _device.ptr(y),
                  self.prior_scale, C.byref(h))
         return h
+
+
+class GaussianTarget(GPUTarget):
+    """N(mean, Lam^-1) with a dense symmetric precision Lam: grad = -Lam (x - mean),
+    Hv = -Lam v; evaluated as MFMA GEMMs over all trajectory columns."""
+
+    def __init__(self, Lam, mean=None):
+        super().__init__()
+        self.Lam = np.asarray(Lam, dtype=np.float64)
+        if self.Lam.ndim != 2 or self.Lam.shape[0] != self.Lam.shape[1]:
+            raise _lib.DimensionMismatch("Lam must be square")
+        self.mean = None if mean is None else np.asarray(mean, dtype=np.float64)
+        self.dim = self.Lam.shape[0]
+
+    def _create(self, ctx):
+        dev = torch.device("cuda", ctx.device)
+        L = _device.to_colmajor(self.Lam, torch.float64, dev)
+        m = torch.from_numpy(self.mean).to(dev) if self.mean is not None else None
+        h = C.c_void_p()
+        ctx.call("pmcmc_target_gaussian", self.dim, _lib.PREC_F64, _device.ptr(L), _device.ptr(m), C.byref(h))
+        return h

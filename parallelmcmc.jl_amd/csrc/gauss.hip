@@ -1,0 +1,431 @@
+// Dense Gaussian target: the Newton-iteration products as Lambda GEMMs.
+//
+// One pass over a column set [x - mean | z] (or [y - mean | w]) computes
+// P = Lambda V on MFMA (K = D streamed through LDS in 32-wide slices) and, in
+// the epilogue, g = -P_x, h = -P_z, logp = (x - mean).g / 2 and, for the pass
+// at Xbar, Y = x + eps g + sqrt(2 eps) xi and W = z + eps h.  The products
+// feed the same MODE_GEMM tile kernels as the logistic target.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "../../include/pmcmc_amd.h"
+#include "gauss.hpp"
+
+namespace pmcmc {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+static int64_t round_up_g(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+static int grid_cap_g(int64_t n, int block = 256, int cap = 8192) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+template <typename Tin>
+__global__ void gauss_prep_kernel(const Tin* L, const Tin* mu, int64_t D, int64_t Dp, float* L32,
+                                  double* L64, float* mu32, double* mu64) {
+    const int64_t n_all = Dp * Dp;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_all;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i / Dp, c = i % Dp;  // row-major Dp x Dp
+        const bool in = r < D && c < D;
+        const double v = in ? (double)L[r + c * D] : 0.0;
+        L32[i] = (float)v;
+        if (in) L64[r + c * D] = v;
+        if (c == 0 && r < D && mu) {
+            mu32[r] = (float)mu[r];
+            mu64[r] = (double)mu[r];
+        }
+    }
+}
+
+int gauss_data_create(GaussData* g, hipStream_t s, int64_t D, bool src_f64, const void* Lam,
+                      const void* mean, std::string* msg) {
+    g->D = D;
+    g->Dp = round_up_g(D, 32);
+    bool ok = hipMalloc(&g->L32, g->Dp * g->Dp * sizeof(float)) == hipSuccess &&
+              hipMalloc(&g->L64, D * D * sizeof(double)) == hipSuccess;
+    if (ok && mean)
+        ok = hipMalloc(&g->mu32, D * sizeof(float)) == hipSuccess &&
+             hipMalloc(&g->mu64, D * sizeof(double)) == hipSuccess;
+    if (!ok) {
+        *msg = "gaussian target: device allocation failed";
+        return PMCMC_ERR_ALLOC;
+    }
+    const int grid = grid_cap_g(g->Dp * g->Dp);
+    if (src_f64)
+        hipLaunchKernelGGL((gauss_prep_kernel<double>), dim3(grid), dim3(256), 0, s, (const double*)Lam,
+                           (const double*)mean, D, g->Dp, g->L32, g->L64, g->mu32, g->mu64);
+    else
+        hipLaunchKernelGGL((gauss_prep_kernel<float>), dim3(grid), dim3(256), 0, s, (const float*)Lam,
+                           (const float*)mean, D, g->Dp, g->L32, g->L64, g->mu32, g->mu64);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        *msg = "gaussian target: preparation kernel failed";
+        return PMCMC_ERR_HIP;
+    }
+    return PMCMC_OK;
+}
+
+void gauss_data_destroy(GaussData* g) {
+    (void)hipFree(g->L32);
+    (void)hipFree(g->L64);
+    (void)hipFree(g->mu32);
+    (void)hipFree(g->mu64);
+    *g = GaussData{};
+}
+
+static constexpr int GCB = 32;  // t-columns per column block
+static constexpr int GKC = 32;  // K slice
+
+// Epilogue shared by both paths: per column t, given the products at
+// (set 0) P0 = Lambda (v0 - mean) and (sets k) Pk = Lambda v_k.
+//   pass 0: gx = -P0, W_k = z_k - eps P_k, Y = x + eps gx + sqrt(2 eps) xi, lpx
+//   pass 1: gy = -P0, hw_k = -P_k, lpy
+
+// ---------------------------------------------------------------------------
+// FP32 MFMA pass (probes K = 1).  grid: ceil(T/GCB) blocks of 256 threads.
+// Wave w owns output column tile w (tiles 0,1: states; 2,3: directions) for
+// every 16-row tile of Lambda's rows.  JBF: the direction tiles on BF16 MFMA.
+// ---------------------------------------------------------------------------
+template <int DPT, bool JBF>
+__global__ __launch_bounds__(256) void gauss_pass_mfma_kernel(ColSrc<float> src, const float* L32,
+                                                              const float* mu, int pass, float eps,
+                                                              const float* Xi, float* gout, float* hout,
+                                                              double* lpout, float* Yout, float* Wout,
+                                                              const int* skip) {
+    constexpr int Dp = DPT * 16;
+    constexpr int NC = 2 * GCB;
+    constexpr int SL = GKC + 4;   // Lambda slice [d][k]
+    constexpr int SVc = GKC + 4;  // column slice [j][k]
+    __shared__ float Ls[Dp * SL];
+    __shared__ float Vc[NC * SVc];
+    __shared__ double lpred[4][16];
+
+    if (skip && *(volatile const int*)skip) return;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t D = src.D, Tn = src.Tn;
+    const int64_t cb = blockIdx.x;
+
+    v4f acc[DPT];
+#pragma unroll
+    for (int dt = 0; dt < DPT; ++dt) acc[dt] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+    for (int k0 = 0; k0 < Dp; k0 += GKC) {
+        __syncthreads();
+        // Lambda[:, k0:k0+32] (rows d, contiguous in k in the row-major copy)
+        for (int i = tid; i < Dp * (GKC / 4); i += 256) {
+            const int d = i / (GKC / 4), q = i % (GKC / 4);
+            const float4 v = *reinterpret_cast<const float4*>(L32 + (size_t)d * Dp + k0 + 4 * q);
+            float* dst = &Ls[d * SL + 4 * q];
+            dst[0] = v.x;
+            dst[1] = v.y;
+            dst[2] = v.z;
+            dst[3] = v.w;
+        }
+        for (int i = tid; i < NC * GKC; i += 256) {
+            const int j = i / GKC, k = i % GKC;
+            const int set = j / GCB;
+            const int64_t t = cb * GCB + (j % GCB);
+            const int64_t kk = k0 + k;
+            float v = 0.f;
+            if (t < Tn && kk < D) {
+                v = (float)src.get(set, t, kk);
+                if (set == 0 && mu) v -= mu[kk];
+            }
+            Vc[j * SVc + k] = v;
+        }
+        __syncthreads();
+        const int ct = wave;  // output column tile
+        if (!JBF || ct < 2) {
+#pragma unroll
+            for (int ks = 0; ks < GKC / 4; ++ks) {
+                const float bv = Vc[(16 * ct + li) * SVc + 4 * ks + lk];
+#pragma unroll
+                for (int dt = 0; dt < DPT; ++dt) {
+                    const float av = Ls[(16 * dt + li) * SL + 4 * ks + lk];
+                    acc[dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[dt], 0, 0, 0);
+                }
+            }
+        } else {
+            v8bf bv;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) bv[e] = (__bf16)Vc[(16 * ct + li) * SVc + 8 * lk + e];
+#pragma unroll
+            for (int dt = 0; dt < DPT; ++dt) {
+                v8bf av;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) av[e] = (__bf16)Ls[(16 * dt + li) * SL + 8 * lk + e];
+                acc[dt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[dt], 0, 0, 0);
+            }
+        }
+    }
+
+    // epilogue: lane holds P[d = 16 dt + 4 lk + r][column 16 wave + li]
+    const int j = 16 * wave + li;
+    const int set = j / GCB;
+    const int64_t t = cb * GCB + (j % GCB);
+    const bool tv = t < Tn;
+    const double sq2e = sqrt(2.0 * (double)eps);
+    double lp = 0.0;
+    if (tv) {
+#pragma unroll
+        for (int dt = 0; dt < DPT; ++dt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t d = 16 * dt + 4 * lk + r;
+                if (d >= D) continue;
+                const size_t gi = (size_t)t * D + d;
+                const float g = -acc[dt][r];
+                if (set == 0) {
+                    const double x = src.get(0, t, d);
+                    const double xm = x - (mu ? (double)mu[d] : 0.0);
+                    lp += 0.5 * xm * (double)g;
+                    gout[gi] = g;
+                    if (pass == 0) Yout[gi] = (float)(x + (double)eps * g + sq2e * (double)Xi[gi]);
+                } else {
+                    if (pass == 0)
+                        Wout[gi] = (float)(src.get(1, t, d) + (double)eps * g);
+                    else
+                        hout[gi] = g;
+                }
+            }
+    }
+    lp += __shfl_xor(lp, 16, 64);
+    lp += __shfl_xor(lp, 32, 64);
+    if (wave < 2 && lk == 0 && tv) lpout[t] = lp;
+}
+
+// ---------------------------------------------------------------------------
+// plain path (FP64, or several probes): thread per (set, t, d) output
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void gauss_plain_kernel(ColSrc<T> src, const double* L64, const double* mu, int pass,
+                                   double eps, const T* Xi, T* gout, T* hout, T* Yout, T* Wout,
+                                   const int* skip) {
+    if (skip && *(volatile const int*)skip) return;
+    const int64_t D = src.D, Tn = src.Tn;
+    const int K = src.K;
+    const int64_t total = (int64_t)(1 + K) * Tn * D;
+    const double sq2e = sqrt(2.0 * eps);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t d = i % D, st = i / D, t = st % Tn;
+        const int set = (int)(st / Tn);
+        double p = 0.0;
+        for (int64_t k = 0; k < D; ++k) {
+            double v = src.get(set, t, k);
+            if (set == 0 && mu) v -= mu[k];
+            p += L64[d + k * D] * v;
+        }
+        const double g = -p;
+        const size_t gi = (size_t)t * D + d;
+        if (set == 0) {
+            gout[gi] = (T)g;
+            if (pass == 0) Yout[gi] = (T)(src.get(0, t, d) + eps * g + sq2e * (double)Xi[gi]);
+        } else {
+            const size_t ki = (size_t)(set - 1) * D * Tn + gi;
+            if (pass == 0)
+                Wout[ki] = (T)(src.get(set, t, d) + eps * g);
+            else
+                hout[ki] = (T)g;
+        }
+    }
+}
+
+// logp of the set-0 columns from the stored gradient: (v - mean).g / 2
+template <typename T>
+__global__ __launch_bounds__(256) void gauss_lp_kernel(ColSrc<T> src, const double* mu, const T* g,
+                                                       double* lpout, const int* skip) {
+    if (skip && *(volatile const int*)skip) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= src.Tn) return;
+    double s = 0.0;
+    for (int64_t d = lane; d < src.D; d += 64) {
+        const double xm = src.get(0, t, d) - (mu ? mu[d] : 0.0);
+        s += 0.5 * xm * (double)g[(size_t)t * src.D + d];
+    }
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+    if (lane == 0) lpout[t] = s;
+}
+
+template <int DPT>
+static void launch_gauss_mfma(const ColSrc<float>& src, const GaussData* g, int pass, double eps,
+                              const float* Xi, float* gout, float* hout, double* lpout, float* Yout,
+                              float* Wout, const int* skip, bool jbf, hipStream_t s) {
+    const dim3 grid((unsigned)((src.Tn + GCB - 1) / GCB));
+    if (jbf)
+        hipLaunchKernelGGL((gauss_pass_mfma_kernel<DPT, true>), grid, dim3(256), 0, s, src, g->L32, g->mu32, pass,
+                           (float)eps, Xi, gout, hout, lpout, Yout, Wout, skip);
+    else
+        hipLaunchKernelGGL((gauss_pass_mfma_kernel<DPT, false>), grid, dim3(256), 0, s, src, g->L32, g->mu32, pass,
+                           (float)eps, Xi, gout, hout, lpout, Yout, Wout, skip);
+}
+
+template <typename T>
+static int gauss_pass(const GaussData* g, hipStream_t s, const ColSrc<T>& src, int pass, double eps,
+                      const T* Xi, T* gout, T* hout, double* lpout, T* Yout, T* Wout, const int* skip,
+                      Prof* prof, int iter, bool jbf, std::string* msg) {
+    const int pe = prof ? prof->begin(KT_GLM_PASS, iter, s) : -1;
+    if (sizeof(T) == 4 && src.K == 1 && g->Dp <= 256) {
+        const ColSrc<float>& fs = reinterpret_cast<const ColSrc<float>&>(src);
+        float* go = (float*)gout;
+        float* ho = (float*)hout;
+        float* yo = (float*)Yout;
+        float* wo = (float*)Wout;
+        const float* xi = (const float*)Xi;
+        switch (g->Dp / 16) {
+#define PMCMC_GDPT(n) \
+    case n: launch_gauss_mfma<n>(fs, g, pass, eps, xi, go, ho, lpout, yo, wo, skip, jbf, s); break;
+            PMCMC_GDPT(2) PMCMC_GDPT(4) PMCMC_GDPT(6) PMCMC_GDPT(8) PMCMC_GDPT(10) PMCMC_GDPT(12)
+            PMCMC_GDPT(14) PMCMC_GDPT(16)
+#undef PMCMC_GDPT
+            default:
+                *msg = "gaussian target: unsupported D";
+                return PMCMC_ERR_UNSUPPORTED;
+        }
+    } else {
+        hipLaunchKernelGGL((gauss_plain_kernel<T>), dim3(grid_cap_g((1 + src.K) * src.Tn * src.D)), dim3(256), 0, s,
+                           src, g->L64, g->mu64, pass, eps, Xi, gout, hout, Yout, Wout, skip);
+        hipLaunchKernelGGL((gauss_lp_kernel<T>), dim3((unsigned)((src.Tn + 3) / 4)), dim3(256), 0, s, src, g->mu64,
+                           (const T*)gout, lpout, skip);
+    }
+    if (prof) prof->end(pe, s);
+    if (hipGetLastError() != hipSuccess) {
+        *msg = "gaussian pass: kernel launch failed";
+        return PMCMC_ERR_HIP;
+    }
+    return PMCMC_OK;
+}
+
+template <typename T>
+int gauss_newton_products(GlmWork* w, const GaussData* g, hipStream_t s, SyncState* sync,
+                          int precision, double eps, const T* x0, const T* S_in, const T* Xi,
+                          uint64_t seed, int iter, int probes, int64_t D, int64_t Tn, Prof* prof,
+                          std::string* msg) {
+    const bool jbf = precision == PMCMC_PREC_BF16_JAC;
+    const int* skip = sync ? &sync->converged : nullptr;
+    ColSrc<T> a{};
+    a.V = S_in;
+    a.x0 = x0;
+    a.Z = nullptr;
+    a.K = probes;
+    a.D = D;
+    a.Tn = Tn;
+    for (int k = 0; k < probes && k < 4; ++k) a.keys[k] = probe_key(seed, (uint32_t)iter, (uint32_t)k);
+    int rc = gauss_pass<T>(g, s, a, 0, eps, Xi, (T*)w->gx, (T*)w->hw, w->lpx, (T*)w->Y, (T*)w->W, skip, prof,
+                           iter, jbf, msg);
+    if (rc != PMCMC_OK) return rc;
+    ColSrc<T> b{};
+    b.V = (const T*)w->Y;
+    b.x0 = nullptr;
+    b.Z = (const T*)w->W;
+    b.K = probes;
+    b.D = D;
+    b.Tn = Tn;
+    return gauss_pass<T>(g, s, b, 1, eps, Xi, (T*)w->gy, (T*)w->hw, w->lpy, nullptr, nullptr, skip, prof, iter,
+                         jbf, msg);
+}
+template int gauss_newton_products<float>(GlmWork*, const GaussData*, hipStream_t, SyncState*, int, double,
+                                          const float*, const float*, const float*, uint64_t, int, int, int64_t,
+                                          int64_t, Prof*, std::string*);
+template int gauss_newton_products<double>(GlmWork*, const GaussData*, hipStream_t, SyncState*, int, double,
+                                           const double*, const double*, const double*, uint64_t, int, int,
+                                           int64_t, int64_t, Prof*, std::string*);
+
+template <typename T>
+static int gauss_fwd_jvp_t(const GaussData* g, hipStream_t s, double eps, const T* X, const T* Xi, const T* U,
+                           const T* Z, T* FT, T* Jt, uint8_t* acc, double* la, int64_t D, int64_t Tn, bool jbf,
+                           std::string* msg) {
+    GlmWork w{};
+    GlmData dummy{};
+    dummy.N = 1;
+    dummy.Dp = g->Dp;
+    int rc = glm_work_ensure(&w, &dummy, s, D, Tn, 1, sizeof(T) == 8, msg);
+    if (rc != PMCMC_OK) return rc;
+    ColSrc<T> a{};
+    a.V = X;
+    a.Z = Z;
+    a.K = 1;
+    a.D = D;
+    a.Tn = Tn;
+    rc = gauss_pass<T>(g, s, a, 0, eps, Xi, (T*)w.gx, (T*)w.hw, w.lpx, (T*)w.Y, (T*)w.W, nullptr, nullptr, 0,
+                       jbf, msg);
+    if (rc == PMCMC_OK) {
+        ColSrc<T> b = a;
+        b.V = (const T*)w.Y;
+        b.Z = (const T*)w.W;
+        rc = gauss_pass<T>(g, s, b, 1, eps, Xi, (T*)w.gy, (T*)w.hw, w.lpy, nullptr, nullptr, nullptr, nullptr, 0,
+                           jbf, msg);
+    }
+    if (rc == PMCMC_OK) {
+        launch_fwd_jvp_finish<T>(s, X, Z, U, &w, eps, D, Tn, FT, Jt, acc, la);
+        if (hipGetLastError() != hipSuccess) {
+            *msg = "gaussian finish kernel failed";
+            rc = PMCMC_ERR_HIP;
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    glm_work_destroy(&w);
+    return rc;
+}
+
+int gauss_fwd_and_jvp(const GaussData* g, hipStream_t s, int prec, double eps, const void* X,
+                      const void* Xi, const void* U, const void* Z, void* FT, void* Jt,
+                      uint8_t* accepted, double* logalpha, int64_t D, int64_t T, std::string* msg) {
+    if (prec == PMCMC_PREC_F64)
+        return gauss_fwd_jvp_t<double>(g, s, eps, (const double*)X, (const double*)Xi, (const double*)U,
+                                       (const double*)Z, (double*)FT, (double*)Jt, accepted, logalpha, D, T,
+                                       false, msg);
+    return gauss_fwd_jvp_t<float>(g, s, eps, (const float*)X, (const float*)Xi, (const float*)U, (const float*)Z,
+                                  (float*)FT, (float*)Jt, accepted, logalpha, D, T,
+                                  prec == PMCMC_PREC_BF16_JAC, msg);
+}
+
+// logp in double: one workgroup per column
+template <typename T>
+__global__ __launch_bounds__(256) void gauss_logp_kernel(const T* S, const double* L64, const double* mu,
+                                                         int64_t D, double* out) {
+    __shared__ double xm[1024];
+    __shared__ double red[4];
+    const int64_t t = blockIdx.x;
+    for (int64_t d = threadIdx.x; d < D; d += 256) xm[d] = (double)S[(size_t)t * D + d] - (mu ? mu[d] : 0.0);
+    __syncthreads();
+    double acc = 0.0;
+    for (int64_t d = threadIdx.x; d < D; d += 256) {
+        double p = 0.0;
+        for (int64_t k = 0; k < D; ++k) p += L64[d + k * D] * xm[k];
+        acc += -0.5 * xm[d] * p;
+    }
+    for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[t] = red[0] + red[1] + red[2] + red[3];
+}
+
+int gauss_logdensity(const GaussData* g, hipStream_t s, int prec, const void* S, double* out,
+                     int64_t T, std::string* msg) {
+    if (g->D > 1024) {
+        *msg = "gaussian log-density: D > 1024";
+        return PMCMC_ERR_UNSUPPORTED;
+    }
+    if (prec == PMCMC_PREC_F64)
+        hipLaunchKernelGGL((gauss_logp_kernel<double>), dim3((unsigned)T), dim3(256), 0, s, (const double*)S, g->L64,
+                           g->mu64, g->D, out);
+    else
+        hipLaunchKernelGGL((gauss_logp_kernel<float>), dim3((unsigned)T), dim3(256), 0, s, (const float*)S, g->L64,
+                           g->mu64, g->D, out);
+    if (hipGetLastError() != hipSuccess) {
+        *msg = "gaussian log-density kernel failed";
+        return PMCMC_ERR_HIP;
+    }
+    return PMCMC_OK;
+}
+
+}  // namespace pmcmc

@@ -1,0 +1,42 @@
+// Dense Gaussian target N(mean, Lambda^-1): grad = -Lambda (x - mean),
+// Hv = -Lambda v, logp = -(x - mean)' Lambda (x - mean) / 2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "common.hpp"
+#include "glm.hpp"
+
+namespace pmcmc {
+
+struct GaussData {
+    int64_t D = 0, Dp = 0;
+    float* L32 = nullptr;    // Dp x Dp row-major, zero padded
+    double* L64 = nullptr;   // D x D column-major
+    float* mu32 = nullptr;   // null: zero mean
+    double* mu64 = nullptr;
+};
+
+int gauss_data_create(GaussData* g, hipStream_t s, int64_t D, bool src_f64, const void* Lam,
+                      const void* mean, std::string* msg);
+void gauss_data_destroy(GaussData* g);
+
+// Same products and buffers as glm_newton_products (GlmWork): gx, W, Y, lpx
+// at Xbar; gy, hw, lpy at Y.
+template <typename T>
+int gauss_newton_products(GlmWork* w, const GaussData* g, hipStream_t s, SyncState* sync,
+                          int precision, double eps, const T* x0, const T* S_in, const T* Xi,
+                          uint64_t seed, int iter, int probes, int64_t D, int64_t Tn, Prof* prof,
+                          std::string* msg);
+
+int gauss_fwd_and_jvp(const GaussData* g, hipStream_t s, int prec, double eps, const void* X,
+                      const void* Xi, const void* U, const void* Z, void* FT, void* Jt,
+                      uint8_t* accepted, double* logalpha, int64_t D, int64_t T, std::string* msg);
+
+int gauss_logdensity(const GaussData* g, hipStream_t s, int prec, const void* S, double* out,
+                     int64_t T, std::string* msg);
+
+}  // namespace pmcmc

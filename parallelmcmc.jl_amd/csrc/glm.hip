@@ -173,32 +173,6 @@ void glm_work_destroy(GlmWork* w) {
 }
 
 // ---------------------------------------------------------------------------
-// column sources of a pass
-// ---------------------------------------------------------------------------
-// Set 0 holds the states: column t is V[:, t], or with `shift` the iterate
-// shifted by one step (x0 for t = 0, V[:, t-1] otherwise) = Xbar.
-// Sets 1..K hold directions: Zsrc[k] if given, else the Rademacher probe k.
-template <typename T>
-struct ColSrc {
-    const T* V;
-    const T* x0;  // non-null: shifted
-    const T* Z;   // K x D x T or null
-    uint32_t keys[4];
-    int K;
-    int64_t D, Tn;
-
-    __device__ inline double get(int set, int64_t t, int64_t d) const {
-        if (set == 0) {
-            if (x0) return t == 0 ? (double)x0[d] : (double)V[(size_t)(t - 1) * D + d];
-            return (double)V[(size_t)t * D + d];
-        }
-        const size_t gi = (size_t)t * D + d;
-        if (Z) return (double)Z[(size_t)(set - 1) * D * Tn + gi];
-        return (double)rademacher(keys[set - 1], gi);
-    }
-};
-
-// ---------------------------------------------------------------------------
 // FP32 MFMA fused pass
 // ---------------------------------------------------------------------------
 // grid: (ceil(T/CB), NS); block 256 = 4 waves.
@@ -726,6 +700,21 @@ __global__ __launch_bounds__(256) void glm_finish_kernel(const T* X, const T* Z,
         Jt[gi] = (T)((acc ? w : z) + (y - x) * dg);
     }
 }
+
+template <typename T>
+void launch_fwd_jvp_finish(hipStream_t s, const T* X, const T* Z, const T* U, const GlmWork* w,
+                           double eps, int64_t D, int64_t Tn, T* FT, T* Jt, uint8_t* acc,
+                           double* la) {
+    hipLaunchKernelGGL((glm_finish_kernel<T>), dim3((unsigned)((Tn + 3) / 4)), dim3(256), 0, s, X, Z, U,
+                       (const T*)w->Y, (const T*)w->gx, (const T*)w->W, (const T*)w->gy, (const T*)w->hw,
+                       w->lpx, w->lpy, eps, D, Tn, FT, Jt, acc, la);
+}
+template void launch_fwd_jvp_finish<float>(hipStream_t, const float*, const float*, const float*,
+                                           const GlmWork*, double, int64_t, int64_t, float*, float*,
+                                           uint8_t*, double*);
+template void launch_fwd_jvp_finish<double>(hipStream_t, const double*, const double*, const double*,
+                                            const GlmWork*, double, int64_t, int64_t, double*, double*,
+                                            uint8_t*, double*);
 
 template <typename T>
 static int fwd_jvp_t(const GlmData* g, hipStream_t s, double eps, double prior_scale, const T* X,

@@ -21,6 +21,30 @@ struct GlmData {
     double* y64 = nullptr;
 };
 
+// Column sources of a GEMM pass.
+// Set 0 holds the states: column t is V[:, t], or with `shift` the iterate
+// shifted by one step (x0 for t = 0, V[:, t-1] otherwise) = Xbar.
+// Sets 1..K hold directions: Zsrc[k] if given, else the Rademacher probe k.
+template <typename T>
+struct ColSrc {
+    const T* V;
+    const T* x0;  // non-null: shifted
+    const T* Z;   // K x D x T or null
+    uint32_t keys[4];
+    int K;
+    int64_t D, Tn;
+
+    __device__ inline double get(int set, int64_t t, int64_t d) const {
+        if (set == 0) {
+            if (x0) return t == 0 ? (double)x0[d] : (double)V[(size_t)(t - 1) * D + d];
+            return (double)V[(size_t)t * D + d];
+        }
+        const size_t gi = (size_t)t * D + d;
+        if (Z) return (double)Z[(size_t)(set - 1) * D * Tn + gi];
+        return (double)rademacher(keys[set - 1], gi);
+    }
+};
+
 struct GlmWork {
     int64_t D = 0, T = 0;
     int probes = 0;
@@ -52,6 +76,13 @@ int glm_newton_products(GlmWork* w, const GlmData* g, hipStream_t s, SyncState* 
                         double eps, double prior_scale, const T* x0, const T* S_in, const T* Xi,
                         uint64_t seed, int iter, int probes, int64_t D, int64_t Tn, Prof* prof,
                         std::string* msg);
+
+// logalpha, accept, FT and Jt of the batched MALA step + JVP from the products
+// of the two passes (one wave per column).
+template <typename T>
+void launch_fwd_jvp_finish(hipStream_t s, const T* X, const T* Z, const T* U, const GlmWork* w,
+                           double eps, int64_t D, int64_t Tn, T* FT, T* Jt, uint8_t* acc,
+                           double* la);
 
 int glm_fwd_and_jvp(const GlmData* g, hipStream_t s, int prec, double eps, double prior_scale,
                     const void* X, const void* Xi, const void* U, const void* Z, void* FT, void* Jt,

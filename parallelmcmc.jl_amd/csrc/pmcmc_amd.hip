@@ -15,6 +15,7 @@
 
 #include "../../include/pmcmc_amd.h"
 #include "common.hpp"
+#include "gauss.hpp"
 #include "glm.hpp"
 #include "tile_kernel.hpp"
 
@@ -49,7 +50,7 @@ struct pmcmc_ctx {
     Prof prof;
 };
 
-enum { TGT_DIAG = 0, TGT_LOGISTIC = 1 };
+enum { TGT_DIAG = 0, TGT_LOGISTIC = 1, TGT_GAUSS = 2 };
 
 struct pmcmc_target {
     pmcmc_ctx* ctx = nullptr;
@@ -63,6 +64,8 @@ struct pmcmc_target {
     // logistic regression
     GlmData glm;
     double prior_scale = 1.0;
+    // dense Gaussian
+    GaussData gauss;
 };
 
 struct pmcmc_deer_ws {
@@ -518,6 +521,30 @@ extern "C" int pmcmc_target_logistic(pmcmc_ctx* c, int64_t N, int64_t D, int dty
     return PMCMC_OK;
 }
 
+extern "C" int pmcmc_target_gaussian(pmcmc_ctx* c, int64_t D, int dtype_prec, const void* Lambda,
+                                     const void* mean, pmcmc_target** out) {
+    if (!c || !out) return PMCMC_ERR_ARGUMENT;
+    *out = nullptr;
+    if (D < 1) return fail(c, PMCMC_ERR_DIMENSION, "target dim must be >= 1");
+    if (!Lambda) return fail(c, PMCMC_ERR_ARGUMENT, "gaussian target: null precision matrix");
+    if (!valid_prec(dtype_prec)) return fail(c, PMCMC_ERR_ARGUMENT, "bad precision %d", dtype_prec);
+    HIPCHK(c, hipSetDevice(c->device));
+    pmcmc_target* t = new (std::nothrow) pmcmc_target();
+    if (!t) return fail(c, PMCMC_ERR_ALLOC, "host allocation failed");
+    t->ctx = c;
+    t->kind = TGT_GAUSS;
+    t->D = D;
+    std::string msg;
+    int rc = gauss_data_create(&t->gauss, c->stream, D, dtype_prec == PMCMC_PREC_F64, Lambda, mean, &msg);
+    if (rc != PMCMC_OK) {
+        fail(c, rc, "%s", msg.c_str());
+        pmcmc_target_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return PMCMC_OK;
+}
+
 extern "C" int pmcmc_target_destroy(pmcmc_target* t) {
     if (!t) return PMCMC_OK;
     if (t->ctx) hipSetDevice(t->ctx->device);
@@ -527,6 +554,7 @@ extern "C" int pmcmc_target_destroy(pmcmc_target* t) {
     hipFree(t->prec_d);
     hipFree(t->mean_d);
     glm_data_destroy(&t->glm);
+    gauss_data_destroy(&t->gauss);
     delete t;
     return PMCMC_OK;
 }
@@ -562,6 +590,11 @@ extern "C" int pmcmc_mala_fwd_and_jvp(pmcmc_ctx* c, const pmcmc_target* tgt, int
                                tgt->prec_f, tgt->mean_f, (float*)FT, (float*)Jt, accepted, logalpha,
                                eps, D, Tn, G);
         HIPCHK(c, hipGetLastError());
+    } else if (tgt->kind == TGT_GAUSS) {
+        std::string msg;
+        int rc = gauss_fwd_and_jvp(&tgt->gauss, c->stream, prec, eps, X, Xi, U, Z, FT, Jt, accepted, logalpha,
+                                   D, Tn, &msg);
+        if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
     } else {
         std::string msg;
         int rc = glm_fwd_and_jvp(&tgt->glm, c->stream, prec, eps, tgt->prior_scale, X, Xi, U, Z, FT,
@@ -591,6 +624,10 @@ extern "C" int pmcmc_target_logdensity_batch(pmcmc_ctx* c, const pmcmc_target* t
             hipLaunchKernelGGL((logp_diag_kernel<float>), dim3(grid), dim3(256), 0, c->stream,
                                (const float*)S, tgt->prec_f, tgt->mean_f, out, D, Tn, G);
         HIPCHK(c, hipGetLastError());
+    } else if (tgt->kind == TGT_GAUSS) {
+        std::string msg;
+        int rc = gauss_logdensity(&tgt->gauss, c->stream, prec, S, out, Tn, &msg);
+        if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
     } else {
         std::string msg;
         int rc = glm_logdensity(&tgt->glm, c->stream, prec, tgt->prior_scale, S, out, Tn, &msg);
@@ -691,9 +728,13 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
     const int mode = tgt->kind == TGT_DIAG ? MODE_ELEM : MODE_GEMM;
     if (mode == MODE_GEMM) {
         if (p->jacobian != PMCMC_JAC_STOCH_DIAG)
-            return fail(c, PMCMC_ERR_UNSUPPORTED, "logistic target supports jacobian=:stoch_diag only");
+            return fail(c, PMCMC_ERR_UNSUPPORTED, "GEMM targets support jacobian=:stoch_diag only");
         std::string msg;
-        int rc = glm_work_ensure(&w->glmw, &tgt->glm, c->stream, D, Tn, p->probes, sizeof(T) == 8, &msg);
+        GlmData gdummy{};
+        gdummy.N = 1;
+        gdummy.Dp = tgt->gauss.Dp;
+        const GlmData* gd = tgt->kind == TGT_GAUSS ? &gdummy : &tgt->glm;
+        int rc = glm_work_ensure(&w->glmw, gd, c->stream, D, Tn, p->probes, sizeof(T) == 8, &msg);
         if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
     }
     T* bufs[2] = {(T*)w->buf[0], (T*)w->buf[1]};
@@ -735,9 +776,13 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
             const int eit = c->prof.begin(KT_ITERATION, it, c->stream);
             if (mode == MODE_GEMM) {
                 std::string msg;
-                int rc = glm_newton_products(&w->glmw, &tgt->glm, c->stream, w->sync, p->precision,
-                                             p->eps, tgt->prior_scale, (const T*)x0, a.S_in,
-                                             (const T*)Xi, p->seed, it, p->probes, D, Tn, &c->prof, &msg);
+                int rc = tgt->kind == TGT_GAUSS
+                             ? gauss_newton_products(&w->glmw, &tgt->gauss, c->stream, w->sync, p->precision,
+                                                     p->eps, (const T*)x0, a.S_in, (const T*)Xi, p->seed, it,
+                                                     p->probes, D, Tn, &c->prof, &msg)
+                             : glm_newton_products(&w->glmw, &tgt->glm, c->stream, w->sync, p->precision,
+                                                   p->eps, tgt->prior_scale, (const T*)x0, a.S_in,
+                                                   (const T*)Xi, p->seed, it, p->probes, D, Tn, &c->prof, &msg);
                 if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
                 a.Y = (const T*)w->glmw.Y;
                 a.gx = (const T*)w->glmw.gx;
