@@ -388,6 +388,38 @@ def mala_jac_diag_batched(target, X, eps, Xi, U):
     return FT, Jd, acc, la
 
 
+def mala_jac_full_batched(target, X, eps, Xi, U):
+    """Dense surrogate Jacobian per column, J_t = a(I + eps H_x) + (1-a) I
+    + g(1-g)(y-x) c^T with c as in mala_jac_diag_batched (the matrix whose
+    columns are the JVPs of src/MALA/MALA.jl:749-795 along basis vectors).
+    No reference counterpart: the reference rejects jacobian=:full
+    (src/DEER/DEER.jl:464-465).  Returns FT, J (T x D x D), acc, logalpha."""
+    D, T = X.shape
+    GX = target.grad_batch(X)
+    Y = X + eps * GX + math.sqrt(2 * eps) * Xi
+    lpx, lpy = target.logp_batch(X), target.logp_batch(Y)
+    GY = target.grad_batch(Y)
+    la = (lpy + logq_mala_batched(X, Y, GY, eps)) - (lpx + logq_mala_batched(Y, X, GX, eps))
+    with np.errstate(divide="ignore", over="ignore"):
+        lu = np.log(U.astype(np.float64))
+        g = 1.0 / (1.0 + np.exp(-(la - lu)))
+    acc = lu < la
+    FT = np.where(acc[None, :], Y, X)
+    R = X - Y - eps * GY
+    Q = R + eps * target.hvp_batch(Y, R)
+    PQ = Q + eps * target.hvp_batch(X, Q)
+    PG = GY + eps * target.hvp_batch(X, GY)
+    Cm = PG - GX - (R - PQ) / (2 * eps)
+    Um = (g * (1 - g))[None, :] * (Y - X)
+    J = np.empty((T, D, D))
+    I = np.eye(D)
+    for t in range(T):
+        Hx = np.stack([target.hvp(X[:, t], I[:, i]) for i in range(D)], axis=1)
+        a = 1.0 if acc[t] else 0.0
+        J[t] = I + a * eps * Hx + np.outer(Um[:, t], Cm[:, t])
+    return FT, J, acc, la
+
+
 # ----------------------------------------------------------------------------
 # Rademacher probes (counter based; see module docstring).
 # ----------------------------------------------------------------------------
@@ -574,7 +606,7 @@ def _xbar(s0, S):
 
 
 def deer_update_rec(fwd_and_jvp_batch, s0, S_in, jacobian="stoch_diag", damping=1.0, probes=1,
-                    seed=0, it=0, scan=None, jac_diag_batch=None):
+                    seed=0, it=0, scan=None, jac_diag_batch=None, jac_full_batch=None):
     """One DEER Newton step for a generic taped recursion.
 
     src/DEER/DEER.jl:375-475, batched branch :398-419:
@@ -584,7 +616,10 @@ def deer_update_rec(fwd_and_jvp_batch, s0, S_in, jacobian="stoch_diag", damping=
     ``fwd_and_jvp_batch(Xbar, Z) -> (FT, Jt, acc, logalpha)``.
     jacobian='diag' needs ``jac_diag_batch(Xbar) -> (FT, diagJ, acc, logalpha)``
     (the reference's scalar branch :456-462 computes the same diagonal with D
-    JVPs per step).  Returns (S_out, accepted, logalpha)."""
+    JVPs per step).  jacobian='full' (no reference counterpart) needs
+    ``jac_full_batch(Xbar) -> (FT, J, acc, logalpha)`` and solves the dense
+    recurrence S_t = J_t S_{t-1} + FT_t - J_t Xbar_t sequentially.
+    Returns (S_out, accepted, logalpha)."""
     D, T = S_in.shape
     if len(s0) != D:
         raise ValueError("S must be D x T with D = length(s0)")
@@ -605,6 +640,13 @@ def deer_update_rec(fwd_and_jvp_batch, s0, S_in, jacobian="stoch_diag", damping=
             A *= 1.0 / probes
     elif jacobian == "diag":
         FT, A, acc, la = jac_diag_batch(Xbar)
+    elif jacobian == "full" and jac_full_batch is not None:
+        FT, J, acc, la = jac_full_batch(Xbar)
+        B = FT - np.einsum("tij,jt->it", J, Xbar)
+        S_out = solve_affine_seq_full(J, B, np.asarray(s0, dtype=np.float64))
+        if damping != 1.0:
+            S_out = (1 - damping) * S_in + damping * S_out
+        return S_out, acc, la
     else:
         raise ValueError("jacobian must be :diag or :stoch_diag")
     B = FT - A * Xbar
@@ -625,6 +667,10 @@ def mala_rec(target, eps, Xi, U):
     def jdiag(Xbar):
         return mala_jac_diag_batched(target, Xbar, eps, Xi, U)
 
+    def jfull(Xbar):
+        return mala_jac_full_batched(target, Xbar, eps, Xi, U)
+
+    fwd.jac_full = jfull
     return fwd, jdiag
 
 
@@ -632,12 +678,12 @@ def deer_update(target, eps, Xi, U, s0, S_in, jacobian="stoch_diag", damping=1.0
                 seed=0, it=0, scan=None):
     """deer_update_rec on the taped MALA recursion of ``target``."""
     fwd, jdiag = mala_rec(target, eps, Xi, U)
-    return deer_update_rec(fwd, s0, S_in, jacobian, damping, probes, seed, it, scan, jdiag)
+    return deer_update_rec(fwd, s0, S_in, jacobian, damping, probes, seed, it, scan, jdiag, fwd.jac_full)
 
 
 def deer_solve_rec(fwd_and_jvp_batch, s0, T, tol_abs=1e-4, tol_rel=1e-3, maxiter=200,
                    jacobian="stoch_diag", damping=1.0, probes=1, seed=0, init=None, scan=None,
-                   jac_diag_batch=None):
+                   jac_diag_batch=None, jac_full_batch=None):
     """Newton iterations to the fixed point S = f(S).  src/DEER/DEER.jl:506-595.
 
     S starts as s0 in every column (:538-541).  metric = max|S_new - S| /
@@ -667,7 +713,7 @@ def deer_solve_rec(fwd_and_jvp_batch, s0, T, tol_abs=1e-4, tol_rel=1e-3, maxiter
     for it in range(maxiter):
         iters = it + 1
         S_new, acc, la = deer_update_rec(fwd_and_jvp_batch, s0, S, jacobian, damping, probes, seed,
-                                         it, scan, jac_diag_batch)
+                                         it, scan, jac_diag_batch, jac_full_batch)
         dmax = float(np.max(np.abs(S_new - S))) if S.size else 0.0
         smax = float(np.max(np.abs(S_new))) if S.size else 0.0
         metric = dmax / (tol_abs + tol_rel * smax)
@@ -684,7 +730,7 @@ def deer_solve(target, eps, Xi, U, s0, tol_abs=1e-4, tol_rel=1e-3, maxiter=200,
     """deer_solve_rec on the taped MALA recursion of ``target``."""
     fwd, jdiag = mala_rec(target, eps, Xi, U)
     return deer_solve_rec(fwd, s0, Xi.shape[1], tol_abs, tol_rel, maxiter, jacobian, damping,
-                          probes, seed, init, scan, jdiag)
+                          probes, seed, init, scan, jdiag, fwd.jac_full)
 
 
 def trace_error(S, S_ref):
@@ -716,5 +762,6 @@ PINNED = {
     "solve_affine_scan_diag": "equals the sequential recurrence at 1e-12 on the reference's shape list; residual known answers 0.25 / 0.5; T=0",
     "deer_update": "batched Hutchinson is exact on a diagonal-Jacobian recursion (z*z=1)",
     "deer_solve": "fixed point equals run_mala_sequential_taped (decisions identical); defect <= 1e-9; converges in <= T undamped iterations",
+    "mala_jac_full_batched": "equals jac_full_via_jvps (D basis JVPs) at 1e-10; full-Jacobian DEER reaches the sequential trajectory in few iterations",
     "rademacher": "entries are +-1; mean ~0 and pairwise decorrelation over 1e6 draws",
 }

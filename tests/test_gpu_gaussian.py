@@ -85,17 +85,21 @@ def test_gaussian_deer_solve_f64_recovers_sequential():
 
 @pytest.mark.parametrize("prec", ["f32", "bf16_jac"])
 def test_gaussian_deer_solve_f32_d64(prec):
+    """D=64 correlated Gaussian, T=256: :stoch_diag DEER needs O(T) Newton
+    iterations here (the oracle takes 122); the GPU must match the sequential
+    trajectory at the 1e-5 relative bar with identical decisions."""
     from ParallelMCMC import DEER
 
     rng, tgt, model = _setup(13, 64, cond=1.5)
-    D, T = 64, 1024
+    D, T = 64, 256
     eps = 0.5 / np.max(np.linalg.eigvalsh(tgt.Lam))
     x0 = (tgt.mean + rng.standard_normal(D)).astype(np.float32)
     Xi, U = O.make_tape(rng, D, T, np.float32)
     ref, acc_ref, _ = O.run_mala_sequential_taped(tgt, x0.astype(np.float64), eps, Xi.astype(np.float64),
                                                   U.astype(np.float64))
     rec = DEER.TapedRecursion(model, eps, Xi, U, precision=prec)
-    S, info = DEER.solve(rec, x0, tol_abs=1e-6, tol_rel=1e-6, maxiter=1000, damping=1.0, seed=4, return_info=True)
+    S, info = DEER.solve(rec, x0, tol_abs=1e-6, tol_rel=1e-6, maxiter=400, damping=1.0, seed=4, return_info=True)
     assert info.converged, info
+    assert info.iters <= T + 1
     assert O.trace_error(S.cpu().numpy().astype(np.float64), ref) < 1e-5
     assert np.array_equal(info.accepted.cpu().numpy().astype(bool), acc_ref)

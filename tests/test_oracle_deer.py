@@ -160,3 +160,26 @@ def test_deer_defect_logreg():
         nxt, _, _ = O.mala_step_taped(tgt, prev, eps, Xi[:, t], U[t])
         assert np.linalg.norm(S[:, t] - nxt) <= 1e-9
         prev = S[:, t]
+
+
+def test_full_jacobian_deer_on_correlated_gaussian():
+    """With the exact dense Jacobian, undamped DEER is Newton's method on the
+    taped recursion (accept decisions frozen per iteration): it reaches the
+    sequential trajectory in a few iterations where the exact diagonal needs
+    O(T) (eps * lambda_max = 1, condition number 1e3)."""
+    rng = np.random.default_rng(13)
+    D, T = 16, 128
+    Lam, ev = O.make_corr_gaussian(rng, D, 3.0)
+    tgt = O.Gaussian(Lam, np.zeros(D))
+    eps = 1.0 / ev.max()
+    x0 = rng.standard_normal(D)
+    Xi, U = O.make_tape(rng, D, T)
+    ref, acc, _ = O.run_mala_sequential_taped(tgt, x0, eps, Xi, U)
+    S, info = O.deer_solve(tgt, eps, Xi, U, x0, tol_abs=1e-12, tol_rel=1e-11, maxiter=100, jacobian="full",
+                           damping=1.0)
+    assert info.converged and info.iters < 30, info.iters
+    assert np.max(np.abs(S - ref)) < 1e-10
+    assert np.array_equal(info.accepted, acc)
+    _, info_d = O.deer_solve(tgt, eps, Xi, U, x0, tol_abs=1e-12, tol_rel=1e-11, maxiter=400,
+                             jacobian="diag", damping=1.0, seed=1)
+    assert info_d.iters > 3 * info.iters

@@ -245,3 +245,17 @@ def test_rademacher_properties():
     assert abs((flat[1:] * flat[:-1]).mean()) < 5e-3
     # column offset t0 gives the matching slice
     assert np.array_equal(O.rademacher(9, 0, 0, 7, 20)[:, 5:], O.rademacher(9, 0, 0, 7, 15, t0=5))
+
+
+def test_exact_full_jacobian_equals_basis_jvps():
+    rng = np.random.default_rng(8)
+    for tgt in _targets(rng):
+        D, T, eps = tgt.dim, 4, 0.05
+        X, Xi = rng.standard_normal((2, D, T))
+        U = rng.uniform(0.01, 0.99, T)
+        FT, J, acc, _ = O.mala_jac_full_batched(tgt, X, eps, Xi, U)
+        for t in range(T):
+            jvp = lambda x, tt, v: O.mala_step_surrogate_sigmoid_jvp(tgt, x, eps, Xi[:, tt], U[tt], v)
+            Jb = O.jac_full_via_jvps(jvp, X[:, t], t)
+            assert np.allclose(J[t], Jb, rtol=1e-10, atol=1e-10)
+            assert np.allclose(np.diag(J[t]), O.mala_jac_diag_batched(tgt, X, eps, Xi, U)[1][:, t], atol=1e-12)
