@@ -43,6 +43,13 @@ extern "C" {
 
 #define PMCMC_JAC_STOCH_DIAG 0 /* Hutchinson diagonal, src/DEER/DEER.jl:398-419 */
 #define PMCMC_JAC_DIAG 1       /* exact diagonal, src/DEER/DEER.jl:456-462 */
+/* Dense Jacobians (dense Gaussian targets, D <= 64); the reference rejects
+ * :full (src/DEER/DEER.jl:464-465).  FULL: derivative of the taped step with
+ * the accept decision held fixed, a (I + eps H) + (1 - a) I.  FULL_SIGMOID:
+ * the surrogate's dense Jacobian, the matrix of the JVPs of
+ * src/MALA/MALA.jl:749-795 (adds g(1-g)(y-x) grad(log alpha)^T). */
+#define PMCMC_JAC_FULL 2
+#define PMCMC_JAC_FULL_SIGMOID 3
 
 typedef struct pmcmc_ctx pmcmc_ctx;
 typedef struct pmcmc_target pmcmc_target;

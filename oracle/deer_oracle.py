@@ -388,10 +388,13 @@ def mala_jac_diag_batched(target, X, eps, Xi, U):
     return FT, Jd, acc, la
 
 
-def mala_jac_full_batched(target, X, eps, Xi, U):
-    """Dense surrogate Jacobian per column, J_t = a(I + eps H_x) + (1-a) I
-    + g(1-g)(y-x) c^T with c as in mala_jac_diag_batched (the matrix whose
-    columns are the JVPs of src/MALA/MALA.jl:749-795 along basis vectors).
+def mala_jac_full_batched(target, X, eps, Xi, U, sigmoid=True):
+    """Dense Jacobian per column.  With ``sigmoid`` the surrogate's
+    J_t = a(I + eps H_x) + (1-a) I + g(1-g)(y-x) c^T with c as in
+    mala_jac_diag_batched (the matrix whose columns are the JVPs of
+    src/MALA/MALA.jl:749-795 along basis vectors); without it the derivative
+    of the taped step with the accept decision held fixed,
+    J_t = a(I + eps H_x) + (1-a) I (exact wherever no decision flips).
     No reference counterpart: the reference rejects jacobian=:full
     (src/DEER/DEER.jl:464-465).  Returns FT, J (T x D x D), acc, logalpha."""
     D, T = X.shape
@@ -411,6 +414,8 @@ def mala_jac_full_batched(target, X, eps, Xi, U):
     PG = GY + eps * target.hvp_batch(X, GY)
     Cm = PG - GX - (R - PQ) / (2 * eps)
     Um = (g * (1 - g))[None, :] * (Y - X)
+    if not sigmoid:
+        Um = np.zeros_like(Um)
     J = np.empty((T, D, D))
     I = np.eye(D)
     for t in range(T):
@@ -616,9 +621,10 @@ def deer_update_rec(fwd_and_jvp_batch, s0, S_in, jacobian="stoch_diag", damping=
     ``fwd_and_jvp_batch(Xbar, Z) -> (FT, Jt, acc, logalpha)``.
     jacobian='diag' needs ``jac_diag_batch(Xbar) -> (FT, diagJ, acc, logalpha)``
     (the reference's scalar branch :456-462 computes the same diagonal with D
-    JVPs per step).  jacobian='full' (no reference counterpart) needs
-    ``jac_full_batch(Xbar) -> (FT, J, acc, logalpha)`` and solves the dense
-    recurrence S_t = J_t S_{t-1} + FT_t - J_t Xbar_t sequentially.
+    JVPs per step).  jacobian='full' / 'full_sigmoid' (no reference
+    counterpart) need ``jac_full_batch(Xbar, sigmoid) -> (FT, J, acc,
+    logalpha)`` (decision-frozen / surrogate dense Jacobian) and solve the
+    dense recurrence S_t = J_t S_{t-1} + FT_t - J_t Xbar_t sequentially.
     Returns (S_out, accepted, logalpha)."""
     D, T = S_in.shape
     if len(s0) != D:
@@ -640,8 +646,8 @@ def deer_update_rec(fwd_and_jvp_batch, s0, S_in, jacobian="stoch_diag", damping=
             A *= 1.0 / probes
     elif jacobian == "diag":
         FT, A, acc, la = jac_diag_batch(Xbar)
-    elif jacobian == "full" and jac_full_batch is not None:
-        FT, J, acc, la = jac_full_batch(Xbar)
+    elif jacobian in ("full", "full_sigmoid") and jac_full_batch is not None:
+        FT, J, acc, la = jac_full_batch(Xbar, jacobian == "full_sigmoid")
         B = FT - np.einsum("tij,jt->it", J, Xbar)
         S_out = solve_affine_seq_full(J, B, np.asarray(s0, dtype=np.float64))
         if damping != 1.0:
@@ -667,8 +673,8 @@ def mala_rec(target, eps, Xi, U):
     def jdiag(Xbar):
         return mala_jac_diag_batched(target, Xbar, eps, Xi, U)
 
-    def jfull(Xbar):
-        return mala_jac_full_batched(target, Xbar, eps, Xi, U)
+    def jfull(Xbar, sigmoid=False):
+        return mala_jac_full_batched(target, Xbar, eps, Xi, U, sigmoid)
 
     fwd.jac_full = jfull
     return fwd, jdiag

@@ -90,7 +90,11 @@ def _jac_code(jacobian):
         return _lib.JAC_STOCH_DIAG, j
     if j == "diag":
         return _lib.JAC_DIAG, j
-    raise ArgumentError("jacobian must be :diag or :stoch_diag")
+    if j == "full":
+        return _lib.JAC_FULL, j
+    if j == "full_sigmoid":
+        return _lib.JAC_FULL_SIGMOID, j
+    raise ArgumentError("jacobian must be :diag, :stoch_diag, :full or :full_sigmoid")
 
 
 def solve(rec: TapedRecursion, s0, init=None, tol_abs=1e-4, tol_rel=1e-3, maxiter=200,
@@ -112,6 +116,13 @@ def solve(rec: TapedRecursion, s0, init=None, tol_abs=1e-4, tol_rel=1e-3, maxite
         raise ArgumentError("probes must be >= 1")
     jcode, jname = _jac_code(jacobian)
     D, T = rec.D, rec.T
+    if jname in ("full", "full_sigmoid"):
+        from .targets import GaussianTarget
+
+        if not isinstance(rec.target, GaussianTarget):
+            raise ArgumentError("jacobian=:full needs a GaussianTarget (dense precision matrix)")
+        if D > 64:
+            raise ArgumentError("jacobian=:full supports D <= 64")
     if len(s0) != D:
         raise ArgumentError("S must be D x T with D = length(s0)")
     if copy_result is None:

@@ -34,6 +34,8 @@ KT_NAMES = ("glm_pass", "tile_reduce", "tile_apply", "glm_reduce", "scan_spine",
 
 JAC_STOCH_DIAG = 0
 JAC_DIAG = 1
+JAC_FULL = 2  # dense decision-frozen Jacobian; Gaussian targets with D <= 64
+JAC_FULL_SIGMOID = 3  # dense surrogate (sigmoid) Jacobian
 
 
 class ArgumentError(ValueError):

@@ -98,8 +98,8 @@ class ParallelMALASampler:
         if not self.epsilon > 0:
             raise ArgumentError(f"epsilon must be > 0, got {self.epsilon}")
         j = str(self.jacobian).lstrip(":")
-        if j not in ("stoch_diag", "diag"):
-            raise ArgumentError("jacobian must be :stoch_diag or :diag")
+        if j not in ("stoch_diag", "diag", "full", "full_sigmoid"):
+            raise ArgumentError("jacobian must be :stoch_diag, :diag, :full or :full_sigmoid")
         self.jacobian = j
         if self.cholM is not None:
             raise NotImplementedError("mass-matrix preconditioning (cholM) is not implemented on the GPU path")

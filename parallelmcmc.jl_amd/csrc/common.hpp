@@ -126,6 +126,10 @@ struct Prof {
     }
 };
 
+// max that propagates NaN from either operand (fmax drops it): a NaN state
+// must fail the convergence test, as maximum() does in src/DEER/DEER.jl:567-571
+__host__ __device__ inline double nan_max(double a, double b) { return (a > b || a != a) ? a : b; }
+
 template <typename T>
 struct DevTraits;
 template <>

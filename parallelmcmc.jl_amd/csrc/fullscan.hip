@@ -1,0 +1,622 @@
+// Full-Jacobian DEER for the dense Gaussian target N(mean, Lambda^-1), D <= 64.
+//
+// For H = -Lambda the Jacobian of the surrogate MALA step (the matrix whose
+// products are the JVPs of src/MALA/MALA.jl:749-795) is structured:
+//   J_t = I - a_t eps Lambda + u_t c_t^T,
+//   a_t = accept flag, u_t = g(1-g)(y - x)  (u_t = 0 for the decision-frozen
+//   Jacobian, the derivative of the taped step away from decision flips),
+//   c_t = B g_y - g_x - (r - B^2 r)/(2 eps),  B = I - eps Lambda,  r = x - y - eps g_y
+//       = g_y - eps Lambda g_y - g_x - Lambda r + (eps/2) Lambda (Lambda r).
+// The Newton update is solved in correction form: with rho_t = FT_t - S_t,
+//   Delta_t = J_t Delta_{t-1} + rho_t,  Delta_{-1} = 0,  S_new = S + damping Delta,
+// which equals the reference's S_new_t = J_t S_new_{t-1} + FT_t - J_t Xbar_t
+// (src/DEER/DEER.jl:409-419, 470-473) in exact arithmetic and keeps the fixed
+// point exact when the Jacobian products are rounded (BF16 MFMA): at the
+// fixed point rho = 0 and so Delta = 0 for any J.
+//
+// Kernels of one iteration:
+//   full_prep_kernel   per column: log alpha (FP64), accept flag, u, c, rho
+//   full_reduce_*      per tile of TC columns: M = J_e ... J_b (64 x 64) and
+//                      r = the tile's Delta from a zero start.  M is held in
+//                      registers (fp32 master); Lambda M runs on MFMA with
+//                      Lambda's operand fragments in registers and M's from an
+//                      LDS copy (FP32 16x16x4 or BF16 16x16x32; FP64 on VALU)
+//   full_spine_kernel  one workgroup: Delta at each tile end,
+//                      Delta_k = M_k Delta_{k-1} + r_k
+//   full_apply_kernel  one wave per tile: Delta through the tile from the
+//                      incoming Delta, S_new, convergence maxima, metric
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "../../include/pmcmc_amd.h"
+#include "fullscan.hpp"
+
+namespace pmcmc {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+
+static constexpr int FD = FULL_MAX_D;  // padded dimension
+static constexpr int FS = FD + 4;      // fp32 LDS column stride
+static constexpr int FSB = FD + 8;     // bf16 LDS column stride
+
+template <typename T>
+struct FullArgs {
+    const T* S_in;
+    const T* s0;
+    const T* U;
+    const T* Y;
+    const T* gx;
+    const T* gy;
+    const double* lpx;
+    const double* lpy;
+    const float* L32;   // Dp x Dp row-major, zero padded
+    const double* L64;  // D x D column-major
+    int64_t D, Dp, Tn;
+    int TC, nT;
+    double eps, damping, tol_abs, tol_rel;
+    int sigmoid;
+    T* a;
+    T* u;
+    T* c;
+    T* rho;
+    T* Magg;
+    T* ragg;
+    T* dend;
+    T* S_out;
+    uint8_t* acc_out;
+    SyncState* sy;
+    int iter;
+};
+
+__device__ inline float rlane(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+__device__ inline double rlane(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Lambda[r][k] with zero padding beyond D
+template <typename T>
+__device__ inline T lam(const FullArgs<T>& p, int r, int k) {
+    if (r >= p.D || k >= p.D) return (T)0;
+    if constexpr (sizeof(T) == 8)
+        return p.L64[r + (int64_t)k * p.D];
+    else
+        return p.L32[(int64_t)r * p.Dp + k];
+}
+
+__device__ inline bool converged(const SyncState* sy) { return *(volatile const int*)&sy->converged != 0; }
+
+// ---------------------------------------------------------------------------
+// prep: one wave per column (grid-stride over columns), lane d = row d;
+// lane d keeps Lambda's row d in registers.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void full_prep_kernel(FullArgs<T> p) {
+    if (converged(p.sy)) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t D = p.D;
+    T lrow[FD];
+#pragma unroll
+    for (int k = 0; k < FD; ++k) lrow[k] = lam<T>(p, lane, k);
+    const bool in = lane < D;
+    const double eps = p.eps;
+    const double c_norm = ((double)D / 2.0) * log(4.0 * 3.14159265358979323846 * eps);
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < p.Tn; t += nw) {
+        const size_t gi = (size_t)t * D + lane;
+        T xv = 0, yv = 0, g0 = 0, g1 = 0, sv = 0;
+        if (in) {
+            xv = t == 0 ? p.s0[lane] : p.S_in[gi - D];
+            yv = p.Y[gi];
+            g0 = p.gx[gi];
+            g1 = p.gy[gi];
+            sv = p.S_in[gi];
+        }
+        const double e1 = (double)yv - (double)xv - eps * (double)g0;
+        const double r = (double)xv - (double)yv - eps * (double)g1;
+        double qyx = e1 * e1, qxy = r * r;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            qyx += __shfl_xor(qyx, m, 64);
+            qxy += __shfl_xor(qxy, m, 64);
+        }
+        // log q(x|y) and log q(y|x), src/MALA/MALA.jl:172-193
+        const double lq_xy = -0.5 * qxy / (2.0 * eps) - c_norm;
+        const double lq_yx = -0.5 * qyx / (2.0 * eps) - c_norm;
+        const double la = (p.lpy[t] + lq_xy) - (p.lpx[t] + lq_yx);
+        const double lu = log((double)p.U[t]);
+        const bool acc = lu < la;
+        const double g = 1.0 / (1.0 + exp(-(la - lu)));
+        T cv = 0, uv = 0;  // the rank-1 term is absent from the decision-frozen Jacobian
+        if (p.sigmoid) {
+            const T rT = (T)r;
+            T Lg = 0, Lr = 0;
+#pragma unroll
+            for (int k = 0; k < FD; ++k) {
+                Lg += lrow[k] * rlane(g1, k);
+                Lr += lrow[k] * rlane(rT, k);
+            }
+            T LLr = 0;
+#pragma unroll
+            for (int k = 0; k < FD; ++k) LLr += lrow[k] * rlane(Lr, k);
+            const T epsT = (T)eps;
+            cv = g1 - epsT * Lg - g0 - Lr + (T)0.5 * epsT * LLr;
+            uv = (T)(g * (1.0 - g)) * (yv - xv);
+        }
+        const T ft = acc ? yv : xv;
+        if (in) {
+            p.u[gi] = uv;
+            p.c[gi] = cv;
+            p.rho[gi] = ft - sv;
+        }
+        if (lane == 0) {
+            p.a[t] = acc ? (T)1 : (T)0;
+            if (p.acc_out) p.acc_out[t] = acc ? 1 : 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reduce, MFMA paths (T = float).  256 threads; thread (wave w, lane = li + 16 lk)
+// owns M[i][j] for i = 16w + 4lk + r (r < 4), j = 16jt + li (jt < 4): the
+// C-fragment layout of a 16x16 MFMA tile, so Lambda M lands on the entries the
+// thread already holds.  Wave 0 also carries the tile's Delta (lane i = row i).
+// ---------------------------------------------------------------------------
+template <int MM>  // 0: FP32 MFMA 16x16x4, 1: BF16 MFMA 16x16x32
+__global__ __launch_bounds__(256) void full_reduce_mfma_kernel(FullArgs<float> p) {
+    __shared__ __attribute__((aligned(16))) float Mf[MM == 0 ? FD * FS : 4];
+    __shared__ __attribute__((aligned(16))) __bf16 Mb[MM == 1 ? FD * FSB : 8];
+    __shared__ float part[4][FD];
+    if (converged(p.sy)) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t D = p.D;
+    const int64_t t0 = (int64_t)blockIdx.x * p.TC;
+    const int tc = (int)((p.Tn - t0) < p.TC ? (p.Tn - t0) : p.TC);
+    const float epsf = (float)p.eps;
+
+    // Lambda's A fragments (row tile = wave), constant over the tile
+    float af[FD / 4];
+    v8bf ab[FD / 32];
+    if (MM == 0) {
+#pragma unroll
+        for (int ks = 0; ks < FD / 4; ++ks) af[ks] = lam<float>(p, 16 * wave + li, 4 * ks + lk);
+    } else {
+#pragma unroll
+        for (int ks = 0; ks < FD / 32; ++ks)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) ab[ks][e] = (__bf16)lam<float>(p, 16 * wave + li, 32 * ks + 8 * lk + e);
+    }
+    // wave 0: Lambda row `lane` for Lambda Delta
+    float lrow[FD];
+    if (wave == 0) {
+#pragma unroll
+        for (int k = 0; k < FD; ++k) lrow[k] = lam<float>(p, lane, k);
+    }
+    // M = I in registers
+    const int ib = 16 * wave + 4 * lk;  // first owned row
+    float m[4][4];
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) m[jt][r] = (ib + r == 16 * jt + li) ? 1.f : 0.f;
+    float rv = 0.f;
+
+    auto put_copy = [&]() {
+#pragma unroll
+        for (int jt = 0; jt < 4; ++jt) {
+            const int j = 16 * jt + li;
+            if (MM == 0) {
+                *reinterpret_cast<float4*>(&Mf[j * FS + ib]) = make_float4(m[jt][0], m[jt][1], m[jt][2], m[jt][3]);
+            } else {
+                v4bf b;
+                b[0] = (__bf16)m[jt][0];
+                b[1] = (__bf16)m[jt][1];
+                b[2] = (__bf16)m[jt][2];
+                b[3] = (__bf16)m[jt][3];
+                *reinterpret_cast<v4bf*>(&Mb[j * FSB + ib]) = b;
+            }
+        }
+    };
+    put_copy();
+
+    // per-step operands: c and u at the owned rows; wave 0: c, u, rho at row `lane`
+    auto ld = [&](const float* v, int64_t t, int i) -> float { return (i < D) ? v[(size_t)t * D + i] : 0.f; };
+    float cn[4], un[4], c0n = 0.f, u0n = 0.f, r0n = 0.f, an = 0.f;
+    auto fetch = [&](int64_t t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cn[r] = ld(p.c, t, ib + r);
+            un[r] = ld(p.u, t, ib + r);
+        }
+        if (wave == 0) {
+            c0n = ld(p.c, t, lane);
+            u0n = ld(p.u, t, lane);
+            r0n = ld(p.rho, t, lane);
+        }
+        an = p.a[t];
+    };
+    if (tc > 0) fetch(t0);
+
+    for (int s = 0; s < tc; ++s) {
+        float cc[4], uu[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cc[r] = cn[r];
+            uu[r] = un[r];
+        }
+        const float c0 = c0n, u0 = u0n, r0 = r0n, a = an;
+        if (s + 1 < tc) fetch(t0 + s + 1);
+        __syncthreads();  // the LDS copy of M is complete
+        v4f acc[4];
+#pragma unroll
+        for (int jt = 0; jt < 4; ++jt) acc[jt] = (v4f){0.f, 0.f, 0.f, 0.f};
+        if (MM == 0) {
+#pragma unroll
+            for (int ks = 0; ks < FD / 4; ++ks)
+#pragma unroll
+                for (int jt = 0; jt < 4; ++jt)
+                    acc[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[ks], Mf[(16 * jt + li) * FS + 4 * ks + lk],
+                                                                   acc[jt], 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < FD / 32; ++ks)
+#pragma unroll
+                for (int jt = 0; jt < 4; ++jt) {
+                    const v8bf bv = *reinterpret_cast<const v8bf*>(&Mb[(16 * jt + li) * FSB + 32 * ks + 8 * lk]);
+                    acc[jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab[ks], bv, acc[jt], 0, 0, 0);
+                }
+        }
+        // w_j = sum_i c_i M_ij: rows of this thread, then lanes lk, then waves
+#pragma unroll
+        for (int jt = 0; jt < 4; ++jt) {
+            float wp = cc[0] * m[jt][0] + cc[1] * m[jt][1] + cc[2] * m[jt][2] + cc[3] * m[jt][3];
+            wp += __shfl_xor(wp, 16, 64);
+            wp += __shfl_xor(wp, 32, 64);
+            if (lk == 0) part[wave][16 * jt + li] = wp;
+        }
+        // wave 0: Delta <- Delta - a eps Lambda Delta + u (c . Delta) + rho
+        if (wave == 0) {
+            float Lr = 0.f;
+#pragma unroll
+            for (int k = 0; k < FD; ++k) Lr += lrow[k] * rlane(rv, k);
+            float cr = c0 * rv;
+#pragma unroll
+            for (int mm = 32; mm >= 1; mm >>= 1) cr += __shfl_xor(cr, mm, 64);
+            rv = rv - a * epsf * Lr + u0 * cr + r0;
+        }
+        __syncthreads();  // partial sums visible; all MFMA reads of the copy done
+        const float ae = a * epsf;
+#pragma unroll
+        for (int jt = 0; jt < 4; ++jt) {
+            const int j = 16 * jt + li;
+            const float wj = part[0][j] + part[1][j] + part[2][j] + part[3][j];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) m[jt][r] = m[jt][r] - ae * acc[jt][r] + uu[r] * wj;
+        }
+        put_copy();
+    }
+    // tile aggregates: M column-major 64 x 64, Delta
+    float* Mo = p.Magg + (size_t)blockIdx.x * FD * FD;
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt) {
+        const int j = 16 * jt + li;
+        *reinterpret_cast<float4*>(&Mo[j * FD + ib]) = make_float4(m[jt][0], m[jt][1], m[jt][2], m[jt][3]);
+    }
+    if (wave == 0) p.ragg[(size_t)blockIdx.x * FD + lane] = rv;
+}
+
+// ---------------------------------------------------------------------------
+// reduce, FP64 on VALU.  Thread (wave w, lane j) owns M[16w + ii][j], ii < 16;
+// the LDS copy is row-major so that lane j reads column j conflict-free.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void full_reduce_f64_kernel(FullArgs<double> p) {
+    __shared__ double Mr[FD * FD];
+    __shared__ double part[4][FD];
+    __shared__ double uvec[FD];
+    if (converged(p.sy)) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int64_t D = p.D;
+    const int64_t t0 = (int64_t)blockIdx.x * p.TC;
+    const int tc = (int)((p.Tn - t0) < p.TC ? (p.Tn - t0) : p.TC);
+    const double eps = p.eps;
+    double lrow[FD];
+    if (wave == 0) {
+#pragma unroll
+        for (int k = 0; k < FD; ++k) lrow[k] = lam<double>(p, lane, k);
+    }
+    double m[16];
+#pragma unroll
+    for (int ii = 0; ii < 16; ++ii) {
+        m[ii] = (16 * wave + ii == lane) ? 1.0 : 0.0;
+        Mr[(16 * wave + ii) * FD + lane] = m[ii];
+    }
+    double rv = 0.0;
+    const bool in = lane < D;
+    for (int s = 0; s < tc; ++s) {
+        const int64_t t = t0 + s;
+        const double a = p.a[t];
+        const double c0 = in ? p.c[(size_t)t * D + lane] : 0.0;
+        const double u0 = in ? p.u[(size_t)t * D + lane] : 0.0;
+        const double r0 = in ? p.rho[(size_t)t * D + lane] : 0.0;
+        if (wave == 0) uvec[lane] = u0;
+        __syncthreads();  // copy of M and u complete
+        double acc[16];
+#pragma unroll
+        for (int ii = 0; ii < 16; ++ii) acc[ii] = 0.0;
+        for (int k = 0; k < D; ++k) {
+            const double mk = Mr[k * FD + lane];
+#pragma unroll
+            for (int ii = 0; ii < 16; ++ii) {
+                const int i = 16 * wave + ii;
+                const double l = i < D ? p.L64[i + (int64_t)k * D] : 0.0;
+                acc[ii] += l * mk;
+            }
+        }
+        // w_j partial over the owned rows: c_i from lane i via readlane
+        double wp = 0.0;
+#pragma unroll
+        for (int ii = 0; ii < 16; ++ii) wp += rlane(c0, 16 * wave + ii) * m[ii];
+        part[wave][lane] = wp;
+        if (wave == 0) {
+            double Lr = 0.0;
+#pragma unroll
+            for (int k = 0; k < FD; ++k) Lr += lrow[k] * rlane(rv, k);
+            double cr = c0 * rv;
+#pragma unroll
+            for (int mm = 32; mm >= 1; mm >>= 1) cr += __shfl_xor(cr, mm, 64);
+            rv = rv - a * eps * Lr + u0 * cr + r0;
+        }
+        __syncthreads();  // partials visible; all reads of the copy done
+        const double wj = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+        const double ae = a * eps;
+#pragma unroll
+        for (int ii = 0; ii < 16; ++ii) {
+            m[ii] = m[ii] - ae * acc[ii] + uvec[16 * wave + ii] * wj;
+            Mr[(16 * wave + ii) * FD + lane] = m[ii];
+        }
+        __syncthreads();  // uvec is rewritten next step
+    }
+    double* Mo = p.Magg + (size_t)blockIdx.x * FD * FD;
+#pragma unroll
+    for (int ii = 0; ii < 16; ++ii) Mo[lane * FD + 16 * wave + ii] = m[ii];
+    if (wave == 0) p.ragg[(size_t)blockIdx.x * FD + lane] = rv;
+}
+
+// ---------------------------------------------------------------------------
+// spine: one workgroup, Delta_k = M_k Delta_{k-1} + r_k over the tiles.  Thread
+// (quarter q, lane i) sums columns 16q..16q+15 of row i; the next tile's
+// entries are loaded while the current one is applied.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void full_spine_kernel(FullArgs<T> p) {
+    __shared__ T v[FD];
+    __shared__ T part[4][FD];
+    if (converged(p.sy)) return;
+    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
+    if (tid < FD) v[tid] = 0;
+    T mc[16], mn[16];
+    T rn = 0;
+    auto fetch = [&](int k) {
+        const T* M = p.Magg + (size_t)k * FD * FD;
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) mn[jj] = M[(size_t)(16 * q + jj) * FD + lane];
+        if (q == 0) rn = p.ragg[(size_t)k * FD + lane];
+    };
+    fetch(0);
+    __syncthreads();
+    for (int k = 0; k < p.nT; ++k) {
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) mc[jj] = mn[jj];
+        const T rc = rn;
+        if (k + 1 < p.nT) fetch(k + 1);
+        T s = 0;
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) s += mc[jj] * v[16 * q + jj];
+        part[q][lane] = s;
+        __syncthreads();
+        if (q == 0) {
+            const T nv = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane] + rc;
+            v[lane] = nv;
+            p.dend[(size_t)k * FD + lane] = nv;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// apply: one wave per tile; lane d = row d with Lambda's row d in registers.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(64) void full_apply_kernel(FullArgs<T> p) {
+    if (converged(p.sy)) return;
+    const int lane = threadIdx.x;
+    const int64_t D = p.D;
+    const int64_t tile = blockIdx.x;
+    const int64_t t0 = tile * p.TC;
+    const int tc = (int)((p.Tn - t0) < p.TC ? (p.Tn - t0) : p.TC);
+    T lrow[FD];
+#pragma unroll
+    for (int k = 0; k < FD; ++k) lrow[k] = lam<T>(p, lane, k);
+    const bool in = lane < D;
+    const T epsT = (T)p.eps, damp = (T)p.damping;
+    T dv = tile == 0 ? (T)0 : p.dend[(size_t)(tile - 1) * FD + lane];
+    double dmax = 0.0, smax = 0.0;
+    T un = 0, cn = 0, rn = 0, sn = 0, an = 0;
+    auto fetch = [&](int64_t t) {
+        const size_t gi = (size_t)t * D + lane;
+        if (in) {
+            un = p.u[gi];
+            cn = p.c[gi];
+            rn = p.rho[gi];
+            sn = p.S_in[gi];
+        }
+        an = p.a[t];
+    };
+    if (tc > 0) fetch(t0);
+    for (int s = 0; s < tc; ++s) {
+        const T u = un, c = cn, rho = rn, old = sn, a = an;
+        if (s + 1 < tc) fetch(t0 + s + 1);
+        T Ld = 0;
+#pragma unroll
+        for (int k = 0; k < FD; ++k) Ld += lrow[k] * rlane(dv, k);
+        T cd = c * dv;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) cd += __shfl_xor(cd, m, 64);
+        dv = dv - a * epsT * Ld + u * cd + rho;
+        if (in) {
+            const T nv = old + damp * dv;
+            p.S_out[(size_t)(t0 + s) * D + lane] = nv;
+            dmax = nan_max(dmax, fabs((double)nv - (double)old));
+            smax = nan_max(smax, fabs((double)nv));
+        }
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        dmax = nan_max(dmax, __shfl_xor(dmax, m, 64));
+        smax = nan_max(smax, __shfl_xor(smax, m, 64));
+    }
+    if (lane == 0) {
+        SyncState* sy = p.sy;
+        atomicMax(&sy->dmax_bits, (unsigned long long)__double_as_longlong(dmax));
+        atomicMax(&sy->smax_bits, (unsigned long long)__double_as_longlong(smax));
+        const unsigned prev = __hip_atomic_fetch_add(&sy->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == (unsigned)p.nT - 1u) {
+            const double dm = __longlong_as_double((long long)atomicAdd(&sy->dmax_bits, 0ull));
+            const double sm = __longlong_as_double((long long)atomicAdd(&sy->smax_bits, 0ull));
+            // src/DEER/DEER.jl:567-577
+            const double metric = dm / (p.tol_abs + p.tol_rel * sm);
+            sy->last_metric = metric;
+            sy->last_iter = p.iter;
+            if (metric <= 1.0) {
+                sy->conv_iter = p.iter;
+                sy->converged = 1;
+            }
+            sy->dmax_bits = 0ull;
+            sy->smax_bits = 0ull;
+            sy->done = 0u;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+int full_work_ensure(FullWork* w, int64_t D, int64_t T, bool f64, std::string* msg) {
+    if (D > FD) {
+        *msg = "jacobian=:full supports D <= 64";
+        return PMCMC_ERR_UNSUPPORTED;
+    }
+    const size_t esz = f64 ? 8 : 4;
+    if (w->a && w->T == T && w->D == D && w->esz == esz) return PMCMC_OK;
+    full_work_destroy(w);
+    w->D = D;
+    w->T = T;
+    w->esz = esz;
+    w->TC = 64;
+    w->nT = (int)((T + w->TC - 1) / w->TC);
+    const size_t mat = (size_t)D * T * esz;
+    bool ok = hipMalloc(&w->a, T * esz) == hipSuccess && hipMalloc(&w->u, mat) == hipSuccess &&
+              hipMalloc(&w->c, mat) == hipSuccess && hipMalloc(&w->rho, mat) == hipSuccess &&
+              hipMalloc(&w->Magg, (size_t)w->nT * FD * FD * esz) == hipSuccess &&
+              hipMalloc(&w->ragg, (size_t)w->nT * FD * esz) == hipSuccess &&
+              hipMalloc(&w->dend, (size_t)w->nT * FD * esz) == hipSuccess;
+    if (!ok) {
+        full_work_destroy(w);
+        *msg = "full-Jacobian workspace allocation failed";
+        return PMCMC_ERR_ALLOC;
+    }
+    return PMCMC_OK;
+}
+
+void full_work_destroy(FullWork* w) {
+    (void)hipFree(w->a);
+    (void)hipFree(w->u);
+    (void)hipFree(w->c);
+    (void)hipFree(w->rho);
+    (void)hipFree(w->Magg);
+    (void)hipFree(w->ragg);
+    (void)hipFree(w->dend);
+    *w = FullWork{};
+}
+
+template <typename T>
+int full_newton_step(FullWork* fw, const GlmWork* gw, const GaussData* g, hipStream_t s, SyncState* sync,
+                     int precision, bool sigmoid, double eps, double damping, double tol_abs, double tol_rel, const T* x0,
+                     const T* S_in, T* S_out, const T* U, uint8_t* acc_out, int iter, Prof* prof,
+                     std::string* msg) {
+    FullArgs<T> p{};
+    p.S_in = S_in;
+    p.s0 = x0;
+    p.U = U;
+    p.Y = (const T*)gw->Y;
+    p.gx = (const T*)gw->gx;
+    p.gy = (const T*)gw->gy;
+    p.lpx = gw->lpx;
+    p.lpy = gw->lpy;
+    p.L32 = g->L32;
+    p.L64 = g->L64;
+    p.D = fw->D;
+    p.Dp = g->Dp;
+    p.Tn = fw->T;
+    p.TC = fw->TC;
+    p.nT = fw->nT;
+    p.eps = eps;
+    p.damping = damping;
+    p.tol_abs = tol_abs;
+    p.tol_rel = tol_rel;
+    p.sigmoid = sigmoid ? 1 : 0;
+    p.a = (T*)fw->a;
+    p.u = (T*)fw->u;
+    p.c = (T*)fw->c;
+    p.rho = (T*)fw->rho;
+    p.Magg = (T*)fw->Magg;
+    p.ragg = (T*)fw->ragg;
+    p.dend = (T*)fw->dend;
+    p.S_out = S_out;
+    p.acc_out = acc_out;
+    p.sy = sync;
+    p.iter = iter;
+    int64_t pg = (fw->T + 3) / 4;
+    if (pg > 2048) pg = 2048;
+    int e = prof ? prof->begin(KT_TILE_REDUCE, iter, s) : -1;
+    hipLaunchKernelGGL((full_prep_kernel<T>), dim3((unsigned)pg), dim3(256), 0, s, p);
+    if constexpr (sizeof(T) == 8) {
+        hipLaunchKernelGGL(full_reduce_f64_kernel, dim3((unsigned)fw->nT), dim3(256), 0, s, p);
+    } else {
+        if (precision == PMCMC_PREC_BF16_JAC)
+            hipLaunchKernelGGL((full_reduce_mfma_kernel<1>), dim3((unsigned)fw->nT), dim3(256), 0, s, p);
+        else
+            hipLaunchKernelGGL((full_reduce_mfma_kernel<0>), dim3((unsigned)fw->nT), dim3(256), 0, s, p);
+    }
+    if (prof) prof->end(e, s);
+    e = prof ? prof->begin(KT_SCAN_SPINE, iter, s) : -1;
+    hipLaunchKernelGGL((full_spine_kernel<T>), dim3(1), dim3(256), 0, s, p);
+    if (prof) prof->end(e, s);
+    e = prof ? prof->begin(KT_TILE_APPLY, iter, s) : -1;
+    hipLaunchKernelGGL((full_apply_kernel<T>), dim3((unsigned)fw->nT), dim3(64), 0, s, p);
+    if (prof) prof->end(e, s);
+    if (hipGetLastError() != hipSuccess) {
+        *msg = "full-Jacobian kernels failed to launch";
+        return PMCMC_ERR_HIP;
+    }
+    return PMCMC_OK;
+}
+
+template int full_newton_step<float>(FullWork*, const GlmWork*, const GaussData*, hipStream_t, SyncState*, int,
+                                     bool, double, double, double, double, const float*, const float*, float*,
+                                     const float*, uint8_t*, int, Prof*, std::string*);
+template int full_newton_step<double>(FullWork*, const GlmWork*, const GaussData*, hipStream_t, SyncState*, int,
+                                      bool, double, double, double, double, const double*, const double*, double*,
+                                      const double*, uint8_t*, int, Prof*, std::string*);
+
+}  // namespace pmcmc

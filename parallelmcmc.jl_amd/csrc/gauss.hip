@@ -87,11 +87,12 @@ static constexpr int GKC = 32;  // K slice
 //   pass 1: gy = -P0, hw_k = -P_k, lpy
 
 // ---------------------------------------------------------------------------
-// FP32 MFMA pass (probes K = 1).  grid: ceil(T/GCB) blocks of 256 threads.
-// Wave w owns output column tile w (tiles 0,1: states; 2,3: directions) for
-// every 16-row tile of Lambda's rows.  JBF: the direction tiles on BF16 MFMA.
+// FP32 MFMA pass over 64 columns per block: KS = 1 probe set (32 states and
+// their 32 directions) or KS = 0 (64 states, no directions).  Wave w owns
+// output column tile w for every 16-row tile of Lambda's rows.  JBF: the
+// direction tiles (KS = 1, waves 2 and 3) on BF16 MFMA.
 // ---------------------------------------------------------------------------
-template <int DPT, bool JBF>
+template <int DPT, int KS, bool JBF>
 __global__ __launch_bounds__(256) void gauss_pass_mfma_kernel(ColSrc<float> src, const float* L32,
                                                               const float* mu, int pass, float eps,
                                                               const float* Xi, float* gout, float* hout,
@@ -99,6 +100,7 @@ __global__ __launch_bounds__(256) void gauss_pass_mfma_kernel(ColSrc<float> src,
                                                               const int* skip) {
     constexpr int Dp = DPT * 16;
     constexpr int NC = 2 * GCB;
+    constexpr int CPB = NC / (1 + KS);  // trajectory columns per block
     constexpr int SL = GKC + 4;   // Lambda slice [d][k]
     constexpr int SVc = GKC + 4;  // column slice [j][k]
     __shared__ float Ls[Dp * SL];
@@ -130,8 +132,8 @@ __global__ __launch_bounds__(256) void gauss_pass_mfma_kernel(ColSrc<float> src,
         }
         for (int i = tid; i < NC * GKC; i += 256) {
             const int j = i / GKC, k = i % GKC;
-            const int set = j / GCB;
-            const int64_t t = cb * GCB + (j % GCB);
+            const int set = j / CPB;
+            const int64_t t = cb * CPB + (j % CPB);
             const int64_t kk = k0 + k;
             float v = 0.f;
             if (t < Tn && kk < D) {
@@ -142,7 +144,7 @@ __global__ __launch_bounds__(256) void gauss_pass_mfma_kernel(ColSrc<float> src,
         }
         __syncthreads();
         const int ct = wave;  // output column tile
-        if (!JBF || ct < 2) {
+        if (!JBF || KS == 0 || ct < 2) {
 #pragma unroll
             for (int ks = 0; ks < GKC / 4; ++ks) {
                 const float bv = Vc[(16 * ct + li) * SVc + 4 * ks + lk];
@@ -168,8 +170,8 @@ __global__ __launch_bounds__(256) void gauss_pass_mfma_kernel(ColSrc<float> src,
 
     // epilogue: lane holds P[d = 16 dt + 4 lk + r][column 16 wave + li]
     const int j = 16 * wave + li;
-    const int set = j / GCB;
-    const int64_t t = cb * GCB + (j % GCB);
+    const int set = j / CPB;
+    const int64_t t = cb * CPB + (j % CPB);
     const bool tv = t < Tn;
     const double sq2e = sqrt(2.0 * (double)eps);
     double lp = 0.0;
@@ -198,7 +200,7 @@ __global__ __launch_bounds__(256) void gauss_pass_mfma_kernel(ColSrc<float> src,
     }
     lp += __shfl_xor(lp, 16, 64);
     lp += __shfl_xor(lp, 32, 64);
-    if (wave < 2 && lk == 0 && tv) lpout[t] = lp;
+    if (set == 0 && lk == 0 && tv) lpout[t] = lp;
 }
 
 // ---------------------------------------------------------------------------
@@ -259,13 +261,19 @@ template <int DPT>
 static void launch_gauss_mfma(const ColSrc<float>& src, const GaussData* g, int pass, double eps,
                               const float* Xi, float* gout, float* hout, double* lpout, float* Yout,
                               float* Wout, const int* skip, bool jbf, hipStream_t s) {
+    if (src.K == 0) {
+        const dim3 grid((unsigned)((src.Tn + 2 * GCB - 1) / (2 * GCB)));
+        hipLaunchKernelGGL((gauss_pass_mfma_kernel<DPT, 0, false>), grid, dim3(256), 0, s, src, g->L32, g->mu32,
+                           pass, (float)eps, Xi, gout, hout, lpout, Yout, Wout, skip);
+        return;
+    }
     const dim3 grid((unsigned)((src.Tn + GCB - 1) / GCB));
     if (jbf)
-        hipLaunchKernelGGL((gauss_pass_mfma_kernel<DPT, true>), grid, dim3(256), 0, s, src, g->L32, g->mu32, pass,
-                           (float)eps, Xi, gout, hout, lpout, Yout, Wout, skip);
+        hipLaunchKernelGGL((gauss_pass_mfma_kernel<DPT, 1, true>), grid, dim3(256), 0, s, src, g->L32, g->mu32,
+                           pass, (float)eps, Xi, gout, hout, lpout, Yout, Wout, skip);
     else
-        hipLaunchKernelGGL((gauss_pass_mfma_kernel<DPT, false>), grid, dim3(256), 0, s, src, g->L32, g->mu32, pass,
-                           (float)eps, Xi, gout, hout, lpout, Yout, Wout, skip);
+        hipLaunchKernelGGL((gauss_pass_mfma_kernel<DPT, 1, false>), grid, dim3(256), 0, s, src, g->L32, g->mu32,
+                           pass, (float)eps, Xi, gout, hout, lpout, Yout, Wout, skip);
 }
 
 template <typename T>
@@ -273,7 +281,7 @@ static int gauss_pass(const GaussData* g, hipStream_t s, const ColSrc<T>& src, i
                       const T* Xi, T* gout, T* hout, double* lpout, T* Yout, T* Wout, const int* skip,
                       Prof* prof, int iter, bool jbf, std::string* msg) {
     const int pe = prof ? prof->begin(KT_GLM_PASS, iter, s) : -1;
-    if (sizeof(T) == 4 && src.K == 1 && g->Dp <= 256) {
+    if (sizeof(T) == 4 && src.K <= 1 && g->Dp <= 256) {
         const ColSrc<float>& fs = reinterpret_cast<const ColSrc<float>&>(src);
         float* go = (float*)gout;
         float* ho = (float*)hout;

@@ -15,6 +15,7 @@
 
 #include "../../include/pmcmc_amd.h"
 #include "common.hpp"
+#include "fullscan.hpp"
 #include "gauss.hpp"
 #include "glm.hpp"
 #include "tile_kernel.hpp"
@@ -79,6 +80,7 @@ struct pmcmc_deer_ws {
     void* blob = nullptr;  // scan levels
     // GEMM-target buffers (allocated on first use)
     GlmWork glmw;
+    FullWork fullw;  // jacobian = :full
 };
 
 static int fail(pmcmc_ctx* ctx, int code, const char* fmt, ...) {
@@ -684,6 +686,7 @@ extern "C" int pmcmc_deer_ws_destroy(pmcmc_deer_ws* w) {
     hipFree(w->sync);
     hipFree(w->blob);
     glm_work_destroy(&w->glmw);
+    full_work_destroy(&w->fullw);
     delete w;
     return PMCMC_OK;
 }
@@ -726,15 +729,19 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
                         const void* init, void* S_out, uint8_t* acc_out, pmcmc_deer_info* info) {
     const int64_t D = w->D, Tn = w->T;
     const int mode = tgt->kind == TGT_DIAG ? MODE_ELEM : MODE_GEMM;
+    const bool full = p->jacobian == PMCMC_JAC_FULL || p->jacobian == PMCMC_JAC_FULL_SIGMOID;
+    if (full && tgt->kind != TGT_GAUSS)
+        return fail(c, PMCMC_ERR_UNSUPPORTED, "jacobian=:full needs a dense Gaussian target");
     if (mode == MODE_GEMM) {
-        if (p->jacobian != PMCMC_JAC_STOCH_DIAG)
-            return fail(c, PMCMC_ERR_UNSUPPORTED, "GEMM targets support jacobian=:stoch_diag only");
+        if (!full && p->jacobian != PMCMC_JAC_STOCH_DIAG)
+            return fail(c, PMCMC_ERR_UNSUPPORTED, "GEMM targets support jacobian=:stoch_diag and :full");
         std::string msg;
         GlmData gdummy{};
         gdummy.N = 1;
         gdummy.Dp = tgt->gauss.Dp;
         const GlmData* gd = tgt->kind == TGT_GAUSS ? &gdummy : &tgt->glm;
-        int rc = glm_work_ensure(&w->glmw, gd, c->stream, D, Tn, p->probes, sizeof(T) == 8, &msg);
+        int rc = glm_work_ensure(&w->glmw, gd, c->stream, D, Tn, full ? 0 : p->probes, sizeof(T) == 8, &msg);
+        if (rc == PMCMC_OK && full) rc = full_work_ensure(&w->fullw, D, Tn, sizeof(T) == 8, &msg);
         if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
     }
     T* bufs[2] = {(T*)w->buf[0], (T*)w->buf[1]};
@@ -774,7 +781,17 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
             a.S_in = bufs[it & 1];
             a.S = bufs[(it + 1) & 1];
             const int eit = c->prof.begin(KT_ITERATION, it, c->stream);
-            if (mode == MODE_GEMM) {
+            if (full) {
+                std::string msg;
+                int rc = gauss_newton_products(&w->glmw, &tgt->gauss, c->stream, w->sync, p->precision, p->eps,
+                                               (const T*)x0, a.S_in, (const T*)Xi, p->seed, it, 0, D, Tn,
+                                               &c->prof, &msg);
+                if (rc == PMCMC_OK)
+                    rc = full_newton_step<T>(&w->fullw, &w->glmw, &tgt->gauss, c->stream, w->sync, p->precision,
+                                             p->jacobian == PMCMC_JAC_FULL_SIGMOID, p->eps, p->damping, p->tol_abs, p->tol_rel, (const T*)x0, a.S_in, a.S,
+                                             (const T*)U, acc_out, it, &c->prof, &msg);
+                if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+            } else if (mode == MODE_GEMM) {
                 std::string msg;
                 int rc = tgt->kind == TGT_GAUSS
                              ? gauss_newton_products(&w->glmw, &tgt->gauss, c->stream, w->sync, p->precision,
@@ -832,8 +849,8 @@ extern "C" int pmcmc_deer_solve(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_targ
     if (!(p->damping > 0 && p->damping <= 1)) return fail(c, PMCMC_ERR_ARGUMENT, "damping must be in (0,1]");
     if (p->probes < 1) return fail(c, PMCMC_ERR_ARGUMENT, "probes must be >= 1");
     if (p->probes > PMCMC_MAXP) return fail(c, PMCMC_ERR_UNSUPPORTED, "probes must be <= %d", PMCMC_MAXP);
-    if (p->jacobian != PMCMC_JAC_STOCH_DIAG && p->jacobian != PMCMC_JAC_DIAG)
-        return fail(c, PMCMC_ERR_ARGUMENT, "jacobian must be :diag or :stoch_diag");
+    if (p->jacobian < PMCMC_JAC_STOCH_DIAG || p->jacobian > PMCMC_JAC_FULL_SIGMOID)
+        return fail(c, PMCMC_ERR_ARGUMENT, "jacobian must be :diag, :stoch_diag, :full or :full_sigmoid");
     if (!(p->eps > 0)) return fail(c, PMCMC_ERR_ARGUMENT, "epsilon must be > 0");
     if (p->precision != w->prec) return fail(c, PMCMC_ERR_ARGUMENT, "params precision %d does not match the workspace (%d)", p->precision, w->prec);
     if (tgt->D != w->D) return fail(c, PMCMC_ERR_DIMENSION, "target dim %lld != workspace D %lld", (long long)tgt->D, (long long)w->D);

@@ -100,7 +100,7 @@ __host__ __device__ inline size_t tile_lds_bytes(int64_t D, int TC, bool with_it
 
 template <typename T>
 __device__ inline T wave_max(T v) {
-    for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+    for (int m = 32; m >= 1; m >>= 1) v = nan_max(v, __shfl_xor(v, m, 64));
     return v;
 }
 
@@ -413,8 +413,8 @@ __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
                 const T old = xs[li + D];
                 const T v = (a.damping == 1.0) ? s : ((T)1 - damp) * old + damp * s;
                 al[li] = v;
-                dmax = fmax(dmax, fabs((double)v - (double)old));
-                smax = fmax(smax, fabs((double)v));
+                dmax = nan_max(dmax, fabs((double)v - (double)old));
+                smax = nan_max(smax, fabs((double)v));
             }
         }
     }
@@ -435,8 +435,8 @@ __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
         if (tid == 0) {
             double m0 = 0, m1 = 0;
             for (int w = 0; w < PMCMC_NT / 64; ++w) {
-                m0 = fmax(m0, s_red[0][w]);
-                m1 = fmax(m1, s_red[1][w]);
+                m0 = nan_max(m0, s_red[0][w]);
+                m1 = nan_max(m1, s_red[1][w]);
             }
             // non-negative doubles order like their bit patterns; NaN sorts above +Inf
             atomicMax(&sy->dmax_bits, (unsigned long long)__double_as_longlong(m0));

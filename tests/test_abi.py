@@ -56,16 +56,31 @@ def test_sampler_argument_validation():
     with pytest.raises(ArgumentError):
         ParallelMALASampler(0.0)
     with pytest.raises(ArgumentError):
-        ParallelMALASampler(0.1, jacobian="full")
+        ParallelMALASampler(0.1, jacobian="bogus")
+    assert ParallelMALASampler(0.1, jacobian=":full").jacobian == "full"
     s = ParallelMALASampler(0.1)
     assert (s.T, s.maxiter, s.tol_abs, s.tol_rel, s.jacobian, s.damping, s.probes) == (64, 200, 1e-6, 1e-5, "stoch_diag", 0.5, 1)
     model = DensityModel(dim=3, target=StdNormalTarget(3))
     rec = DEER.TapedRecursion(model, 0.1, np.zeros((3, 4)), np.full(4, 0.5))
     for kw in (dict(maxiter=0), dict(tol_abs=-1.0), dict(tol_rel=-1.0), dict(damping=0.0), dict(probes=0),
-               dict(jacobian="full")):
+               dict(jacobian="bogus"), dict(jacobian="full")):  # :full needs a dense Gaussian target
         with pytest.raises(ArgumentError):
             DEER.solve(rec, np.zeros(3), **kw)
     with pytest.raises(ArgumentError):
         DEER.solve(rec, np.zeros(2))
     with pytest.raises(ArgumentError):  # closures only: no GPU target descriptor
         DEER.TapedRecursion(DensityModel(lambda x: 0.0, lambda x: x, 3), 0.1, np.zeros((3, 4)), np.full(4, 0.5))
+
+
+def test_full_jacobian_host_validation():
+    """jacobian=:full needs a dense Gaussian target with D <= 64 (host-side checks)."""
+    from ParallelMCMC import ArgumentError, DEER, DensityModel, GaussianTarget, StdNormalTarget
+
+    rec = DEER.TapedRecursion(DensityModel(dim=3, target=StdNormalTarget(3)), 0.1, np.zeros((3, 4)),
+                              np.full(4, 0.5))
+    with pytest.raises(ArgumentError):
+        DEER.solve(rec, np.zeros(3), jacobian="full")
+    rec = DEER.TapedRecursion(DensityModel(dim=65, target=GaussianTarget(np.eye(65))), 0.1, np.zeros((65, 4)),
+                              np.full(4, 0.5))
+    with pytest.raises(ArgumentError):
+        DEER.solve(rec, np.zeros(65), jacobian="full")

@@ -158,6 +158,25 @@ def test_nonconvergence_reported():
     assert abs(info.metric - io.metric) <= 1e-9 * io.metric
 
 
+def test_nan_never_converges():
+    """A NaN entering the Newton update (here through the JVP of a NaN
+    proposal, which the accept test itself rejects) fails the convergence
+    test, as maximum() makes it do in src/DEER/DEER.jl:567-571."""
+    from ParallelMCMC import DEER
+
+    rng = np.random.default_rng(10)
+    D, T = 3, 100
+    tgt = O.DiagGaussian.std_normal(D)
+    x0 = rng.standard_normal(D)
+    Xi, U = O.make_tape(rng, D, T)
+    Xi[1, 40] = np.nan
+    _, io = O.deer_solve(tgt, 0.2, Xi, U, x0, tol_abs=1e-6, tol_rel=1e-5, maxiter=5, damping=1.0)
+    assert not io.converged and np.isnan(io.metric)
+    rec = DEER.TapedRecursion(_model(tgt), 0.2, Xi, U)
+    S, info = DEER.solve(rec, x0, tol_abs=1e-6, tol_rel=1e-5, maxiter=5, damping=1.0, return_info=True)
+    assert not info.converged and info.iters == 5 and np.isnan(info.metric)
+
+
 def test_c0_sampler_blocks_match_sequential():
     """C0: 2-D standard normal, eps=0.1, T=64, :stoch_diag, FP64, 500 samples
     via sample(); every block equals sequential taped MALA on its tape."""

@@ -162,9 +162,10 @@ def test_deer_defect_logreg():
         prev = S[:, t]
 
 
-def test_full_jacobian_deer_on_correlated_gaussian():
-    """With the exact dense Jacobian, undamped DEER is Newton's method on the
-    taped recursion (accept decisions frozen per iteration): it reaches the
+@pytest.mark.parametrize("jac", ["full", "full_sigmoid"])
+def test_full_jacobian_deer_on_correlated_gaussian(jac):
+    """With a dense Jacobian, undamped DEER is Newton's method on the taped
+    recursion (accept decisions frozen per iteration): it reaches the
     sequential trajectory in a few iterations where the exact diagonal needs
     O(T) (eps * lambda_max = 1, condition number 1e3)."""
     rng = np.random.default_rng(13)
@@ -175,7 +176,7 @@ def test_full_jacobian_deer_on_correlated_gaussian():
     x0 = rng.standard_normal(D)
     Xi, U = O.make_tape(rng, D, T)
     ref, acc, _ = O.run_mala_sequential_taped(tgt, x0, eps, Xi, U)
-    S, info = O.deer_solve(tgt, eps, Xi, U, x0, tol_abs=1e-12, tol_rel=1e-11, maxiter=100, jacobian="full",
+    S, info = O.deer_solve(tgt, eps, Xi, U, x0, tol_abs=1e-12, tol_rel=1e-11, maxiter=100, jacobian=jac,
                            damping=1.0)
     assert info.converged and info.iters < 30, info.iters
     assert np.max(np.abs(S - ref)) < 1e-10

@@ -254,8 +254,44 @@ def test_exact_full_jacobian_equals_basis_jvps():
         X, Xi = rng.standard_normal((2, D, T))
         U = rng.uniform(0.01, 0.99, T)
         FT, J, acc, _ = O.mala_jac_full_batched(tgt, X, eps, Xi, U)
+        _, Jh, _, _ = O.mala_jac_full_batched(tgt, X, eps, Xi, U, sigmoid=False)
         for t in range(T):
             jvp = lambda x, tt, v: O.mala_step_surrogate_sigmoid_jvp(tgt, x, eps, Xi[:, tt], U[tt], v)
             Jb = O.jac_full_via_jvps(jvp, X[:, t], t)
             assert np.allclose(J[t], Jb, rtol=1e-10, atol=1e-10)
+            # decision frozen: the JVP with g(1-g) = 0, i.e. central differences
+            # of the taped step wherever the decision does not flip
+            fd = np.stack([(O.mala_step_taped(tgt, X[:, t] + 1e-6 * e, eps, Xi[:, t], U[t])[0]
+                            - O.mala_step_taped(tgt, X[:, t] - 1e-6 * e, eps, Xi[:, t], U[t])[0]) / 2e-6
+                           for e in np.eye(D)], axis=1)
+            assert np.allclose(Jh[t], fd, rtol=1e-6, atol=1e-6)
             assert np.allclose(np.diag(J[t]), O.mala_jac_diag_batched(tgt, X, eps, Xi, U)[1][:, t], atol=1e-12)
+
+
+def test_gaussian_structured_jacobian_form():
+    """For H = -Lambda the dense Jacobian (pinned above against basis JVPs)
+    is I - a eps Lambda + u c^T with u = g(1-g)(y-x) and
+    c = g_y - eps Lambda g_y - g_x - Lambda r + (eps/2) Lambda^2 r,
+    r = x - y - eps g_y: the expanded form the GPU's full-Jacobian scan uses."""
+    rng = np.random.default_rng(81)
+    D, T = 6, 16
+    Lam, ev = O.make_corr_gaussian(rng, D, 2.0)
+    mu = rng.standard_normal(D)
+    tgt = O.Gaussian(Lam, mu)
+    eps = 2.0 / ev.max()
+    X = mu[:, None] + rng.standard_normal((D, T))
+    Xi = rng.standard_normal((D, T))
+    U = rng.uniform(0.01, 0.99, T)
+    FT, J, acc, la = O.mala_jac_full_batched(tgt, X, eps, Xi, U)
+    assert 0 < acc.sum() < T
+    for t in range(T):
+        x = X[:, t]
+        gx = -Lam @ (x - mu)
+        y = x + eps * gx + np.sqrt(2 * eps) * Xi[:, t]
+        gy = -Lam @ (y - mu)
+        r = x - y - eps * gy
+        g = 1.0 / (1.0 + np.exp(-(la[t] - np.log(U[t]))))
+        c = gy - eps * Lam @ gy - gx - Lam @ r + 0.5 * eps * Lam @ (Lam @ r)
+        u = g * (1 - g) * (y - x)
+        Js = np.eye(D) - (1.0 if acc[t] else 0.0) * eps * Lam + np.outer(u, c)
+        assert np.allclose(Js, J[t], rtol=1e-11, atol=1e-12)
