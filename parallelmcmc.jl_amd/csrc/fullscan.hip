@@ -152,8 +152,10 @@ __global__ __launch_bounds__(256) void full_prep_kernel(FullArgs<T> p) {
         }
         const T ft = acc ? yv : xv;
         if (in) {
-            p.u[gi] = uv;
-            p.c[gi] = cv;
+            if (p.sigmoid) {
+                p.u[gi] = uv;
+                p.c[gi] = cv;
+            }
             p.rho[gi] = ft - sv;
         }
         if (lane == 0) {
@@ -164,16 +166,36 @@ __global__ __launch_bounds__(256) void full_prep_kernel(FullArgs<T> p) {
 }
 
 // ---------------------------------------------------------------------------
-// reduce, MFMA paths (T = float).  256 threads; thread (wave w, lane = li + 16 lk)
-// owns M[i][j] for i = 16w + 4lk + r (r < 4), j = 16jt + li (jt < 4): the
-// C-fragment layout of a 16x16 MFMA tile, so Lambda M lands on the entries the
-// thread already holds.  Wave 0 also carries the tile's Delta (lane i = row i).
+// reduce, MFMA paths (T = float).  The tile's state is the augmented 64 x 80
+// matrix [M | Delta | 0]: column tiles 0..3 hold M, column 64 (tile 4) holds
+// the tile's Delta, so one step is [M | Delta] <- J_t [M | Delta] + [0 | rho_t]
+// and Lambda [M | Delta] is a single 64 x 80 x 64 MFMA product.
+// 256 threads; thread (wave w, lane = li + 16 lk) owns entries (i, j) with
+// i = 16w + 4lk + r (r < 4), j = 16jt + li (jt < 5): the C-fragment layout of
+// a 16x16 MFMA tile, so Lambda [M | Delta] lands on the entries the thread
+// holds (fp32 master in registers).  The B operand comes from an LDS copy
+// (double-buffered: one barrier per step), Lambda's A fragments stay in
+// registers.  FP32 16x16x4 runs its k-slots as k = 16 lk + ks so a lane's 16
+// values of one column are contiguous (4 x 128-bit LDS reads).  u, c, rho and
+// the accept flags are staged through LDS in chunks of FCH columns, the next
+// chunk's global loads in flight while the current one is consumed.
+// With SIG (sigmoid surrogate) the rank-1 term u (c^T [M | Delta]) needs
+// w = c^T [M | Delta]: its per-wave partials are formed right after the
+// update with the next step's c and published with the copy.
 // ---------------------------------------------------------------------------
-template <int MM>  // 0: FP32 MFMA 16x16x4, 1: BF16 MFMA 16x16x32
+static constexpr int FNJT = 5;          // column tiles of [M | Delta]
+static constexpr int FNC = 16 * FNJT;  // 80 columns
+static constexpr int FCH = 8;           // steps per staged chunk
+
+template <int MM, bool SIG>  // MM 0: FP32 MFMA 16x16x4, 1: BF16 MFMA 16x16x32
 __global__ __launch_bounds__(256) void full_reduce_mfma_kernel(FullArgs<float> p) {
-    __shared__ __attribute__((aligned(16))) float Mf[MM == 0 ? FD * FS : 4];
-    __shared__ __attribute__((aligned(16))) __bf16 Mb[MM == 1 ? FD * FSB : 8];
-    __shared__ float part[4][FD];
+    __shared__ __attribute__((aligned(16))) float Mf[MM == 0 ? 2 * FNC * FS : 4];
+    __shared__ __attribute__((aligned(16))) __bf16 Mb[MM == 1 ? 2 * FNC * FSB : 8];
+    __shared__ __attribute__((aligned(16))) float part[SIG ? 2 : 1][4][FNC];
+    __shared__ __attribute__((aligned(16))) float su[SIG ? 2 : 1][FCH][FD];
+    __shared__ __attribute__((aligned(16))) float sc[SIG ? 2 : 1][FCH][FD];
+    __shared__ __attribute__((aligned(16))) float sr[2][FCH][FD];
+    __shared__ float sa[2][FCH];
     if (converged(p.sy)) return;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -182,136 +204,196 @@ __global__ __launch_bounds__(256) void full_reduce_mfma_kernel(FullArgs<float> p
     const int64_t t0 = (int64_t)blockIdx.x * p.TC;
     const int tc = (int)((p.Tn - t0) < p.TC ? (p.Tn - t0) : p.TC);
     const float epsf = (float)p.eps;
+    const int ib = 16 * wave + 4 * lk;  // first owned row
 
     // Lambda's A fragments (row tile = wave), constant over the tile
     float af[FD / 4];
     v8bf ab[FD / 32];
     if (MM == 0) {
 #pragma unroll
-        for (int ks = 0; ks < FD / 4; ++ks) af[ks] = lam<float>(p, 16 * wave + li, 4 * ks + lk);
+        for (int ks = 0; ks < FD / 4; ++ks) af[ks] = lam<float>(p, 16 * wave + li, 16 * lk + ks);
     } else {
 #pragma unroll
         for (int ks = 0; ks < FD / 32; ++ks)
 #pragma unroll
             for (int e = 0; e < 8; ++e) ab[ks][e] = (__bf16)lam<float>(p, 16 * wave + li, 32 * ks + 8 * lk + e);
     }
-    // wave 0: Lambda row `lane` for Lambda Delta
-    float lrow[FD];
-    if (wave == 0) {
-#pragma unroll
-        for (int k = 0; k < FD; ++k) lrow[k] = lam<float>(p, lane, k);
+
+    // staging: zero the padded rows once, then chunk loads (register prefetch)
+    for (int i = tid; i < 2 * FCH * FD; i += 256) {
+        (&sr[0][0][0])[i] = 0.f;
+        if (SIG) {
+            (&su[0][0][0])[i] = 0.f;
+            (&sc[0][0][0])[i] = 0.f;
+        }
     }
-    // M = I in registers
-    const int ib = 16 * wave + 4 * lk;  // first owned row
-    float m[4][4];
+    constexpr int PER = (FCH * FD + 255) / 256;  // staged elements per thread and array
+    float pu[PER], pc[PER], pr[PER], pa = 0.f;
+    auto fetch = [&](int chunk) {  // global -> registers
+        const int64_t cb = t0 + (int64_t)chunk * FCH;
+        const int ncol = (tc - chunk * FCH) < FCH ? (tc - chunk * FCH) : FCH;
+        const int n = ncol > 0 ? ncol * (int)D : 0;
 #pragma unroll
-    for (int jt = 0; jt < 4; ++jt)
+        for (int q = 0; q < PER; ++q) {
+            const int e = tid + 256 * q;
+            const bool ok = e < n;
+            const size_t g = (size_t)cb * D + e;
+            pr[q] = ok ? p.rho[g] : 0.f;
+            if (SIG) {
+                pu[q] = ok ? p.u[g] : 0.f;
+                pc[q] = ok ? p.c[g] : 0.f;
+            }
+        }
+        if (tid < ncol) pa = p.a[cb + tid];
+    };
+    auto stash = [&](int chunk) {  // registers -> LDS buffer chunk & 1
+        const int b = chunk & 1;
+        const int ncol = (tc - chunk * FCH) < FCH ? (tc - chunk * FCH) : FCH;
+        const int n = ncol > 0 ? ncol * (int)D : 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = tid + 256 * q;
+            if (e < n) {
+                const int col = e / (int)D, row = e - col * (int)D;
+                sr[b][col][row] = pr[q];
+                if (SIG) {
+                    su[b][col][row] = pu[q];
+                    sc[b][col][row] = pc[q];
+                }
+            }
+        }
+        if (tid < FCH) sa[b][tid] = tid < ncol ? pa : 0.f;
+    };
+    const int nchunks = (tc + FCH - 1) / FCH;
+    __syncthreads();  // zero fill done
+    fetch(0);
+    stash(0);
+    if (nchunks > 1) fetch(1);
+
+    // [M | Delta] = [I | 0] in registers
+    float m[FNJT][4];
+#pragma unroll
+    for (int jt = 0; jt < FNJT; ++jt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) m[jt][r] = (ib + r == 16 * jt + li) ? 1.f : 0.f;
-    float rv = 0.f;
 
-    auto put_copy = [&]() {
+    auto put_copy = [&](int b) {
 #pragma unroll
-        for (int jt = 0; jt < 4; ++jt) {
+        for (int jt = 0; jt < FNJT; ++jt) {
             const int j = 16 * jt + li;
             if (MM == 0) {
-                *reinterpret_cast<float4*>(&Mf[j * FS + ib]) = make_float4(m[jt][0], m[jt][1], m[jt][2], m[jt][3]);
+                *reinterpret_cast<float4*>(&Mf[(b * FNC + j) * FS + ib]) =
+                    make_float4(m[jt][0], m[jt][1], m[jt][2], m[jt][3]);
             } else {
-                v4bf b;
-                b[0] = (__bf16)m[jt][0];
-                b[1] = (__bf16)m[jt][1];
-                b[2] = (__bf16)m[jt][2];
-                b[3] = (__bf16)m[jt][3];
-                *reinterpret_cast<v4bf*>(&Mb[j * FSB + ib]) = b;
+                v4bf v;
+                v[0] = (__bf16)m[jt][0];
+                v[1] = (__bf16)m[jt][1];
+                v[2] = (__bf16)m[jt][2];
+                v[3] = (__bf16)m[jt][3];
+                *reinterpret_cast<v4bf*>(&Mb[(b * FNC + j) * FSB + ib]) = v;
             }
         }
     };
-    put_copy();
-
-    // per-step operands: c and u at the owned rows; wave 0: c, u, rho at row `lane`
-    auto ld = [&](const float* v, int64_t t, int i) -> float { return (i < D) ? v[(size_t)t * D + i] : 0.f; };
-    float cn[4], un[4], c0n = 0.f, u0n = 0.f, r0n = 0.f, an = 0.f;
-    auto fetch = [&](int64_t t) {
+    // per-wave partials of w = c^T [M | Delta] for the step about to run
+    auto put_w = [&](int b, const float* cv) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            cn[r] = ld(p.c, t, ib + r);
-            un[r] = ld(p.u, t, ib + r);
+        for (int jt = 0; jt < FNJT; ++jt) {
+            float wp = cv[0] * m[jt][0] + cv[1] * m[jt][1] + cv[2] * m[jt][2] + cv[3] * m[jt][3];
+            wp += __shfl_xor(wp, 16, 64);
+            wp += __shfl_xor(wp, 32, 64);
+            if (lk == 0) part[b][wave][16 * jt + li] = wp;
         }
-        if (wave == 0) {
-            c0n = ld(p.c, t, lane);
-            u0n = ld(p.u, t, lane);
-            r0n = ld(p.rho, t, lane);
-        }
-        an = p.a[t];
     };
-    if (tc > 0) fetch(t0);
+    put_copy(0);
+    if (SIG) {
+        __syncthreads();  // chunk 0 staged
+        const float4 c4 = *reinterpret_cast<const float4*>(&sc[0][0][ib]);
+        const float cv[4] = {c4.x, c4.y, c4.z, c4.w};
+        put_w(0, cv);
+    }
 
     for (int s = 0; s < tc; ++s) {
-        float cc[4], uu[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            cc[r] = cn[r];
-            uu[r] = un[r];
+        const int ch = s / FCH, col = s - ch * FCH, sb = ch & 1;
+        __syncthreads();  // copy s&1, partials s&1 and chunk ch visible; reads of step s-1 done
+        if (col == 0 && ch + 1 < nchunks) {
+            stash(ch + 1);  // buffer (ch+1)&1 was last read in chunk ch-1
+            if (ch + 2 < nchunks) fetch(ch + 2);
         }
-        const float c0 = c0n, u0 = u0n, r0 = r0n, a = an;
-        if (s + 1 < tc) fetch(t0 + s + 1);
-        __syncthreads();  // the LDS copy of M is complete
-        v4f acc[4];
+        const int b = s & 1;
+        v4f acc[FNJT];
 #pragma unroll
-        for (int jt = 0; jt < 4; ++jt) acc[jt] = (v4f){0.f, 0.f, 0.f, 0.f};
+        for (int jt = 0; jt < FNJT; ++jt) acc[jt] = (v4f){0.f, 0.f, 0.f, 0.f};
         if (MM == 0) {
 #pragma unroll
-            for (int ks = 0; ks < FD / 4; ++ks)
+            for (int jt = 0; jt < FNJT; ++jt) {
+                const float* src = &Mf[(b * FNC + 16 * jt + li) * FS + 16 * lk];
+                float bv[16];
 #pragma unroll
-                for (int jt = 0; jt < 4; ++jt)
-                    acc[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[ks], Mf[(16 * jt + li) * FS + 4 * ks + lk],
-                                                                   acc[jt], 0, 0, 0);
+                for (int q = 0; q < 4; ++q) {
+                    const float4 v = *reinterpret_cast<const float4*>(src + 4 * q);
+                    bv[4 * q] = v.x;
+                    bv[4 * q + 1] = v.y;
+                    bv[4 * q + 2] = v.z;
+                    bv[4 * q + 3] = v.w;
+                }
+#pragma unroll
+                for (int ks = 0; ks < FD / 4; ++ks)
+                    acc[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[ks], bv[ks], acc[jt], 0, 0, 0);
+            }
         } else {
 #pragma unroll
-            for (int ks = 0; ks < FD / 32; ++ks)
+            for (int jt = 0; jt < FNJT; ++jt)
 #pragma unroll
-                for (int jt = 0; jt < 4; ++jt) {
-                    const v8bf bv = *reinterpret_cast<const v8bf*>(&Mb[(16 * jt + li) * FSB + 32 * ks + 8 * lk]);
+                for (int ks = 0; ks < FD / 32; ++ks) {
+                    const v8bf bv =
+                        *reinterpret_cast<const v8bf*>(&Mb[(b * FNC + 16 * jt + li) * FSB + 32 * ks + 8 * lk]);
                     acc[jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab[ks], bv, acc[jt], 0, 0, 0);
                 }
         }
-        // w_j = sum_i c_i M_ij: rows of this thread, then lanes lk, then waves
-#pragma unroll
-        for (int jt = 0; jt < 4; ++jt) {
-            float wp = cc[0] * m[jt][0] + cc[1] * m[jt][1] + cc[2] * m[jt][2] + cc[3] * m[jt][3];
-            wp += __shfl_xor(wp, 16, 64);
-            wp += __shfl_xor(wp, 32, 64);
-            if (lk == 0) part[wave][16 * jt + li] = wp;
+        const float ae = sa[sb][col] * epsf;
+        const float4 r4 = *reinterpret_cast<const float4*>(&sr[sb][col][ib]);
+        const float rho[4] = {r4.x, r4.y, r4.z, r4.w};
+        float uu[4] = {0.f, 0.f, 0.f, 0.f};
+        if (SIG) {
+            const float4 u4 = *reinterpret_cast<const float4*>(&su[sb][col][ib]);
+            uu[0] = u4.x;
+            uu[1] = u4.y;
+            uu[2] = u4.z;
+            uu[3] = u4.w;
         }
-        // wave 0: Delta <- Delta - a eps Lambda Delta + u (c . Delta) + rho
-        if (wave == 0) {
-            float Lr = 0.f;
 #pragma unroll
-            for (int k = 0; k < FD; ++k) Lr += lrow[k] * rlane(rv, k);
-            float cr = c0 * rv;
-#pragma unroll
-            for (int mm = 32; mm >= 1; mm >>= 1) cr += __shfl_xor(cr, mm, 64);
-            rv = rv - a * epsf * Lr + u0 * cr + r0;
-        }
-        __syncthreads();  // partial sums visible; all MFMA reads of the copy done
-        const float ae = a * epsf;
-#pragma unroll
-        for (int jt = 0; jt < 4; ++jt) {
+        for (int jt = 0; jt < FNJT; ++jt) {
             const int j = 16 * jt + li;
-            const float wj = part[0][j] + part[1][j] + part[2][j] + part[3][j];
+            float wj = 0.f;
+            if (SIG) wj = part[b][0][j] + part[b][1][j] + part[b][2][j] + part[b][3][j];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) m[jt][r] = m[jt][r] - ae * acc[jt][r] + uu[r] * wj;
+            for (int r = 0; r < 4; ++r) {
+                float v = m[jt][r] - ae * acc[jt][r];
+                if (SIG) v += uu[r] * wj;
+                if (jt == FNJT - 1 && li == 0) v += rho[r];  // column 64: Delta
+                m[jt][r] = v;
+            }
         }
-        put_copy();
+        if (s + 1 < tc) {
+            put_copy(b ^ 1);
+            if (SIG) {
+                const int s1 = s + 1, ch1 = s1 / FCH, col1 = s1 - ch1 * FCH;
+                const float4 c4 = *reinterpret_cast<const float4*>(&sc[ch1 & 1][col1][ib]);
+                const float cv[4] = {c4.x, c4.y, c4.z, c4.w};
+                put_w(b ^ 1, cv);
+            }
+        }
     }
     // tile aggregates: M column-major 64 x 64, Delta
     float* Mo = p.Magg + (size_t)blockIdx.x * FD * FD;
 #pragma unroll
-    for (int jt = 0; jt < 4; ++jt) {
+    for (int jt = 0; jt < FNJT - 1; ++jt) {
         const int j = 16 * jt + li;
         *reinterpret_cast<float4*>(&Mo[j * FD + ib]) = make_float4(m[jt][0], m[jt][1], m[jt][2], m[jt][3]);
     }
-    if (wave == 0) p.ragg[(size_t)blockIdx.x * FD + lane] = rv;
+    if (li == 0)
+        *reinterpret_cast<float4*>(&p.ragg[(size_t)blockIdx.x * FD + ib]) =
+            make_float4(m[FNJT - 1][0], m[FNJT - 1][1], m[FNJT - 1][2], m[FNJT - 1][3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -345,8 +427,9 @@ __global__ __launch_bounds__(256) void full_reduce_f64_kernel(FullArgs<double> p
     for (int s = 0; s < tc; ++s) {
         const int64_t t = t0 + s;
         const double a = p.a[t];
-        const double c0 = in ? p.c[(size_t)t * D + lane] : 0.0;
-        const double u0 = in ? p.u[(size_t)t * D + lane] : 0.0;
+        const bool sg = in && p.sigmoid;  // u and c exist only for the sigmoid variant
+        const double c0 = sg ? p.c[(size_t)t * D + lane] : 0.0;
+        const double u0 = sg ? p.u[(size_t)t * D + lane] : 0.0;
         const double r0 = in ? p.rho[(size_t)t * D + lane] : 0.0;
         if (wave == 0) uvec[lane] = u0;
         __syncthreads();  // copy of M and u complete
@@ -394,9 +477,12 @@ __global__ __launch_bounds__(256) void full_reduce_f64_kernel(FullArgs<double> p
 
 // ---------------------------------------------------------------------------
 // spine: one workgroup, Delta_k = M_k Delta_{k-1} + r_k over the tiles.  Thread
-// (quarter q, lane i) sums columns 16q..16q+15 of row i; the next tile's
-// entries are loaded while the current one is applied.
+// (quarter q, lane i) sums columns 16q..16q+15 of row i.  The chain is a
+// dependent mat-vec per tile, so the tiles' entries stream through a ring of
+// SPF register slots: loads run SPF tiles ahead of the chain.
 // ---------------------------------------------------------------------------
+static constexpr int SPF = 4;
+
 template <typename T>
 __global__ __launch_bounds__(256) void full_spine_kernel(FullArgs<T> p) {
     __shared__ T v[FD];
@@ -404,38 +490,53 @@ __global__ __launch_bounds__(256) void full_spine_kernel(FullArgs<T> p) {
     if (converged(p.sy)) return;
     const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
     if (tid < FD) v[tid] = 0;
-    T mc[16], mn[16];
-    T rn = 0;
-    auto fetch = [&](int k) {
+    T ring[SPF][16];
+    T rring[SPF];
+    auto fetch = [&](int k, int slot) {
         const T* M = p.Magg + (size_t)k * FD * FD;
 #pragma unroll
-        for (int jj = 0; jj < 16; ++jj) mn[jj] = M[(size_t)(16 * q + jj) * FD + lane];
-        if (q == 0) rn = p.ragg[(size_t)k * FD + lane];
+        for (int jj = 0; jj < 16; ++jj) ring[slot][jj] = M[(size_t)(16 * q + jj) * FD + lane];
+        rring[slot] = q == 0 ? p.ragg[(size_t)k * FD + lane] : (T)0;
     };
-    fetch(0);
+#pragma unroll
+    for (int sl = 0; sl < SPF; ++sl)
+        if (sl < p.nT) fetch(sl, sl);
     __syncthreads();
-    for (int k = 0; k < p.nT; ++k) {
+    for (int k0 = 0; k0 < p.nT; k0 += SPF) {
 #pragma unroll
-        for (int jj = 0; jj < 16; ++jj) mc[jj] = mn[jj];
-        const T rc = rn;
-        if (k + 1 < p.nT) fetch(k + 1);
-        T s = 0;
+        for (int sl = 0; sl < SPF; ++sl) {
+            const int k = k0 + sl;
+            if (k < p.nT) {
+                T mc[16];
 #pragma unroll
-        for (int jj = 0; jj < 16; ++jj) s += mc[jj] * v[16 * q + jj];
-        part[q][lane] = s;
-        __syncthreads();
-        if (q == 0) {
-            const T nv = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane] + rc;
-            v[lane] = nv;
-            p.dend[(size_t)k * FD + lane] = nv;
+                for (int jj = 0; jj < 16; ++jj) mc[jj] = ring[sl][jj];
+                const T rc = rring[sl];
+                if (k + SPF < p.nT) fetch(k + SPF, sl);
+                T s0 = 0, s1 = 0;
+#pragma unroll
+                for (int jj = 0; jj < 16; jj += 2) {
+                    s0 += mc[jj] * v[16 * q + jj];
+                    s1 += mc[jj + 1] * v[16 * q + jj + 1];
+                }
+                part[q][lane] = s0 + s1;
+                __syncthreads();
+                if (q == 0) {
+                    const T nv = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane] + rc;
+                    v[lane] = nv;
+                    p.dend[(size_t)k * FD + lane] = nv;
+                }
+                __syncthreads();
+            }
         }
-        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
 // apply: one wave per tile; lane d = row d with Lambda's row d in registers.
+// The step's operands stream through a ring of APF register slots.
 // ---------------------------------------------------------------------------
+static constexpr int APF = 8;
+
 template <typename T>
 __global__ __launch_bounds__(64) void full_apply_kernel(FullArgs<T> p) {
     if (converged(p.sy)) return;
@@ -448,36 +549,60 @@ __global__ __launch_bounds__(64) void full_apply_kernel(FullArgs<T> p) {
 #pragma unroll
     for (int k = 0; k < FD; ++k) lrow[k] = lam<T>(p, lane, k);
     const bool in = lane < D;
+    const bool sig = p.sigmoid != 0;
     const T epsT = (T)p.eps, damp = (T)p.damping;
     T dv = tile == 0 ? (T)0 : p.dend[(size_t)(tile - 1) * FD + lane];
     double dmax = 0.0, smax = 0.0;
-    T un = 0, cn = 0, rn = 0, sn = 0, an = 0;
-    auto fetch = [&](int64_t t) {
-        const size_t gi = (size_t)t * D + lane;
+    T ru[APF], rc[APF], rr[APF], rs[APF], ra[APF];
+    auto fetch = [&](int s, int sl) {
+        const size_t gi = (size_t)(t0 + s) * D + lane;
+        ru[sl] = (T)0;
+        rc[sl] = (T)0;
+        rr[sl] = (T)0;
+        rs[sl] = (T)0;
         if (in) {
-            un = p.u[gi];
-            cn = p.c[gi];
-            rn = p.rho[gi];
-            sn = p.S_in[gi];
+            if (sig) {
+                ru[sl] = p.u[gi];
+                rc[sl] = p.c[gi];
+            }
+            rr[sl] = p.rho[gi];
+            rs[sl] = p.S_in[gi];
         }
-        an = p.a[t];
+        ra[sl] = p.a[t0 + s];
     };
-    if (tc > 0) fetch(t0);
-    for (int s = 0; s < tc; ++s) {
-        const T u = un, c = cn, rho = rn, old = sn, a = an;
-        if (s + 1 < tc) fetch(t0 + s + 1);
-        T Ld = 0;
 #pragma unroll
-        for (int k = 0; k < FD; ++k) Ld += lrow[k] * rlane(dv, k);
-        T cd = c * dv;
+    for (int sl = 0; sl < APF; ++sl)
+        if (sl < tc) fetch(sl, sl);
+    for (int s0 = 0; s0 < tc; s0 += APF) {
 #pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) cd += __shfl_xor(cd, m, 64);
-        dv = dv - a * epsT * Ld + u * cd + rho;
-        if (in) {
-            const T nv = old + damp * dv;
-            p.S_out[(size_t)(t0 + s) * D + lane] = nv;
-            dmax = nan_max(dmax, fabs((double)nv - (double)old));
-            smax = nan_max(smax, fabs((double)nv));
+        for (int sl = 0; sl < APF; ++sl) {
+            const int s = s0 + sl;
+            if (s < tc) {
+                const T u = ru[sl], c = rc[sl], rho = rr[sl], old = rs[sl], a = ra[sl];
+                if (s + APF < tc) fetch(s + APF, sl);
+                T L0 = 0, L1 = 0, L2 = 0, L3 = 0;
+#pragma unroll
+                for (int k = 0; k < FD; k += 4) {
+                    L0 += lrow[k] * rlane(dv, k);
+                    L1 += lrow[k + 1] * rlane(dv, k + 1);
+                    L2 += lrow[k + 2] * rlane(dv, k + 2);
+                    L3 += lrow[k + 3] * rlane(dv, k + 3);
+                }
+                T nd = dv - a * epsT * ((L0 + L1) + (L2 + L3)) + rho;
+                if (sig) {
+                    T cd = c * dv;
+#pragma unroll
+                    for (int m = 32; m >= 1; m >>= 1) cd += __shfl_xor(cd, m, 64);
+                    nd += u * cd;
+                }
+                dv = nd;
+                if (in) {
+                    const T nv = old + damp * dv;
+                    p.S_out[(size_t)(t0 + s) * D + lane] = nv;
+                    dmax = nan_max(dmax, fabs((double)nv - (double)old));
+                    smax = nan_max(smax, fabs((double)nv));
+                }
+            }
         }
     }
 #pragma unroll
@@ -593,10 +718,18 @@ int full_newton_step(FullWork* fw, const GlmWork* gw, const GaussData* g, hipStr
     if constexpr (sizeof(T) == 8) {
         hipLaunchKernelGGL(full_reduce_f64_kernel, dim3((unsigned)fw->nT), dim3(256), 0, s, p);
     } else {
-        if (precision == PMCMC_PREC_BF16_JAC)
-            hipLaunchKernelGGL((full_reduce_mfma_kernel<1>), dim3((unsigned)fw->nT), dim3(256), 0, s, p);
-        else
-            hipLaunchKernelGGL((full_reduce_mfma_kernel<0>), dim3((unsigned)fw->nT), dim3(256), 0, s, p);
+        const dim3 g((unsigned)fw->nT), blk(256);
+        if (precision == PMCMC_PREC_BF16_JAC) {
+            if (sigmoid)
+                hipLaunchKernelGGL((full_reduce_mfma_kernel<1, true>), g, blk, 0, s, p);
+            else
+                hipLaunchKernelGGL((full_reduce_mfma_kernel<1, false>), g, blk, 0, s, p);
+        } else {
+            if (sigmoid)
+                hipLaunchKernelGGL((full_reduce_mfma_kernel<0, true>), g, blk, 0, s, p);
+            else
+                hipLaunchKernelGGL((full_reduce_mfma_kernel<0, false>), g, blk, 0, s, p);
+        }
     }
     if (prof) prof->end(e, s);
     e = prof ? prof->begin(KT_SCAN_SPINE, iter, s) : -1;
