@@ -175,6 +175,22 @@ int pmcmc_deer_solve(pmcmc_ctx* ctx, pmcmc_deer_ws* ws, const pmcmc_target* tgt,
 int pmcmc_target_logdensity_batch(pmcmc_ctx* ctx, const pmcmc_target* tgt, int prec,
                                   const void* S, double* out, int64_t T);
 
+/* Adjoint of a converged trajectory S (decisions `accepted` held fixed): given
+ * G = dL/dS (D x T), the gradients of the statistic L with respect to eps and
+ * the target's hyperparameter (the logistic prior scale; 0 for the Gaussian
+ * targets) and dL/dx0.  Solves the reverse recurrence
+ * lambda_t = G_t + J_{t+1}^T lambda_{t+1} by reverse-time DEER sweeps (batched
+ * HVPs + diagonal scan) to params->tol_abs / tol_rel within params->maxiter;
+ * params->seed picks the Hutchinson probe.  No reference counterpart: the
+ * reference differentiates sequential MALA with Enzyme (ext/EnzymeExt.jl);
+ * this replaces that for the DEER path.  grad_out: HOST double[2];
+ * dx0_out: D (device, or NULL); lambda_out: D x T (device, or NULL).
+ * Synchronises the context stream.                                         */
+int pmcmc_deer_adjoint(pmcmc_ctx* ctx, pmcmc_deer_ws* ws, const pmcmc_target* tgt,
+                       const pmcmc_deer_params* params, const void* x0, const void* Xi,
+                       const void* S, const uint8_t* accepted, const void* G, double* grad_out,
+                       void* dx0_out, void* lambda_out, pmcmc_deer_info* info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -161,6 +161,10 @@ class LogReg(Target):
         W = P * (1.0 - P)
         return -((self.Xd * self.Xd).T @ W) - 1.0 / self.prior_scale**2
 
+    def dgrad_dhyper(self, B):
+        """d grad / d prior_scale = 2 B / prior_scale^3 (the prior's term)."""
+        return 2.0 * B / self.prior_scale**3
+
 
 def make_logreg_data(rng, N, D, dtype=np.float64):
     """Synthetic logistic data: X_ij ~ N(0,1)/sqrt(D), theta* ~ N(0,I),
@@ -745,6 +749,53 @@ def trace_error(S, S_ref):
 
 
 # ----------------------------------------------------------------------------
+# Adjoint of the taped recursion (no reference counterpart: the reference
+# differentiates sequential MALA with Enzyme, ext/EnzymeExt.jl).
+# ----------------------------------------------------------------------------
+
+
+def run_mala_frozen(target, x0, eps, Xi, acc):
+    """The taped MALA recursion with the accept decisions held at ``acc``:
+    x_t = acc_t ? x_{t-1} + eps grad(x_{t-1}) + sqrt(2 eps) xi_t : x_{t-1}
+    (the proposal of src/MALA/MALA.jl:207-229).  Returns D x T."""
+    D, T = Xi.shape
+    S = np.empty((D, T))
+    x = np.asarray(x0, dtype=np.float64).copy()
+    for t in range(T):
+        if acc[t]:
+            x = x + eps * target.grad(x) + math.sqrt(2 * eps) * Xi[:, t]
+        S[:, t] = x
+    return S
+
+
+def mala_adjoint(target, eps, Xi, s0, S, acc, G):
+    """Gradient of a trajectory statistic L(S) through the taped recursion at
+    the trajectory S with the decisions ``acc`` held fixed.
+
+    With Xbar_t = [s0, S][:, t] and J_t = a_t (I + eps H(Xbar_t)) + (1 - a_t) I:
+      lambda_{T-1} = G_{T-1},  lambda_t = G_t + J_{t+1}^T lambda_{t+1},
+      dL/deps   = sum_t a_t lambda_t . (grad(Xbar_t) + xi_t / sqrt(2 eps)),
+      dL/dhyper = sum_t a_t eps lambda_t . dgrad/dhyper(Xbar_t),
+      dL/ds0    = J_0^T lambda_0,
+    where G = dL/dS.  H is symmetric, so J^T v = v + a eps H v.
+    Returns (dL/deps, dL/dhyper (0 for targets without one), dL/ds0, lambda)."""
+    D, T = S.shape
+    Xbar = _xbar(s0, S)
+    lam = np.zeros((D, T))
+    carry = np.zeros(D)
+    for t in range(T - 1, -1, -1):
+        lam[:, t] = G[:, t] + carry
+        carry = lam[:, t] + (eps * target.hvp(Xbar[:, t], lam[:, t]) if acc[t] else 0.0)
+    a = np.asarray(acc, dtype=np.float64)[None, :]
+    gx = target.grad_batch(Xbar)
+    deps = float(np.sum(a * lam * (gx + Xi / math.sqrt(2 * eps))))
+    dh = 0.0
+    if hasattr(target, "dgrad_dhyper"):
+        dh = float(np.sum(a * eps * lam * target.dgrad_dhyper(Xbar)))
+    return deps, dh, carry, lam
+
+
+# ----------------------------------------------------------------------------
 # Tapes.
 # ----------------------------------------------------------------------------
 
@@ -759,6 +810,9 @@ def make_tape(rng, D, T, dtype=np.float64):
 
 # How each oracle function is pinned (tests/test_oracle_*.py).
 PINNED = {
+    "run_mala_frozen": "equals run_mala_sequential_taped when fed that run's own decisions",
+    "mala_adjoint": "central differences of L(run_mala_frozen) in eps, prior_scale and s0; "
+                    "hard-decision differences where no decision flips",
     "logq_mala": "closed-form Gaussian log-density (scipy.stats.multivariate_normal) at 1e-12",
     "mala_step_taped": "forced accept/reject via u=tau/2 and (tau+1)/2; rejected => unchanged; determinism",
     "run_mala_sequential_taped": "stationary moments on N(0,I); DEER fixed point equals it",

@@ -154,6 +154,52 @@ def solve(rec: TapedRecursion, s0, init=None, tol_abs=1e-4, tol_rel=1e-3, maxite
     return S, Info(bool(info.converged), int(info.iters), float(info.metric), jname, float(damping), int(probes), acc)
 
 
+def adjoint(rec: TapedRecursion, s0, S, accepted, G, tol_abs=0.0, tol_rel=None, maxiter=500, seed=0,
+            workspace=None, return_lambda=False):
+    """Gradient of a statistic L(S) of the converged trajectory S of ``rec``
+    with the accept decisions ``accepted`` held fixed, given G = dL/dS.
+
+    Returns (dL/deps, dL/dhyper, dL/ds0, info[, lambda]); dL/dhyper is the
+    derivative with respect to the logistic target's prior scale (0 for the
+    Gaussian targets).  No reference counterpart (the reference differentiates
+    sequential MALA with Enzyme); the reverse recurrence is solved by
+    reverse-time DEER sweeps to tol_abs / tol_rel (default 1e-6 in FP32,
+    1e-12 in FP64)."""
+    D, T = rec.D, rec.T
+    if tol_rel is None:
+        tol_rel = 1e-12 if rec.precision == _lib.PREC_F64 else 1e-6
+    if maxiter < 1:
+        raise ArgumentError("maxiter must be >= 1")
+    if tol_abs < 0 or tol_rel < 0:
+        raise ArgumentError("tolerances must be >= 0")
+    ws = workspace if workspace is not None else DEERWorkspace(D, T, rec.precision)
+    if (ws.D, ws.T) != (D, T) or ws.precision != rec.precision:
+        raise ArgumentError("workspace does not match the recursion")
+    dev = ws.device
+    dt = _device.torch_dtype(rec.precision)
+    Xi, _ = rec.device_tape(dev)
+    x0 = _device.to_vec(s0, dt, dev)
+    S_ = _device.to_colmajor(S, dt, dev)
+    G_ = _device.to_colmajor(G, dt, dev)
+    if tuple(S_.shape) != (D, T) or tuple(G_.shape) != (D, T):
+        raise DimensionMismatch("S and G must be D x T")
+    acc = torch.as_tensor(accepted).to(device=dev, dtype=torch.uint8).contiguous()
+    if acc.numel() != T:
+        raise DimensionMismatch("accepted must have T entries")
+    dx0 = torch.empty(D, dtype=dt, device=dev)
+    lam = _device.colmajor_empty(D, T, dt, dev) if return_lambda else None
+    params = _lib.DeerParams(float(rec.eps), float(tol_abs), float(tol_rel), 1.0, int(maxiter),
+                             _lib.JAC_STOCH_DIAG, 1, rec.precision, int(seed) & 0xFFFFFFFFFFFFFFFF)
+    grad = (C.c_double * 2)()
+    info = _lib.DeerInfo()
+    ws.ctx.call("pmcmc_deer_adjoint", ws.h, rec.target.handle(ws.ctx), C.byref(params), _device.ptr(x0),
+                _device.ptr(Xi), _device.ptr(S_), _device.ptr(acc), _device.ptr(G_), grad, _device.ptr(dx0),
+                _device.ptr(lam), C.byref(info))
+    inf = Info(bool(info.converged), int(info.iters), float(info.metric), "adjoint", 1.0, 1)
+    out = (float(grad[0]), float(grad[1]), dx0, inf)
+    return out + (lam,) if return_lambda else out
+
+
 def deer_update(rec: TapedRecursion, s0, S, jacobian="stoch_diag", damping=1.0, probes=1, seed=0):
     """One Newton step from the iterate S (src/DEER/DEER.jl:477-492), with
     the probes of iteration 0 of ``seed``."""

@@ -19,6 +19,7 @@ from .interface import (
 )
 from .targets import DiagGaussianTarget, GaussianTarget, LogisticTarget, StdNormalTarget
 from . import DEER, DEERScan, MALA  # noqa: E402
+from . import Adjoint  # noqa: E402
 
 __all__ = [
     "DensityModel",
@@ -31,6 +32,7 @@ __all__ = [
     "DEER",
     "DEERScan",
     "MALA",
+    "Adjoint",
     "DiagGaussianTarget",
     "StdNormalTarget",
     "LogisticTarget",

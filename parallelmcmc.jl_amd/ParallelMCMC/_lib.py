@@ -95,6 +95,8 @@ _SIGS = {
     "pmcmc_deer_ws_destroy": ([_P], C.c_int),
     "pmcmc_deer_solve": ([_P, _P, _P, C.POINTER(DeerParams), _P, _P, _P, _P, _P, _P, C.POINTER(DeerInfo)], C.c_int),
     "pmcmc_target_logdensity_batch": ([_P, _P, C.c_int, _P, _P, _I64], C.c_int),
+    "pmcmc_deer_adjoint": ([_P, _P, _P, C.POINTER(DeerParams), _P, _P, _P, _P, _P, C.POINTER(C.c_double), _P, _P,
+                            C.POINTER(DeerInfo)], C.c_int),
 }
 
 _lib = None

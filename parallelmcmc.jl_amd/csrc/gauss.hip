@@ -348,6 +348,28 @@ template int gauss_newton_products<double>(GlmWork*, const GaussData*, hipStream
                                            int64_t, int64_t, Prof*, std::string*);
 
 template <typename T>
+int gauss_dir_products(GlmWork* w, const GaussData* g, hipStream_t s, double eps, const T* V, const T* x0,
+                       const T* Z, uint32_t key, const T* Xi, int64_t D, int64_t Tn, const int* skip, Prof* prof,
+                       int iter, std::string* msg) {
+    ColSrc<T> a{};
+    a.V = V;
+    a.x0 = x0;
+    a.Z = Z;
+    a.keys[0] = key;
+    a.K = 1;
+    a.D = D;
+    a.Tn = Tn;
+    return gauss_pass<T>(g, s, a, 0, eps, Xi, (T*)w->gx, (T*)w->hw, w->lpx, (T*)w->Y, (T*)w->W, skip, prof, iter,
+                         false, msg);
+}
+template int gauss_dir_products<float>(GlmWork*, const GaussData*, hipStream_t, double, const float*, const float*,
+                                       const float*, uint32_t, const float*, int64_t, int64_t, const int*, Prof*, int,
+                                       std::string*);
+template int gauss_dir_products<double>(GlmWork*, const GaussData*, hipStream_t, double, const double*,
+                                        const double*, const double*, uint32_t, const double*, int64_t, int64_t,
+                                        const int*, Prof*, int, std::string*);
+
+template <typename T>
 static int gauss_fwd_jvp_t(const GaussData* g, hipStream_t s, double eps, const T* X, const T* Xi, const T* U,
                            const T* Z, T* FT, T* Jt, uint8_t* acc, double* la, int64_t D, int64_t Tn, bool jbf,
                            std::string* msg) {

@@ -32,6 +32,12 @@ int gauss_newton_products(GlmWork* w, const GaussData* g, hipStream_t s, SyncSta
                           uint64_t seed, int iter, int probes, int64_t D, int64_t Tn, Prof* prof,
                           std::string* msg);
 
+// As glm_dir_products: w->gx = grad(states), w->W = dir + eps H dir.
+template <typename T>
+int gauss_dir_products(GlmWork* w, const GaussData* g, hipStream_t s, double eps, const T* V, const T* x0,
+                       const T* Z, uint32_t key, const T* Xi, int64_t D, int64_t Tn, const int* skip, Prof* prof,
+                       int iter, std::string* msg);
+
 int gauss_fwd_and_jvp(const GaussData* g, hipStream_t s, int prec, double eps, const void* X,
                       const void* Xi, const void* U, const void* Z, void* FT, void* Jt,
                       uint8_t* accepted, double* logalpha, int64_t D, int64_t T, std::string* msg);

@@ -651,6 +651,30 @@ template int glm_newton_products<double>(GlmWork*, const GlmData*, hipStream_t, 
                                          const double*, uint64_t, int, int, int64_t, int64_t, Prof*,
                                          std::string*);
 
+// One pass at the states V (shifted by x0 when given) with one direction set
+// (Z, or the probe with `key` when Z is null): w->gx = grad, w->W = dir + eps H dir.
+template <typename T>
+int glm_dir_products(GlmWork* w, const GlmData* g, hipStream_t s, double eps, double prior_scale, const T* V,
+                     const T* x0, const T* Z, uint32_t key, const T* Xi, int64_t D, int64_t Tn, const int* skip,
+                     Prof* prof, int iter, std::string* msg) {
+    ColSrc<T> a{};
+    a.V = V;
+    a.x0 = x0;
+    a.Z = Z;
+    a.keys[0] = key;
+    a.K = 1;
+    a.D = D;
+    a.Tn = Tn;
+    return glm_pass<T>(g, w, s, a, 0, eps, prior_scale, Xi, (T*)w->gx, (T*)w->hw, w->lpx, (T*)w->Y, (T*)w->W,
+                       skip, prof, iter, false, msg);
+}
+template int glm_dir_products<float>(GlmWork*, const GlmData*, hipStream_t, double, double, const float*,
+                                     const float*, const float*, uint32_t, const float*, int64_t, int64_t,
+                                     const int*, Prof*, int, std::string*);
+template int glm_dir_products<double>(GlmWork*, const GlmData*, hipStream_t, double, double, const double*,
+                                      const double*, const double*, uint32_t, const double*, int64_t, int64_t,
+                                      const int*, Prof*, int, std::string*);
+
 // ---------------------------------------------------------------------------
 // standalone batched MALA step + JVP, and log-densities
 // ---------------------------------------------------------------------------

@@ -84,6 +84,14 @@ void launch_fwd_jvp_finish(hipStream_t s, const T* X, const T* Z, const T* U, co
                            double eps, int64_t D, int64_t Tn, T* FT, T* Jt, uint8_t* acc,
                            double* la);
 
+// One pass at the states V (shifted by x0 when given) with one direction set
+// (Z, or the Rademacher probe with `key` when Z is null): w->gx = grad(states),
+// w->W = dir + eps H(states) dir.  w must hold one probe set.
+template <typename T>
+int glm_dir_products(GlmWork* w, const GlmData* g, hipStream_t s, double eps, double prior_scale, const T* V,
+                     const T* x0, const T* Z, uint32_t key, const T* Xi, int64_t D, int64_t Tn, const int* skip,
+                     Prof* prof, int iter, std::string* msg);
+
 int glm_fwd_and_jvp(const GlmData* g, hipStream_t s, int prec, double eps, double prior_scale,
                     const void* X, const void* Xi, const void* U, const void* Z, void* FT, void* Jt,
                     uint8_t* accepted, double* logalpha, int64_t D, int64_t T, std::string* msg);

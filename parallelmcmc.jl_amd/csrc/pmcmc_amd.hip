@@ -14,6 +14,7 @@
 #include <string>
 
 #include "../../include/pmcmc_amd.h"
+#include "adjoint.hpp"
 #include "common.hpp"
 #include "fullscan.hpp"
 #include "gauss.hpp"
@@ -81,6 +82,15 @@ struct pmcmc_deer_ws {
     // GEMM-target buffers (allocated on first use)
     GlmWork glmw;
     FullWork fullw;  // jacobian = :full
+    // adjoint buffers (allocated on first use)
+    void* adj_lam = nullptr;   // D x (T+1), last column zero
+    void* adj_A = nullptr;     // reversed-time scan coefficients, D x T
+    void* adj_B = nullptr;     // reversed-time residuals, D x T
+    void* adj_D = nullptr;     // reversed-time corrections, D x T
+    void* adj_zero = nullptr;  // D zeros (scan start)
+    double* adj_part = nullptr;  // 2 T column partials + 2 sums
+    ScanPlan adj_plan;
+    void* adj_blob = nullptr;
 };
 
 static int fail(pmcmc_ctx* ctx, int code, const char* fmt, ...) {
@@ -687,6 +697,13 @@ extern "C" int pmcmc_deer_ws_destroy(pmcmc_deer_ws* w) {
     hipFree(w->blob);
     glm_work_destroy(&w->glmw);
     full_work_destroy(&w->fullw);
+    hipFree(w->adj_lam);
+    hipFree(w->adj_A);
+    hipFree(w->adj_B);
+    hipFree(w->adj_D);
+    hipFree(w->adj_zero);
+    hipFree(w->adj_part);
+    hipFree(w->adj_blob);
     delete w;
     return PMCMC_OK;
 }
@@ -859,4 +876,145 @@ extern "C" int pmcmc_deer_solve(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_targ
     if (w->prec == PMCMC_PREC_F64)
         return deer_solve_t<double>(c, w, tgt, p, x0, Xi, U, init, S_out, acc_out, info);
     return deer_solve_t<float>(c, w, tgt, p, x0, Xi, U, init, S_out, acc_out, info);
+}
+
+// ---------------------------------------------------------------------------
+// adjoint of a converged trajectory (decisions held fixed)
+// ---------------------------------------------------------------------------
+template <typename T>
+static int adj_products(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt, double eps, const T* V,
+                        const T* x0, const T* Z, uint32_t key, const T* Xi, const int* skip, int iter,
+                        std::string* msg) {
+    const int64_t D = w->D, Tn = w->T;
+    if (tgt->kind == TGT_DIAG) {
+        const T* prec = sizeof(T) == 8 ? (const T*)tgt->prec_d : (const T*)tgt->prec_f;
+        const T* mean = sizeof(T) == 8 ? (const T*)tgt->mean_d : (const T*)tgt->mean_f;
+        hipLaunchKernelGGL((adj_diag_products_kernel<T>), dim3(grid_for(D * Tn)), dim3(256), 0, c->stream, V, x0, Z,
+                           key, prec, mean, eps, D, Tn, (T*)w->glmw.gx, (T*)w->glmw.W, skip);
+        if (hipGetLastError() != hipSuccess) {
+            *msg = "adjoint products kernel failed";
+            return PMCMC_ERR_HIP;
+        }
+        return PMCMC_OK;
+    }
+    if (tgt->kind == TGT_GAUSS)
+        return gauss_dir_products<T>(&w->glmw, &tgt->gauss, c->stream, eps, V, x0, Z, key, Xi, D, Tn, skip,
+                                     &c->prof, iter, msg);
+    return glm_dir_products<T>(&w->glmw, &tgt->glm, c->stream, eps, tgt->prior_scale, V, x0, Z, key, Xi, D, Tn,
+                               skip, &c->prof, iter, msg);
+}
+
+template <typename T>
+static int deer_adjoint_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt, const pmcmc_deer_params* p,
+                          const T* x0, const T* Xi, const T* S, const uint8_t* acc, const T* G, double* grad_out,
+                          T* dx0_out, T* lam_out, pmcmc_deer_info* info) {
+    const int64_t D = w->D, Tn = w->T;
+    std::string msg;
+    GlmData gdummy{};
+    gdummy.N = 1;
+    gdummy.Dp = tgt->kind == TGT_GAUSS ? tgt->gauss.Dp : (D + 15) / 16 * 16;
+    const GlmData* gd = tgt->kind == TGT_LOGISTIC ? &tgt->glm : &gdummy;
+    int rc = glm_work_ensure(&w->glmw, gd, c->stream, D, Tn, 1, sizeof(T) == 8, &msg);
+    if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+    if (!w->adj_lam) {
+        const size_t mat = (size_t)D * Tn * sizeof(T);
+        if (!plan_scan<T>(&w->adj_plan, D, Tn, false))
+            return fail(c, PMCMC_ERR_UNSUPPORTED, "adjoint: D=%lld too large for the scan tile", (long long)D);
+        bool ok = hipMalloc(&w->adj_lam, mat + D * sizeof(T)) == hipSuccess &&
+                  hipMalloc(&w->adj_A, mat) == hipSuccess && hipMalloc(&w->adj_B, mat) == hipSuccess &&
+                  hipMalloc(&w->adj_D, mat) == hipSuccess && hipMalloc(&w->adj_zero, D * sizeof(T)) == hipSuccess &&
+                  hipMemset(w->adj_zero, 0, D * sizeof(T)) == hipSuccess &&
+                  hipMalloc(&w->adj_part, (2 * Tn + 2) * sizeof(double)) == hipSuccess &&
+                  hipMalloc(&w->adj_blob, w->adj_plan.bytes > 0 ? w->adj_plan.bytes : 16) == hipSuccess;
+        if (!ok) return fail(c, PMCMC_ERR_ALLOC, "adjoint workspace allocation failed");
+    }
+    T* lam = (T*)w->adj_lam;
+    T* Ar = (T*)w->adj_A;
+    T* Br = (T*)w->adj_B;
+    T* Dr = (T*)w->adj_D;
+    const int* skip = &w->sync->converged;
+    const uint32_t key = probe_key(p->seed, 0x41444A31u, 0u);
+    const bool exact = tgt->kind == TGT_DIAG;
+    const T* prec = sizeof(T) == 8 ? (const T*)tgt->prec_d : (const T*)tgt->prec_f;
+
+    hipLaunchKernelGGL(reset_sync_kernel, dim3(1), dim3(64), 0, c->stream, w->sync);
+    if (!exact) {
+        rc = adj_products<T>(c, w, tgt, p->eps, S, nullptr, nullptr, key, Xi, nullptr, 0, &msg);
+        if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+    }
+    hipLaunchKernelGGL((adj_coef_kernel<T>), dim3(grid_for(D * Tn + D)), dim3(256), 0, c->stream,
+                       (const T*)w->glmw.W, key, exact ? prec : nullptr, exact, p->eps, acc, D, Tn, Ar, lam);
+    HIPCHK(c, hipGetLastError());
+    const int ublocks = grid_for(D * Tn, 256, 2048);
+    int k = 0, chunk = 1;
+    bool done = false;
+    while (k < p->maxiter && !done) {
+        const int n = (p->maxiter - k) < chunk ? (p->maxiter - k) : chunk;
+        for (int i = 0; i < n; ++i) {
+            const int it = k + i;
+            const int e = c->prof.begin(KT_ITERATION, it, c->stream);
+            rc = adj_products<T>(c, w, tgt, p->eps, S, nullptr, lam + D, 0u, Xi, skip, it, &msg);
+            if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+            hipLaunchKernelGGL((adj_resid_kernel<T>), dim3(grid_for(D * Tn)), dim3(256), 0, c->stream, G,
+                               (const T*)w->glmw.W, (const T*)lam, acc, D, Tn, Br, skip);
+            run_scan_levels<T>(w->adj_plan, 0, Ar, Br, Dr, (const T*)w->adj_zero, (char*)w->adj_blob, skip,
+                               c->stream);
+            hipLaunchKernelGGL((adj_update_kernel<T>), dim3(ublocks), dim3(256), 0, c->stream, (const T*)Dr, lam, D,
+                               Tn, w->sync, it, p->tol_abs, p->tol_rel);
+            c->prof.end(e, c->stream);
+            HIPCHK(c, hipGetLastError());
+        }
+        k += n;
+        HIPCHK(c, hipMemcpyAsync(c->host_sync, w->sync, sizeof(SyncState), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        done = c->host_sync->converged != 0;
+        if (chunk < 16) chunk <<= 1;
+    }
+    const SyncState hs = *c->host_sync;
+    const int iters = hs.converged ? hs.conv_iter + 1 : (hs.last_iter + 1);
+    c->prof.flush(iters - 1);
+    // gradient at Xbar: grad and J^T lambda (for dx0), then the column sums
+    rc = adj_products<T>(c, w, tgt, p->eps, S, x0, (const T*)lam, 0u, Xi, nullptr, iters, &msg);
+    if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+    hipLaunchKernelGGL((adj_grad_kernel<T>), dim3((unsigned)((Tn + 3) / 4)), dim3(256), 0, c->stream, S, x0, Xi,
+                       (const T*)w->glmw.gx, (const T*)w->glmw.W, (const T*)lam, acc, p->eps,
+                       tgt->kind == TGT_LOGISTIC ? 1 : 0, tgt->prior_scale, D, Tn, w->adj_part, dx0_out);
+    hipLaunchKernelGGL(adj_sum_kernel, dim3(1), dim3(256), 0, c->stream, (const double*)w->adj_part, Tn,
+                       w->adj_part + 2 * Tn);
+    HIPCHK(c, hipGetLastError());
+    if (lam_out)
+        HIPCHK(c, hipMemcpyAsync(lam_out, lam, (size_t)D * Tn * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    double host[2];
+    HIPCHK(c, hipMemcpyAsync(host, w->adj_part + 2 * Tn, sizeof host, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    grad_out[0] = host[0];
+    grad_out[1] = host[1];
+    if (info) {
+        info->converged = hs.converged;
+        info->iters = iters;
+        info->metric = hs.last_metric;
+    }
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_deer_adjoint(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
+                                  const pmcmc_deer_params* p, const void* x0, const void* Xi, const void* S,
+                                  const uint8_t* accepted, const void* G, double* grad_out, void* dx0_out,
+                                  void* lam_out, pmcmc_deer_info* info) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    if (!w || !tgt || !p) return fail(c, PMCMC_ERR_ARGUMENT, "null workspace/target/params");
+    if (p->maxiter < 1) return fail(c, PMCMC_ERR_ARGUMENT, "maxiter must be >= 1");
+    if (!(p->tol_abs >= 0)) return fail(c, PMCMC_ERR_ARGUMENT, "tol_abs must be >= 0");
+    if (!(p->tol_rel >= 0)) return fail(c, PMCMC_ERR_ARGUMENT, "tol_rel must be >= 0");
+    if (!(p->eps > 0)) return fail(c, PMCMC_ERR_ARGUMENT, "epsilon must be > 0");
+    if (p->precision != w->prec) return fail(c, PMCMC_ERR_ARGUMENT, "params precision %d does not match the workspace (%d)", p->precision, w->prec);
+    if (tgt->D != w->D) return fail(c, PMCMC_ERR_DIMENSION, "target dim %lld != workspace D %lld", (long long)tgt->D, (long long)w->D);
+    if (!x0 || !Xi || !S || !accepted || !G || !grad_out) return fail(c, PMCMC_ERR_ARGUMENT, "null buffer");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (w->prec == PMCMC_PREC_F64)
+        return deer_adjoint_t<double>(c, w, tgt, p, (const double*)x0, (const double*)Xi, (const double*)S,
+                                      accepted, (const double*)G, grad_out, (double*)dx0_out, (double*)lam_out,
+                                      info);
+    return deer_adjoint_t<float>(c, w, tgt, p, (const float*)x0, (const float*)Xi, (const float*)S, accepted,
+                                 (const float*)G, grad_out, (float*)dx0_out, (float*)lam_out, info);
 }
