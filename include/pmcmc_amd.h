@@ -191,6 +191,38 @@ int pmcmc_deer_adjoint(pmcmc_ctx* ctx, pmcmc_deer_ws* ws, const pmcmc_target* tg
                        const void* S, const uint8_t* accepted, const void* G, double* grad_out,
                        void* dx0_out, void* lambda_out, pmcmc_deer_info* info);
 
+/* ---- many chains in one solve ---------------------------------------------
+ * Chains of uneven D and T (logistic or dense-Gaussian targets, D <= 256),
+ * each solved window by window (`window` steps; window k starts from the
+ * last state of window k-1), all chains' Newton iterations running as one
+ * set of batched launches (work items over per-chain descriptor tables;
+ * each chain stops at its own convergence), an iteration pair captured as
+ * a hipGraph per window round.  params: jacobian = :stoch_diag with one
+ * probe, precision F32 or BF16_JAC, maxiter per window; eps per chain.
+ * The batched counterpart of running src/interface.jl:475-515 once per
+ * chain and block; the reference has no batched or windowed solve.
+ * All pointers in a descriptor are device pointers (FP32); S_out: D x T. */
+typedef struct pmcmc_chain_desc {
+    const pmcmc_target* target;
+    int64_t T;
+    double eps;
+    const void* x0;
+    const void* Xi;
+    const void* U;
+    void* S_out;
+    uint8_t* accepted_out; /* T bytes or NULL */
+} pmcmc_chain_desc;
+
+typedef struct pmcmc_batch pmcmc_batch;
+
+int pmcmc_batch_create(pmcmc_ctx* ctx, int32_t n, const pmcmc_chain_desc* chains,
+                       const pmcmc_deer_params* params, int64_t window, pmcmc_batch** out);
+/* 0: launch the batched kernels directly instead of replaying graphs */
+int pmcmc_batch_set_graph(pmcmc_batch* b, int32_t on);
+/* infos: n HOST records or NULL (iters summed over windows).  Synchronises. */
+int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos);
+int pmcmc_batch_destroy(pmcmc_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

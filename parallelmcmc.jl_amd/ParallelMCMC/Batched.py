@@ -1,0 +1,106 @@
+"""Many chains of uneven D and T in one batched DEER solve.
+
+Each chain is a taped MALA recursion (a DensityModel with a logistic or
+dense-Gaussian GPU target, its own eps, x0 and tape).  The chains are solved
+window by window; within a window round every Newton iteration of every
+chain runs as one set of batched kernel launches over per-chain descriptor
+tables, replayed from a captured hipGraph, and each chain stops at its own
+convergence (include/pmcmc_amd.h, pmcmc_batch_*).  The reference solves one
+chain per call (src/interface.jl:475-515) and has no windowing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _device, _lib
+from ._lib import ArgumentError
+from .interface import _target_of
+
+
+@dataclass
+class Chain:
+    """One chain: model (DensityModel with a GPU target), eps, x0 (D), tape
+    Xi (D x T) and U (T), numpy arrays or device tensors."""
+
+    model: object
+    eps: float
+    x0: object
+    Xi: object
+    U: object
+
+
+class BatchPlan:
+    """Device buffers and descriptor tables for repeated solves of the same
+    chains (the tapes are uploaded once).  ``solve()`` returns per-chain
+    (S, accepted, Info)."""
+
+    def __init__(self, chains, window=4096, precision="bf16_jac", tol_abs=1e-6, tol_rel=1e-5, maxiter=200,
+                 damping=1.0, seed=0, use_graph=True, device=0):
+        from .DEER import Info  # noqa: F401  (imported for the return type)
+
+        if not chains:
+            raise ArgumentError("no chains")
+        prec = {"f32": _lib.PREC_F32, "bf16_jac": _lib.PREC_BF16_JAC}.get(str(precision).lstrip(":"))
+        if prec is None:
+            raise ArgumentError("batched solves run in f32 or bf16_jac")
+        if window < 1 or maxiter < 1 or not (0 < damping <= 1) or tol_abs < 0 or tol_rel < 0:
+            raise ArgumentError("bad window / maxiter / damping / tolerances")
+        self.ctx = _device.context(device)
+        dev = torch.device("cuda", device)
+        self.damping, self.seed = damping, seed
+        self.S, self.acc, self._keep, descs = [], [], [], []
+        for ch in chains:
+            tgt = _target_of(ch.model)
+            Xi = _device.to_colmajor(ch.Xi, torch.float32, dev)
+            D, T = Xi.shape
+            if D != tgt.dim:
+                raise ArgumentError("tape rows must equal the target dimension")
+            U = _device.to_vec(ch.U, torch.float32, dev)
+            x0 = _device.to_vec(ch.x0, torch.float32, dev)
+            if U.numel() != T or x0.numel() != D:
+                raise ArgumentError("U must have T entries and x0 D entries")
+            S = _device.colmajor_empty(D, T, torch.float32, dev)
+            acc = torch.zeros(T, dtype=torch.uint8, device=dev)
+            self.S.append(S)
+            self.acc.append(acc)
+            self._keep += [Xi, U, x0, tgt]
+            descs.append(_lib.ChainDesc(tgt.handle(self.ctx).value, T, float(ch.eps), x0.data_ptr(), Xi.data_ptr(),
+                                        U.data_ptr(), S.data_ptr(), acc.data_ptr()))
+        self.n = len(descs)
+        arr = (_lib.ChainDesc * self.n)(*descs)
+        params = _lib.DeerParams(0.0, float(tol_abs), float(tol_rel), float(damping), int(maxiter),
+                                 _lib.JAC_STOCH_DIAG, 1, prec, int(seed) & 0xFFFFFFFFFFFFFFFF)
+        torch.cuda.synchronize(dev)
+        h = C.c_void_p()
+        lib = _lib.load()
+        _lib.check(lib.pmcmc_batch_create(self.ctx.h, self.n, arr, C.byref(params), int(window), C.byref(h)),
+                   self.ctx.h, "pmcmc_batch_create")
+        self.h = h
+        _lib.check(lib.pmcmc_batch_set_graph(self.h, 1 if use_graph else 0), self.ctx.h, "pmcmc_batch_set_graph")
+
+    def solve(self):
+        from .DEER import Info
+
+        infos = (_lib.DeerInfo * self.n)()
+        torch.cuda.synchronize()
+        _lib.check(_lib.load().pmcmc_batch_solve(self.h, infos), self.ctx.h, "pmcmc_batch_solve")
+        return [(self.S[i], self.acc[i],
+                 Info(bool(infos[i].converged), int(infos[i].iters), float(infos[i].metric), "stoch_diag",
+                      self.damping, 1, self.acc[i]))
+                for i in range(self.n)]
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib.load().pmcmc_batch_destroy(self.h)
+        except Exception:
+            pass
+
+
+def solve_batched(chains, window=4096, precision="bf16_jac", **kw):
+    """One batched solve; returns [(S, accepted, Info)] in chain order."""
+    return BatchPlan(chains, window, precision, **kw).solve()

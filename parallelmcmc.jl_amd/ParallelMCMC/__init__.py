@@ -19,7 +19,7 @@ from .interface import (
 )
 from .targets import DiagGaussianTarget, GaussianTarget, LogisticTarget, StdNormalTarget
 from . import DEER, DEERScan, MALA  # noqa: E402
-from . import Adjoint  # noqa: E402
+from . import Adjoint, Batched  # noqa: E402
 
 __all__ = [
     "DensityModel",
@@ -33,6 +33,7 @@ __all__ = [
     "DEERScan",
     "MALA",
     "Adjoint",
+    "Batched",
     "DiagGaussianTarget",
     "StdNormalTarget",
     "LogisticTarget",

@@ -70,6 +70,21 @@ class DeerInfo(C.Structure):
 
 _P = C.c_void_p
 _I64 = C.c_int64
+class ChainDesc(C.Structure):
+    """pmcmc_chain_desc (include/pmcmc_amd.h)."""
+
+    _fields_ = [
+        ("target", C.c_void_p),
+        ("T", C.c_int64),
+        ("eps", C.c_double),
+        ("x0", C.c_void_p),
+        ("Xi", C.c_void_p),
+        ("U", C.c_void_p),
+        ("S_out", C.c_void_p),
+        ("accepted_out", C.c_void_p),
+    ]
+
+
 _SIGS = {
     "pmcmc_version": ([], C.c_char_p),
     "pmcmc_ctx_create": ([C.c_int, _P, C.POINTER(_P)], C.c_int),
@@ -95,6 +110,11 @@ _SIGS = {
     "pmcmc_deer_ws_destroy": ([_P], C.c_int),
     "pmcmc_deer_solve": ([_P, _P, _P, C.POINTER(DeerParams), _P, _P, _P, _P, _P, _P, C.POINTER(DeerInfo)], C.c_int),
     "pmcmc_target_logdensity_batch": ([_P, _P, C.c_int, _P, _P, _I64], C.c_int),
+    "pmcmc_batch_create": ([_P, C.c_int32, C.POINTER(ChainDesc), C.POINTER(DeerParams), _I64, C.POINTER(_P)],
+                           C.c_int),
+    "pmcmc_batch_set_graph": ([_P, C.c_int32], C.c_int),
+    "pmcmc_batch_solve": ([_P, C.POINTER(DeerInfo)], C.c_int),
+    "pmcmc_batch_destroy": ([_P], C.c_int),
     "pmcmc_deer_adjoint": ([_P, _P, _P, C.POINTER(DeerParams), _P, _P, _P, _P, _P, C.POINTER(C.c_double), _P, _P,
                             C.POINTER(DeerInfo)], C.c_int),
 }

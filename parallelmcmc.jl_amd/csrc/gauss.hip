@@ -78,7 +78,7 @@ void gauss_data_destroy(GaussData* g) {
     *g = GaussData{};
 }
 
-static constexpr int GCB = 32;  // t-columns per column block
+static constexpr int GCB = GAUSS_CB;  // t-columns per column block
 static constexpr int GKC = 32;  // K slice
 
 // Epilogue shared by both paths: per column t, given the products at
@@ -93,11 +93,10 @@ static constexpr int GKC = 32;  // K slice
 // direction tiles (KS = 1, waves 2 and 3) on BF16 MFMA.
 // ---------------------------------------------------------------------------
 template <int DPT, int KS, bool JBF>
-__global__ __launch_bounds__(256) void gauss_pass_mfma_kernel(ColSrc<float> src, const float* L32,
-                                                              const float* mu, int pass, float eps,
-                                                              const float* Xi, float* gout, float* hout,
-                                                              double* lpout, float* Yout, float* Wout,
-                                                              const int* skip) {
+__device__ __forceinline__ void gauss_pass_body(const ColSrc<float>& src, const float* L32, const float* mu,
+                                                int pass, float eps, const float* Xi, float* gout, float* hout,
+                                                double* lpout, float* Yout, float* Wout, const int* skip,
+                                                const int64_t cb) {
     constexpr int Dp = DPT * 16;
     constexpr int NC = 2 * GCB;
     constexpr int CPB = NC / (1 + KS);  // trajectory columns per block
@@ -112,7 +111,6 @@ __global__ __launch_bounds__(256) void gauss_pass_mfma_kernel(ColSrc<float> src,
     const int lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int64_t D = src.D, Tn = src.Tn;
-    const int64_t cb = blockIdx.x;
 
     v4f acc[DPT];
 #pragma unroll
@@ -201,6 +199,25 @@ __global__ __launch_bounds__(256) void gauss_pass_mfma_kernel(ColSrc<float> src,
     lp += __shfl_xor(lp, 16, 64);
     lp += __shfl_xor(lp, 32, 64);
     if (set == 0 && lk == 0 && tv) lpout[t] = lp;
+}
+
+template <int DPT, int KS, bool JBF>
+__global__ __launch_bounds__(256) void gauss_pass_mfma_kernel(ColSrc<float> src, const float* L32,
+                                                              const float* mu, int pass, float eps,
+                                                              const float* Xi, float* gout, float* hout,
+                                                              double* lpout, float* Yout, float* Wout,
+                                                              const int* skip) {
+    gauss_pass_body<DPT, KS, JBF>(src, L32, mu, pass, eps, Xi, gout, hout, lpout, Yout, Wout, skip, blockIdx.x);
+}
+
+// Batched over chains of one padded width: work item blockIdx.x is
+// (chain, column block) = items[blockIdx.x].
+template <int DPT, bool JBF>
+__global__ __launch_bounds__(256) void gauss_pass_mfma_batched(const GaussPassArgs* args, const int2* items) {
+    const int2 w = items[blockIdx.x];
+    const GaussPassArgs& a = args[w.x];
+    gauss_pass_body<DPT, 1, JBF>(a.src, a.L32, a.mu, a.pass, a.eps, a.Xi, a.gout, a.hout, a.lpout, a.Yout, a.Wout,
+                                 a.skip, w.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -346,6 +363,30 @@ template int gauss_newton_products<float>(GlmWork*, const GaussData*, hipStream_
 template int gauss_newton_products<double>(GlmWork*, const GaussData*, hipStream_t, SyncState*, int, double,
                                            const double*, const double*, const double*, uint64_t, int, int,
                                            int64_t, int64_t, Prof*, std::string*);
+
+template <int DPT>
+static void launch_gauss_batched_dpt(bool jbf, const GaussPassArgs* args, const int2* items, int nitems,
+                                     hipStream_t s) {
+    if (jbf)
+        hipLaunchKernelGGL((gauss_pass_mfma_batched<DPT, true>), dim3((unsigned)nitems), dim3(256), 0, s, args,
+                           items);
+    else
+        hipLaunchKernelGGL((gauss_pass_mfma_batched<DPT, false>), dim3((unsigned)nitems), dim3(256), 0, s, args,
+                           items);
+}
+
+void launch_gauss_pass_batched(int dpt, bool jbf, const GaussPassArgs* args, const int2* items, int nitems,
+                               hipStream_t s) {
+    if (nitems <= 0) return;
+    switch (dpt) {
+#define PMCMC_BGDPT(n) \
+    case n: launch_gauss_batched_dpt<n>(jbf, args, items, nitems, s); break;
+        PMCMC_BGDPT(2) PMCMC_BGDPT(4) PMCMC_BGDPT(6) PMCMC_BGDPT(8) PMCMC_BGDPT(10) PMCMC_BGDPT(12)
+        PMCMC_BGDPT(14) PMCMC_BGDPT(16)
+#undef PMCMC_BGDPT
+        default: break;
+    }
+}
 
 template <typename T>
 int gauss_dir_products(GlmWork* w, const GaussData* g, hipStream_t s, double eps, const T* V, const T* x0,

@@ -12,6 +12,8 @@
 
 namespace pmcmc {
 
+static constexpr int GAUSS_CB = 32;  // trajectory columns per workgroup of the Lambda pass (one probe set)
+
 struct GaussData {
     int64_t D = 0, Dp = 0;
     float* L32 = nullptr;    // Dp x Dp row-major, zero padded
@@ -19,6 +21,26 @@ struct GaussData {
     float* mu32 = nullptr;   // null: zero mean
     double* mu64 = nullptr;
 };
+
+// One chain's operands of a batched Lambda pass (one probe set).
+struct GaussPassArgs {
+    ColSrc<float> src;
+    const float* L32;
+    const float* mu;
+    int pass;
+    float eps;
+    const float* Xi;
+    float* gout;
+    float* hout;
+    double* lpout;
+    float* Yout;
+    float* Wout;
+    const int* skip;
+};
+
+// Batched FP32 Lambda passes (one probe set) over chains of padded width 16 * dpt.
+void launch_gauss_pass_batched(int dpt, bool jbf, const GaussPassArgs* args, const int2* items, int nitems,
+                               hipStream_t s);
 
 int gauss_data_create(GaussData* g, hipStream_t s, int64_t D, bool src_f64, const void* Lam,
                       const void* mean, std::string* msg);

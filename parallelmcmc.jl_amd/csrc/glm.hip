@@ -114,7 +114,7 @@ void glm_data_destroy(GlmData* g) {
 // ---------------------------------------------------------------------------
 // work buffers
 // ---------------------------------------------------------------------------
-static constexpr int CB = 32;   // t-columns per column block of the MFMA pass
+static constexpr int CB = GLM_CB;  // t-columns per column block of the MFMA pass
 static constexpr int NB = 32;   // design rows per LDS chunk
 
 static int glm_splits(int64_t N, int64_t T) {
@@ -127,6 +127,16 @@ static int glm_splits(int64_t N, int64_t T) {
     if (ns > 64) ns = 64;
     return (int)ns;
 }
+
+void glm_split_plan(const GlmData* g, int64_t Tn, int* ns, int64_t* rows_per_split) {
+    *ns = glm_splits(g->N, Tn);
+    const int64_t nch = (g->N + NB - 1) / NB;
+    *rows_per_split = ((nch + *ns - 1) / *ns) * NB;
+}
+
+void launch_glm_pass_batched(int dpt, bool jbf, const GlmPassArgs* args, const int4* items, int nitems,
+                             hipStream_t s);
+void launch_glm_reduce_batched(const GlmReduceArgs* args, const int2* items, int nitems, hipStream_t s);
 
 int glm_work_ensure(GlmWork* w, const GlmData* g, hipStream_t, int64_t D, int64_t T, int probes,
                     bool f64, std::string* msg) {
@@ -181,11 +191,9 @@ void glm_work_destroy(GlmWork* w) {
 // (float4) while the current one is multiplied, then stored into the other
 // LDS buffer: two barriers per chunk, global latency off the critical path.
 template <int DPT, int K, bool JBF>
-__global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, const float* Xrm,
-                                                            const float* y32, int64_t N,
-                                                            int64_t rows_per_split,
-                                                            float* part, double* llpart,
-                                                            const int* skip) {
+__device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const float* Xrm, const float* y32,
+                                              int64_t N, int64_t rows_per_split, float* part, double* llpart,
+                                              const int* skip, const int64_t cb, const int split) {
     constexpr int Dp = DPT * 16;
     constexpr int NC = CB * (1 + K);
     constexpr int SV = Dp + 2;
@@ -208,8 +216,6 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
     const int li = lane & 15;  // row (A) / column (B, C) within a tile
     const int lk = lane >> 4;  // k index within a 4-step (A, B); row group (C)
     const int64_t D = src.D, T = src.Tn;
-    const int64_t cb = blockIdx.x;
-    const int split = blockIdx.y;
     const int64_t n_begin = (int64_t)split * rows_per_split;
     int64_t n_end = n_begin + rows_per_split;
     if (n_end > N) n_end = N;
@@ -430,6 +436,24 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
     }
 }
 
+template <int DPT, int K, bool JBF>
+__global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, const float* Xrm,
+                                                            const float* y32, int64_t N,
+                                                            int64_t rows_per_split,
+                                                            float* part, double* llpart,
+                                                            const int* skip) {
+    glm_pass_body<DPT, K, JBF>(src, Xrm, y32, N, rows_per_split, part, llpart, skip, blockIdx.x, blockIdx.y);
+}
+
+// Batched over chains of one padded width: work item blockIdx.x is
+// (chain, column block, N split) = items[blockIdx.x].
+template <int DPT, int K, bool JBF>
+__global__ __launch_bounds__(256) void glm_pass_mfma_batched(const GlmPassArgs* args, const int4* items) {
+    const int4 w = items[blockIdx.x];
+    const GlmPassArgs& a = args[w.x];
+    glm_pass_body<DPT, K, JBF>(a.src, a.Xrm, a.y32, a.N, a.rows_per_split, a.part, a.llpart, a.skip, w.y, w.z);
+}
+
 // Sum the splits, add the prior terms, and form the next pass's columns.
 //   pass 0 (at Xbar): gx = G0 - x/s^2 ; hz_k = -G_k - z_k/s^2 ;
 //                     Y = x + eps gx + sqrt(2 eps) xi ; W_k = z_k + eps hz_k ;
@@ -437,14 +461,13 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
 //   pass 1 (at Y):    gy = G0 - y/s^2 ; hw_k = -G_k - w_k/s^2 ; lpy likewise
 // One wave per column.
 template <typename T, typename P>
-__global__ __launch_bounds__(256) void glm_reduce_kernel(ColSrc<T> src, const P* part, const double* llpart,
-                                                         int ns, int64_t Dp, int64_t llstride, int pass,
-                                                         double inv_s2, double eps, const T* Xi, T* gout,
-                                                         T* hout, double* lpout, T* Yout, T* Wout,
-                                                         const int* skip) {
+__device__ __forceinline__ void glm_reduce_body(const ColSrc<T>& src, const P* part, const double* llpart, int ns,
+                                                int64_t Dp, int64_t llstride, int pass, double inv_s2, double eps,
+                                                const T* Xi, T* gout, T* hout, double* lpout, T* Yout, T* Wout,
+                                                const int* skip, const int64_t blk) {
     if (skip && *(volatile const int*)skip) return;
     const int lane = threadIdx.x & 63;
-    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t t = blk * 4 + (threadIdx.x >> 6);
     const int64_t D = src.D, Tn = src.Tn;
     const int K = src.K;
     if (t >= Tn) return;
@@ -479,6 +502,24 @@ __global__ __launch_bounds__(256) void glm_reduce_kernel(ColSrc<T> src, const P*
         for (int s = 0; s < ns; ++s) ll += llpart[(size_t)s * llstride + t];
         lpout[t] = ll - 0.5 * nrm * inv_s2;
     }
+}
+
+template <typename T, typename P>
+__global__ __launch_bounds__(256) void glm_reduce_kernel(ColSrc<T> src, const P* part, const double* llpart,
+                                                         int ns, int64_t Dp, int64_t llstride, int pass,
+                                                         double inv_s2, double eps, const T* Xi, T* gout,
+                                                         T* hout, double* lpout, T* Yout, T* Wout,
+                                                         const int* skip) {
+    glm_reduce_body<T, P>(src, part, llpart, ns, Dp, llstride, pass, inv_s2, eps, Xi, gout, hout, lpout, Yout,
+                          Wout, skip, blockIdx.x);
+}
+
+// Batched: work item blockIdx.x is (chain, block of 4 columns) = items[blockIdx.x].
+__global__ __launch_bounds__(256) void glm_reduce_batched(const GlmReduceArgs* args, const int2* items) {
+    const int2 w = items[blockIdx.x];
+    const GlmReduceArgs& a = args[w.x];
+    glm_reduce_body<float, float>(a.src, a.part, a.llpart, a.ns, a.Dp, a.llstride, a.pass, a.inv_s2, a.eps, a.Xi,
+                                  a.gout, a.hout, a.lpout, a.Yout, a.Wout, a.skip, w.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -650,6 +691,35 @@ template int glm_newton_products<double>(GlmWork*, const GlmData*, hipStream_t, 
                                          double, double, const double*, const double*,
                                          const double*, uint64_t, int, int, int64_t, int64_t, Prof*,
                                          std::string*);
+
+template <int DPT>
+static void launch_glm_batched_dpt(bool jbf, const GlmPassArgs* args, const int4* items, int nitems,
+                                   hipStream_t s) {
+    if (jbf)
+        hipLaunchKernelGGL((glm_pass_mfma_batched<DPT, 1, true>), dim3((unsigned)nitems), dim3(256), 0, s, args,
+                           items);
+    else
+        hipLaunchKernelGGL((glm_pass_mfma_batched<DPT, 1, false>), dim3((unsigned)nitems), dim3(256), 0, s, args,
+                           items);
+}
+
+void launch_glm_pass_batched(int dpt, bool jbf, const GlmPassArgs* args, const int4* items, int nitems,
+                             hipStream_t s) {
+    if (nitems <= 0) return;
+    switch (dpt) {
+#define PMCMC_BDPT(n) \
+    case n: launch_glm_batched_dpt<n>(jbf, args, items, nitems, s); break;
+        PMCMC_BDPT(1) PMCMC_BDPT(2) PMCMC_BDPT(3) PMCMC_BDPT(4) PMCMC_BDPT(5) PMCMC_BDPT(6) PMCMC_BDPT(7)
+        PMCMC_BDPT(8) PMCMC_BDPT(9) PMCMC_BDPT(10) PMCMC_BDPT(11) PMCMC_BDPT(12) PMCMC_BDPT(13) PMCMC_BDPT(14)
+        PMCMC_BDPT(15) PMCMC_BDPT(16)
+#undef PMCMC_BDPT
+        default: break;
+    }
+}
+
+void launch_glm_reduce_batched(const GlmReduceArgs* args, const int2* items, int nitems, hipStream_t s) {
+    if (nitems > 0) hipLaunchKernelGGL(glm_reduce_batched, dim3((unsigned)nitems), dim3(256), 0, s, args, items);
+}
 
 // One pass at the states V (shifted by x0 when given) with one direction set
 // (Z, or the probe with `key` when Z is null): w->gx = grad, w->W = dir + eps H dir.

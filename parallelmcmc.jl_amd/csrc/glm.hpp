@@ -11,6 +11,8 @@
 
 namespace pmcmc {
 
+static constexpr int GLM_CB = 32;  // trajectory columns per workgroup of the FP32 GEMM pass (one probe set)
+
 struct GlmData {
     int64_t N = 0, D = 0;
     int64_t Dp = 0;         // D padded to the MFMA K granularity
@@ -59,6 +61,45 @@ struct GlmWork {
     void* part = nullptr;  // split-N partial sums
     size_t part_bytes = 0;
 };
+
+// One chain's operands of a batched FP32 GEMM pass (glm_pass_mfma_batched).
+struct GlmPassArgs {
+    ColSrc<float> src;
+    const float* Xrm;
+    const float* y32;
+    int64_t N;
+    int64_t rows_per_split;
+    float* part;
+    double* llpart;
+    const int* skip;
+};
+
+// One chain's operands of a batched split-sum / epilogue (glm_reduce_batched).
+struct GlmReduceArgs {
+    ColSrc<float> src;
+    const float* part;
+    const double* llpart;
+    int ns;
+    int64_t Dp, llstride;
+    int pass;
+    double inv_s2, eps;
+    const float* Xi;
+    float* gout;
+    float* hout;
+    double* lpout;
+    float* Yout;
+    float* Wout;
+    const int* skip;
+};
+
+// N-split plan of the FP32 GEMM pass over Tn columns (ns splits of
+// rows_per_split rows); the pass's partial sums take ns * 2 * Tn * Dp floats
+// followed by ns * Tn doubles of log-likelihood partials.
+void glm_split_plan(const GlmData* g, int64_t Tn, int* ns, int64_t* rows_per_split);
+// Batched FP32 passes (one probe set) over chains of padded width 16 * dpt.
+void launch_glm_pass_batched(int dpt, bool jbf, const GlmPassArgs* args, const int4* items, int nitems,
+                             hipStream_t s);
+void launch_glm_reduce_batched(const GlmReduceArgs* args, const int2* items, int nitems, hipStream_t s);
 
 int glm_data_create(GlmData* g, hipStream_t s, int64_t N, int64_t D, bool src_f64, const void* Xd,
                     const void* y, std::string* msg);

@@ -246,20 +246,6 @@ static int grid_for(int64_t n, int block = 256, int cap = 8192) {
 // Tile shape of one level: TC columns so that TC*D is about the per-type
 // element budget, shrunk until the LDS footprint fits 63 KB.
 template <typename T>
-static void tile_shape(int64_t D, int64_t Tn, bool with_iterate, int* TC, int* P, int* numTiles) {
-    int64_t E = DevTraits<T>::tile_elems;
-    int64_t tc = E / D;
-    if (tc < 1) tc = 1;
-    while (tc > 1 && tile_lds_bytes<T>(D, (int)tc, with_iterate) > 64512) tc >>= 1;
-    if (tc > Tn && Tn > 0) tc = Tn;
-    *TC = (int)tc;
-    *P = D >= PMCMC_NT ? 1 : (int)(PMCMC_NT / D);
-    if (*P > tc) *P = (int)tc;
-    if (*P < 1) *P = 1;
-    *numTiles = (int)((Tn + tc - 1) / tc);
-}
-
-template <typename T>
 static bool plan_scan(ScanPlan* p, int64_t D, int64_t Tn, bool deer_top) {
     p->D = D;
     p->nlev = 0;
@@ -1017,4 +1003,608 @@ extern "C" int pmcmc_deer_adjoint(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_ta
                                       info);
     return deer_adjoint_t<float>(c, w, tgt, p, (const float*)x0, (const float*)Xi, (const float*)S, accepted,
                                  (const float*)G, grad_out, (float*)dx0_out, (float*)lam_out, info);
+}
+
+// ---------------------------------------------------------------------------
+// batched solve of many chains (uneven D, T, targets) in windows
+// ---------------------------------------------------------------------------
+// Every kernel of an iteration is one launch over all active chains: the
+// work items (chain, tile) / (chain, column block, N split) index per-chain
+// descriptor tables in device memory, GEMM passes grouped by padded width.
+// Each chain's convergence flag turns its items into no-ops; the host only
+// reads the flags between chunks of iterations.  An iteration pair (the
+// iterate double buffer alternates) is captured once per window round as a
+// hipGraph and replayed.
+namespace pmcmc {
+
+struct BInit {
+    const float* x0;  // window start state (user x0 or the previous window's last column)
+    float* S0;        // iterate buffer 0
+    int64_t D, n;     // n = D * L
+    SyncState* sy;
+};
+
+struct BCopy {
+    const float* b0;
+    const float* b1;
+    float* dst;
+    int64_t n;
+    const SyncState* sy;
+};
+
+__global__ __launch_bounds__(256) void batch_init_kernel(const BInit* t) {
+    const BInit& a = t[blockIdx.y];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        SyncState* s = a.sy;
+        s->done = 0u;
+        s->err = 0u;
+        s->dmax_bits = 0ull;
+        s->smax_bits = 0ull;
+        s->converged = 0;
+        s->conv_iter = -1;
+        s->last_metric = INFINITY;
+        s->last_iter = -1;
+    }
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256) a.S0[i] = a.x0[i % a.D];
+}
+
+// the converged iterate of iteration k sits in buffer (k + 1) & 1
+__global__ __launch_bounds__(256) void batch_copy_kernel(const BCopy* t) {
+    const BCopy& a = t[blockIdx.y];
+    const SyncState* s = a.sy;
+    const int k = s->converged ? s->conv_iter : s->last_iter;
+    const float* src = (k >= 0 && ((k + 1) & 1)) ? a.b1 : a.b0;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256) a.dst[i] = src[i];
+}
+
+// Scan of the level-0 tile aggregates of each chain (one workgroup per chain,
+// rows over threads): prev[:, k] = state after tile k, from s0.
+__global__ __launch_bounds__(256) void batch_agg_scan_kernel(const TileArgs<float>* args, const int* chains) {
+    const TileArgs<float>& a = args[chains[blockIdx.x]];
+    if (a.skip && *(volatile const int*)a.skip) return;
+    const int64_t D = a.D;
+    const int nT = a.numTiles;
+    float* prev = const_cast<float*>(a.prev);
+    for (int64_t d = threadIdx.x; d < D; d += 256) {
+        float s = a.s0[d];
+        int k = 0;
+        for (; k + 4 <= nT; k += 4) {
+            float aa[4], bb[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                aa[q] = a.agg_a[(size_t)(k + q) * D + d];
+                bb[q] = a.agg_b[(size_t)(k + q) * D + d];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                s = aa[q] * s + bb[q];
+                prev[(size_t)(k + q) * D + d] = s;
+            }
+        }
+        for (; k < nT; ++k) {
+            s = a.agg_a[(size_t)k * D + d] * s + a.agg_b[(size_t)k * D + d];
+            prev[(size_t)k * D + d] = s;
+        }
+    }
+}
+
+}  // namespace pmcmc
+
+struct BatchChain {
+    const pmcmc_target* tgt = nullptr;
+    int64_t D = 0, T = 0, Dp = 0;
+    double eps = 0;
+    const float* x0 = nullptr;
+    const float* Xi = nullptr;
+    const float* U = nullptr;
+    float* S_out = nullptr;
+    uint8_t* acc_out = nullptr;
+    // device state (carved from the arena)
+    float* buf[2] = {nullptr, nullptr};
+    float *Y = nullptr, *gx = nullptr, *gy = nullptr, *W = nullptr, *hw = nullptr;
+    double *lpx = nullptr, *lpy = nullptr;
+    float* part = nullptr;  // logistic only
+    float *agg_a = nullptr, *agg_b = nullptr, *prev = nullptr;
+    int ns = 1;
+    int64_t rows_per_split = 0;
+    // per-solve results
+    int iters = 0;
+    bool converged = true;
+    double metric = 0;
+};
+
+struct pmcmc_batch {
+    pmcmc_ctx* ctx = nullptr;
+    std::vector<BatchChain> ch;
+    pmcmc_deer_params p{};
+    int64_t W = 0;
+    bool jbf = false;
+    bool use_graph = true;
+    void* arena = nullptr;
+    SyncState* syncs = nullptr;  // n, device
+    SyncState* hsync = nullptr;  // n, pinned host
+    void* tables = nullptr;      // device descriptor tables
+    size_t tables_cap = 0;
+};
+
+namespace {
+
+struct RoundPlan {
+    // host mirrors of the device tables of one window round
+    std::vector<TileArgs<float>> targs[2];
+    std::vector<GlmPassArgs> gpass[3];  // pass 0 parity 0, pass 0 parity 1, pass 1
+    std::vector<GlmReduceArgs> gred[3];
+    std::vector<GaussPassArgs> qpass[3];
+    std::vector<int2> red_items, single_items, glm_red_items;
+    std::vector<int> multi_chains;
+    std::vector<int4> glm_items[17];
+    std::vector<int2> gauss_items[17];
+    std::vector<BInit> inits;
+    std::vector<BCopy> copies;
+    std::vector<int> active;
+    size_t lds_multi = 0, lds_single = 0;
+    int64_t max_init = 0;
+};
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+}  // namespace
+
+static void batch_free(pmcmc_batch* b) {
+    hipFree(b->arena);
+    hipFree(b->syncs);
+    hipHostFree(b->hsync);
+    hipFree(b->tables);
+    b->arena = nullptr;
+    b->syncs = nullptr;
+    b->hsync = nullptr;
+    b->tables = nullptr;
+}
+
+extern "C" int pmcmc_batch_destroy(pmcmc_batch* b) {
+    if (!b) return PMCMC_OK;
+    if (b->ctx) {
+        hipSetDevice(b->ctx->device);
+        hipStreamSynchronize(b->ctx->stream);
+    }
+    batch_free(b);
+    delete b;
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_desc* chains,
+                                  const pmcmc_deer_params* p, int64_t window, pmcmc_batch** out) {
+    if (!c || !out) return PMCMC_ERR_ARGUMENT;
+    *out = nullptr;
+    if (n < 1 || !chains || !p) return fail(c, PMCMC_ERR_ARGUMENT, "batch: need n >= 1 chains and params");
+    if (window < 1) return fail(c, PMCMC_ERR_ARGUMENT, "batch: window must be >= 1");
+    if (p->maxiter < 1) return fail(c, PMCMC_ERR_ARGUMENT, "maxiter must be >= 1");
+    if (!(p->tol_abs >= 0) || !(p->tol_rel >= 0)) return fail(c, PMCMC_ERR_ARGUMENT, "tolerances must be >= 0");
+    if (!(p->damping > 0 && p->damping <= 1)) return fail(c, PMCMC_ERR_ARGUMENT, "damping must be in (0,1]");
+    if (p->jacobian != PMCMC_JAC_STOCH_DIAG || p->probes != 1)
+        return fail(c, PMCMC_ERR_UNSUPPORTED, "batch: jacobian=:stoch_diag with one probe");
+    if (p->precision != PMCMC_PREC_F32 && p->precision != PMCMC_PREC_BF16_JAC)
+        return fail(c, PMCMC_ERR_UNSUPPORTED, "batch: FP32 or BF16-Jacobian precision");
+    HIPCHK(c, hipSetDevice(c->device));
+    pmcmc_batch* b = new (std::nothrow) pmcmc_batch();
+    if (!b) return fail(c, PMCMC_ERR_ALLOC, "host allocation failed");
+    b->ctx = c;
+    b->p = *p;
+    b->W = window;
+    b->jbf = p->precision == PMCMC_PREC_BF16_JAC;
+    b->ch.resize(n);
+    size_t bytes = 0;
+    std::vector<size_t> off(n);
+    for (int i = 0; i < n; ++i) {
+        const pmcmc_chain_desc& d = chains[i];
+        BatchChain& h = b->ch[i];
+        if (!d.target || d.T < 1 || !(d.eps > 0) || !d.x0 || !d.Xi || !d.U || !d.S_out) {
+            delete b;
+            return fail(c, PMCMC_ERR_ARGUMENT, "batch: chain %d: bad descriptor", i);
+        }
+        if (d.target->kind != TGT_LOGISTIC && d.target->kind != TGT_GAUSS) {
+            delete b;
+            return fail(c, PMCMC_ERR_UNSUPPORTED, "batch: chain %d: logistic or dense Gaussian targets only", i);
+        }
+        h.tgt = d.target;
+        h.D = d.target->D;
+        h.T = d.T;
+        h.eps = d.eps;
+        h.x0 = (const float*)d.x0;
+        h.Xi = (const float*)d.Xi;
+        h.U = (const float*)d.U;
+        h.S_out = (float*)d.S_out;
+        h.acc_out = d.accepted_out;
+        h.Dp = d.target->kind == TGT_GAUSS ? d.target->gauss.Dp : d.target->glm.Dp;
+        if (h.Dp > 256) {
+            delete b;
+            return fail(c, PMCMC_ERR_UNSUPPORTED, "batch: chain %d: D > 256", i);
+        }
+        const int64_t L = window < d.T ? window : d.T;
+        int TC, P, nT;
+        tile_shape<float>(h.D, L, true, &TC, &P, &nT);
+        if (tile_lds_bytes<float>(h.D, TC, true) > 64512) {
+            delete b;
+            return fail(c, PMCMC_ERR_UNSUPPORTED, "batch: chain %d: D too large for the tile kernel", i);
+        }
+        size_t mat = align256((size_t)h.D * L * sizeof(float));
+        size_t sz = 7 * mat + 2 * align256((size_t)L * sizeof(double)) + 3 * align256((size_t)h.D * nT * sizeof(float));
+        if (d.target->kind == TGT_LOGISTIC) {
+            glm_split_plan(&d.target->glm, L, &h.ns, &h.rows_per_split);
+            sz += align256((size_t)h.ns * 2 * L * h.Dp * sizeof(float) + (size_t)h.ns * L * sizeof(double));
+        }
+        off[i] = bytes;
+        bytes += sz;
+    }
+    bool ok = hipMalloc(&b->arena, bytes > 0 ? bytes : 256) == hipSuccess &&
+              hipMalloc(&b->syncs, n * sizeof(SyncState)) == hipSuccess &&
+              hipMemset(b->syncs, 0, n * sizeof(SyncState)) == hipSuccess &&
+              hipHostMalloc(&b->hsync, n * sizeof(SyncState)) == hipSuccess;
+    if (!ok) {
+        batch_free(b);
+        delete b;
+        return fail(c, PMCMC_ERR_ALLOC, "batch: device allocation failed (%zu bytes)", bytes);
+    }
+    for (int i = 0; i < n; ++i) {
+        BatchChain& h = b->ch[i];
+        const int64_t L = window < h.T ? window : h.T;
+        int TC, P, nT;
+        tile_shape<float>(h.D, L, true, &TC, &P, &nT);
+        char* q = (char*)b->arena + off[i];
+        const size_t mat = align256((size_t)h.D * L * sizeof(float));
+        auto take = [&](size_t s) {
+            char* r = q;
+            q += s;
+            return r;
+        };
+        h.buf[0] = (float*)take(mat);
+        h.buf[1] = (float*)take(mat);
+        h.Y = (float*)take(mat);
+        h.gx = (float*)take(mat);
+        h.gy = (float*)take(mat);
+        h.W = (float*)take(mat);
+        h.hw = (float*)take(mat);
+        h.lpx = (double*)take(align256((size_t)L * sizeof(double)));
+        h.lpy = (double*)take(align256((size_t)L * sizeof(double)));
+        const size_t agg = align256((size_t)h.D * nT * sizeof(float));
+        h.agg_a = (float*)take(agg);
+        h.agg_b = (float*)take(agg);
+        h.prev = (float*)take(agg);
+        if (h.tgt->kind == TGT_LOGISTIC) h.part = (float*)q;
+    }
+    *out = b;
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_batch_set_graph(pmcmc_batch* b, int32_t on) {
+    if (!b) return PMCMC_ERR_ARGUMENT;
+    b->use_graph = on != 0;
+    return PMCMC_OK;
+}
+
+// Tables of window round r; returns false when no chain is active.
+static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
+    const pmcmc_deer_params& p = b->p;
+    const int n = (int)b->ch.size();
+    for (int i = 0; i < n; ++i) {
+        BatchChain& h = b->ch[i];
+        const int64_t t0 = r * b->W;
+        if (t0 >= h.T) continue;
+        const int64_t L = (h.T - t0) < b->W ? (h.T - t0) : b->W;
+        const int k = (int)rp->active.size();
+        rp->active.push_back(i);
+        const int64_t D = h.D;
+        const float* s0 = t0 == 0 ? h.x0 : h.S_out + (size_t)(t0 - 1) * D;
+        const float* Xi = h.Xi + (size_t)t0 * D;
+        const float* U = h.U + t0;
+        SyncState* sy = b->syncs + i;
+        const int* skip = &sy->converged;
+        // a fresh probe per chain and window, fixed over the window's iterations
+        const uint64_t seed = p.seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(i + 1)) ^ ((uint64_t)r << 40);
+        const uint32_t key = probe_key(seed, 0xFFFFFFFFu, 0u);
+        int TC, P, nT;
+        tile_shape<float>(D, L, true, &TC, &P, &nT);
+        for (int par = 0; par < 2; ++par) {
+            TileArgs<float> a{};
+            a.s0 = s0;
+            a.D = D;
+            a.T_len = L;
+            a.TC = TC;
+            a.P = P;
+            a.numTiles = nT;
+            a.skip = skip;
+            a.sync = sy;
+            a.S_in = h.buf[par];
+            a.S = h.buf[par ^ 1];
+            a.Xi = Xi;
+            a.U = U;
+            a.acc_out = h.acc_out ? h.acc_out + t0 : nullptr;
+            a.la_out = nullptr;
+            a.eps = h.eps;
+            a.damping = p.damping;
+            a.tol_abs = p.tol_abs;
+            a.tol_rel = p.tol_rel;
+            a.iter = -1;
+            a.probes = 1;
+            a.jacobian = JAC_STOCH;
+            a.seed = seed;
+            a.Y = h.Y;
+            a.gx = h.gx;
+            a.gy = h.gy;
+            a.W = h.W;
+            a.hw = h.hw;
+            a.lpx = h.lpx;
+            a.lpy = h.lpy;
+            a.agg_a = h.agg_a;
+            a.agg_b = h.agg_b;
+            a.prev = h.prev;
+            rp->targs[par].push_back(a);
+        }
+        const size_t lds = tile_lds_bytes<float>(D, TC, true);
+        if (nT > 1) {
+            for (int t = 0; t < nT; ++t) rp->red_items.push_back(make_int2(k, t));
+            rp->multi_chains.push_back(k);
+            if (lds > rp->lds_multi) rp->lds_multi = lds;
+        } else {
+            rp->single_items.push_back(make_int2(k, 0));
+            if (lds > rp->lds_single) rp->lds_single = lds;
+        }
+        // products
+        ColSrc<float> s0src{};
+        s0src.V = nullptr;
+        s0src.x0 = s0;
+        s0src.Z = nullptr;
+        s0src.keys[0] = key;
+        s0src.K = 1;
+        s0src.D = D;
+        s0src.Tn = L;
+        ColSrc<float> s1src = s0src;
+        s1src.V = h.Y;
+        s1src.x0 = nullptr;
+        s1src.Z = h.W;
+        const int ncb = (int)((L + GLM_CB - 1) / GLM_CB);
+        if (h.tgt->kind == TGT_LOGISTIC) {
+            const GlmData& g = h.tgt->glm;
+            const double inv_s2 = 1.0 / (h.tgt->prior_scale * h.tgt->prior_scale);
+            double* llpart = (double*)((char*)h.part + (size_t)h.ns * 2 * L * h.Dp * sizeof(float));
+            for (int v = 0; v < 3; ++v) {
+                GlmPassArgs ga{};
+                ga.src = v < 2 ? s0src : s1src;
+                if (v < 2) ga.src.V = h.buf[v];
+                ga.Xrm = g.Xrm;
+                ga.y32 = g.y;
+                ga.N = g.N;
+                ga.rows_per_split = h.rows_per_split;
+                ga.part = h.part;
+                ga.llpart = llpart;
+                ga.skip = skip;
+                rp->gpass[v].push_back(ga);
+                GlmReduceArgs gr{};
+                gr.src = ga.src;
+                gr.part = h.part;
+                gr.llpart = llpart;
+                gr.ns = h.ns;
+                gr.Dp = h.Dp;
+                gr.llstride = L;
+                gr.pass = v < 2 ? 0 : 1;
+                gr.inv_s2 = inv_s2;
+                gr.eps = h.eps;
+                gr.Xi = Xi;
+                gr.gout = v < 2 ? h.gx : h.gy;
+                gr.hout = h.hw;
+                gr.lpout = v < 2 ? h.lpx : h.lpy;
+                gr.Yout = v < 2 ? h.Y : nullptr;
+                gr.Wout = v < 2 ? h.W : nullptr;
+                gr.skip = skip;
+                rp->gred[v].push_back(gr);
+                rp->qpass[v].push_back(GaussPassArgs{});
+            }
+            const int kg = (int)rp->gpass[0].size() - 1;
+            const int dpt = (int)(h.Dp / 16);
+            for (int cb = 0; cb < ncb; ++cb)
+                for (int sp = 0; sp < h.ns; ++sp) rp->glm_items[dpt].push_back(make_int4(kg, cb, sp, 0));
+            for (int64_t blk = 0; blk < (L + 3) / 4; ++blk) rp->glm_red_items.push_back(make_int2(kg, (int)blk));
+        } else {
+            const GaussData& g = h.tgt->gauss;
+            for (int v = 0; v < 3; ++v) {
+                GaussPassArgs qa{};
+                qa.src = v < 2 ? s0src : s1src;
+                if (v < 2) qa.src.V = h.buf[v];
+                qa.L32 = g.L32;
+                qa.mu = g.mu32;
+                qa.pass = v < 2 ? 0 : 1;
+                qa.eps = (float)h.eps;
+                qa.Xi = Xi;
+                qa.gout = v < 2 ? h.gx : h.gy;
+                qa.hout = h.hw;
+                qa.lpout = v < 2 ? h.lpx : h.lpy;
+                qa.Yout = v < 2 ? h.Y : nullptr;
+                qa.Wout = v < 2 ? h.W : nullptr;
+                qa.skip = skip;
+                rp->qpass[v].push_back(qa);
+                rp->gpass[v].push_back(GlmPassArgs{});
+                rp->gred[v].push_back(GlmReduceArgs{});
+            }
+            const int kq = (int)rp->qpass[0].size() - 1;
+            const int dpt = (int)(h.Dp / 16);
+            const int gcb = (int)((L + GAUSS_CB - 1) / GAUSS_CB);
+            for (int cb = 0; cb < gcb; ++cb) rp->gauss_items[dpt].push_back(make_int2(kq, cb));
+        }
+        BInit in{};
+        in.x0 = s0;
+        in.S0 = h.buf[0];
+        in.D = D;
+        in.n = D * L;
+        in.sy = sy;
+        rp->inits.push_back(in);
+        BCopy cp{};
+        cp.b0 = h.buf[0];
+        cp.b1 = h.buf[1];
+        cp.dst = h.S_out + (size_t)t0 * D;
+        cp.n = D * L;
+        cp.sy = sy;
+        rp->copies.push_back(cp);
+        if (D * L > rp->max_init) rp->max_init = D * L;
+    }
+    return !rp->active.empty();
+}
+
+extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
+    if (!b) return PMCMC_ERR_ARGUMENT;
+    pmcmc_ctx* c = b->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int n = (int)b->ch.size();
+    for (auto& h : b->ch) {
+        h.iters = 0;
+        h.converged = true;
+        h.metric = 0;
+    }
+    for (int64_t r = 0;; ++r) {
+        RoundPlan rp;
+        if (!batch_round_plan(b, r, &rp)) break;
+        const int na = (int)rp.active.size();
+        // device tables: one contiguous upload
+        std::vector<char> host;
+        auto put = [&](const void* src, size_t bytes) {
+            const size_t o = align256(host.size());
+            host.resize(o + bytes);
+            if (bytes) memcpy(host.data() + o, src, bytes);
+            return o;
+        };
+        size_t o_t[2], o_gp[3], o_gr[3], o_qp[3];
+        for (int v = 0; v < 2; ++v) o_t[v] = put(rp.targs[v].data(), rp.targs[v].size() * sizeof(TileArgs<float>));
+        for (int v = 0; v < 3; ++v) {
+            o_gp[v] = put(rp.gpass[v].data(), rp.gpass[v].size() * sizeof(GlmPassArgs));
+            o_gr[v] = put(rp.gred[v].data(), rp.gred[v].size() * sizeof(GlmReduceArgs));
+            o_qp[v] = put(rp.qpass[v].data(), rp.qpass[v].size() * sizeof(GaussPassArgs));
+        }
+        const size_t o_red = put(rp.red_items.data(), rp.red_items.size() * sizeof(int2));
+        const size_t o_single = put(rp.single_items.data(), rp.single_items.size() * sizeof(int2));
+        const size_t o_multi = put(rp.multi_chains.data(), rp.multi_chains.size() * sizeof(int));
+        const size_t o_glmred = put(rp.glm_red_items.data(), rp.glm_red_items.size() * sizeof(int2));
+        size_t o_gi[17], o_qi[17];
+        for (int d = 0; d < 17; ++d) {
+            o_gi[d] = put(rp.glm_items[d].data(), rp.glm_items[d].size() * sizeof(int4));
+            o_qi[d] = put(rp.gauss_items[d].data(), rp.gauss_items[d].size() * sizeof(int2));
+        }
+        const size_t o_init = put(rp.inits.data(), rp.inits.size() * sizeof(BInit));
+        const size_t o_copy = put(rp.copies.data(), rp.copies.size() * sizeof(BCopy));
+        if (host.size() > b->tables_cap) {
+            HIPCHK(c, hipStreamSynchronize(s));
+            if (b->tables) HIPCHK(c, hipFree(b->tables));
+            b->tables = nullptr;
+            b->tables_cap = 0;
+            HIPCHK(c, hipMalloc(&b->tables, host.size()));
+            b->tables_cap = host.size();
+        }
+        HIPCHK(c, hipMemcpyAsync(b->tables, host.data(), host.size(), hipMemcpyHostToDevice, s));
+        char* T0 = (char*)b->tables;
+        auto at = [&](size_t o) { return (void*)(T0 + o); };
+        const dim3 ig(64, (unsigned)na);
+        hipLaunchKernelGGL(batch_init_kernel, ig, dim3(256), 0, s, (const BInit*)at(o_init));
+        HIPCHK(c, hipGetLastError());
+
+        auto iteration = [&](int par) {
+            for (int d = 1; d <= 16; ++d) {
+                launch_glm_pass_batched(d, b->jbf, (const GlmPassArgs*)at(o_gp[par]), (const int4*)at(o_gi[d]),
+                                        (int)rp.glm_items[d].size(), s);
+                launch_gauss_pass_batched(d, b->jbf, (const GaussPassArgs*)at(o_qp[par]), (const int2*)at(o_qi[d]),
+                                          (int)rp.gauss_items[d].size(), s);
+            }
+            launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[par]), (const int2*)at(o_glmred),
+                                      (int)rp.glm_red_items.size(), s);
+            for (int d = 1; d <= 16; ++d) {
+                launch_glm_pass_batched(d, b->jbf, (const GlmPassArgs*)at(o_gp[2]), (const int4*)at(o_gi[d]),
+                                        (int)rp.glm_items[d].size(), s);
+                launch_gauss_pass_batched(d, b->jbf, (const GaussPassArgs*)at(o_qp[2]), (const int2*)at(o_qi[d]),
+                                          (int)rp.gauss_items[d].size(), s);
+            }
+            launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[2]), (const int2*)at(o_glmred),
+                                      (int)rp.glm_red_items.size(), s);
+            const TileArgs<float>* ta = (const TileArgs<float>*)at(o_t[par]);
+            if (!rp.single_items.empty())
+                hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_SINGLE>),
+                                   dim3((unsigned)rp.single_items.size()), dim3(PMCMC_NT), rp.lds_single, s, ta,
+                                   (const int2*)at(o_single));
+            if (!rp.red_items.empty()) {
+                hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_REDUCE>),
+                                   dim3((unsigned)rp.red_items.size()), dim3(PMCMC_NT), rp.lds_multi, s, ta,
+                                   (const int2*)at(o_red));
+                hipLaunchKernelGGL(batch_agg_scan_kernel, dim3((unsigned)rp.multi_chains.size()), dim3(256), 0, s,
+                                   ta, (const int*)at(o_multi));
+                hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_APPLY>),
+                                   dim3((unsigned)rp.red_items.size()), dim3(PMCMC_NT), rp.lds_multi, s, ta,
+                                   (const int2*)at(o_red));
+            }
+        };
+
+        hipGraphExec_t exec = nullptr;
+        hipGraph_t graph = nullptr;
+        if (b->use_graph) {
+            HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            iteration(0);
+            iteration(1);
+            const hipError_t ec = hipStreamEndCapture(s, &graph);
+            if (ec != hipSuccess) return fail(c, PMCMC_ERR_HIP, "batch: graph capture failed: %s", hipGetErrorString(ec));
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (ei != hipSuccess) {
+                hipGraphDestroy(graph);
+                return fail(c, PMCMC_ERR_HIP, "batch: graph instantiate failed: %s", hipGetErrorString(ei));
+            }
+        }
+        int k = 0, chunk = 2;
+        bool all = false;
+        while (k < b->p.maxiter && !all) {
+            int m = chunk;  // even; only the last chunk before maxiter can be odd
+            if (m > b->p.maxiter - k) m = b->p.maxiter - k;
+            for (int j = 0; j < m / 2; ++j) {
+                if (b->use_graph) {
+                    const hipError_t el = hipGraphLaunch(exec, s);
+                    if (el != hipSuccess) {
+                        hipGraphExecDestroy(exec);
+                        hipGraphDestroy(graph);
+                        return fail(c, PMCMC_ERR_HIP, "batch: graph launch failed");
+                    }
+                } else {
+                    iteration(0);
+                    iteration(1);
+                }
+            }
+            if (m & 1) iteration(0);  // k is even here, so the buffers are at parity 0
+            k += m;
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipMemcpyAsync(b->hsync, b->syncs, n * sizeof(SyncState), hipMemcpyDeviceToHost, s));
+            HIPCHK(c, hipStreamSynchronize(s));
+            all = true;
+            for (int ci : rp.active)
+                if (!b->hsync[ci].converged) {
+                    all = false;
+                    break;
+                }
+            if (chunk < 32) chunk <<= 1;
+        }
+        if (exec) {
+            hipGraphExecDestroy(exec);
+            hipGraphDestroy(graph);
+        }
+        hipLaunchKernelGGL(batch_copy_kernel, ig, dim3(256), 0, s, (const BCopy*)at(o_copy));
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(b->hsync, b->syncs, n * sizeof(SyncState), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        for (int ci : rp.active) {
+            const SyncState& hs = b->hsync[ci];
+            BatchChain& h = b->ch[ci];
+            h.iters += hs.converged ? hs.conv_iter + 1 : hs.last_iter + 1;
+            h.converged = h.converged && hs.converged;
+            h.metric = hs.last_metric;
+        }
+    }
+    if (infos)
+        for (int i = 0; i < n; ++i) {
+            infos[i].converged = b->ch[i].converged;
+            infos[i].iters = b->ch[i].iters;
+            infos[i].metric = b->ch[i].metric;
+        }
+    return PMCMC_OK;
 }

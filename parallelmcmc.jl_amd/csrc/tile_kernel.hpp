@@ -98,6 +98,22 @@ __host__ __device__ inline size_t tile_lds_bytes(int64_t D, int TC, bool with_it
     return n * sizeof(T) + (size_t)TC + 64;
 }
 
+// Level-0 tile shape of a D x Tn problem: TC columns per tile (as many as fit
+// DevTraits<T>::tile_elems elements and the LDS budget), P parts per row.
+template <typename T>
+inline void tile_shape(int64_t D, int64_t Tn, bool with_iterate, int* TC, int* P, int* numTiles) {
+    int64_t E = DevTraits<T>::tile_elems;
+    int64_t tc = E / D;
+    if (tc < 1) tc = 1;
+    while (tc > 1 && tile_lds_bytes<T>(D, (int)tc, with_iterate) > 64512) tc >>= 1;
+    if (tc > Tn && Tn > 0) tc = Tn;
+    *TC = (int)tc;
+    *P = D >= PMCMC_NT ? 1 : (int)(PMCMC_NT / D);
+    if (*P > tc) *P = (int)tc;
+    if (*P < 1) *P = 1;
+    *numTiles = (int)((Tn + tc - 1) / tc);
+}
+
 template <typename T>
 __device__ inline T wave_max(T v) {
     for (int m = 32; m >= 1; m >>= 1) v = nan_max(v, __shfl_xor(v, m, 64));
@@ -144,15 +160,17 @@ __device__ inline void stage_lds(T* dst, const T* src, int64_t n, int tid) {
     for (int64_t i = head + nv * VEC + tid; i < n; i += PMCMC_NT) dst[i] = src[i];
 }
 
+// The kernel body, for tile `tile` of the problem `a`.  a.iter < 0 marks a
+// batched solve: the iteration number is then kept on the device
+// (sync->last_iter + 1) and the probes do not change between iterations.
 template <typename T, int MODE, int PASS>
-__global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
+__device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double s_red[2][PMCMC_NT / 64];
 
     const int tid = threadIdx.x;
     // Newton loop already converged: later queued iterations are no-ops.
     if (a.skip && *(volatile const int*)a.skip) return;
-    const int tile = blockIdx.x;
     const int64_t D = a.D;
     const int TC = a.TC;
     const int64_t c0 = (int64_t)tile * TC;
@@ -449,10 +467,11 @@ __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
                 const double dm = __longlong_as_double((long long)atomicAdd(&sy->dmax_bits, 0ull));
                 const double sm = __longlong_as_double((long long)atomicAdd(&sy->smax_bits, 0ull));
                 const double metric = dm / (a.tol_abs + a.tol_rel * sm);
+                const int it = a.iter >= 0 ? a.iter : sy->last_iter + 1;
                 sy->last_metric = metric;
-                sy->last_iter = a.iter;
+                sy->last_iter = it;
                 if (metric <= 1.0) {
-                    sy->conv_iter = a.iter;
+                    sy->conv_iter = it;
                     sy->converged = 1;
                 }
                 sy->dmax_bits = 0ull;
@@ -461,6 +480,19 @@ __global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
             }
         }
     }
+}
+
+template <typename T, int MODE, int PASS>
+__global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
+    tile_body<T, MODE, PASS>(a, blockIdx.x);
+}
+
+// Batched launch over many problems: work item blockIdx.x is (problem, tile) =
+// items[blockIdx.x]; args[problem] describes the problem.
+template <typename T, int MODE, int PASS>
+__global__ __launch_bounds__(PMCMC_NT) void tile_kernel_batched(const TileArgs<T>* args, const int2* items) {
+    const int2 w = items[blockIdx.x];
+    tile_body<T, MODE, PASS>(args[w.x], w.y);
 }
 
 }  // namespace pmcmc
