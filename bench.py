@@ -19,6 +19,9 @@ the FP64 sequential taped MALA run (trace error, accept decisions).
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--precision both|f32|bf16_jac]
     python bench.py --workload scan --scan-D 64 --scan-T 1048576
+    python bench.py --workload full       # correlated Gaussian D=64 T=16384, dense Jacobian
+    python bench.py --workload c3         # 256 uneven chains, batched BF16 vs per-chain FP32
+    python bench.py --workload adjoint    # gradient of mean |x_t|^2, logistic T=32768
 """
 import argparse
 import json
@@ -44,6 +47,32 @@ def make_data(seed):
     X = rng.standard_normal((N_DATA, D)) / math.sqrt(D)
     y = (rng.random(N_DATA) < 1.0 / (1.0 + np.exp(-(X @ theta)))).astype(np.float64)
     return X, y, rng
+
+
+def log(msg):
+    """Progress line on stderr (the JSON result line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def corr_gaussian(rng, Dg, cond_log10=3.0):
+    """Precision Q diag(10^U(0, cond_log10)) Q^T with Q from the QR of a Gaussian matrix."""
+    import numpy as np
+
+    Q, R = np.linalg.qr(rng.standard_normal((Dg, Dg)))
+    Q = Q * np.sign(np.diag(R))[None, :]
+    ev = 10.0 ** rng.uniform(0.0, cond_log10, Dg)
+    Lam = (Q * ev[None, :]) @ Q.T
+    return 0.5 * (Lam + Lam.T), ev
+
+
+def logreg_data(rng, N, Dl):
+    """X ~ N(0,1)/sqrt(D), theta* ~ N(0, I), y ~ Bernoulli(sigmoid(X theta*))."""
+    import numpy as np
+
+    theta = rng.standard_normal(Dl)
+    X = rng.standard_normal((N, Dl)) / math.sqrt(Dl)
+    y = (rng.random(N) < 1.0 / (1.0 + np.exp(-(X @ theta)))).astype(np.float64)
+    return X, y
 
 
 def cpu_baseline(X, y, x0, Xi0, U0, S_gpu0, acc_gpu0, iters0, budget_s):
@@ -127,6 +156,255 @@ def bench_scan(args):
     }))
 
 
+def bench_full(args):
+    """Full-Jacobian DEER on a correlated Gaussian: D=64, precision
+    Q diag(10^U(0,3)) Q^T, T=16384, eps=2e-3 (eps*lambda_max up to 2),
+    decision-frozen dense Jacobian, FP32 and BF16 tile products; trace error
+    against the FP64 sequential taped MALA run (oracle)."""
+    import numpy as np
+    import torch
+
+    from oracle import deer_oracle as O
+    from ParallelMCMC import DEER, DensityModel, GaussianTarget, _device
+
+    Dg, Tg, eps = 64, 16384, 2e-3
+    rng = np.random.default_rng(2024)
+    Lam, ev = corr_gaussian(rng, Dg, 3.0)
+    model = DensityModel(dim=Dg, target=GaussianTarget(Lam))
+    tgt = O.Gaussian(Lam, np.zeros(Dg))
+    x0 = rng.standard_normal(Dg) / np.sqrt(ev)
+    Xi = np.asfortranarray(rng.standard_normal((Tg, Dg)).T)
+    U = rng.random(Tg)
+    t0 = time.perf_counter()
+    ref, acc_ref, _ = O.run_mala_sequential_taped(tgt, x0, eps, Xi, U)
+    seq_s = time.perf_counter() - t0
+    ctx = _device.context(0)
+    res = {}
+    for prec in ("f32", "bf16_jac"):
+        rec = DEER.TapedRecursion(model, eps, Xi.astype(np.float32), U.astype(np.float32), precision=prec)
+        ws = DEER.DEERWorkspace(Dg, Tg, rec.precision)
+        DEER.solve(rec, x0.astype(np.float32), tol_abs=1e-6, tol_rel=1e-6, maxiter=4, jacobian="full", workspace=ws)
+        torch.cuda.synchronize()
+        ctx.reset_kernel_times()
+        ctx.set_profiling(True)
+        t0 = time.perf_counter()
+        S, info = DEER.solve(rec, x0.astype(np.float32), tol_abs=1e-6, tol_rel=1e-6, maxiter=Tg + 1,
+                             jacobian="full", return_info=True, workspace=ws)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        ctx.set_profiling(False)
+        kt = ctx.kernel_times()
+        Sn = S.cpu().numpy().astype(np.float64)
+        log(f"full {prec}: {info.iters} iterations, {el:.2f} s")
+        res[prec] = {"samples_per_s": Tg / el, "seconds": el, "newton_iters": info.iters, "converged": info.converged,
+                     "ms_per_iter": 1e3 * el / info.iters,
+                     "trace_err_vs_fp64_sequential": O.trace_error(Sn, ref),
+                     "accept_mismatches": int(np.sum(info.accepted.cpu().numpy().astype(bool) != acc_ref)),
+                     "kernel_us": {k: (1e3 * ms / n if n else None) for k, (n, ms) in kt.items()}}
+    head = res["bf16_jac"]
+    # dominant kernel: the tile products, 64 x 80 x 64 per step (2 flops per MAC), incl. the prep pass
+    n_red = head["kernel_us"]["tile_reduce"]
+    flops = 2.0 * 64 * 80 * 64 * Tg
+    ach = flops / (n_red * 1e-6) / 1e12
+    print(json.dumps({
+        "metric": "MALA samples/s, full-Jacobian DEER (correlated Gaussian)", "value": head["samples_per_s"],
+        "unit": "samples/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1e3 * head["seconds"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16+f32",
+        "data": "synthetic (seeded precision matrix and tape)",
+        "config": {"workload": f"dense Gaussian D={Dg} cond 1e3 T={Tg} eps={eps} (eps*lambda_max "
+                               f"{eps * ev.max():.2f}), jacobian=:full (decision-frozen), tol 1e-6/1e-6",
+                   "variants": res, "acceptance_rate": float(acc_ref.mean())},
+        "roofline": {"kernel": "full_prep_kernel + full_reduce_mfma_kernel<bf16> (64x80x64 per step)",
+                     "bound": "mfma", "achieved": ach, "peak": 2500.0, "unit": "TFLOP/s", "frac": ach / 2500.0,
+                     "traffic": None},
+        "cpu_baseline": {"value": Tg / seq_s, "unit": "samples/s", "cores": 1, "kind": "port",
+                         "sample": f"sequential taped MALA (numpy oracle, FP64), the whole T={Tg} trajectory"},
+    }))
+
+
+def make_c3(n, seed=3, tmax=65536):
+    """Batched uneven chains: T ~ round(exp U(ln 1024, ln tmax)), D ~ U{2..256};
+    even chains dense Gaussians (precision Q diag(10^U(0,3)) Q^T, eps =
+    0.5/lambda_max), odd chains logistic regressions (N=2000, X ~ N(0,1)/sqrt(D),
+    eps = 0.15/lambda_max of 0.25 X^T X + I), tapes N(0,1) / U(0,1)."""
+    import numpy as np
+
+    from ParallelMCMC import DensityModel, GaussianTarget, LogisticTarget
+    from ParallelMCMC.Batched import Chain
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        Tc = int(round(np.exp(rng.uniform(np.log(1024), np.log(tmax)))))
+        Dc = int(rng.integers(2, 257))
+        if i % 2 == 0:
+            Lam, ev = corr_gaussian(rng, Dc, 3.0)
+            model = DensityModel(dim=Dc, target=GaussianTarget(Lam))
+            eps = 0.5 / ev.max()
+            x0 = rng.standard_normal(Dc) / np.sqrt(ev.mean())
+            kind = "gauss"
+        else:
+            Xd, y = logreg_data(rng, 2000, Dc)
+            eps = 0.15 / np.linalg.eigvalsh(0.25 * Xd.T @ Xd + np.eye(Dc)).max()
+            model = DensityModel(dim=Dc, target=LogisticTarget(Xd, y, 1.0))
+            x0 = 0.1 * rng.standard_normal(Dc)
+            kind = "logreg"
+        Xi = rng.standard_normal((Dc, Tc), dtype=np.float32)
+        U = rng.random(Tc, dtype=np.float32)
+        out.append((kind, Chain(model, eps, x0.astype(np.float32), Xi, U)))
+    return out
+
+
+def bench_c3(args):
+    """Batched BF16-Jacobian DEER over uneven chains (one set of batched
+    launches per iteration, graph replay) against plain FP32 DEER one chain at
+    a time (single-chain solves per window, no graphs).  Both keep one probe
+    set per window: with probes redrawn every iteration :stoch_diag DEER
+    overflows FP32 on the condition-1e3 Gaussian chains."""
+    import numpy as np
+    import torch
+
+    from ParallelMCMC.Adjoint import solve_windowed
+    from ParallelMCMC.Batched import BatchPlan
+
+    W = args.window
+    chains = make_c3(args.c3_chains)
+    log(f"c3: {len(chains)} chains generated")
+    cl = [c for _, c in chains]
+    total = sum(c.Xi.shape[1] for c in cl)
+    kw = dict(tol_abs=args.tol_abs, tol_rel=args.tol_rel, maxiter=W + 1)
+    out = {}
+    for prec in ("bf16_jac", "f32"):
+        plan = BatchPlan(cl, window=W, precision=prec, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = plan.solve()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        log(f"c3 batched {prec}: {el:.2f} s")
+        its = {k: [r[2].iters for r, (kk, _) in zip(res, chains) if kk == k] for k in ("gauss", "logreg")}
+        out[prec] = {"samples_per_s": total / el, "seconds": el, "converged": all(r[2].converged for r in res),
+                     "newton_iters_mean": {k: float(np.mean(v)) for k, v in its.items()},
+                     "newton_iters_max": {k: int(np.max(v)) for k, v in its.items()}}
+        del plan
+    # per-chain FP32 baseline on a systematic subset (every k-th pair), the
+    # batched BF16 run on that same subset, and both against FP64 sequential MALA
+    pick = [j for j in range(len(cl)) if (j // 2) % args.c3_baseline_every == 0]
+    sub = [cl[j] for j in pick]
+    sub_total = sum(c.Xi.shape[1] for c in sub)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    base = []
+    for j, c in enumerate(sub):
+        S, _, infos = solve_windowed(c.model, c.eps, c.Xi, c.U, c.x0, W, "f32", 0, damping=1.0, fixed_probes=True,
+                                     **kw)
+        base.append(S)
+        log(f"c3 plain f32 chain {j + 1}/{len(sub)} (T={c.Xi.shape[1]}, D={c.Xi.shape[0]}): "
+            f"{time.perf_counter() - t0:.1f} s")
+    torch.cuda.synchronize()
+    base_s = time.perf_counter() - t0
+    plan = BatchPlan(sub, window=W, precision="bf16_jac", **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res_sub = plan.solve()
+    torch.cuda.synchronize()
+    sub_s = time.perf_counter() - t0
+    log(f"c3 subset: plain {base_s:.1f} s, batched {sub_s:.1f} s; FP64 sequential checks next")
+    from oracle import deer_oracle as O
+
+    acc_rows = []
+    for j, c, r, bS in zip(pick, sub, res_sub, base):
+        tg = c.model.target
+        if hasattr(tg, "Lam"):
+            ot = O.Gaussian(tg.Lam, np.zeros(tg.dim))
+        else:
+            ot = O.LogReg(tg.X, tg.y, tg.prior_scale)
+        ref, acc_ref, _ = O.run_mala_sequential_taped(ot, c.x0.astype(np.float64), c.eps, c.Xi.astype(np.float64),
+                                                      c.U.astype(np.float64))
+        Sb, Sp = r[0].cpu().numpy().astype(np.float64), bS.cpu().numpy().astype(np.float64)
+        mism = np.nonzero(r[1].cpu().numpy().astype(bool) != acc_ref)[0]
+        acc_rows.append({"chain": j, "kind": chains[j][0], "D": c.Xi.shape[0], "T": c.Xi.shape[1],
+                         "trace_err_batched": O.trace_error(Sb, ref), "trace_err_plain": O.trace_error(Sp, ref),
+                         "accept_mismatches_batched": int(len(mism)),
+                         "first_mismatch_batched": int(mism[0]) if len(mism) else None})
+    print(json.dumps({
+        "metric": "MALA samples/s, batched uneven chains: BF16-Jac batched vs plain FP32 DEER", "value":
+        out["bf16_jac"]["samples_per_s"], "unit": "samples/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1e3 * out["bf16_jac"]["seconds"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16+f32", "data": "synthetic (seeded targets and tapes)",
+        "config": {"workload": f"{len(cl)} chains, T~logU[1024,65536], D~U{{2..256}}, half dense Gaussian "
+                               f"(cond 1e3) half logistic N=2000, :stoch_diag, window {W}, tol "
+                               f"{args.tol_abs:g}/{args.tol_rel:g}", "samples": total, "batched": out,
+                   "subset": {"every": args.c3_baseline_every, "chains": len(sub), "samples": sub_total,
+                              "plain_f32_per_chain_samples_per_s": sub_total / base_s,
+                              "batched_bf16_samples_per_s": sub_total / sub_s,
+                              "batched_bf16_over_plain_f32": base_s / sub_s,
+                              "vs_fp64_sequential": acc_rows}},
+        "roofline": None,
+    }))
+
+
+def bench_adjoint(args):
+    """Gradient of L = mean_t |x_t|^2 with respect to eps and the prior scale
+    (and x0) for the logistic target (N=10000, D=100), T=32768, FP32, windows
+    of 4096 with checkpoint re-solves.  Accuracy: the first window's GPU
+    gradient against the oracle's sequential adjoint and frozen-decision
+    central differences (FP64)."""
+    import numpy as np
+    import torch
+
+    from oracle import deer_oracle as O
+    from ParallelMCMC import DEER, DensityModel, LogisticTarget
+    from ParallelMCMC.Adjoint import trajectory_mean_sq_grad
+
+    Ta, W, scale = 32768, args.window, 1.5
+    X, y, rng = make_data(77)
+    X = X.astype(np.float32).astype(np.float64)
+    model = DensityModel(dim=D, target=LogisticTarget(X, y, scale))
+    x0 = (0.1 * rng.standard_normal(D)).astype(np.float32)
+    Xi = rng.standard_normal((D, Ta), dtype=np.float32)
+    U = rng.random(Ta, dtype=np.float32)
+    kw = dict(tol_abs=args.tol_abs, tol_rel=args.tol_rel, maxiter=200, damping=1.0)
+    trajectory_mean_sq_grad(model, EPS, Xi[:, :W], U[:W], x0, W, "f32", 0, kw)  # warmup
+    torch.cuda.synchronize()
+    log("adjoint: warmup done")
+    t0 = time.perf_counter()
+    L, de, ds, dx0, infos, ainfos = trajectory_mean_sq_grad(model, EPS, Xi, U, x0, W, "f32", 0, kw)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    log(f"adjoint: T={Ta} forward+backward {el:.2f} s; CPU checks of the first window next")
+    # accuracy on the first window
+    tgt = O.LogReg(X, y, scale)
+    Lw, dew, dsw, _, _, _ = trajectory_mean_sq_grad(model, EPS, Xi[:, :W], U[:W], x0, W, "f32", 0, kw)
+    t1 = time.perf_counter()
+    Xi64, U64, x064 = Xi[:, :W].astype(np.float64), U[:W].astype(np.float64), x0.astype(np.float64)
+    Sref, acc, _ = O.run_mala_sequential_taped(tgt, x064, EPS, Xi64, U64)
+    ref = O.mala_adjoint(tgt, EPS, Xi64, x064, Sref, acc, 2.0 * Sref / W)
+    Lf = lambda e, s: float(np.mean(np.sum(O.run_mala_frozen(O.LogReg(X, y, s), x064, e, Xi64, acc) ** 2, axis=0)))
+    h = 1e-4
+    fd_e = (Lf(EPS * (1 + h), scale) - Lf(EPS * (1 - h), scale)) / (2 * h * EPS)
+    fd_s = (Lf(EPS, scale * (1 + h)) - Lf(EPS, scale * (1 - h))) / (2 * h * scale)
+    cpu_s = time.perf_counter() - t1
+    print(json.dumps({
+        "metric": "trajectory-gradient throughput (adjoint DEER), samples/s", "value": Ta / el,
+        "unit": "samples/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1e3 * el,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded logistic design, labels and tape)",
+        "config": {"workload": f"logistic N={N_DATA} D={D} T={Ta} eps={EPS} prior scale {scale}, window {W}, "
+                               "forward DEER + checkpointed reverse-time adjoint, FP32",
+                   "L": L, "dL_deps": de, "dL_dscale": ds,
+                   "forward_newton_iters": [i.iters for i in infos], "adjoint_iters": [i.iters for i in ainfos],
+                   "first_window": {"gpu": [dew, dsw], "oracle_adjoint_fp64": [ref[0], ref[1]],
+                                    "frozen_central_fd_fp64": [fd_e, fd_s],
+                                    "rel_err_vs_oracle": [abs(dew - ref[0]) / abs(ref[0]),
+                                                          abs(dsw - ref[1]) / abs(ref[1])]}},
+        "roofline": None,
+        "cpu_baseline": {"value": W / cpu_s, "unit": "samples/s", "cores": 1, "kind": "port",
+                         "sample": f"first window (T={W}): FP64 sequential MALA + sequential adjoint + 4 "
+                                   "frozen-decision runs for the central differences"},
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,7 +416,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--damping", type=float, default=1.0)
-    ap.add_argument("--workload", default="deer", choices=["deer", "scan"])
+    ap.add_argument("--workload", default="deer", choices=["deer", "scan", "full", "c3", "adjoint"])
+    ap.add_argument("--c3-chains", type=int, default=256)
+    ap.add_argument("--c3-baseline-every", type=int, default=8,
+                    help="per-chain FP32 baseline on every k-th pair of chains (one Gaussian, one logistic; 1 = all)")
+    ap.add_argument("--window", type=int, default=4096)
     ap.add_argument("--scan-D", type=int, default=64)
     ap.add_argument("--scan-T", type=int, default=1 << 20)
     ap.add_argument("--scan-dtype", default="f32", choices=["f32", "f64"])
@@ -150,6 +432,15 @@ def main():
 
     if args.workload == "scan":
         bench_scan(args)
+        return
+    if args.workload == "full":
+        bench_full(args)
+        return
+    if args.workload == "c3":
+        bench_c3(args)
+        return
+    if args.workload == "adjoint":
+        bench_adjoint(args)
         return
 
     import numpy as np

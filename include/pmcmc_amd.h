@@ -148,6 +148,9 @@ typedef struct pmcmc_deer_params {
     int32_t probes;   /* >= 1; used by PMCMC_JAC_STOCH_DIAG */
     int32_t precision; /* PMCMC_PREC_* */
     uint64_t seed;    /* probe stream */
+    int32_t fixed_probes; /* 0: new probes every iteration (src/DEER/DEER.jl:407);
+                             1: one probe set for the whole solve (the batched
+                             solve always uses one set per window) */
 } pmcmc_deer_params;
 
 typedef struct pmcmc_deer_info {

@@ -99,11 +99,15 @@ def _jac_code(jacobian):
 
 def solve(rec: TapedRecursion, s0, init=None, tol_abs=1e-4, tol_rel=1e-3, maxiter=200,
           jacobian="stoch_diag", damping=1.0, probes=1, seed=0, return_info=False, workspace=None,
-          copy_result=None, device=None):
+          copy_result=None, device=None, fixed_probes=False):
     """Newton iterations to the fixed point of the taped recursion
     (src/DEER/DEER.jl:506-595).  Returns S (D x T device tensor) and, with
     ``return_info``, an Info record whose ``accepted`` is the final
-    iteration's accept mask."""
+    iteration's accept mask.  ``fixed_probes`` keeps one Rademacher probe set
+    for the whole solve instead of drawing new ones every iteration (the
+    reference's choice, src/DEER/DEER.jl:407): the Newton iteration is then a
+    stationary linear iteration, which on ill-conditioned targets does not
+    show the transient growth that can overflow FP32 with redrawn probes."""
     if maxiter < 1:
         raise ArgumentError("maxiter must be >= 1")
     if tol_abs < 0:
@@ -142,7 +146,8 @@ def solve(rec: TapedRecursion, s0, init=None, tol_abs=1e-4, tol_rel=1e-3, maxite
             raise ArgumentError("init must be size (D,T)")
         init_ = _device.to_colmajor(init, dt, dev)
     params = _lib.DeerParams(float(rec.eps), float(tol_abs), float(tol_rel), float(damping), int(maxiter),
-                             jcode, int(probes), rec.precision, int(seed) & 0xFFFFFFFFFFFFFFFF)
+                             jcode, int(probes), rec.precision, int(seed) & 0xFFFFFFFFFFFFFFFF,
+                             1 if fixed_probes else 0)
     info = _lib.DeerInfo()
     ws.ctx.call("pmcmc_deer_solve", ws.h, rec.target.handle(ws.ctx), C.byref(params), _device.ptr(x0),
                 _device.ptr(Xi), _device.ptr(U), _device.ptr(init_), _device.ptr(ws.S_tmp),

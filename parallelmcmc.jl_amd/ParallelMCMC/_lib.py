@@ -61,6 +61,7 @@ class DeerParams(C.Structure):
         ("probes", C.c_int32),
         ("precision", C.c_int32),
         ("seed", C.c_uint64),
+        ("fixed_probes", C.c_int32),
     ]
 
 

@@ -332,8 +332,8 @@ static int gauss_pass(const GaussData* g, hipStream_t s, const ColSrc<T>& src, i
 template <typename T>
 int gauss_newton_products(GlmWork* w, const GaussData* g, hipStream_t s, SyncState* sync,
                           int precision, double eps, const T* x0, const T* S_in, const T* Xi,
-                          uint64_t seed, int iter, int probes, int64_t D, int64_t Tn, Prof* prof,
-                          std::string* msg) {
+                          uint64_t seed, int iter, uint32_t probe_iter, int probes, int64_t D, int64_t Tn,
+                          Prof* prof, std::string* msg) {
     const bool jbf = precision == PMCMC_PREC_BF16_JAC;
     const int* skip = sync ? &sync->converged : nullptr;
     ColSrc<T> a{};
@@ -343,7 +343,7 @@ int gauss_newton_products(GlmWork* w, const GaussData* g, hipStream_t s, SyncSta
     a.K = probes;
     a.D = D;
     a.Tn = Tn;
-    for (int k = 0; k < probes && k < 4; ++k) a.keys[k] = probe_key(seed, (uint32_t)iter, (uint32_t)k);
+    for (int k = 0; k < probes && k < 4; ++k) a.keys[k] = probe_key(seed, probe_iter, (uint32_t)k);
     int rc = gauss_pass<T>(g, s, a, 0, eps, Xi, (T*)w->gx, (T*)w->hw, w->lpx, (T*)w->Y, (T*)w->W, skip, prof,
                            iter, jbf, msg);
     if (rc != PMCMC_OK) return rc;
@@ -358,11 +358,11 @@ int gauss_newton_products(GlmWork* w, const GaussData* g, hipStream_t s, SyncSta
                          jbf, msg);
 }
 template int gauss_newton_products<float>(GlmWork*, const GaussData*, hipStream_t, SyncState*, int, double,
-                                          const float*, const float*, const float*, uint64_t, int, int, int64_t,
-                                          int64_t, Prof*, std::string*);
+                                          const float*, const float*, const float*, uint64_t, int, uint32_t, int,
+                                          int64_t, int64_t, Prof*, std::string*);
 template int gauss_newton_products<double>(GlmWork*, const GaussData*, hipStream_t, SyncState*, int, double,
-                                           const double*, const double*, const double*, uint64_t, int, int,
-                                           int64_t, int64_t, Prof*, std::string*);
+                                           const double*, const double*, const double*, uint64_t, int, uint32_t,
+                                           int, int64_t, int64_t, Prof*, std::string*);
 
 template <int DPT>
 static void launch_gauss_batched_dpt(bool jbf, const GaussPassArgs* args, const int2* items, int nitems,

@@ -51,8 +51,8 @@ void gauss_data_destroy(GaussData* g);
 template <typename T>
 int gauss_newton_products(GlmWork* w, const GaussData* g, hipStream_t s, SyncState* sync,
                           int precision, double eps, const T* x0, const T* S_in, const T* Xi,
-                          uint64_t seed, int iter, int probes, int64_t D, int64_t Tn, Prof* prof,
-                          std::string* msg);
+                          uint64_t seed, int iter, uint32_t probe_iter, int probes, int64_t D, int64_t Tn,
+                          Prof* prof, std::string* msg);
 
 // As glm_dir_products: w->gx = grad(states), w->W = dir + eps H dir.
 template <typename T>

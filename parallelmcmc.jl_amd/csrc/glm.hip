@@ -659,8 +659,8 @@ static int glm_pass(const GlmData* g, GlmWork* w, hipStream_t s, const ColSrc<T>
 template <typename T>
 int glm_newton_products(GlmWork* w, const GlmData* g, hipStream_t s, SyncState* sync, int precision,
                         double eps, double prior_scale, const T* x0, const T* S_in, const T* Xi,
-                        uint64_t seed, int iter, int probes, int64_t D, int64_t Tn, Prof* prof,
-                        std::string* msg) {
+                        uint64_t seed, int iter, uint32_t probe_iter, int probes, int64_t D, int64_t Tn,
+                        Prof* prof, std::string* msg) {
     const bool jbf = precision == PMCMC_PREC_BF16_JAC;
     const int* skip = sync ? &sync->converged : nullptr;
     ColSrc<T> a{};
@@ -670,7 +670,7 @@ int glm_newton_products(GlmWork* w, const GlmData* g, hipStream_t s, SyncState* 
     a.K = probes;
     a.D = D;
     a.Tn = Tn;
-    for (int k = 0; k < probes && k < 4; ++k) a.keys[k] = probe_key(seed, (uint32_t)iter, (uint32_t)k);
+    for (int k = 0; k < probes && k < 4; ++k) a.keys[k] = probe_key(seed, probe_iter, (uint32_t)k);
     int rc = glm_pass<T>(g, w, s, a, 0, eps, prior_scale, Xi, (T*)w->gx, (T*)w->hw, w->lpx, (T*)w->Y,
                          (T*)w->W, skip, prof, iter, jbf, msg);
     if (rc != PMCMC_OK) return rc;
@@ -686,10 +686,10 @@ int glm_newton_products(GlmWork* w, const GlmData* g, hipStream_t s, SyncState* 
 }
 template int glm_newton_products<float>(GlmWork*, const GlmData*, hipStream_t, SyncState*, int,
                                         double, double, const float*, const float*, const float*,
-                                        uint64_t, int, int, int64_t, int64_t, Prof*, std::string*);
+                                        uint64_t, int, uint32_t, int, int64_t, int64_t, Prof*, std::string*);
 template int glm_newton_products<double>(GlmWork*, const GlmData*, hipStream_t, SyncState*, int,
                                          double, double, const double*, const double*,
-                                         const double*, uint64_t, int, int, int64_t, int64_t, Prof*,
+                                         const double*, uint64_t, int, uint32_t, int, int64_t, int64_t, Prof*,
                                          std::string*);
 
 template <int DPT>

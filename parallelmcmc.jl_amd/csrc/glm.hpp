@@ -115,8 +115,8 @@ void glm_work_destroy(GlmWork* w);
 template <typename T>
 int glm_newton_products(GlmWork* w, const GlmData* g, hipStream_t s, SyncState* sync, int precision,
                         double eps, double prior_scale, const T* x0, const T* S_in, const T* Xi,
-                        uint64_t seed, int iter, int probes, int64_t D, int64_t Tn, Prof* prof,
-                        std::string* msg);
+                        uint64_t seed, int iter, uint32_t probe_iter, int probes, int64_t D, int64_t Tn,
+                        Prof* prof, std::string* msg);
 
 // logalpha, accept, FT and Jt of the batched MALA step + JVP from the products
 // of the two passes (one wave per column).

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -781,14 +782,15 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
         for (int i = 0; i < n; ++i) {
             const int it = k + i;
             a.iter = it;
+            a.probe_iter = p->fixed_probes ? 0xFFFFFFFFu : (uint32_t)it;
             a.S_in = bufs[it & 1];
             a.S = bufs[(it + 1) & 1];
             const int eit = c->prof.begin(KT_ITERATION, it, c->stream);
             if (full) {
                 std::string msg;
                 int rc = gauss_newton_products(&w->glmw, &tgt->gauss, c->stream, w->sync, p->precision, p->eps,
-                                               (const T*)x0, a.S_in, (const T*)Xi, p->seed, it, 0, D, Tn,
-                                               &c->prof, &msg);
+                                               (const T*)x0, a.S_in, (const T*)Xi, p->seed, it, a.probe_iter, 0,
+                                               D, Tn, &c->prof, &msg);
                 if (rc == PMCMC_OK)
                     rc = full_newton_step<T>(&w->fullw, &w->glmw, &tgt->gauss, c->stream, w->sync, p->precision,
                                              p->jacobian == PMCMC_JAC_FULL_SIGMOID, p->eps, p->damping, p->tol_abs, p->tol_rel, (const T*)x0, a.S_in, a.S,
@@ -799,10 +801,11 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
                 int rc = tgt->kind == TGT_GAUSS
                              ? gauss_newton_products(&w->glmw, &tgt->gauss, c->stream, w->sync, p->precision,
                                                      p->eps, (const T*)x0, a.S_in, (const T*)Xi, p->seed, it,
-                                                     p->probes, D, Tn, &c->prof, &msg)
+                                                     a.probe_iter, p->probes, D, Tn, &c->prof, &msg)
                              : glm_newton_products(&w->glmw, &tgt->glm, c->stream, w->sync, p->precision,
                                                    p->eps, tgt->prior_scale, (const T*)x0, a.S_in,
-                                                   (const T*)Xi, p->seed, it, p->probes, D, Tn, &c->prof, &msg);
+                                                   (const T*)Xi, p->seed, it, a.probe_iter, p->probes, D, Tn,
+                                                   &c->prof, &msg);
                 if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
                 a.Y = (const T*)w->glmw.Y;
                 a.gx = (const T*)w->glmw.gx;
@@ -1125,25 +1128,41 @@ struct pmcmc_batch {
     SyncState* hsync = nullptr;  // n, pinned host
     void* tables = nullptr;      // device descriptor tables
     size_t tables_cap = 0;
+    void* items = nullptr;       // device work-item lists
+    size_t items_cap = 0;
 };
 
 namespace {
 
 struct RoundPlan {
-    // host mirrors of the device tables of one window round
+    // host mirrors of the per-chain descriptor tables of one window round
     std::vector<TileArgs<float>> targs[2];
     std::vector<GlmPassArgs> gpass[3];  // pass 0 parity 0, pass 0 parity 1, pass 1
     std::vector<GlmReduceArgs> gred[3];
     std::vector<GaussPassArgs> qpass[3];
-    std::vector<int2> red_items, single_items, glm_red_items;
-    std::vector<int> multi_chains;
-    std::vector<int4> glm_items[17];
-    std::vector<int2> gauss_items[17];
     std::vector<BInit> inits;
     std::vector<BCopy> copies;
-    std::vector<int> active;
-    size_t lds_multi = 0, lds_single = 0;
-    int64_t max_init = 0;
+    std::vector<int> active;  // chain index of each active slot
+    // per active slot: shape of its work
+    struct Shape {
+        int kind, nT, ncb, ns, dpt, gcb;
+        int64_t L;
+        size_t lds;
+    };
+    std::vector<Shape> shape;
+};
+
+// Work-item lists over the live (not yet converged) slots of a round.  Tile
+// items are split into LDS-size classes so that small tiles are not launched
+// with the largest chain's LDS footprint (and its occupancy).
+static constexpr int NLDS = 3;
+static int lds_class(size_t lds) { return lds <= 16384 ? 0 : (lds <= 32768 ? 1 : 2); }
+struct Items {
+    std::vector<int2> red[NLDS], single[NLDS], glm_red;
+    std::vector<int> multi;
+    std::vector<int4> glm[17];
+    std::vector<int2> gauss[17];
+    size_t lds_multi[NLDS] = {0, 0, 0}, lds_single[NLDS] = {0, 0, 0};
 };
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -1155,6 +1174,8 @@ static void batch_free(pmcmc_batch* b) {
     hipFree(b->syncs);
     hipHostFree(b->hsync);
     hipFree(b->tables);
+    hipFree(b->items);
+    b->items = nullptr;
     b->arena = nullptr;
     b->syncs = nullptr;
     b->hsync = nullptr;
@@ -1291,7 +1312,6 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
         const int64_t t0 = r * b->W;
         if (t0 >= h.T) continue;
         const int64_t L = (h.T - t0) < b->W ? (h.T - t0) : b->W;
-        const int k = (int)rp->active.size();
         rp->active.push_back(i);
         const int64_t D = h.D;
         const float* s0 = t0 == 0 ? h.x0 : h.S_out + (size_t)(t0 - 1) * D;
@@ -1325,6 +1345,7 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
             a.tol_abs = p.tol_abs;
             a.tol_rel = p.tol_rel;
             a.iter = -1;
+            a.probe_iter = 0xFFFFFFFFu;
             a.probes = 1;
             a.jacobian = JAC_STOCH;
             a.seed = seed;
@@ -1340,15 +1361,16 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
             a.prev = h.prev;
             rp->targs[par].push_back(a);
         }
-        const size_t lds = tile_lds_bytes<float>(D, TC, true);
-        if (nT > 1) {
-            for (int t = 0; t < nT; ++t) rp->red_items.push_back(make_int2(k, t));
-            rp->multi_chains.push_back(k);
-            if (lds > rp->lds_multi) rp->lds_multi = lds;
-        } else {
-            rp->single_items.push_back(make_int2(k, 0));
-            if (lds > rp->lds_single) rp->lds_single = lds;
-        }
+        RoundPlan::Shape sh{};
+        sh.kind = h.tgt->kind;
+        sh.nT = nT;
+        sh.L = L;
+        sh.lds = tile_lds_bytes<float>(D, TC, true);
+        sh.dpt = (int)(h.Dp / 16);
+        sh.ns = h.ns;
+        sh.ncb = (int)((L + GLM_CB - 1) / GLM_CB);
+        sh.gcb = (int)((L + GAUSS_CB - 1) / GAUSS_CB);
+        rp->shape.push_back(sh);
         // products
         ColSrc<float> s0src{};
         s0src.V = nullptr;
@@ -1362,7 +1384,6 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
         s1src.V = h.Y;
         s1src.x0 = nullptr;
         s1src.Z = h.W;
-        const int ncb = (int)((L + GLM_CB - 1) / GLM_CB);
         if (h.tgt->kind == TGT_LOGISTIC) {
             const GlmData& g = h.tgt->glm;
             const double inv_s2 = 1.0 / (h.tgt->prior_scale * h.tgt->prior_scale);
@@ -1399,11 +1420,6 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
                 rp->gred[v].push_back(gr);
                 rp->qpass[v].push_back(GaussPassArgs{});
             }
-            const int kg = (int)rp->gpass[0].size() - 1;
-            const int dpt = (int)(h.Dp / 16);
-            for (int cb = 0; cb < ncb; ++cb)
-                for (int sp = 0; sp < h.ns; ++sp) rp->glm_items[dpt].push_back(make_int4(kg, cb, sp, 0));
-            for (int64_t blk = 0; blk < (L + 3) / 4; ++blk) rp->glm_red_items.push_back(make_int2(kg, (int)blk));
         } else {
             const GaussData& g = h.tgt->gauss;
             for (int v = 0; v < 3; ++v) {
@@ -1425,10 +1441,6 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
                 rp->gpass[v].push_back(GlmPassArgs{});
                 rp->gred[v].push_back(GlmReduceArgs{});
             }
-            const int kq = (int)rp->qpass[0].size() - 1;
-            const int dpt = (int)(h.Dp / 16);
-            const int gcb = (int)((L + GAUSS_CB - 1) / GAUSS_CB);
-            for (int cb = 0; cb < gcb; ++cb) rp->gauss_items[dpt].push_back(make_int2(kq, cb));
         }
         BInit in{};
         in.x0 = s0;
@@ -1444,13 +1456,64 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
         cp.n = D * L;
         cp.sy = sy;
         rp->copies.push_back(cp);
-        if (D * L > rp->max_init) rp->max_init = D * L;
     }
     return !rp->active.empty();
 }
 
+static void build_items(const RoundPlan& rp, const std::vector<char>& live, Items* it) {
+    for (int k = 0; k < (int)rp.shape.size(); ++k) {
+        if (!live[k]) continue;
+        const RoundPlan::Shape& sh = rp.shape[k];
+        const int lc = lds_class(sh.lds);
+        if (sh.nT > 1) {
+            for (int t = 0; t < sh.nT; ++t) it->red[lc].push_back(make_int2(k, t));
+            it->multi.push_back(k);
+            if (sh.lds > it->lds_multi[lc]) it->lds_multi[lc] = sh.lds;
+        } else {
+            it->single[lc].push_back(make_int2(k, 0));
+            if (sh.lds > it->lds_single[lc]) it->lds_single[lc] = sh.lds;
+        }
+        if (sh.kind == TGT_LOGISTIC) {
+            for (int cb = 0; cb < sh.ncb; ++cb)
+                for (int sp = 0; sp < sh.ns; ++sp) it->glm[sh.dpt].push_back(make_int4(k, cb, sp, 0));
+            for (int64_t blk = 0; blk < (sh.L + 3) / 4; ++blk) it->glm_red.push_back(make_int2(k, (int)blk));
+        } else {
+            for (int cb = 0; cb < sh.gcb; ++cb) it->gauss[sh.dpt].push_back(make_int2(k, cb));
+        }
+    }
+}
+
+namespace {
+// Host staging of a set of device tables, each 256-byte aligned.
+struct Pack {
+    std::vector<char> host;
+    size_t put(const void* src, size_t bytes) {
+        const size_t o = align256(host.size());
+        host.resize(o + bytes);
+        if (bytes) memcpy(host.data() + o, src, bytes);
+        return o;
+    }
+};
+}  // namespace
+
+static int upload(pmcmc_ctx* c, hipStream_t s, const Pack& pk, void** buf, size_t* cap) {
+    if (pk.host.size() > *cap) {
+        HIPCHK(c, hipStreamSynchronize(s));
+        if (*buf) HIPCHK(c, hipFree(*buf));
+        *buf = nullptr;
+        *cap = 0;
+        HIPCHK(c, hipMalloc(buf, pk.host.size()));
+        *cap = pk.host.size();
+    }
+    HIPCHK(c, hipMemcpyAsync(*buf, pk.host.data(), pk.host.size(), hipMemcpyHostToDevice, s));
+    return PMCMC_OK;
+}
+
 extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
     if (!b) return PMCMC_ERR_ARGUMENT;
+    // PMCMC_BATCH_PROGRESS=1: one line per window round on stderr
+    const char* pe = getenv("PMCMC_BATCH_PROGRESS");
+    const bool progress = pe && pe[0] == '1';
     pmcmc_ctx* c = b->ctx;
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
@@ -1464,84 +1527,100 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
         RoundPlan rp;
         if (!batch_round_plan(b, r, &rp)) break;
         const int na = (int)rp.active.size();
-        // device tables: one contiguous upload
-        std::vector<char> host;
-        auto put = [&](const void* src, size_t bytes) {
-            const size_t o = align256(host.size());
-            host.resize(o + bytes);
-            if (bytes) memcpy(host.data() + o, src, bytes);
-            return o;
-        };
+        // per-chain descriptor tables: uploaded once per round (pageable staging
+        // stays alive until the round's last synchronisation)
+        Pack tp;
         size_t o_t[2], o_gp[3], o_gr[3], o_qp[3];
-        for (int v = 0; v < 2; ++v) o_t[v] = put(rp.targs[v].data(), rp.targs[v].size() * sizeof(TileArgs<float>));
+        for (int v = 0; v < 2; ++v) o_t[v] = tp.put(rp.targs[v].data(), rp.targs[v].size() * sizeof(TileArgs<float>));
         for (int v = 0; v < 3; ++v) {
-            o_gp[v] = put(rp.gpass[v].data(), rp.gpass[v].size() * sizeof(GlmPassArgs));
-            o_gr[v] = put(rp.gred[v].data(), rp.gred[v].size() * sizeof(GlmReduceArgs));
-            o_qp[v] = put(rp.qpass[v].data(), rp.qpass[v].size() * sizeof(GaussPassArgs));
+            o_gp[v] = tp.put(rp.gpass[v].data(), rp.gpass[v].size() * sizeof(GlmPassArgs));
+            o_gr[v] = tp.put(rp.gred[v].data(), rp.gred[v].size() * sizeof(GlmReduceArgs));
+            o_qp[v] = tp.put(rp.qpass[v].data(), rp.qpass[v].size() * sizeof(GaussPassArgs));
         }
-        const size_t o_red = put(rp.red_items.data(), rp.red_items.size() * sizeof(int2));
-        const size_t o_single = put(rp.single_items.data(), rp.single_items.size() * sizeof(int2));
-        const size_t o_multi = put(rp.multi_chains.data(), rp.multi_chains.size() * sizeof(int));
-        const size_t o_glmred = put(rp.glm_red_items.data(), rp.glm_red_items.size() * sizeof(int2));
-        size_t o_gi[17], o_qi[17];
-        for (int d = 0; d < 17; ++d) {
-            o_gi[d] = put(rp.glm_items[d].data(), rp.glm_items[d].size() * sizeof(int4));
-            o_qi[d] = put(rp.gauss_items[d].data(), rp.gauss_items[d].size() * sizeof(int2));
-        }
-        const size_t o_init = put(rp.inits.data(), rp.inits.size() * sizeof(BInit));
-        const size_t o_copy = put(rp.copies.data(), rp.copies.size() * sizeof(BCopy));
-        if (host.size() > b->tables_cap) {
-            HIPCHK(c, hipStreamSynchronize(s));
-            if (b->tables) HIPCHK(c, hipFree(b->tables));
-            b->tables = nullptr;
-            b->tables_cap = 0;
-            HIPCHK(c, hipMalloc(&b->tables, host.size()));
-            b->tables_cap = host.size();
-        }
-        HIPCHK(c, hipMemcpyAsync(b->tables, host.data(), host.size(), hipMemcpyHostToDevice, s));
+        const size_t o_init = tp.put(rp.inits.data(), rp.inits.size() * sizeof(BInit));
+        const size_t o_copy = tp.put(rp.copies.data(), rp.copies.size() * sizeof(BCopy));
+        int rc = upload(c, s, tp, &b->tables, &b->tables_cap);
+        if (rc != PMCMC_OK) return rc;
         char* T0 = (char*)b->tables;
         auto at = [&](size_t o) { return (void*)(T0 + o); };
         const dim3 ig(64, (unsigned)na);
         hipLaunchKernelGGL(batch_init_kernel, ig, dim3(256), 0, s, (const BInit*)at(o_init));
         HIPCHK(c, hipGetLastError());
 
-        auto iteration = [&](int par) {
-            for (int d = 1; d <= 16; ++d) {
-                launch_glm_pass_batched(d, b->jbf, (const GlmPassArgs*)at(o_gp[par]), (const int4*)at(o_gi[d]),
-                                        (int)rp.glm_items[d].size(), s);
-                launch_gauss_pass_batched(d, b->jbf, (const GaussPassArgs*)at(o_qp[par]), (const int2*)at(o_qi[d]),
-                                          (int)rp.gauss_items[d].size(), s);
-            }
-            launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[par]), (const int2*)at(o_glmred),
-                                      (int)rp.glm_red_items.size(), s);
-            for (int d = 1; d <= 16; ++d) {
-                launch_glm_pass_batched(d, b->jbf, (const GlmPassArgs*)at(o_gp[2]), (const int4*)at(o_gi[d]),
-                                        (int)rp.glm_items[d].size(), s);
-                launch_gauss_pass_batched(d, b->jbf, (const GaussPassArgs*)at(o_qp[2]), (const int2*)at(o_qi[d]),
-                                          (int)rp.gauss_items[d].size(), s);
-            }
-            launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[2]), (const int2*)at(o_glmred),
-                                      (int)rp.glm_red_items.size(), s);
-            const TileArgs<float>* ta = (const TileArgs<float>*)at(o_t[par]);
-            if (!rp.single_items.empty())
-                hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_SINGLE>),
-                                   dim3((unsigned)rp.single_items.size()), dim3(PMCMC_NT), rp.lds_single, s, ta,
-                                   (const int2*)at(o_single));
-            if (!rp.red_items.empty()) {
-                hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_REDUCE>),
-                                   dim3((unsigned)rp.red_items.size()), dim3(PMCMC_NT), rp.lds_multi, s, ta,
-                                   (const int2*)at(o_red));
-                hipLaunchKernelGGL(batch_agg_scan_kernel, dim3((unsigned)rp.multi_chains.size()), dim3(256), 0, s,
-                                   ta, (const int*)at(o_multi));
-                hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_APPLY>),
-                                   dim3((unsigned)rp.red_items.size()), dim3(PMCMC_NT), rp.lds_multi, s, ta,
-                                   (const int2*)at(o_red));
-            }
-        };
-
+        // work items over the live slots; rebuilt (and the graph re-captured)
+        // whenever an eighth of them has converged
+        std::vector<char> live(na, 1);
+        int nlive = na, built_live = 0;
+        Items it;
+        Pack ip;
+        size_t o_red[NLDS] = {0, 0, 0}, o_single[NLDS] = {0, 0, 0}, o_multi = 0, o_glmred = 0, o_gi[17], o_qi[17];
         hipGraphExec_t exec = nullptr;
         hipGraph_t graph = nullptr;
-        if (b->use_graph) {
+        auto drop_graph = [&]() {
+            if (exec) hipGraphExecDestroy(exec);
+            if (graph) hipGraphDestroy(graph);
+            exec = nullptr;
+            graph = nullptr;
+        };
+        auto iteration = [&](int par) {
+            char* I0 = (char*)b->items;
+            auto ia = [&](size_t o) { return (void*)(I0 + o); };
+            for (int d = 1; d <= 16; ++d) {
+                launch_glm_pass_batched(d, b->jbf, (const GlmPassArgs*)at(o_gp[par]), (const int4*)ia(o_gi[d]),
+                                        (int)it.glm[d].size(), s);
+                launch_gauss_pass_batched(d, b->jbf, (const GaussPassArgs*)at(o_qp[par]), (const int2*)ia(o_qi[d]),
+                                          (int)it.gauss[d].size(), s);
+            }
+            launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[par]), (const int2*)ia(o_glmred),
+                                      (int)it.glm_red.size(), s);
+            for (int d = 1; d <= 16; ++d) {
+                launch_glm_pass_batched(d, b->jbf, (const GlmPassArgs*)at(o_gp[2]), (const int4*)ia(o_gi[d]),
+                                        (int)it.glm[d].size(), s);
+                launch_gauss_pass_batched(d, b->jbf, (const GaussPassArgs*)at(o_qp[2]), (const int2*)ia(o_qi[d]),
+                                          (int)it.gauss[d].size(), s);
+            }
+            launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[2]), (const int2*)ia(o_glmred),
+                                      (int)it.glm_red.size(), s);
+            const TileArgs<float>* ta = (const TileArgs<float>*)at(o_t[par]);
+            for (int lc = 0; lc < NLDS; ++lc)
+                if (!it.single[lc].empty())
+                    hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_SINGLE>),
+                                       dim3((unsigned)it.single[lc].size()), dim3(PMCMC_NT), it.lds_single[lc], s, ta,
+                                       (const int2*)ia(o_single[lc]));
+            if (!it.multi.empty()) {
+                for (int lc = 0; lc < NLDS; ++lc)
+                    if (!it.red[lc].empty())
+                        hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_REDUCE>),
+                                           dim3((unsigned)it.red[lc].size()), dim3(PMCMC_NT), it.lds_multi[lc], s, ta,
+                                           (const int2*)ia(o_red[lc]));
+                hipLaunchKernelGGL(batch_agg_scan_kernel, dim3((unsigned)it.multi.size()), dim3(256), 0, s, ta,
+                                   (const int*)ia(o_multi));
+                for (int lc = 0; lc < NLDS; ++lc)
+                    if (!it.red[lc].empty())
+                        hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_APPLY>),
+                                           dim3((unsigned)it.red[lc].size()), dim3(PMCMC_NT), it.lds_multi[lc], s, ta,
+                                           (const int2*)ia(o_red[lc]));
+            }
+        };
+        auto rebuild = [&]() -> int {
+            drop_graph();
+            it = Items();
+            build_items(rp, live, &it);
+            ip = Pack();
+            for (int lc = 0; lc < NLDS; ++lc) {
+                o_red[lc] = ip.put(it.red[lc].data(), it.red[lc].size() * sizeof(int2));
+                o_single[lc] = ip.put(it.single[lc].data(), it.single[lc].size() * sizeof(int2));
+            }
+            o_multi = ip.put(it.multi.data(), it.multi.size() * sizeof(int));
+            o_glmred = ip.put(it.glm_red.data(), it.glm_red.size() * sizeof(int2));
+            for (int d = 0; d < 17; ++d) {
+                o_gi[d] = ip.put(it.glm[d].data(), it.glm[d].size() * sizeof(int4));
+                o_qi[d] = ip.put(it.gauss[d].data(), it.gauss[d].size() * sizeof(int2));
+            }
+            const int rc2 = upload(c, s, ip, &b->items, &b->items_cap);
+            if (rc2 != PMCMC_OK) return rc2;
+            built_live = nlive;
+            if (!b->use_graph) return PMCMC_OK;
             HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             iteration(0);
             iteration(1);
@@ -1549,21 +1628,22 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
             if (ec != hipSuccess) return fail(c, PMCMC_ERR_HIP, "batch: graph capture failed: %s", hipGetErrorString(ec));
             const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
             if (ei != hipSuccess) {
-                hipGraphDestroy(graph);
+                drop_graph();
                 return fail(c, PMCMC_ERR_HIP, "batch: graph instantiate failed: %s", hipGetErrorString(ei));
             }
-        }
+            return PMCMC_OK;
+        };
+        rc = rebuild();
+        if (rc != PMCMC_OK) return rc;
         int k = 0, chunk = 2;
-        bool all = false;
-        while (k < b->p.maxiter && !all) {
+        while (k < b->p.maxiter && nlive > 0) {
             int m = chunk;  // even; only the last chunk before maxiter can be odd
             if (m > b->p.maxiter - k) m = b->p.maxiter - k;
             for (int j = 0; j < m / 2; ++j) {
                 if (b->use_graph) {
                     const hipError_t el = hipGraphLaunch(exec, s);
                     if (el != hipSuccess) {
-                        hipGraphExecDestroy(exec);
-                        hipGraphDestroy(graph);
+                        drop_graph();
                         return fail(c, PMCMC_ERR_HIP, "batch: graph launch failed");
                     }
                 } else {
@@ -1576,18 +1656,21 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipMemcpyAsync(b->hsync, b->syncs, n * sizeof(SyncState), hipMemcpyDeviceToHost, s));
             HIPCHK(c, hipStreamSynchronize(s));
-            all = true;
-            for (int ci : rp.active)
-                if (!b->hsync[ci].converged) {
-                    all = false;
-                    break;
-                }
+            nlive = 0;
+            for (int q = 0; q < na; ++q) {
+                live[q] = b->hsync[rp.active[q]].converged ? 0 : 1;
+                nlive += live[q];
+            }
             if (chunk < 32) chunk <<= 1;
+            if (nlive > 0 && k < b->p.maxiter && 8 * nlive <= 7 * built_live) {
+                rc = rebuild();
+                if (rc != PMCMC_OK) {
+                    drop_graph();
+                    return rc;
+                }
+            }
         }
-        if (exec) {
-            hipGraphExecDestroy(exec);
-            hipGraphDestroy(graph);
-        }
+        drop_graph();
         hipLaunchKernelGGL(batch_copy_kernel, ig, dim3(256), 0, s, (const BCopy*)at(o_copy));
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipMemcpyAsync(b->hsync, b->syncs, n * sizeof(SyncState), hipMemcpyDeviceToHost, s));
@@ -1598,6 +1681,10 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
             h.iters += hs.converged ? hs.conv_iter + 1 : hs.last_iter + 1;
             h.converged = h.converged && hs.converged;
             h.metric = hs.last_metric;
+        }
+        if (progress) {
+            fprintf(stderr, "[pmcmc batch] window round %lld: %d chains, %d iterations\n", (long long)r, na, k);
+            fflush(stderr);
         }
     }
     if (infos)

@@ -67,6 +67,7 @@ struct TileArgs {
     double* la_out;
     double eps, damping, tol_abs, tol_rel;
     int iter;
+    uint32_t probe_iter;  // iteration field of the probe keys (fixed for fixed probes)
     int probes;
     int jacobian;
     uint64_t seed;
@@ -224,7 +225,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         const int K = a.probes;
         uint32_t keys[PMCMC_MAXP];
 #pragma unroll
-        for (int k = 0; k < PMCMC_MAXP; ++k) keys[k] = probe_key(a.seed, (uint32_t)a.iter, (uint32_t)k);
+        for (int k = 0; k < PMCMC_MAXP; ++k) keys[k] = probe_key(a.seed, a.probe_iter, (uint32_t)k);
         const size_t DT = (size_t)D * (size_t)a.T_len;
 
         for (int c = grp; c < tc; c += ngrp) {
