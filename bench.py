@@ -36,6 +36,11 @@ sys.path.insert(0, os.path.join(ROOT, "parallelmcmc.jl_amd"))
 METRIC = "MALA samples/s: DEER BF16-Jac vs plain FP32 DEER at ≤1e-5 trace err, 1 GPU"
 PEAK_TFLOPS = {"f32": 157.3}  # MI355X dense FP32 MFMA
 PEAK_BF16_TFLOPS = 2500.0     # MI355X dense BF16 MFMA
+# HBM bytes per launch of the FP32 GEMM pass on this workload, from rocprofv3
+# FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) counters, the factors measured on a
+# device copy of known size: 37.9 MiB read (the 4.5 MB design once per XCD L2,
+# plus the columns) + 14.1 MiB of split-N partial sums written.
+PASS_TRAFFIC_BYTES = 37.9 * 2**20 + 14.1 * 2**20
 N_DATA, D, T, EPS = 10000, 100, 4096, 5e-3
 
 
@@ -545,10 +550,11 @@ def main():
             "peak": peak_eff,
             "unit": "TFLOP/s",
             "frac": achieved / peak_eff,
-            "traffic": None,
+            "traffic": PASS_TRAFFIC_BYTES,
             "note": ("peak is the time-weighted roofline of the pass: residual half at 157.3 TFLOP/s FP32 MFMA, "
                      "probe half at 2500 TFLOP/s BF16 MFMA" if head["prec"] == "bf16_jac"
-                     else "157.3 TFLOP/s FP32 MFMA"),
+                     else "157.3 TFLOP/s FP32 MFMA") + "; traffic: calibrated FETCH_SIZE + WRITE_SIZE "
+                    "counters per launch of the FP32 pass (the BF16-Jacobian pass moves the same data)",
         },
     }
     if "f32" in res and "bf16_jac" in res:
