@@ -137,6 +137,11 @@ __global__ void init_iterate_kernel(T* S, const T* s0, const T* init, int64_t D,
         S[i] = init ? init[i] : s0[i % D];
 }
 
+__global__ void zero_ints_kernel(int* p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
 template <typename Tin, typename Tout>
 __global__ void convert_kernel(Tout* out, const Tin* in, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -419,18 +424,74 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
     };
     if (overlap(A)) return fail(c, PMCMC_ERR_ARGUMENT, "S must not alias A");
     if (overlap(B)) return fail(c, PMCMC_ERR_ARGUMENT, "S must not alias B");
-    ScanPlan plan;
-    if (!plan_scan<T>(&plan, D, Tn, false))
-        return fail(c, PMCMC_ERR_UNSUPPORTED, "scan: D=%lld too large for one tile", (long long)D);
     HIPCHK(c, hipSetDevice(c->device));
-    if (plan.bytes > c->scan_blob_cap) {
+    // PMCMC_SCAN_CHAINED=1: single pass with decoupled look-back (one read of
+    // A and B).  Not the default: on this GPU the look-back front (one window
+    // of predecessors per coherent round trip) is slower than the two-pass
+    // reduce/apply scan at D >= 64 and about even at D = 1.
+    const char* tp = getenv("PMCMC_SCAN_CHAINED");
+    ScanLevel L0;
+    tile_shape<T>(D, Tn, false, &L0.TC, &L0.P, &L0.nT);
+    {
+        // the single-pass scan is bound by the look-back front, so it takes
+        // the widest tile that fits the LDS budget
+        int tc = 1;
+        while (tc * 2 <= Tn && tile_lds_bytes<T>(D, tc * 2, false, true) <= 64512) tc *= 2;
+        while (tc + tc / 8 <= Tn && tile_lds_bytes<T>(D, tc + tc / 8, false, true) <= 64512 && tc >= 8) tc += tc / 8;
+        L0.TC = tc;
+        L0.nT = (int)((Tn + tc - 1) / tc);
+        L0.P = D >= PMCMC_NT ? 1 : (int)(PMCMC_NT / D);
+        if (L0.P > tc) L0.P = tc;
+        if (L0.P < 1) L0.P = 1;
+    }
+    const bool chained = L0.nT > 1 && tp && tp[0] == '1' && tile_lds_bytes<T>(D, L0.TC, false, true) <= 64512;
+    ScanPlan plan;
+    if (!chained && !plan_scan<T>(&plan, D, Tn, false))
+        return fail(c, PMCMC_ERR_UNSUPPORTED, "scan: D=%lld too large for one tile", (long long)D);
+    const size_t lbm = (size_t)D * L0.nT * sizeof(T);
+    const size_t lb_bytes = 3 * lbm + ((size_t)L0.nT + 64) * sizeof(int);
+    const size_t need = chained ? lb_bytes : plan.bytes;
+    if (need > c->scan_blob_cap) {
         if (c->scan_blob) HIPCHK(c, hipFree(c->scan_blob));
         c->scan_blob = nullptr;
         c->scan_blob_cap = 0;
-        HIPCHK(c, hipMalloc(&c->scan_blob, plan.bytes));
-        c->scan_blob_cap = plan.bytes;
+        HIPCHK(c, hipMalloc(&c->scan_blob, need));
+        c->scan_blob_cap = need;
     }
     const int e = c->prof.begin(KT_ITERATION, 0, c->stream);
+    if (chained) {
+        char* blob = (char*)c->scan_blob;
+        TileArgs<T> a{};
+        a.A = A;
+        a.B = B;
+        a.s0 = s0;
+        a.S = S;
+        a.D = D;
+        a.T_len = Tn;
+        a.TC = L0.TC;
+        a.P = L0.P;
+        a.numTiles = L0.nT;
+        a.damping = 1.0;
+        a.lb_agg_a = (T*)blob;
+        a.lb_agg_b = (T*)(blob + lbm);
+        a.lb_incl = (T*)(blob + 2 * lbm);
+        int* ints = (int*)(blob + 3 * lbm);
+        a.lb_status = ints;
+        a.lb_ticket = (unsigned*)(ints + L0.nT);
+        a.lb_err = (unsigned*)(ints + L0.nT + 16);
+        hipLaunchKernelGGL(zero_ints_kernel, dim3(grid_for(L0.nT + 64)), dim3(256), 0, c->stream, ints,
+                           (int64_t)L0.nT + 64);
+        hipLaunchKernelGGL((tile_kernel_chained<T, MODE_SCAN>), dim3((unsigned)L0.nT), dim3(PMCMC_NT),
+                           tile_lds_bytes<T>(D, L0.TC, false, true), c->stream, a);
+        c->prof.end(e, c->stream);
+        HIPCHK(c, hipGetLastError());
+        unsigned err = 0;
+        HIPCHK(c, hipMemcpyAsync(&err, a.lb_err, sizeof err, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->prof.flush(0);
+        if (err) return fail(c, PMCMC_ERR_DEVICE, "scan: look-back wait exceeded its bound");
+        return PMCMC_OK;
+    }
     run_scan_levels<T>(plan, 0, A, B, S, s0, (char*)c->scan_blob, nullptr, c->stream);
     c->prof.end(e, c->stream);
     HIPCHK(c, hipGetLastError());

@@ -37,7 +37,10 @@
 namespace pmcmc {
 
 enum { MODE_SCAN = 0, MODE_ELEM = 1, MODE_GEMM = 2 };
-enum { PASS_REDUCE = 0, PASS_APPLY = 1, PASS_SINGLE = 2 };
+enum { PASS_REDUCE = 0, PASS_APPLY = 1, PASS_SINGLE = 2, PASS_CHAINED = 3 };
+// spin cap of one look-back poll: far beyond any legitimate wait (which is
+// microseconds); hitting it sets lb_err and the scan reports an error
+#define PMCMC_LB_SPIN_CAP (1u << 21)
 enum { JAC_STOCH = 0, JAC_DIAG = 1 };
 #define PMCMC_MAXP 4
 
@@ -74,6 +77,16 @@ struct TileArgs {
     // diagonal-Hessian target (nullable: standard normal)
     const T* prec;
     const T* mean;
+    // PASS_CHAINED (single pass with decoupled look-back): per-tile status
+    // (0 none, 1 aggregate, 2 inclusive state), aggregates and inclusive
+    // states (D x numTiles), the tile ticket and an error flag; zeroed before
+    // each launch
+    int* lb_status;
+    T* lb_agg_a;
+    T* lb_agg_b;
+    T* lb_incl;
+    unsigned* lb_ticket;
+    unsigned* lb_err;
     // GEMM target products (MODE_GEMM): D x T, probe k of W/hw at k*D*T
     const T* Y;
     const T* gx;
@@ -91,12 +104,18 @@ struct TileArgs {
 //   pa,pb: max(NT, D) part aggregates
 //   sin : D         incoming state per row
 //   acc : TC bytes  accept flags
+//   cmp : 2*D       composed predecessor aggregates (PASS_CHAINED only; at a
+//                   16-byte aligned offset after acc)
 template <typename T>
-__host__ __device__ inline size_t tile_lds_bytes(int64_t D, int TC, bool with_iterate) {
+__host__ __device__ inline size_t tile_lds_core(int64_t D, int TC, bool with_iterate) {
     int64_t PA = D > PMCMC_NT ? D : PMCMC_NT;
     int64_t xs = with_iterate ? ((int64_t)TC + 1) * D : 0;
     size_t n = (size_t)(xs + 2 * (int64_t)TC * D + 2 * PA + D);
-    return n * sizeof(T) + (size_t)TC + 64;
+    return (n * sizeof(T) + (size_t)TC + 15) / 16 * 16;
+}
+template <typename T>
+__host__ __device__ inline size_t tile_lds_bytes(int64_t D, int TC, bool with_iterate, bool chained = false) {
+    return tile_lds_core<T>(D, TC, with_iterate) + (chained ? 2 * (size_t)D * sizeof(T) : 0) + 64;
 }
 
 // Level-0 tile shape of a D x Tn problem: TC columns per tile (as many as fit
@@ -161,6 +180,122 @@ __device__ inline void stage_lds(T* dst, const T* src, int64_t n, int tid) {
     for (int64_t i = head + nv * VEC + tid; i < n; i += PMCMC_NT) dst[i] = src[i];
 }
 
+// Publication between workgroups of one launch: values are written and read
+// with agent-scope relaxed atomics (coherent across the XCDs' L2s without an
+// L2 write-back per publication); a workgroup-scope release (wait for this
+// thread's stores) plus the barrier orders them before the status word.
+template <typename T>
+__device__ inline void lb_store(T* p, T v) {
+    if constexpr (sizeof(T) == 8)
+        __hip_atomic_store((unsigned long long*)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    else
+        __hip_atomic_store((unsigned*)p, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void lb_publish_fence() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); }
+
+// Coherent load of a value another workgroup published.
+template <typename T>
+__device__ inline T lb_load(const T* p) {
+    if constexpr (sizeof(T) == 8) {
+        const unsigned long long v =
+            __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return __longlong_as_double((long long)v);
+    } else {
+        const unsigned v = __hip_atomic_load((const unsigned*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return __uint_as_float(v);
+    }
+}
+
+// Decoupled look-back of tile `tile` (tiles numbered in start order by a
+// ticket, so every predecessor is resident and never waits on a successor).
+// ga/gb: this tile's aggregate per row.  Publishes the aggregate (status 1),
+// composes predecessor aggregates over windows of 256 tiles (one status word
+// per thread) back to the nearest published inclusive state, sets sin_ to the
+// incoming state, then publishes this tile's inclusive state (status 2).
+// Inclusive states advance one window per look-back round trip, so the
+// window is as wide as the workgroup.
+template <typename T>
+__device__ void chained_lookback(const TileArgs<T>& a, const int tile, const int64_t D, const T* ga, const T* gb,
+                                 T* sin_, T* cmp_a, T* cmp_b) {
+    __shared__ int lb_k, lb_lo;
+    __shared__ int lb_w[PMCMC_NT / 64];
+    const int tid = threadIdx.x;
+    int* st = a.lb_status;
+    if (tile == 0) {
+        for (int64_t r = tid; r < D; r += PMCMC_NT) {
+            const T s = a.s0[r];
+            sin_[r] = s;
+            lb_store<T>(&a.lb_incl[r], ga[r] * s + gb[r]);
+        }
+        lb_publish_fence();
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(&st[0], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    for (int64_t r = tid; r < D; r += PMCMC_NT) {
+        lb_store<T>(&a.lb_agg_a[(size_t)tile * D + r], ga[r]);
+        lb_store<T>(&a.lb_agg_b[(size_t)tile * D + r], gb[r]);
+        cmp_a[r] = (T)1;
+        cmp_b[r] = (T)0;
+    }
+    lb_publish_fence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&st[tile], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int j = tile - 1;
+    for (;;) {
+        {
+            const int idx = j - tid;
+            int f = 0;
+            if (idx >= 0) {
+                unsigned spins = 0;
+                f = __hip_atomic_load(&st[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                while (f == 0) {
+                    if (++spins > PMCMC_LB_SPIN_CAP) {
+                        atomicOr(a.lb_err, 1u);
+                        f = 2;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                    f = __hip_atomic_load(&st[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+            const unsigned long long m = __ballot(f == 2);
+            if ((tid & 63) == 0) lb_w[tid >> 6] = m ? (tid + (int)__builtin_ctzll(m)) : -1;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int l = -1;  // nearest predecessor with an inclusive state
+            for (int w = 0; w < PMCMC_NT / 64 && l < 0; ++w) l = lb_w[w];
+            lb_k = l >= 0 ? j - l : -1;
+            lb_lo = j - (PMCMC_NT - 1) < 0 ? 0 : j - (PMCMC_NT - 1);
+        }
+        __syncthreads();
+        const int k = lb_k, lo = lb_lo;
+        const int stop = k >= 0 ? k + 1 : lo;
+        for (int64_t r = tid; r < D; r += PMCMC_NT) {
+            T ca = cmp_a[r], cb = cmp_b[r];
+#pragma unroll 8
+            for (int jj = j; jj >= stop; --jj) {
+                const T xa = lb_load<T>(&a.lb_agg_a[(size_t)jj * D + r]);
+                const T xb = lb_load<T>(&a.lb_agg_b[(size_t)jj * D + r]);
+                cb = ca * xb + cb;
+                ca = ca * xa;
+            }
+            if (k >= 0) sin_[r] = ca * lb_load<T>(&a.lb_incl[(size_t)k * D + r]) + cb;
+            cmp_a[r] = ca;
+            cmp_b[r] = cb;
+        }
+        if (k >= 0) break;
+        j = lo - 1;
+        __syncthreads();  // lb_w / lb_k / lb_lo are rewritten
+    }
+    for (int64_t r = tid; r < D; r += PMCMC_NT) lb_store<T>(&a.lb_incl[(size_t)tile * D + r], ga[r] * sin_[r] + gb[r]);
+    lb_publish_fence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&st[tile], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // The kernel body, for tile `tile` of the problem `a`.  a.iter < 0 marks a
 // batched solve: the iteration number is then kept on the device
 // (sync->last_iter + 1) and the probes do not change between iterations.
@@ -188,6 +323,8 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
     T* pb = pa + PA;
     T* sin_ = pb + PA;
     uint8_t* accl = reinterpret_cast<uint8_t*>(sin_ + D);
+    T* cmp_a = reinterpret_cast<T*>(smem + tile_lds_core<T>(D, TC, DEERM));
+    T* cmp_b = cmp_a + D;
 
     const int64_t nel = (int64_t)tc * D;
     const int64_t gbase = c0 * D;
@@ -206,7 +343,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         stage_lds<T>(bl, a.Xi + gbase, nel, tid);
     }
     // incoming state of the tile (needed only by the output passes)
-    if (OUT) {
+    if (OUT && PASS != PASS_CHAINED) {
         for (int64_t r = tid; r < D; r += PMCMC_NT)
             sin_[r] = (PASS == PASS_SINGLE || tile == 0) ? a.s0[r] : a.prev[(size_t)(tile - 1) * D + r];
     }
@@ -412,6 +549,10 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         }
         return;
     }
+    if (PASS == PASS_CHAINED) {
+        chained_lookback<T>(a, tile, D, pa + (size_t)(P - 1) * D, pb + (size_t)(P - 1) * D, sin_, cmp_a, cmp_b);
+        __syncthreads();
+    }
 
     // ---- phase 4: apply the incoming state, damping, convergence maxima ---
     double dmax = 0, smax = 0;
@@ -494,6 +635,16 @@ template <typename T, int MODE, int PASS>
 __global__ __launch_bounds__(PMCMC_NT) void tile_kernel_batched(const TileArgs<T>* args, const int2* items) {
     const int2 w = items[blockIdx.x];
     tile_body<T, MODE, PASS>(args[w.x], w.y);
+}
+
+// Single pass with decoupled look-back: tiles are numbered by a ticket in the
+// order workgroups start.
+template <typename T, int MODE>
+__global__ __launch_bounds__(PMCMC_NT) void tile_kernel_chained(TileArgs<T> a) {
+    __shared__ int ticket;
+    if (threadIdx.x == 0) ticket = (int)atomicAdd(a.lb_ticket, 1u);
+    __syncthreads();
+    tile_body<T, MODE, PASS_CHAINED>(a, ticket);
 }
 
 }  // namespace pmcmc

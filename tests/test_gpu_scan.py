@@ -132,3 +132,31 @@ def test_scan_large_sizes(D, T):
     if D * T <= 4_000_000:
         S64 = _gpu_scan(A.astype(np.float64), B.astype(np.float64), s0.astype(np.float64), torch.float64)
         assert np.allclose(S64, ref, rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("D,T,dt", [(1, 1 << 20, "f64"), (64, 65536, "f64"), (300, 4000, "f64"),
+                                    (64, 1 << 18, "f32"), (7, 123457, "f32")])
+def test_scan_long_carry_chained_vs_two_pass(D, T, dt, monkeypatch):
+    """A near 1 carries every tile's state into all later tiles, so a wrong
+    look-back composition anywhere shows up downstream.  The single-pass
+    (decoupled look-back) scan matches the sequential recurrence and the
+    two-pass reduce/apply scan."""
+    import torch
+
+    rng = np.random.default_rng(D + T)
+    dtype = np.float64 if dt == "f64" else np.float32
+    A = rng.uniform(0.999, 1.0005, (D, T)).astype(dtype)
+    B = (1e-3 * rng.standard_normal((D, T))).astype(dtype)
+    s0 = rng.standard_normal(D).astype(dtype)
+    tdt = torch.float64 if dt == "f64" else torch.float32
+    monkeypatch.setenv("PMCMC_SCAN_CHAINED", "1")
+    S1 = _gpu_scan(A, B, s0, tdt)
+    monkeypatch.delenv("PMCMC_SCAN_CHAINED", raising=False)
+    S2 = _gpu_scan(A, B, s0, tdt)
+    ref = _seq(A, B, s0)
+    tol = 1e-10 if dt == "f64" else 1e-3
+    scale = np.max(np.abs(ref))
+    print(f"D={D} T={T} {dt}: chained err {np.max(np.abs(S1 - ref)) / scale:.2e}, "
+          f"two-pass err {np.max(np.abs(S2 - ref)) / scale:.2e}", flush=True)
+    assert np.max(np.abs(S1 - ref)) <= tol * scale
+    assert np.max(np.abs(S2 - ref)) <= tol * scale
