@@ -58,6 +58,10 @@ typedef struct pmcmc_deer_ws pmcmc_deer_ws;
 /* ---- context ------------------------------------------------------------ */
 /* stream: a hipStream_t, or NULL for a stream the library creates.          */
 int pmcmc_ctx_create(int device, void* stream, pmcmc_ctx** out);
+/* Drops the creator's reference.  Targets, workspaces and batches created
+ * on the context hold references of their own, so they may be destroyed
+ * before or after it (finalizers of a garbage-collected host language run in
+ * no fixed order); the stream and buffers are released with the last one. */
 int pmcmc_ctx_destroy(pmcmc_ctx* ctx);
 const char* pmcmc_last_error(const pmcmc_ctx* ctx);
 int pmcmc_ctx_synchronize(pmcmc_ctx* ctx);

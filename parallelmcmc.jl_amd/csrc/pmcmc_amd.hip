@@ -6,6 +6,7 @@
 // syncs two maxima to the host every iteration, src/DEER/DEER.jl:567-571).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -18,6 +19,7 @@
 #include "adjoint.hpp"
 #include "common.hpp"
 #include "fullscan.hpp"
+#include "refine.hpp"
 #include "gauss.hpp"
 #include "glm.hpp"
 #include "tile_kernel.hpp"
@@ -51,12 +53,21 @@ struct pmcmc_ctx {
     size_t scan_blob_cap = 0;
     SyncState* host_sync = nullptr;  // pinned
     Prof prof;
+    // the creator's reference plus one per live target, workspace and batch
+    // created on this context: pmcmc_ctx_destroy drops the creator's, the
+    // stream and buffers go with the last one, so objects may be destroyed
+    // in any order
+    std::atomic<int> refs{1};
 };
+
+static void ctx_retain(pmcmc_ctx* c) { c->refs.fetch_add(1, std::memory_order_relaxed); }
+static void ctx_release(pmcmc_ctx* c);
 
 enum { TGT_DIAG = 0, TGT_LOGISTIC = 1, TGT_GAUSS = 2 };
 
 struct pmcmc_target {
     pmcmc_ctx* ctx = nullptr;
+    bool ctx_ref = false;  // holds a context reference (set once created)
     int kind = TGT_DIAG;
     int64_t D = 0, N = 0;
     // diagonal Gaussian (both precisions; null = standard normal)
@@ -73,6 +84,7 @@ struct pmcmc_target {
 
 struct pmcmc_deer_ws {
     pmcmc_ctx* ctx = nullptr;
+    bool ctx_ref = false;  // holds a context reference (set once created)
     int64_t D = 0, T = 0;
     int prec = PMCMC_PREC_F32;
     size_t esz = 4;
@@ -83,6 +95,8 @@ struct pmcmc_deer_ws {
     // GEMM-target buffers (allocated on first use)
     GlmWork glmw;
     FullWork fullw;  // jacobian = :full
+    RefineWork refw;  // FP64 decision overrides (GEMM targets, FP32)
+    int8_t* prev_dec = nullptr;  // previous iteration's decisions (FP32 hysteresis)
     // adjoint buffers (allocated on first use)
     void* adj_lam = nullptr;   // D x (T+1), last column zero
     void* adj_A = nullptr;     // reversed-time scan coefficients, D x T
@@ -135,6 +149,11 @@ __global__ void init_iterate_kernel(T* S, const T* s0, const T* init, int64_t D,
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         S[i] = init ? init[i] : s0[i % D];
+}
+
+__global__ void fill_i8_kernel(int8_t* p, int8_t v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 __global__ void zero_ints_kernel(int* p, int64_t n) {
@@ -342,14 +361,19 @@ extern "C" int pmcmc_ctx_create(int device, void* stream, pmcmc_ctx** out) {
     return PMCMC_OK;
 }
 
-extern "C" int pmcmc_ctx_destroy(pmcmc_ctx* c) {
-    if (!c) return PMCMC_OK;
+static void ctx_release(pmcmc_ctx* c) {
+    if (c->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->scan_blob) hipFree(c->scan_blob);
     if (c->host_sync) hipHostFree(c->host_sync);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
+}
+
+extern "C" int pmcmc_ctx_destroy(pmcmc_ctx* c) {
+    if (!c) return PMCMC_OK;
+    ctx_release(c);
     return PMCMC_OK;
 }
 
@@ -549,6 +573,8 @@ extern "C" int pmcmc_target_diag_gaussian(pmcmc_ctx* c, int64_t D, int dtype_pre
         pmcmc_target_destroy(t);
         return rc;
     }
+    ctx_retain(c);
+    t->ctx_ref = true;
     *out = t;
     return PMCMC_OK;
 }
@@ -577,6 +603,8 @@ extern "C" int pmcmc_target_logistic(pmcmc_ctx* c, int64_t N, int64_t D, int dty
         pmcmc_target_destroy(t);
         return rc;
     }
+    ctx_retain(c);
+    t->ctx_ref = true;
     *out = t;
     return PMCMC_OK;
 }
@@ -601,6 +629,8 @@ extern "C" int pmcmc_target_gaussian(pmcmc_ctx* c, int64_t D, int dtype_prec, co
         pmcmc_target_destroy(t);
         return rc;
     }
+    ctx_retain(c);
+    t->ctx_ref = true;
     *out = t;
     return PMCMC_OK;
 }
@@ -615,7 +645,9 @@ extern "C" int pmcmc_target_destroy(pmcmc_target* t) {
     hipFree(t->mean_d);
     glm_data_destroy(&t->glm);
     gauss_data_destroy(&t->gauss);
+    pmcmc_ctx* c = t->ctx_ref ? t->ctx : nullptr;
     delete t;
+    if (c) ctx_release(c);
     return PMCMC_OK;
 }
 
@@ -729,6 +761,8 @@ extern "C" int pmcmc_deer_ws_create(pmcmc_ctx* c, int64_t D, int64_t Tn, int pre
         pmcmc_deer_ws_destroy(w);
         return fail(c, PMCMC_ERR_ALLOC, "workspace allocation failed (D=%lld T=%lld)", (long long)D, (long long)Tn);
     }
+    ctx_retain(c);
+    w->ctx_ref = true;
     *out = w;
     return PMCMC_OK;
 }
@@ -745,6 +779,8 @@ extern "C" int pmcmc_deer_ws_destroy(pmcmc_deer_ws* w) {
     hipFree(w->blob);
     glm_work_destroy(&w->glmw);
     full_work_destroy(&w->fullw);
+    refine_work_destroy(&w->refw);
+    hipFree(w->prev_dec);
     hipFree(w->adj_lam);
     hipFree(w->adj_A);
     hipFree(w->adj_B);
@@ -752,7 +788,9 @@ extern "C" int pmcmc_deer_ws_destroy(pmcmc_deer_ws* w) {
     hipFree(w->adj_zero);
     hipFree(w->adj_part);
     hipFree(w->adj_blob);
+    pmcmc_ctx* c = w->ctx_ref ? w->ctx : nullptr;
     delete w;
+    if (c) ctx_release(c);
     return PMCMC_OK;
 }
 
@@ -780,7 +818,8 @@ static void launch_deer_iteration(pmcmc_deer_ws* w, TileArgs<T> a, hipStream_t s
     launch_tile<T, MODE, PASS_REDUCE>(a, L.nT, lds, s);
     prof->end(e, s);
     e = prof->begin(KT_SCAN_SPINE, a.iter, s);
-    run_scan_levels<T>(p, 1, a.agg_a, a.agg_b, (T*)(blob + L.off_s), a.s0, blob, a.skip, s);
+    // the corrections start from zero (x0 is fixed)
+    run_scan_levels<T>(p, 1, a.agg_a, a.agg_b, (T*)(blob + L.off_s), nullptr, blob, a.skip, s);
     prof->end(e, s);
     a.prev = (const T*)(blob + L.off_s);
     e = prof->begin(KT_TILE_APPLY, a.iter, s);
@@ -809,6 +848,40 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
         if (rc == PMCMC_OK && full) rc = full_work_ensure(&w->fullw, D, Tn, sizeof(T) == 8, &msg);
         if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
     }
+    // FP64 recompute of near-threshold decisions for the GEMM targets in FP32,
+    // from the first iteration whose predecessor's update is within 100
+    // tolerance units, or from iteration 16.  PMCMC_REFINE=0 turns it off,
+    // PMCMC_REFINE=all runs it from the first iteration.
+    const char* rv = getenv("PMCMC_REFINE");
+    const bool refine = sizeof(T) == 4 && mode == MODE_GEMM && !full && tgt->D <= 256 && !(rv && !strcmp(rv, "0"));
+    const double refine_band = 1e-3;
+    const double refine_gate = rv && !strcmp(rv, "all") ? INFINITY : 100.0;
+    RefineTarget rtg{};
+    if (refine) {
+        rtg.D = D;
+        if (tgt->kind == TGT_LOGISTIC) {
+            rtg.kind = 0;
+            rtg.N = tgt->glm.N;
+            rtg.X64 = tgt->glm.Xd64;
+            rtg.y64 = tgt->glm.y64;
+            rtg.inv_s2 = 1.0 / (tgt->prior_scale * tgt->prior_scale);
+        } else {
+            rtg.kind = 1;
+            rtg.L64 = tgt->gauss.L64;
+            rtg.mu64 = tgt->gauss.mu64;
+        }
+        std::string msg;
+        int rc = refine_work_ensure(&w->refw, rtg, Tn, refine_band, &msg);
+        if (rc == PMCMC_OK) rc = refine_reset(&w->refw, c->stream, &msg);
+        if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+    }
+    // decisions of FP32 iterates within 1e-6 of the threshold are held (the
+    // tile kernel's hysteresis); a fill kernel, not a memset, starts them unset
+    const bool hold = sizeof(T) == 4 && !full;
+    if (hold) {
+        if (!w->prev_dec) HIPCHK(c, hipMalloc(&w->prev_dec, (size_t)Tn));
+        hipLaunchKernelGGL(fill_i8_kernel, dim3(grid_for(Tn)), dim3(256), 0, c->stream, w->prev_dec, (int8_t)-1, Tn);
+    }
     T* bufs[2] = {(T*)w->buf[0], (T*)w->buf[1]};
     hipLaunchKernelGGL(reset_sync_kernel, dim3(1), dim3(64), 0, c->stream, w->sync);
     hipLaunchKernelGGL((init_iterate_kernel<T>), dim3(grid_for(D * Tn)), dim3(256), 0, c->stream,
@@ -831,6 +904,8 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
     a.probes = p->probes;
     a.jacobian = p->jacobian;
     a.seed = p->seed;
+    a.prev_dec = hold ? w->prev_dec : nullptr;
+    a.hyst = hold ? 1e-6 : 0.0;
     if (tgt->kind == TGT_DIAG) {
         a.prec = sizeof(T) == 8 ? (const T*)tgt->prec_d : (const T*)tgt->prec_f;
         a.mean = sizeof(T) == 8 ? (const T*)tgt->mean_d : (const T*)tgt->mean_f;
@@ -875,6 +950,32 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
                 a.hw = (const T*)w->glmw.hw;
                 a.lpx = w->glmw.lpx;
                 a.lpy = w->glmw.lpy;
+                a.dec = nullptr;
+                a.refine_on = nullptr;
+                if (refine) {
+                    RefineArgs ra{};
+                    ra.x0 = (const float*)x0;
+                    ra.S_in = (const float*)a.S_in;
+                    ra.Xi = (const float*)Xi;
+                    ra.U = (const float*)U;
+                    ra.Y = (const float*)a.Y;
+                    ra.gx = (const float*)a.gx;
+                    ra.gy = (const float*)a.gy;
+                    ra.lpx = a.lpx;
+                    ra.lpy = a.lpy;
+                    ra.eps = p->eps;
+                    ra.band = refine_band;
+                    ra.skip = &w->sync->converged;
+                    ra.last_metric = &w->sync->last_metric;
+                    ra.gate = refine_gate;
+                    ra.iter = it;
+                    ra.gate_iters = 16;
+                    ra.hyst = a.hyst;
+                    rc = refine_decisions(&w->refw, rtg, ra, it, c->stream, &msg);
+                    if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+                    a.dec = w->refw.dec;
+                    a.refine_on = w->refw.cnt + 2;
+                }
                 launch_deer_iteration<T, MODE_GEMM>(w, a, c->stream, &c->prof);
             } else {
                 launch_deer_iteration<T, MODE_ELEM>(w, a, c->stream, &c->prof);
@@ -1122,7 +1223,8 @@ __global__ __launch_bounds__(256) void batch_copy_kernel(const BCopy* t) {
 }
 
 // Scan of the level-0 tile aggregates of each chain (one workgroup per chain,
-// rows over threads): prev[:, k] = state after tile k, from s0.
+// rows over threads): prev[:, k] = Newton correction after tile k, from zero
+// (the window's start state is fixed).
 __global__ __launch_bounds__(256) void batch_agg_scan_kernel(const TileArgs<float>* args, const int* chains) {
     const TileArgs<float>& a = args[chains[blockIdx.x]];
     if (a.skip && *(volatile const int*)a.skip) return;
@@ -1130,7 +1232,7 @@ __global__ __launch_bounds__(256) void batch_agg_scan_kernel(const TileArgs<floa
     const int nT = a.numTiles;
     float* prev = const_cast<float*>(a.prev);
     for (int64_t d = threadIdx.x; d < D; d += 256) {
-        float s = a.s0[d];
+        float s = 0.0f;
         int k = 0;
         for (; k + 4 <= nT; k += 4) {
             float aa[4], bb[4];
@@ -1179,6 +1281,7 @@ struct BatchChain {
 
 struct pmcmc_batch {
     pmcmc_ctx* ctx = nullptr;
+    bool ctx_ref = false;  // holds a context reference (set once created)
     std::vector<BatchChain> ch;
     pmcmc_deer_params p{};
     int64_t W = 0;
@@ -1250,7 +1353,9 @@ extern "C" int pmcmc_batch_destroy(pmcmc_batch* b) {
         hipStreamSynchronize(b->ctx->stream);
     }
     batch_free(b);
+    pmcmc_ctx* c = b->ctx_ref ? b->ctx : nullptr;
     delete b;
+    if (c) ctx_release(c);
     return PMCMC_OK;
 }
 
@@ -1354,6 +1459,8 @@ extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_des
         h.prev = (float*)take(agg);
         if (h.tgt->kind == TGT_LOGISTIC) h.part = (float*)q;
     }
+    ctx_retain(c);
+    b->ctx_ref = true;
     *out = b;
     return PMCMC_OK;
 }

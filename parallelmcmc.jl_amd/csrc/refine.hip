@@ -1,0 +1,332 @@
+// FP64 recompute of the accept decision near the uniform threshold, for the
+// GEMM targets in FP32 / BF16-Jacobian precision.
+//
+// The MALA accept test (src/MALA/MALA.jl:589-597) compares log u with log
+// alpha.  In the FP32 path log alpha is formed in double from FP32 GEMM
+// products (log-densities, gradients, the proposal Y); their rounding moves it
+// by up to ~5e-5 on the logistic workloads, which decides the test only when
+// log alpha is that close to log u.  Per Newton iteration:
+//
+//   flag    one wave per column re-forms log alpha from the FP32 products
+//           (as the tile kernel does), writes dec[t] = -1 and appends the
+//           columns with |log alpha - log u| < band to a compact list;
+//   pass 0  FP64 gradient / log-density partials at the state Xbar_t of every
+//           listed column, split over row ranges of X (logistic) or column
+//           ranges of Lambda (Gaussian) so one column's evaluation is spread
+//           over many workgroups;
+//   fin 0   fixed-order sum of the partials, FP64 proposal
+//           y = x + eps g(x) + sqrt(2 eps) xi;
+//   pass 1  the same partials at y;
+//   fin 1   FP64 log alpha and the decision (dec[t] bit 0; bit 1 flags a
+//           margin below the tile kernel's hysteresis), which overrides the
+//           FP32 decision in the tile kernel.
+//
+// The state x is the iterate's FP32 value, exact in double; the target data
+// are the FP64 copies.  Each pass reads one counter, so with nothing flagged
+// the four FP64 launches exit at once.
+//
+// Decisions matter only at the fixed point, so the FP64 work is gated: it
+// starts once the previous iteration's update metric is within `gate`
+// tolerance units, or at iteration `gate_iters` (FP32 decisions of columns
+// closer to the threshold than their ~1e-5 error flip as the iterate's last
+// ulps settle, which can keep the metric up for ever), and then stays on (a
+// gate that opened and closed could alternate FP32 and FP64 decisions of one
+// column for ever).  The tile kernel
+// counts an iteration as converged only once the gate is on, so the returned
+// trajectory always carries FP64-checked decisions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "../../include/pmcmc_amd.h"
+#include "refine.hpp"
+
+namespace pmcmc {
+
+static constexpr int RNT = 256;
+static constexpr int RMAXD = 256;
+// list capacity beyond the expected count: as many columns as 32 MiB of partials hold
+static constexpr int64_t REFINE_PART_BUDGET = 32ll << 20;
+
+__device__ inline double softplus_r(double x) { return fmax(x, 0.0) + log1p(exp(-fabs(x))); }
+__device__ inline double sigmoid_r(double x) {
+    if (x >= 0.0) return 1.0 / (1.0 + exp(-x));
+    const double e = exp(x);
+    return e / (1.0 + e);
+}
+
+__device__ inline double wave_sum_r(double v) {
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// sum over the 256-thread block; every thread gets the total.  The trailing
+// barrier also orders all earlier LDS reads before the caller's next writes.
+__device__ inline double block_sum(double v, double* red) {
+    v = wave_sum_r(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double s = (red[0] + red[1]) + (red[2] + red[3]);
+    __syncthreads();
+    return s;
+}
+
+// Xbar_t[d]: x0 for t = 0, else column t-1 of the iterate
+__device__ inline double state_at(const RefineArgs& a, int64_t t, int64_t d, int64_t D) {
+    return t == 0 ? (double)a.x0[d] : (double)a.S_in[(size_t)(t - 1) * D + d];
+}
+
+__device__ inline bool skipped(const RefineArgs& a) { return a.skip && *(volatile const int*)a.skip; }
+
+__global__ __launch_bounds__(RNT) void refine_flag_kernel(RefineArgs a, int8_t* dec, int* cnt, int* list, int lmax,
+                                                         int par, int64_t D, int64_t T) {
+    if (skipped(a)) return;
+    // every block reaches the same answer: cnt[2] only turns on when the gate
+    // condition (fixed during this launch) holds
+    const bool on = cnt[2] != 0 || *a.last_metric <= a.gate || a.iter >= a.gate_iters;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cnt[par ^ 1] = 0;
+        if (on) cnt[2] = 1;
+    }
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * (RNT / 64) + (threadIdx.x >> 6);
+    if (t >= T) return;
+    if (!on) {
+        if (lane == 0) dec[t] = -1;
+        return;
+    }
+    const double eps = a.eps;
+    double qyx = 0.0, qxy = 0.0;
+    for (int64_t d = lane; d < D; d += 64) {
+        const size_t gi = (size_t)t * D + d;
+        const double x = state_at(a, t, d, D);
+        const double y = (double)a.Y[gi];
+        const double e1 = y - x - eps * (double)a.gx[gi];
+        const double e2 = x - y - eps * (double)a.gy[gi];
+        qyx += e1 * e1;
+        qxy += e2 * e2;
+    }
+    qyx = wave_sum_r(qyx);
+    qxy = wave_sum_r(qxy);
+    if (lane == 0) {
+        const double c_norm = ((double)D / 2.0) * log(4.0 * 3.14159265358979323846 * eps);
+        const double la = (a.lpy[t] + (-0.5 * qxy / (2.0 * eps) - c_norm)) -
+                          (a.lpx[t] + (-0.5 * qyx / (2.0 * eps) - c_norm));
+        const double lu = log((double)a.U[t]);
+        dec[t] = -1;
+        if (fabs(la - lu) < a.band) {
+            const int j = atomicAdd(&cnt[par], 1);
+            if (j < lmax) list[j] = (int)t;
+        }
+    }
+}
+
+// partials of the FP64 gradient and log-density of listed column j at
+// v = Xbar_t (phase 0) or at its FP64 proposal (phase 1); blockIdx.x = split
+__global__ __launch_bounds__(RNT) void refine_pass_kernel(RefineTarget tg, RefineArgs a, RefineWork w, int par,
+                                                         int phase) {
+    if (skipped(a)) return;
+    __shared__ double v[RMAXD], gacc[RMAXD], rch[RNT], red[4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t D = w.D;
+    const int n_ent = min(w.cnt[par], w.lmax);
+    const int64_t split = blockIdx.x;
+    for (int j = blockIdx.y; j < n_ent; j += gridDim.y) {
+        const int64_t t = w.list[j];
+        for (int64_t d = tid; d < D; d += RNT) {
+            v[d] = phase == 0 ? state_at(a, t, d, D) : w.y[(size_t)j * D + d];
+            gacc[d] = 0.0;
+        }
+        __syncthreads();
+        double* out = w.part + ((size_t)j * w.nsplit + split) * (D + 1);
+        double lpp = 0.0;
+        if (tg.kind == 0) {
+            // rows [r0, r1) of X: z = X v, r = y - sigmoid(z), grad partial X^T r
+            const int64_t N = tg.N;
+            const int64_t r0 = split * w.rows;
+            const int64_t r1 = std::min(N, r0 + w.rows);
+            for (int64_t n0 = r0; n0 < r1; n0 += RNT) {
+                const int64_t n = n0 + tid;
+                double rr = 0.0;
+                if (n < r1) {
+                    double z = 0.0;
+                    for (int64_t d = 0; d < D; ++d) z += tg.X64[n + d * N] * v[d];
+                    const double yv = tg.y64[n];
+                    lpp += -yv * softplus_r(-z) - (1.0 - yv) * softplus_r(z);
+                    rr = yv - sigmoid_r(z);
+                }
+                rch[tid] = rr;
+                __syncthreads();
+                const int64_t m = std::min((int64_t)RNT, r1 - n0);
+                for (int64_t d = wv; d < D; d += RNT / 64) {
+                    double s = 0.0;
+                    for (int64_t k = lane; k < m; k += 64) s += tg.X64[n0 + k + d * N] * rch[k];
+                    s = wave_sum_r(s);
+                    if (lane == 0) gacc[d] += s;
+                }
+                __syncthreads();
+            }
+            for (int64_t d = tid; d < D; d += RNT) out[d] = gacc[d];
+        } else {
+            // columns [k0, k1) of Lambda: grad partial -Lambda[:, k0:k1] (v - mu)[k0:k1];
+            // log-density 0.5 (v - mu).grad is linear in the partials
+            const int64_t k0 = split * w.rows;
+            const int64_t k1 = std::min(D, k0 + w.rows);
+            for (int64_t d = tid; d < D; d += RNT) {
+                double p = 0.0;
+                for (int64_t k = k0; k < k1; ++k) p += tg.L64[d + k * D] * (v[k] - (tg.mu64 ? tg.mu64[k] : 0.0));
+                out[d] = -p;
+                lpp += 0.5 * (v[d] - (tg.mu64 ? tg.mu64[d] : 0.0)) * (-p);
+            }
+        }
+        const double s = block_sum(lpp, red);
+        if (tid == 0) out[D] = s;
+    }
+}
+
+// fixed-order sum of the partials of listed column j; phase 0 forms the FP64
+// proposal, phase 1 the FP64 log alpha and the decision
+__global__ __launch_bounds__(RNT) void refine_finish_kernel(RefineTarget tg, RefineArgs a, RefineWork w, int par,
+                                                           int phase) {
+    if (skipped(a)) return;
+    __shared__ double red[4];
+    const int tid = threadIdx.x;
+    const int64_t D = w.D;
+    const int n_ent = min(w.cnt[par], w.lmax);
+    const double eps = a.eps;
+    const double sq2e = sqrt(2.0 * eps);
+    for (int j = blockIdx.x; j < n_ent; j += gridDim.x) {
+        const int64_t t = w.list[j];
+        const double* pj = w.part + (size_t)j * w.nsplit * (D + 1);
+        double lp = 0.0;
+        for (int s = 0; s < w.nsplit; ++s) lp += pj[(size_t)s * (D + 1) + D];
+        double nrm = 0.0, qyx = 0.0, qxy = 0.0;
+        for (int64_t d = tid; d < D; d += RNT) {
+            double g = 0.0;
+            for (int s = 0; s < w.nsplit; ++s) g += pj[(size_t)s * (D + 1) + d];
+            const double x = state_at(a, t, d, D);
+            const size_t jd = (size_t)j * D + d;
+            if (phase == 0) {
+                if (tg.kind == 0) {
+                    g -= x * tg.inv_s2;
+                    nrm += x * x;
+                }
+                w.g0[jd] = g;
+                w.y[jd] = x + eps * g + sq2e * (double)a.Xi[(size_t)t * D + d];
+            } else {
+                const double y = w.y[jd];
+                if (tg.kind == 0) {
+                    g -= y * tg.inv_s2;
+                    nrm += y * y;
+                }
+                const double e1 = y - x - eps * w.g0[jd];
+                const double e2 = x - y - eps * g;
+                qyx += e1 * e1;
+                qxy += e2 * e2;
+            }
+        }
+        lp -= 0.5 * tg.inv_s2 * block_sum(nrm, red);
+        if (phase == 0) {
+            if (tid == 0) w.lp0[j] = lp;
+        } else {
+            qyx = block_sum(qyx, red);
+            qxy = block_sum(qxy, red);
+            if (tid == 0) {
+                const double c_norm = ((double)D / 2.0) * log(4.0 * 3.14159265358979323846 * eps);
+                const double la = (lp + (-0.5 * qxy / (2.0 * eps) - c_norm)) -
+                                  (w.lp0[j] + (-0.5 * qyx / (2.0 * eps) - c_norm));
+                const double lu = log((double)a.U[t]);
+                w.dec[t] = (int8_t)((lu < la ? 1 : 0) | (fabs(la - lu) < a.hyst ? 2 : 0));
+            }
+        }
+    }
+}
+
+__global__ void refine_zero_kernel(int* cnt) {
+    if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+}
+
+void refine_work_destroy(RefineWork* w) {
+    hipFree(w->dec);
+    hipFree(w->cnt);
+    hipFree(w->list);
+    hipFree(w->part);
+    hipFree(w->g0);
+    hipFree(w->y);
+    hipFree(w->lp0);
+    *w = RefineWork{};
+}
+
+int refine_work_ensure(RefineWork* w, const RefineTarget& tg, int64_t T, double band, std::string* msg) {
+    if (tg.D < 1 || tg.D > RMAXD || T < 1 || (tg.kind == 0 && tg.N < 1)) {
+        *msg = "decision refinement: unsupported target shape";
+        return PMCMC_ERR_UNSUPPORTED;
+    }
+    int64_t rows, nsplit;
+    if (tg.kind == 0) {
+        rows = std::max<int64_t>(128, (tg.N + 63) / 64);
+        nsplit = (tg.N + rows - 1) / rows;
+    } else {
+        rows = 32;
+        nsplit = (tg.D + rows - 1) / rows;
+    }
+    const int64_t per_entry = nsplit * (tg.D + 1) * (int64_t)sizeof(double);
+    const int64_t lm = std::min(T, std::max((int64_t)std::ceil(16.0 * band * (double)T), REFINE_PART_BUDGET / per_entry));
+    const int64_t N = tg.kind == 0 ? tg.N : 0;
+    if (w->dec && w->D == tg.D && w->T == T && w->N == N && w->kind == tg.kind && w->lmax == lm &&
+        w->nsplit == nsplit)
+        return PMCMC_OK;
+    refine_work_destroy(w);
+    bool ok = hipMalloc(&w->dec, (size_t)T) == hipSuccess;
+    ok = ok && hipMalloc(&w->cnt, 4 * sizeof(int)) == hipSuccess;
+    ok = ok && hipMalloc(&w->list, (size_t)lm * sizeof(int)) == hipSuccess;
+    ok = ok && hipMalloc(&w->part, (size_t)lm * nsplit * (tg.D + 1) * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&w->g0, (size_t)lm * tg.D * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&w->y, (size_t)lm * tg.D * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&w->lp0, (size_t)lm * sizeof(double)) == hipSuccess;
+    if (!ok) {
+        refine_work_destroy(w);
+        *msg = "decision refinement: buffer allocation failed";
+        return PMCMC_ERR_ALLOC;
+    }
+    w->D = tg.D;
+    w->T = T;
+    w->N = N;
+    w->kind = tg.kind;
+    w->lmax = (int)lm;
+    w->nsplit = (int)nsplit;
+    w->rows = rows;
+    return PMCMC_OK;
+}
+
+int refine_reset(RefineWork* w, hipStream_t s, std::string* msg) {
+    hipLaunchKernelGGL(refine_zero_kernel, dim3(1), dim3(64), 0, s, w->cnt);
+    if (hipGetLastError() != hipSuccess) {
+        *msg = "decision refinement: reset launch failed";
+        return PMCMC_ERR_HIP;
+    }
+    return PMCMC_OK;
+}
+
+int refine_decisions(RefineWork* w, const RefineTarget& tg, const RefineArgs& a, int iter, hipStream_t s,
+                     std::string* msg) {
+    const int par = iter & 1;
+    const int64_t cols_per_block = RNT / 64;
+    hipLaunchKernelGGL(refine_flag_kernel, dim3((unsigned)((w->T + cols_per_block - 1) / cols_per_block)), dim3(RNT),
+                       0, s, a, w->dec, w->cnt, w->list, w->lmax, par, w->D, w->T);
+    // 16 slots of listed columns (2 band T ~ 8 expected at T = 4096, band 1e-3);
+    // more are taken in turn by the same workgroups
+    const unsigned gy = (unsigned)std::min(w->lmax, 16);
+    for (int phase = 0; phase < 2; ++phase) {
+        hipLaunchKernelGGL(refine_pass_kernel, dim3((unsigned)w->nsplit, gy), dim3(RNT), 0, s, tg, a, *w, par, phase);
+        hipLaunchKernelGGL(refine_finish_kernel, dim3(gy), dim3(RNT), 0, s, tg, a, *w, par, phase);
+    }
+    if (hipGetLastError() != hipSuccess) {
+        *msg = "decision refinement kernels failed to launch";
+        return PMCMC_ERR_HIP;
+    }
+    return PMCMC_OK;
+}
+
+}  // namespace pmcmc

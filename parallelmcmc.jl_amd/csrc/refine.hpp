@@ -1,0 +1,88 @@
+// FP64 recompute of accept decisions near the threshold (GEMM targets, FP32).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace pmcmc {
+
+// FP64 view of a GEMM target: kind 0 logistic (X64 N x D column-major, y64,
+// 1/prior_scale^2), kind 1 dense Gaussian (L64 D x D column-major, mu64 or null).
+struct RefineTarget {
+    int kind;
+    int64_t D, N;
+    const double* X64;
+    const double* y64;
+    double inv_s2;
+    const double* L64;
+    const double* mu64;
+};
+
+// One Newton iteration's columns: the iterate (Xbar = [x0, S_in]), tape,
+// FP32 products and log-densities; dec[t] receives -1 (keep the FP32
+// decision) or, where |log alpha - log u| < band, the FP64 decision in bit 0
+// and bit 1 set when the FP64 margin is below hyst.
+struct RefineArgs {
+    const float* x0;
+    const float* S_in;
+    const float* Xi;
+    const float* U;
+    const float* Y;
+    const float* gx;
+    const float* gy;
+    const double* lpx;
+    const double* lpy;
+    double eps;
+    double band;
+    const int* skip;
+    // gate: the FP64 work starts at the first iteration whose predecessor's
+    // update metric (in tolerance units, the solve's SyncState) is <= gate,
+    // or at iteration gate_iters (FP32 decisions near the threshold can keep
+    // the metric from ever falling), and stays on for the rest of the solve
+    const double* last_metric;
+    double gate;
+    int iter;
+    int gate_iters;
+    // FP64 margins below this set the tie bit (bit 1) of dec[t]
+    double hyst;
+};
+
+// Device buffers of the refinement.  Flagged columns are compacted into
+// list[0 .. min(ctl[par], lmax)); the counter is double-buffered by iteration
+// parity (ctl[0], ctl[1]) so the flag pass of one iteration clears the one the
+// next iteration appends to.  ctl[2] is the sticky gate flag.
+// Each flagged column's FP64 gradient is split over nsplit workgroups (row
+// ranges of X for the logistic target, column ranges of Lambda for the
+// Gaussian) whose partials part[j][split][0..D] (D gradient entries and one
+// log-density term) are summed in a fixed order.
+struct RefineWork {
+    int64_t D = 0, T = 0, N = 0;
+    int kind = 0;
+    int lmax = 0, nsplit = 0;
+    int64_t rows = 0;       // rows of X (logistic) or columns of Lambda (Gaussian) per split
+    int8_t* dec = nullptr;  // T: -1 keep, else bit 0 accept, bit 1 tie
+    int* cnt = nullptr;     // ctl: 2 counters, gate flag, spare
+    int* list = nullptr;    // lmax flagged column indices
+    double* part = nullptr;  // lmax x nsplit x (D + 1)
+    double* g0 = nullptr;    // lmax x D: FP64 gradient at the state
+    double* y = nullptr;     // lmax x D: FP64 proposal
+    double* lp0 = nullptr;   // lmax: FP64 log-density at the state
+};
+
+// Size the buffers for a target, trajectory length and band (reuses buffers
+// of the same shape).  lmax is at least 16x the expected number of flagged
+// columns (at most 2 band T for any iterate, since u is uniform: the density
+// of log u is u <= 1) and at least what 32 MiB of partials hold, capped at T.
+// Flagged columns beyond lmax keep their FP32 decision.
+int refine_work_ensure(RefineWork* w, const RefineTarget& tg, int64_t T, double band, std::string* msg);
+void refine_work_destroy(RefineWork* w);
+// clear the counters and the gate flag (once per solve, before the first iteration)
+int refine_reset(RefineWork* w, hipStream_t s, std::string* msg);
+// one iteration's refinement: flag pass, then FP64 gradient and
+// log-density at the state and at the proposal of every flagged column
+int refine_decisions(RefineWork* w, const RefineTarget& tg, const RefineArgs& a, int iter, hipStream_t s,
+                     std::string* msg);
+
+}  // namespace pmcmc

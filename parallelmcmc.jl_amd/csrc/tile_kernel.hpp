@@ -28,6 +28,17 @@
 //   MODE_GEMM: as MODE_ELEM but the gradients / HVPs / log-densities of the
 //              current and proposed states come precomputed by the GEMM
 //              target kernels.
+// The DEER modes scan the Newton correction, not the state:
+//   d_t = A_t d_{t-1} + (FT_t - S_t),  d_{-1} = 0,  S_new_t = S_t + damping d_t
+// (S_t replaced by 0 where it is not finite)
+// (the same update as s_t = A_t s_{t-1} + FT_t - A_t Xbar_t with
+// s_t = S_t + d_t).  Its fixed point is exact in floating point: once the
+// prefix up to t-1 has stopped moving, d_t = FT_t - S_t is exact (the two are
+// close, Sterbenz) and S_t + d_t lands on FT_t and stays there, whatever the
+// rounding of A_t.  The state form lands within an ulp of FT_t that moves
+// with every iteration's probes, and that jitter alone can flip an accept
+// decision sitting within ~1e-6 of its threshold, for ever.
+// MODE_SCAN with s0 == nullptr starts from zero.
 #pragma once
 
 #include <type_traits>
@@ -43,6 +54,14 @@ enum { PASS_REDUCE = 0, PASS_APPLY = 1, PASS_SINGLE = 2, PASS_CHAINED = 3 };
 #define PMCMC_LB_SPIN_CAP (1u << 21)
 enum { JAC_STOCH = 0, JAC_DIAG = 1 };
 #define PMCMC_MAXP 4
+
+// Base of the Newton correction of one entry: the iterate, or 0 where the
+// iterate is not finite, so that a non-finite region of S_in shrinks by (at
+// least) one column per iteration, as it does in the state form.
+template <typename T>
+__device__ inline T corr_base(T v) {
+    return isfinite(v) ? v : (T)0;
+}
 
 template <typename T>
 struct TileArgs {
@@ -95,6 +114,20 @@ struct TileArgs {
     const T* hw;  // H(y) w_k
     const double* lpx;
     const double* lpy;
+    // MODE_GEMM: FP64 decision override per column (-1 none; else bit 0 the
+    // decision, bit 1 set when |log alpha - log u| < hyst), or null
+    const int8_t* dec;
+    // Hysteresis of FP32 iterates (null / 0 for FP64): where |log alpha - log u|
+    // < hyst the previous iteration's decision (prev_dec, -1 none) is kept.
+    // Near the fixed point the last ulps of the iterate settle one column per
+    // iteration from the front; until they do, log alpha of a later column
+    // moves by ~1e-7 per ulp, and a decision that close to its threshold would
+    // flip back and forth, each flip moving the whole tail.
+    int8_t* prev_dec;
+    double hyst;
+    // with dec: nonzero once the FP64 override is active; an iteration that
+    // meets the tolerance before then does not count as converged
+    const int* refine_on;
 };
 
 // LDS layout (elements of T):
@@ -224,7 +257,7 @@ __device__ void chained_lookback(const TileArgs<T>& a, const int tile, const int
     int* st = a.lb_status;
     if (tile == 0) {
         for (int64_t r = tid; r < D; r += PMCMC_NT) {
-            const T s = a.s0[r];
+            const T s = a.s0 ? a.s0[r] : (T)0;
             sin_[r] = s;
             lb_store<T>(&a.lb_incl[r], ga[r] * s + gb[r]);
         }
@@ -345,7 +378,8 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
     // incoming state of the tile (needed only by the output passes)
     if (OUT && PASS != PASS_CHAINED) {
         for (int64_t r = tid; r < D; r += PMCMC_NT)
-            sin_[r] = (PASS == PASS_SINGLE || tile == 0) ? a.s0[r] : a.prev[(size_t)(tile - 1) * D + r];
+            sin_[r] = (PASS == PASS_SINGLE || tile == 0) ? ((DEERM || !a.s0) ? (T)0 : a.s0[r])
+                                                         : a.prev[(size_t)(tile - 1) * D + r];
     }
     __syncthreads();
 
@@ -442,13 +476,28 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
             const double la = (s_lpy + lq_xy) - (s_lpx + lq_yx);
             const double lu = log((double)a.U[t]);
             const double g = 1.0 / (1.0 + exp(-(la - lu)));
-            const bool acc = lu < la;
+            bool acc = lu < la;
+            bool tie = fabs(la - lu) < a.hyst;
+            if (MODE == MODE_GEMM && a.dec) {
+                const int8_t o = a.dec[t];
+                if (o >= 0) {  // FP64 decision (bit 0) and its tie flag (bit 1)
+                    acc = (o & 1) != 0;
+                    tie = (o & 2) != 0;
+                }
+            }
+            // decisions an FP32 iterate cannot resolve keep their previous value
+            // (all lanes read before lane 0 writes: one column per group)
+            if (a.prev_dec && tie) {
+                const int8_t pv = a.prev_dec[t];
+                if (pv >= 0) acc = pv != 0;
+            }
             const double gg = g * (1.0 - g);
             double dg[PMCMC_MAXP];
 #pragma unroll
             for (int k = 0; k < PMCMC_MAXP; ++k) dg[k] = gg * (d1[k] - d2[k] - d3[k] / (2.0 * eps));
             if (OUT && lane == 0) {
                 accl[c] = acc ? 1 : 0;
+                if (a.prev_dec) a.prev_dec[t] = acc ? 1 : 0;
                 if (a.acc_out) a.acc_out[t] = acc ? 1 : 0;
                 if (a.la_out) a.la_out[t] = la;
             }
@@ -492,9 +541,8 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                     }
                     if (K > 1) Av *= 1.0 / (double)K;
                 }
-                const T Ar = (T)Av;
-                acol[d] = Ar;
-                bcol[d] = (T)(ft - (double)Ar * x);
+                acol[d] = (T)Av;
+                bcol[d] = (T)(ft - (double)corr_base(xcol[D + d]));  // FT_t - S_t
             }
         }
         __syncthreads();
@@ -571,7 +619,8 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                 al[li] = s;
             } else {
                 const T old = xs[li + D];
-                const T v = (a.damping == 1.0) ? s : ((T)1 - damp) * old + damp * s;
+                const T base = corr_base(old);
+                const T v = (a.damping == 1.0) ? base + s : base + damp * s;
                 al[li] = v;
                 dmax = nan_max(dmax, fabs((double)v - (double)old));
                 smax = nan_max(smax, fabs((double)v));
@@ -612,7 +661,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                 const int it = a.iter >= 0 ? a.iter : sy->last_iter + 1;
                 sy->last_metric = metric;
                 sy->last_iter = it;
-                if (metric <= 1.0) {
+                if (metric <= 1.0 && (!a.refine_on || *(volatile const int*)a.refine_on)) {
                     sy->conv_iter = it;
                     sy->converged = 1;
                 }

@@ -144,3 +144,110 @@ def test_logistic_deer_solve_bf16_jac_mini_c1():
     assert info.converged, info
     assert O.trace_error(S.cpu().numpy().astype(np.float64), ref) < 1e-5
     assert np.array_equal(info.accepted.cpu().numpy().astype(bool), acc_ref)
+
+
+def test_fp64_decision_refinement_near_threshold(monkeypatch):
+    """Columns whose log alpha sits within 1e-7 of log u: the FP32 products
+    cannot decide them (their rounding moves log alpha by ~1e-5), the FP64
+    recompute must reproduce the FP64 decisions exactly.  Negative control:
+    with the recompute off, some of them flip."""
+    from ParallelMCMC import DEER, DensityModel, LogisticTarget
+
+    rng = np.random.default_rng(77)
+    N, D, T, eps = 2000, 20, 512, 1e-2
+    Xd, y, _ = O.make_logreg_data(rng, N, D)
+    tgt = O.LogReg(Xd, y)
+    model = DensityModel(dim=D, target=LogisticTarget(Xd, y))
+    # states from the chain itself (burned in), so log alpha spreads around 0
+    Xb, Ub = O.make_tape(rng, D, 1000)
+    Sb, _, _ = O.run_mala_sequential_taped(tgt, np.zeros(D), eps, Xb, Ub)
+    x0 = Sb[:, -1].astype(np.float32)
+    Xi = rng.standard_normal((D, T)).astype(np.float32)
+    S_in, _, _ = O.run_mala_sequential_taped(tgt, x0.astype(np.float64), eps, Xi.astype(np.float64), rng.random(T))
+    S_in = S_in.astype(np.float32)
+    Xbar = O._xbar(x0.astype(np.float64), S_in.astype(np.float64))
+    la = O.mala_step_batched_fwd_and_jvp(tgt, Xbar, eps, Xi.astype(np.float64), np.full(T, 0.5),
+                                         np.ones((D, T))).logalpha
+    near = la < -1e-5
+    delta = np.where(np.arange(T) % 2 == 0, 1e-7, -1e-7) * (1.0 + np.abs(la))
+    U = np.where(near, np.exp(la - delta), rng.uniform(0.05, 0.95, T))
+    U32 = U.astype(np.float32)
+    lu = np.log(U32.astype(np.float64))
+    want = lu < la
+    assert near.sum() > 50
+    rec = DEER.TapedRecursion(model, eps, Xi, U32, precision="f32")
+
+    def decisions():
+        _, info = DEER.solve(rec, x0, init=S_in, tol_abs=0.0, tol_rel=0.0, maxiter=1, damping=1.0,
+                             return_info=True, copy_result=True)
+        return info.accepted.cpu().numpy().astype(bool)
+
+    # maxiter=1 with tol 0: the gate (previous update within 100 tolerance
+    # units) never opens, so run the recompute from the first iteration
+    monkeypatch.setenv("PMCMC_REFINE", "all")
+    got = decisions()
+    monkeypatch.setenv("PMCMC_REFINE", "0")
+    raw = decisions()
+    print(f"near-threshold columns {near.sum()}: mismatches refined {np.sum(got != want)}, "
+          f"FP32 only {np.sum(raw != want)}", flush=True)
+    assert np.array_equal(got, want)
+    assert np.sum(raw[near] != want[near]) > 0
+
+
+def test_fp64_decisions_on_converged_trajectory(monkeypatch):
+    """Solve from scratch with the default gated FP64 recompute on a tape whose
+    every twelfth column sits 4e-6 from its accept threshold along the FP64
+    chain (inside the FP32 products' log-alpha error, up to ~1e-5 here; such
+    columns slow DEER itself down, the FP64 oracle needs 73 iterations).  Every
+    returned decision must equal the FP64 decision evaluated at the returned
+    trajectory, except columns within 2e-6 of the threshold there (the last
+    iteration decides at its input iterate, which differs from the returned
+    one by FP32 noise, ~1e-7, moving log alpha by < 1e-6).  Negative control:
+    the FP32-only solve gets some of them wrong."""
+    from ParallelMCMC import DEER, DensityModel, LogisticTarget
+
+    rng = np.random.default_rng(78)
+    N, D, T, eps = 2000, 20, 768, 1e-2
+    Xd, y, _ = O.make_logreg_data(rng, N, D)
+    tgt = O.LogReg(Xd, y)
+    model = DensityModel(dim=D, target=LogisticTarget(Xd, y))
+    Xb, Ub = O.make_tape(rng, D, 1000)
+    Sb, _, _ = O.run_mala_sequential_taped(tgt, np.zeros(D), eps, Xb, Ub)
+    x0 = Sb[:, -1].astype(np.float32)
+    Xi = rng.standard_normal((D, T)).astype(np.float32)
+    U = rng.uniform(0.05, 0.95, T)
+    x = x0.astype(np.float64)
+    for t in range(T):  # the FP64 chain, placing u near the threshold as it goes
+        yv, _ = O.mala_proposal(tgt, x, eps, Xi[:, t].astype(np.float64))
+        la = O.mala_logalpha(tgt, x, yv, eps)
+        if t % 12 == 0 and la < -1e-4:
+            U[t] = np.exp(la - (4e-6 if t % 24 == 0 else -4e-6))
+        if np.log(U[t]) < la:
+            x = yv
+    U32 = U.astype(np.float32)
+    lu = np.log(U32.astype(np.float64))
+    rec = DEER.TapedRecursion(model, eps, Xi, U32, precision="f32")
+
+    def run(mode):
+        if mode is None:
+            monkeypatch.delenv("PMCMC_REFINE", raising=False)
+        else:
+            monkeypatch.setenv("PMCMC_REFINE", mode)
+        S, info = DEER.solve(rec, x0, tol_abs=1e-6, tol_rel=1e-6, maxiter=500, damping=1.0, seed=3,
+                             return_info=True, copy_result=True)
+        assert info.converged, f"PMCMC_REFINE={mode}: iters {info.iters}, metric {info.metric}"
+        S = S.cpu().numpy().astype(np.float64)
+        la = O.mala_step_batched_fwd_and_jvp(tgt, O._xbar(x0.astype(np.float64), S), eps,
+                                             Xi.astype(np.float64), np.full(T, 0.5), np.ones((D, T))).logalpha
+        margin = np.abs(la - lu)
+        got = info.accepted.cpu().numpy().astype(bool)
+        checked = margin > 2e-6
+        return np.sum(got[checked] != (lu < la)[checked]), np.sum(checked & (margin < 2e-5)), info.iters
+
+    bad, close, iters = run(None)
+    bad_raw, close_raw, iters_raw = run("0")
+    print(f"columns within 2e-5 of the threshold: {close} / {close_raw}; mismatches gated FP64 {bad} "
+          f"({iters} iterations), FP32 only {bad_raw} ({iters_raw} iterations)", flush=True)
+    assert close >= 25
+    assert bad == 0
+    assert bad_raw > 0
