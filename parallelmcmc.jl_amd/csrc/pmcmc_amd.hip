@@ -6,6 +6,7 @@
 // syncs two maxima to the host every iteration, src/DEER/DEER.jl:567-571).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
@@ -854,7 +855,10 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
     // PMCMC_REFINE=all runs it from the first iteration.
     const char* rv = getenv("PMCMC_REFINE");
     const bool refine = sizeof(T) == 4 && mode == MODE_GEMM && !full && tgt->D <= 256 && !(rv && !strcmp(rv, "0"));
-    const double refine_band = 1e-3;
+    // band: above the FP32 products' log-alpha error, which grows about like
+    // N for the logistic target (max 4.6e-5 at N = 10000, D = 100 and 2e-5 at
+    // N = 2000, D = 250 over 4096 columns each); 1e-3 for the Gaussian
+    const double refine_band = tgt->kind == TGT_LOGISTIC ? std::max(1e-4, 2.5e-8 * (double)tgt->glm.N) : 1e-3;
     const double refine_gate = rv && !strcmp(rv, "all") ? INFINITY : 100.0;
     RefineTarget rtg{};
     if (refine) {

@@ -122,89 +122,142 @@ __global__ __launch_bounds__(RNT) void refine_flag_kernel(RefineArgs a, int8_t* 
     }
 }
 
-// partials of the FP64 gradient and log-density of listed column j at
-// v = Xbar_t (phase 0) or at its FP64 proposal (phase 1); blockIdx.x = split
-__global__ __launch_bounds__(RNT) void refine_pass_kernel(RefineTarget tg, RefineArgs a, RefineWork w, int par,
-                                                         int phase) {
+// Logistic partials of listed column j at v = Xbar_t (phase 0) or at its FP64
+// proposal (phase 1), over rows [split * rows, ...) of X: z = X v,
+// r = y - sigmoid(z), grad partial X^T r, log-likelihood partial.  A block
+// takes 64-row chunks (lane = row); wave w owns the columns d = w + 4q of X,
+// holds that chunk's X[n, d] in registers (all Q loads in flight at once) for
+// both products, and reduces its X^T r accumulators across lanes once at the
+// end.  Q >= ceil(D / 4).
+template <int Q>
+__global__ __launch_bounds__(RNT) void refine_pass_glm_kernel(RefineTarget tg, RefineArgs a, RefineWork w, int par,
+                                                             int phase) {
     if (skipped(a)) return;
-    __shared__ double v[RMAXD], gacc[RMAXD], rch[RNT], red[4];
+    __shared__ double v[RMAXD], zpart[RNT / 64][64], red[4];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int64_t D = w.D;
+    const int64_t D = w.D, N = tg.N;
     const int n_ent = min(w.cnt[par], w.lmax);
     const int64_t split = blockIdx.x;
+    const int64_t r0 = split * w.rows;
+    const int64_t r1 = std::min(N, r0 + w.rows);
     for (int j = blockIdx.y; j < n_ent; j += gridDim.y) {
         const int64_t t = w.list[j];
-        for (int64_t d = tid; d < D; d += RNT) {
-            v[d] = phase == 0 ? state_at(a, t, d, D) : w.y[(size_t)j * D + d];
-            gacc[d] = 0.0;
-        }
+        for (int64_t d = tid; d < RMAXD; d += RNT)
+            v[d] = d >= D ? 0.0 : (phase == 0 ? state_at(a, t, d, D) : w.y[(size_t)j * D + d]);
         __syncthreads();
-        double* out = w.part + ((size_t)j * w.nsplit + split) * (D + 1);
-        double lpp = 0.0;
-        if (tg.kind == 0) {
-            // rows [r0, r1) of X: z = X v, r = y - sigmoid(z), grad partial X^T r
-            const int64_t N = tg.N;
-            const int64_t r0 = split * w.rows;
-            const int64_t r1 = std::min(N, r0 + w.rows);
-            for (int64_t n0 = r0; n0 < r1; n0 += RNT) {
-                const int64_t n = n0 + tid;
-                double rr = 0.0;
-                if (n < r1) {
-                    double z = 0.0;
-                    for (int64_t d = 0; d < D; ++d) z += tg.X64[n + d * N] * v[d];
-                    const double yv = tg.y64[n];
-                    lpp += -yv * softplus_r(-z) - (1.0 - yv) * softplus_r(z);
-                    rr = yv - sigmoid_r(z);
-                }
-                rch[tid] = rr;
-                __syncthreads();
-                const int64_t m = std::min((int64_t)RNT, r1 - n0);
-                for (int64_t d = wv; d < D; d += RNT / 64) {
-                    double s = 0.0;
-                    for (int64_t k = lane; k < m; k += 64) s += tg.X64[n0 + k + d * N] * rch[k];
-                    s = wave_sum_r(s);
-                    if (lane == 0) gacc[d] += s;
-                }
-                __syncthreads();
+        double gacc[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) gacc[q] = 0.0;
+        double ll = 0.0;
+        for (int64_t n0 = r0; n0 < r1; n0 += 64) {
+            const int64_t n = n0 + lane;
+            const bool row = n < r1;
+            const int64_t nc = row ? n : r0;  // padding lanes read a valid row, weight 0
+            double xr[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int64_t d = wv + 4 * q;
+                const int64_t dc = d < D ? d : 0;
+                xr[q] = tg.X64[nc + dc * N] * (d < D ? 1.0 : 0.0);
             }
-            for (int64_t d = tid; d < D; d += RNT) out[d] = gacc[d];
-        } else {
-            // columns [k0, k1) of Lambda: grad partial -Lambda[:, k0:k1] (v - mu)[k0:k1];
-            // log-density 0.5 (v - mu).grad is linear in the partials
-            const int64_t k0 = split * w.rows;
-            const int64_t k1 = std::min(D, k0 + w.rows);
-            for (int64_t d = tid; d < D; d += RNT) {
-                double p = 0.0;
-                for (int64_t k = k0; k < k1; ++k) p += tg.L64[d + k * D] * (v[k] - (tg.mu64 ? tg.mu64[k] : 0.0));
-                out[d] = -p;
-                lpp += 0.5 * (v[d] - (tg.mu64 ? tg.mu64[d] : 0.0)) * (-p);
+            double zp = 0.0;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) zp += xr[q] * v[wv + 4 * q];
+            zpart[wv][lane] = zp;
+            __syncthreads();
+            const double z = (zpart[0][lane] + zpart[1][lane]) + (zpart[2][lane] + zpart[3][lane]);
+            double r = 0.0;
+            if (row) {
+                const double yv = tg.y64[n];
+                r = yv - sigmoid_r(z);
+                if (wv == 0) ll += -yv * softplus_r(-z) - (1.0 - yv) * softplus_r(z);
             }
+#pragma unroll
+            for (int q = 0; q < Q; ++q) gacc[q] += xr[q] * r;
+            __syncthreads();  // zpart is rewritten by the next chunk
         }
-        const double s = block_sum(lpp, red);
-        if (tid == 0) out[D] = s;
+        double* out = w.part + ((size_t)j * w.nsplit + split) * (D + 1);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const double sq = wave_sum_r(gacc[q]);
+            const int64_t d = wv + 4 * q;
+            if (lane == 0 && d < D) out[d] = sq;
+        }
+        const double sl = block_sum(ll, red);
+        if (tid == 0) out[D] = sl;
     }
 }
 
-// fixed-order sum of the partials of listed column j; phase 0 forms the FP64
-// proposal, phase 1 the FP64 log alpha and the decision
+// Gaussian partials over columns [split * rows, ...) of Lambda:
+// grad partial -Lambda[:, k0:k1] (v - mu)[k0:k1]; the log-density
+// 0.5 (v - mu).grad is linear in the partials
+__global__ __launch_bounds__(RNT) void refine_pass_gauss_kernel(RefineTarget tg, RefineArgs a, RefineWork w, int par,
+                                                               int phase) {
+    if (skipped(a)) return;
+    __shared__ double v[RMAXD], red[4];
+    const int tid = threadIdx.x;
+    const int64_t D = w.D;
+    const int n_ent = min(w.cnt[par], w.lmax);
+    const int64_t split = blockIdx.x;
+    const int64_t k0 = split * w.rows;
+    const int64_t k1 = std::min(D, k0 + w.rows);
+    for (int j = blockIdx.y; j < n_ent; j += gridDim.y) {
+        const int64_t t = w.list[j];
+        for (int64_t d = tid; d < D; d += RNT)
+            v[d] = (phase == 0 ? state_at(a, t, d, D) : w.y[(size_t)j * D + d]) - (tg.mu64 ? tg.mu64[d] : 0.0);
+        __syncthreads();
+        double* out = w.part + ((size_t)j * w.nsplit + split) * (D + 1);
+        double lpp = 0.0;
+        for (int64_t d = tid; d < D; d += RNT) {
+            double p = 0.0;
+            for (int64_t k = k0; k < k1; ++k) p += tg.L64[d + k * D] * v[k];
+            out[d] = -p;
+            lpp += 0.5 * v[d] * (-p);
+        }
+        const double sl = block_sum(lpp, red);
+        if (tid == 0) out[D] = sl;
+    }
+}
+
+// fixed-order sum of the partials of listed column j (wave w takes the splits
+// w, w + 4, ..., eight loads in flight; the four waves combine in order);
+// phase 0 forms the FP64 proposal, phase 1 the FP64 log alpha and the decision
 __global__ __launch_bounds__(RNT) void refine_finish_kernel(RefineTarget tg, RefineArgs a, RefineWork w, int par,
                                                            int phase) {
     if (skipped(a)) return;
-    __shared__ double red[4];
-    const int tid = threadIdx.x;
+    __shared__ double gs[RMAXD + 1], wpart[RNT / 64][64], red[4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t D = w.D;
+    const int ns = w.nsplit;
     const int n_ent = min(w.cnt[par], w.lmax);
     const double eps = a.eps;
     const double sq2e = sqrt(2.0 * eps);
     for (int j = blockIdx.x; j < n_ent; j += gridDim.x) {
         const int64_t t = w.list[j];
-        const double* pj = w.part + (size_t)j * w.nsplit * (D + 1);
-        double lp = 0.0;
-        for (int s = 0; s < w.nsplit; ++s) lp += pj[(size_t)s * (D + 1) + D];
+        const double* pj = w.part + (size_t)j * ns * (D + 1);
+        for (int64_t dc = 0; dc <= D; dc += 64) {  // index D is the log-density term
+            const int64_t d = dc + lane;
+            double acc = 0.0;
+            if (d <= D) {
+                int s = wv;
+                for (; s + 28 < ns; s += 32) {
+                    double p[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) p[u] = pj[(size_t)(s + 4 * u) * (D + 1) + d];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) acc += p[u];
+                }
+                for (; s < ns; s += 4) acc += pj[(size_t)s * (D + 1) + d];
+            }
+            wpart[wv][lane] = acc;
+            __syncthreads();
+            if (wv == 0 && d <= D) gs[d] = (wpart[0][lane] + wpart[1][lane]) + (wpart[2][lane] + wpart[3][lane]);
+            __syncthreads();
+        }
+        double lp = gs[D];
         double nrm = 0.0, qyx = 0.0, qxy = 0.0;
         for (int64_t d = tid; d < D; d += RNT) {
-            double g = 0.0;
-            for (int s = 0; s < w.nsplit; ++s) g += pj[(size_t)s * (D + 1) + d];
+            double g = gs[d];
             const double x = state_at(a, t, d, D);
             const size_t jd = (size_t)j * D + d;
             if (phase == 0) {
@@ -265,7 +318,8 @@ int refine_work_ensure(RefineWork* w, const RefineTarget& tg, int64_t T, double 
     }
     int64_t rows, nsplit;
     if (tg.kind == 0) {
-        rows = std::max<int64_t>(128, (tg.N + 63) / 64);
+        // 64-row chunks, at most 64 splits
+        rows = 64 * std::max<int64_t>(1, (tg.N + 64 * 64 - 1) / (64 * 64));
         nsplit = (tg.N + rows - 1) / rows;
     } else {
         rows = 32;
@@ -318,8 +372,18 @@ int refine_decisions(RefineWork* w, const RefineTarget& tg, const RefineArgs& a,
     // 16 slots of listed columns (2 band T ~ 8 expected at T = 4096, band 1e-3);
     // more are taken in turn by the same workgroups
     const unsigned gy = (unsigned)std::min(w->lmax, 16);
+    const dim3 pg((unsigned)w->nsplit, gy);
     for (int phase = 0; phase < 2; ++phase) {
-        hipLaunchKernelGGL(refine_pass_kernel, dim3((unsigned)w->nsplit, gy), dim3(RNT), 0, s, tg, a, *w, par, phase);
+        if (tg.kind == 1)
+            hipLaunchKernelGGL(refine_pass_gauss_kernel, pg, dim3(RNT), 0, s, tg, a, *w, par, phase);
+        else if (w->D <= 32)
+            hipLaunchKernelGGL(refine_pass_glm_kernel<8>, pg, dim3(RNT), 0, s, tg, a, *w, par, phase);
+        else if (w->D <= 64)
+            hipLaunchKernelGGL(refine_pass_glm_kernel<16>, pg, dim3(RNT), 0, s, tg, a, *w, par, phase);
+        else if (w->D <= 128)
+            hipLaunchKernelGGL(refine_pass_glm_kernel<32>, pg, dim3(RNT), 0, s, tg, a, *w, par, phase);
+        else
+            hipLaunchKernelGGL(refine_pass_glm_kernel<64>, pg, dim3(RNT), 0, s, tg, a, *w, par, phase);
         hipLaunchKernelGGL(refine_finish_kernel, dim3(gy), dim3(RNT), 0, s, tg, a, *w, par, phase);
     }
     if (hipGetLastError() != hipSuccess) {
