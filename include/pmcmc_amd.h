@@ -101,6 +101,26 @@ int pmcmc_scan_diag_f32(pmcmc_ctx* ctx, const float* A, const float* B, const fl
 int pmcmc_scan_diag_f64(pmcmc_ctx* ctx, const double* A, const double* B, const double* s0,
                         double* S, int64_t D, int64_t T);
 
+/* ---- owned matmul and reductions ----------------------------------------
+ * src/ParallelMCMC.jl:23-25: pmcmc_matmul(A, B) = A * B,
+ * pmcmc_dot(a, b) = dot(a, b), pmcmc_dotsum(A, B) = sum(A .* B).
+ * Device pointers, column-major; prec PMCMC_PREC_F32 or PMCMC_PREC_F64.
+ * pmcmc_matmul: C (m x n, ldc) = op(A) op(B) with op(A) m x k, op(B) k x n,
+ * op = transpose when trans_* != 0 (the pullbacks of ext/EnzymeExt.jl:78-284
+ * are dY B^T and A^T dY); k = 0 gives zeros; C aliases neither operand;
+ * lda >= max(1, rows of A as stored), likewise ldb, ldc >= max(1, m).
+ * MFMA tiles (f32 or f64), FP32 accumulation for FP32 operands.
+ * pmcmc_dot / pmcmc_dotsum: FP64 accumulation in a fixed order; the result
+ * (one element of the operands' type) is written to device memory at out.
+ * pmcmc_dotsum sums over an m x n block of A and B (leading dims lda, ldb).
+ * All three synchronise the context stream.                                */
+int pmcmc_matmul(pmcmc_ctx* ctx, int prec, int trans_a, int trans_b, int64_t m, int64_t n,
+                 int64_t k, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
+                 int64_t ldc);
+int pmcmc_dot(pmcmc_ctx* ctx, int prec, int64_t n, const void* a, const void* b, void* out);
+int pmcmc_dotsum(pmcmc_ctx* ctx, int prec, int64_t m, int64_t n, const void* A, int64_t lda,
+                 const void* B, int64_t ldb, void* out);
+
 /* ---- targets -------------------------------------------------------------
  * Kernels cannot call user closures, so a DensityModel carries one of these
  * descriptors (the analytic counterpart of its logdensity / grad / hvp

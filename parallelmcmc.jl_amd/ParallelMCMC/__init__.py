@@ -3,8 +3,9 @@ run by hand-written gfx950 HIP kernels behind the C ABI of
 include/pmcmc_amd.h.
 
 Surface mirrored from the reference package (src/ParallelMCMC.jl:32-41):
-``DensityModel``, ``ParallelMALASampler``, ``sample``, ``step`` and the
-``DEER``, ``DEERScan``, ``MALA`` submodules.
+``DensityModel``, ``ParallelMALASampler``, ``sample``, ``step``, the
+``DEER``, ``DEERScan``, ``MALA`` submodules and the owned ``pmcmc_matmul``,
+``pmcmc_dot``, ``pmcmc_dotsum``.
 """
 from . import _lib
 from ._lib import ArgumentError, DimensionMismatch, PmcmcError
@@ -20,6 +21,7 @@ from .interface import (
 from .targets import DiagGaussianTarget, GaussianTarget, LogisticTarget, StdNormalTarget
 from . import DEER, DEERScan, MALA  # noqa: E402
 from . import Adjoint, Batched  # noqa: E402
+from .owned import pmcmc_dot, pmcmc_dotsum, pmcmc_matmul  # noqa: E402
 
 __all__ = [
     "DensityModel",
@@ -34,6 +36,9 @@ __all__ = [
     "MALA",
     "Adjoint",
     "Batched",
+    "pmcmc_matmul",
+    "pmcmc_dot",
+    "pmcmc_dotsum",
     "DiagGaussianTarget",
     "StdNormalTarget",
     "LogisticTarget",

@@ -20,6 +20,7 @@
 #include "adjoint.hpp"
 #include "common.hpp"
 #include "fullscan.hpp"
+#include "linalg.hpp"
 #include "refine.hpp"
 #include "gauss.hpp"
 #include "glm.hpp"
@@ -53,6 +54,7 @@ struct pmcmc_ctx {
     void* scan_blob = nullptr;
     size_t scan_blob_cap = 0;
     SyncState* host_sync = nullptr;  // pinned
+    double* red_scratch = nullptr;   // partial sums of pmcmc_dot / pmcmc_dotsum
     Prof prof;
     // the creator's reference plus one per live target, workspace and batch
     // created on this context: pmcmc_ctx_destroy drops the creator's, the
@@ -367,6 +369,7 @@ static void ctx_release(pmcmc_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->scan_blob) hipFree(c->scan_blob);
+    if (c->red_scratch) hipFree(c->red_scratch);
     if (c->host_sync) hipHostFree(c->host_sync);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -696,6 +699,52 @@ extern "C" int pmcmc_mala_fwd_and_jvp(pmcmc_ctx* c, const pmcmc_target* tgt, int
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PMCMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// owned matmul and reductions (src/ParallelMCMC.jl:23-25)
+// ---------------------------------------------------------------------------
+static bool linalg_prec(int prec) { return prec == PMCMC_PREC_F32 || prec == PMCMC_PREC_F64; }
+
+extern "C" int pmcmc_matmul(pmcmc_ctx* c, int prec, int trans_a, int trans_b, int64_t m, int64_t n, int64_t k,
+                            const void* A, int64_t lda, const void* B, int64_t ldb, void* C, int64_t ldc) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    if (!linalg_prec(prec)) return fail(c, PMCMC_ERR_ARGUMENT, "matmul: precision must be F32 or F64");
+    if (m < 0 || n < 0 || k < 0) return fail(c, PMCMC_ERR_DIMENSION, "matmul: negative size");
+    const int64_t rows_a = trans_a ? k : m, rows_b = trans_b ? n : k;
+    if (lda < (rows_a > 1 ? rows_a : 1) || ldb < (rows_b > 1 ? rows_b : 1) || ldc < (m > 1 ? m : 1))
+        return fail(c, PMCMC_ERR_DIMENSION, "matmul: leading dimension smaller than the stored rows");
+    if (m > 0 && n > 0 && (!C || (k > 0 && (!A || !B))))
+        return fail(c, PMCMC_ERR_ARGUMENT, "matmul: null operand");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::string msg;
+    const int rc = linalg_gemm(prec, trans_a != 0, trans_b != 0, m, n, k, A, lda, B, ldb, C, ldc, c->stream, &msg);
+    if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_dotsum(pmcmc_ctx* c, int prec, int64_t m, int64_t n, const void* A, int64_t lda,
+                            const void* B, int64_t ldb, void* out) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    if (!linalg_prec(prec)) return fail(c, PMCMC_ERR_ARGUMENT, "dotsum: precision must be F32 or F64");
+    if (m < 0 || n < 0) return fail(c, PMCMC_ERR_DIMENSION, "dotsum: negative size");
+    if (lda < (m > 1 ? m : 1) || ldb < (m > 1 ? m : 1))
+        return fail(c, PMCMC_ERR_DIMENSION, "dotsum: leading dimension smaller than m");
+    if (!out || (m > 0 && n > 0 && (!A || !B))) return fail(c, PMCMC_ERR_ARGUMENT, "dotsum: null operand");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->red_scratch) HIPCHK(c, hipMalloc(&c->red_scratch, linalg_dotsum_scratch_bytes()));
+    std::string msg;
+    const int rc = linalg_dotsum(prec, m, n, A, lda, B, ldb, out, c->red_scratch, c->stream, &msg);
+    if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_dot(pmcmc_ctx* c, int prec, int64_t n, const void* a, const void* b, void* out) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    if (n < 0) return fail(c, PMCMC_ERR_DIMENSION, "dot: negative length");
+    return pmcmc_dotsum(c, prec, n, 1, a, n > 1 ? n : 1, b, n > 1 ? n : 1, out);
 }
 
 extern "C" int pmcmc_target_logdensity_batch(pmcmc_ctx* c, const pmcmc_target* tgt, int prec,
