@@ -19,6 +19,7 @@ the FP64 sequential taped MALA run (trace error, accept decisions).
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--precision both|f32|bf16_jac]
     python bench.py --workload scan --scan-D 64 --scan-T 1048576
+    python bench.py --workload scanfull   # dense-Jacobian scan D=64 T=16384 (materialised J)
     python bench.py --workload full       # correlated Gaussian D=64 T=16384, dense Jacobian
     python bench.py --workload c3         # 256 uneven chains, batched BF16 vs per-chain FP32
     python bench.py --workload adjoint    # gradient of mean |x_t|^2, logistic T=32768
@@ -156,6 +157,49 @@ def bench_scan(args):
         "dtype": args.scan_dtype, "config": {"workload": f"scan D={D_} T={T_}",
                                               "kernel_us": kern_s * 1e6,
                                               "host_us_per_call": 1e6 * el / args.steps},
+        "roofline": {"bound": "hbm", "achieved": bytes_ / kern_s / 1e9, "peak": 8000.0, "unit": "GB/s",
+                     "frac": bytes_ / kern_s / 8e12, "traffic": None},
+    }))
+
+
+def bench_scanfull(args):
+    """Standalone dense-Jacobian scan (pmcmc_scan_full_*): GB/s of the algorithmic
+    (D^2 + 2 D) T element bytes (every J_t, B and S once), timed with HIP events on
+    the library stream around the whole scan (reduce, recursive spine, apply)."""
+    import torch
+
+    from ParallelMCMC import _device
+
+    dev = torch.device("cuda", 0)
+    dt = torch.float32 if args.scan_dtype == "f32" else torch.float64
+    D_, T_ = args.scan_D, args.scanfull_T
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    J = torch.randn(T_, D_, D_, dtype=dt, device=dev, generator=g).mul_(0.45 / D_ ** 0.5)  # column-major per t
+    B = _device.colmajor_empty(D_, T_, dt, dev).normal_(generator=g)
+    s0 = torch.zeros(D_, dtype=dt, device=dev)
+    S = _device.colmajor_empty(D_, T_, dt, dev)
+    ctx = _device.context(0)
+    name = "pmcmc_scan_full_f32" if dt == torch.float32 else "pmcmc_scan_full_f64"
+    call = lambda: ctx.call(name, _device.ptr(J), _device.ptr(B), _device.ptr(s0), _device.ptr(S), D_, T_)
+    for _ in range(args.warmup):
+        call()
+    ctx.reset_kernel_times()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        call()
+    el = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    n, ms = ctx.kernel_times()["iteration"]
+    bytes_ = (D_ * D_ + 2 * D_) * T_ * J.element_size()
+    kern_s = ms / n / 1e3
+    print(json.dumps({
+        "metric": "dense-Jacobian affine scan throughput", "value": bytes_ / kern_s / 1e9, "unit": "GB/s",
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
+        "dtype": args.scan_dtype, "config": {"workload": f"dense scan D={D_} T={T_}",
+                                              "kernel_us": kern_s * 1e6,
+                                              "gflops": 2.0 * D_ ** 3 * T_ / kern_s / 1e9},
         "roofline": {"bound": "hbm", "achieved": bytes_ / kern_s / 1e9, "peak": 8000.0, "unit": "GB/s",
                      "frac": bytes_ / kern_s / 8e12, "traffic": None},
     }))
@@ -421,7 +465,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--damping", type=float, default=1.0)
-    ap.add_argument("--workload", default="deer", choices=["deer", "scan", "full", "c3", "adjoint"])
+    ap.add_argument("--workload", default="deer", choices=["deer", "scan", "scanfull", "full", "c3", "adjoint"])
     ap.add_argument("--c3-chains", type=int, default=256)
     ap.add_argument("--c3-baseline-every", type=int, default=8,
                     help="per-chain FP32 baseline on every k-th pair of chains (one Gaussian, one logistic; 1 = all)")
@@ -429,6 +473,7 @@ def main():
     ap.add_argument("--scan-D", type=int, default=64)
     ap.add_argument("--scan-T", type=int, default=1 << 20)
     ap.add_argument("--scan-dtype", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--scanfull-T", type=int, default=16384)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -437,6 +482,9 @@ def main():
 
     if args.workload == "scan":
         bench_scan(args)
+        return
+    if args.workload == "scanfull":
+        bench_scanfull(args)
         return
     if args.workload == "full":
         bench_full(args)

@@ -101,6 +101,20 @@ int pmcmc_scan_diag_f32(pmcmc_ctx* ctx, const float* A, const float* B, const fl
 int pmcmc_scan_diag_f64(pmcmc_ctx* ctx, const double* A, const double* B, const double* s0,
                         double* S, int64_t D, int64_t T);
 
+/* ---- dense-Jacobian affine scan -----------------------------------------
+ * S[:,t] = J_t S[:,t-1] + B[:,t], S[:,0] uses s0, with J_t = J + t*D*D the
+ * t-th D x D column-major matrix; D <= 64.  The general form of the :full
+ * Newton update for callers that hold their Jacobians; no reference
+ * counterpart (the reference rejects :full, src/DEER/DEER.jl:464-465; its
+ * diagonal scan is src/DEER/DEERScan.jl:102-173).  Tile maps composed on chip
+ * (64 x 64 products per column), a recursive scan of the tile maps, then
+ * each tile run from its incoming state.  S aliases neither J nor B.
+ * T == 0 is a no-op.  Synchronises the context stream.                     */
+int pmcmc_scan_full_f32(pmcmc_ctx* ctx, const float* J, const float* B, const float* s0,
+                        float* S, int64_t D, int64_t T);
+int pmcmc_scan_full_f64(pmcmc_ctx* ctx, const double* J, const double* B, const double* s0,
+                        double* S, int64_t D, int64_t T);
+
 /* ---- owned matmul and reductions ----------------------------------------
  * src/ParallelMCMC.jl:23-25: pmcmc_matmul(A, B) = A * B,
  * pmcmc_dot(a, b) = dot(a, b), pmcmc_dotsum(A, B) = sum(A .* B).

@@ -101,6 +101,8 @@ _SIGS = {
     "pmcmc_buffer_download": ([_P, _P, _P, _I64], C.c_int),
     "pmcmc_scan_diag_f32": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
     "pmcmc_scan_diag_f64": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
+    "pmcmc_scan_full_f32": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
+    "pmcmc_scan_full_f64": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
     "pmcmc_matmul": ([_P, C.c_int, C.c_int, C.c_int, _I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64], C.c_int),
     "pmcmc_dot": ([_P, C.c_int, _I64, _P, _P, _P], C.c_int),
     "pmcmc_dotsum": ([_P, C.c_int, _I64, _I64, _P, _I64, _P, _I64, _P], C.c_int),

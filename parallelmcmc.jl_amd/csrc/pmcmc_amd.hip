@@ -22,6 +22,7 @@
 #include "fullscan.hpp"
 #include "linalg.hpp"
 #include "refine.hpp"
+#include "scanfull.hpp"
 #include "gauss.hpp"
 #include "glm.hpp"
 #include "tile_kernel.hpp"
@@ -535,6 +536,54 @@ extern "C" int pmcmc_scan_diag_f32(pmcmc_ctx* c, const float* A, const float* B,
 extern "C" int pmcmc_scan_diag_f64(pmcmc_ctx* c, const double* A, const double* B,
                                    const double* s0, double* S, int64_t D, int64_t T) {
     return scan_diag<double>(c, A, B, s0, S, D, T);
+}
+
+// dense-Jacobian scan with materialised Jacobians (no reference counterpart)
+static int scan_full_entry(pmcmc_ctx* c, bool f64, const void* J, const void* B, const void* s0, void* S, int64_t D,
+                           int64_t Tn) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    if (D < 1 || Tn < 0)
+        return fail(c, PMCMC_ERR_DIMENSION, "dense scan: need D >= 1 and T >= 0 (D=%lld T=%lld)", (long long)D,
+                    (long long)Tn);
+    if (D > 64) return fail(c, PMCMC_ERR_UNSUPPORTED, "dense scan: D=%lld > 64", (long long)D);
+    if (Tn == 0) return PMCMC_OK;
+    if (!J || !B || !s0 || !S) return fail(c, PMCMC_ERR_ARGUMENT, "dense scan: null pointer");
+    const size_t esz = f64 ? 8 : 4;
+    const size_t sb = (size_t)D * Tn * esz;
+    auto overlap = [&](const void* p, size_t pb) {
+        const char* a0 = (const char*)p;
+        const char* b0 = (const char*)S;
+        return a0 < b0 + sb && b0 < a0 + pb;
+    };
+    if (overlap(J, sb * (size_t)D)) return fail(c, PMCMC_ERR_ARGUMENT, "S must not alias J");
+    if (overlap(B, sb)) return fail(c, PMCMC_ERR_ARGUMENT, "S must not alias B");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t need = scan_full_scratch_bytes(Tn, f64);
+    if (need > c->scan_blob_cap) {
+        if (c->scan_blob) HIPCHK(c, hipFree(c->scan_blob));
+        c->scan_blob = nullptr;
+        c->scan_blob_cap = 0;
+        HIPCHK(c, hipMalloc(&c->scan_blob, need));
+        c->scan_blob_cap = need;
+    }
+    std::string msg;
+    const int e = c->prof.begin(KT_ITERATION, 0, c->stream);
+    const int rc = scan_full(f64, J, B, s0, S, D, Tn, c->scan_blob, c->stream, &msg);
+    c->prof.end(e, c->stream);
+    if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->prof.flush(0);
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_scan_full_f32(pmcmc_ctx* c, const float* J, const float* B, const float* s0, float* S,
+                                   int64_t D, int64_t T) {
+    return scan_full_entry(c, false, J, B, s0, S, D, T);
+}
+
+extern "C" int pmcmc_scan_full_f64(pmcmc_ctx* c, const double* J, const double* B, const double* s0, double* S,
+                                   int64_t D, int64_t T) {
+    return scan_full_entry(c, true, J, B, s0, S, D, T);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,261 @@
+// Dense-Jacobian affine scan with materialised Jacobians, D <= 64:
+//   S[:,t] = J_t S[:,t-1] + B[:,t],  S[:,-1] = s0,
+// J_t the t-th D x D column-major matrix (J + t*strideJ, leading dim ldJ).
+// The reference has no such scan (it rejects :full, src/DEER/DEER.jl:464-465;
+// its diagonal scan is src/DEER/DEERScan.jl:102-173); this is the general form
+// of the structured full-Jacobian solve in fullscan.hip, for callers that
+// hold their Jacobians.
+//
+//   reduce  one workgroup per tile of TC columns composes the tile's map
+//           [M | r] <- J_t [M | r] + [0 | B_t] from [I | 0] (64 x 64 x 65 FMAs
+//           per column on 4 x 4 register blocks; J_t in LDS, J_{t+1} loaded
+//           into registers while J_t is used)
+//   spine   the tile maps form a dense scan of their own over the tiles
+//           (J' = M_k, B' = r_k, from s0); it is solved by the same kernels
+//           with 8-column tiles, recursively, down to one tile
+//   apply   one workgroup per tile runs its columns from the incoming state
+//           (four waves split the k sum of each mat-vec; the columns' J and B
+//           stream through a ring of SAPF register slots)
+// Padded rows and columns (>= D) are zero in every map, so they stay out of
+// every product.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/pmcmc_amd.h"
+#include "scanfull.hpp"
+
+namespace pmcmc {
+
+static constexpr int SF = 64;       // padded dimension
+static constexpr int SFS = SF + 4;  // LDS row stride (keeps 4-wide reads aligned)
+static constexpr int SAPF = 4;      // apply prefetch depth
+
+template <typename T>
+struct SFArgs {
+    const T* J;
+    int64_t ldJ, strideJ;
+    const T* B;
+    int64_t ldB;
+    const T* s0;    // D entries
+    const T* prev;  // state after each tile of this level (ld SF), from the level above
+    T* S;
+    int64_t ldS;
+    int64_t D, len;
+    int TC, nT;
+    T* Magg;  // per tile SF x SF, column-major (ld SF)
+    T* ragg;  // per tile SF
+};
+
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec4<double> {
+    typedef double type __attribute__((ext_vector_type(4)));
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void sfull_reduce_kernel(SFArgs<T> p) {
+    typedef typename Vec4<T>::type v4;
+    __shared__ __attribute__((aligned(32))) T Jl[SF][SFS];  // Jl[k][i] = J_t(i, k)
+    __shared__ __attribute__((aligned(32))) T Ml[SF][SFS];  // Ml[k][j] = M(k, j)
+    __shared__ T rl[SF];
+    const int tid = threadIdx.x, ib = tid & 15, jb = tid >> 4;
+    const int64_t D = p.D;
+    const int64_t tile = blockIdx.x;
+    const int64_t t0 = tile * p.TC;
+    const int tc = (int)std::min<int64_t>(p.TC, p.len - t0);
+    for (int e = tid; e < SF * SF; e += 256) Ml[e >> 6][e & 63] = (e >> 6) == (e & 63) ? (T)1 : (T)0;
+    if (tid < SF) rl[tid] = 0;
+    T jr[16];
+    T br = 0, bcur = 0;
+    auto fetch = [&](int64_t t) {
+        const T* Jt = p.J + t * p.strideJ;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int e = tid + 256 * q;
+            const int k = e >> 6, i = e & 63;
+            jr[q] = (i < D && k < D) ? Jt[i + k * p.ldJ] : (T)0;
+        }
+        if (tid < SF) br = tid < D ? p.B[t * p.ldB + tid] : (T)0;
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int e = tid + 256 * q;
+            Jl[e >> 6][e & 63] = jr[q];
+        }
+        bcur = br;
+    };
+    fetch(t0);
+    stash();
+    __syncthreads();
+    for (int s = 0; s < tc; ++s) {
+        if (s + 1 < tc) fetch(t0 + s + 1);
+        T acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = 0;
+#pragma unroll 8
+        for (int k = 0; k < SF; ++k) {
+            const v4 av = *reinterpret_cast<const v4*>(&Jl[k][4 * ib]);
+            const v4 bv = *reinterpret_cast<const v4*>(&Ml[k][4 * jb]);
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+        }
+        T rn = 0;
+        if (tid < SF) {
+            T r0 = 0, r1 = 0;
+#pragma unroll 8
+            for (int k = 0; k < SF; k += 2) {
+                r0 += Jl[k][tid] * rl[k];
+                r1 += Jl[k + 1][tid] * rl[k + 1];
+            }
+            rn = (r0 + r1) + bcur;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            v4 row;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) row[b] = acc[a][b];
+            *reinterpret_cast<v4*>(&Ml[4 * ib + a][4 * jb]) = row;
+        }
+        if (tid < SF) rl[tid] = rn;
+        if (s + 1 < tc) stash();
+        __syncthreads();
+    }
+    T* Mo = p.Magg + (size_t)tile * SF * SF;
+    for (int e = tid; e < SF * SF; e += 256) Mo[e] = Ml[e & 63][e >> 6];  // column-major: e = i + 64 j
+    if (tid < SF) p.ragg[(size_t)tile * SF + tid] = rl[tid];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void sfull_apply_kernel(SFArgs<T> p) {
+    __shared__ T v[SF];
+    __shared__ T part[4][SF];
+    const int tid = threadIdx.x, i = tid & 63, q = tid >> 6;
+    const int64_t D = p.D;
+    const int64_t tile = blockIdx.x;
+    const int64_t t0 = tile * p.TC;
+    const int tc = (int)std::min<int64_t>(p.TC, p.len - t0);
+    if (tid < SF) v[i] = i >= D ? (T)0 : (tile == 0 ? p.s0[i] : p.prev[(size_t)(tile - 1) * SF + i]);
+    T jr[SAPF][16];
+    T br[SAPF];
+    auto fetch = [&](int s, int sl) {
+        const T* Jt = p.J + (t0 + s) * p.strideJ;
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            const int k = 16 * q + kk;
+            jr[sl][kk] = (i < D && k < D) ? Jt[i + k * p.ldJ] : (T)0;
+        }
+        br[sl] = (q == 0 && i < D) ? p.B[(t0 + s) * p.ldB + i] : (T)0;
+    };
+#pragma unroll
+    for (int sl = 0; sl < SAPF; ++sl)
+        if (sl < tc) fetch(sl, sl);
+    __syncthreads();
+    for (int s0 = 0; s0 < tc; s0 += SAPF) {
+#pragma unroll
+        for (int sl = 0; sl < SAPF; ++sl) {
+            const int s = s0 + sl;
+            if (s < tc) {
+                T a0 = 0, a1 = 0;
+#pragma unroll
+                for (int kk = 0; kk < 16; kk += 2) {
+                    a0 += jr[sl][kk] * v[16 * q + kk];
+                    a1 += jr[sl][kk + 1] * v[16 * q + kk + 1];
+                }
+                const T bc = br[sl];
+                if (s + SAPF < tc) fetch(s + SAPF, sl);
+                part[q][i] = a0 + a1;
+                __syncthreads();
+                if (q == 0) {
+                    const T nv = ((part[0][i] + part[1][i]) + (part[2][i] + part[3][i])) + bc;
+                    v[i] = nv;
+                    if (i < D) p.S[(t0 + s) * p.ldS + i] = nv;
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static int level_tc(int level) { return level == 0 ? 64 : 8; }
+
+template <typename T>
+static size_t sfull_bytes_t(int64_t T_len) {
+    size_t bytes = 0;
+    int64_t len = T_len;
+    for (int level = 0; len > level_tc(level); ++level) {
+        const int64_t nT = (len + level_tc(level) - 1) / level_tc(level);
+        bytes += (size_t)nT * (SF * SF + 2 * SF) * sizeof(T);
+        len = nT;
+    }
+    return bytes;
+}
+
+size_t scan_full_scratch_bytes(int64_t T, bool f64) { return f64 ? sfull_bytes_t<double>(T) : sfull_bytes_t<float>(T); }
+
+template <typename T>
+static void sfull_rec(int level, const T* J, int64_t ldJ, int64_t strideJ, const T* B, int64_t ldB, const T* s0,
+                      T* S, int64_t ldS, int64_t D, int64_t len, char* scratch, hipStream_t st) {
+    SFArgs<T> a{};
+    a.J = J;
+    a.ldJ = ldJ;
+    a.strideJ = strideJ;
+    a.B = B;
+    a.ldB = ldB;
+    a.s0 = s0;
+    a.S = S;
+    a.ldS = ldS;
+    a.D = D;
+    a.len = len;
+    a.TC = level_tc(level);
+    if (len <= a.TC) {
+        a.nT = 1;
+        a.prev = nullptr;
+        hipLaunchKernelGGL(sfull_apply_kernel<T>, dim3(1), dim3(256), 0, st, a);
+        return;
+    }
+    a.nT = (int)((len + a.TC - 1) / a.TC);
+    T* Magg = reinterpret_cast<T*>(scratch);
+    T* ragg = Magg + (size_t)a.nT * SF * SF;
+    T* dend = ragg + (size_t)a.nT * SF;
+    char* rest = reinterpret_cast<char*>(dend + (size_t)a.nT * SF);
+    a.Magg = Magg;
+    a.ragg = ragg;
+    hipLaunchKernelGGL(sfull_reduce_kernel<T>, dim3((unsigned)a.nT), dim3(256), 0, st, a);
+    // state after every tile: the dense scan of the tile maps
+    sfull_rec<T>(level + 1, Magg, SF, SF * SF, ragg, SF, s0, dend, SF, D, a.nT, rest, st);
+    a.prev = dend;
+    hipLaunchKernelGGL(sfull_apply_kernel<T>, dim3((unsigned)a.nT), dim3(256), 0, st, a);
+}
+
+int scan_full(bool f64, const void* J, const void* B, const void* s0, void* S, int64_t D, int64_t T, void* scratch,
+              hipStream_t st, std::string* msg) {
+    if (T == 0) return PMCMC_OK;
+    if (f64)
+        sfull_rec<double>(0, (const double*)J, D, D * D, (const double*)B, D, (const double*)s0, (double*)S, D, D, T,
+                          (char*)scratch, st);
+    else
+        sfull_rec<float>(0, (const float*)J, D, D * D, (const float*)B, D, (const float*)s0, (float*)S, D, D, T,
+                         (char*)scratch, st);
+    if (hipGetLastError() != hipSuccess) {
+        *msg = "dense scan: kernel launch failed";
+        return PMCMC_ERR_HIP;
+    }
+    return PMCMC_OK;
+}
+
+}  // namespace pmcmc
