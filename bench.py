@@ -138,6 +138,12 @@ def bench_scan(args):
     s0 = torch.zeros(D_, dtype=dt, device=dev)
     S = _device.colmajor_empty(D_, T_, dt, dev)
     ctx = _device.context(0)
+    shape = None
+    if not args.no_tune:
+        from ParallelMCMC import Tune
+
+        shape = Tune.tune_scan(D_, T_, dt, reps=3)
+        log(f"scan: tuned tile shape {shape}")
     for _ in range(args.warmup):
         DEERScan.solve_affine_scan_diag_(S, A, B, s0)
     ctx.reset_kernel_times()
@@ -155,7 +161,7 @@ def bench_scan(args):
         "metric": "diagonal affine scan throughput", "value": bytes_ / kern_s / 1e9, "unit": "GB/s",
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
         "dtype": args.scan_dtype, "config": {"workload": f"scan D={D_} T={T_}",
-                                              "kernel_us": kern_s * 1e6,
+                                              "kernel_us": kern_s * 1e6, "tile_shape": shape,
                                               "host_us_per_call": 1e6 * el / args.steps},
         "roofline": {"bound": "hbm", "achieved": bytes_ / kern_s / 1e9, "peak": 8000.0, "unit": "GB/s",
                      "frac": bytes_ / kern_s / 8e12, "traffic": None},
@@ -474,6 +480,7 @@ def main():
     ap.add_argument("--scan-T", type=int, default=1 << 20)
     ap.add_argument("--scan-dtype", default="f32", choices=["f32", "f64"])
     ap.add_argument("--scanfull-T", type=int, default=16384)
+    ap.add_argument("--no-tune", action="store_true", help="skip the tile-shape autotuner (use the fixed rule)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -499,7 +506,7 @@ def main():
     import numpy as np
     import torch
 
-    from ParallelMCMC import DEER, DensityModel, LogisticTarget, _device, _lib
+    from ParallelMCMC import DEER, DensityModel, LogisticTarget, Tune, _device, _lib
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(0)
@@ -527,6 +534,13 @@ def main():
             return DEER.solve(rec, xs, seed=1000 + b, **kw)
 
         xs = torch.from_numpy(x0).to(dev)
+        shape = None
+        if not args.no_tune:
+            # autotuner: level-0 tile shape of this workload (outside the timed region)
+            Xi_d, U_d, _, _ = tapes[0]
+            shape = Tune.tune_solve(DEER.TapedRecursion(model, EPS, Xi_d, U_d, precision=precode), xs, iters=4,
+                                    workspace=ws, seed=1000)
+            log(f"{prec}: tuned tile shape {shape}")
         first = None
         for b in range(args.warmup):
             S, info = run_block(b, xs)
@@ -554,7 +568,7 @@ def main():
         if first is None:
             first = (keep[0].cpu().numpy().astype(np.float64), keep[1].cpu().numpy().astype(bool), keep[2])
         return {"prec": prec, "elapsed": elapsed, "sps": args.steps * T / elapsed, "iters": iters, "kt": kt,
-                "first": first}
+                "first": first, "shape": shape}
 
     res = {v: run_variant(v) for v in variants}
     head = res["bf16_jac"] if "bf16_jac" in res else res[variants[0]]
@@ -588,6 +602,7 @@ def main():
             "precision": head["prec"],
             "variants": {v: {"samples_per_s": r["sps"], "ms_per_block": 1e3 * r["elapsed"] / args.steps,
                              "newton_iters_per_block": r["iters"],
+                             "tile_shape": r["shape"],
                              "kernel_ms": {k: (x[1] / x[0] if x[0] else None) for k, x in r["kt"].items()}}
                          for v, r in res.items()},
         },

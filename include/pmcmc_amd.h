@@ -232,6 +232,42 @@ int pmcmc_deer_adjoint(pmcmc_ctx* ctx, pmcmc_deer_ws* ws, const pmcmc_target* tg
                        const void* S, const uint8_t* accepted, const void* G, double* grad_out,
                        void* dx0_out, void* lambda_out, pmcmc_deer_info* info);
 
+/* ---- autotuner -------------------------------------------------------------
+ * No reference counterpart.  Level-0 tile shapes of the diagonal scan and of
+ * the fused DEER tile kernels (tc columns per tile, `parts` parts per row in
+ * the in-tile scan, and for the standalone scan the single-pass look-back
+ * form) are looked up in a process-wide table keyed by (kind, precision, D,
+ * ceil(log2 T)); without an entry (or with one that does not fit) a fixed
+ * rule applies.  pmcmc_scan_tune times candidate shapes of the standalone
+ * diagonal scan on synthetic operands; pmcmc_deer_tune times the
+ * shape-dependent kernels (tile reduce, spine, tile apply) of `iters`
+ * Newton iterations of the caller's problem on `ws` (tolerances ignored), and
+ * re-plans ws with the winner.  Both record the winner in the table (used by
+ * later scans, workspaces and batches) and return it; `us` is its measured
+ * time per scan / per iteration.  get/set/count/entry let a host save and
+ * restore the table (set rejects a shape that cannot run: UNSUPPORTED);
+ * get returns tc = 0 when there is no entry.                               */
+#define PMCMC_TUNE_SCAN_DIAG 0
+#define PMCMC_TUNE_DEER 1
+typedef struct pmcmc_tile_shape {
+    int32_t tc;
+    int32_t parts;
+    int32_t single_pass;
+    int32_t pad;
+    double us;
+} pmcmc_tile_shape;
+int pmcmc_scan_tune(pmcmc_ctx* ctx, int prec, int64_t D, int64_t T, int32_t reps,
+                    pmcmc_tile_shape* best);
+int pmcmc_deer_tune(pmcmc_ctx* ctx, pmcmc_deer_ws* ws, const pmcmc_target* tgt,
+                    const pmcmc_deer_params* params, const void* x0, const void* Xi, const void* U,
+                    int32_t iters, pmcmc_tile_shape* best);
+int pmcmc_tune_get(int kind, int prec, int64_t D, int64_t T, pmcmc_tile_shape* out);
+int pmcmc_tune_set(int kind, int prec, int64_t D, int64_t T, const pmcmc_tile_shape* shape);
+int pmcmc_tune_clear(void);
+int64_t pmcmc_tune_count(void);
+int pmcmc_tune_entry(int64_t i, int* kind, int* prec, int64_t* D, int32_t* log2T,
+                     pmcmc_tile_shape* out);
+
 /* ---- many chains in one solve ---------------------------------------------
  * Chains of uneven D and T (logistic or dense-Gaussian targets, D <= 256),
  * each solved window by window (`window` steps; window k starts from the

@@ -20,7 +20,7 @@ from .interface import (
 )
 from .targets import DiagGaussianTarget, GaussianTarget, LogisticTarget, StdNormalTarget
 from . import DEER, DEERScan, MALA  # noqa: E402
-from . import Adjoint, Batched  # noqa: E402
+from . import Adjoint, Batched, Tune  # noqa: E402
 from .owned import pmcmc_dot, pmcmc_dotsum, pmcmc_matmul  # noqa: E402
 
 __all__ = [
@@ -36,6 +36,7 @@ __all__ = [
     "MALA",
     "Adjoint",
     "Batched",
+    "Tune",
     "pmcmc_matmul",
     "pmcmc_dot",
     "pmcmc_dotsum",

@@ -71,6 +71,15 @@ class DeerInfo(C.Structure):
 
 _P = C.c_void_p
 _I64 = C.c_int64
+class TileShape(C.Structure):
+    _fields_ = [("tc", C.c_int32), ("parts", C.c_int32), ("single_pass", C.c_int32), ("pad", C.c_int32),
+                ("us", C.c_double)]
+
+
+TUNE_SCAN_DIAG = 0
+TUNE_DEER = 1
+
+
 class ChainDesc(C.Structure):
     """pmcmc_chain_desc (include/pmcmc_amd.h)."""
 
@@ -103,6 +112,14 @@ _SIGS = {
     "pmcmc_scan_diag_f64": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
     "pmcmc_scan_full_f32": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
     "pmcmc_scan_full_f64": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
+    "pmcmc_scan_tune": ([_P, C.c_int, _I64, _I64, C.c_int32, C.POINTER(TileShape)], C.c_int),
+    "pmcmc_deer_tune": ([_P, _P, _P, C.POINTER(DeerParams), _P, _P, _P, C.c_int32, C.POINTER(TileShape)], C.c_int),
+    "pmcmc_tune_get": ([C.c_int, C.c_int, _I64, _I64, C.POINTER(TileShape)], C.c_int),
+    "pmcmc_tune_set": ([C.c_int, C.c_int, _I64, _I64, C.POINTER(TileShape)], C.c_int),
+    "pmcmc_tune_clear": ([], C.c_int),
+    "pmcmc_tune_count": ([], _I64),
+    "pmcmc_tune_entry": ([_I64, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(_I64), C.POINTER(C.c_int32),
+                          C.POINTER(TileShape)], C.c_int),
     "pmcmc_matmul": ([_P, C.c_int, C.c_int, C.c_int, _I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64], C.c_int),
     "pmcmc_dot": ([_P, C.c_int, _I64, _P, _P, _P], C.c_int),
     "pmcmc_dotsum": ([_P, C.c_int, _I64, _I64, _P, _I64, _P, _I64, _P], C.c_int),

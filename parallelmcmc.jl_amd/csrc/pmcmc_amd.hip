@@ -13,8 +13,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
+#include <tuple>
 
 #include "../../include/pmcmc_amd.h"
 #include "adjoint.hpp"
@@ -96,6 +99,7 @@ struct pmcmc_deer_ws {
     SyncState* sync = nullptr;
     ScanPlan plan;
     void* blob = nullptr;  // scan levels
+    size_t blob_bytes = 0;
     // GEMM-target buffers (allocated on first use)
     GlmWork glmw;
     FullWork fullw;  // jacobian = :full
@@ -274,8 +278,73 @@ static int grid_for(int64_t n, int block = 256, int cap = 8192) {
 
 // Tile shape of one level: TC columns so that TC*D is about the per-type
 // element budget, shrunk until the LDS footprint fits 63 KB.
+// ---------------------------------------------------------------------------
+// Tile-shape table of the autotuner.  Level-0 tile shapes (TC columns, P
+// parts, and for the standalone diagonal scan the single-pass look-back
+// form) keyed by (kind, element size, D, ceil(log2 T)).  Filled by
+// pmcmc_scan_tune / pmcmc_deer_tune (measured) or pmcmc_tune_set (a host's
+// saved table); consulted by every plan, which falls back to tile_shape's
+// rule when there is no entry or the entry does not fit.  Process-wide,
+// guarded by a mutex.
+// ---------------------------------------------------------------------------
+struct TileChoice {
+    int TC = 0, P = 0, chained = 0;
+    double us = 0.0;
+};
+
+static int tune_bin(int64_t T) {
+    int b = 0;
+    while (b < 62 && ((int64_t)1 << b) < T) ++b;
+    return b;
+}
+
+struct TuneTable {
+    std::mutex mu;
+    std::map<std::tuple<int, int, int64_t, int>, TileChoice> m;
+};
+static TuneTable& tune_table() {
+    static TuneTable t;
+    return t;
+}
+static bool tune_get(int kind, int esz, int64_t D, int64_t T, TileChoice* out) {
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    auto it = t.m.find(std::make_tuple(kind, esz, D, tune_bin(T)));
+    if (it == t.m.end()) return false;
+    *out = it->second;
+    return true;
+}
+static void tune_put(int kind, int esz, int64_t D, int64_t T, const TileChoice& v) {
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    t.m[std::make_tuple(kind, esz, D, tune_bin(T))] = v;
+}
+
+// a level-0 shape the tile kernels can run: P parts fit the LDS part arrays
+// (P * D <= max(D, PMCMC_NT)), the tile fits the LDS budget
 template <typename T>
-static bool plan_scan(ScanPlan* p, int64_t D, int64_t Tn, bool deer_top) {
+static bool shape_fits(int64_t D, bool with_iterate, bool chained, const TileChoice& c) {
+    if (c.TC < 1 || c.P < 1 || c.P > c.TC) return false;
+    const int64_t pmax = D >= PMCMC_NT ? 1 : PMCMC_NT / D;
+    if (c.P > pmax) return false;
+    return tile_lds_bytes<T>(D, c.TC, with_iterate, chained) <= 64512;
+}
+
+// level-0 tile of one window (L columns) of a batched chain: the tuned DEER
+// entry for (D, L) when it fits, else the rule
+static void batch_tile_shape(int64_t D, int64_t L, int* TC, int* P, int* nT) {
+    TileChoice ch;
+    if (tune_get(PMCMC_TUNE_DEER, 4, D, L, &ch) && ch.TC > 0 && shape_fits<float>(D, true, false, ch)) {
+        *TC = (int)std::min<int64_t>(ch.TC, std::max<int64_t>(L, 1));
+        *P = std::min(ch.P, *TC);
+        *nT = (int)((L + *TC - 1) / *TC);
+        return;
+    }
+    tile_shape<float>(D, L, true, TC, P, nT);
+}
+
+template <typename T>
+static bool plan_scan(ScanPlan* p, int64_t D, int64_t Tn, bool deer_top, const TileChoice* ov = nullptr) {
     p->D = D;
     p->nlev = 0;
     p->bytes = 0;
@@ -283,7 +352,13 @@ static bool plan_scan(ScanPlan* p, int64_t D, int64_t Tn, bool deer_top) {
     for (int l = 0; l < 24; ++l) {
         ScanLevel& L = p->lev[l];
         L.len = len;
-        tile_shape<T>(D, len, deer_top && l == 0, &L.TC, &L.P, &L.nT);
+        if (l == 0 && ov && ov->TC > 0 && shape_fits<T>(D, deer_top, false, *ov)) {
+            L.TC = (int)std::min<int64_t>(ov->TC, std::max<int64_t>(len, 1));
+            L.P = std::min(ov->P, L.TC);
+            L.nT = (int)((len + L.TC - 1) / L.TC);
+        } else {
+            tile_shape<T>(D, len, deer_top && l == 0, &L.TC, &L.P, &L.nT);
+        }
         p->nlev = l + 1;
         if (L.nT <= 1) return tile_lds_bytes<T>(D, L.TC, deer_top && l == 0) <= 64512;
         const size_t m = (size_t)D * L.nT * sizeof(T);
@@ -440,7 +515,7 @@ extern "C" int pmcmc_buffer_download(pmcmc_ctx* c, void* host, const void* src, 
 // ---------------------------------------------------------------------------
 template <typename T>
 static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, int64_t D,
-                     int64_t Tn) {
+                     int64_t Tn, const TileChoice* force = nullptr) {
     if (!c) return PMCMC_ERR_ARGUMENT;
     if (D < 1 || Tn < 0) return fail(c, PMCMC_ERR_DIMENSION, "scan: need D >= 1 and T >= 0 (D=%lld T=%lld)", (long long)D, (long long)Tn);
     if (Tn == 0) return PMCMC_OK;
@@ -454,28 +529,42 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
     if (overlap(A)) return fail(c, PMCMC_ERR_ARGUMENT, "S must not alias A");
     if (overlap(B)) return fail(c, PMCMC_ERR_ARGUMENT, "S must not alias B");
     HIPCHK(c, hipSetDevice(c->device));
-    // PMCMC_SCAN_CHAINED=1: single pass with decoupled look-back (one read of
-    // A and B).  Not the default: on this GPU the look-back front (one window
-    // of predecessors per coherent round trip) is slower than the two-pass
+    // Shape: the caller's (the tuner's candidates), else the tuned table entry,
+    // else the rule.  The single-pass form (decoupled look-back, one read of A
+    // and B) is used when the table says so, or by default with
+    // PMCMC_SCAN_CHAINED=1: on this GPU its look-back front (one window of
+    // predecessors per coherent round trip) is slower than the two-pass
     // reduce/apply scan at D >= 64 and about even at D = 1.
-    const char* tp = getenv("PMCMC_SCAN_CHAINED");
-    ScanLevel L0;
-    tile_shape<T>(D, Tn, false, &L0.TC, &L0.P, &L0.nT);
-    {
-        // the single-pass scan is bound by the look-back front, so it takes
-        // the widest tile that fits the LDS budget
-        int tc = 1;
-        while (tc * 2 <= Tn && tile_lds_bytes<T>(D, tc * 2, false, true) <= 64512) tc *= 2;
-        while (tc + tc / 8 <= Tn && tile_lds_bytes<T>(D, tc + tc / 8, false, true) <= 64512 && tc >= 8) tc += tc / 8;
-        L0.TC = tc;
-        L0.nT = (int)((Tn + tc - 1) / tc);
-        L0.P = D >= PMCMC_NT ? 1 : (int)(PMCMC_NT / D);
-        if (L0.P > tc) L0.P = tc;
-        if (L0.P < 1) L0.P = 1;
+    TileChoice ch;
+    const bool have = force ? (ch = *force, true) : tune_get(PMCMC_TUNE_SCAN_DIAG, (int)sizeof(T), D, Tn, &ch);
+    bool chained;
+    if (have) {
+        chained = ch.chained != 0;
+    } else {
+        const char* tp = getenv("PMCMC_SCAN_CHAINED");
+        chained = tp && tp[0] == '1';
     }
-    const bool chained = L0.nT > 1 && tp && tp[0] == '1' && tile_lds_bytes<T>(D, L0.TC, false, true) <= 64512;
+    ScanLevel L0;
+    if (chained) {
+        if (have && ch.TC > 0 && shape_fits<T>(D, false, true, ch)) {
+            L0.TC = (int)std::min<int64_t>(ch.TC, Tn);
+            L0.P = std::min(ch.P, L0.TC);
+        } else {
+            // bound by the look-back front: the widest tile that fits the LDS budget
+            int tc = 1;
+            while (tc * 2 <= Tn && tile_lds_bytes<T>(D, tc * 2, false, true) <= 64512) tc *= 2;
+            while (tc + tc / 8 <= Tn && tile_lds_bytes<T>(D, tc + tc / 8, false, true) <= 64512 && tc >= 8)
+                tc += tc / 8;
+            L0.TC = tc;
+            L0.P = D >= PMCMC_NT ? 1 : (int)(PMCMC_NT / D);
+            if (L0.P > tc) L0.P = tc;
+            if (L0.P < 1) L0.P = 1;
+        }
+        L0.nT = (int)((Tn + L0.TC - 1) / L0.TC);
+        chained = L0.nT > 1 && tile_lds_bytes<T>(D, L0.TC, false, true) <= 64512;
+    }
     ScanPlan plan;
-    if (!chained && !plan_scan<T>(&plan, D, Tn, false))
+    if (!chained && !plan_scan<T>(&plan, D, Tn, false, have && !ch.chained ? &ch : nullptr))
         return fail(c, PMCMC_ERR_UNSUPPORTED, "scan: D=%lld too large for one tile", (long long)D);
     const size_t lbm = (size_t)D * L0.nT * sizeof(T);
     const size_t lb_bytes = 3 * lbm + ((size_t)L0.nT + 64) * sizeof(int);
@@ -845,8 +934,10 @@ extern "C" int pmcmc_deer_ws_create(pmcmc_ctx* c, int64_t D, int64_t Tn, int pre
     w->T = Tn;
     w->prec = prec;
     w->esz = prec == PMCMC_PREC_F64 ? 8 : 4;
-    const bool ok_plan = w->esz == 8 ? plan_scan<double>(&w->plan, D, Tn, true)
-                                     : plan_scan<float>(&w->plan, D, Tn, true);
+    TileChoice tuned;
+    const bool have = tune_get(PMCMC_TUNE_DEER, (int)w->esz, D, Tn, &tuned);
+    const bool ok_plan = w->esz == 8 ? plan_scan<double>(&w->plan, D, Tn, true, have ? &tuned : nullptr)
+                                     : plan_scan<float>(&w->plan, D, Tn, true, have ? &tuned : nullptr);
     if (!ok_plan) {
         delete w;
         return fail(c, PMCMC_ERR_UNSUPPORTED, "D=%lld too large for the fused tile kernel", (long long)D);
@@ -856,6 +947,7 @@ extern "C" int pmcmc_deer_ws_create(pmcmc_ctx* c, int64_t D, int64_t Tn, int pre
               hipMalloc(&w->sync, sizeof(SyncState)) == hipSuccess &&
               hipMemset(w->sync, 0, sizeof(SyncState)) == hipSuccess &&
               hipMalloc(&w->blob, w->plan.bytes > 0 ? w->plan.bytes : 16) == hipSuccess;
+    w->blob_bytes = w->plan.bytes > 0 ? w->plan.bytes : 16;
     if (!ok) {
         pmcmc_deer_ws_destroy(w);
         return fail(c, PMCMC_ERR_ALLOC, "workspace allocation failed (D=%lld T=%lld)", (long long)D, (long long)Tn);
@@ -1129,6 +1221,265 @@ extern "C" int pmcmc_deer_solve(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_targ
     if (w->prec == PMCMC_PREC_F64)
         return deer_solve_t<double>(c, w, tgt, p, x0, Xi, U, init, S_out, acc_out, info);
     return deer_solve_t<float>(c, w, tgt, p, x0, Xi, U, init, S_out, acc_out, info);
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: measure candidate level-0 tile shapes, keep the fastest
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void fill_value_kernel(T* p, T v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// the profiler's accumulated times are the caller's: tuning runs on a copy
+struct ProfSaved {
+    bool on;
+    double ms[KT_COUNT];
+    long long n[KT_COUNT];
+    explicit ProfSaved(Prof& p) : on(p.on) {
+        for (int k = 0; k < KT_COUNT; ++k) {
+            ms[k] = p.ms[k];
+            n[k] = p.n[k];
+        }
+    }
+    void restore(Prof& p) const {
+        p.on = on;
+        for (int k = 0; k < KT_COUNT; ++k) {
+            p.ms[k] = ms[k];
+            p.n[k] = n[k];
+        }
+    }
+};
+
+// Level-0 candidates around the rule's shape: TC x {1/4 .. 8}, P from the
+// most parts that fit down to 1; for the standalone scan also the single-pass
+// look-back form at its widest tile and two narrower ones.
+template <typename T>
+static std::vector<TileChoice> tile_candidates(int64_t D, int64_t Tn, bool with_iterate, bool chained_too) {
+    std::vector<TileChoice> out;
+    int tc0, p0, nt0;
+    tile_shape<T>(D, Tn, with_iterate, &tc0, &p0, &nt0);
+    const int pmax = D >= PMCMC_NT ? 1 : (int)(PMCMC_NT / D);
+    const int mults[6][2] = {{1, 4}, {1, 2}, {1, 1}, {2, 1}, {4, 1}, {8, 1}};
+    auto add = [&](TileChoice c) {
+        for (const auto& o : out)
+            if (o.TC == c.TC && o.P == c.P && o.chained == c.chained) return;
+        if (shape_fits<T>(D, with_iterate, c.chained != 0, c)) out.push_back(c);
+    };
+    for (const auto& m : mults) {
+        const int64_t tc = (int64_t)tc0 * m[0] / m[1];
+        if (tc < 1 || tc > Tn) continue;
+        for (int pp = pmax; pp >= 1; pp = pp > 1 ? pp / 2 : 0) {
+            TileChoice c;
+            c.TC = (int)tc;
+            c.P = std::min<int64_t>(pp, tc);
+            add(c);
+            if (pp == 1) break;
+        }
+    }
+    if (chained_too) {
+        int tc = 1;
+        while (tc * 2 <= Tn && tile_lds_bytes<T>(D, tc * 2, false, true) <= 64512) tc *= 2;
+        for (int div = 1; div <= 4; div *= 2) {
+            TileChoice c;
+            c.TC = std::max(1, tc / div);
+            c.P = std::min(pmax, c.TC);
+            c.chained = 1;
+            if ((Tn + c.TC - 1) / c.TC > 1) add(c);
+        }
+    }
+    return out;
+}
+
+template <typename T>
+static int scan_tune_t(pmcmc_ctx* c, int64_t D, int64_t Tn, int reps, TileChoice* best) {
+    const size_t n = (size_t)D * Tn;
+    T *A = nullptr, *B = nullptr, *s0 = nullptr, *S = nullptr;
+    bool ok = hipMalloc(&A, n * sizeof(T)) == hipSuccess && hipMalloc(&B, n * sizeof(T)) == hipSuccess &&
+              hipMalloc(&s0, (size_t)D * sizeof(T)) == hipSuccess && hipMalloc(&S, n * sizeof(T)) == hipSuccess;
+    auto release = [&]() {
+        hipFree(A);
+        hipFree(B);
+        hipFree(s0);
+        hipFree(S);
+    };
+    if (!ok) {
+        release();
+        return fail(c, PMCMC_ERR_ALLOC, "scan tune: cannot allocate the synthetic operands");
+    }
+    hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for((int64_t)n)), dim3(256), 0, c->stream, A, (T)0.999,
+                       (int64_t)n);
+    hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for((int64_t)n)), dim3(256), 0, c->stream, B, (T)1e-3,
+                       (int64_t)n);
+    hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for(D)), dim3(256), 0, c->stream, s0, (T)0, D);
+    const ProfSaved saved(c->prof);
+    best->TC = 0;
+    best->us = 1e300;
+    int rc = PMCMC_OK;
+    for (const TileChoice& cand : tile_candidates<T>(D, Tn, false, true)) {
+        c->prof.on = false;
+        rc = scan_diag<T>(c, A, B, s0, S, D, Tn, &cand);  // warm-up (and allocation)
+        if (rc != PMCMC_OK) break;
+        c->prof.reset();
+        c->prof.on = true;
+        for (int r = 0; r < reps && rc == PMCMC_OK; ++r) rc = scan_diag<T>(c, A, B, s0, S, D, Tn, &cand);
+        if (rc != PMCMC_OK) break;
+        const double us = c->prof.n[KT_ITERATION] ? 1e3 * c->prof.ms[KT_ITERATION] / c->prof.n[KT_ITERATION] : 1e300;
+        if (us < best->us) {
+            *best = cand;
+            best->us = us;
+        }
+    }
+    saved.restore(c->prof);
+    release();
+    if (rc != PMCMC_OK) return rc;
+    if (best->TC > 0) tune_put(PMCMC_TUNE_SCAN_DIAG, (int)sizeof(T), D, Tn, *best);
+    return PMCMC_OK;
+}
+
+static void shape_out(const TileChoice& c, pmcmc_tile_shape* out) {
+    if (!out) return;
+    out->tc = c.TC;
+    out->parts = c.P;
+    out->single_pass = c.chained;
+    out->pad = 0;
+    out->us = c.us;
+}
+
+extern "C" int pmcmc_scan_tune(pmcmc_ctx* c, int prec, int64_t D, int64_t T, int32_t reps, pmcmc_tile_shape* best) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    if (prec != PMCMC_PREC_F32 && prec != PMCMC_PREC_F64)
+        return fail(c, PMCMC_ERR_ARGUMENT, "scan tune: precision must be F32 or F64");
+    if (D < 1 || T < 1) return fail(c, PMCMC_ERR_DIMENSION, "scan tune: need D >= 1 and T >= 1");
+    if (reps < 1) return fail(c, PMCMC_ERR_ARGUMENT, "scan tune: reps must be >= 1");
+    HIPCHK(c, hipSetDevice(c->device));
+    TileChoice b;
+    const int rc = prec == PMCMC_PREC_F64 ? scan_tune_t<double>(c, D, T, reps, &b) : scan_tune_t<float>(c, D, T, reps, &b);
+    if (rc != PMCMC_OK) return rc;
+    shape_out(b, best);
+    return PMCMC_OK;
+}
+
+// re-plan a workspace's level-0 tile (the blob grows when the plan needs more)
+template <typename T>
+static int ws_replan(pmcmc_ctx* c, pmcmc_deer_ws* w, const TileChoice* ch) {
+    ScanPlan plan;
+    if (!plan_scan<T>(&plan, w->D, w->T, true, ch))
+        return fail(c, PMCMC_ERR_UNSUPPORTED, "tile shape does not fit the fused tile kernel");
+    if (plan.bytes > w->blob_bytes) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        void* nb = nullptr;
+        HIPCHK(c, hipMalloc(&nb, plan.bytes));
+        hipFree(w->blob);
+        w->blob = nb;
+        w->blob_bytes = plan.bytes;
+    }
+    w->plan = plan;
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_deer_tune(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt, const pmcmc_deer_params* p,
+                               const void* x0, const void* Xi, const void* U, int32_t iters,
+                               pmcmc_tile_shape* best) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    if (!w || !tgt || !p || !x0 || !Xi || !U) return fail(c, PMCMC_ERR_ARGUMENT, "deer tune: null argument");
+    if (iters < 1) return fail(c, PMCMC_ERR_ARGUMENT, "deer tune: iters must be >= 1");
+    if (p->jacobian == PMCMC_JAC_FULL || p->jacobian == PMCMC_JAC_FULL_SIGMOID)
+        return fail(c, PMCMC_ERR_UNSUPPORTED, "deer tune: the dense-Jacobian solve has no diagonal tile to tune");
+    HIPCHK(c, hipSetDevice(c->device));
+    const bool f64 = w->prec == PMCMC_PREC_F64;
+    void* S = nullptr;
+    HIPCHK(c, hipMalloc(&S, (size_t)w->D * w->T * w->esz));
+    pmcmc_deer_params q = *p;
+    q.tol_abs = 0.0;  // never converges: every candidate runs `iters` iterations
+    q.tol_rel = 0.0;
+    q.maxiter = iters;
+    const ProfSaved saved(c->prof);
+    TileChoice b;
+    b.us = 1e300;
+    const ScanPlan orig = w->plan;
+    const std::vector<TileChoice> cands =
+        f64 ? tile_candidates<double>(w->D, w->T, true, false) : tile_candidates<float>(w->D, w->T, true, false);
+    int rc = PMCMC_OK;
+    for (const TileChoice& cand : cands) {
+        rc = f64 ? ws_replan<double>(c, w, &cand) : ws_replan<float>(c, w, &cand);
+        if (rc != PMCMC_OK) break;
+        c->prof.on = true;
+        c->prof.reset();
+        rc = pmcmc_deer_solve(c, w, tgt, &q, x0, Xi, U, nullptr, S, nullptr, nullptr);
+        if (rc != PMCMC_OK) break;
+        // the shape-dependent kernels: level-0 reduce and apply, the spine
+        const long long nit = c->prof.n[KT_TILE_APPLY];
+        const double ms = c->prof.ms[KT_TILE_REDUCE] + c->prof.ms[KT_TILE_APPLY] + c->prof.ms[KT_SCAN_SPINE];
+        const double us = nit ? 1e3 * ms / nit : 1e300;
+        if (us < b.us) {
+            b = cand;
+            b.us = us;
+        }
+    }
+    saved.restore(c->prof);
+    hipFree(S);
+    if (rc != PMCMC_OK) {
+        w->plan = orig;
+        return rc;
+    }
+    rc = f64 ? ws_replan<double>(c, w, &b) : ws_replan<float>(c, w, &b);
+    if (rc != PMCMC_OK) return rc;
+    tune_put(PMCMC_TUNE_DEER, (int)w->esz, w->D, w->T, b);
+    shape_out(b, best);
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_tune_get(int kind, int prec, int64_t D, int64_t T, pmcmc_tile_shape* out) {
+    if (!out || (kind != PMCMC_TUNE_SCAN_DIAG && kind != PMCMC_TUNE_DEER)) return PMCMC_ERR_ARGUMENT;
+    TileChoice ch;
+    if (!tune_get(kind, prec == PMCMC_PREC_F64 ? 8 : 4, D, T, &ch)) ch = TileChoice{};
+    shape_out(ch, out);
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_tune_set(int kind, int prec, int64_t D, int64_t T, const pmcmc_tile_shape* shape) {
+    if (!shape || (kind != PMCMC_TUNE_SCAN_DIAG && kind != PMCMC_TUNE_DEER) || D < 1 || T < 1)
+        return PMCMC_ERR_ARGUMENT;
+    TileChoice ch;
+    ch.TC = shape->tc;
+    ch.P = shape->parts;
+    ch.chained = kind == PMCMC_TUNE_SCAN_DIAG ? (shape->single_pass != 0) : 0;
+    ch.us = shape->us;
+    const bool with_iterate = kind == PMCMC_TUNE_DEER;
+    const bool fits = prec == PMCMC_PREC_F64 ? shape_fits<double>(D, with_iterate, ch.chained != 0, ch)
+                                             : shape_fits<float>(D, with_iterate, ch.chained != 0, ch);
+    if (!fits) return PMCMC_ERR_UNSUPPORTED;
+    tune_put(kind, prec == PMCMC_PREC_F64 ? 8 : 4, D, T, ch);
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_tune_clear(void) {
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    t.m.clear();
+    return PMCMC_OK;
+}
+
+extern "C" int64_t pmcmc_tune_count(void) {
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    return (int64_t)t.m.size();
+}
+
+extern "C" int pmcmc_tune_entry(int64_t i, int* kind, int* prec, int64_t* D, int32_t* log2T, pmcmc_tile_shape* out) {
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    if (i < 0 || i >= (int64_t)t.m.size()) return PMCMC_ERR_ARGUMENT;
+    auto it = t.m.begin();
+    std::advance(it, i);
+    if (kind) *kind = std::get<0>(it->first);
+    if (prec) *prec = std::get<1>(it->first) == 8 ? PMCMC_PREC_F64 : PMCMC_PREC_F32;
+    if (D) *D = std::get<2>(it->first);
+    if (log2T) *log2T = std::get<3>(it->first);
+    shape_out(it->second, out);
+    return PMCMC_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1511,7 +1862,7 @@ extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_des
         }
         const int64_t L = window < d.T ? window : d.T;
         int TC, P, nT;
-        tile_shape<float>(h.D, L, true, &TC, &P, &nT);
+        batch_tile_shape(h.D, L, &TC, &P, &nT);
         if (tile_lds_bytes<float>(h.D, TC, true) > 64512) {
             delete b;
             return fail(c, PMCMC_ERR_UNSUPPORTED, "batch: chain %d: D too large for the tile kernel", i);
@@ -1538,7 +1889,7 @@ extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_des
         BatchChain& h = b->ch[i];
         const int64_t L = window < h.T ? window : h.T;
         int TC, P, nT;
-        tile_shape<float>(h.D, L, true, &TC, &P, &nT);
+        batch_tile_shape(h.D, L, &TC, &P, &nT);
         char* q = (char*)b->arena + off[i];
         const size_t mat = align256((size_t)h.D * L * sizeof(float));
         auto take = [&](size_t s) {
