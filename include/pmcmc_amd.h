@@ -238,7 +238,10 @@ int pmcmc_deer_adjoint(pmcmc_ctx* ctx, pmcmc_deer_ws* ws, const pmcmc_target* tg
  * the in-tile scan, and for the standalone scan the single-pass look-back
  * form) are looked up in a process-wide table keyed by (kind, precision, D,
  * ceil(log2 T)); without an entry (or with one that does not fit) a fixed
- * rule applies.  pmcmc_scan_tune times candidate shapes of the standalone
+ * rule applies.  single_pass: 0 two-pass reduce/apply, 1 LDS-tiled
+ * decoupled look-back, 2 register-resident tiles with look-back (D = 64;
+ * tc = 64, 128 or 256 columns, 256 FP32 only; parts unused).
+ * pmcmc_scan_tune times candidate shapes of the standalone
  * diagonal scan on synthetic operands; pmcmc_deer_tune times the
  * shape-dependent kernels (tile reduce, spine, tile apply) of `iters`
  * Newton iterations of the caller's problem on `ws` (tolerances ignored), and

@@ -4,7 +4,8 @@ reference's scan has no tile shapes (src/DEER/DEERScan.jl:102-173).
 
 The level-0 tile of the diagonal scan and of the fused DEER tile kernels
 (``tc`` columns per tile, ``parts`` parts per row of the in-tile scan, and
-for the standalone scan the single-pass look-back form) is looked up in a
+for the standalone scan the single-pass forms: 1 LDS-tiled look-back, 2
+register-resident tiles with look-back for D = 64) is looked up in a
 process-wide table keyed by (kind, precision, D, ceil(log2 T)); without an
 entry a fixed rule applies.  ``tune_scan`` measures candidate shapes of the
 standalone diagonal scan on synthetic operands, ``tune_solve`` the
@@ -36,7 +37,7 @@ def _prec_code(dtype):
 
 
 def _shape_dict(s: _lib.TileShape):
-    return {"tc": int(s.tc), "parts": int(s.parts), "single_pass": bool(s.single_pass), "us": float(s.us)}
+    return {"tc": int(s.tc), "parts": int(s.parts), "single_pass": int(s.single_pass), "us": float(s.us)}
 
 
 def tune_scan(D: int, T: int, dtype=torch.float32, reps: int = 5, device=None) -> dict:
@@ -81,9 +82,10 @@ def lookup(kind: str, dtype, D: int, T: int):
     return _shape_dict(out) if out.tc > 0 else None
 
 
-def set_entry(kind: str, dtype, D: int, T: int, tc: int, parts: int, single_pass: bool = False, us: float = 0.0):
-    """Put a shape in the table (raises if the tile kernels cannot run it)."""
-    s = _lib.TileShape(int(tc), int(parts), 1 if single_pass else 0, 0, float(us))
+def set_entry(kind: str, dtype, D: int, T: int, tc: int, parts: int, single_pass: int = 0, us: float = 0.0):
+    """Put a shape in the table (raises if the tile kernels cannot run it).
+    ``single_pass``: 0 two-pass, 1 LDS-tiled look-back, 2 register tiles (D = 64)."""
+    s = _lib.TileShape(int(tc), int(parts), int(single_pass), 0, float(us))
     rc = _lib.load().pmcmc_tune_set(KINDS[kind], _prec_code(dtype), int(D), int(T), C.byref(s))
     if rc != 0:
         raise ArgumentError(f"tile shape tc={tc} parts={parts} does not fit D={D} ({kind})")
@@ -120,4 +122,4 @@ def load(path: str):
     with open(path) as f:
         for e in json.load(f):
             set_entry(e["kind"], e["precision"], e["D"], 1 << int(e["log2T"]), e["tc"], e["parts"],
-                      e.get("single_pass", False), e.get("us", 0.0))
+                      int(e.get("single_pass", 0)), e.get("us", 0.0))

@@ -26,6 +26,7 @@
 #include "linalg.hpp"
 #include "refine.hpp"
 #include "scanfull.hpp"
+#include "scanreg.hpp"
 #include "gauss.hpp"
 #include "glm.hpp"
 #include "tile_kernel.hpp"
@@ -322,8 +323,16 @@ static void tune_put(int kind, int esz, int64_t D, int64_t T, const TileChoice& 
 
 // a level-0 shape the tile kernels can run: P parts fit the LDS part arrays
 // (P * D <= max(D, PMCMC_NT)), the tile fits the LDS budget
+// the register-tile single-pass form (chained == 2): D = 64, tiles of
+// 4 CPW columns with CPW in {16, 32, 64} (FP32) or {16, 32} (FP64)
+template <typename T>
+static bool reg_tile_fits(int64_t D, const TileChoice& c) {
+    return D == 64 && (c.TC == 64 || c.TC == 128 || (c.TC == 256 && sizeof(T) == 4));
+}
+
 template <typename T>
 static bool shape_fits(int64_t D, bool with_iterate, bool chained, const TileChoice& c) {
+    if (c.chained == 2) return !with_iterate && reg_tile_fits<T>(D, c);
     if (c.TC < 1 || c.P < 1 || c.P > c.TC) return false;
     const int64_t pmax = D >= PMCMC_NT ? 1 : PMCMC_NT / D;
     if (c.P > pmax) return false;
@@ -536,13 +545,64 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
     // predecessors per coherent round trip) is slower than the two-pass
     // reduce/apply scan at D >= 64 and about even at D = 1.
     TileChoice ch;
-    const bool have = force ? (ch = *force, true) : tune_get(PMCMC_TUNE_SCAN_DIAG, (int)sizeof(T), D, Tn, &ch);
+    bool have = force ? (ch = *force, true) : tune_get(PMCMC_TUNE_SCAN_DIAG, (int)sizeof(T), D, Tn, &ch);
     bool chained;
     if (have) {
         chained = ch.chained != 0;
     } else {
         const char* tp = getenv("PMCMC_SCAN_CHAINED");
         chained = tp && tp[0] == '1';
+    }
+    if (have && ch.chained == 2 && reg_tile_fits<T>(D, ch) && Tn > ch.TC) {
+        const int TC = ch.TC;
+        const int nT = (int)((Tn + TC - 1) / TC);
+        const size_t lbm = (size_t)D * nT * sizeof(T);
+        const size_t need = 3 * lbm + ((size_t)nT + 64) * sizeof(int);
+        if (need > c->scan_blob_cap) {
+            if (c->scan_blob) HIPCHK(c, hipFree(c->scan_blob));
+            c->scan_blob = nullptr;
+            c->scan_blob_cap = 0;
+            HIPCHK(c, hipMalloc(&c->scan_blob, need));
+            c->scan_blob_cap = need;
+        }
+        char* blob = (char*)c->scan_blob;
+        TileArgs<T> a{};
+        a.A = A;
+        a.B = B;
+        a.s0 = s0;
+        a.S = S;
+        a.D = D;
+        a.T_len = Tn;
+        a.TC = TC;
+        a.numTiles = nT;
+        a.lb_agg_a = (T*)blob;
+        a.lb_agg_b = (T*)(blob + lbm);
+        a.lb_incl = (T*)(blob + 2 * lbm);
+        int* ints = (int*)(blob + 3 * lbm);
+        a.lb_status = ints;
+        a.lb_ticket = (unsigned*)(ints + nT);
+        a.lb_err = (unsigned*)(ints + nT + 16);
+        const int e = c->prof.begin(KT_ITERATION, 0, c->stream);
+        hipLaunchKernelGGL(zero_ints_kernel, dim3(grid_for(nT + 64)), dim3(256), 0, c->stream, ints, (int64_t)nT + 64);
+        if (TC == 64)
+            hipLaunchKernelGGL((scan_reg64_kernel<T, 16>), dim3((unsigned)nT), dim3(256), 0, c->stream, a);
+        else if (TC == 128)
+            hipLaunchKernelGGL((scan_reg64_kernel<T, 32>), dim3((unsigned)nT), dim3(256), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((scan_reg64_kernel<T, sizeof(T) == 4 ? 64 : 32>), dim3((unsigned)nT), dim3(256), 0,
+                               c->stream, a);
+        c->prof.end(e, c->stream);
+        HIPCHK(c, hipGetLastError());
+        unsigned err = 0;
+        HIPCHK(c, hipMemcpyAsync(&err, a.lb_err, sizeof err, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->prof.flush(0);
+        if (err) return fail(c, PMCMC_ERR_DEVICE, "scan: look-back wait exceeded its bound");
+        return PMCMC_OK;
+    }
+    if (have && ch.chained == 2) {  // register form not applicable here: the rule
+        have = false;
+        chained = false;
     }
     ScanLevel L0;
     if (chained) {
@@ -1278,6 +1338,15 @@ static std::vector<TileChoice> tile_candidates(int64_t D, int64_t Tn, bool with_
             if (pp == 1) break;
         }
     }
+    if (chained_too && D == 64) {
+        for (int tc : {64, 128, 256}) {
+            TileChoice c;
+            c.TC = tc;
+            c.P = 1;
+            c.chained = 2;
+            if (Tn > tc) add(c);
+        }
+    }
     if (chained_too) {
         int tc = 1;
         while (tc * 2 <= Tn && tile_lds_bytes<T>(D, tc * 2, false, true) <= 64512) tc *= 2;
@@ -1445,7 +1514,7 @@ extern "C" int pmcmc_tune_set(int kind, int prec, int64_t D, int64_t T, const pm
     TileChoice ch;
     ch.TC = shape->tc;
     ch.P = shape->parts;
-    ch.chained = kind == PMCMC_TUNE_SCAN_DIAG ? (shape->single_pass != 0) : 0;
+    ch.chained = kind == PMCMC_TUNE_SCAN_DIAG ? std::min(std::max(shape->single_pass, 0), 2) : 0;
     ch.us = shape->us;
     const bool with_iterate = kind == PMCMC_TUNE_DEER;
     const bool fits = prec == PMCMC_PREC_F64 ? shape_fits<double>(D, with_iterate, ch.chained != 0, ch)

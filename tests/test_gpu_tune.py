@@ -41,7 +41,9 @@ def test_every_candidate_scan_shape_matches_sequential(D, T):
         for p in sorted({pmax, max(1, pmax // 2), 1}):
             if p <= tc:
                 shapes.add((tc, p, False))
-        shapes.add((tc, min(pmax, tc), True))
+        shapes.add((tc, min(pmax, tc), 1))
+    if D == 64:  # register-resident tiles (FP64: 64 or 128 columns)
+        shapes.update({(64, 1, 2), (128, 1, 2)})
     checked = 0
     for tc, p, sp in sorted(shapes):
         try:
@@ -117,3 +119,27 @@ def test_deer_tune_then_solve():
     assert info.converged
     assert O.trace_error(S.cpu().numpy().astype(np.float64), ref) < 1e-5
     assert np.array_equal(info.accepted.cpu().numpy().astype(bool), acc)
+
+
+@pytest.mark.parametrize("tc", [64, 128, 256])
+def test_register_tile_scan_long_carry_fp32(tc):
+    """Register-tile single-pass scan (D = 64, FP32) on a ragged T with A near 1,
+    so each tile's incoming state depends on many predecessors' maps."""
+    import torch
+
+    from ParallelMCMC import DEERScan, Tune
+
+    D, T = 64, 300017
+    rng = np.random.default_rng(tc)
+    A = rng.uniform(0.999, 1.0, (D, T)).astype(np.float32)
+    B = rng.standard_normal((D, T)).astype(np.float32)
+    s0 = rng.standard_normal(D).astype(np.float32)
+    ref = np.empty((D, T))
+    x = s0.astype(np.float64)
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    for t in range(T):
+        x = A64[:, t] * x + B64[:, t]
+        ref[:, t] = x
+    Tune.set_entry("scan_diag", torch.float32, D, T, tc=tc, parts=1, single_pass=2)
+    S = DEERScan.solve_affine_scan_diag(A, B, s0).cpu().numpy().astype(np.float64)
+    assert np.max(np.abs(S - ref)) <= 1e-4 * np.max(np.abs(ref))

@@ -9,10 +9,10 @@ from ParallelMCMC import ArgumentError, Tune
 def test_table_round_trip(tmp_path):
     Tune.clear()
     assert Tune.lookup("scan_diag", "f32", 64, 1 << 20) is None
-    Tune.set_entry("scan_diag", "f32", 64, 1 << 20, tc=64, parts=2, single_pass=True, us=12.5)
+    Tune.set_entry("scan_diag", "f32", 64, 1 << 20, tc=64, parts=2, single_pass=1, us=12.5)
     Tune.set_entry("deer", "f64", 100, 4096, tc=16, parts=2)
     got = Tune.lookup("scan_diag", "f32", 64, (1 << 19) + 1)  # same bin: ceil(log2 T) = 20
-    assert got == {"tc": 64, "parts": 2, "single_pass": True, "us": 12.5}
+    assert got == {"tc": 64, "parts": 2, "single_pass": 1, "us": 12.5}
     assert Tune.lookup("scan_diag", "f64", 64, 1 << 20) is None  # other precision
     assert Tune.lookup("deer", "f64", 100, 3000)["tc"] == 16
     assert Tune.lookup("deer", "f64", 100, 5000) is None  # other bin
@@ -35,4 +35,10 @@ def test_shapes_that_cannot_run_are_rejected():
         Tune.set_entry("deer", "f32", 4, 4096, tc=2, parts=4)
     with pytest.raises(ArgumentError):  # beyond the LDS budget
         Tune.set_entry("deer", "f32", 100, 1 << 20, tc=4096, parts=1)
+    with pytest.raises(ArgumentError):  # register tiles: D = 64 only
+        Tune.set_entry("scan_diag", "f32", 32, 4096, tc=128, parts=1, single_pass=2)
+    with pytest.raises(ArgumentError):  # 256-column register tiles are FP32 only
+        Tune.set_entry("scan_diag", "f64", 64, 4096, tc=256, parts=1, single_pass=2)
+    Tune.set_entry("scan_diag", "f64", 64, 4096, tc=128, parts=1, single_pass=2)
+    Tune.clear()
     assert Tune.entries() == []
