@@ -1709,6 +1709,9 @@ struct BInit {
     float* S0;        // iterate buffer 0
     int64_t D, n;     // n = D * L
     SyncState* sy;
+    int* rcnt;         // refinement counters and gate flag (4), cleared
+    int8_t* prev_dec;  // L previous decisions, set to -1 (none)
+    int64_t L;
 };
 
 struct BCopy {
@@ -1732,7 +1735,10 @@ __global__ __launch_bounds__(256) void batch_init_kernel(const BInit* t) {
         s->last_metric = INFINITY;
         s->last_iter = -1;
     }
+    if (blockIdx.x == 0 && threadIdx.x < 4 && a.rcnt) a.rcnt[threadIdx.x] = 0;
     for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256) a.S0[i] = a.x0[i % a.D];
+    if (a.prev_dec)
+        for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.L; i += (int64_t)gridDim.x * 256) a.prev_dec[i] = -1;
 }
 
 // the converged iterate of iteration k sits in buffer (k + 1) & 1
@@ -1795,6 +1801,14 @@ struct BatchChain {
     float *agg_a = nullptr, *agg_b = nullptr, *prev = nullptr;
     int ns = 1;
     int64_t rows_per_split = 0;
+    // level-0 tile of every window, fixed at creation from the longest window
+    // (the aggregates are sized for it; shorter windows clamp TC to their length)
+    int TC = 0, P = 0;
+    // FP64 decisions near the threshold (as in the single-chain solve)
+    RefineTarget rtg{};
+    double band = 0;
+    RefineWork rw;              // buffers sized for the longest window
+    int8_t* prev_dec = nullptr;  // hysteresis: previous iteration's decisions
     // per-solve results
     int iters = 0;
     bool converged = true;
@@ -1809,6 +1823,7 @@ struct pmcmc_batch {
     int64_t W = 0;
     bool jbf = false;
     bool use_graph = true;
+    bool refine = true;  // FP64 decisions near the threshold (PMCMC_REFINE=0: off)
     void* arena = nullptr;
     SyncState* syncs = nullptr;  // n, device
     SyncState* hsync = nullptr;  // n, pinned host
@@ -1828,6 +1843,9 @@ struct RoundPlan {
     std::vector<GaussPassArgs> qpass[3];
     std::vector<BInit> inits;
     std::vector<BCopy> copies;
+    std::vector<RefineTarget> rtgs;      // per active slot
+    std::vector<RefineArgs> rargs[2];    // per parity
+    std::vector<RefineWork> rws;
     std::vector<int> active;  // chain index of each active slot
     // per active slot: shape of its work
     struct Shape {
@@ -1849,6 +1867,10 @@ struct Items {
     std::vector<int4> glm[17];
     std::vector<int2> gauss[17];
     size_t lds_multi[NLDS] = {0, 0, 0}, lds_single[NLDS] = {0, 0, 0};
+    // FP64 decision recompute: live slots (all, logistic, Gaussian)
+    std::vector<int> r_all, r_glm, r_gauss;
+    int64_t r_maxT = 0;
+    int r_split_glm = 0, r_split_gauss = 0;
 };
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -1932,12 +1954,30 @@ extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_des
         const int64_t L = window < d.T ? window : d.T;
         int TC, P, nT;
         batch_tile_shape(h.D, L, &TC, &P, &nT);
+        h.TC = TC;
+        h.P = P;
         if (tile_lds_bytes<float>(h.D, TC, true) > 64512) {
             delete b;
             return fail(c, PMCMC_ERR_UNSUPPORTED, "batch: chain %d: D too large for the tile kernel", i);
         }
         size_t mat = align256((size_t)h.D * L * sizeof(float));
         size_t sz = 7 * mat + 2 * align256((size_t)L * sizeof(double)) + 3 * align256((size_t)h.D * nT * sizeof(float));
+        h.rtg = RefineTarget{};
+        h.rtg.D = h.D;
+        if (d.target->kind == TGT_LOGISTIC) {
+            h.rtg.kind = 0;
+            h.rtg.N = d.target->glm.N;
+            h.rtg.X64 = d.target->glm.Xd64;
+            h.rtg.y64 = d.target->glm.y64;
+            h.rtg.inv_s2 = 1.0 / (d.target->prior_scale * d.target->prior_scale);
+            h.band = std::max(1e-4, 2.5e-8 * (double)d.target->glm.N);
+        } else {
+            h.rtg.kind = 1;
+            h.rtg.L64 = d.target->gauss.L64;
+            h.rtg.mu64 = d.target->gauss.mu64;
+            h.band = 1e-3;
+        }
+        sz += align256(refine_chain_bytes(h.rtg, L, h.band)) + align256((size_t)L);
         if (d.target->kind == TGT_LOGISTIC) {
             glm_split_plan(&d.target->glm, L, &h.ns, &h.rows_per_split);
             sz += align256((size_t)h.ns * 2 * L * h.Dp * sizeof(float) + (size_t)h.ns * L * sizeof(double));
@@ -1957,8 +1997,7 @@ extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_des
     for (int i = 0; i < n; ++i) {
         BatchChain& h = b->ch[i];
         const int64_t L = window < h.T ? window : h.T;
-        int TC, P, nT;
-        batch_tile_shape(h.D, L, &TC, &P, &nT);
+        const int nT = (int)((L + h.TC - 1) / h.TC);
         char* q = (char*)b->arena + off[i];
         const size_t mat = align256((size_t)h.D * L * sizeof(float));
         auto take = [&](size_t s) {
@@ -1979,7 +2018,13 @@ extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_des
         h.agg_a = (float*)take(agg);
         h.agg_b = (float*)take(agg);
         h.prev = (float*)take(agg);
+        refine_chain_carve(&h.rw, h.rtg, L, h.band, take(align256(refine_chain_bytes(h.rtg, L, h.band))));
+        h.prev_dec = (int8_t*)take(align256((size_t)L));
         if (h.tgt->kind == TGT_LOGISTIC) h.part = (float*)q;
+    }
+    {
+        const char* rv = getenv("PMCMC_REFINE");
+        b->refine = !(rv && !strcmp(rv, "0"));
     }
     ctx_retain(c);
     b->ctx_ref = true;
@@ -2012,8 +2057,9 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
         // a fresh probe per chain and window, fixed over the window's iterations
         const uint64_t seed = p.seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(i + 1)) ^ ((uint64_t)r << 40);
         const uint32_t key = probe_key(seed, 0xFFFFFFFFu, 0u);
-        int TC, P, nT;
-        tile_shape<float>(D, L, true, &TC, &P, &nT);
+        const int TC = (int)std::min<int64_t>(h.TC, L);
+        const int P = std::min(h.P, TC);
+        const int nT = (int)((L + TC - 1) / TC);
         for (int par = 0; par < 2; ++par) {
             TileArgs<float> a{};
             a.s0 = s0;
@@ -2049,7 +2095,39 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
             a.agg_a = h.agg_a;
             a.agg_b = h.agg_b;
             a.prev = h.prev;
+            a.prev_dec = h.prev_dec;
+            a.hyst = 1e-6;
+            if (b->refine) {
+                a.dec = h.rw.dec;
+                a.refine_on = h.rw.cnt + 2;
+            }
             rp->targs[par].push_back(a);
+            RefineArgs ra{};
+            ra.x0 = s0;
+            ra.S_in = h.buf[par];
+            ra.Xi = Xi;
+            ra.U = U;
+            ra.Y = h.Y;
+            ra.gx = h.gx;
+            ra.gy = h.gy;
+            ra.lpx = h.lpx;
+            ra.lpy = h.lpy;
+            ra.eps = h.eps;
+            ra.band = h.band;
+            ra.skip = skip;
+            ra.last_metric = &sy->last_metric;
+            ra.gate = 100.0;
+            ra.iter = 0;
+            ra.gate_iters = 16;
+            ra.iter_dev = &sy->last_iter;
+            ra.hyst = 1e-6;
+            rp->rargs[par].push_back(ra);
+        }
+        {
+            RefineWork w = h.rw;
+            w.T = L;
+            rp->rws.push_back(w);
+            rp->rtgs.push_back(h.rtg);
         }
         RoundPlan::Shape sh{};
         sh.kind = h.tgt->kind;
@@ -2138,6 +2216,9 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
         in.D = D;
         in.n = D * L;
         in.sy = sy;
+        in.rcnt = h.rw.cnt;
+        in.prev_dec = h.prev_dec;
+        in.L = L;
         rp->inits.push_back(in);
         BCopy cp{};
         cp.b0 = h.buf[0];
@@ -2162,6 +2243,16 @@ static void build_items(const RoundPlan& rp, const std::vector<char>& live, Item
         } else {
             it->single[lc].push_back(make_int2(k, 0));
             if (sh.lds > it->lds_single[lc]) it->lds_single[lc] = sh.lds;
+        }
+        const RefineWork& rw = rp.rws[k];
+        it->r_all.push_back(k);
+        it->r_maxT = std::max(it->r_maxT, rw.T);
+        if (rw.kind == 0) {
+            it->r_glm.push_back(k);
+            it->r_split_glm = std::max(it->r_split_glm, rw.nsplit);
+        } else {
+            it->r_gauss.push_back(k);
+            it->r_split_gauss = std::max(it->r_split_gauss, rw.nsplit);
         }
         if (sh.kind == TGT_LOGISTIC) {
             for (int cb = 0; cb < sh.ncb; ++cb)
@@ -2227,6 +2318,10 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
             o_gr[v] = tp.put(rp.gred[v].data(), rp.gred[v].size() * sizeof(GlmReduceArgs));
             o_qp[v] = tp.put(rp.qpass[v].data(), rp.qpass[v].size() * sizeof(GaussPassArgs));
         }
+        const size_t o_rt = tp.put(rp.rtgs.data(), rp.rtgs.size() * sizeof(RefineTarget));
+        const size_t o_rw = tp.put(rp.rws.data(), rp.rws.size() * sizeof(RefineWork));
+        size_t o_ra[2];
+        for (int v = 0; v < 2; ++v) o_ra[v] = tp.put(rp.rargs[v].data(), rp.rargs[v].size() * sizeof(RefineArgs));
         const size_t o_init = tp.put(rp.inits.data(), rp.inits.size() * sizeof(BInit));
         const size_t o_copy = tp.put(rp.copies.data(), rp.copies.size() * sizeof(BCopy));
         int rc = upload(c, s, tp, &b->tables, &b->tables_cap);
@@ -2244,6 +2339,7 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
         Items it;
         Pack ip;
         size_t o_red[NLDS] = {0, 0, 0}, o_single[NLDS] = {0, 0, 0}, o_multi = 0, o_glmred = 0, o_gi[17], o_qi[17];
+        size_t o_rall = 0, o_rglm = 0, o_rgauss = 0;
         hipGraphExec_t exec = nullptr;
         hipGraph_t graph = nullptr;
         auto drop_graph = [&]() {
@@ -2271,6 +2367,22 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
             }
             launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[2]), (const int2*)ia(o_glmred),
                                       (int)it.glm_red.size(), s);
+            if (b->refine) {
+                RefineBatch rb{};
+                rb.tgs = (const RefineTarget*)at(o_rt);
+                rb.ras = (const RefineArgs*)at(o_ra[par]);
+                rb.rws = (const RefineWork*)at(o_rw);
+                rb.all = (const int*)ia(o_rall);
+                rb.n_all = (int)it.r_all.size();
+                rb.glm = (const int*)ia(o_rglm);
+                rb.n_glm = (int)it.r_glm.size();
+                rb.gauss = (const int*)ia(o_rgauss);
+                rb.n_gauss = (int)it.r_gauss.size();
+                rb.max_T = it.r_maxT;
+                rb.max_split_glm = it.r_split_glm;
+                rb.max_split_gauss = it.r_split_gauss;
+                refine_decisions_batched(rb, par, s);
+            }
             const TileArgs<float>* ta = (const TileArgs<float>*)at(o_t[par]);
             for (int lc = 0; lc < NLDS; ++lc)
                 if (!it.single[lc].empty())
@@ -2302,6 +2414,9 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
                 o_single[lc] = ip.put(it.single[lc].data(), it.single[lc].size() * sizeof(int2));
             }
             o_multi = ip.put(it.multi.data(), it.multi.size() * sizeof(int));
+            o_rall = ip.put(it.r_all.data(), it.r_all.size() * sizeof(int));
+            o_rglm = ip.put(it.r_glm.data(), it.r_glm.size() * sizeof(int));
+            o_rgauss = ip.put(it.r_gauss.data(), it.r_gauss.size() * sizeof(int));
             o_glmred = ip.put(it.glm_red.data(), it.glm_red.size() * sizeof(int2));
             for (int d = 0; d < 17; ++d) {
                 o_gi[d] = ip.put(it.glm[d].data(), it.glm[d].size() * sizeof(int4));

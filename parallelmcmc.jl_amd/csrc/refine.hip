@@ -79,18 +79,21 @@ __device__ inline double state_at(const RefineArgs& a, int64_t t, int64_t d, int
 
 __device__ inline bool skipped(const RefineArgs& a) { return a.skip && *(volatile const int*)a.skip; }
 
-__global__ __launch_bounds__(RNT) void refine_flag_kernel(RefineArgs a, int8_t* dec, int* cnt, int* list, int lmax,
-                                                         int par, int64_t D, int64_t T) {
+__device__ inline int iter_of(const RefineArgs& a) { return a.iter_dev ? *a.iter_dev + 1 : a.iter; }
+
+// columns 4 blk .. 4 blk + 3 (one wave each)
+__device__ void refine_flag_body(const RefineArgs& a, int8_t* dec, int* cnt, int* list, int lmax, int par, int64_t D,
+                                 int64_t T, int64_t blk) {
     if (skipped(a)) return;
     // every block reaches the same answer: cnt[2] only turns on when the gate
     // condition (fixed during this launch) holds
-    const bool on = cnt[2] != 0 || *a.last_metric <= a.gate || a.iter >= a.gate_iters;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const bool on = cnt[2] != 0 || *a.last_metric <= a.gate || iter_of(a) >= a.gate_iters;
+    if (blk == 0 && threadIdx.x == 0) {
         cnt[par ^ 1] = 0;
         if (on) cnt[2] = 1;
     }
     const int lane = threadIdx.x & 63;
-    const int64_t t = (int64_t)blockIdx.x * (RNT / 64) + (threadIdx.x >> 6);
+    const int64_t t = blk * (RNT / 64) + (threadIdx.x >> 6);
     if (t >= T) return;
     if (!on) {
         if (lane == 0) dec[t] = -1;
@@ -130,17 +133,16 @@ __global__ __launch_bounds__(RNT) void refine_flag_kernel(RefineArgs a, int8_t* 
 // both products, and reduces its X^T r accumulators across lanes once at the
 // end.  Q >= ceil(D / 4).
 template <int Q>
-__global__ __launch_bounds__(RNT) void refine_pass_glm_kernel(RefineTarget tg, RefineArgs a, RefineWork w, int par,
-                                                             int phase) {
-    if (skipped(a)) return;
+__device__ void refine_pass_glm_body(const RefineTarget& tg, const RefineArgs& a, const RefineWork& w, int par,
+                                     int phase, int64_t split, int slot, int nslots) {
+    if (skipped(a) || split >= w.nsplit) return;
     __shared__ double v[RMAXD], zpart[RNT / 64][64], red[4];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t D = w.D, N = tg.N;
     const int n_ent = min(w.cnt[par], w.lmax);
-    const int64_t split = blockIdx.x;
     const int64_t r0 = split * w.rows;
     const int64_t r1 = std::min(N, r0 + w.rows);
-    for (int j = blockIdx.y; j < n_ent; j += gridDim.y) {
+    for (int j = slot; j < n_ent; j += nslots) {
         const int64_t t = w.list[j];
         for (int64_t d = tid; d < RMAXD; d += RNT)
             v[d] = d >= D ? 0.0 : (phase == 0 ? state_at(a, t, d, D) : w.y[(size_t)j * D + d]);
@@ -191,17 +193,16 @@ __global__ __launch_bounds__(RNT) void refine_pass_glm_kernel(RefineTarget tg, R
 // Gaussian partials over columns [split * rows, ...) of Lambda:
 // grad partial -Lambda[:, k0:k1] (v - mu)[k0:k1]; the log-density
 // 0.5 (v - mu).grad is linear in the partials
-__global__ __launch_bounds__(RNT) void refine_pass_gauss_kernel(RefineTarget tg, RefineArgs a, RefineWork w, int par,
-                                                               int phase) {
-    if (skipped(a)) return;
+__device__ void refine_pass_gauss_body(const RefineTarget& tg, const RefineArgs& a, const RefineWork& w, int par,
+                                       int phase, int64_t split, int slot, int nslots) {
+    if (skipped(a) || split >= w.nsplit) return;
     __shared__ double v[RMAXD], red[4];
     const int tid = threadIdx.x;
     const int64_t D = w.D;
     const int n_ent = min(w.cnt[par], w.lmax);
-    const int64_t split = blockIdx.x;
     const int64_t k0 = split * w.rows;
     const int64_t k1 = std::min(D, k0 + w.rows);
-    for (int j = blockIdx.y; j < n_ent; j += gridDim.y) {
+    for (int j = slot; j < n_ent; j += nslots) {
         const int64_t t = w.list[j];
         for (int64_t d = tid; d < D; d += RNT)
             v[d] = (phase == 0 ? state_at(a, t, d, D) : w.y[(size_t)j * D + d]) - (tg.mu64 ? tg.mu64[d] : 0.0);
@@ -222,8 +223,8 @@ __global__ __launch_bounds__(RNT) void refine_pass_gauss_kernel(RefineTarget tg,
 // fixed-order sum of the partials of listed column j (wave w takes the splits
 // w, w + 4, ..., eight loads in flight; the four waves combine in order);
 // phase 0 forms the FP64 proposal, phase 1 the FP64 log alpha and the decision
-__global__ __launch_bounds__(RNT) void refine_finish_kernel(RefineTarget tg, RefineArgs a, RefineWork w, int par,
-                                                           int phase) {
+__device__ void refine_finish_body(const RefineTarget& tg, const RefineArgs& a, const RefineWork& w, int par,
+                                   int phase, int slot, int nslots) {
     if (skipped(a)) return;
     __shared__ double gs[RMAXD + 1], wpart[RNT / 64][64], red[4];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -232,7 +233,7 @@ __global__ __launch_bounds__(RNT) void refine_finish_kernel(RefineTarget tg, Ref
     const int n_ent = min(w.cnt[par], w.lmax);
     const double eps = a.eps;
     const double sq2e = sqrt(2.0 * eps);
-    for (int j = blockIdx.x; j < n_ent; j += gridDim.x) {
+    for (int j = slot; j < n_ent; j += nslots) {
         const int64_t t = w.list[j];
         const double* pj = w.part + (size_t)j * ns * (D + 1);
         for (int64_t dc = 0; dc <= D; dc += 64) {  // index D is the log-density term
@@ -294,6 +295,51 @@ __global__ __launch_bounds__(RNT) void refine_finish_kernel(RefineTarget tg, Ref
             }
         }
     }
+}
+
+
+__global__ __launch_bounds__(RNT) void refine_flag_kernel(RefineArgs a, RefineWork w, int par) {
+    refine_flag_body(a, w.dec, w.cnt, w.list, w.lmax, par, w.D, w.T, blockIdx.x);
+}
+template <int Q>
+__global__ __launch_bounds__(RNT) void refine_pass_glm_kernel(RefineTarget tg, RefineArgs a, RefineWork w, int par,
+                                                             int phase) {
+    refine_pass_glm_body<Q>(tg, a, w, par, phase, blockIdx.x, blockIdx.y, gridDim.y);
+}
+__global__ __launch_bounds__(RNT) void refine_pass_gauss_kernel(RefineTarget tg, RefineArgs a, RefineWork w, int par,
+                                                               int phase) {
+    refine_pass_gauss_body(tg, a, w, par, phase, blockIdx.x, blockIdx.y, gridDim.y);
+}
+__global__ __launch_bounds__(RNT) void refine_finish_kernel(RefineTarget tg, RefineArgs a, RefineWork w, int par,
+                                                           int phase) {
+    refine_finish_body(tg, a, w, par, phase, blockIdx.x, gridDim.x);
+}
+
+// batched: blockIdx.z picks the chain from a live-chain list
+__global__ __launch_bounds__(RNT) void refine_flag_batched(const RefineArgs* ras, const RefineWork* rws,
+                                                          const int* ids, int par) {
+    const int c = ids[blockIdx.z];
+    const RefineWork& w = rws[c];
+    if ((int64_t)blockIdx.x * (RNT / 64) >= w.T) return;
+    refine_flag_body(ras[c], w.dec, w.cnt, w.list, w.lmax, par, w.D, w.T, blockIdx.x);
+}
+__global__ __launch_bounds__(RNT) void refine_pass_glm_batched(const RefineTarget* tgs, const RefineArgs* ras,
+                                                              const RefineWork* rws, const int* ids, int par,
+                                                              int phase) {
+    const int c = ids[blockIdx.z];
+    refine_pass_glm_body<RMAXD / 4>(tgs[c], ras[c], rws[c], par, phase, blockIdx.x, blockIdx.y, gridDim.y);
+}
+__global__ __launch_bounds__(RNT) void refine_pass_gauss_batched(const RefineTarget* tgs, const RefineArgs* ras,
+                                                                const RefineWork* rws, const int* ids, int par,
+                                                                int phase) {
+    const int c = ids[blockIdx.z];
+    refine_pass_gauss_body(tgs[c], ras[c], rws[c], par, phase, blockIdx.x, blockIdx.y, gridDim.y);
+}
+__global__ __launch_bounds__(RNT) void refine_finish_batched(const RefineTarget* tgs, const RefineArgs* ras,
+                                                            const RefineWork* rws, const int* ids, int par,
+                                                            int phase) {
+    const int c = ids[blockIdx.z];
+    refine_finish_body(tgs[c], ras[c], rws[c], par, phase, blockIdx.x, gridDim.x);
 }
 
 __global__ void refine_zero_kernel(int* cnt) {
@@ -368,7 +414,7 @@ int refine_decisions(RefineWork* w, const RefineTarget& tg, const RefineArgs& a,
     const int par = iter & 1;
     const int64_t cols_per_block = RNT / 64;
     hipLaunchKernelGGL(refine_flag_kernel, dim3((unsigned)((w->T + cols_per_block - 1) / cols_per_block)), dim3(RNT),
-                       0, s, a, w->dec, w->cnt, w->list, w->lmax, par, w->D, w->T);
+                       0, s, a, *w, par);
     // 16 slots of listed columns (2 band T ~ 8 expected at T = 4096, band 1e-3);
     // more are taken in turn by the same workgroups
     const unsigned gy = (unsigned)std::min(w->lmax, 16);
@@ -391,6 +437,73 @@ int refine_decisions(RefineWork* w, const RefineTarget& tg, const RefineArgs& a,
         return PMCMC_ERR_HIP;
     }
     return PMCMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched solves: one chain's buffers carved from the batch arena
+// ---------------------------------------------------------------------------
+static void chain_plan(const RefineTarget& tg, int64_t L, double band, int64_t* lm, int64_t* rows, int64_t* nsplit) {
+    if (tg.kind == 0) {
+        *rows = 64 * std::max<int64_t>(1, (tg.N + 64 * 64 - 1) / (64 * 64));
+        *nsplit = (tg.N + *rows - 1) / *rows;
+    } else {
+        *rows = 32;
+        *nsplit = (tg.D + *rows - 1) / *rows;
+    }
+    *lm = std::min(L, std::max<int64_t>((int64_t)std::ceil(16.0 * band * (double)L), 64));
+}
+
+static size_t align256r(size_t x) { return (x + 255) / 256 * 256; }
+
+size_t refine_chain_bytes(const RefineTarget& tg, int64_t L, double band) {
+    int64_t lm, rows, nsplit;
+    chain_plan(tg, L, band, &lm, &rows, &nsplit);
+    return align256r((size_t)L) + align256r(4 * sizeof(int)) + align256r((size_t)lm * sizeof(int)) +
+           align256r((size_t)lm * nsplit * (tg.D + 1) * sizeof(double)) +
+           2 * align256r((size_t)lm * tg.D * sizeof(double)) + align256r((size_t)lm * sizeof(double));
+}
+
+void refine_chain_carve(RefineWork* w, const RefineTarget& tg, int64_t L, double band, char* base) {
+    int64_t lm, rows, nsplit;
+    chain_plan(tg, L, band, &lm, &rows, &nsplit);
+    char* q = base;
+    auto take = [&](size_t n) {
+        char* r = q;
+        q += align256r(n);
+        return r;
+    };
+    *w = RefineWork{};
+    w->D = tg.D;
+    w->T = L;
+    w->N = tg.kind == 0 ? tg.N : 0;
+    w->kind = tg.kind;
+    w->lmax = (int)lm;
+    w->nsplit = (int)nsplit;
+    w->rows = rows;
+    w->dec = (int8_t*)take((size_t)L);
+    w->cnt = (int*)take(4 * sizeof(int));
+    w->list = (int*)take((size_t)lm * sizeof(int));
+    w->part = (double*)take((size_t)lm * nsplit * (tg.D + 1) * sizeof(double));
+    w->g0 = (double*)take((size_t)lm * tg.D * sizeof(double));
+    w->y = (double*)take((size_t)lm * tg.D * sizeof(double));
+    w->lp0 = (double*)take((size_t)lm * sizeof(double));
+}
+
+void refine_decisions_batched(const RefineBatch& b, int par, hipStream_t s) {
+    if (b.n_all == 0) return;
+    const unsigned slots = 16;
+    hipLaunchKernelGGL(refine_flag_batched, dim3((unsigned)((b.max_T + 3) / 4), 1, (unsigned)b.n_all), dim3(RNT), 0,
+                       s, b.ras, b.rws, b.all, par);
+    for (int phase = 0; phase < 2; ++phase) {
+        if (b.n_glm > 0)
+            hipLaunchKernelGGL(refine_pass_glm_batched, dim3((unsigned)b.max_split_glm, slots, (unsigned)b.n_glm),
+                               dim3(RNT), 0, s, b.tgs, b.ras, b.rws, b.glm, par, phase);
+        if (b.n_gauss > 0)
+            hipLaunchKernelGGL(refine_pass_gauss_batched, dim3((unsigned)b.max_split_gauss, slots, (unsigned)b.n_gauss),
+                               dim3(RNT), 0, s, b.tgs, b.ras, b.rws, b.gauss, par, phase);
+        hipLaunchKernelGGL(refine_finish_batched, dim3(slots, 1, (unsigned)b.n_all), dim3(RNT), 0, s, b.tgs, b.ras,
+                           b.rws, b.all, par, phase);
+    }
 }
 
 }  // namespace pmcmc

@@ -45,6 +45,9 @@ struct RefineArgs {
     double gate;
     int iter;
     int gate_iters;
+    // batched solves keep the iteration on the device: iter = *iter_dev + 1
+    // (the SyncState's last finished iteration) when set
+    const int* iter_dev;
     // FP64 margins below this set the tie bit (bit 1) of dec[t]
     double hyst;
 };
@@ -84,5 +87,28 @@ int refine_reset(RefineWork* w, hipStream_t s, std::string* msg);
 // log-density at the state and at the proposal of every flagged column
 int refine_decisions(RefineWork* w, const RefineTarget& tg, const RefineArgs& a, int iter, hipStream_t s,
                      std::string* msg);
+
+// Batched form: per-chain descriptor tables (device) indexed by the live-chain
+// lists (device); `par` is the iteration parity of the launch.  glm / gauss
+// list the live chains of each target kind, all lists every live chain.
+struct RefineBatch {
+    const RefineTarget* tgs;
+    const RefineArgs* ras;  // for this parity
+    const RefineWork* rws;
+    const int* all;
+    int n_all;
+    const int* glm;
+    int n_glm;
+    const int* gauss;
+    int n_gauss;
+    int64_t max_T;       // longest live window
+    int max_split_glm;   // most row splits of a live logistic chain
+    int max_split_gauss;
+};
+void refine_decisions_batched(const RefineBatch& b, int par, hipStream_t s);
+// device bytes of one chain's refinement buffers (its lmax, nsplit, D, window)
+size_t refine_chain_bytes(const RefineTarget& tg, int64_t L, double band);
+// carve a chain's buffers from `base` (refine_chain_bytes of them)
+void refine_chain_carve(RefineWork* w, const RefineTarget& tg, int64_t L, double band, char* base);
 
 }  // namespace pmcmc

@@ -75,3 +75,69 @@ def test_batched_graph_and_direct_launches_identical():
     r2 = plan.solve()
     for (S1, it1), (S2, _, i2) in zip(r1, r2):
         assert it1 == i2.iters and np.array_equal(S1.cpu().numpy(), S2.cpu().numpy())
+
+
+def _near_threshold_chain(seed, N=2000, D=20, T=768, eps=1e-2):
+    """A logistic chain whose tape puts every twelfth column 4e-6 from its
+    accept threshold along the FP64 chain (inside the FP32 products' error)."""
+    from ParallelMCMC import DensityModel, LogisticTarget
+    from ParallelMCMC.Batched import Chain
+
+    rng = np.random.default_rng(seed)
+    Xd, y, _ = O.make_logreg_data(rng, N, D)
+    tgt = O.LogReg(Xd, y)
+    Xb, Ub = O.make_tape(rng, D, 1000)
+    Sb, _, _ = O.run_mala_sequential_taped(tgt, np.zeros(D), eps, Xb, Ub)
+    x0 = Sb[:, -1].astype(np.float32)
+    Xi = rng.standard_normal((D, T)).astype(np.float32)
+    U = rng.uniform(0.05, 0.95, T)
+    x = x0.astype(np.float64)
+    for t in range(T):
+        yv, _ = O.mala_proposal(tgt, x, eps, Xi[:, t].astype(np.float64))
+        la = O.mala_logalpha(tgt, x, yv, eps)
+        if t % 12 == 0 and la < -1e-4:
+            U[t] = np.exp(la - (4e-6 if t % 24 == 0 else -4e-6))
+        if np.log(U[t]) < la:
+            x = yv
+    return tgt, Chain(DensityModel(dim=D, target=LogisticTarget(Xd, y)), eps, x0, Xi, U.astype(np.float32))
+
+
+def test_batched_fp64_decisions_near_threshold(monkeypatch):
+    """The batched solve re-decides near-threshold columns in FP64 like the
+    single-chain solve: every returned decision equals the FP64 decision at
+    the returned trajectory (columns within 2e-6 of the threshold there
+    excepted, see the single-chain test).  FP32-only batches get some wrong."""
+    from ParallelMCMC.Batched import BatchPlan
+
+    pairs = [_near_threshold_chain(s) for s in (81, 82, 83)]
+
+    def run(mode):
+        if mode is None:
+            monkeypatch.delenv("PMCMC_REFINE", raising=False)
+        else:
+            monkeypatch.setenv("PMCMC_REFINE", mode)
+        plan = BatchPlan([c for _, c in pairs], window=256, precision="bf16_jac", tol_abs=1e-6, tol_rel=1e-6,
+                         maxiter=600)
+        bad = close = 0
+        for (tgt, ch), (S, acc, info) in zip(pairs, plan.solve()):
+            assert info.converged, f"PMCMC_REFINE={mode}: iters {info.iters}, metric {info.metric}"
+            S = S.cpu().numpy().astype(np.float64)
+            T = S.shape[1]
+            la = O.mala_step_batched_fwd_and_jvp(tgt, O._xbar(ch.x0.astype(np.float64), S), ch.eps,
+                                                 ch.Xi.astype(np.float64), np.full(T, 0.5),
+                                                 np.ones_like(S)).logalpha
+            lu = np.log(ch.U.astype(np.float64))
+            margin = np.abs(la - lu)
+            got = acc.cpu().numpy().astype(bool)
+            checked = margin > 2e-6
+            bad += int(np.sum(got[checked] != (lu < la)[checked]))
+            close += int(np.sum(checked & (margin < 2e-5)))
+        return bad, close
+
+    bad, close = run(None)
+    bad_raw, close_raw = run("0")
+    print(f"near-threshold columns {close} / {close_raw}; mismatches batched FP64 {bad}, FP32 only {bad_raw}",
+          flush=True)
+    assert close >= 60
+    assert bad == 0
+    assert bad_raw > 0
