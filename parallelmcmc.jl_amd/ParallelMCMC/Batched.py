@@ -33,6 +33,37 @@ class Chain:
     U: object
 
 
+def tune_chain_widths(chains, window, prec, seed=0):
+    """Autotune the level-0 tile of the single-chain DEER solves of these
+    chains (one window at a time): one representative per bucket of 16
+    consecutive widths (its widest chain, on its first window), its shape
+    recorded for every chain of the bucket (a shape that fits the widest chain
+    fits the narrower ones: the tile's LDS grows with D).  The batched solve
+    keeps the fixed rule (see batch_tile_shape in the library).  Returns
+    {bucket width: shape}."""
+    from . import DEER, Tune
+
+    buckets = {}
+    for ch in chains:
+        D = int(np.asarray(ch.x0).shape[0])
+        buckets.setdefault((D + 15) // 16 * 16, []).append(ch)
+    out = {}
+    for D16, group in sorted(buckets.items()):
+        rep = max(group, key=lambda c: (int(np.asarray(c.x0).shape[0]), np.asarray(c.U).shape[0]))
+        T = np.asarray(rep.U).shape[0]
+        L = min(window, T)
+        Xi = np.asarray(rep.Xi)[:, :L]
+        U = np.asarray(rep.U)[:L]
+        rec = DEER.TapedRecursion(rep.model, rep.eps, Xi, U, precision=prec)
+        shape = Tune.tune_solve(rec, np.asarray(rep.x0), iters=3, seed=seed)
+        for c in group:
+            Lc = min(window, np.asarray(c.U).shape[0])
+            Tune.set_entry("deer", "f32", int(np.asarray(c.x0).shape[0]), Lc, shape["tc"], shape["parts"],
+                           us=shape["us"])
+        out[D16] = shape
+    return out
+
+
 class BatchPlan:
     """Device buffers and descriptor tables for repeated solves of the same
     chains (the tapes are uploaded once).  ``solve()`` returns per-chain

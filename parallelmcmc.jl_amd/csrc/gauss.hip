@@ -92,11 +92,14 @@ static constexpr int GKC = 32;  // K slice
 // output column tile w for every 16-row tile of Lambda's rows.  JBF: the
 // direction tiles (KS = 1, waves 2 and 3) on BF16 MFMA.
 // ---------------------------------------------------------------------------
+// ldL: Lambda's padded width (its row stride, a multiple of 32), at most
+// 16 DPT; rows and columns of the DPT-wide tile beyond it are zero, so a
+// narrower target can run in a wider launch class.
 template <int DPT, int KS, bool JBF>
 __device__ __forceinline__ void gauss_pass_body(const ColSrc<float>& src, const float* L32, const float* mu,
                                                 int pass, float eps, const float* Xi, float* gout, float* hout,
                                                 double* lpout, float* Yout, float* Wout, const int* skip,
-                                                const int64_t cb) {
+                                                const int64_t cb, const int ldL) {
     constexpr int Dp = DPT * 16;
     constexpr int NC = 2 * GCB;
     constexpr int CPB = NC / (1 + KS);  // trajectory columns per block
@@ -121,7 +124,8 @@ __device__ __forceinline__ void gauss_pass_body(const ColSrc<float>& src, const 
         // Lambda[:, k0:k0+32] (rows d, contiguous in k in the row-major copy)
         for (int i = tid; i < Dp * (GKC / 4); i += 256) {
             const int d = i / (GKC / 4), q = i % (GKC / 4);
-            const float4 v = *reinterpret_cast<const float4*>(L32 + (size_t)d * Dp + k0 + 4 * q);
+            const float4 v = (d < ldL && k0 < ldL) ? *reinterpret_cast<const float4*>(L32 + (size_t)d * ldL + k0 + 4 * q)
+                                                    : make_float4(0.f, 0.f, 0.f, 0.f);
             float* dst = &Ls[d * SL + 4 * q];
             dst[0] = v.x;
             dst[1] = v.y;
@@ -207,17 +211,18 @@ __global__ __launch_bounds__(256) void gauss_pass_mfma_kernel(ColSrc<float> src,
                                                               const float* Xi, float* gout, float* hout,
                                                               double* lpout, float* Yout, float* Wout,
                                                               const int* skip) {
-    gauss_pass_body<DPT, KS, JBF>(src, L32, mu, pass, eps, Xi, gout, hout, lpout, Yout, Wout, skip, blockIdx.x);
+    gauss_pass_body<DPT, KS, JBF>(src, L32, mu, pass, eps, Xi, gout, hout, lpout, Yout, Wout, skip, blockIdx.x,
+                                  DPT * 16);
 }
 
-// Batched over chains of one padded width: work item blockIdx.x is
-// (chain, column block) = items[blockIdx.x].
+// Batched over chains of padded width up to 16 DPT (each chain's own width
+// in its args): work item blockIdx.x is (chain, column block) = items[blockIdx.x].
 template <int DPT, bool JBF>
 __global__ __launch_bounds__(256) void gauss_pass_mfma_batched(const GaussPassArgs* args, const int2* items) {
     const int2 w = items[blockIdx.x];
     const GaussPassArgs& a = args[w.x];
     gauss_pass_body<DPT, 1, JBF>(a.src, a.L32, a.mu, a.pass, a.eps, a.Xi, a.gout, a.hout, a.lpout, a.Yout, a.Wout,
-                                 a.skip, w.y);
+                                 a.skip, w.y, a.ldL);
 }
 
 // ---------------------------------------------------------------------------

@@ -36,9 +36,10 @@ struct GaussPassArgs {
     float* Yout;
     float* Wout;
     const int* skip;
+    int ldL;  // Lambda's padded width (row stride of L32), <= 16 * dpt of the launch
 };
 
-// Batched FP32 Lambda passes (one probe set) over chains of padded width 16 * dpt.
+// Batched FP32 Lambda passes (one probe set) over chains of padded width up to 16 * dpt.
 void launch_gauss_pass_batched(int dpt, bool jbf, const GaussPassArgs* args, const int2* items, int nitems,
                                hipStream_t s);
 

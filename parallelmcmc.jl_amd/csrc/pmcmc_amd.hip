@@ -339,16 +339,21 @@ static bool shape_fits(int64_t D, bool with_iterate, bool chained, const TileCho
     return tile_lds_bytes<T>(D, c.TC, with_iterate, chained) <= 64512;
 }
 
-// level-0 tile of one window (L columns) of a batched chain: the tuned DEER
-// entry for (D, L) when it fits, else the rule
+// a tuned DEER entry for (D, T), else the entry of D rounded up to a multiple
+// of 16 (one tuning run covers a bucket of nearby widths: the shape's LDS
+// only shrinks with D), when it fits this D
+static bool deer_tuned(int esz, int64_t D, int64_t T, TileChoice* ch) {
+    if (tune_get(PMCMC_TUNE_DEER, esz, D, T, ch)) return true;
+    const int64_t D16 = (D + 15) / 16 * 16;
+    return D16 != D && tune_get(PMCMC_TUNE_DEER, esz, D16, T, ch);
+}
+
+// level-0 tile of one window (L columns) of a batched chain.  The rule, not
+// the single-chain tuner's entries: the narrower tiles that win for one chain
+// lose in a batch (measured on 32 uneven chains: 31.7 s against 28.9 s), where
+// the chains already fill the GPU and more tiles lengthen each chain's
+// aggregate scan.
 static void batch_tile_shape(int64_t D, int64_t L, int* TC, int* P, int* nT) {
-    TileChoice ch;
-    if (tune_get(PMCMC_TUNE_DEER, 4, D, L, &ch) && ch.TC > 0 && shape_fits<float>(D, true, false, ch)) {
-        *TC = (int)std::min<int64_t>(ch.TC, std::max<int64_t>(L, 1));
-        *P = std::min(ch.P, *TC);
-        *nT = (int)((L + *TC - 1) / *TC);
-        return;
-    }
     tile_shape<float>(D, L, true, TC, P, nT);
 }
 
@@ -995,7 +1000,7 @@ extern "C" int pmcmc_deer_ws_create(pmcmc_ctx* c, int64_t D, int64_t Tn, int pre
     w->prec = prec;
     w->esz = prec == PMCMC_PREC_F64 ? 8 : 4;
     TileChoice tuned;
-    const bool have = tune_get(PMCMC_TUNE_DEER, (int)w->esz, D, Tn, &tuned);
+    const bool have = deer_tuned((int)w->esz, D, Tn, &tuned);
     const bool ok_plan = w->esz == 8 ? plan_scan<double>(&w->plan, D, Tn, true, have ? &tuned : nullptr)
                                      : plan_scan<float>(&w->plan, D, Tn, true, have ? &tuned : nullptr);
     if (!ok_plan) {
@@ -1762,15 +1767,17 @@ __global__ __launch_bounds__(256) void batch_agg_scan_kernel(const TileArgs<floa
     for (int64_t d = threadIdx.x; d < D; d += 256) {
         float s = 0.0f;
         int k = 0;
-        for (; k + 4 <= nT; k += 4) {
-            float aa[4], bb[4];
+        // the chain through the tiles is cheap; the loads are not: 16 tiles'
+        // aggregates in flight per round trip
+        for (; k + 16 <= nT; k += 16) {
+            float aa[16], bb[16];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < 16; ++q) {
                 aa[q] = a.agg_a[(size_t)(k + q) * D + d];
                 bb[q] = a.agg_b[(size_t)(k + q) * D + d];
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < 16; ++q) {
                 s = aa[q] * s + bb[q];
                 prev[(size_t)(k + q) * D + d] = s;
             }
@@ -2195,6 +2202,7 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
                 qa.src = v < 2 ? s0src : s1src;
                 if (v < 2) qa.src.V = h.buf[v];
                 qa.L32 = g.L32;
+                qa.ldL = (int)g.Dp;
                 qa.mu = g.mu32;
                 qa.pass = v < 2 ? 0 : 1;
                 qa.eps = (float)h.eps;
@@ -2231,7 +2239,19 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
     return !rp->active.empty();
 }
 
+// Launch class of a Gaussian chain's Lambda pass.  With few live Gaussian
+// chains an iteration is bound by its launches, not its flops, so the widths
+// are bucketed into three classes (64, 128, 256 rows; the narrower chains pad
+// with zero tiles); with many, every width keeps its own class.
+static int gauss_class(int dpt, int n_live_gauss) {
+    if (n_live_gauss > 32) return dpt;
+    return dpt <= 4 ? 4 : (dpt <= 8 ? 8 : 16);
+}
+
 static void build_items(const RoundPlan& rp, const std::vector<char>& live, Items* it) {
+    int n_live_gauss = 0;
+    for (int k = 0; k < (int)rp.shape.size(); ++k)
+        if (live[k] && rp.shape[k].kind != TGT_LOGISTIC) ++n_live_gauss;
     for (int k = 0; k < (int)rp.shape.size(); ++k) {
         if (!live[k]) continue;
         const RoundPlan::Shape& sh = rp.shape[k];
@@ -2259,7 +2279,8 @@ static void build_items(const RoundPlan& rp, const std::vector<char>& live, Item
                 for (int sp = 0; sp < sh.ns; ++sp) it->glm[sh.dpt].push_back(make_int4(k, cb, sp, 0));
             for (int64_t blk = 0; blk < (sh.L + 3) / 4; ++blk) it->glm_red.push_back(make_int2(k, (int)blk));
         } else {
-            for (int cb = 0; cb < sh.gcb; ++cb) it->gauss[sh.dpt].push_back(make_int2(k, cb));
+            const int cls = gauss_class(sh.dpt, n_live_gauss);
+            for (int cb = 0; cb < sh.gcb; ++cb) it->gauss[cls].push_back(make_int2(k, cb));
         }
     }
 }
