@@ -88,9 +88,11 @@ static constexpr int GKC = 32;  // K slice
 
 // ---------------------------------------------------------------------------
 // FP32 MFMA pass over 64 columns per block: KS = 1 probe set (32 states and
-// their 32 directions) or KS = 0 (64 states, no directions).  Wave w owns
-// output column tile w for every 16-row tile of Lambda's rows.  JBF: the
-// direction tiles (KS = 1, waves 2 and 3) on BF16 MFMA.
+// their 32 directions) or KS = 0 (64 states, no directions), four 16-column
+// output tiles.  Wave w owns the 16-row tiles w, w + 4, ... of Lambda for all
+// four column tiles, so every SIMD gets the same share of the FP32 MFMAs (and,
+// with JBF, of the BF16 MFMAs of the direction tiles 2 and 3); the
+// log-density of a column sums the four waves' row partials in a fixed order.
 // ---------------------------------------------------------------------------
 // ldL: Lambda's padded width (its row stride, a multiple of 32), at most
 // 16 DPT; rows and columns of the DPT-wide tile beyond it are zero, so a
@@ -105,9 +107,12 @@ __device__ __forceinline__ void gauss_pass_body(const ColSrc<float>& src, const 
     constexpr int CPB = NC / (1 + KS);  // trajectory columns per block
     constexpr int SL = GKC + 4;   // Lambda slice [d][k]
     constexpr int SVc = GKC + 4;  // column slice [j][k]
+    constexpr int NT = (DPT + 3) / 4;         // row tiles of a wave
+    constexpr bool BFD = JBF && KS == 1;      // direction tiles on BF16 MFMA
+    constexpr int NF = BFD ? 2 : 4;           // column tiles on FP32 MFMA
     __shared__ float Ls[Dp * SL];
     __shared__ float Vc[NC * SVc];
-    __shared__ double lpred[4][16];
+    __shared__ double lpred[4][NC];
 
     if (skip && *(volatile const int*)skip) return;
     const int tid = threadIdx.x;
@@ -115,9 +120,11 @@ __device__ __forceinline__ void gauss_pass_body(const ColSrc<float>& src, const 
     const int li = lane & 15, lk = lane >> 4;
     const int64_t D = src.D, Tn = src.Tn;
 
-    v4f acc[DPT];
+    v4f acc[NT][4];
 #pragma unroll
-    for (int dt = 0; dt < DPT; ++dt) acc[dt] = (v4f){0.f, 0.f, 0.f, 0.f};
+    for (int q = 0; q < NT; ++q)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[q][ct] = (v4f){0.f, 0.f, 0.f, 0.f};
 
     for (int k0 = 0; k0 < Dp; k0 += GKC) {
         __syncthreads();
@@ -145,51 +152,66 @@ __device__ __forceinline__ void gauss_pass_body(const ColSrc<float>& src, const 
             Vc[j * SVc + k] = v;
         }
         __syncthreads();
-        const int ct = wave;  // output column tile
-        if (!JBF || KS == 0 || ct < 2) {
 #pragma unroll
-            for (int ks = 0; ks < GKC / 4; ++ks) {
-                const float bv = Vc[(16 * ct + li) * SVc + 4 * ks + lk];
+        for (int ks = 0; ks < GKC / 4; ++ks) {
+            float bv[NF];
 #pragma unroll
-                for (int dt = 0; dt < DPT; ++dt) {
+            for (int ct = 0; ct < NF; ++ct) bv[ct] = Vc[(16 * ct + li) * SVc + 4 * ks + lk];
+#pragma unroll
+            for (int q = 0; q < NT; ++q) {
+                const int dt = wave + 4 * q;
+                if (dt < DPT) {
                     const float av = Ls[(16 * dt + li) * SL + 4 * ks + lk];
-                    acc[dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[dt], 0, 0, 0);
+#pragma unroll
+                    for (int ct = 0; ct < NF; ++ct)
+                        acc[q][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[ct], acc[q][ct], 0, 0, 0);
                 }
             }
-        } else {
-            v8bf bv;
+        }
+        if (BFD) {
+            v8bf b2, b3;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) bv[e] = (__bf16)Vc[(16 * ct + li) * SVc + 8 * lk + e];
+            for (int e = 0; e < 8; ++e) {
+                b2[e] = (__bf16)Vc[(32 + li) * SVc + 8 * lk + e];
+                b3[e] = (__bf16)Vc[(48 + li) * SVc + 8 * lk + e];
+            }
 #pragma unroll
-            for (int dt = 0; dt < DPT; ++dt) {
-                v8bf av;
+            for (int q = 0; q < NT; ++q) {
+                const int dt = wave + 4 * q;
+                if (dt < DPT) {
+                    v8bf av;
 #pragma unroll
-                for (int e = 0; e < 8; ++e) av[e] = (__bf16)Ls[(16 * dt + li) * SL + 8 * lk + e];
-                acc[dt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[dt], 0, 0, 0);
+                    for (int e = 0; e < 8; ++e) av[e] = (__bf16)Ls[(16 * dt + li) * SL + 8 * lk + e];
+                    acc[q][2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b2, acc[q][2], 0, 0, 0);
+                    acc[q][3] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b3, acc[q][3], 0, 0, 0);
+                }
             }
         }
     }
 
-    // epilogue: lane holds P[d = 16 dt + 4 lk + r][column 16 wave + li]
-    const int j = 16 * wave + li;
-    const int set = j / CPB;
-    const int64_t t = cb * CPB + (j % CPB);
-    const bool tv = t < Tn;
+    // epilogue: lane holds P[d = 16 dt + 4 lk + r][column 16 ct + li] for its row tiles dt
     const double sq2e = sqrt(2.0 * (double)eps);
-    double lp = 0.0;
-    if (tv) {
+    double lpc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int dt = 0; dt < DPT; ++dt)
+    for (int ct = 0; ct < 4; ++ct) {
+        const int j = 16 * ct + li;
+        const int set = j / CPB;
+        const int64_t t = cb * CPB + (j % CPB);
+        if (t >= Tn) continue;
+#pragma unroll
+        for (int q = 0; q < NT; ++q) {
+            const int dt = wave + 4 * q;
+            if (dt >= DPT) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t d = 16 * dt + 4 * lk + r;
                 if (d >= D) continue;
                 const size_t gi = (size_t)t * D + d;
-                const float g = -acc[dt][r];
+                const float g = -acc[q][ct][r];
                 if (set == 0) {
                     const double x = src.get(0, t, d);
                     const double xm = x - (mu ? (double)mu[d] : 0.0);
-                    lp += 0.5 * xm * (double)g;
+                    lpc[ct] += 0.5 * xm * (double)g;
                     gout[gi] = g;
                     if (pass == 0) Yout[gi] = (float)(x + (double)eps * g + sq2e * (double)Xi[gi]);
                 } else {
@@ -199,10 +221,20 @@ __device__ __forceinline__ void gauss_pass_body(const ColSrc<float>& src, const 
                         hout[gi] = g;
                 }
             }
+        }
     }
-    lp += __shfl_xor(lp, 16, 64);
-    lp += __shfl_xor(lp, 32, 64);
-    if (set == 0 && lk == 0 && tv) lpout[t] = lp;
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        lpc[ct] += __shfl_xor(lpc[ct], 16, 64);
+        lpc[ct] += __shfl_xor(lpc[ct], 32, 64);
+        if (lk == 0) lpred[wave][16 * ct + li] = lpc[ct];
+    }
+    __syncthreads();
+    if (tid < NC) {
+        const int j = tid;
+        const int64_t t = cb * CPB + (j % CPB);
+        if (j / CPB == 0 && t < Tn) lpout[t] = (lpred[0][j] + lpred[1][j]) + (lpred[2][j] + lpred[3][j]);
+    }
 }
 
 template <int DPT, int KS, bool JBF>

@@ -1070,6 +1070,12 @@ static void launch_deer_iteration(pmcmc_deer_ws* w, TileArgs<T> a, hipStream_t s
     }
     a.agg_a = (T*)(blob + L.off_a);
     a.agg_b = (T*)(blob + L.off_b);
+    if (MODE == MODE_GEMM) {
+        // the reduce pass keeps A and B in the probe-product buffers it has
+        // consumed (rebuilt by the target kernels every iteration)
+        a.a_keep = const_cast<T*>(a.W);
+        a.b_keep = const_cast<T*>(a.hw);
+    }
     int e = prof->begin(KT_TILE_REDUCE, a.iter, s);
     launch_tile<T, MODE, PASS_REDUCE>(a, L.nT, lds, s);
     prof->end(e, s);
@@ -2097,6 +2103,8 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
             a.gy = h.gy;
             a.W = h.W;
             a.hw = h.hw;
+            a.a_keep = h.W;  // used by multi-tile windows only (reduce keeps A, B)
+            a.b_keep = h.hw;
             a.lpx = h.lpx;
             a.lpy = h.lpy;
             a.agg_a = h.agg_a;

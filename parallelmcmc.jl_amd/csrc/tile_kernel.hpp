@@ -128,6 +128,13 @@ struct TileArgs {
     // with dec: nonzero once the FP64 override is active; an iteration that
     // meets the tolerance before then does not count as converged
     const int* refine_on;
+    // MODE_GEMM multi-tile solves (null otherwise): PASS_REDUCE stores the
+    // tile's A and B (D x T each) here and writes the per-column outputs
+    // (decisions, log alpha, prev_dec); PASS_APPLY then stages A and B instead
+    // of building them again.  They may alias W and hw: a tile reads and
+    // writes only its own columns, and writes them after it has read them.
+    T* a_keep;
+    T* b_keep;
 };
 
 // LDS layout (elements of T):
@@ -363,6 +370,9 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
     const int64_t gbase = c0 * D;
 
     // ---- phase 0: stage inputs (coalesced) -------------------------------
+    // A and B kept by the reduce pass of this iteration (MODE_GEMM multi-tile)
+    const bool keep = DEERM && (PASS == PASS_REDUCE || PASS == PASS_APPLY) && a.a_keep != nullptr;
+    const bool build = DEERM && !(PASS == PASS_APPLY && keep);
     if (!DEERM) {
         stage_lds<T>(al, a.A + gbase, nel, tid);
         stage_lds<T>(bl, a.B + gbase, nel, tid);
@@ -373,7 +383,12 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         } else {
             stage_lds<T>(xs, a.S_in + gbase - D, nel + D, tid);
         }
-        stage_lds<T>(bl, a.Xi + gbase, nel, tid);
+        if (build) {
+            stage_lds<T>(bl, a.Xi + gbase, nel, tid);
+        } else {
+            stage_lds<T>(al, a.a_keep + gbase, nel, tid);
+            stage_lds<T>(bl, a.b_keep + gbase, nel, tid);
+        }
     }
     // incoming state of the tile (needed only by the output passes)
     if (OUT && PASS != PASS_CHAINED) {
@@ -384,7 +399,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
     __syncthreads();
 
     // ---- phase 1: build A, B per column (MALA step + surrogate JVP) -------
-    if (DEERM) {
+    if (build) {
         int G = 1;
         while (G < D && G < 64) G <<= 1;
         const int lane = tid & (G - 1);
@@ -398,154 +413,262 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
 #pragma unroll
         for (int k = 0; k < PMCMC_MAXP; ++k) keys[k] = probe_key(a.seed, a.probe_iter, (uint32_t)k);
         const size_t DT = (size_t)D * (size_t)a.T_len;
+        // per-column outputs: by the pass that writes the tile, or by the
+        // reduce pass when it keeps A and B
+        const bool side = keep ? PASS == PASS_REDUCE : OUT;
 
-        for (int c = grp; c < tc; c += ngrp) {
-            const int64_t t = c0 + c;
-            const T* xcol = xs + (size_t)c * D;
-            T* bcol = bl + (size_t)c * D;
-            T* acol = al + (size_t)c * D;
-            double s_lpx = 0, s_lpy = 0, s_qyx = 0, s_qxy = 0;
-            double d1[PMCMC_MAXP], d2[PMCMC_MAXP], d3[PMCMC_MAXP];
-#pragma unroll
-            for (int k = 0; k < PMCMC_MAXP; ++k) d1[k] = d2[k] = d3[k] = 0;
-            for (int64_t d = lane; d < D; d += G) {
-                const double x = (double)xcol[d];
-                const size_t gi = (size_t)t * D + d;
-                double gx, y, gy;
-                if (MODE == MODE_ELEM) {
-                    const double p = a.prec ? (double)a.prec[d] : 1.0;
-                    const double m = a.mean ? (double)a.mean[d] : 0.0;
-                    gx = -p * (x - m);
-                    y = x + eps * gx + sq2e * (double)bcol[d];
-                    gy = -p * (y - m);
-                    s_lpx += -0.5 * p * (x - m) * (x - m);
-                    s_lpy += -0.5 * p * (y - m) * (y - m);
-#pragma unroll
-                    for (int k = 0; k < PMCMC_MAXP; ++k) {
-                        if (k >= K) break;
-                        const double z = (double)rademacher(keys[k], gi);
-                        const double w = z + eps * (-p * z);
-                        const double hy = -p * w;
-                        const double r = x - y - eps * gy;
-                        const double dr = z - w - eps * hy;
-                        d1[k] += gy * w;
-                        d2[k] += gx * z;
-                        d3[k] += r * dr;
-                    }
-                } else {
-                    gx = (double)a.gx[gi];
-                    y = (double)a.Y[gi];
-                    gy = (double)a.gy[gi];
-#pragma unroll
-                    for (int k = 0; k < PMCMC_MAXP; ++k) {
-                        if (k >= K) break;
-                        const double z = (double)rademacher(keys[k], gi);
-                        const double w = (double)a.W[k * DT + gi];
-                        const double hy = (double)a.hw[k * DT + gi];
-                        const double r = x - y - eps * gy;
-                        const double dr = z - w - eps * hy;
-                        d1[k] += gy * w;
-                        d2[k] += gx * z;
-                        d3[k] += r * dr;
-                    }
-                }
-                const double e1 = y - x - eps * gx;
-                const double e2 = x - y - eps * gy;
-                s_qyx += e1 * e1;
-                s_qxy += e2 * e2;
-            }
-            s_qyx = group_sum(s_qyx, G);
-            s_qxy = group_sum(s_qxy, G);
-            if (MODE == MODE_ELEM) {
-                s_lpx = group_sum(s_lpx, G);
-                s_lpy = group_sum(s_lpy, G);
-            } else {
-                s_lpx = a.lpx[t];
-                s_lpy = a.lpy[t];
-            }
-#pragma unroll
-            for (int k = 0; k < PMCMC_MAXP; ++k) {
-                if (k >= K) break;
-                d1[k] = group_sum(d1[k], G);
-                d2[k] = group_sum(d2[k], G);
-                d3[k] = group_sum(d3[k], G);
-            }
+        // Accept decision of column c from its (group-summed) sums; returns it
+        // and sets gg = g (1 - g) of the acceptance probability g and dg[k],
+        // its derivative along probe k.  u: the tape's uniform; o: the FP64 override (-1 none);
+        // pv: the previous decision (-1 none).
+        auto decide = [&](int c, int64_t t, double s_lpx, double s_lpy, double s_qyx, double s_qxy,
+                          const double* d1, const double* d2, const double* d3, double u, int8_t o, int8_t pv,
+                          double* dg, double& gg) -> bool {
             // log acceptance ratio (MALA.jl:589-595), in double
             const double lq_yx = -0.5 * s_qyx / (2.0 * eps) - c_norm;
             const double lq_xy = -0.5 * s_qxy / (2.0 * eps) - c_norm;
             const double la = (s_lpy + lq_xy) - (s_lpx + lq_yx);
-            const double lu = log((double)a.U[t]);
+            const double lu = log(u);
             const double g = 1.0 / (1.0 + exp(-(la - lu)));
             bool acc = lu < la;
             bool tie = fabs(la - lu) < a.hyst;
-            if (MODE == MODE_GEMM && a.dec) {
-                const int8_t o = a.dec[t];
-                if (o >= 0) {  // FP64 decision (bit 0) and its tie flag (bit 1)
-                    acc = (o & 1) != 0;
-                    tie = (o & 2) != 0;
-                }
+            if (MODE == MODE_GEMM && o >= 0) {  // FP64 decision (bit 0) and its tie flag (bit 1)
+                acc = (o & 1) != 0;
+                tie = (o & 2) != 0;
             }
             // decisions an FP32 iterate cannot resolve keep their previous value
-            // (all lanes read before lane 0 writes: one column per group)
-            if (a.prev_dec && tie) {
-                const int8_t pv = a.prev_dec[t];
-                if (pv >= 0) acc = pv != 0;
-            }
-            const double gg = g * (1.0 - g);
-            double dg[PMCMC_MAXP];
+            // (every lane has read pv before lane 0 writes: one column per group)
+            if (a.prev_dec && tie && pv >= 0) acc = pv != 0;
+            gg = g * (1.0 - g);
 #pragma unroll
             for (int k = 0; k < PMCMC_MAXP; ++k) dg[k] = gg * (d1[k] - d2[k] - d3[k] / (2.0 * eps));
-            if (OUT && lane == 0) {
+            if (side && lane == 0) {
                 accl[c] = acc ? 1 : 0;
                 if (a.prev_dec) a.prev_dec[t] = acc ? 1 : 0;
                 if (a.acc_out) a.acc_out[t] = acc ? 1 : 0;
                 if (a.la_out) a.la_out[t] = la;
             }
-            // second pass: A = mean_k z_k * Jt_k (or the exact diagonal), B = FT - A * x
-            for (int64_t d = lane; d < D; d += G) {
-                const double x = (double)xcol[d];
-                const size_t gi = (size_t)t * D + d;
-                double gx, y, gy, p = 1.0;
-                if (MODE == MODE_ELEM) {
-                    p = a.prec ? (double)a.prec[d] : 1.0;
-                    const double m = a.mean ? (double)a.mean[d] : 0.0;
-                    gx = -p * (x - m);
-                    y = x + eps * gx + sq2e * (double)bcol[d];
-                    gy = -p * (y - m);
-                } else {
-                    gx = (double)a.gx[gi];
-                    y = (double)a.Y[gi];
-                    gy = (double)a.gy[gi];
-                }
-                const double ft = acc ? y : x;
-                double Av = 0;
-                if (a.jacobian == JAC_DIAG && MODE == MODE_ELEM) {
-                    // exact diag of J = a(I + eps Hx) + (1-a)I + g(1-g)(y-x)c^T
-                    const double hx = -p, hy = -p;
-                    const double r = x - y - eps * gy;
-                    const double cc = (1.0 + eps * hx) * gy - gx -
-                                      (r - (1.0 + eps * hx) * (1.0 + eps * hy) * r) / (2.0 * eps);
-                    Av = (acc ? (1.0 + eps * hx) : 1.0) + gg * (y - x) * cc;
-                } else {
+            return acc;
+        };
+
+        if (MODE == MODE_GEMM && K == 1 && D <= 4 * G) {
+            // At most four rows per lane and one probe: a column's operands stay
+            // in registers for both sweeps, and the next column's loads are in
+            // flight while this one is reduced.
+            constexpr int R = 4;
+            T pgx[R], py[R], pgy[R], pw[R], phw[R];
+            double plpx = 0, plpy = 0, pu = 1;
+            int8_t po = -1, ppv = -1;
+            auto fetch = [&](int cc) {
+                if (cc >= tc) return;
+                const int64_t t = c0 + cc;
 #pragma unroll
-                    for (int k = 0; k < PMCMC_MAXP; ++k) {
-                        if (k >= K) break;
-                        const double z = (double)rademacher(keys[k], gi);
-                        double w;
-                        if (MODE == MODE_ELEM)
-                            w = z + eps * (-p * z);
-                        else
-                            w = (double)a.W[k * DT + gi];
-                        const double jt = (acc ? w : z) + (y - x) * dg[k];
-                        Av += z * jt;
+                for (int i = 0; i < R; ++i) {
+                    const int64_t d = lane + (int64_t)i * G;
+                    if (d < D) {
+                        const size_t gi = (size_t)t * D + d;
+                        pgx[i] = a.gx[gi];
+                        py[i] = a.Y[gi];
+                        pgy[i] = a.gy[gi];
+                        pw[i] = a.W[gi];
+                        phw[i] = a.hw[gi];
                     }
-                    if (K > 1) Av *= 1.0 / (double)K;
                 }
-                acol[d] = (T)Av;
-                bcol[d] = (T)(ft - (double)corr_base(xcol[D + d]));  // FT_t - S_t
+                plpx = a.lpx[t];
+                plpy = a.lpy[t];
+                pu = (double)a.U[t];
+                po = a.dec ? a.dec[t] : (int8_t)-1;
+                ppv = a.prev_dec ? a.prev_dec[t] : (int8_t)-1;
+            };
+            fetch(grp);
+            for (int c = grp; c < tc; c += ngrp) {
+                T vgx[R], vy[R], vgy[R], vw[R], vhw[R];
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    vgx[i] = pgx[i];
+                    vy[i] = py[i];
+                    vgy[i] = pgy[i];
+                    vw[i] = pw[i];
+                    vhw[i] = phw[i];
+                }
+                const double s_lpx = plpx, s_lpy = plpy, u = pu;
+                const int8_t o = po, pv = ppv;
+                fetch(c + ngrp);
+                const int64_t t = c0 + c;
+                const T* xcol = xs + (size_t)c * D;
+                T* bcol = bl + (size_t)c * D;
+                T* acol = al + (size_t)c * D;
+                double s_qyx = 0, s_qxy = 0;
+                double d1[PMCMC_MAXP] = {0, 0, 0, 0}, d2[PMCMC_MAXP] = {0, 0, 0, 0}, d3[PMCMC_MAXP] = {0, 0, 0, 0};
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    const int64_t d = lane + (int64_t)i * G;
+                    if (d < D) {
+                        const double x = (double)xcol[d];
+                        const size_t gi = (size_t)t * D + d;
+                        const double gx = (double)vgx[i], y = (double)vy[i], gy = (double)vgy[i];
+                        const double z = (double)rademacher(keys[0], gi);
+                        const double w = (double)vw[i], hy = (double)vhw[i];
+                        const double r = x - y - eps * gy;
+                        const double dr = z - w - eps * hy;
+                        d1[0] += gy * w;
+                        d2[0] += gx * z;
+                        d3[0] += r * dr;
+                        const double e1 = y - x - eps * gx;
+                        s_qyx += e1 * e1;
+                        s_qxy += r * r;
+                    }
+                }
+                s_qyx = group_sum(s_qyx, G);
+                s_qxy = group_sum(s_qxy, G);
+                d1[0] = group_sum(d1[0], G);
+                d2[0] = group_sum(d2[0], G);
+                d3[0] = group_sum(d3[0], G);
+                double dg[PMCMC_MAXP], gg;
+                const bool acc = decide(c, t, s_lpx, s_lpy, s_qyx, s_qxy, d1, d2, d3, u, o, pv, dg, gg);
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    const int64_t d = lane + (int64_t)i * G;
+                    if (d < D) {
+                        const double x = (double)xcol[d];
+                        const size_t gi = (size_t)t * D + d;
+                        const double y = (double)vy[i];
+                        const double z = (double)rademacher(keys[0], gi);
+                        const double jt = (acc ? (double)vw[i] : z) + (y - x) * dg[0];
+                        acol[d] = (T)(z * jt);
+                        bcol[d] = (T)((acc ? y : x) - (double)corr_base(xcol[D + d]));  // FT_t - S_t
+                    }
+                }
+            }
+        } else {
+            for (int c = grp; c < tc; c += ngrp) {
+                const int64_t t = c0 + c;
+                const T* xcol = xs + (size_t)c * D;
+                T* bcol = bl + (size_t)c * D;
+                T* acol = al + (size_t)c * D;
+                double s_lpx = 0, s_lpy = 0, s_qyx = 0, s_qxy = 0;
+                double d1[PMCMC_MAXP], d2[PMCMC_MAXP], d3[PMCMC_MAXP];
+#pragma unroll
+                for (int k = 0; k < PMCMC_MAXP; ++k) d1[k] = d2[k] = d3[k] = 0;
+                for (int64_t d = lane; d < D; d += G) {
+                    const double x = (double)xcol[d];
+                    const size_t gi = (size_t)t * D + d;
+                    double gx, y, gy;
+                    if (MODE == MODE_ELEM) {
+                        const double p = a.prec ? (double)a.prec[d] : 1.0;
+                        const double m = a.mean ? (double)a.mean[d] : 0.0;
+                        gx = -p * (x - m);
+                        y = x + eps * gx + sq2e * (double)bcol[d];
+                        gy = -p * (y - m);
+                        s_lpx += -0.5 * p * (x - m) * (x - m);
+                        s_lpy += -0.5 * p * (y - m) * (y - m);
+#pragma unroll
+                        for (int k = 0; k < PMCMC_MAXP; ++k) {
+                            if (k >= K) break;
+                            const double z = (double)rademacher(keys[k], gi);
+                            const double w = z + eps * (-p * z);
+                            const double hy = -p * w;
+                            const double r = x - y - eps * gy;
+                            const double dr = z - w - eps * hy;
+                            d1[k] += gy * w;
+                            d2[k] += gx * z;
+                            d3[k] += r * dr;
+                        }
+                    } else {
+                        gx = (double)a.gx[gi];
+                        y = (double)a.Y[gi];
+                        gy = (double)a.gy[gi];
+#pragma unroll
+                        for (int k = 0; k < PMCMC_MAXP; ++k) {
+                            if (k >= K) break;
+                            const double z = (double)rademacher(keys[k], gi);
+                            const double w = (double)a.W[k * DT + gi];
+                            const double hy = (double)a.hw[k * DT + gi];
+                            const double r = x - y - eps * gy;
+                            const double dr = z - w - eps * hy;
+                            d1[k] += gy * w;
+                            d2[k] += gx * z;
+                            d3[k] += r * dr;
+                        }
+                    }
+                    const double e1 = y - x - eps * gx;
+                    const double e2 = x - y - eps * gy;
+                    s_qyx += e1 * e1;
+                    s_qxy += e2 * e2;
+                }
+                s_qyx = group_sum(s_qyx, G);
+                s_qxy = group_sum(s_qxy, G);
+                if (MODE == MODE_ELEM) {
+                    s_lpx = group_sum(s_lpx, G);
+                    s_lpy = group_sum(s_lpy, G);
+                } else {
+                    s_lpx = a.lpx[t];
+                    s_lpy = a.lpy[t];
+                }
+#pragma unroll
+                for (int k = 0; k < PMCMC_MAXP; ++k) {
+                    if (k >= K) break;
+                    d1[k] = group_sum(d1[k], G);
+                    d2[k] = group_sum(d2[k], G);
+                    d3[k] = group_sum(d3[k], G);
+                }
+                const int8_t o = (MODE == MODE_GEMM && a.dec) ? a.dec[t] : (int8_t)-1;
+                const int8_t pv = a.prev_dec ? a.prev_dec[t] : (int8_t)-1;
+                double dg[PMCMC_MAXP], gg;
+                const bool acc =
+                    decide(c, t, s_lpx, s_lpy, s_qyx, s_qxy, d1, d2, d3, (double)a.U[t], o, pv, dg, gg);
+                // second pass: A = mean_k z_k * Jt_k (or the exact diagonal), B = FT - A * x
+                for (int64_t d = lane; d < D; d += G) {
+                    const double x = (double)xcol[d];
+                    const size_t gi = (size_t)t * D + d;
+                    double gx, y, gy, p = 1.0;
+                    if (MODE == MODE_ELEM) {
+                        p = a.prec ? (double)a.prec[d] : 1.0;
+                        const double m = a.mean ? (double)a.mean[d] : 0.0;
+                        gx = -p * (x - m);
+                        y = x + eps * gx + sq2e * (double)bcol[d];
+                        gy = -p * (y - m);
+                    } else {
+                        gx = (double)a.gx[gi];
+                        y = (double)a.Y[gi];
+                        gy = (double)a.gy[gi];
+                    }
+                    const double ft = acc ? y : x;
+                    double Av = 0;
+                    if (a.jacobian == JAC_DIAG && MODE == MODE_ELEM) {
+                        // exact diag of J = a(I + eps Hx) + (1-a)I + g(1-g)(y-x)c^T
+                        const double hx = -p, hy = -p;
+                        const double r = x - y - eps * gy;
+                        const double cc = (1.0 + eps * hx) * gy - gx -
+                                          (r - (1.0 + eps * hx) * (1.0 + eps * hy) * r) / (2.0 * eps);
+                        Av = (acc ? (1.0 + eps * hx) : 1.0) + gg * (y - x) * cc;
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < PMCMC_MAXP; ++k) {
+                            if (k >= K) break;
+                            const double z = (double)rademacher(keys[k], gi);
+                            double w;
+                            if (MODE == MODE_ELEM)
+                                w = z + eps * (-p * z);
+                            else
+                                w = (double)a.W[k * DT + gi];
+                            const double jt = (acc ? w : z) + (y - x) * dg[k];
+                            Av += z * jt;
+                        }
+                        if (K > 1) Av *= 1.0 / (double)K;
+                    }
+                    acol[d] = (T)Av;
+                    bcol[d] = (T)(ft - (double)corr_base(xcol[D + d]));  // FT_t - S_t
+                }
             }
         }
         __syncthreads();
+        if (PASS == PASS_REDUCE && keep) {
+            for (int64_t i = tid; i < nel; i += PMCMC_NT) {
+                a.a_keep[gbase + i] = al[i];
+                a.b_keep[gbase + i] = bl[i];
+            }
+        }
     }
 
     // ---- phase 2: in-tile scan (rows x P parts, then Hillis-Steele over P)
