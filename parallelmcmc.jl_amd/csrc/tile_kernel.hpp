@@ -400,15 +400,24 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
 
     // ---- phase 1: build A, B per column (MALA step + surrogate JVP) -------
     if (build) {
+        const int K = a.probes;
+        // GEMM targets with one probe and D <= 1024: a group of G lanes per
+        // column with up to 16 rows per lane (G = D / 16 rounded up to a power
+        // of two), so each group has one or two columns of the tile and the
+        // column's reductions take log2(G) <= 6 shuffle steps.  Otherwise one
+        // row per lane (G = D rounded up, at most 64).
+        const bool fast = MODE == MODE_GEMM && K == 1 && D <= 8 * 64;
         int G = 1;
-        while (G < D && G < 64) G <<= 1;
+        if (fast)
+            while (G * 8 < D) G <<= 1;
+        else
+            while (G < D && G < 64) G <<= 1;
         const int lane = tid & (G - 1);
         const int grp = tid / G;
         const int ngrp = PMCMC_NT / G;
         const double eps = a.eps;
         const double sq2e = sqrt(2.0 * eps);
         const double c_norm = ((double)D / 2.0) * log(4.0 * 3.14159265358979323846 * eps);
-        const int K = a.probes;
         uint32_t keys[PMCMC_MAXP];
 #pragma unroll
         for (int k = 0; k < PMCMC_MAXP; ++k) keys[k] = probe_key(a.seed, a.probe_iter, (uint32_t)k);
@@ -451,57 +460,35 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
             return acc;
         };
 
-        if (MODE == MODE_GEMM && K == 1 && D <= 4 * G) {
-            // At most four rows per lane and one probe: a column's operands stay
-            // in registers for both sweeps, and the next column's loads are in
-            // flight while this one is reduced.
-            constexpr int R = 4;
-            T pgx[R], py[R], pgy[R], pw[R], phw[R];
-            double plpx = 0, plpy = 0, pu = 1;
-            int8_t po = -1, ppv = -1;
-            auto fetch = [&](int cc) {
-                if (cc >= tc) return;
-                const int64_t t = c0 + cc;
+        if (fast) {
+            // the column's operands are loaded once, all in flight together,
+            // and kept in registers for both sweeps
+            constexpr int RM = 8;
+            for (int c = grp; c < tc; c += ngrp) {
+                const int64_t t = c0 + c;
+                T vgx[RM], vy[RM], vgy[RM], vw[RM], vhw[RM];
 #pragma unroll
-                for (int i = 0; i < R; ++i) {
+                for (int i = 0; i < RM; ++i) {
                     const int64_t d = lane + (int64_t)i * G;
                     if (d < D) {
                         const size_t gi = (size_t)t * D + d;
-                        pgx[i] = a.gx[gi];
-                        py[i] = a.Y[gi];
-                        pgy[i] = a.gy[gi];
-                        pw[i] = a.W[gi];
-                        phw[i] = a.hw[gi];
+                        vgx[i] = a.gx[gi];
+                        vy[i] = a.Y[gi];
+                        vgy[i] = a.gy[gi];
+                        vw[i] = a.W[gi];
+                        vhw[i] = a.hw[gi];
                     }
                 }
-                plpx = a.lpx[t];
-                plpy = a.lpy[t];
-                pu = (double)a.U[t];
-                po = a.dec ? a.dec[t] : (int8_t)-1;
-                ppv = a.prev_dec ? a.prev_dec[t] : (int8_t)-1;
-            };
-            fetch(grp);
-            for (int c = grp; c < tc; c += ngrp) {
-                T vgx[R], vy[R], vgy[R], vw[R], vhw[R];
-#pragma unroll
-                for (int i = 0; i < R; ++i) {
-                    vgx[i] = pgx[i];
-                    vy[i] = py[i];
-                    vgy[i] = pgy[i];
-                    vw[i] = pw[i];
-                    vhw[i] = phw[i];
-                }
-                const double s_lpx = plpx, s_lpy = plpy, u = pu;
-                const int8_t o = po, pv = ppv;
-                fetch(c + ngrp);
-                const int64_t t = c0 + c;
+                const double s_lpx = a.lpx[t], s_lpy = a.lpy[t], u = (double)a.U[t];
+                const int8_t o = a.dec ? a.dec[t] : (int8_t)-1;
+                const int8_t pv = a.prev_dec ? a.prev_dec[t] : (int8_t)-1;
                 const T* xcol = xs + (size_t)c * D;
                 T* bcol = bl + (size_t)c * D;
                 T* acol = al + (size_t)c * D;
                 double s_qyx = 0, s_qxy = 0;
                 double d1[PMCMC_MAXP] = {0, 0, 0, 0}, d2[PMCMC_MAXP] = {0, 0, 0, 0}, d3[PMCMC_MAXP] = {0, 0, 0, 0};
 #pragma unroll
-                for (int i = 0; i < R; ++i) {
+                for (int i = 0; i < RM; ++i) {
                     const int64_t d = lane + (int64_t)i * G;
                     if (d < D) {
                         const double x = (double)xcol[d];
@@ -526,17 +513,18 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                 d3[0] = group_sum(d3[0], G);
                 double dg[PMCMC_MAXP], gg;
                 const bool acc = decide(c, t, s_lpx, s_lpy, s_qyx, s_qxy, d1, d2, d3, u, o, pv, dg, gg);
+                // A and B in the iterate's precision (they are stored in it)
+                const T dgt = (T)dg[0];
 #pragma unroll
-                for (int i = 0; i < R; ++i) {
+                for (int i = 0; i < RM; ++i) {
                     const int64_t d = lane + (int64_t)i * G;
                     if (d < D) {
-                        const double x = (double)xcol[d];
-                        const size_t gi = (size_t)t * D + d;
-                        const double y = (double)vy[i];
-                        const double z = (double)rademacher(keys[0], gi);
-                        const double jt = (acc ? (double)vw[i] : z) + (y - x) * dg[0];
-                        acol[d] = (T)(z * jt);
-                        bcol[d] = (T)((acc ? y : x) - (double)corr_base(xcol[D + d]));  // FT_t - S_t
+                        const T x = xcol[d];
+                        const T y = vy[i];
+                        const T z = (T)rademacher(keys[0], (size_t)t * D + d);
+                        const T jt = (acc ? vw[i] : z) + (y - x) * dgt;
+                        acol[d] = z * jt;
+                        bcol[d] = (acc ? y : x) - corr_base(xcol[D + d]);  // FT_t - S_t
                     }
                 }
             }
@@ -797,14 +785,14 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
 }
 
 template <typename T, int MODE, int PASS>
-__global__ __launch_bounds__(PMCMC_NT) void tile_kernel(TileArgs<T> a) {
+__global__ __launch_bounds__(PMCMC_NT, sizeof(T) == 4 ? 3 : 2) void tile_kernel(TileArgs<T> a) {
     tile_body<T, MODE, PASS>(a, blockIdx.x);
 }
 
 // Batched launch over many problems: work item blockIdx.x is (problem, tile) =
 // items[blockIdx.x]; args[problem] describes the problem.
 template <typename T, int MODE, int PASS>
-__global__ __launch_bounds__(PMCMC_NT) void tile_kernel_batched(const TileArgs<T>* args, const int2* items) {
+__global__ __launch_bounds__(PMCMC_NT, sizeof(T) == 4 ? 3 : 2) void tile_kernel_batched(const TileArgs<T>* args, const int2* items) {
     const int2 w = items[blockIdx.x];
     tile_body<T, MODE, PASS>(args[w.x], w.y);
 }
