@@ -126,32 +126,69 @@ __device__ __forceinline__ void gauss_pass_body(const ColSrc<float>& src, const 
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) acc[q][ct] = (v4f){0.f, 0.f, 0.f, 0.f};
 
-    for (int k0 = 0; k0 < Dp; k0 += GKC) {
-        __syncthreads();
-        // Lambda[:, k0:k0+32] (rows d, contiguous in k in the row-major copy)
-        for (int i = tid; i < Dp * (GKC / 4); i += 256) {
+    // The k-slices are software-pipelined: slice s + 1 (Lambda rows as
+    // float4, NC x 32 column values and the one mu entry a thread needs) is
+    // loaded into registers while the MFMAs run on slice s in LDS.
+    constexpr int NLV = (Dp * (GKC / 4) + 255) / 256;  // float4 of Lambda per thread
+    constexpr int NCV = NC * GKC / 256;                 // column values per thread
+    float4 lv[NLV];
+    float cv[NCV];
+    float muv = 0.f;
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int m = 0; m < NLV; ++m) {
+            const int i = tid + 256 * m;
             const int d = i / (GKC / 4), q = i % (GKC / 4);
-            const float4 v = (d < ldL && k0 < ldL) ? *reinterpret_cast<const float4*>(L32 + (size_t)d * ldL + k0 + 4 * q)
-                                                    : make_float4(0.f, 0.f, 0.f, 0.f);
-            float* dst = &Ls[d * SL + 4 * q];
-            dst[0] = v.x;
-            dst[1] = v.y;
-            dst[2] = v.z;
-            dst[3] = v.w;
+            lv[m] = (i < Dp * (GKC / 4) && d < ldL && k0 < ldL)
+                        ? *reinterpret_cast<const float4*>(L32 + (size_t)d * ldL + k0 + 4 * q)
+                        : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        for (int i = tid; i < NC * GKC; i += 256) {
-            const int j = i / GKC, k = i % GKC;
+        const int64_t kk = k0 + (tid % GKC);  // the same row for all of this thread's values
+#pragma unroll
+        for (int m = 0; m < NCV; ++m) {
+            const int j = (tid + 256 * m) / GKC;
             const int set = j / CPB;
             const int64_t t = cb * CPB + (j % CPB);
-            const int64_t kk = k0 + k;
             float v = 0.f;
             if (t < Tn && kk < D) {
-                v = (float)src.get(set, t, kk);
-                if (set == 0 && mu) v -= mu[kk];
+                if (set == 0) {
+                    v = src.x0 ? (t == 0 ? src.x0[kk] : src.V[(size_t)(t - 1) * D + kk]) : src.V[(size_t)t * D + kk];
+                } else {
+                    const size_t gi = (size_t)t * D + kk;
+                    v = src.Z ? src.Z[(size_t)(set - 1) * D * Tn + gi] : rademacher(src.keys[set - 1], gi);
+                }
             }
-            Vc[j * SVc + k] = v;
+            cv[m] = v;
+        }
+        muv = (mu && kk < D) ? mu[kk] : 0.f;
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < Dp; k0 += GKC) {
+        __syncthreads();  // the previous slice's MFMAs are done with LDS
+        // Lambda[:, k0:k0+32] (rows d, contiguous in k in the row-major copy)
+#pragma unroll
+        for (int m = 0; m < NLV; ++m) {
+            const int i = tid + 256 * m;
+            if (i < Dp * (GKC / 4)) {
+                const int d = i / (GKC / 4), q = i % (GKC / 4);
+                float* dst = &Ls[d * SL + 4 * q];
+                dst[0] = lv[m].x;
+                dst[1] = lv[m].y;
+                dst[2] = lv[m].z;
+                dst[3] = lv[m].w;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < NCV; ++m) {
+            const int i = tid + 256 * m;
+            const int j = i / GKC, k = i % GKC;
+            // centred states; beyond D value and mu are both zero, and
+            // columns past Tn stay zero
+            const bool cen = j / CPB == 0 && cb * CPB + (j % CPB) < Tn;
+            Vc[j * SVc + k] = cen ? cv[m] - muv : cv[m];
         }
         __syncthreads();
+        if (k0 + GKC < Dp) fetch(k0 + GKC);
 #pragma unroll
         for (int ks = 0; ks < GKC / 4; ++ks) {
             float bv[NF];
