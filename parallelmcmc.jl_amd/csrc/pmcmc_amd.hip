@@ -165,6 +165,8 @@ __global__ void fill_i8_kernel(int8_t* p, int8_t v, int64_t n) {
         p[i] = v;
 }
 
+static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
 __global__ void zero_ints_kernel(int* p, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0;
@@ -561,8 +563,11 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
     if (have && ch.chained == 2 && reg_tile_fits<T>(D, ch) && Tn > ch.TC) {
         const int TC = ch.TC;
         const int nT = (int)((Tn + TC - 1) / TC);
-        const size_t lbm = (size_t)D * nT * sizeof(T);
-        const size_t need = 3 * lbm + ((size_t)nT + 64) * sizeof(int);
+        const int nS = (nT + SCAN_SUPER - 1) / SCAN_SUPER;
+        const size_t lbm = align256((size_t)D * nT * sizeof(T));
+        const size_t sbm = align256((size_t)D * nS * sizeof(T));
+        const size_t nint = (size_t)nT + nS + 64;
+        const size_t need = 2 * lbm + 3 * sbm + nint * sizeof(int);
         if (need > c->scan_blob_cap) {
             if (c->scan_blob) HIPCHK(c, hipFree(c->scan_blob));
             c->scan_blob = nullptr;
@@ -582,13 +587,17 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
         a.numTiles = nT;
         a.lb_agg_a = (T*)blob;
         a.lb_agg_b = (T*)(blob + lbm);
-        a.lb_incl = (T*)(blob + 2 * lbm);
-        int* ints = (int*)(blob + 3 * lbm);
+        a.lb_incl = (T*)(blob + 2 * lbm);  // end states of the super-tiles
+        a.lb_sagg_a = (T*)(blob + 2 * lbm + sbm);
+        a.lb_sagg_b = (T*)(blob + 2 * lbm + 2 * sbm);
+        int* ints = (int*)(blob + 2 * lbm + 3 * sbm);
         a.lb_status = ints;
-        a.lb_ticket = (unsigned*)(ints + nT);
-        a.lb_err = (unsigned*)(ints + nT + 16);
+        a.lb_sstatus = ints + nT;
+        a.lb_ticket = (unsigned*)(ints + nT + nS);
+        a.lb_err = (unsigned*)(ints + nT + nS + 16);
         const int e = c->prof.begin(KT_ITERATION, 0, c->stream);
-        hipLaunchKernelGGL(zero_ints_kernel, dim3(grid_for(nT + 64)), dim3(256), 0, c->stream, ints, (int64_t)nT + 64);
+        hipLaunchKernelGGL(zero_ints_kernel, dim3(grid_for((int64_t)nint)), dim3(256), 0, c->stream, ints,
+                           (int64_t)nint);
         if (TC == 64)
             hipLaunchKernelGGL((scan_reg64_kernel<T, 16>), dim3((unsigned)nT), dim3(256), 0, c->stream, a);
         else if (TC == 128)
@@ -1885,8 +1894,6 @@ struct Items {
     int64_t r_maxT = 0;
     int r_split_glm = 0, r_split_gauss = 0;
 };
-
-size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 }  // namespace
 

@@ -8,23 +8,81 @@
 // lane = row, so every load and store is one coalesced 256-byte column.
 //   1. load the tile (A and B, once), each lane composing its row's map over
 //      its CPW columns; the four waves' maps compose into the tile's map
-//   2. decoupled look-back on tickets (tiles run in ticket order): publish the
-//      tile map, walk back to the nearest inclusive state, the walk split into
-//      four contiguous chunks (one per wave) composed in order
+//   2. two-level decoupled look-back on tickets (tiles run in ticket order).
+//      Tiles are grouped into super-tiles of SUPER consecutive tiles.  A tile
+//      publishes its map, composes the maps of the tiles before it in its
+//      super-tile (at most SUPER - 1), and takes the state at the end of the
+//      previous super-tile from a look-back over super-tiles (their maps,
+//      back to the nearest published end state, or s0).  The last tile of a
+//      super-tile publishes the super-tile's map, then its end state.  With
+//      one level, a tile walks back to the nearest published state over about
+//      half the resident tiles' maps; with two, over fewer than SUPER tile
+//      maps plus resident / SUPER super-tile maps.  Walks are split into four
+//      contiguous chunks (one per wave) composed in order.
 //   3. apply from registers: S written once
-// Look-back publication and loads are the ones of the LDS-tiled form
-// (lb_store / lb_load / lb_publish_fence, bounded spins reported in lb_err).
+// Publication and loads are the ones of the LDS-tiled form (lb_store /
+// lb_load / lb_publish_fence); every wait is a bounded spin reported in lb_err.
 #pragma once
 
 #include "tile_kernel.hpp"
 
 namespace pmcmc {
 
+static constexpr int SCAN_SUPER = 64;  // tiles per super-tile
+
+// Wait until st[idx] >= lvl (bounded); returns the value seen.
+__device__ inline int lb_wait(int* st, int idx, int lvl, unsigned* err) {
+    unsigned spins = 0;
+    int f = __hip_atomic_load(&st[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (f < lvl) {
+        if (++spins > PMCMC_LB_SPIN_CAP) {
+            atomicOr(err, 1u);
+            return 2;
+        }
+        __builtin_amdgcn_s_sleep(2);
+        f = __hip_atomic_load(&st[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return f;
+}
+
+// Map of entries lo..hi of (ma, mb) (D x n, row `lane`), composed so that
+// entry hi is applied last, split over the four waves in contiguous chunks
+// (wave 0 nearest hi); the result is in (ra, rb) of row `lane`, all waves.
+// Needs a barrier before (entries published) and uses pa / pb.
+template <typename T>
+__device__ inline void lb_compose_range(const T* ma, const T* mb, int lo, int hi, int w, int lane, T (*pa)[64],
+                                        T (*pb)[64], T& ra, T& rb) {
+    constexpr int D = 64;
+    const int n = hi - lo + 1;
+    const int chunk = n > 0 ? (n + 3) / 4 : 0;
+    const int hq = hi - w * chunk;
+    const int lq = (hq - chunk + 1) > lo ? (hq - chunk + 1) : lo;
+    T xa = 1, xb = 0;
+    // 8 loads in flight: deeper unrolling costs registers, hence resident
+    // tiles, which is what keeps HBM busy
+#pragma unroll 8
+    for (int jj = hq; jj >= lq; --jj) {
+        xb = xa * lb_load<T>(&mb[(size_t)jj * D + lane]) + xb;
+        xa = xa * lb_load<T>(&ma[(size_t)jj * D + lane]);
+    }
+    pa[w][lane] = xa;
+    pb[w][lane] = xb;
+    __syncthreads();
+    T ca = 1, cb = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        cb = ca * pb[q][lane] + cb;
+        ca = ca * pa[q][lane];
+    }
+    ra = ca;
+    rb = cb;
+}
+
 template <typename T, int CPW>
 __global__ __launch_bounds__(256) void scan_reg64_kernel(TileArgs<T> a) {
     constexpr int D = 64, TC = 4 * CPW;
     __shared__ T wa[4][D], wb[4][D];  // per-wave maps of the tile
-    __shared__ T pa[4][D], pb[4][D];  // look-back chunk maps
+    __shared__ T pa[4][D], pb[4][D];  // look-back chunk maps (D = 64)
     __shared__ T sin_[D];
     __shared__ int tk, lb_k, lb_lo;
     __shared__ int lb_w[4];
@@ -57,90 +115,77 @@ __global__ __launch_bounds__(256) void scan_reg64_kernel(TileArgs<T> a) {
         gb = wa[q][lane] * gb + wb[q][lane];
         ga = wa[q][lane] * ga;
     }
+    const int sup = tile / SCAN_SUPER, pos = tile % SCAN_SUPER;
+    const int s_first = sup * SCAN_SUPER;
     int* st = a.lb_status;
-    if (tile == 0) {
+    int* sst = a.lb_sstatus;
+    // publish this tile's map
+    if (w == 0) {
+        lb_store<T>(&a.lb_agg_a[(size_t)tile * D + lane], ga);
+        lb_store<T>(&a.lb_agg_b[(size_t)tile * D + lane], gb);
+    }
+    lb_publish_fence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&st[tile], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // maps of the tiles before this one in its super-tile (earlier tickets)
+    if (tid < pos) lb_wait(st, tile - 1 - tid, 1, a.lb_err);
+    __syncthreads();
+    T ma = 1, mb = 0;
+    if (pos > 0) lb_compose_range<T>(a.lb_agg_a, a.lb_agg_b, s_first, tile - 1, w, lane, pa, pb, ma, mb);
+    const bool last = pos == SCAN_SUPER - 1;
+    if (last) {  // the super-tile's map
+        __syncthreads();  // pa / pb are reused below
         if (w == 0) {
-            const T s0 = a.s0 ? a.s0[lane] : (T)0;
-            sin_[lane] = s0;
-            lb_store<T>(&a.lb_incl[lane], ga * s0 + gb);
+            lb_store<T>(&a.lb_sagg_a[(size_t)sup * D + lane], ga * ma);
+            lb_store<T>(&a.lb_sagg_b[(size_t)sup * D + lane], ga * mb + gb);
         }
         lb_publish_fence();
         __syncthreads();
-        if (tid == 0) __hip_atomic_store(&st[0], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-        if (w == 0) {
-            lb_store<T>(&a.lb_agg_a[(size_t)tile * D + lane], ga);
-            lb_store<T>(&a.lb_agg_b[(size_t)tile * D + lane], gb);
+        if (tid == 0) __hip_atomic_store(&sst[sup], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // state at the end of super-tile sup - 1: look back over super-tiles in
+    // windows of 256 to the nearest published end state (index -1: s0)
+    T ca = 1, cb = 0;  // map of the super-tiles walked so far
+    T base = 0;
+    int j = sup - 1;
+    for (;;) {
+        {
+            const int idx = j - tid;
+            int f = 0;
+            if (idx == -1) f = 2;
+            else if (idx >= 0) f = lb_wait(sst, idx, 1, a.lb_err);
+            const unsigned long long m = __ballot(f == 2);
+            if (lane == 0) lb_w[w] = m ? (tid + (int)__builtin_ctzll(m)) : -1;
         }
+        __syncthreads();
+        if (tid == 0) {
+            int l = -1;  // nearest super-tile with an end state
+            for (int q = 0; q < 4 && l < 0; ++q) l = lb_w[q];
+            lb_k = l >= 0 ? j - l : -2;
+            lb_lo = j - 255;
+        }
+        __syncthreads();
+        const int k = lb_k, lo = lb_lo;
+        const int stop = k >= -1 ? k + 1 : lo;
+        T xa, xb;
+        lb_compose_range<T>(a.lb_sagg_a, a.lb_sagg_b, stop, j, w, lane, pa, pb, xa, xb);
+        cb = ca * xb + cb;
+        ca = ca * xa;
+        if (k >= -1) {
+            base = k >= 0 ? lb_load<T>(&a.lb_incl[(size_t)k * D + lane]) : (a.s0 ? a.s0[lane] : (T)0);
+            break;
+        }
+        j = lo - 1;
+        __syncthreads();  // lb_w, lb_k, lb_lo, pa, pb are rewritten
+    }
+    const T sprev = ca * base + cb;  // state at the end of super-tile sup - 1
+    const T sin_t = ma * sprev + mb;  // incoming state of this tile
+    if (w == 0) sin_[lane] = sin_t;
+    if (last) {
+        if (w == 0) lb_store<T>(&a.lb_incl[(size_t)sup * D + lane], ga * sin_t + gb);
         lb_publish_fence();
         __syncthreads();
-        if (tid == 0) __hip_atomic_store(&st[tile], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        T ca = 1, cb = 0;  // map of the predecessors walked so far (wave 0)
-        int j = tile - 1;
-        for (;;) {
-            {
-                const int idx = j - tid;
-                int f = 0;
-                if (idx >= 0) {
-                    unsigned spins = 0;
-                    f = __hip_atomic_load(&st[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    while (f == 0) {
-                        if (++spins > PMCMC_LB_SPIN_CAP) {
-                            atomicOr(a.lb_err, 1u);
-                            f = 2;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(2);
-                        f = __hip_atomic_load(&st[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    }
-                }
-                const unsigned long long m = __ballot(f == 2);
-                if (lane == 0) lb_w[w] = m ? (tid + (int)__builtin_ctzll(m)) : -1;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                int l = -1;  // nearest predecessor with an inclusive state
-                for (int q = 0; q < 4 && l < 0; ++q) l = lb_w[q];
-                lb_k = l >= 0 ? j - l : -1;
-                lb_lo = j - 255 < 0 ? 0 : j - 255;
-            }
-            __syncthreads();
-            const int k = lb_k, lo = lb_lo;
-            const int stop = k >= 0 ? k + 1 : lo;
-            // maps of tiles j .. stop, in four contiguous chunks (chunk 0 nearest)
-            const int n = j - stop + 1;
-            const int chunk = (n + 3) / 4;
-            const int hi_q = j - w * chunk;
-            const int lo_q = (hi_q - chunk + 1) > stop ? (hi_q - chunk + 1) : stop;
-            T xa = 1, xb = 0;
-            // 8 loads in flight: deeper unrolling costs registers, hence
-            // resident tiles, which is what keeps HBM busy (measured: 32 in
-            // flight made the 128-column form 1.7x slower)
-#pragma unroll 8
-            for (int jj = hi_q; jj >= lo_q; --jj) {
-                xb = xa * lb_load<T>(&a.lb_agg_b[(size_t)jj * D + lane]) + xb;
-                xa = xa * lb_load<T>(&a.lb_agg_a[(size_t)jj * D + lane]);
-            }
-            pa[w][lane] = xa;
-            pb[w][lane] = xb;
-            __syncthreads();
-            if (w == 0) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    cb = ca * pb[q][lane] + cb;
-                    ca = ca * pa[q][lane];
-                }
-                if (k >= 0) sin_[lane] = ca * lb_load<T>(&a.lb_incl[(size_t)k * D + lane]) + cb;
-            }
-            if (k >= 0) break;
-            j = lo - 1;
-            __syncthreads();  // lb_w, lb_k, lb_lo, pa, pb are rewritten
-        }
-        __syncthreads();  // sin_ is complete
-        if (w == 0) lb_store<T>(&a.lb_incl[(size_t)tile * D + lane], ga * sin_[lane] + gb);
-        lb_publish_fence();
-        __syncthreads();
-        if (tid == 0) __hip_atomic_store(&st[tile], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) __hip_atomic_store(&sst[sup], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     T s = sin_[lane];

@@ -106,6 +106,12 @@ struct TileArgs {
     T* lb_incl;
     unsigned* lb_ticket;
     unsigned* lb_err;
+    // two-level look-back of the register-tile scan: per super-tile (64
+    // consecutive tiles) status (1 aggregate, 2 end state), aggregate maps;
+    // its end states go in lb_incl (D x super-tiles)
+    int* lb_sstatus;
+    T* lb_sagg_a;
+    T* lb_sagg_b;
     // GEMM target products (MODE_GEMM): D x T, probe k of W/hw at k*D*T
     const T* Y;
     const T* gx;
@@ -222,8 +228,11 @@ __device__ inline void stage_lds(T* dst, const T* src, int64_t n, int tid) {
 
 // Publication between workgroups of one launch: values are written and read
 // with agent-scope relaxed atomics (coherent across the XCDs' L2s without an
-// L2 write-back per publication); a workgroup-scope release (wait for this
-// thread's stores) plus the barrier orders them before the status word.
+// L2 write-back per publication).  Before the status word is stored, each
+// publishing thread waits until its stores have completed (vmcnt(0)): a
+// workgroup-scope fence does not wait for global stores, and without the wait
+// a reader can see the status before the values.  The barrier that follows
+// then orders the status store after every publishing thread's wait.
 template <typename T>
 __device__ inline void lb_store(T* p, T v) {
     if constexpr (sizeof(T) == 8)
@@ -232,7 +241,10 @@ __device__ inline void lb_store(T* p, T v) {
     else
         __hip_atomic_store((unsigned*)p, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ inline void lb_publish_fence() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); }
+__device__ inline void lb_publish_fence() {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt and lgkmcnt not waited on
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+}
 
 // Coherent load of a value another workgroup published.
 template <typename T>

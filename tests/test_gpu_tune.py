@@ -143,3 +143,25 @@ def test_register_tile_scan_long_carry_fp32(tc):
     Tune.set_entry("scan_diag", torch.float32, D, T, tc=tc, parts=1, single_pass=2)
     S = DEERScan.solve_affine_scan_diag(A, B, s0).cpu().numpy().astype(np.float64)
     assert np.max(np.abs(S - ref)) <= 1e-4 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("tc,T", [(128, 5000), (256, 64 * 256 * 3 + 17), (64, 1_200_000)])
+def test_register_tile_scan_exact_running_sum(tc, T):
+    """Register-tile scan with A = 1 and integer B (|S| < 2^24, so FP32 is
+    exact): S must equal s0 + cumsum(B) exactly.  Covers one partial group of
+    tiles (T = 5000, 40 tiles), whole groups, and more than 256 groups of 64
+    tiles (T = 1.2e6 at 64 columns per tile: 293 groups, a multi-window
+    look-back)."""
+    import torch
+
+    from ParallelMCMC import DEERScan, Tune
+
+    D = 64
+    rng = np.random.default_rng(T)
+    A = np.ones((D, T), dtype=np.float32)
+    B = rng.integers(-1, 2, (D, T)).astype(np.float32)
+    s0 = rng.integers(-50, 50, D).astype(np.float32)
+    ref = s0[:, None].astype(np.float64) + np.cumsum(B.astype(np.float64), axis=1)
+    Tune.set_entry("scan_diag", torch.float32, D, T, tc=tc, parts=1, single_pass=2)
+    S = DEERScan.solve_affine_scan_diag(A, B, s0).cpu().numpy().astype(np.float64)
+    assert np.array_equal(S, ref)
