@@ -246,7 +246,8 @@ int pmcmc_deer_adjoint(pmcmc_ctx* ctx, pmcmc_deer_ws* ws, const pmcmc_target* tg
  * shape-dependent kernels (tile reduce, spine, tile apply) of `iters`
  * Newton iterations of the caller's problem on `ws` (tolerances ignored), and
  * re-plans ws with the winner.  Both record the winner in the table (used by
- * later scans, workspaces and batches) and return it; `us` is its measured
+ * later scans and workspaces; batched solves keep the rule's shapes, which
+ * measured faster there) and return it; `us` is its measured
  * time per scan / per iteration.  get/set/count/entry let a host save and
  * restore the table (set rejects a shape that cannot run: UNSUPPORTED);
  * get returns tc = 0 when there is no entry.                               */

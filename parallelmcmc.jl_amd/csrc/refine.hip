@@ -81,34 +81,46 @@ __device__ inline bool skipped(const RefineArgs& a) { return a.skip && *(volatil
 
 __device__ inline int iter_of(const RefineArgs& a) { return a.iter_dev ? *a.iter_dev + 1 : a.iter; }
 
-// columns 4 blk .. 4 blk + 3 (one wave each)
-__device__ void refine_flag_body(const RefineArgs& a, int8_t* dec, int* cnt, int* list, int lmax, int par, int64_t D,
-                                 int64_t T, int64_t blk) {
-    if (skipped(a)) return;
-    // every block reaches the same answer: cnt[2] only turns on when the gate
-    // condition (fixed during this launch) holds
-    const bool on = cnt[2] != 0 || *a.last_metric <= a.gate || iter_of(a) >= a.gate_iters;
-    if (blk == 0 && threadIdx.x == 0) {
-        cnt[par ^ 1] = 0;
-        if (on) cnt[2] = 1;
-    }
-    const int lane = threadIdx.x & 63;
-    const int64_t t = blk * (RNT / 64) + (threadIdx.x >> 6);
-    if (t >= T) return;
-    if (!on) {
-        if (lane == 0) dec[t] = -1;
-        return;
-    }
+// log alpha of column t from the FP32 products (one wave); lists t when it is
+// within the band of log u
+__device__ void refine_flag_column(const RefineArgs& a, int8_t* dec, int* cnt, int* list, int lmax, int par,
+                                   int64_t D, int64_t t, int lane) {
     const double eps = a.eps;
     double qyx = 0.0, qxy = 0.0;
-    for (int64_t d = lane; d < D; d += 64) {
-        const size_t gi = (size_t)t * D + d;
-        const double x = state_at(a, t, d, D);
-        const double y = (double)a.Y[gi];
-        const double e1 = y - x - eps * (double)a.gx[gi];
-        const double e2 = x - y - eps * (double)a.gy[gi];
-        qyx += e1 * e1;
-        qxy += e2 * e2;
+    if (D <= 256) {
+        // at most four rows per lane: every load of the column in flight at once
+        float xv[4], yv[4], gxv[4], gyv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t d = lane + 64 * i;
+            if (d < D) {
+                const size_t gi = (size_t)t * D + d;
+                xv[i] = t == 0 ? a.x0[d] : a.S_in[(size_t)(t - 1) * D + d];
+                yv[i] = a.Y[gi];
+                gxv[i] = a.gx[gi];
+                gyv[i] = a.gy[gi];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (lane + 64 * i < D) {
+                const double x = (double)xv[i], y = (double)yv[i];
+                const double e1 = y - x - eps * (double)gxv[i];
+                const double e2 = x - y - eps * (double)gyv[i];
+                qyx += e1 * e1;
+                qxy += e2 * e2;
+            }
+        }
+    } else {
+        for (int64_t d = lane; d < D; d += 64) {
+            const size_t gi = (size_t)t * D + d;
+            const double x = state_at(a, t, d, D);
+            const double y = (double)a.Y[gi];
+            const double e1 = y - x - eps * (double)a.gx[gi];
+            const double e2 = x - y - eps * (double)a.gy[gi];
+            qyx += e1 * e1;
+            qxy += e2 * e2;
+        }
     }
     qyx = wave_sum_r(qyx);
     qxy = wave_sum_r(qxy);
@@ -122,6 +134,30 @@ __device__ void refine_flag_body(const RefineArgs& a, int8_t* dec, int* cnt, int
             const int j = atomicAdd(&cnt[par], 1);
             if (j < lmax) list[j] = (int)t;
         }
+    }
+}
+
+// Columns FLAG_COLS blk .. FLAG_COLS (blk + 1) - 1, one wave per column at a
+// time (a workgroup per handful of columns made the launch dispatch-bound).
+static constexpr int FLAG_COLS = 32;
+__device__ void refine_flag_body(const RefineArgs& a, int8_t* dec, int* cnt, int* list, int lmax, int par, int64_t D,
+                                 int64_t T, int64_t blk) {
+    if (skipped(a)) return;
+    // every block reaches the same answer: cnt[2] only turns on when the gate
+    // condition (fixed during this launch) holds
+    const bool on = cnt[2] != 0 || *a.last_metric <= a.gate || iter_of(a) >= a.gate_iters;
+    if (blk == 0 && threadIdx.x == 0) {
+        cnt[par ^ 1] = 0;
+        if (on) cnt[2] = 1;
+    }
+    const int lane = threadIdx.x & 63;
+    const int64_t t_end = (blk + 1) * FLAG_COLS < T ? (blk + 1) * FLAG_COLS : T;
+    for (int64_t t = blk * FLAG_COLS + (threadIdx.x >> 6); t < t_end; t += RNT / 64) {
+        if (!on) {
+            if (lane == 0) dec[t] = -1;
+            continue;
+        }
+        refine_flag_column(a, dec, cnt, list, lmax, par, D, t, lane);
     }
 }
 
@@ -320,7 +356,7 @@ __global__ __launch_bounds__(RNT) void refine_flag_batched(const RefineArgs* ras
                                                           const int* ids, int par) {
     const int c = ids[blockIdx.z];
     const RefineWork& w = rws[c];
-    if ((int64_t)blockIdx.x * (RNT / 64) >= w.T) return;
+    if ((int64_t)blockIdx.x * FLAG_COLS >= w.T) return;
     refine_flag_body(ras[c], w.dec, w.cnt, w.list, w.lmax, par, w.D, w.T, blockIdx.x);
 }
 __global__ __launch_bounds__(RNT) void refine_pass_glm_batched(const RefineTarget* tgs, const RefineArgs* ras,
@@ -412,8 +448,7 @@ int refine_reset(RefineWork* w, hipStream_t s, std::string* msg) {
 int refine_decisions(RefineWork* w, const RefineTarget& tg, const RefineArgs& a, int iter, hipStream_t s,
                      std::string* msg) {
     const int par = iter & 1;
-    const int64_t cols_per_block = RNT / 64;
-    hipLaunchKernelGGL(refine_flag_kernel, dim3((unsigned)((w->T + cols_per_block - 1) / cols_per_block)), dim3(RNT),
+    hipLaunchKernelGGL(refine_flag_kernel, dim3((unsigned)((w->T + FLAG_COLS - 1) / FLAG_COLS)), dim3(RNT),
                        0, s, a, *w, par);
     // 16 slots of listed columns (2 band T ~ 8 expected at T = 4096, band 1e-3);
     // more are taken in turn by the same workgroups
@@ -492,7 +527,8 @@ void refine_chain_carve(RefineWork* w, const RefineTarget& tg, int64_t L, double
 void refine_decisions_batched(const RefineBatch& b, int par, hipStream_t s) {
     if (b.n_all == 0) return;
     const unsigned slots = 16;
-    hipLaunchKernelGGL(refine_flag_batched, dim3((unsigned)((b.max_T + 3) / 4), 1, (unsigned)b.n_all), dim3(RNT), 0,
+    hipLaunchKernelGGL(refine_flag_batched, dim3((unsigned)((b.max_T + FLAG_COLS - 1) / FLAG_COLS), 1, (unsigned)b.n_all),
+                       dim3(RNT), 0,
                        s, b.ras, b.rws, b.all, par);
     for (int phase = 0; phase < 2; ++phase) {
         if (b.n_glm > 0)
