@@ -325,16 +325,20 @@ static void tune_put(int kind, int esz, int64_t D, int64_t T, const TileChoice& 
 
 // a level-0 shape the tile kernels can run: P parts fit the LDS part arrays
 // (P * D <= max(D, PMCMC_NT)), the tile fits the LDS budget
-// the register-tile single-pass form (chained == 2): D = 64, tiles of
-// 4 CPW columns with CPW in {16, 32, 64} (FP32) or {16, 32} (FP64)
+// the register-tile single-pass forms: D = 64, tiles of 4 CPW columns; one
+// tile per workgroup (chained == 2): CPW in {16, 32, 64} (FP32) or {16, 32}
+// (FP64); persistent workgroups with the next tile in registers too
+// (chained == 3): CPW in {16, 32}
 template <typename T>
 static bool reg_tile_fits(int64_t D, const TileChoice& c) {
-    return D == 64 && (c.TC == 64 || c.TC == 128 || (c.TC == 256 && sizeof(T) == 4));
+    if (D != 64) return false;
+    if (c.chained == 3) return c.TC == 64 || c.TC == 128;
+    return c.TC == 64 || c.TC == 128 || (c.TC == 256 && sizeof(T) == 4);
 }
 
 template <typename T>
 static bool shape_fits(int64_t D, bool with_iterate, bool chained, const TileChoice& c) {
-    if (c.chained == 2) return !with_iterate && reg_tile_fits<T>(D, c);
+    if (c.chained >= 2) return !with_iterate && reg_tile_fits<T>(D, c);
     if (c.TC < 1 || c.P < 1 || c.P > c.TC) return false;
     const int64_t pmax = D >= PMCMC_NT ? 1 : PMCMC_NT / D;
     if (c.P > pmax) return false;
@@ -560,7 +564,7 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
         const char* tp = getenv("PMCMC_SCAN_CHAINED");
         chained = tp && tp[0] == '1';
     }
-    if (have && ch.chained == 2 && reg_tile_fits<T>(D, ch) && Tn > ch.TC) {
+    if (have && ch.chained >= 2 && reg_tile_fits<T>(D, ch) && Tn > ch.TC) {
         const int TC = ch.TC;
         const int nT = (int)((Tn + TC - 1) / TC);
         const int nS = (nT + SCAN_SUPER - 1) / SCAN_SUPER;
@@ -598,13 +602,23 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
         const int e = c->prof.begin(KT_ITERATION, 0, c->stream);
         hipLaunchKernelGGL(zero_ints_kernel, dim3(grid_for((int64_t)nint)), dim3(256), 0, c->stream, ints,
                            (int64_t)nint);
-        if (TC == 64)
+        if (ch.chained == 3) {
+            // persistent: no more workgroups than are resident at once
+            auto kern = TC == 64 ? scan_reg64_persist_kernel<T, 16> : scan_reg64_persist_kernel<T, 32>;
+            int per_cu = 0, n_cu = 0;
+            HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0));
+            HIPCHK(c, hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c->device));
+            const int64_t resident = (int64_t)std::max(per_cu, 1) * std::max(n_cu, 1);
+            const unsigned grid = (unsigned)std::min<int64_t>(nT, resident);
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, c->stream, a);
+        } else if (TC == 64) {
             hipLaunchKernelGGL((scan_reg64_kernel<T, 16>), dim3((unsigned)nT), dim3(256), 0, c->stream, a);
-        else if (TC == 128)
+        } else if (TC == 128) {
             hipLaunchKernelGGL((scan_reg64_kernel<T, 32>), dim3((unsigned)nT), dim3(256), 0, c->stream, a);
-        else
+        } else {
             hipLaunchKernelGGL((scan_reg64_kernel<T, sizeof(T) == 4 ? 64 : 32>), dim3((unsigned)nT), dim3(256), 0,
                                c->stream, a);
+        }
         c->prof.end(e, c->stream);
         HIPCHK(c, hipGetLastError());
         unsigned err = 0;
@@ -614,7 +628,7 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
         if (err) return fail(c, PMCMC_ERR_DEVICE, "scan: look-back wait exceeded its bound");
         return PMCMC_OK;
     }
-    if (have && ch.chained == 2) {  // register form not applicable here: the rule
+    if (have && ch.chained >= 2) {  // register forms not applicable here: the rule
         have = false;
         chained = false;
     }
@@ -1359,13 +1373,14 @@ static std::vector<TileChoice> tile_candidates(int64_t D, int64_t Tn, bool with_
         }
     }
     if (chained_too && D == 64) {
-        for (int tc : {64, 128, 256}) {
-            TileChoice c;
-            c.TC = tc;
-            c.P = 1;
-            c.chained = 2;
-            if (Tn > tc) add(c);
-        }
+        for (int form : {2, 3})
+            for (int tc : {64, 128, 256}) {
+                TileChoice c;
+                c.TC = tc;
+                c.P = 1;
+                c.chained = form;
+                if (Tn > tc && reg_tile_fits<T>(D, c)) add(c);
+            }
     }
     if (chained_too) {
         int tc = 1;
@@ -1534,7 +1549,7 @@ extern "C" int pmcmc_tune_set(int kind, int prec, int64_t D, int64_t T, const pm
     TileChoice ch;
     ch.TC = shape->tc;
     ch.P = shape->parts;
-    ch.chained = kind == PMCMC_TUNE_SCAN_DIAG ? std::min(std::max(shape->single_pass, 0), 2) : 0;
+    ch.chained = kind == PMCMC_TUNE_SCAN_DIAG ? std::min(std::max(shape->single_pass, 0), 3) : 0;
     ch.us = shape->us;
     const bool with_iterate = kind == PMCMC_TUNE_DEER;
     const bool fits = prec == PMCMC_PREC_F64 ? shape_fits<double>(D, with_iterate, ch.chained != 0, ch)

@@ -31,7 +31,7 @@ namespace pmcmc {
 static constexpr int SCAN_SUPER = 64;  // tiles per super-tile
 
 // Wait until st[idx] >= lvl (bounded); returns the value seen.
-__device__ inline int lb_wait(int* st, int idx, int lvl, unsigned* err) {
+__device__ __forceinline__ int lb_wait(int* st, int idx, int lvl, unsigned* err) {
     unsigned spins = 0;
     int f = __hip_atomic_load(&st[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     while (f < lvl) {
@@ -50,7 +50,7 @@ __device__ inline int lb_wait(int* st, int idx, int lvl, unsigned* err) {
 // (wave 0 nearest hi); the result is in (ra, rb) of row `lane`, all waves.
 // Needs a barrier before (entries published) and uses pa / pb.
 template <typename T>
-__device__ inline void lb_compose_range(const T* ma, const T* mb, int lo, int hi, int w, int lane, T (*pa)[64],
+__device__ __forceinline__ void lb_compose_range(const T* ma, const T* mb, int lo, int hi, int w, int lane, T (*pa)[64],
                                         T (*pb)[64], T& ra, T& rb) {
     constexpr int D = 64;
     const int n = hi - lo + 1;
@@ -78,63 +78,85 @@ __device__ inline void lb_compose_range(const T* ma, const T* mb, int lo, int hi
     rb = cb;
 }
 
+// Workgroup state of the register-tile scans (D = 64).
+template <typename T>
+struct RegScanLds {
+    T wa[4][64], wb[4][64];  // per-wave maps of the tile
+    T pa[4][64], pb[4][64];  // look-back chunk maps
+    T sin_[64];              // incoming state of the tile
+    int tk, lb_k, lb_lo;
+    int lb_w[4];
+};
+
+// Issue the loads of tile `tile` (wave w: columns w CPW .. w CPW + CPW - 1 of
+// the tile, lane = row); columns past T read as the identity map.
 template <typename T, int CPW>
-__global__ __launch_bounds__(256) void scan_reg64_kernel(TileArgs<T> a) {
-    constexpr int D = 64, TC = 4 * CPW;
-    __shared__ T wa[4][D], wb[4][D];  // per-wave maps of the tile
-    __shared__ T pa[4][D], pb[4][D];  // look-back chunk maps (D = 64)
-    __shared__ T sin_[D];
-    __shared__ int tk, lb_k, lb_lo;
-    __shared__ int lb_w[4];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    if (tid == 0) tk = (int)atomicAdd(a.lb_ticket, 1u);
-    __syncthreads();
-    const int tile = tk;
-    const int64_t Tn = a.T_len;
-    const int64_t t0 = (int64_t)tile * TC + (int64_t)w * CPW;
-    T av[CPW], bv[CPW];
+__device__ inline void reg_load(const TileArgs<T>& a, int tile, int w, int lane, T (&av)[CPW], T (&bv)[CPW]) {
+    constexpr int D = 64;
+    const int64_t t0 = (int64_t)tile * (4 * CPW) + (int64_t)w * CPW;
 #pragma unroll
     for (int i = 0; i < CPW; ++i) {
         const int64_t t = t0 + i;
-        const bool in = t < Tn;
+        const bool in = t < a.T_len;
         av[i] = in ? a.A[t * D + lane] : (T)1;
         bv[i] = in ? a.B[t * D + lane] : (T)0;
     }
+}
+
+// The tile's map of row `lane` (ga, gb): each wave composes its columns, the
+// four wave maps compose in order.  Leaves the wave maps in L.wa / L.wb.
+template <typename T, int CPW>
+__device__ inline void reg_map(RegScanLds<T>& L, int w, int lane, const T (&av)[CPW], const T (&bv)[CPW], T& ga,
+                               T& gb) {
     T al = 1, be = 0;
 #pragma unroll
     for (int i = 0; i < CPW; ++i) {
         be = av[i] * be + bv[i];
         al = av[i] * al;
     }
-    wa[w][lane] = al;
-    wb[w][lane] = be;
+    L.wa[w][lane] = al;
+    L.wb[w][lane] = be;
     __syncthreads();
-    T ga = 1, gb = 0;  // the tile's map of row `lane`
+    ga = 1;
+    gb = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        gb = wa[q][lane] * gb + wb[q][lane];
-        ga = wa[q][lane] * ga;
+        gb = L.wa[q][lane] * gb + L.wb[q][lane];
+        ga = L.wa[q][lane] * ga;
     }
-    const int sup = tile / SCAN_SUPER, pos = tile % SCAN_SUPER;
-    const int s_first = sup * SCAN_SUPER;
-    int* st = a.lb_status;
-    int* sst = a.lb_sstatus;
-    // publish this tile's map
+}
+
+// Publish tile `tile`'s map (ga, gb) of row `lane` (status 1).
+template <typename T>
+__device__ inline void reg_publish(const TileArgs<T>& a, int tile, T ga, T gb) {
+    constexpr int D = 64;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (w == 0) {
         lb_store<T>(&a.lb_agg_a[(size_t)tile * D + lane], ga);
         lb_store<T>(&a.lb_agg_b[(size_t)tile * D + lane], gb);
     }
     lb_publish_fence();
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(&st[tile], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) __hip_atomic_store(&a.lb_status[tile], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Two-level look-back of tile `tile` (map (ga, gb), already published); on
+// return (after a barrier) L.sin_ holds the tile's incoming state.
+template <typename T>
+__device__ __forceinline__ void reg_lookback(const TileArgs<T>& a, RegScanLds<T>& L, int tile, T ga, T gb) {
+    constexpr int D = 64;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int sup = tile / SCAN_SUPER, pos = tile % SCAN_SUPER;
+    const int s_first = sup * SCAN_SUPER;
+    int* st = a.lb_status;
+    int* sst = a.lb_sstatus;
     // maps of the tiles before this one in its super-tile (earlier tickets)
     if (tid < pos) lb_wait(st, tile - 1 - tid, 1, a.lb_err);
     __syncthreads();
     T ma = 1, mb = 0;
-    if (pos > 0) lb_compose_range<T>(a.lb_agg_a, a.lb_agg_b, s_first, tile - 1, w, lane, pa, pb, ma, mb);
+    if (pos > 0) lb_compose_range<T>(a.lb_agg_a, a.lb_agg_b, s_first, tile - 1, w, lane, L.pa, L.pb, ma, mb);
     const bool last = pos == SCAN_SUPER - 1;
     if (last) {  // the super-tile's map
-        __syncthreads();  // pa / pb are reused below
         if (w == 0) {
             lb_store<T>(&a.lb_sagg_a[(size_t)sup * D + lane], ga * ma);
             lb_store<T>(&a.lb_sagg_b[(size_t)sup * D + lane], ga * mb + gb);
@@ -152,23 +174,25 @@ __global__ __launch_bounds__(256) void scan_reg64_kernel(TileArgs<T> a) {
         {
             const int idx = j - tid;
             int f = 0;
-            if (idx == -1) f = 2;
-            else if (idx >= 0) f = lb_wait(sst, idx, 1, a.lb_err);
+            if (idx == -1)
+                f = 2;
+            else if (idx >= 0)
+                f = lb_wait(sst, idx, 1, a.lb_err);
             const unsigned long long m = __ballot(f == 2);
-            if (lane == 0) lb_w[w] = m ? (tid + (int)__builtin_ctzll(m)) : -1;
+            if (lane == 0) L.lb_w[w] = m ? (tid + (int)__builtin_ctzll(m)) : -1;
         }
         __syncthreads();
         if (tid == 0) {
             int l = -1;  // nearest super-tile with an end state
-            for (int q = 0; q < 4 && l < 0; ++q) l = lb_w[q];
-            lb_k = l >= 0 ? j - l : -2;
-            lb_lo = j - 255;
+            for (int q = 0; q < 4 && l < 0; ++q) l = L.lb_w[q];
+            L.lb_k = l >= 0 ? j - l : -2;
+            L.lb_lo = j - 255;
         }
         __syncthreads();
-        const int k = lb_k, lo = lb_lo;
+        const int k = L.lb_k, lo = L.lb_lo;
         const int stop = k >= -1 ? k + 1 : lo;
         T xa, xb;
-        lb_compose_range<T>(a.lb_sagg_a, a.lb_sagg_b, stop, j, w, lane, pa, pb, xa, xb);
+        lb_compose_range<T>(a.lb_sagg_a, a.lb_sagg_b, stop, j, w, lane, L.pa, L.pb, xa, xb);
         cb = ca * xb + cb;
         ca = ca * xa;
         if (k >= -1) {
@@ -178,23 +202,93 @@ __global__ __launch_bounds__(256) void scan_reg64_kernel(TileArgs<T> a) {
         j = lo - 1;
         __syncthreads();  // lb_w, lb_k, lb_lo, pa, pb are rewritten
     }
-    const T sprev = ca * base + cb;  // state at the end of super-tile sup - 1
+    const T sprev = ca * base + cb;   // state at the end of super-tile sup - 1
     const T sin_t = ma * sprev + mb;  // incoming state of this tile
-    if (w == 0) sin_[lane] = sin_t;
+    if (w == 0) L.sin_[lane] = sin_t;
     if (last) {
         if (w == 0) lb_store<T>(&a.lb_incl[(size_t)sup * D + lane], ga * sin_t + gb);
         lb_publish_fence();
         __syncthreads();
         if (tid == 0) __hip_atomic_store(&sst[sup], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __syncthreads();
-    T s = sin_[lane];
-    for (int q = 0; q < w; ++q) s = wa[q][lane] * s + wb[q][lane];
+    __syncthreads();  // L.sin_ is complete
+}
+
+// Apply the tile from registers: S written once.
+template <typename T, int CPW>
+__device__ inline void reg_apply(const TileArgs<T>& a, const RegScanLds<T>& L, int tile, int w, int lane,
+                                 const T (&av)[CPW], const T (&bv)[CPW]) {
+    constexpr int D = 64;
+    T s = L.sin_[lane];
+    for (int q = 0; q < w; ++q) s = L.wa[q][lane] * s + L.wb[q][lane];
+    const int64_t t0 = (int64_t)tile * (4 * CPW) + (int64_t)w * CPW;
 #pragma unroll
     for (int i = 0; i < CPW; ++i) {
         s = av[i] * s + bv[i];
         const int64_t t = t0 + i;
-        if (t < Tn) a.S[t * D + lane] = s;
+        if (t < a.T_len) a.S[t * D + lane] = s;
+    }
+}
+
+// One tile per workgroup.
+template <typename T, int CPW>
+__global__ __launch_bounds__(256) void scan_reg64_kernel(TileArgs<T> a) {
+    __shared__ RegScanLds<T> L;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) L.tk = (int)atomicAdd(a.lb_ticket, 1u);
+    __syncthreads();
+    const int tile = L.tk;
+    T av[CPW], bv[CPW];
+    reg_load<T, CPW>(a, tile, w, lane, av, bv);
+    T ga, gb;
+    reg_map<T, CPW>(L, w, lane, av, bv, ga, gb);
+    reg_publish<T>(a, tile, ga, gb);
+    reg_lookback<T>(a, L, tile, ga, gb);
+    reg_apply<T, CPW>(a, L, tile, w, lane, av, bv);
+}
+
+// Persistent form: a workgroup keeps taking tickets.  Once its current tile's
+// map is published it takes the next ticket and issues that tile's loads, so
+// HBM streams the next tiles while the resident tiles wait on their
+// predecessors (with one tile per workgroup, the resident tiles load
+// together, then all wait, then all store).  The loads are issued after the
+// publication because publishing waits for the wave's outstanding memory
+// operations.  Progress: the lowest unfinished tile is always being worked
+// on by its holder (a workgroup holds a next ticket only while working on a
+// lower tile), and everything it waits on belongs to lower, finished tiles.
+template <typename T, int CPW>
+__device__ inline void reg_persist_step(const TileArgs<T>& a, RegScanLds<T>& L, int tile, int w, int lane,
+                                        const T (&ca)[CPW], const T (&cb)[CPW], T (&na)[CPW], T (&nb)[CPW],
+                                        int& next) {
+    __syncthreads();  // the previous tile's apply is done with L; L.tk has been read
+    if (threadIdx.x == 0) L.tk = (int)atomicAdd(a.lb_ticket, 1u);
+    T ga, gb;
+    reg_map<T, CPW>(L, w, lane, ca, cb, ga, gb);  // its barrier also publishes L.tk
+    next = L.tk;
+    reg_publish<T>(a, tile, ga, gb);
+    if (next < a.numTiles) reg_load<T, CPW>(a, next, w, lane, na, nb);
+    reg_lookback<T>(a, L, tile, ga, gb);
+    reg_apply<T, CPW>(a, L, tile, w, lane, ca, cb);
+}
+
+template <typename T, int CPW>
+__global__ __launch_bounds__(256) void scan_reg64_persist_kernel(TileArgs<T> a) {
+    __shared__ RegScanLds<T> L;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) L.tk = (int)atomicAdd(a.lb_ticket, 1u);
+    __syncthreads();
+    int tile = L.tk;
+    if (tile >= a.numTiles) return;
+    T a0[CPW], b0[CPW], a1[CPW], b1[CPW];
+    reg_load<T, CPW>(a, tile, w, lane, a0, b0);
+    for (;;) {
+        int next;
+        reg_persist_step<T, CPW>(a, L, tile, w, lane, a0, b0, a1, b1, next);
+        if (next >= a.numTiles) break;
+        tile = next;
+        reg_persist_step<T, CPW>(a, L, tile, w, lane, a1, b1, a0, b0, next);
+        if (next >= a.numTiles) break;
+        tile = next;
     }
 }
 

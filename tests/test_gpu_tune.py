@@ -42,8 +42,8 @@ def test_every_candidate_scan_shape_matches_sequential(D, T):
             if p <= tc:
                 shapes.add((tc, p, False))
         shapes.add((tc, min(pmax, tc), 1))
-    if D == 64:  # register-resident tiles (FP64: 64 or 128 columns)
-        shapes.update({(64, 1, 2), (128, 1, 2)})
+    if D == 64:  # register-resident tiles (FP64: 64 or 128 columns), one per workgroup or persistent
+        shapes.update({(64, 1, 2), (128, 1, 2), (64, 1, 3), (128, 1, 3)})
     checked = 0
     for tc, p, sp in sorted(shapes):
         try:
@@ -121,10 +121,11 @@ def test_deer_tune_then_solve():
     assert np.array_equal(info.accepted.cpu().numpy().astype(bool), acc)
 
 
-@pytest.mark.parametrize("tc", [64, 128, 256])
-def test_register_tile_scan_long_carry_fp32(tc):
-    """Register-tile single-pass scan (D = 64, FP32) on a ragged T with A near 1,
-    so each tile's incoming state depends on many predecessors' maps."""
+@pytest.mark.parametrize("tc,form", [(64, 2), (128, 2), (256, 2), (64, 3), (128, 3)])
+def test_register_tile_scan_long_carry_fp32(tc, form):
+    """Register-tile single-pass scan (D = 64, FP32; one tile per workgroup, or
+    persistent workgroups) on a ragged T with A near 1, so each tile's incoming
+    state depends on many predecessors' maps."""
     import torch
 
     from ParallelMCMC import DEERScan, Tune
@@ -140,15 +141,17 @@ def test_register_tile_scan_long_carry_fp32(tc):
     for t in range(T):
         x = A64[:, t] * x + B64[:, t]
         ref[:, t] = x
-    Tune.set_entry("scan_diag", torch.float32, D, T, tc=tc, parts=1, single_pass=2)
+    Tune.set_entry("scan_diag", torch.float32, D, T, tc=tc, parts=1, single_pass=form)
     S = DEERScan.solve_affine_scan_diag(A, B, s0).cpu().numpy().astype(np.float64)
     assert np.max(np.abs(S - ref)) <= 1e-4 * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("tc,T", [(128, 5000), (256, 64 * 256 * 3 + 17), (64, 1_200_000)])
-def test_register_tile_scan_exact_running_sum(tc, T):
-    """Register-tile scan with A = 1 and integer B (|S| < 2^24, so FP32 is
-    exact): S must equal s0 + cumsum(B) exactly.  Covers one partial group of
+@pytest.mark.parametrize("tc,T,form", [(128, 5000, 2), (256, 64 * 256 * 3 + 17, 2), (64, 1_200_000, 2),
+                                       (128, 5000, 3), (64, 1_200_000, 3), (128, 64 * 128 * 5 + 3, 3)])
+def test_register_tile_scan_exact_running_sum(tc, T, form):
+    """Register-tile scans (form 2: one tile per workgroup; 3: persistent
+    workgroups) with A = 1 and integer B (|S| < 2^24, so FP32 is exact): S
+    must equal s0 + cumsum(B) exactly.  Covers one partial group of
     tiles (T = 5000, 40 tiles), whole groups, and more than 256 groups of 64
     tiles (T = 1.2e6 at 64 columns per tile: 293 groups, a multi-window
     look-back)."""
@@ -162,6 +165,6 @@ def test_register_tile_scan_exact_running_sum(tc, T):
     B = rng.integers(-1, 2, (D, T)).astype(np.float32)
     s0 = rng.integers(-50, 50, D).astype(np.float32)
     ref = s0[:, None].astype(np.float64) + np.cumsum(B.astype(np.float64), axis=1)
-    Tune.set_entry("scan_diag", torch.float32, D, T, tc=tc, parts=1, single_pass=2)
+    Tune.set_entry("scan_diag", torch.float32, D, T, tc=tc, parts=1, single_pass=form)
     S = DEERScan.solve_affine_scan_diag(A, B, s0).cpu().numpy().astype(np.float64)
     assert np.array_equal(S, ref)
