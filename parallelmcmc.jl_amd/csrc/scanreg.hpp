@@ -81,9 +81,9 @@ __device__ __forceinline__ void lb_compose_range(const T* ma, const T* mb, int l
 // Workgroup state of the register-tile scans (D = 64).
 template <typename T>
 struct RegScanLds {
-    T wa[4][64], wb[4][64];  // per-wave maps of the tile
-    T pa[4][64], pb[4][64];  // look-back chunk maps
-    T sin_[64];              // incoming state of the tile
+    T wa[2][4][64], wb[2][4][64];  // per-wave maps of the tile (two tiles in flight: parity)
+    T pa[4][64], pb[4][64];        // look-back chunk maps
+    T sin_[2][64];                 // incoming state of the tile (parity)
     int tk, lb_k, lb_lo;
     int lb_w[4];
 };
@@ -106,23 +106,23 @@ __device__ inline void reg_load(const TileArgs<T>& a, int tile, int w, int lane,
 // The tile's map of row `lane` (ga, gb): each wave composes its columns, the
 // four wave maps compose in order.  Leaves the wave maps in L.wa / L.wb.
 template <typename T, int CPW>
-__device__ inline void reg_map(RegScanLds<T>& L, int w, int lane, const T (&av)[CPW], const T (&bv)[CPW], T& ga,
-                               T& gb) {
+__device__ inline void reg_map(RegScanLds<T>& L, int par, int w, int lane, const T (&av)[CPW], const T (&bv)[CPW],
+                               T& ga, T& gb) {
     T al = 1, be = 0;
 #pragma unroll
     for (int i = 0; i < CPW; ++i) {
         be = av[i] * be + bv[i];
         al = av[i] * al;
     }
-    L.wa[w][lane] = al;
-    L.wb[w][lane] = be;
+    L.wa[par][w][lane] = al;
+    L.wb[par][w][lane] = be;
     __syncthreads();
     ga = 1;
     gb = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        gb = L.wa[q][lane] * gb + L.wb[q][lane];
-        ga = L.wa[q][lane] * ga;
+        gb = L.wa[par][q][lane] * gb + L.wb[par][q][lane];
+        ga = L.wa[par][q][lane] * ga;
     }
 }
 
@@ -141,9 +141,10 @@ __device__ inline void reg_publish(const TileArgs<T>& a, int tile, T ga, T gb) {
 }
 
 // Two-level look-back of tile `tile` (map (ga, gb), already published); on
-// return (after a barrier) L.sin_ holds the tile's incoming state.
+// return (after a barrier) L.sin_[par] holds the tile's incoming state.
 template <typename T>
-__device__ __forceinline__ void reg_lookback(const TileArgs<T>& a, RegScanLds<T>& L, int tile, T ga, T gb) {
+__device__ __forceinline__ void reg_lookback(const TileArgs<T>& a, RegScanLds<T>& L, int par, int tile, T ga,
+                                             T gb) {
     constexpr int D = 64;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int sup = tile / SCAN_SUPER, pos = tile % SCAN_SUPER;
@@ -204,23 +205,23 @@ __device__ __forceinline__ void reg_lookback(const TileArgs<T>& a, RegScanLds<T>
     }
     const T sprev = ca * base + cb;   // state at the end of super-tile sup - 1
     const T sin_t = ma * sprev + mb;  // incoming state of this tile
-    if (w == 0) L.sin_[lane] = sin_t;
+    if (w == 0) L.sin_[par][lane] = sin_t;
     if (last) {
         if (w == 0) lb_store<T>(&a.lb_incl[(size_t)sup * D + lane], ga * sin_t + gb);
         lb_publish_fence();
         __syncthreads();
         if (tid == 0) __hip_atomic_store(&sst[sup], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __syncthreads();  // L.sin_ is complete
+    __syncthreads();  // L.sin_[par] is complete
 }
 
 // Apply the tile from registers: S written once.
 template <typename T, int CPW>
-__device__ inline void reg_apply(const TileArgs<T>& a, const RegScanLds<T>& L, int tile, int w, int lane,
+__device__ inline void reg_apply(const TileArgs<T>& a, const RegScanLds<T>& L, int par, int tile, int w, int lane,
                                  const T (&av)[CPW], const T (&bv)[CPW]) {
     constexpr int D = 64;
-    T s = L.sin_[lane];
-    for (int q = 0; q < w; ++q) s = L.wa[q][lane] * s + L.wb[q][lane];
+    T s = L.sin_[par][lane];
+    for (int q = 0; q < w; ++q) s = L.wa[par][q][lane] * s + L.wb[par][q][lane];
     const int64_t t0 = (int64_t)tile * (4 * CPW) + (int64_t)w * CPW;
 #pragma unroll
     for (int i = 0; i < CPW; ++i) {
@@ -241,34 +242,36 @@ __global__ __launch_bounds__(256) void scan_reg64_kernel(TileArgs<T> a) {
     T av[CPW], bv[CPW];
     reg_load<T, CPW>(a, tile, w, lane, av, bv);
     T ga, gb;
-    reg_map<T, CPW>(L, w, lane, av, bv, ga, gb);
+    reg_map<T, CPW>(L, 0, w, lane, av, bv, ga, gb);
     reg_publish<T>(a, tile, ga, gb);
-    reg_lookback<T>(a, L, tile, ga, gb);
-    reg_apply<T, CPW>(a, L, tile, w, lane, av, bv);
+    reg_lookback<T>(a, L, 0, tile, ga, gb);
+    reg_apply<T, CPW>(a, L, 0, tile, w, lane, av, bv);
 }
 
-// Persistent form: a workgroup keeps taking tickets.  Once its current tile's
-// map is published it takes the next ticket and issues that tile's loads, so
-// HBM streams the next tiles while the resident tiles wait on their
-// predecessors (with one tile per workgroup, the resident tiles load
-// together, then all wait, then all store).  The loads are issued after the
-// publication because publishing waits for the wave's outstanding memory
-// operations.  Progress: the lowest unfinished tile is always being worked
-// on by its holder (a workgroup holds a next ticket only while working on a
-// lower tile), and everything it waits on belongs to lower, finished tiles.
+// Persistent form: a workgroup keeps taking tickets, two tiles in flight.
+// Step i (parity p): map and publish tile i (registers X); apply and store
+// tile i - 1 (registers Y, its look-back done in the previous step); load
+// the next ticket's tile into Y; look back for tile i.  So HBM streams the
+// next tiles while the resident tiles wait on their predecessors (with one
+// tile per workgroup, the resident tiles load together, then all wait, then
+// all store), and publishing tile i (which waits for the wave's outstanding
+// memory operations) does not wait for stores just issued.  Progress: the
+// lowest unfinished tile is always being worked on by its holder (a
+// workgroup holds a next ticket only while working on a lower tile), and
+// everything it waits on belongs to lower tiles that have published.
 template <typename T, int CPW>
-__device__ inline void reg_persist_step(const TileArgs<T>& a, RegScanLds<T>& L, int tile, int w, int lane,
-                                        const T (&ca)[CPW], const T (&cb)[CPW], T (&na)[CPW], T (&nb)[CPW],
-                                        int& next) {
-    __syncthreads();  // the previous tile's apply is done with L; L.tk has been read
+__device__ inline void reg_persist_step(const TileArgs<T>& a, RegScanLds<T>& L, int par, int tile, int prev, int w,
+                                        int lane, const T (&xa)[CPW], const T (&xb)[CPW], T (&ya)[CPW],
+                                        T (&yb)[CPW], int& next) {
+    __syncthreads();  // L.tk has been read; the previous step is done with L.wa[par]
     if (threadIdx.x == 0) L.tk = (int)atomicAdd(a.lb_ticket, 1u);
     T ga, gb;
-    reg_map<T, CPW>(L, w, lane, ca, cb, ga, gb);  // its barrier also publishes L.tk
+    reg_map<T, CPW>(L, par, w, lane, xa, xb, ga, gb);  // its barrier also publishes L.tk
     next = L.tk;
     reg_publish<T>(a, tile, ga, gb);
-    if (next < a.numTiles) reg_load<T, CPW>(a, next, w, lane, na, nb);
-    reg_lookback<T>(a, L, tile, ga, gb);
-    reg_apply<T, CPW>(a, L, tile, w, lane, ca, cb);
+    if (prev >= 0) reg_apply<T, CPW>(a, L, par ^ 1, prev, w, lane, ya, yb);
+    if (next < a.numTiles) reg_load<T, CPW>(a, next, w, lane, ya, yb);
+    reg_lookback<T>(a, L, par, tile, ga, gb);
 }
 
 template <typename T, int CPW>
@@ -281,13 +284,22 @@ __global__ __launch_bounds__(256) void scan_reg64_persist_kernel(TileArgs<T> a) 
     if (tile >= a.numTiles) return;
     T a0[CPW], b0[CPW], a1[CPW], b1[CPW];
     reg_load<T, CPW>(a, tile, w, lane, a0, b0);
+    int prev = -1;
     for (;;) {
         int next;
-        reg_persist_step<T, CPW>(a, L, tile, w, lane, a0, b0, a1, b1, next);
-        if (next >= a.numTiles) break;
+        reg_persist_step<T, CPW>(a, L, 0, tile, prev, w, lane, a0, b0, a1, b1, next);
+        if (next >= a.numTiles) {
+            reg_apply<T, CPW>(a, L, 0, tile, w, lane, a0, b0);
+            break;
+        }
+        prev = tile;
         tile = next;
-        reg_persist_step<T, CPW>(a, L, tile, w, lane, a1, b1, a0, b0, next);
-        if (next >= a.numTiles) break;
+        reg_persist_step<T, CPW>(a, L, 1, tile, prev, w, lane, a1, b1, a0, b0, next);
+        if (next >= a.numTiles) {
+            reg_apply<T, CPW>(a, L, 1, tile, w, lane, a1, b1);
+            break;
+        }
+        prev = tile;
         tile = next;
     }
 }
