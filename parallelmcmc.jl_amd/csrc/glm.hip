@@ -118,9 +118,10 @@ static constexpr int CB = GLM_CB;  // t-columns per column block of the MFMA pas
 static constexpr int NB = 32;   // design rows per LDS chunk
 
 static int glm_splits(int64_t N, int64_t T) {
-    // enough workgroups to fill 256 CUs twice, chunks of >= 8 row blocks each
+    // enough workgroups to fill 256 CUs three times (three fit per CU), chunks
+    // of >= 8 row blocks each
     const int64_t nblk = (T + CB - 1) / CB;
-    int64_t ns = (512 + nblk - 1) / nblk;
+    int64_t ns = (768 + nblk - 1) / nblk;
     const int64_t nchunks = (N + NB - 1) / NB;
     if (ns > nchunks / 8) ns = nchunks / 8;
     if (ns < 1) ns = 1;
@@ -186,10 +187,11 @@ void glm_work_destroy(GlmWork* w) {
 // FP32 MFMA fused pass
 // ---------------------------------------------------------------------------
 // grid: (ceil(T/CB), NS); block 256 = 4 waves.
-// LDS: Vs[NC][Dp+2] column block, Xs[2][NB][Dp+2] design chunks (double
-// buffered), Rs[NB][NC+16].  The next design chunk is fetched into registers
-// (float4) while the current one is multiplied, then stored into the other
-// LDS buffer: two barriers per chunk, global latency off the critical path.
+// LDS: Vs[NC][Dp+2] column block, Xs[NB][Dp+2] design chunk, Rs[NB][NC+4]
+// (53 KB at D = 100: three workgroups per CU).  The next design chunk is
+// fetched into registers (float4) while the current one is multiplied, then
+// stored over it once both GEMMs are done with it: three barriers per chunk,
+// global latency off the critical path.
 template <int DPT, int K, bool JBF>
 __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const float* Xrm, const float* y32,
                                               int64_t N, int64_t rows_per_split, float* part, double* llpart,
@@ -198,14 +200,14 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
     constexpr int NC = CB * (1 + K);
     constexpr int SV = Dp + 2;
     constexpr int SX = Dp + 2;
-    constexpr int SR = NC + 16;
+    constexpr int SR = NC + 4;
     constexpr int NCT = NC / 16;             // column tiles
     static_assert(NCT % 4 == 0, "column tiles must divide over the 4 waves");
     constexpr int CT_PER_WAVE = NCT / 4;
     constexpr int F4 = NB * Dp / 4;          // float4 per design chunk
     constexpr int PF = (F4 + 255) / 256;     // float4 per thread per chunk
     __shared__ float Vs[NC * SV];
-    __shared__ float Xs[2][NB * SX];
+    __shared__ float Xs[NB * SX];
     __shared__ float Rs[NB * SR];
     __shared__ double llred[4][16];
 
@@ -231,14 +233,14 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
             if (idx < F4) pf[i] = src4[idx];
         }
     };
-    auto stash = [&](int b) {
+    auto stash = [&]() {
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
             const int idx = tid + 256 * i;
             if (idx < F4) {
                 const int e = 4 * idx;
                 const int r = e / Dp, d = e % Dp;
-                float2* dst = reinterpret_cast<float2*>(&Xs[b][r * SX + d]);
+                float2* dst = reinterpret_cast<float2*>(&Xs[r * SX + d]);
                 dst[0] = make_float2(pf[i].x, pf[i].y);
                 dst[1] = make_float2(pf[i].z, pf[i].w);
             }
@@ -255,7 +257,7 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
         if (t < T && d < D) v = (float)src.get(set, t, d);
         Vs[j * SV + d] = v;
     }
-    if (nchunk > 0) stash(0);
+    if (nchunk > 0) stash();
     __syncthreads();
 
     // FP32 mode: wave w owns column tile w for every d tile.
@@ -281,10 +283,9 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
     const int cx = wave & 1;   // GEMM-A x column tile
 
     for (int c = 0; c < nchunk; ++c) {
-        const int b = c & 1;
         const int64_t n0 = n_begin + (int64_t)c * NB;
         if (c + 1 < nchunk) fetch(c + 1);
-        const float* X_ = Xs[b];
+        const float* X_ = Xs;
         // GEMM-A: logits of the x tile and the matching probe tiles
         v4f cl[1 + K];
 #pragma unroll
@@ -385,7 +386,10 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
                 }
             }
         }
-        if (c + 1 < nchunk) stash(b ^ 1);
+        if (c + 1 < nchunk) {
+            __syncthreads();  // both GEMMs are done with this chunk
+            stash();
+        }
         __syncthreads();
     }
 
