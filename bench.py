@@ -39,9 +39,9 @@ PEAK_TFLOPS = {"f32": 157.3}  # MI355X dense FP32 MFMA
 PEAK_BF16_TFLOPS = 2500.0     # MI355X dense BF16 MFMA
 # HBM bytes per launch of the FP32 GEMM pass on this workload, from rocprofv3
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) counters, the factors measured on a
-# device copy of known size: 37.9 MiB read (the 4.5 MB design once per XCD L2,
-# plus the columns) + 14.1 MiB of split-N partial sums written.
-PASS_TRAFFIC_BYTES = 37.9 * 2**20 + 14.1 * 2**20
+# device copy of known size: 37.0 MiB read (the 4.5 MB design once per XCD L2,
+# plus the columns) + 21.2 MiB of split-N partial sums written (6 splits).
+PASS_TRAFFIC_BYTES = 37.0 * 2**20 + 21.2 * 2**20
 N_DATA, D, T, EPS = 10000, 100, 4096, 5e-3
 
 
