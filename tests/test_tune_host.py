@@ -40,5 +40,11 @@ def test_shapes_that_cannot_run_are_rejected():
     with pytest.raises(ArgumentError):  # 256-column register tiles are FP32 only
         Tune.set_entry("scan_diag", "f64", 64, 4096, tc=256, parts=1, single_pass=2)
     Tune.set_entry("scan_diag", "f64", 64, 4096, tc=128, parts=1, single_pass=2)
+    with pytest.raises(ArgumentError):  # persistent register tiles: 64 or 128 columns
+        Tune.set_entry("scan_diag", "f32", 64, 4096, tc=256, parts=1, single_pass=3)
+    with pytest.raises(ArgumentError):  # persistent register tiles: D = 64 only
+        Tune.set_entry("scan_diag", "f32", 128, 4096, tc=64, parts=1, single_pass=3)
+    Tune.set_entry("scan_diag", "f32", 64, 4096, tc=128, parts=1, single_pass=3)
+    assert Tune.lookup("scan_diag", "f32", 64, 4096)["single_pass"] == 3
     Tune.clear()
     assert Tune.entries() == []
