@@ -7,9 +7,11 @@
 // hold their Jacobians.
 //
 //   reduce  one workgroup per tile of TC columns composes the tile's map
-//           [M | r] <- J_t [M | r] + [0 | B_t] from [I | 0] (64 x 64 x 65 FMAs
-//           per column on 4 x 4 register blocks; J_t in LDS, J_{t+1} loaded
-//           into registers while J_t is used)
+//           [M | r] <- J_t [M | r] + [0 | B_t] from [I | 0]: J_t M on the 16x16x4
+//           MFMA (wave w: rows 16 w .. 16 w + 15, all 64 columns), J_t's A
+//           fragments loaded from HBM straight into registers (J_{t+1}'s
+//           while J_t is used), M in LDS (two buffers: one barrier per
+//           column); J_t r from the same fragments on the VALU
 //   spine   the tile maps form a dense scan of their own over the tiles
 //           (J' = M_k, B' = r_k, from s0); it is solved by the same kernels
 //           with 8-column tiles, recursively, down to one tile
@@ -23,6 +25,7 @@
 #include <algorithm>
 
 #include "../../include/pmcmc_amd.h"
+#include "mfma.hpp"
 #include "scanfull.hpp"
 
 namespace pmcmc {
@@ -47,94 +50,75 @@ struct SFArgs {
     T* ragg;  // per tile SF
 };
 
+// LDS row stride of the map in the MFMA reduce: 80 floats puts the 64 lanes of
+// a B-fragment read (16 columns x 4 rows) on 64 distinct banks; FP64 keeps
+// the two buffers near 64 KB.
 template <typename T>
-struct Vec4;
-template <>
-struct Vec4<float> {
-    typedef float type __attribute__((ext_vector_type(4)));
-};
-template <>
-struct Vec4<double> {
-    typedef double type __attribute__((ext_vector_type(4)));
+struct SFStride {
+    static constexpr int v = sizeof(T) == 4 ? 80 : 66;
 };
 
 template <typename T>
 __global__ __launch_bounds__(256) void sfull_reduce_kernel(SFArgs<T> p) {
-    typedef typename Vec4<T>::type v4;
-    __shared__ __attribute__((aligned(32))) T Jl[SF][SFS];  // Jl[k][i] = J_t(i, k)
-    __shared__ __attribute__((aligned(32))) T Ml[SF][SFS];  // Ml[k][j] = M(k, j)
-    __shared__ T rl[SF];
-    const int tid = threadIdx.x, ib = tid & 15, jb = tid >> 4;
+    typedef typename Mfma<T>::acc_t acc_t;
+    constexpr int MS = SFStride<T>::v;
+    __shared__ __attribute__((aligned(16))) T Ml[2][SF][MS];  // Ml[b][k][j] = M(k, j)
+    __shared__ T rl[2][SF];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
     const int64_t D = p.D;
     const int64_t tile = blockIdx.x;
     const int64_t t0 = tile * p.TC;
     const int tc = (int)std::min<int64_t>(p.TC, p.len - t0);
-    for (int e = tid; e < SF * SF; e += 256) Ml[e >> 6][e & 63] = (e >> 6) == (e & 63) ? (T)1 : (T)0;
-    if (tid < SF) rl[tid] = 0;
-    T jr[16];
-    T br = 0, bcur = 0;
-    auto fetch = [&](int64_t t) {
+    for (int e = tid; e < SF * SF; e += 256) Ml[0][e >> 6][e & 63] = (e >> 6) == (e & 63) ? (T)1 : (T)0;
+    if (tid < SF) rl[0][tid] = 0;
+    // lane (li, lk) of wave w holds J_t(i = 16 w + li, k = 4 ks + lk), ks = 0..15,
+    // its A fragments, and B_t(i)
+    const int i = 16 * w + li;
+    T ja[16], jn[16];
+    T bc = 0, bn = 0;
+    auto fetch = [&](int64_t t, T(&dst)[16], T& bdst) {
         const T* Jt = p.J + t * p.strideJ;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int e = tid + 256 * q;
-            const int k = e >> 6, i = e & 63;
-            jr[q] = (i < D && k < D) ? Jt[i + k * p.ldJ] : (T)0;
+        for (int ks = 0; ks < 16; ++ks) {
+            const int k = 4 * ks + lk;
+            dst[ks] = (i < D && k < D) ? Jt[i + k * p.ldJ] : (T)0;
         }
-        if (tid < SF) br = tid < D ? p.B[t * p.ldB + tid] : (T)0;
+        bdst = i < D ? p.B[t * p.ldB + i] : (T)0;
     };
-    auto stash = [&]() {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int e = tid + 256 * q;
-            Jl[e >> 6][e & 63] = jr[q];
-        }
-        bcur = br;
-    };
-    fetch(t0);
-    stash();
+    fetch(t0, ja, bc);
     __syncthreads();
     for (int s = 0; s < tc; ++s) {
-        if (s + 1 < tc) fetch(t0 + s + 1);
-        T acc[4][4];
+        const int cur = s & 1;
+        if (s + 1 < tc) fetch(t0 + s + 1, jn, bn);
+        acc_t acc[4];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int ct = 0; ct < 4; ++ct) acc[ct] = (acc_t){0, 0, 0, 0};
+        T rp = 0;
 #pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = 0;
-#pragma unroll 8
-        for (int k = 0; k < SF; ++k) {
-            const v4 av = *reinterpret_cast<const v4*>(&Jl[k][4 * ib]);
-            const v4 bv = *reinterpret_cast<const v4*>(&Ml[k][4 * jb]);
+        for (int ks = 0; ks < 16; ++ks) {
+            const T a = ja[ks];
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+            for (int ct = 0; ct < 4; ++ct) acc[ct] = Mfma<T>::step(a, Ml[cur][4 * ks + lk][16 * ct + li], acc[ct]);
+            rp += a * rl[cur][4 * ks + lk];
         }
-        T rn = 0;
-        if (tid < SF) {
-            T r0 = 0, r1 = 0;
-#pragma unroll 8
-            for (int k = 0; k < SF; k += 2) {
-                r0 += Jl[k][tid] * rl[k];
-                r1 += Jl[k + 1][tid] * rl[k + 1];
-            }
-            rn = (r0 + r1) + bcur;
-        }
+        // r_new(i): this lane's k terms, then the four lanes of row i (lk = 0..3)
+        rp += __shfl_xor(rp, 16, 64);
+        rp += __shfl_xor(rp, 32, 64);
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Ml[cur ^ 1][16 * w + Mfma<T>::row(lk, r)][16 * ct + li] = acc[ct][r];
+        if (lk == 0) rl[cur ^ 1][i] = rp + bc;
         __syncthreads();
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            v4 row;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) row[b] = acc[a][b];
-            *reinterpret_cast<v4*>(&Ml[4 * ib + a][4 * jb]) = row;
-        }
-        if (tid < SF) rl[tid] = rn;
-        if (s + 1 < tc) stash();
-        __syncthreads();
+        for (int ks = 0; ks < 16; ++ks) ja[ks] = jn[ks];
+        bc = bn;
     }
+    const int fin = tc & 1;
     T* Mo = p.Magg + (size_t)tile * SF * SF;
-    for (int e = tid; e < SF * SF; e += 256) Mo[e] = Ml[e & 63][e >> 6];  // column-major: e = i + 64 j
-    if (tid < SF) p.ragg[(size_t)tile * SF + tid] = rl[tid];
+    for (int e = tid; e < SF * SF; e += 256) Mo[e] = Ml[fin][e & 63][e >> 6];  // column-major: e = i + 64 j
+    if (tid < SF) p.ragg[(size_t)tile * SF + tid] = rl[fin][tid];
 }
 
 template <typename T>
