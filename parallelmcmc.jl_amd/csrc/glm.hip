@@ -198,16 +198,16 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
                                               const int* skip, const int64_t cb, const int split) {
     constexpr int Dp = DPT * 16;
     constexpr int NC = CB * (1 + K);
-    constexpr int SV = Dp + 2;
-    constexpr int SX = Dp + 2;
+    constexpr int SV = Dp + 4;  // multiples of 4 floats: 16-byte rows for the BF16 operand reads
+    constexpr int SX = Dp + 4;
     constexpr int SR = NC + 4;
     constexpr int NCT = NC / 16;             // column tiles
     static_assert(NCT % 4 == 0, "column tiles must divide over the 4 waves");
     constexpr int CT_PER_WAVE = NCT / 4;
     constexpr int F4 = NB * Dp / 4;          // float4 per design chunk
     constexpr int PF = (F4 + 255) / 256;     // float4 per thread per chunk
-    __shared__ float Vs[NC * SV];
-    __shared__ float Xs[NB * SX];
+    __shared__ __align__(16) float Vs[NC * SV];
+    __shared__ __align__(16) float Xs[NB * SX];
     __shared__ float Rs[NB * SR];
     __shared__ double llred[4][16];
 
@@ -308,12 +308,20 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
             for (int ks = 0; ks < (Dp + 31) / 32; ++ks) {
                 v8bf av, bv;
                 const int col = CB + 16 * cx + li;
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int d = 32 * ks + 8 * lk + e;
-                    av[e] = (__bf16)(d < Dp ? X_[(16 * rt + li) * SX + d] : 0.f);
-                    bv[e] = (__bf16)(d < Dp ? Vs[col * SV + d] : 0.f);
+                const int d0 = 32 * ks + 8 * lk;  // Dp % 8 == 0: the 8 entries are all in or all out
+                float4 xa0 = make_float4(0.f, 0.f, 0.f, 0.f), xa1 = xa0, vb0 = xa0, vb1 = xa0;
+                if (d0 < Dp) {
+                    const float4* xp = reinterpret_cast<const float4*>(&X_[(16 * rt + li) * SX + d0]);
+                    const float4* vp = reinterpret_cast<const float4*>(&Vs[col * SV + d0]);
+                    xa0 = xp[0];
+                    xa1 = xp[1];
+                    vb0 = vp[0];
+                    vb1 = vp[1];
                 }
+                av[0] = (__bf16)xa0.x; av[1] = (__bf16)xa0.y; av[2] = (__bf16)xa0.z; av[3] = (__bf16)xa0.w;
+                av[4] = (__bf16)xa1.x; av[5] = (__bf16)xa1.y; av[6] = (__bf16)xa1.z; av[7] = (__bf16)xa1.w;
+                bv[0] = (__bf16)vb0.x; bv[1] = (__bf16)vb0.y; bv[2] = (__bf16)vb0.z; bv[3] = (__bf16)vb0.w;
+                bv[4] = (__bf16)vb1.x; bv[5] = (__bf16)vb1.y; bv[6] = (__bf16)vb1.z; bv[7] = (__bf16)vb1.w;
                 cl[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, cl[1], 0, 0, 0);
             }
         }
