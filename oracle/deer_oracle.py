@@ -815,7 +815,9 @@ PINNED = {
                     "hard-decision differences where no decision flips",
     "logq_mala": "closed-form Gaussian log-density (scipy.stats.multivariate_normal) at 1e-12",
     "mala_step_taped": "forced accept/reject via u=tau/2 and (tau+1)/2; rejected => unchanged; determinism",
-    "run_mala_sequential_taped": "stationary moments on N(0,I); DEER fixed point equals it",
+    "run_mala_sequential_taped": "stationary moments on N(0,I); DEER fixed point equals it; equals the "
+                                 "independent C restatement (oracle/mala_seq.c) on every target: identical "
+                                 "decisions, states to 1e-12; committed fixtures tests/golden/mala_seq_*.npz",
     "mala_step_batched_fwd_and_jvp": "column-by-column equals the scalar step+JVP; JVP equals central FD of the frozen-accept surrogate",
     "mala_jac_diag_batched": "equals jac_diag_via_jvps (D basis JVPs) at 1e-10",
     "targets": "grad = central FD of logp, hvp = central FD of grad (FP64), logreg vs explicit sums",
