@@ -21,16 +21,21 @@
 //                      registers (fp32 master); Lambda M runs on MFMA with
 //                      Lambda's operand fragments in registers and M's from an
 //                      LDS copy (FP32 16x16x4 or BF16 16x16x32; FP64 on VALU)
-//   full_spine_kernel  one workgroup: Delta at each tile end,
-//                      Delta_k = M_k Delta_{k-1} + r_k
+//   spine              Delta at each tile end, Delta_k = M_k Delta_{k-1} + r_k:
+//                      a dense scan over the tile maps, solved by the
+//                      recursive reduce/apply kernels of scanfull.hip
+//                      (8-map tiles, MFMA composition), not a chain of
+//                      dependent mat-vecs
 //   full_apply_kernel  one wave per tile: Delta through the tile from the
 //                      incoming Delta, S_new, convergence maxima, metric
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "../../include/pmcmc_amd.h"
 #include "fullscan.hpp"
+#include "scanfull.hpp"
 
 namespace pmcmc {
 
@@ -476,62 +481,6 @@ __global__ __launch_bounds__(256) void full_reduce_f64_kernel(FullArgs<double> p
 }
 
 // ---------------------------------------------------------------------------
-// spine: one workgroup, Delta_k = M_k Delta_{k-1} + r_k over the tiles.  Thread
-// (quarter q, lane i) sums columns 16q..16q+15 of row i.  The chain is a
-// dependent mat-vec per tile, so the tiles' entries stream through a ring of
-// SPF register slots: loads run SPF tiles ahead of the chain.
-// ---------------------------------------------------------------------------
-static constexpr int SPF = 4;
-
-template <typename T>
-__global__ __launch_bounds__(256) void full_spine_kernel(FullArgs<T> p) {
-    __shared__ T v[FD];
-    __shared__ T part[4][FD];
-    if (converged(p.sy)) return;
-    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
-    if (tid < FD) v[tid] = 0;
-    T ring[SPF][16];
-    T rring[SPF];
-    auto fetch = [&](int k, int slot) {
-        const T* M = p.Magg + (size_t)k * FD * FD;
-#pragma unroll
-        for (int jj = 0; jj < 16; ++jj) ring[slot][jj] = M[(size_t)(16 * q + jj) * FD + lane];
-        rring[slot] = q == 0 ? p.ragg[(size_t)k * FD + lane] : (T)0;
-    };
-#pragma unroll
-    for (int sl = 0; sl < SPF; ++sl)
-        if (sl < p.nT) fetch(sl, sl);
-    __syncthreads();
-    for (int k0 = 0; k0 < p.nT; k0 += SPF) {
-#pragma unroll
-        for (int sl = 0; sl < SPF; ++sl) {
-            const int k = k0 + sl;
-            if (k < p.nT) {
-                T mc[16];
-#pragma unroll
-                for (int jj = 0; jj < 16; ++jj) mc[jj] = ring[sl][jj];
-                const T rc = rring[sl];
-                if (k + SPF < p.nT) fetch(k + SPF, sl);
-                T s0 = 0, s1 = 0;
-#pragma unroll
-                for (int jj = 0; jj < 16; jj += 2) {
-                    s0 += mc[jj] * v[16 * q + jj];
-                    s1 += mc[jj + 1] * v[16 * q + jj + 1];
-                }
-                part[q][lane] = s0 + s1;
-                __syncthreads();
-                if (q == 0) {
-                    const T nv = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane] + rc;
-                    v[lane] = nv;
-                    p.dend[(size_t)k * FD + lane] = nv;
-                }
-                __syncthreads();
-            }
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // apply: one wave per tile; lane d = row d with Lambda's row d in registers.
 // The step's operands stream through a ring of APF register slots.
 // ---------------------------------------------------------------------------
@@ -654,7 +603,10 @@ int full_work_ensure(FullWork* w, int64_t D, int64_t T, bool f64, std::string* m
               hipMalloc(&w->c, mat) == hipSuccess && hipMalloc(&w->rho, mat) == hipSuccess &&
               hipMalloc(&w->Magg, (size_t)w->nT * FD * FD * esz) == hipSuccess &&
               hipMalloc(&w->ragg, (size_t)w->nT * FD * esz) == hipSuccess &&
-              hipMalloc(&w->dend, (size_t)w->nT * FD * esz) == hipSuccess;
+              hipMalloc(&w->dend, (size_t)w->nT * FD * esz) == hipSuccess &&
+              hipMalloc(&w->spine, std::max<size_t>(scan_full_spine_scratch_bytes(w->nT, f64), 256)) == hipSuccess &&
+              // the spine writes rows < D only; the apply reads all 64 lanes
+              hipMemset(w->dend, 0, (size_t)w->nT * FD * esz) == hipSuccess;
     if (!ok) {
         full_work_destroy(w);
         *msg = "full-Jacobian workspace allocation failed";
@@ -671,6 +623,7 @@ void full_work_destroy(FullWork* w) {
     (void)hipFree(w->Magg);
     (void)hipFree(w->ragg);
     (void)hipFree(w->dend);
+    (void)hipFree(w->spine);
     *w = FullWork{};
 }
 
@@ -733,7 +686,7 @@ int full_newton_step(FullWork* fw, const GlmWork* gw, const GaussData* g, hipStr
     }
     if (prof) prof->end(e, s);
     e = prof ? prof->begin(KT_SCAN_SPINE, iter, s) : -1;
-    hipLaunchKernelGGL((full_spine_kernel<T>), dim3(1), dim3(256), 0, s, p);
+    scan_full_spine<T>(p.Magg, p.ragg, p.dend, fw->D, fw->nT, &sync->converged, fw->spine, s);
     if (prof) prof->end(e, s);
     e = prof ? prof->begin(KT_TILE_APPLY, iter, s) : -1;
     hipLaunchKernelGGL((full_apply_kernel<T>), dim3((unsigned)fw->nT), dim3(64), 0, s, p);

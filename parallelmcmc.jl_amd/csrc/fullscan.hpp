@@ -24,7 +24,8 @@ struct FullWork {
     void* rho = nullptr;   // D x T: FT - S (Newton residual)
     void* Magg = nullptr;  // nT x 64 x 64: product of the tile's Jacobians (column-major)
     void* ragg = nullptr;  // nT x 64: the tile's Delta from a zero start
-    void* dend = nullptr;  // nT x 64: Delta at each tile end
+    void* dend = nullptr;  // nT x 64: Delta at each tile end (rows >= D stay zero)
+    void* spine = nullptr;  // scratch of the recursive scan over the tile maps
 };
 
 int full_work_ensure(FullWork* w, int64_t D, int64_t T, bool f64, std::string* msg);

@@ -40,7 +40,7 @@ struct SFArgs {
     int64_t ldJ, strideJ;
     const T* B;
     int64_t ldB;
-    const T* s0;    // D entries
+    const T* s0;    // D entries, or null for a zero start
     const T* prev;  // state after each tile of this level (ld SF), from the level above
     T* S;
     int64_t ldS;
@@ -48,6 +48,7 @@ struct SFArgs {
     int TC, nT;
     T* Magg;  // per tile SF x SF, column-major (ld SF)
     T* ragg;  // per tile SF
+    const int* skip;  // nonzero => no-op (a converged Newton loop); may be null
 };
 
 // LDS row stride of the map in the MFMA reduce: 80 floats puts the 64 lanes of
@@ -70,6 +71,7 @@ __global__ __launch_bounds__(256) void sfull_reduce_kernel(SFArgs<T> p) {
     const int64_t tile = blockIdx.x;
     const int64_t t0 = tile * p.TC;
     const int tc = (int)std::min<int64_t>(p.TC, p.len - t0);
+    if (p.skip && *(volatile const int*)p.skip) return;
     for (int e = tid; e < SF * SF; e += 256) Ml[0][e >> 6][e & 63] = (e >> 6) == (e & 63) ? (T)1 : (T)0;
     if (tid < SF) rl[0][tid] = 0;
     // lane (li, lk) of wave w holds J_t(i = 16 w + li, k = 4 ks + lk), ks = 0..15,
@@ -130,7 +132,9 @@ __global__ __launch_bounds__(256) void sfull_apply_kernel(SFArgs<T> p) {
     const int64_t tile = blockIdx.x;
     const int64_t t0 = tile * p.TC;
     const int tc = (int)std::min<int64_t>(p.TC, p.len - t0);
-    if (tid < SF) v[i] = i >= D ? (T)0 : (tile == 0 ? p.s0[i] : p.prev[(size_t)(tile - 1) * SF + i]);
+    if (p.skip && *(volatile const int*)p.skip) return;
+    if (tid < SF)
+        v[i] = i >= D ? (T)0 : (tile == 0 ? (p.s0 ? p.s0[i] : (T)0) : p.prev[(size_t)(tile - 1) * SF + i]);
     T jr[SAPF][16];
     T br[SAPF];
     auto fetch = [&](int s, int sl) {
@@ -178,10 +182,10 @@ __global__ __launch_bounds__(256) void sfull_apply_kernel(SFArgs<T> p) {
 static int level_tc(int level) { return level == 0 ? 64 : 8; }
 
 template <typename T>
-static size_t sfull_bytes_t(int64_t T_len) {
+static size_t sfull_bytes_t(int64_t T_len, int level0 = 0) {
     size_t bytes = 0;
     int64_t len = T_len;
-    for (int level = 0; len > level_tc(level); ++level) {
+    for (int level = level0; len > level_tc(level); ++level) {
         const int64_t nT = (len + level_tc(level) - 1) / level_tc(level);
         bytes += (size_t)nT * (SF * SF + 2 * SF) * sizeof(T);
         len = nT;
@@ -191,10 +195,16 @@ static size_t sfull_bytes_t(int64_t T_len) {
 
 size_t scan_full_scratch_bytes(int64_t T, bool f64) { return f64 ? sfull_bytes_t<double>(T) : sfull_bytes_t<float>(T); }
 
+size_t scan_full_spine_scratch_bytes(int64_t nT, bool f64) {
+    return f64 ? sfull_bytes_t<double>(nT, 1) : sfull_bytes_t<float>(nT, 1);
+}
+
 template <typename T>
 static void sfull_rec(int level, const T* J, int64_t ldJ, int64_t strideJ, const T* B, int64_t ldB, const T* s0,
-                      T* S, int64_t ldS, int64_t D, int64_t len, char* scratch, hipStream_t st) {
+                      T* S, int64_t ldS, int64_t D, int64_t len, char* scratch, hipStream_t st,
+                      const int* skip = nullptr) {
     SFArgs<T> a{};
+    a.skip = skip;
     a.J = J;
     a.ldJ = ldJ;
     a.strideJ = strideJ;
@@ -221,7 +231,7 @@ static void sfull_rec(int level, const T* J, int64_t ldJ, int64_t strideJ, const
     a.ragg = ragg;
     hipLaunchKernelGGL(sfull_reduce_kernel<T>, dim3((unsigned)a.nT), dim3(256), 0, st, a);
     // state after every tile: the dense scan of the tile maps
-    sfull_rec<T>(level + 1, Magg, SF, SF * SF, ragg, SF, s0, dend, SF, D, a.nT, rest, st);
+    sfull_rec<T>(level + 1, Magg, SF, SF * SF, ragg, SF, s0, dend, SF, D, a.nT, rest, st, skip);
     a.prev = dend;
     hipLaunchKernelGGL(sfull_apply_kernel<T>, dim3((unsigned)a.nT), dim3(256), 0, st, a);
 }
@@ -241,5 +251,16 @@ int scan_full(bool f64, const void* J, const void* B, const void* s0, void* S, i
     }
     return PMCMC_OK;
 }
+
+template <typename T>
+void scan_full_spine(const T* M, const T* r, T* out, int64_t D, int64_t nT, const int* skip, void* scratch,
+                     hipStream_t st) {
+    if (nT > 0) sfull_rec<T>(1, M, SF, SF * SF, r, SF, nullptr, out, SF, D, nT, (char*)scratch, st, skip);
+}
+
+template void scan_full_spine<float>(const float*, const float*, float*, int64_t, int64_t, const int*, void*,
+                                     hipStream_t);
+template void scan_full_spine<double>(const double*, const double*, double*, int64_t, int64_t, const int*, void*,
+                                      hipStream_t);
 
 }  // namespace pmcmc

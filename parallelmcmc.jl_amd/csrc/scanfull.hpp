@@ -14,4 +14,14 @@ size_t scan_full_scratch_bytes(int64_t T, bool f64);
 int scan_full(bool f64, const void* J, const void* B, const void* s0, void* S, int64_t D, int64_t T, void* scratch,
               hipStream_t st, std::string* msg);
 
+// The scan over tile maps of a structured solve: out[:,k] = M_k out[:,k-1] + r[:,k]
+// from a zero start, k < nT; M_k the k-th 64 x 64 column-major map, r and out
+// 64 apart (rows >= D of out are left untouched).  The same recursive
+// reduce/apply kernels with 8-map tiles; every kernel is a no-op while *skip
+// is nonzero (skip may be null).
+size_t scan_full_spine_scratch_bytes(int64_t nT, bool f64);
+template <typename T>
+void scan_full_spine(const T* M, const T* r, T* out, int64_t D, int64_t nT, const int* skip, void* scratch,
+                     hipStream_t st);
+
 }  // namespace pmcmc

@@ -29,7 +29,10 @@ def _setup(seed, D, cond=2.0, with_mean=True):
 
 
 @pytest.mark.parametrize("jac", ["full", "full_sigmoid"])
-@pytest.mark.parametrize("D,T,damping", [(5, 200, 0.7), (16, 64, 1.0), (64, 130, 1.0)])
+# T = 4477 (70 tiles) and 38417 (601 tiles) take the scan over the tile maps
+# through two and three reduce levels of 8-map tiles
+@pytest.mark.parametrize("D,T,damping", [(5, 200, 0.7), (16, 64, 1.0), (64, 130, 1.0), (16, 4477, 1.0),
+                                         (4, 38417, 0.9)])
 def test_full_newton_step_f64_matches_dense_oracle(D, T, damping, jac):
     from ParallelMCMC import DEER
 
@@ -50,13 +53,15 @@ def test_full_newton_step_f64_matches_dense_oracle(D, T, damping, jac):
 
 @pytest.mark.parametrize("jac", ["full", "full_sigmoid"])
 @pytest.mark.parametrize("prec,tol", [("f32", 2e-4), ("bf16_jac", 5e-2)])
-def test_full_newton_step_f32_bf16(prec, tol, jac):
-    """One step at D=64 over 3 tiles: FP32 products agree with the dense FP64
-    step to FP32 accuracy; BF16 products perturb only the Jacobian part."""
+@pytest.mark.parametrize("T", [150, 2567])
+def test_full_newton_step_f32_bf16(prec, tol, jac, T):
+    """One step at D=64 over 3 tiles (41 at T = 2567: one reduce level over the
+    tile maps): FP32 products agree with the dense FP64 step to FP32 accuracy;
+    BF16 products perturb only the Jacobian part."""
     from ParallelMCMC import DEER
 
     rng, tgt, model, ev = _setup(7, 64)
-    D, T = 64, 150
+    D = 64
     eps = 0.8 / ev.max()
     x0 = tgt.mean + rng.standard_normal(D)
     Xi, U = O.make_tape(rng, D, T)
