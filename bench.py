@@ -240,15 +240,25 @@ def bench_full(args):
         ws = DEER.DEERWorkspace(Dg, Tg, rec.precision)
         DEER.solve(rec, x0.astype(np.float32), tol_abs=1e-6, tol_rel=1e-6, maxiter=4, jacobian="full", workspace=ws)
         torch.cuda.synchronize()
+        # events only around the dominant kernel inside the timed solve
         ctx.reset_kernel_times()
-        ctx.set_profiling(True)
+        ctx.set_profiling(True, kinds=["tile_reduce"])
         t0 = time.perf_counter()
         S, info = DEER.solve(rec, x0.astype(np.float32), tol_abs=1e-6, tol_rel=1e-6, maxiter=Tg + 1,
                              jacobian="full", return_info=True, workspace=ws)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         ctx.set_profiling(False)
+        kt_timed = ctx.kernel_times()
+        # per-kernel split: 64 more iterations (untimed) with every kind profiled
+        ctx.reset_kernel_times()
+        ctx.set_profiling(True)
+        DEER.solve(rec, x0.astype(np.float32), tol_abs=1e-6, tol_rel=1e-6, maxiter=64, jacobian="full",
+                   workspace=ws)
+        torch.cuda.synchronize()
+        ctx.set_profiling(False)
         kt = ctx.kernel_times()
+        kt["tile_reduce"] = kt_timed["tile_reduce"]
         Sn = S.cpu().numpy().astype(np.float64)
         log(f"full {prec}: {info.iters} iterations, {el:.2f} s")
         res[prec] = {"samples_per_s": Tg / el, "seconds": el, "newton_iters": info.iters, "converged": info.converged,
@@ -566,8 +576,10 @@ def main():
             if b == 0:
                 first = (S.cpu().numpy().astype(np.float64), info.accepted.cpu().numpy().astype(bool), info.iters)
             xs = S[:, T - 1].contiguous().clone()
+        # events only around the dominant kernel inside the timed region: a pair
+        # around every launch costs about 11% of the throughput on this workload
         ctx.reset_kernel_times()
-        ctx.set_profiling(True)
+        ctx.set_profiling(True, kinds=["glm_pass"])
         torch.cuda.synchronize()
         iters = []
         keep = None
@@ -584,10 +596,18 @@ def main():
         elapsed = time.perf_counter() - t0
         ctx.set_profiling(False)
         kt = ctx.kernel_times()
+        # per-kernel split of one Newton iteration: one more block (untimed) with
+        # every kind profiled
+        ctx.reset_kernel_times()
+        ctx.set_profiling(True)
+        run_block(0, torch.from_numpy(x0).to(dev))
+        torch.cuda.synchronize()
+        ctx.set_profiling(False)
+        kt_all = ctx.kernel_times()
         if first is None:
             first = (keep[0].cpu().numpy().astype(np.float64), keep[1].cpu().numpy().astype(bool), keep[2])
         return {"prec": prec, "elapsed": elapsed, "sps": args.steps * T / elapsed, "iters": iters, "kt": kt,
-                "first": first, "shape": shape}
+                "kt_all": kt_all, "first": first, "shape": shape}
 
     res = {v: run_variant(v) for v in variants}
     head = res["bf16_jac"] if "bf16_jac" in res else res[variants[0]]
@@ -622,7 +642,10 @@ def main():
             "variants": {v: {"samples_per_s": r["sps"], "ms_per_block": 1e3 * r["elapsed"] / args.steps,
                              "newton_iters_per_block": r["iters"],
                              "tile_shape": r["shape"],
-                             "kernel_ms": {k: (x[1] / x[0] if x[0] else None) for k, x in r["kt"].items()}}
+                             "glm_pass_ms_timed": (r["kt"]["glm_pass"][1] / r["kt"]["glm_pass"][0]
+                                                   if r["kt"]["glm_pass"][0] else None),
+                             "kernel_ms_untimed_block": {k: (x[1] / x[0] if x[0] else None)
+                                                         for k, x in r["kt_all"].items()}}
                          for v, r in res.items()},
         },
         "roofline": {

@@ -79,7 +79,11 @@ const char* pmcmc_version(void);
 #define PMCMC_KT_GLM_REDUCE 3  /* split-sum + prior terms of a GEMM pass */
 #define PMCMC_KT_SCAN_SPINE 4  /* scan of the tile aggregates (all levels) */
 #define PMCMC_KT_ITERATION 5   /* one whole Newton iteration */
-int pmcmc_ctx_set_profiling(pmcmc_ctx* ctx, int on);
+int pmcmc_ctx_set_profiling(pmcmc_ctx* ctx, int on); /* on: every kind */
+/* Profile only the kinds in `kinds` (bit k = 1u << PMCMC_KT_k; 0 turns
+ * profiling off).  Each event pair adds a marker to the stream, so timing a
+ * solve with every kind on slows it: a throughput run profiles one kind.   */
+int pmcmc_ctx_set_profiling_kinds(pmcmc_ctx* ctx, unsigned kinds);
 int pmcmc_ctx_kernel_time(pmcmc_ctx* ctx, int kind, int64_t* launches, double* total_ms);
 int pmcmc_ctx_reset_kernel_times(pmcmc_ctx* ctx);
 

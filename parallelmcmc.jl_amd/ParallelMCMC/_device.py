@@ -36,8 +36,16 @@ class Context:
         rc = getattr(_lib.load(), name)(self.h, *args)
         _lib.check(rc, self.h, name)
 
-    def set_profiling(self, on=True):
-        _lib.check(_lib.load().pmcmc_ctx_set_profiling(self.h, 1 if on else 0), self.h)
+    def set_profiling(self, on=True, kinds=None):
+        """Per-kernel event timing of DEER solves: every kind, or only the names
+        in `kinds` (entries of ``_lib.KT_NAMES``)."""
+        if kinds is None or not on:
+            _lib.check(_lib.load().pmcmc_ctx_set_profiling(self.h, 1 if on else 0), self.h)
+            return
+        mask = 0
+        for name in kinds:
+            mask |= 1 << _lib.KT_NAMES.index(name)
+        _lib.check(_lib.load().pmcmc_ctx_set_profiling_kinds(self.h, mask), self.h)
 
     def reset_kernel_times(self):
         _lib.check(_lib.load().pmcmc_ctx_reset_kernel_times(self.h), self.h)

@@ -102,6 +102,7 @@ _SIGS = {
     "pmcmc_last_error": ([_P], C.c_char_p),
     "pmcmc_ctx_synchronize": ([_P], C.c_int),
     "pmcmc_ctx_set_profiling": ([_P, C.c_int], C.c_int),
+    "pmcmc_ctx_set_profiling_kinds": ([_P, C.c_uint], C.c_int),
     "pmcmc_ctx_kernel_time": ([_P, C.c_int, C.POINTER(_I64), C.POINTER(C.c_double)], C.c_int),
     "pmcmc_ctx_reset_kernel_times": ([_P], C.c_int),
     "pmcmc_buffer_alloc": ([_P, _I64, C.POINTER(_P)], C.c_int),

@@ -66,6 +66,7 @@ enum { KT_GLM_PASS = 0, KT_TILE_REDUCE = 1, KT_TILE_APPLY = 2, KT_GLM_REDUCE = 3
 
 struct Prof {
     bool on = false;
+    unsigned kinds = ~0u;  // bit k: record pairs of kind k
     struct Pair {
         hipEvent_t a, b;
         int kind, iter;
@@ -87,7 +88,7 @@ struct Prof {
     }
     // returns the index of the open pair, or -1
     int begin(int kind, int iter, hipStream_t s) {
-        if (!on) return -1;
+        if (!on || !((kinds >> kind) & 1u)) return -1;
         Pair p{get(), get(), kind, iter};
         (void)hipEventRecord(p.a, s);
         pending.push_back(p);

@@ -488,6 +488,15 @@ extern "C" int pmcmc_ctx_synchronize(pmcmc_ctx* c) {
 extern "C" int pmcmc_ctx_set_profiling(pmcmc_ctx* c, int on) {
     if (!c) return PMCMC_ERR_ARGUMENT;
     c->prof.on = on != 0;
+    c->prof.kinds = ~0u;
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_ctx_set_profiling_kinds(pmcmc_ctx* c, unsigned kinds) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    if (kinds >> KT_COUNT) return fail(c, PMCMC_ERR_ARGUMENT, "bad kernel-time kind mask 0x%x", kinds);
+    c->prof.on = kinds != 0;
+    c->prof.kinds = kinds;
     return PMCMC_OK;
 }
 
