@@ -53,7 +53,14 @@ struct SyncState {
     int conv_iter;                 // that iteration (0-based)
     double last_metric;
     int last_iter;
+    // Final prefix of a structured (full-Jacobian) solve: columns below
+    // skip_cols had residual FT - S exactly 0, so they sit on the fixed point in
+    // both iterate buffers and later iterations leave them out.  live_from
+    // collects this iteration's lowest tile start with a nonzero residual.
+    int live_from;
+    int skip_cols;
     int pad;
+    double smax_prefix;  // max |S| over final columns that the current tiling no longer covers
 };
 
 // ---------------------------------------------------------------------------

@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 
 #include "../../include/pmcmc_amd.h"
@@ -60,6 +61,7 @@ struct FullArgs {
     const float* L32;   // Dp x Dp row-major, zero padded
     const double* L64;  // D x D column-major
     int64_t D, Dp, Tn;
+    int64_t col0;  // first column of tile 0 (the columns before it are final)
     int TC, nT;
     double eps, damping, tol_abs, tol_rel;
     int sigmoid;
@@ -73,8 +75,26 @@ struct FullArgs {
     T* S_out;
     uint8_t* acc_out;
     SyncState* sy;
+    double* tile_smax;  // max |S| of each tile when it last ran (the final value once it is skipped)
     int iter;
 };
+
+// First column of this iteration's work: columns below it are final (their
+// residual FT - S was exactly 0, so both iterate buffers hold the fixed point
+// there).  A multiple of TC.
+__device__ inline int64_t live_col(const SyncState* sy) { return (int64_t)*(volatile const int*)&sy->skip_cols; }
+
+// A reduce workgroup whose tile is final: the spine then carries Delta = 0
+// through it (its map and its Delta from a zero start are written as 0).
+template <typename T>
+__device__ inline bool final_tile_zero_map(const FullArgs<T>& p) {
+    const int64_t t0 = p.col0 + (int64_t)blockIdx.x * p.TC;
+    if (t0 + p.TC > live_col(p.sy)) return false;
+    T* Mo = p.Magg + (size_t)blockIdx.x * FD * FD;
+    for (int e = threadIdx.x; e < FD * FD; e += blockDim.x) Mo[e] = (T)0;
+    if (threadIdx.x < FD) p.ragg[(size_t)blockIdx.x * FD + threadIdx.x] = (T)0;
+    return true;
+}
 
 __device__ inline float rlane(float v, int k) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
@@ -107,14 +127,16 @@ __global__ __launch_bounds__(256) void full_prep_kernel(FullArgs<T> p) {
     if (converged(p.sy)) return;
     const int lane = threadIdx.x & 63;
     const int64_t D = p.D;
-    T lrow[FD];
+    T lrow[FD];  // only the surrogate's c needs Lambda
+    if (p.sigmoid) {
 #pragma unroll
-    for (int k = 0; k < FD; ++k) lrow[k] = lam<T>(p, lane, k);
+        for (int k = 0; k < FD; ++k) lrow[k] = lam<T>(p, lane, k);
+    }
     const bool in = lane < D;
     const double eps = p.eps;
     const double c_norm = ((double)D / 2.0) * log(4.0 * 3.14159265358979323846 * eps);
     const int64_t nw = (int64_t)gridDim.x * 4;
-    for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < p.Tn; t += nw) {
+    for (int64_t t = live_col(p.sy) + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < p.Tn; t += nw) {
         const size_t gi = (size_t)t * D + lane;
         T xv = 0, yv = 0, g0 = 0, g1 = 0, sv = 0;
         if (in) {
@@ -202,11 +224,12 @@ __global__ __launch_bounds__(256) void full_reduce_mfma_kernel(FullArgs<float> p
     __shared__ __attribute__((aligned(16))) float sr[2][FCH][FD];
     __shared__ float sa[2][FCH];
     if (converged(p.sy)) return;
+    if (final_tile_zero_map(p)) return;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int li = lane & 15, lk = lane >> 4;
     const int64_t D = p.D;
-    const int64_t t0 = (int64_t)blockIdx.x * p.TC;
+    const int64_t t0 = p.col0 + (int64_t)blockIdx.x * p.TC;
     const int tc = (int)((p.Tn - t0) < p.TC ? (p.Tn - t0) : p.TC);
     const float epsf = (float)p.eps;
     const int ib = 16 * wave + 4 * lk;  // first owned row
@@ -410,10 +433,11 @@ __global__ __launch_bounds__(256) void full_reduce_f64_kernel(FullArgs<double> p
     __shared__ double part[4][FD];
     __shared__ double uvec[FD];
     if (converged(p.sy)) return;
+    if (final_tile_zero_map(p)) return;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t D = p.D;
-    const int64_t t0 = (int64_t)blockIdx.x * p.TC;
+    const int64_t t0 = p.col0 + (int64_t)blockIdx.x * p.TC;
     const int tc = (int)((p.Tn - t0) < p.TC ? (p.Tn - t0) : p.TC);
     const double eps = p.eps;
     double lrow[FD];
@@ -486,14 +510,62 @@ __global__ __launch_bounds__(256) void full_reduce_f64_kernel(FullArgs<double> p
 // ---------------------------------------------------------------------------
 static constexpr int APF = 8;
 
+// End of an apply wave: maxima over the wave, the tile's cached max |S| and
+// its residual report (live tiles), the iteration's maxima; the last wave
+// forms the metric and the next iteration's final prefix.
+template <typename T>
+__device__ inline void full_apply_finish(const FullArgs<T>& p, double dmax, double smax, bool nz, bool final_tile) {
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        dmax = nan_max(dmax, __shfl_xor(dmax, m, 64));
+        smax = nan_max(smax, __shfl_xor(smax, m, 64));
+    }
+    const bool any_nz = __any(nz ? 1 : 0) != 0;
+    if (lane == 0) {
+        SyncState* sy = p.sy;
+        if (!final_tile) {
+            p.tile_smax[blockIdx.x] = smax;
+            if (any_nz) atomicMin(&sy->live_from, (int)(p.col0 + (int64_t)blockIdx.x * p.TC));
+        }
+        atomicMax(&sy->dmax_bits, (unsigned long long)__double_as_longlong(dmax));
+        atomicMax(&sy->smax_bits, (unsigned long long)__double_as_longlong(smax));
+        const unsigned prev = __hip_atomic_fetch_add(&sy->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == (unsigned)p.nT - 1u) {
+            const double dm = __longlong_as_double((long long)atomicAdd(&sy->dmax_bits, 0ull));
+            const double sm = nan_max(__longlong_as_double((long long)atomicAdd(&sy->smax_bits, 0ull)),
+                                      sy->smax_prefix);
+            // src/DEER/DEER.jl:567-577
+            const double metric = dm / (p.tol_abs + p.tol_rel * sm);
+            sy->last_metric = metric;
+            sy->last_iter = p.iter;
+            if (metric <= 1.0) {
+                sy->conv_iter = p.iter;
+                sy->converged = 1;
+            }
+            // tiles before the first one with a nonzero residual are final
+            const int lf = __hip_atomic_load(&sy->live_from, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            sy->skip_cols = lf == INT_MAX ? (int)(p.col0 + (int64_t)p.nT * p.TC) : lf;
+            sy->live_from = INT_MAX;
+            sy->dmax_bits = 0ull;
+            sy->smax_bits = 0ull;
+            sy->done = 0u;
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(64) void full_apply_kernel(FullArgs<T> p) {
     if (converged(p.sy)) return;
     const int lane = threadIdx.x;
     const int64_t D = p.D;
     const int64_t tile = blockIdx.x;
-    const int64_t t0 = tile * p.TC;
+    const int64_t t0 = p.col0 + tile * p.TC;
     const int tc = (int)((p.Tn - t0) < p.TC ? (p.Tn - t0) : p.TC);
+    if (t0 + p.TC <= live_col(p.sy)) {  // final tile: S_out = S_in there already
+        full_apply_finish<T>(p, 0.0, lane == 0 ? p.tile_smax[tile] : 0.0, false, true);
+        return;
+    }
     T lrow[FD];
 #pragma unroll
     for (int k = 0; k < FD; ++k) lrow[k] = lam<T>(p, lane, k);
@@ -502,6 +574,7 @@ __global__ __launch_bounds__(64) void full_apply_kernel(FullArgs<T> p) {
     const T epsT = (T)p.eps, damp = (T)p.damping;
     T dv = tile == 0 ? (T)0 : p.dend[(size_t)(tile - 1) * FD + lane];
     double dmax = 0.0, smax = 0.0;
+    bool nz = false;  // a nonzero residual in this tile
     T ru[APF], rc[APF], rr[APF], rs[APF], ra[APF];
     auto fetch = [&](int s, int sl) {
         const size_t gi = (size_t)(t0 + s) * D + lane;
@@ -545,6 +618,7 @@ __global__ __launch_bounds__(64) void full_apply_kernel(FullArgs<T> p) {
                     nd += u * cd;
                 }
                 dv = nd;
+                nz = nz || rho != (T)0;  // NaN counts as nonzero
                 if (in) {
                     const T nv = old + damp * dv;
                     p.S_out[(size_t)(t0 + s) * D + lane] = nv;
@@ -554,32 +628,7 @@ __global__ __launch_bounds__(64) void full_apply_kernel(FullArgs<T> p) {
             }
         }
     }
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        dmax = nan_max(dmax, __shfl_xor(dmax, m, 64));
-        smax = nan_max(smax, __shfl_xor(smax, m, 64));
-    }
-    if (lane == 0) {
-        SyncState* sy = p.sy;
-        atomicMax(&sy->dmax_bits, (unsigned long long)__double_as_longlong(dmax));
-        atomicMax(&sy->smax_bits, (unsigned long long)__double_as_longlong(smax));
-        const unsigned prev = __hip_atomic_fetch_add(&sy->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == (unsigned)p.nT - 1u) {
-            const double dm = __longlong_as_double((long long)atomicAdd(&sy->dmax_bits, 0ull));
-            const double sm = __longlong_as_double((long long)atomicAdd(&sy->smax_bits, 0ull));
-            // src/DEER/DEER.jl:567-577
-            const double metric = dm / (p.tol_abs + p.tol_rel * sm);
-            sy->last_metric = metric;
-            sy->last_iter = p.iter;
-            if (metric <= 1.0) {
-                sy->conv_iter = p.iter;
-                sy->converged = 1;
-            }
-            sy->dmax_bits = 0ull;
-            sy->smax_bits = 0ull;
-            sy->done = 0u;
-        }
-    }
+    full_apply_finish<T>(p, dmax, smax, nz, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -605,6 +654,7 @@ int full_work_ensure(FullWork* w, int64_t D, int64_t T, bool f64, std::string* m
               hipMalloc(&w->ragg, (size_t)w->nT * FD * esz) == hipSuccess &&
               hipMalloc(&w->dend, (size_t)w->nT * FD * esz) == hipSuccess &&
               hipMalloc(&w->spine, std::max<size_t>(scan_full_spine_scratch_bytes(w->nT, f64), 256)) == hipSuccess &&
+              hipMalloc(&w->tile_smax, (size_t)w->nT * sizeof(double)) == hipSuccess &&
               // the spine writes rows < D only; the apply reads all 64 lanes
               hipMemset(w->dend, 0, (size_t)w->nT * FD * esz) == hipSuccess;
     if (!ok) {
@@ -613,6 +663,48 @@ int full_work_ensure(FullWork* w, int64_t D, int64_t T, bool f64, std::string* m
         return PMCMC_ERR_ALLOC;
     }
     return PMCMC_OK;
+}
+
+// max |S| of the tiles of the tiling (col0, TC, nT) that end at or before lo,
+// folded into sy->smax_prefix; one wave
+__global__ __launch_bounds__(64) void full_fold_prefix_kernel(SyncState* sy, const double* tile_smax, int64_t col0,
+                                                               int TC, int nT, int64_t lo) {
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nT; b += 64)
+        if (col0 + (int64_t)(b + 1) * TC <= lo) m = nan_max(m, tile_smax[b]);
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1) m = nan_max(m, __shfl_xor(m, k, 64));
+    if (threadIdx.x == 0) sy->smax_prefix = nan_max(sy->smax_prefix, m);
+}
+
+void full_geometry_reset(FullWork* w) {
+    w->col0 = 0;
+    w->TCc = w->TC;
+    w->nTc = w->nT;
+}
+
+void full_retile(FullWork* w, SyncState* sync, int64_t lo, hipStream_t s) {
+    if (lo <= w->col0 || lo >= w->T) return;
+    hipLaunchKernelGGL(full_fold_prefix_kernel, dim3(1), dim3(64), 0, s, sync, (const double*)w->tile_smax, w->col0,
+                       w->TCc, w->nTc, lo);
+    const int64_t live = w->T - lo;
+    // per-iteration latency model of the tile kernels (measured at D = 64): the
+    // reduce and apply chains cost about 2.5 us per column of a tile, the scan
+    // over the tile maps about 5 us up to 8 maps, 35 us up to 64, 70 us beyond
+    int best = w->TC;
+    double best_us = 1e30;
+    for (int tc = 8; tc <= w->TC; tc *= 2) {
+        const int64_t n = (live + tc - 1) / tc;
+        if (n > w->nT) continue;
+        const double us = 2.5 * tc + (n <= 8 ? 5.0 : n <= 64 ? 35.0 : 70.0);
+        if (us < best_us) {
+            best_us = us;
+            best = tc;
+        }
+    }
+    w->col0 = lo;
+    w->TCc = best;
+    w->nTc = (int)((live + best - 1) / best);
 }
 
 void full_work_destroy(FullWork* w) {
@@ -624,6 +716,7 @@ void full_work_destroy(FullWork* w) {
     (void)hipFree(w->ragg);
     (void)hipFree(w->dend);
     (void)hipFree(w->spine);
+    (void)hipFree(w->tile_smax);
     *w = FullWork{};
 }
 
@@ -646,8 +739,9 @@ int full_newton_step(FullWork* fw, const GlmWork* gw, const GaussData* g, hipStr
     p.D = fw->D;
     p.Dp = g->Dp;
     p.Tn = fw->T;
-    p.TC = fw->TC;
-    p.nT = fw->nT;
+    p.col0 = fw->col0;
+    p.TC = fw->TCc;
+    p.nT = fw->nTc;
     p.eps = eps;
     p.damping = damping;
     p.tol_abs = tol_abs;
@@ -663,15 +757,16 @@ int full_newton_step(FullWork* fw, const GlmWork* gw, const GaussData* g, hipStr
     p.S_out = S_out;
     p.acc_out = acc_out;
     p.sy = sync;
+    p.tile_smax = fw->tile_smax;
     p.iter = iter;
     int64_t pg = (fw->T + 3) / 4;
     if (pg > 2048) pg = 2048;
     int e = prof ? prof->begin(KT_TILE_REDUCE, iter, s) : -1;
     hipLaunchKernelGGL((full_prep_kernel<T>), dim3((unsigned)pg), dim3(256), 0, s, p);
     if constexpr (sizeof(T) == 8) {
-        hipLaunchKernelGGL(full_reduce_f64_kernel, dim3((unsigned)fw->nT), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(full_reduce_f64_kernel, dim3((unsigned)fw->nTc), dim3(256), 0, s, p);
     } else {
-        const dim3 g((unsigned)fw->nT), blk(256);
+        const dim3 g((unsigned)fw->nTc), blk(256);
         if (precision == PMCMC_PREC_BF16_JAC) {
             if (sigmoid)
                 hipLaunchKernelGGL((full_reduce_mfma_kernel<1, true>), g, blk, 0, s, p);
@@ -686,10 +781,10 @@ int full_newton_step(FullWork* fw, const GlmWork* gw, const GaussData* g, hipStr
     }
     if (prof) prof->end(e, s);
     e = prof ? prof->begin(KT_SCAN_SPINE, iter, s) : -1;
-    scan_full_spine<T>(p.Magg, p.ragg, p.dend, fw->D, fw->nT, &sync->converged, fw->spine, s);
+    scan_full_spine<T>(p.Magg, p.ragg, p.dend, fw->D, fw->nTc, &sync->converged, fw->spine, s);
     if (prof) prof->end(e, s);
     e = prof ? prof->begin(KT_TILE_APPLY, iter, s) : -1;
-    hipLaunchKernelGGL((full_apply_kernel<T>), dim3((unsigned)fw->nT), dim3(64), 0, s, p);
+    hipLaunchKernelGGL((full_apply_kernel<T>), dim3((unsigned)fw->nTc), dim3(64), 0, s, p);
     if (prof) prof->end(e, s);
     if (hipGetLastError() != hipSuccess) {
         *msg = "full-Jacobian kernels failed to launch";

@@ -115,6 +115,7 @@ __device__ __forceinline__ void gauss_pass_body(const ColSrc<float>& src, const 
     __shared__ double lpred[4][NC];
 
     if (skip && *(volatile const int*)skip) return;
+    if (src.col_lo && (cb + 1) * CPB <= (int64_t)*src.col_lo) return;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
@@ -407,7 +408,7 @@ template <typename T>
 int gauss_newton_products(GlmWork* w, const GaussData* g, hipStream_t s, SyncState* sync,
                           int precision, double eps, const T* x0, const T* S_in, const T* Xi,
                           uint64_t seed, int iter, uint32_t probe_iter, int probes, int64_t D, int64_t Tn,
-                          Prof* prof, std::string* msg) {
+                          Prof* prof, std::string* msg, const int* col_lo) {
     const bool jbf = precision == PMCMC_PREC_BF16_JAC;
     const int* skip = sync ? &sync->converged : nullptr;
     ColSrc<T> a{};
@@ -417,6 +418,7 @@ int gauss_newton_products(GlmWork* w, const GaussData* g, hipStream_t s, SyncSta
     a.K = probes;
     a.D = D;
     a.Tn = Tn;
+    a.col_lo = col_lo;
     for (int k = 0; k < probes && k < 4; ++k) a.keys[k] = probe_key(seed, probe_iter, (uint32_t)k);
     int rc = gauss_pass<T>(g, s, a, 0, eps, Xi, (T*)w->gx, (T*)w->hw, w->lpx, (T*)w->Y, (T*)w->W, skip, prof,
                            iter, jbf, msg);
@@ -428,15 +430,16 @@ int gauss_newton_products(GlmWork* w, const GaussData* g, hipStream_t s, SyncSta
     b.K = probes;
     b.D = D;
     b.Tn = Tn;
+    b.col_lo = col_lo;
     return gauss_pass<T>(g, s, b, 1, eps, Xi, (T*)w->gy, (T*)w->hw, w->lpy, nullptr, nullptr, skip, prof, iter,
                          jbf, msg);
 }
 template int gauss_newton_products<float>(GlmWork*, const GaussData*, hipStream_t, SyncState*, int, double,
                                           const float*, const float*, const float*, uint64_t, int, uint32_t, int,
-                                          int64_t, int64_t, Prof*, std::string*);
+                                          int64_t, int64_t, Prof*, std::string*, const int*);
 template int gauss_newton_products<double>(GlmWork*, const GaussData*, hipStream_t, SyncState*, int, double,
                                            const double*, const double*, const double*, uint64_t, int, uint32_t,
-                                           int, int64_t, int64_t, Prof*, std::string*);
+                                           int, int64_t, int64_t, Prof*, std::string*, const int*);
 
 template <int DPT>
 static void launch_gauss_batched_dpt(bool jbf, const GaussPassArgs* args, const int2* items, int nitems,

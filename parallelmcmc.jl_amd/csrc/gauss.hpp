@@ -48,12 +48,14 @@ int gauss_data_create(GaussData* g, hipStream_t s, int64_t D, bool src_f64, cons
 void gauss_data_destroy(GaussData* g);
 
 // Same products and buffers as glm_newton_products (GlmWork): gx, W, Y, lpx
-// at Xbar; gy, hw, lpy at Y.
+// at Xbar; gy, hw, lpy at Y.  col_lo (device, may be null): the FP32 MFMA
+// passes leave out column blocks wholly below *col_lo (their outputs keep
+// older values).
 template <typename T>
 int gauss_newton_products(GlmWork* w, const GaussData* g, hipStream_t s, SyncState* sync,
                           int precision, double eps, const T* x0, const T* S_in, const T* Xi,
                           uint64_t seed, int iter, uint32_t probe_iter, int probes, int64_t D, int64_t Tn,
-                          Prof* prof, std::string* msg);
+                          Prof* prof, std::string* msg, const int* col_lo = nullptr);
 
 // As glm_dir_products: w->gx = grad(states), w->W = dir + eps H dir.
 template <typename T>

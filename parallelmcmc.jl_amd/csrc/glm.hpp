@@ -35,6 +35,7 @@ struct ColSrc {
     uint32_t keys[4];
     int K;
     int64_t D, Tn;
+    const int* col_lo;  // device; column blocks wholly below *col_lo are left out (null: none)
 
     __device__ inline double get(int set, int64_t t, int64_t d) const {
         if (set == 0) {

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -150,6 +151,9 @@ __global__ void reset_sync_kernel(SyncState* s) {
         s->conv_iter = -1;
         s->last_metric = INFINITY;
         s->last_iter = -1;
+        s->live_from = INT_MAX;
+        s->skip_cols = 0;
+        s->smax_prefix = 0.0;
     }
 }
 
@@ -1181,6 +1185,7 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
     }
     T* bufs[2] = {(T*)w->buf[0], (T*)w->buf[1]};
     hipLaunchKernelGGL(reset_sync_kernel, dim3(1), dim3(64), 0, c->stream, w->sync);
+    if (full) full_geometry_reset(&w->fullw);
     hipLaunchKernelGGL((init_iterate_kernel<T>), dim3(grid_for(D * Tn)), dim3(256), 0, c->stream,
                        bufs[0], (const T*)x0, (const T*)init, D, D * Tn);
     HIPCHK(c, hipGetLastError());
@@ -1221,9 +1226,10 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
             const int eit = c->prof.begin(KT_ITERATION, it, c->stream);
             if (full) {
                 std::string msg;
+                // columns below sync->skip_cols are final (see full_newton_step)
                 int rc = gauss_newton_products(&w->glmw, &tgt->gauss, c->stream, w->sync, p->precision, p->eps,
                                                (const T*)x0, a.S_in, (const T*)Xi, p->seed, it, a.probe_iter, 0,
-                                               D, Tn, &c->prof, &msg);
+                                               D, Tn, &c->prof, &msg, &w->sync->skip_cols);
                 if (rc == PMCMC_OK)
                     rc = full_newton_step<T>(&w->fullw, &w->glmw, &tgt->gauss, c->stream, w->sync, p->precision,
                                              p->jacobian == PMCMC_JAC_FULL_SIGMOID, p->eps, p->damping, p->tol_abs, p->tol_rel, (const T*)x0, a.S_in, a.S,
@@ -1285,6 +1291,8 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
         HIPCHK(c, hipStreamSynchronize(c->stream));
         done = c->host_sync->converged != 0;
         if (chunk < 16) chunk <<= 1;
+        // full Jacobian: tile only the columns that are not final yet
+        if (full && !done) full_retile(&w->fullw, w->sync, c->host_sync->skip_cols, c->stream);
     }
     const SyncState hs = *c->host_sync;
     const int iters = hs.converged ? hs.conv_iter + 1 : (hs.last_iter + 1);

@@ -129,3 +129,31 @@ def test_full_deer_f32_bf16_d64(prec):
     print(f"{prec}: stoch_diag iters {info_sd.iters}", flush=True)
     assert info.iters * 3 < info_sd.iters
 
+
+
+@pytest.mark.parametrize("prec,D,T", [("f64", 16, 2000), ("f32", 64, 4096), ("bf16_jac", 64, 4096)])
+def test_full_deer_final_prefix_at_the_stability_edge(prec, D, T):
+    """eps * lambda_max = 1.9 on a condition-1e3 precision (eigenvalues
+    10^U(0, 3)): low acceptance, so
+    Newton advances a few columns per iteration and the solve leaves out the
+    tiles whose residual is exactly 0 (the final prefix).  The result is still
+    sequential MALA: FP64 within 1e-10, FP32 / BF16 within 1e-5 relative,
+    every decision identical."""
+    from ParallelMCMC import DEER
+
+    rng, tgt, model, ev = _setup(41 + D, D, cond=3.0, with_mean=False)
+    eps = 1.9 / ev.max()
+    x0 = rng.standard_normal(D)
+    dt = np.float64 if prec == "f64" else np.float32
+    Xi, U = O.make_tape(rng, D, T, dt)
+    ref, acc_ref, _ = O.run_mala_sequential_taped(tgt, x0, eps, Xi.astype(np.float64), U.astype(np.float64))
+    rec = DEER.TapedRecursion(model, eps, Xi, U, precision=prec)
+    tol = (1e-12, 1e-11) if prec == "f64" else (1e-6, 1e-6)
+    S, info = DEER.solve(rec, x0.astype(dt), tol_abs=tol[0], tol_rel=tol[1], maxiter=T + 1, jacobian="full",
+                         damping=1.0, return_info=True)
+    err = O.trace_error(S.cpu().numpy().astype(np.float64), ref)
+    print(f"{prec} D={D} T={T}: iters {info.iters}, acceptance {acc_ref.mean():.2f}, trace error {err:.3e}",
+          flush=True)
+    assert info.converged, info
+    assert err < (1e-10 if prec == "f64" else 1e-5)
+    assert np.array_equal(info.accepted.cpu().numpy().astype(bool), acc_ref)
