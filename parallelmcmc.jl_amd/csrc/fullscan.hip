@@ -556,6 +556,7 @@ __device__ inline void full_apply_finish(const FullArgs<T>& p, double dmax, doub
 
 template <typename T>
 __global__ __launch_bounds__(64) void full_apply_kernel(FullArgs<T> p) {
+    __shared__ __attribute__((aligned(16))) T dvs[FD];  // Delta of the previous step, for the broadcast reads
     if (converged(p.sy)) return;
     const int lane = threadIdx.x;
     const int64_t D = p.D;
@@ -602,14 +603,22 @@ __global__ __launch_bounds__(64) void full_apply_kernel(FullArgs<T> p) {
             if (s < tc) {
                 const T u = ru[sl], c = rc[sl], rho = rr[sl], old = rs[sl], a = ra[sl];
                 if (s + APF < tc) fetch(s + APF, sl);
+                // Lambda Delta: the wave's Delta through LDS (one write, then
+                // broadcast vector reads), four independent FMA chains.  The
+                // workgroup is one wave and a wave's LDS operations complete in
+                // issue order, so only the compiler's order needs pinning (a
+                // __syncthreads would also wait for the prefetch ring's loads)
+                dvs[lane] = dv;
+                __builtin_amdgcn_wave_barrier();
                 T L0 = 0, L1 = 0, L2 = 0, L3 = 0;
 #pragma unroll
                 for (int k = 0; k < FD; k += 4) {
-                    L0 += lrow[k] * rlane(dv, k);
-                    L1 += lrow[k + 1] * rlane(dv, k + 1);
-                    L2 += lrow[k + 2] * rlane(dv, k + 2);
-                    L3 += lrow[k + 3] * rlane(dv, k + 3);
+                    L0 += lrow[k] * dvs[k];
+                    L1 += lrow[k + 1] * dvs[k + 1];
+                    L2 += lrow[k + 2] * dvs[k + 2];
+                    L3 += lrow[k + 3] * dvs[k + 3];
                 }
+                __builtin_amdgcn_wave_barrier();  // reads before the next step's write
                 T nd = dv - a * epsT * ((L0 + L1) + (L2 + L3)) + rho;
                 if (sig) {
                     T cd = c * dv;
