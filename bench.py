@@ -157,6 +157,17 @@ def bench_scan(args):
     n, ms = kt["iteration"]
     bytes_ = 3 * D_ * T_ * A.element_size()
     kern_s = ms / n / 1e3
+    # streaming reference on this GPU: an elementwise kernel with the scan's
+    # traffic (read two D x T arrays, write one) and no dependence between columns
+    torch.add(A, B, out=S)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        torch.add(A, B, out=S)
+    e1.record()
+    torch.cuda.synchronize()
+    stream_s = e0.elapsed_time(e1) / 20 / 1e3
     print(json.dumps({
         "metric": "diagonal affine scan throughput", "value": bytes_ / kern_s / 1e9, "unit": "GB/s",
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
@@ -165,6 +176,9 @@ def bench_scan(args):
                                               "host_us_per_call": 1e6 * el / args.steps},
         "roofline": {"bound": "hbm", "achieved": bytes_ / kern_s / 1e9, "peak": 8000.0, "unit": "GB/s",
                      "frac": bytes_ / kern_s / 8e12, "traffic": None},
+        "streaming_reference": {"kernel": "torch.add(A, B, out=S): the scan's bytes without its dependence",
+                                "GBps": bytes_ / stream_s / 1e9, "frac_of_peak": bytes_ / stream_s / 8e12,
+                                "scan_over_streaming": stream_s / kern_s},
     }))
 
 
