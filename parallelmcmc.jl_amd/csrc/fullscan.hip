@@ -662,7 +662,9 @@ int full_work_ensure(FullWork* w, int64_t D, int64_t T, bool f64, std::string* m
               hipMalloc(&w->Magg, (size_t)w->nT * FD * FD * esz) == hipSuccess &&
               hipMalloc(&w->ragg, (size_t)w->nT * FD * esz) == hipSuccess &&
               hipMalloc(&w->dend, (size_t)w->nT * FD * esz) == hipSuccess &&
-              hipMalloc(&w->spine, std::max<size_t>(scan_full_spine_scratch_bytes(w->nT, f64), 256)) == hipSuccess &&
+              hipMalloc(&w->spine, scan_full_spine_scratch_bytes(w->nT, f64)) == hipSuccess &&
+              // the fused spine's counter and epoch start at zero
+              hipMemset(w->spine, 0, scan_full_spine_scratch_bytes(w->nT, f64)) == hipSuccess &&
               hipMalloc(&w->tile_smax, (size_t)w->nT * sizeof(double)) == hipSuccess &&
               // the spine writes rows < D only; the apply reads all 64 lanes
               hipMemset(w->dend, 0, (size_t)w->nT * FD * esz) == hipSuccess;
@@ -790,7 +792,7 @@ int full_newton_step(FullWork* fw, const GlmWork* gw, const GaussData* g, hipStr
     }
     if (prof) prof->end(e, s);
     e = prof ? prof->begin(KT_SCAN_SPINE, iter, s) : -1;
-    scan_full_spine<T>(p.Magg, p.ragg, p.dend, fw->D, fw->nTc, &sync->converged, fw->spine, s);
+    scan_full_spine<T>(p.Magg, p.ragg, p.dend, fw->D, fw->nTc, &sync->converged, fw->spine, s, &sync->err);
     if (prof) prof->end(e, s);
     e = prof ? prof->begin(KT_TILE_APPLY, iter, s) : -1;
     hipLaunchKernelGGL((full_apply_kernel<T>), dim3((unsigned)fw->nTc), dim3(64), 0, s, p);

@@ -1295,6 +1295,7 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
         if (full && !done) full_retile(&w->fullw, w->sync, c->host_sync->skip_cols, c->stream);
     }
     const SyncState hs = *c->host_sync;
+    if (hs.err) return fail(c, PMCMC_ERR_HIP, "DEER solve: a device-side wait hit its spin limit");
     const int iters = hs.converged ? hs.conv_iter + 1 : (hs.last_iter + 1);
     T* result = bufs[iters & 1];
     if ((void*)result != S_out)

@@ -59,8 +59,10 @@ struct SFStride {
     static constexpr int v = sizeof(T) == 4 ? 80 : 66;
 };
 
+// Tile `tile`'s map: [M | r] <- J_t [M | r] + [0 | B_t] over its columns,
+// written to Magg / ragg.  Called by all 256 threads of the workgroup.
 template <typename T>
-__global__ __launch_bounds__(256) void sfull_reduce_kernel(SFArgs<T> p) {
+__device__ __forceinline__ void sfull_reduce_tile(const SFArgs<T>& p, int64_t tile) {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int MS = SFStride<T>::v;
     __shared__ __attribute__((aligned(16))) T Ml[2][SF][MS];  // Ml[b][k][j] = M(k, j)
@@ -68,10 +70,8 @@ __global__ __launch_bounds__(256) void sfull_reduce_kernel(SFArgs<T> p) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int64_t D = p.D;
-    const int64_t tile = blockIdx.x;
     const int64_t t0 = tile * p.TC;
     const int tc = (int)std::min<int64_t>(p.TC, p.len - t0);
-    if (p.skip && *(volatile const int*)p.skip) return;
     for (int e = tid; e < SF * SF; e += 256) Ml[0][e >> 6][e & 63] = (e >> 6) == (e & 63) ? (T)1 : (T)0;
     if (tid < SF) rl[0][tid] = 0;
     // lane (li, lk) of wave w holds J_t(i = 16 w + li, k = 4 ks + lk), ks = 0..15,
@@ -124,15 +124,21 @@ __global__ __launch_bounds__(256) void sfull_reduce_kernel(SFArgs<T> p) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void sfull_apply_kernel(SFArgs<T> p) {
+__global__ __launch_bounds__(256) void sfull_reduce_kernel(SFArgs<T> p) {
+    if (p.skip && *(volatile const int*)p.skip) return;
+    sfull_reduce_tile<T>(p, blockIdx.x);
+}
+
+// Tile `tile`'s columns from its incoming state (s0 for tile 0, else
+// prev[tile - 1]), written to S.  Called by all 256 threads of the workgroup.
+template <typename T>
+__device__ __forceinline__ void sfull_apply_tile(const SFArgs<T>& p, int64_t tile) {
     __shared__ T v[SF];
     __shared__ T part[4][SF];
     const int tid = threadIdx.x, i = tid & 63, q = tid >> 6;
     const int64_t D = p.D;
-    const int64_t tile = blockIdx.x;
     const int64_t t0 = tile * p.TC;
     const int tc = (int)std::min<int64_t>(p.TC, p.len - t0);
-    if (p.skip && *(volatile const int*)p.skip) return;
     if (tid < SF)
         v[i] = i >= D ? (T)0 : (tile == 0 ? (p.s0 ? p.s0[i] : (T)0) : p.prev[(size_t)(tile - 1) * SF + i]);
     T jr[SAPF][16];
@@ -176,6 +182,65 @@ __global__ __launch_bounds__(256) void sfull_apply_kernel(SFArgs<T> p) {
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void sfull_apply_kernel(SFArgs<T> p) {
+    if (p.skip && *(volatile const int*)p.skip) return;
+    sfull_apply_tile<T>(p, blockIdx.x);
+}
+
+// The scan over at most FUSE_MAX_GROUPS * 8 tile maps in one launch (the
+// recursive form takes five dependent launches at 256 maps).  Workgroup g
+// composes maps 8g .. 8g + 7 into its group map; the last workgroup to arrive
+// (acquire-release counter) runs the chain over the group maps, writes the
+// group end states and releases them with a new epoch; every other workgroup
+// waits for that epoch (bounded: hitting the cap sets *err), then applies its
+// eight maps from its group's incoming state.  All groups are resident (at
+// most FUSE_MAX_GROUPS workgroups), so the wait ends.  ss: {count, epoch},
+// zeroed when the scratch is allocated; the counter is reset by the last
+// workgroup, the epoch only grows.
+static constexpr int FUSE_MAX_GROUPS = 32;
+static constexpr unsigned FUSE_SPIN_CAP = 1u << 21;
+
+template <typename T>
+__global__ __launch_bounds__(256) void sfull_spine_fused_kernel(SFArgs<T> lvl, SFArgs<T> top, unsigned* ss,
+                                                                 unsigned* err) {
+    if (lvl.skip && *(volatile const int*)lvl.skip) return;
+    __shared__ unsigned e_start;
+    __shared__ int is_last;
+    if (threadIdx.x == 0) e_start = __hip_atomic_load(&ss[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sfull_reduce_tile<T>(lvl, blockIdx.x);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's map stores are done
+    __syncthreads();  // ... for every wave, before thread 0's release below
+    if (threadIdx.x == 0) {
+        const unsigned prev = __hip_atomic_fetch_add(&ss[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = prev == gridDim.x - 1u;
+    }
+    __syncthreads();
+    const unsigned e = e_start;
+    if (is_last) {
+        sfull_apply_tile<T>(top, 0);  // group end states
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            ss[0] = 0u;
+            __hip_atomic_store(&ss[1], e + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    } else {
+        if (threadIdx.x == 0) {
+            unsigned spins = 0;
+            while (__hip_atomic_load(&ss[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == e) {
+                if (++spins > FUSE_SPIN_CAP) {
+                    atomicOr(err, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();
+    }
+    sfull_apply_tile<T>(lvl, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -195,8 +260,10 @@ static size_t sfull_bytes_t(int64_t T_len, int level0 = 0) {
 
 size_t scan_full_scratch_bytes(int64_t T, bool f64) { return f64 ? sfull_bytes_t<double>(T) : sfull_bytes_t<float>(T); }
 
+static constexpr size_t SPINE_SYNC_BYTES = 256;  // the fused launch's counter and epoch, then the maps
+
 size_t scan_full_spine_scratch_bytes(int64_t nT, bool f64) {
-    return f64 ? sfull_bytes_t<double>(nT, 1) : sfull_bytes_t<float>(nT, 1);
+    return SPINE_SYNC_BYTES + (f64 ? sfull_bytes_t<double>(nT, 1) : sfull_bytes_t<float>(nT, 1));
 }
 
 template <typename T>
@@ -254,13 +321,55 @@ int scan_full(bool f64, const void* J, const void* B, const void* s0, void* S, i
 
 template <typename T>
 void scan_full_spine(const T* M, const T* r, T* out, int64_t D, int64_t nT, const int* skip, void* scratch,
-                     hipStream_t st) {
-    if (nT > 0) sfull_rec<T>(1, M, SF, SF * SF, r, SF, nullptr, out, SF, D, nT, (char*)scratch, st, skip);
+                     hipStream_t st, unsigned* err) {
+    if (nT <= 0) return;
+    unsigned* ss = reinterpret_cast<unsigned*>(scratch);  // fused launch: {count, epoch}
+    char* maps = (char*)scratch + SPINE_SYNC_BYTES;
+    const int64_t G = (nT + level_tc(1) - 1) / level_tc(1);
+    if (nT <= level_tc(1) || G > FUSE_MAX_GROUPS || !err) {
+        sfull_rec<T>(1, M, SF, SF * SF, r, SF, nullptr, out, SF, D, nT, maps, st, skip);
+        return;
+    }
+    // the level-1 scratch layout of sfull_rec: group maps, group r, group end states
+    T* Mg = reinterpret_cast<T*>(maps);
+    T* rg = Mg + (size_t)G * SF * SF;
+    T* eg = rg + (size_t)G * SF;
+    SFArgs<T> lvl{};
+    lvl.J = M;
+    lvl.ldJ = SF;
+    lvl.strideJ = SF * SF;
+    lvl.B = r;
+    lvl.ldB = SF;
+    lvl.s0 = nullptr;
+    lvl.prev = eg;
+    lvl.S = out;
+    lvl.ldS = SF;
+    lvl.D = D;
+    lvl.len = nT;
+    lvl.TC = level_tc(1);
+    lvl.nT = (int)G;
+    lvl.Magg = Mg;
+    lvl.ragg = rg;
+    lvl.skip = skip;
+    SFArgs<T> top{};
+    top.J = Mg;
+    top.ldJ = SF;
+    top.strideJ = SF * SF;
+    top.B = rg;
+    top.ldB = SF;
+    top.s0 = nullptr;
+    top.S = eg;
+    top.ldS = SF;
+    top.D = D;
+    top.len = G;
+    top.TC = (int)G;
+    top.nT = 1;
+    hipLaunchKernelGGL(sfull_spine_fused_kernel<T>, dim3((unsigned)G), dim3(256), 0, st, lvl, top, ss, err);
 }
 
 template void scan_full_spine<float>(const float*, const float*, float*, int64_t, int64_t, const int*, void*,
-                                     hipStream_t);
+                                     hipStream_t, unsigned*);
 template void scan_full_spine<double>(const double*, const double*, double*, int64_t, int64_t, const int*, void*,
-                                      hipStream_t);
+                                      hipStream_t, unsigned*);
 
 }  // namespace pmcmc
