@@ -107,6 +107,8 @@ struct pmcmc_deer_ws {
     FullWork fullw;  // jacobian = :full
     RefineWork refw;  // FP64 decision overrides (GEMM targets, FP32)
     int8_t* prev_dec = nullptr;  // previous iteration's decisions (FP32 hysteresis)
+    double* tile_smax = nullptr;  // max |S| per level-0 tile when it last ran (final prefix)
+    int tile_smax_n = 0;
     // adjoint buffers (allocated on first use)
     void* adj_lam = nullptr;   // D x (T+1), last column zero
     void* adj_A = nullptr;     // reversed-time scan coefficients, D x T
@@ -1073,6 +1075,7 @@ extern "C" int pmcmc_deer_ws_destroy(pmcmc_deer_ws* w) {
     full_work_destroy(&w->fullw);
     refine_work_destroy(&w->refw);
     hipFree(w->prev_dec);
+    hipFree(w->tile_smax);
     hipFree(w->adj_lam);
     hipFree(w->adj_A);
     hipFree(w->adj_B);
@@ -1106,6 +1109,10 @@ static void launch_deer_iteration(pmcmc_deer_ws* w, TileArgs<T> a, hipStream_t s
     }
     a.agg_a = (T*)(blob + L.off_a);
     a.agg_b = (T*)(blob + L.off_b);
+    if (w->tile_smax && w->tile_smax_n >= L.nT) {
+        a.final_cols = &w->sync->skip_cols;
+        a.tile_smax = w->tile_smax;
+    }
     if (MODE == MODE_GEMM) {
         // the reduce pass keeps A and B in the probe-product buffers it has
         // consumed (rebuilt by the target kernels every iteration)
@@ -1183,6 +1190,13 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
         if (!w->prev_dec) HIPCHK(c, hipMalloc(&w->prev_dec, (size_t)Tn));
         hipLaunchKernelGGL(fill_i8_kernel, dim3(grid_for(Tn)), dim3(256), 0, c->stream, w->prev_dec, (int8_t)-1, Tn);
     }
+    if (!full && w->plan.lev[0].nT > w->tile_smax_n) {
+        hipFree(w->tile_smax);
+        w->tile_smax = nullptr;
+        w->tile_smax_n = 0;
+        HIPCHK(c, hipMalloc(&w->tile_smax, (size_t)w->plan.lev[0].nT * sizeof(double)));
+        w->tile_smax_n = w->plan.lev[0].nT;
+    }
     T* bufs[2] = {(T*)w->buf[0], (T*)w->buf[1]};
     hipLaunchKernelGGL(reset_sync_kernel, dim3(1), dim3(64), 0, c->stream, w->sync);
     if (full) full_geometry_reset(&w->fullw);
@@ -1240,7 +1254,8 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
                 int rc = tgt->kind == TGT_GAUSS
                              ? gauss_newton_products(&w->glmw, &tgt->gauss, c->stream, w->sync, p->precision,
                                                      p->eps, (const T*)x0, a.S_in, (const T*)Xi, p->seed, it,
-                                                     a.probe_iter, p->probes, D, Tn, &c->prof, &msg)
+                                                     a.probe_iter, p->probes, D, Tn, &c->prof, &msg,
+                                                     &w->sync->skip_cols)
                              : glm_newton_products(&w->glmw, &tgt->glm, c->stream, w->sync, p->precision,
                                                    p->eps, tgt->prior_scale, (const T*)x0, a.S_in,
                                                    (const T*)Xi, p->seed, it, a.probe_iter, p->probes, D, Tn,
@@ -1787,6 +1802,9 @@ __global__ __launch_bounds__(256) void batch_init_kernel(const BInit* t) {
         s->conv_iter = -1;
         s->last_metric = INFINITY;
         s->last_iter = -1;
+        s->live_from = INT_MAX;
+        s->skip_cols = 0;
+        s->smax_prefix = 0.0;
     }
     if (blockIdx.x == 0 && threadIdx.x < 4 && a.rcnt) a.rcnt[threadIdx.x] = 0;
     for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256) a.S0[i] = a.x0[i % a.D];
@@ -1854,6 +1872,7 @@ struct BatchChain {
     double *lpx = nullptr, *lpy = nullptr;
     float* part = nullptr;  // logistic only
     float *agg_a = nullptr, *agg_b = nullptr, *prev = nullptr;
+    double* tile_smax = nullptr;  // per level-0 tile (final prefix)
     int ns = 1;
     int64_t rows_per_split = 0;
     // level-0 tile of every window, fixed at creation from the longest window
@@ -2014,7 +2033,8 @@ extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_des
             return fail(c, PMCMC_ERR_UNSUPPORTED, "batch: chain %d: D too large for the tile kernel", i);
         }
         size_t mat = align256((size_t)h.D * L * sizeof(float));
-        size_t sz = 7 * mat + 2 * align256((size_t)L * sizeof(double)) + 3 * align256((size_t)h.D * nT * sizeof(float));
+        size_t sz = 7 * mat + 2 * align256((size_t)L * sizeof(double)) + 3 * align256((size_t)h.D * nT * sizeof(float)) +
+                    align256((size_t)nT * sizeof(double));
         h.rtg = RefineTarget{};
         h.rtg.D = h.D;
         if (d.target->kind == TGT_LOGISTIC) {
@@ -2071,6 +2091,7 @@ extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_des
         h.agg_a = (float*)take(agg);
         h.agg_b = (float*)take(agg);
         h.prev = (float*)take(agg);
+        h.tile_smax = (double*)take(align256((size_t)nT * sizeof(double)));
         refine_chain_carve(&h.rw, h.rtg, L, h.band, take(align256(refine_chain_bytes(h.rtg, L, h.band))));
         h.prev_dec = (int8_t*)take(align256((size_t)L));
         if (h.tgt->kind == TGT_LOGISTIC) h.part = (float*)q;
@@ -2150,6 +2171,8 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
             a.agg_a = h.agg_a;
             a.agg_b = h.agg_b;
             a.prev = h.prev;
+            a.final_cols = &sy->skip_cols;
+            a.tile_smax = h.tile_smax;
             a.prev_dec = h.prev_dec;
             a.hyst = 1e-6;
             if (b->refine) {
@@ -2261,6 +2284,7 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
                 qa.Yout = v < 2 ? h.Y : nullptr;
                 qa.Wout = v < 2 ? h.W : nullptr;
                 qa.skip = skip;
+                qa.src.col_lo = &sy->skip_cols;  // columns below it are final
                 rp->qpass[v].push_back(qa);
                 rp->gpass[v].push_back(GlmPassArgs{});
                 rp->gred[v].push_back(GlmReduceArgs{});

@@ -41,6 +41,8 @@
 // MODE_SCAN with s0 == nullptr starts from zero.
 #pragma once
 
+#include <climits>
+
 #include <type_traits>
 
 #include "common.hpp"
@@ -141,6 +143,16 @@ struct TileArgs {
     // writes only its own columns, and writes them after it has read them.
     T* a_keep;
     T* b_keep;
+    // Final prefix (DEER modes, two-pass tiles; null otherwise): columns below
+    // *final_cols had a residual FT - S of exactly 0, so both iterate buffers
+    // hold the fixed point there and no later iteration changes them.  Their
+    // tiles are left out: PASS_REDUCE writes the identity aggregate (Delta
+    // stays 0 through it), PASS_APPLY only adds the tile's cached max |S|
+    // (tile_smax, numTiles entries, written whenever the tile runs) to the
+    // metric.  The last workgroup moves the floor to the first tile with a
+    // nonzero residual, once the FP64 decision override (refine_on) is active.
+    const int* final_cols;
+    double* tile_smax;
 };
 
 // LDS layout (elements of T):
@@ -367,6 +379,75 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
     const int64_t PA = D > PMCMC_NT ? D : PMCMC_NT;
     constexpr bool DEERM = MODE != MODE_SCAN;
     constexpr bool OUT = PASS != PASS_REDUCE;
+
+    // End of a DEER output pass: the tile's maxima, its cached max |S| and
+    // residual report (live tiles), the iteration's maxima; the last
+    // workgroup forms the metric, the convergence flag and the next floor.
+    auto finish = [&](double dmax, double smax, bool nz, bool final_tile) {
+        SyncState* sy = a.sync;
+        dmax = wave_max(dmax);
+        smax = wave_max(smax);
+        const int any_nz = __syncthreads_or(nz ? 1 : 0);
+        if ((tid & 63) == 0) {
+            s_red[0][tid >> 6] = dmax;
+            s_red[1][tid >> 6] = smax;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double m0 = 0, m1 = 0;
+            for (int w = 0; w < PMCMC_NT / 64; ++w) {
+                m0 = nan_max(m0, s_red[0][w]);
+                m1 = nan_max(m1, s_red[1][w]);
+            }
+            if (a.final_cols && !final_tile) {
+                a.tile_smax[tile] = m1;
+                if (any_nz) atomicMin(&sy->live_from, (int)c0);
+            }
+            // non-negative doubles order like their bit patterns; NaN sorts above +Inf
+            atomicMax(&sy->dmax_bits, (unsigned long long)__double_as_longlong(m0));
+            atomicMax(&sy->smax_bits, (unsigned long long)__double_as_longlong(m1));
+            // the last workgroup closes the iteration: metric and convergence flag.
+            // acq_rel: this block's maxima are visible before its arrival counts,
+            // and the last arrival sees every block's maxima.
+            const unsigned prev = __hip_atomic_fetch_add(&sy->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (prev == (unsigned)a.numTiles - 1u) {
+                const double dm = __longlong_as_double((long long)atomicAdd(&sy->dmax_bits, 0ull));
+                const double sm = __longlong_as_double((long long)atomicAdd(&sy->smax_bits, 0ull));
+                const double metric = dm / (a.tol_abs + a.tol_rel * sm);
+                const int it = a.iter >= 0 ? a.iter : sy->last_iter + 1;
+                sy->last_metric = metric;
+                sy->last_iter = it;
+                const bool dec_final = !a.refine_on || *(volatile const int*)a.refine_on;
+                if (metric <= 1.0 && dec_final) {
+                    sy->conv_iter = it;
+                    sy->converged = 1;
+                }
+                if (a.final_cols) {
+                    // tiles before the first one with a nonzero residual are final,
+                    // once decisions no longer change with the override's gate
+                    const int lf = __hip_atomic_load(&sy->live_from, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (dec_final) sy->skip_cols = lf == INT_MAX ? (int)((int64_t)a.numTiles * TC) : lf;
+                    sy->live_from = INT_MAX;
+                }
+                sy->dmax_bits = 0ull;
+                sy->smax_bits = 0ull;
+                sy->done = 0u;
+            }
+        }
+    };
+    if constexpr (DEERM && (PASS == PASS_REDUCE || PASS == PASS_APPLY)) {
+        if (a.final_cols && c0 + TC <= (int64_t)*(volatile const int*)a.final_cols) {
+            if constexpr (PASS == PASS_REDUCE) {
+                for (int64_t r = tid; r < D; r += PMCMC_NT) {
+                    a.agg_a[(size_t)tile * D + r] = (T)1;
+                    a.agg_b[(size_t)tile * D + r] = (T)0;
+                }
+            } else {
+                finish(0.0, tid == 0 ? a.tile_smax[tile] : 0.0, false, true);
+            }
+            return;
+        }
+    }
 
     T* xs = reinterpret_cast<T*>(smem);
     T* bl = xs + (DEERM ? (size_t)(TC + 1) * D : (size_t)0);
@@ -727,6 +808,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
 
     // ---- phase 4: apply the incoming state, damping, convergence maxima ---
     double dmax = 0, smax = 0;
+    bool nz = false;  // a nonzero residual (B) in this tile
     const T damp = (T)a.damping;
     for (int64_t i = tid; i < nitems; i += PMCMC_NT) {
         const int64_t r = i % D;
@@ -744,6 +826,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                 const T old = xs[li + D];
                 const T base = corr_base(old);
                 const T v = (a.damping == 1.0) ? base + s : base + damp * s;
+                nz = nz || bl[li] != (T)0;  // NaN counts as nonzero
                 al[li] = v;
                 dmax = nan_max(dmax, fabs((double)v - (double)old));
                 smax = nan_max(smax, fabs((double)v));
@@ -755,45 +838,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
     // ---- phase 5: store the tile (coalesced) ------------------------------
     for (int64_t i = tid; i < nel; i += PMCMC_NT) a.S[gbase + i] = al[i];
 
-    if (DEERM) {
-        SyncState* sy = a.sync;
-        dmax = wave_max(dmax);
-        smax = wave_max(smax);
-        if ((tid & 63) == 0) {
-            s_red[0][tid >> 6] = dmax;
-            s_red[1][tid >> 6] = smax;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            double m0 = 0, m1 = 0;
-            for (int w = 0; w < PMCMC_NT / 64; ++w) {
-                m0 = nan_max(m0, s_red[0][w]);
-                m1 = nan_max(m1, s_red[1][w]);
-            }
-            // non-negative doubles order like their bit patterns; NaN sorts above +Inf
-            atomicMax(&sy->dmax_bits, (unsigned long long)__double_as_longlong(m0));
-            atomicMax(&sy->smax_bits, (unsigned long long)__double_as_longlong(m1));
-            // the last workgroup closes the iteration: metric and convergence flag.
-            // acq_rel: this block's maxima are visible before its arrival counts,
-            // and the last arrival sees every block's maxima.
-            const unsigned prev = __hip_atomic_fetch_add(&sy->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            if (prev == (unsigned)a.numTiles - 1u) {
-                const double dm = __longlong_as_double((long long)atomicAdd(&sy->dmax_bits, 0ull));
-                const double sm = __longlong_as_double((long long)atomicAdd(&sy->smax_bits, 0ull));
-                const double metric = dm / (a.tol_abs + a.tol_rel * sm);
-                const int it = a.iter >= 0 ? a.iter : sy->last_iter + 1;
-                sy->last_metric = metric;
-                sy->last_iter = it;
-                if (metric <= 1.0 && (!a.refine_on || *(volatile const int*)a.refine_on)) {
-                    sy->conv_iter = it;
-                    sy->converged = 1;
-                }
-                sy->dmax_bits = 0ull;
-                sy->smax_bits = 0ull;
-                sy->done = 0u;
-            }
-        }
-    }
+    if (DEERM) finish(dmax, smax, nz, false);
 }
 
 template <typename T, int MODE, int PASS>
