@@ -1289,6 +1289,7 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
                     ra.iter = it;
                     ra.gate_iters = 16;
                     ra.hyst = a.hyst;
+                    ra.final_cols = &w->sync->skip_cols;
                     rc = refine_decisions(&w->refw, rtg, ra, it, c->stream, &msg);
                     if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
                     a.dec = w->refw.dec;
@@ -2199,6 +2200,7 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
             ra.gate_iters = 16;
             ra.iter_dev = &sy->last_iter;
             ra.hyst = 1e-6;
+            ra.final_cols = &sy->skip_cols;
             rp->rargs[par].push_back(ra);
         }
         {

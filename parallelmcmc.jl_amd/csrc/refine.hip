@@ -152,7 +152,9 @@ __device__ void refine_flag_body(const RefineArgs& a, int8_t* dec, int* cnt, int
     }
     const int lane = threadIdx.x & 63;
     const int64_t t_end = (blk + 1) * FLAG_COLS < T ? (blk + 1) * FLAG_COLS : T;
+    const int64_t lo = a.final_cols ? (int64_t)*(volatile const int*)a.final_cols : 0;
     for (int64_t t = blk * FLAG_COLS + (threadIdx.x >> 6); t < t_end; t += RNT / 64) {
+        if (t < lo) continue;  // final column: its decision is settled
         if (!on) {
             if (lane == 0) dec[t] = -1;
             continue;

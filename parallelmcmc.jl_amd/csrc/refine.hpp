@@ -50,6 +50,10 @@ struct RefineArgs {
     const int* iter_dev;
     // FP64 margins below this set the tie bit (bit 1) of dec[t]
     double hyst;
+    // columns below *final_cols are final (the solve's final prefix, which
+    // only moves once the gate is on): their decisions stay as they are and
+    // the flag pass leaves them out (null: none)
+    const int* final_cols;
 };
 
 // Device buffers of the refinement.  Flagged columns are compacted into
