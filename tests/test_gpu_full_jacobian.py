@@ -29,10 +29,11 @@ def _setup(seed, D, cond=2.0, with_mean=True):
 
 
 @pytest.mark.parametrize("jac", ["full", "full_sigmoid"])
-# T = 4477 (70 tiles) and 38417 (601 tiles) take the scan over the tile maps
-# through two and three reduce levels of 8-map tiles
+# T = 4477 (70 tiles) and 16384 / 16320 (256 / 255 tiles: 32 groups of 8 maps)
+# take the one-launch scan over the tile maps; 38417 (601 tiles) the recursive
+# form through three reduce levels of 8-map tiles
 @pytest.mark.parametrize("D,T,damping", [(5, 200, 0.7), (16, 64, 1.0), (64, 130, 1.0), (16, 4477, 1.0),
-                                         (4, 38417, 0.9)])
+                                         (4, 16384, 1.0), (6, 16320, 0.8), (4, 38417, 0.9)])
 def test_full_newton_step_f64_matches_dense_oracle(D, T, damping, jac):
     from ParallelMCMC import DEER
 
