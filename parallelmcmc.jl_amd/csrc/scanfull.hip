@@ -18,6 +18,10 @@
 //   apply   one workgroup per tile runs its columns from the incoming state
 //           (four waves split the k sum of each mat-vec; the columns' J and B
 //           stream through a ring of SAPF register slots)
+// The structured full-Jacobian solve runs its scan over tile maps through
+// scan_full_spine: up to 256 maps in one launch (sfull_spine_fused_kernel:
+// 8-map groups, the chain over the groups in the last workgroup to arrive),
+// beyond that the recursion above from level 1.
 // Padded rows and columns (>= D) are zero in every map, so they stay out of
 // every product.
 #include <hip/hip_runtime.h>
