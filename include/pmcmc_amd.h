@@ -105,6 +105,22 @@ int pmcmc_scan_diag_f32(pmcmc_ctx* ctx, const float* A, const float* B, const fl
 int pmcmc_scan_diag_f64(pmcmc_ctx* ctx, const double* A, const double* B, const double* s0,
                         double* S, int64_t D, int64_t T);
 
+/* ---- allocation-free diagonal scan ---------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same scan as pmcmc_scan_diag_f32 (same kernels,
+ * shapes and results), with its scratch (tile aggregates, look-back status
+ * words) in a fixed per-device array of PMCMC_NOALLOC_SCRATCH_BYTES instead
+ * of memory the context allocates.  It calls no hipMalloc / hipFree.  A D x T
+ * whose scratch exceeds that size returns PMCMC_ERR_ARGUMENT (about
+ * 12 D^2 T / 4096 bytes at D <= 1024: D = 64 up to T ~ 2.8M).  Every call on a
+ * device shares that device's array, which keeps the last call's contents:
+ * calls that overlap on one device (two contexts' streams) race.  The array's
+ * device address is looked up by pmcmc_ctx_create.                         */
+#define PMCMC_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int pmcmc_scan_diag_f32_noalloc(pmcmc_ctx* ctx, const float* A, const float* B, const float* s0,
+                                float* S, int64_t D, int64_t T);
+
 /* ---- dense-Jacobian affine scan -----------------------------------------
  * S[:,t] = J_t S[:,t-1] + B[:,t], S[:,0] uses s0, with J_t = J + t*D*D the
  * t-th D x D column-major matrix; D <= 64.  The general form of the :full

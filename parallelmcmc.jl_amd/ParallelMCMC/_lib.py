@@ -111,6 +111,7 @@ _SIGS = {
     "pmcmc_buffer_download": ([_P, _P, _P, _I64], C.c_int),
     "pmcmc_scan_diag_f32": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
     "pmcmc_scan_diag_f64": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
+    "pmcmc_scan_diag_f32_noalloc": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
     "pmcmc_scan_full_f32": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
     "pmcmc_scan_full_f64": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
     "pmcmc_scan_tune": ([_P, C.c_int, _I64, _I64, C.c_int32, C.POINTER(TileShape)], C.c_int),

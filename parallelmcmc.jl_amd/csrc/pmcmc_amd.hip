@@ -70,6 +70,34 @@ struct pmcmc_ctx {
 };
 
 static void ctx_retain(pmcmc_ctx* c) { c->refs.fetch_add(1, std::memory_order_relaxed); }
+
+// Scratch of the allocation-free scan entry point (pmcmc_scan_diag_f32_noalloc):
+// one array per device, resident for the life of the process.  Host code
+// never passes the array's name to a kernel or a copy (on the host it is a
+// stand-in, not a device address): pmcmc_ctx_create looks up the array's
+// device address once per device with hipGetSymbolAddress, and only that
+// address is used.  Every call on a device shares its array, so calls that
+// overlap on one device race.
+static __device__ __align__(256) unsigned char scratch[PMCMC_NOALLOC_SCRATCH_BYTES];
+static constexpr int PMCMC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[PMCMC_MAX_DEVICES] = {};
+
+// device address of this device's scratch array (the device is current);
+// looked up on the first context of the device
+static hipError_t scratch_lookup(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= PMCMC_MAX_DEVICES) return hipErrorInvalidDevice;
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) g_scratch_ptr[device] = ptr;
+    return e;
+}
+static void* scratch_of(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < PMCMC_MAX_DEVICES) ? g_scratch_ptr[device] : nullptr;
+}
 static void ctx_release(pmcmc_ctx* c);
 
 enum { TGT_DIAG = 0, TGT_LOGISTIC = 1, TGT_GAUSS = 2 };
@@ -462,6 +490,10 @@ extern "C" int pmcmc_ctx_create(int device, void* stream, pmcmc_ctx** out) {
         pmcmc_ctx_destroy(c);
         return PMCMC_ERR_ALLOC;
     }
+    if (scratch_lookup(device) != hipSuccess) {
+        pmcmc_ctx_destroy(c);
+        return PMCMC_ERR_HIP;
+    }
     *out = c;
     return PMCMC_OK;
 }
@@ -548,9 +580,48 @@ extern "C" int pmcmc_buffer_download(pmcmc_ctx* c, void* host, const void* src, 
 // ---------------------------------------------------------------------------
 // standalone diagonal scan
 // ---------------------------------------------------------------------------
+// Scratch of one standalone scan: the device's fixed array (noalloc; too
+// small a problem's need is an argument error) or the context's blob, grown
+// on demand.
+static int scan_scratch(pmcmc_ctx* c, size_t need, bool noalloc, char** out) {
+    if (noalloc) {
+        char* p = (char*)scratch_of(c->device);
+        if (!p) return fail(c, PMCMC_ERR_HIP, "scan: no scratch address for device %d", c->device);
+        if (need > PMCMC_NOALLOC_SCRATCH_BYTES)
+            return fail(c, PMCMC_ERR_ARGUMENT,
+                        "scan: this D x T needs %zu bytes of scratch, above the allocation-free entry point's "
+                        "%zu (use pmcmc_scan_diag_f32)",
+                        need, (size_t)PMCMC_NOALLOC_SCRATCH_BYTES);
+        *out = p;
+        return PMCMC_OK;
+    }
+    if (need > c->scan_blob_cap) {
+        if (c->scan_blob) HIPCHK(c, hipFree(c->scan_blob));
+        c->scan_blob = nullptr;
+        c->scan_blob_cap = 0;
+        HIPCHK(c, hipMalloc(&c->scan_blob, need));
+        c->scan_blob_cap = need;
+    }
+    *out = (char*)c->scan_blob;
+    return PMCMC_OK;
+}
+
+// Look-back error word of a single-pass scan, read into the context's pinned
+// word after the scan (an error means a bounded wait ran out).
+static int scan_err_check(pmcmc_ctx* c, const unsigned* err_dev) {
+    unsigned* hw = reinterpret_cast<unsigned*>(c->host_sync);
+    HIPCHK(c, hipMemcpyAsync(hw, err_dev, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->prof.flush(0);
+    if (*hw) return fail(c, PMCMC_ERR_DEVICE, "scan: look-back wait exceeded its bound");
+    return PMCMC_OK;
+}
+// noalloc: the scratch is the device's fixed array (no allocation; a problem
+// that needs more fails with PMCMC_ERR_ARGUMENT) instead of the context's
+// blob, grown on demand; no kernel-time events are recorded.
 template <typename T>
 static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, int64_t D,
-                     int64_t Tn, const TileChoice* force = nullptr) {
+                     int64_t Tn, const TileChoice* force = nullptr, bool noalloc = false) {
     if (!c) return PMCMC_ERR_ARGUMENT;
     if (D < 1 || Tn < 0) return fail(c, PMCMC_ERR_DIMENSION, "scan: need D >= 1 and T >= 0 (D=%lld T=%lld)", (long long)D, (long long)Tn);
     if (Tn == 0) return PMCMC_OK;
@@ -587,14 +658,9 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
         const size_t sbm = align256((size_t)D * nS * sizeof(T));
         const size_t nint = (size_t)nT + nS + 64;
         const size_t need = 2 * lbm + 3 * sbm + nint * sizeof(int);
-        if (need > c->scan_blob_cap) {
-            if (c->scan_blob) HIPCHK(c, hipFree(c->scan_blob));
-            c->scan_blob = nullptr;
-            c->scan_blob_cap = 0;
-            HIPCHK(c, hipMalloc(&c->scan_blob, need));
-            c->scan_blob_cap = need;
-        }
-        char* blob = (char*)c->scan_blob;
+        char* blob = nullptr;
+        const int rb = scan_scratch(c, need, noalloc, &blob);
+        if (rb != PMCMC_OK) return rb;
         TileArgs<T> a{};
         a.A = A;
         a.B = B;
@@ -614,7 +680,7 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
         a.lb_sstatus = ints + nT;
         a.lb_ticket = (unsigned*)(ints + nT + nS);
         a.lb_err = (unsigned*)(ints + nT + nS + 16);
-        const int e = c->prof.begin(KT_ITERATION, 0, c->stream);
+        const int e = noalloc ? -1 : c->prof.begin(KT_ITERATION, 0, c->stream);
         hipLaunchKernelGGL(zero_ints_kernel, dim3(grid_for((int64_t)nint)), dim3(256), 0, c->stream, ints,
                            (int64_t)nint);
         if (ch.chained == 3) {
@@ -636,12 +702,7 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
         }
         c->prof.end(e, c->stream);
         HIPCHK(c, hipGetLastError());
-        unsigned err = 0;
-        HIPCHK(c, hipMemcpyAsync(&err, a.lb_err, sizeof err, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        c->prof.flush(0);
-        if (err) return fail(c, PMCMC_ERR_DEVICE, "scan: look-back wait exceeded its bound");
-        return PMCMC_OK;
+        return scan_err_check(c, a.lb_err);
     }
     if (have && ch.chained >= 2) {  // register forms not applicable here: the rule
         have = false;
@@ -672,16 +733,11 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
     const size_t lbm = (size_t)D * L0.nT * sizeof(T);
     const size_t lb_bytes = 3 * lbm + ((size_t)L0.nT + 64) * sizeof(int);
     const size_t need = chained ? lb_bytes : plan.bytes;
-    if (need > c->scan_blob_cap) {
-        if (c->scan_blob) HIPCHK(c, hipFree(c->scan_blob));
-        c->scan_blob = nullptr;
-        c->scan_blob_cap = 0;
-        HIPCHK(c, hipMalloc(&c->scan_blob, need));
-        c->scan_blob_cap = need;
-    }
-    const int e = c->prof.begin(KT_ITERATION, 0, c->stream);
+    char* blob = nullptr;
+    const int rb = scan_scratch(c, need, noalloc, &blob);
+    if (rb != PMCMC_OK) return rb;
+    const int e = noalloc ? -1 : c->prof.begin(KT_ITERATION, 0, c->stream);
     if (chained) {
-        char* blob = (char*)c->scan_blob;
         TileArgs<T> a{};
         a.A = A;
         a.B = B;
@@ -706,14 +762,9 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
                            tile_lds_bytes<T>(D, L0.TC, false, true), c->stream, a);
         c->prof.end(e, c->stream);
         HIPCHK(c, hipGetLastError());
-        unsigned err = 0;
-        HIPCHK(c, hipMemcpyAsync(&err, a.lb_err, sizeof err, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        c->prof.flush(0);
-        if (err) return fail(c, PMCMC_ERR_DEVICE, "scan: look-back wait exceeded its bound");
-        return PMCMC_OK;
+        return scan_err_check(c, a.lb_err);
     }
-    run_scan_levels<T>(plan, 0, A, B, S, s0, (char*)c->scan_blob, nullptr, c->stream);
+    run_scan_levels<T>(plan, 0, A, B, S, s0, blob, nullptr, c->stream);
     c->prof.end(e, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -724,6 +775,10 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
 extern "C" int pmcmc_scan_diag_f32(pmcmc_ctx* c, const float* A, const float* B, const float* s0,
                                    float* S, int64_t D, int64_t T) {
     return scan_diag<float>(c, A, B, s0, S, D, T);
+}
+extern "C" int pmcmc_scan_diag_f32_noalloc(pmcmc_ctx* c, const float* A, const float* B, const float* s0,
+                                           float* S, int64_t D, int64_t T) {
+    return scan_diag<float>(c, A, B, s0, S, D, T, nullptr, true);
 }
 extern "C" int pmcmc_scan_diag_f64(pmcmc_ctx* c, const double* A, const double* B,
                                    const double* s0, double* S, int64_t D, int64_t T) {
