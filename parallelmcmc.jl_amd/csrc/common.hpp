@@ -64,6 +64,30 @@ struct SyncState {
 };
 
 // ---------------------------------------------------------------------------
+// Sliding window of one chain of a batched solve.  The Newton iteration runs
+// over the columns [front, hi) of the chain's whole trajectory, hi =
+// min(front + span, T).  After each iteration the window's first columns
+// whose residual FT - S was exactly 0 (its final prefix, at tile granularity)
+// are final, and the whole window is accepted once the convergence metric is
+// <= 1 (the reference's block test, src/DEER/DEER.jl:567-577); either way the
+// front moves past them and the window slides on.  The device moves `front`
+// (last workgroup of the apply pass) and `hi` (window kernel, from the span
+// the host last chose); accepted columns [pend_lo, pend_hi) are copied into
+// the other iterate buffer by the next window kernel, and columns entering
+// the window for the first time (at or beyond `hw`) start from the state just
+// before them (the reference's block start, S = broadcast(s0)).
+// ---------------------------------------------------------------------------
+struct SlideState {
+    long long front, hi, T;
+    long long hw;                // columns below hw hold an iterate or an initial guess
+    long long pend_lo, pend_hi;  // accepted columns to copy into the other buffer
+    int seg_start;               // first iteration of the current window segment
+    int accepts;                 // windows accepted by the metric test
+    int last_adv_iter;           // last iteration that moved the front
+    int pad;
+};
+
+// ---------------------------------------------------------------------------
 // Optional per-kernel timing with HIP events on the launching stream.  Pairs
 // are tagged with the Newton iteration so that launches queued after
 // convergence (which exit at once) are not counted.

@@ -156,7 +156,8 @@ __device__ __forceinline__ void gauss_pass_body(const ColSrc<float>& src, const 
                     v = src.x0 ? (t == 0 ? src.x0[kk] : src.V[(size_t)(t - 1) * D + kk]) : src.V[(size_t)t * D + kk];
                 } else {
                     const size_t gi = (size_t)t * D + kk;
-                    v = src.Z ? src.Z[(size_t)(set - 1) * D * Tn + gi] : rademacher(src.keys[set - 1], gi);
+                    v = src.Z ? src.Z[(size_t)(set - 1) * D * Tn + gi]
+                              : rademacher(src.keys[set - 1], gi + (size_t)src.zoff);
                 }
             }
             cv[m] = v;
@@ -291,8 +292,183 @@ template <int DPT, bool JBF>
 __global__ __launch_bounds__(256) void gauss_pass_mfma_batched(const GaussPassArgs* args, const int2* items) {
     const int2 w = items[blockIdx.x];
     const GaussPassArgs& a = args[w.x];
+    if ((int64_t)w.y * GCB >= a.src.Tn) return;  // beyond the chain's current window
     gauss_pass_body<DPT, 1, JBF>(a.src, a.L32, a.mu, a.pass, a.eps, a.Xi, a.gout, a.hout, a.lpout, a.Yout, a.Wout,
                                  a.skip, w.y, a.ldL);
+}
+
+// ---------------------------------------------------------------------------
+// Batched pass over one 64-row slab of Lambda: work item (chain, column
+// block, slab) computes rows [64 slab, 64 slab + 64) of P = Lambda [v - mean |
+// dir] for the block's 32 states and 32 directions, K = the chain's own
+// padded width (any width in one launch).  Wave w owns row tile 4 slab + w
+// for all four column tiles.  Splitting the rows over workgroups keeps a
+// column block's latency at a quarter of the whole-column pass's when few
+// chains are live (a chain's window then has only a handful of blocks); the
+// log-density of a column is written per slab (lpout[slab * lp_stride + t])
+// and summed over the slabs, in slab order, by its readers.
+// ---------------------------------------------------------------------------
+template <bool JBF>
+__device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const int64_t cb, const int slab) {
+    constexpr int NC = 2 * GCB;
+    constexpr int CPB = GCB;
+    constexpr int SL = GKC + 4;
+    constexpr int SVc = GKC + 4;
+    constexpr int NF = JBF ? 2 : 4;  // column tiles on FP32 MFMA
+    __shared__ float Ls[64 * SL];
+    __shared__ float Vc[NC * SVc];
+    __shared__ double lpred[4][NC];
+
+    const ColSrc<float>& src = a.src;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t D = src.D, Tn = src.Tn;
+    const int ldL = a.ldL;
+    const int r0 = 64 * slab;
+    const float* mu = a.mu;
+
+    v4f acc[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) acc[ct] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+    // k-slices software-pipelined as in the whole-column pass: 64 rows x 32 k
+    // of Lambda (two float4 per thread) and 64 columns x 32 k of values
+    constexpr int NCV = NC * GKC / 256;
+    float4 lv[2];
+    float cv[NCV];
+    float muv = 0.f;
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const int i = tid + 256 * m;
+            const int d = r0 + i / (GKC / 4), q = i % (GKC / 4);
+            lv[m] = d < ldL ? *reinterpret_cast<const float4*>(a.L32 + (size_t)d * ldL + k0 + 4 * q)
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        const int64_t kk = k0 + (tid % GKC);
+#pragma unroll
+        for (int m = 0; m < NCV; ++m) {
+            const int j = (tid + 256 * m) / GKC;
+            const int set = j / CPB;
+            const int64_t t = cb * CPB + (j % CPB);
+            float v = 0.f;
+            if (t < Tn && kk < D) {
+                if (set == 0) {
+                    v = src.x0 ? (t == 0 ? src.x0[kk] : src.V[(size_t)(t - 1) * D + kk]) : src.V[(size_t)t * D + kk];
+                } else {
+                    const size_t gi = (size_t)t * D + kk;
+                    v = src.Z ? src.Z[gi] : rademacher(src.keys[0], gi + (size_t)src.zoff);
+                }
+            }
+            cv[m] = v;
+        }
+        muv = (mu && kk < D) ? mu[kk] : 0.f;
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < ldL; k0 += GKC) {
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const int i = tid + 256 * m;
+            const int d = i / (GKC / 4), q = i % (GKC / 4);
+            float* dst = &Ls[d * SL + 4 * q];
+            dst[0] = lv[m].x;
+            dst[1] = lv[m].y;
+            dst[2] = lv[m].z;
+            dst[3] = lv[m].w;
+        }
+#pragma unroll
+        for (int m = 0; m < NCV; ++m) {
+            const int i = tid + 256 * m;
+            const int j = i / GKC, k = i % GKC;
+            const bool cen = j / CPB == 0 && cb * CPB + (j % CPB) < Tn;
+            Vc[j * SVc + k] = cen ? cv[m] - muv : cv[m];
+        }
+        __syncthreads();
+        if (k0 + GKC < ldL) fetch(k0 + GKC);
+#pragma unroll
+        for (int ks = 0; ks < GKC / 4; ++ks) {
+            const float av = Ls[(16 * wave + li) * SL + 4 * ks + lk];
+#pragma unroll
+            for (int ct = 0; ct < NF; ++ct)
+                acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, Vc[(16 * ct + li) * SVc + 4 * ks + lk], acc[ct],
+                                                               0, 0, 0);
+        }
+        if (JBF) {
+            v8bf av, b2, b3;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                av[e] = (__bf16)Ls[(16 * wave + li) * SL + 8 * lk + e];
+                b2[e] = (__bf16)Vc[(32 + li) * SVc + 8 * lk + e];
+                b3[e] = (__bf16)Vc[(48 + li) * SVc + 8 * lk + e];
+            }
+            acc[2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b2, acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b3, acc[3], 0, 0, 0);
+        }
+    }
+
+    // epilogue: lane holds P[d = r0 + 16 wave + 4 lk + r][column 16 ct + li]
+    const double eps = (double)a.eps;
+    const double sq2e = sqrt(2.0 * eps);
+    double lpc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        const int j = 16 * ct + li;
+        const int set = j / CPB;
+        const int64_t t = cb * CPB + (j % CPB);
+        if (t >= Tn) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t d = r0 + 16 * wave + 4 * lk + r;
+            if (d >= D) continue;
+            const size_t gi = (size_t)t * D + d;
+            const float g = -acc[ct][r];
+            if (set == 0) {
+                const double x = src.get(0, t, d);
+                const double xm = x - (mu ? (double)mu[d] : 0.0);
+                lpc[ct] += 0.5 * xm * (double)g;
+                a.gout[gi] = g;
+                if (a.pass == 0) a.Yout[gi] = (float)(x + eps * g + sq2e * (double)a.Xi[gi]);
+            } else {
+                if (a.pass == 0)
+                    a.Wout[gi] = (float)(src.get(1, t, d) + eps * g);
+                else
+                    a.hout[gi] = g;
+            }
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        lpc[ct] += __shfl_xor(lpc[ct], 16, 64);
+        lpc[ct] += __shfl_xor(lpc[ct], 32, 64);
+        if (lk == 0) lpred[wave][16 * ct + li] = lpc[ct];
+    }
+    __syncthreads();
+    if (tid < CPB) {
+        const int64_t t = cb * CPB + tid;
+        if (t < Tn)
+            a.lpout[(size_t)slab * a.lp_stride + t] =
+                (lpred[0][tid] + lpred[1][tid]) + (lpred[2][tid] + lpred[3][tid]);
+    }
+}
+
+// work item blockIdx.x is (chain, column block, slab) = items[blockIdx.x]
+template <bool JBF>
+__global__ __launch_bounds__(256) void gauss_slab_batched(const GaussPassArgs* args, const int4* items) {
+    const int4 w = items[blockIdx.x];
+    const GaussPassArgs& a = args[w.x];
+    if (a.skip && *(volatile const int*)a.skip) return;
+    if ((int64_t)w.y * GCB >= a.src.Tn) return;  // beyond the chain's current window
+    gauss_slab_body<JBF>(a, w.y, w.z);
+}
+
+void launch_gauss_slab_batched(bool jbf, const GaussPassArgs* args, const int4* items, int nitems, hipStream_t s) {
+    if (nitems <= 0) return;
+    if (jbf)
+        hipLaunchKernelGGL((gauss_slab_batched<true>), dim3((unsigned)nitems), dim3(256), 0, s, args, items);
+    else
+        hipLaunchKernelGGL((gauss_slab_batched<false>), dim3((unsigned)nitems), dim3(256), 0, s, args, items);
 }
 
 // ---------------------------------------------------------------------------

@@ -37,7 +37,13 @@ struct GaussPassArgs {
     float* Wout;
     const int* skip;
     int ldL;  // Lambda's padded width (row stride of L32), <= 16 * dpt of the launch
+    // slab passes: the log-density partial of slab s goes to lpout[s * lp_stride + t]
+    int64_t lp_stride;
 };
+
+// Batched FP32 Lambda passes over 64-row slabs (one probe set), any widths in
+// one launch: items (chain, column block of GAUSS_CB, slab).
+void launch_gauss_slab_batched(bool jbf, const GaussPassArgs* args, const int4* items, int nitems, hipStream_t s);
 
 // Batched FP32 Lambda passes (one probe set) over chains of padded width up to 16 * dpt.
 void launch_gauss_pass_batched(int dpt, bool jbf, const GaussPassArgs* args, const int2* items, int nitems,

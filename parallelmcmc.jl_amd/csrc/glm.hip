@@ -463,6 +463,7 @@ template <int DPT, int K, bool JBF>
 __global__ __launch_bounds__(256) void glm_pass_mfma_batched(const GlmPassArgs* args, const int4* items) {
     const int4 w = items[blockIdx.x];
     const GlmPassArgs& a = args[w.x];
+    if ((int64_t)w.y * CB >= a.src.Tn) return;  // beyond the chain's current window
     glm_pass_body<DPT, K, JBF>(a.src, a.Xrm, a.y32, a.N, a.rows_per_split, a.part, a.llpart, a.skip, w.y, w.z);
 }
 

@@ -36,6 +36,9 @@ struct ColSrc {
     int K;
     int64_t D, Tn;
     const int* col_lo;  // device; column blocks wholly below *col_lo are left out (null: none)
+    // element offset of column 0 in the probe stream: a window of a longer
+    // trajectory keeps the probes of its absolute columns
+    int64_t zoff;
 
     __device__ inline double get(int set, int64_t t, int64_t d) const {
         if (set == 0) {
@@ -44,7 +47,7 @@ struct ColSrc {
         }
         const size_t gi = (size_t)t * D + d;
         if (Z) return (double)Z[(size_t)(set - 1) * D * Tn + gi];
-        return (double)rademacher(keys[set - 1], gi);
+        return (double)rademacher(keys[set - 1], gi + (size_t)zoff);
     }
 };
 

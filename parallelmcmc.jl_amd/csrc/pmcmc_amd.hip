@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -1817,25 +1818,52 @@ extern "C" int pmcmc_deer_adjoint(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_ta
 }
 
 // ---------------------------------------------------------------------------
-// batched solve of many chains (uneven D, T, targets) in windows
+// batched solve of many chains (uneven D, T, targets) in sliding windows
 // ---------------------------------------------------------------------------
-// Every kernel of an iteration is one launch over all active chains: the
-// work items (chain, tile) / (chain, column block, N split) index per-chain
+// Every kernel of an iteration is one launch over all live chains: the work
+// items (chain, tile) / (chain, column block, N split) index per-chain
 // descriptor tables in device memory, GEMM passes grouped by padded width.
-// Each chain's convergence flag turns its items into no-ops; the host only
-// reads the flags between chunks of iterations.  An iteration pair (the
-// iterate double buffer alternates) is captured once per window round as a
-// hipGraph and replayed.
+// Each chain iterates over its own window [front, hi) of its whole
+// trajectory (SlideState).  A window kernel at the start of every iteration
+// places each chain's window: it rewrites the window-dependent fields of the
+// chain's descriptors, gives columns entering the window for the first time
+// the state just before them, and copies the columns the last iteration
+// accepted into the other iterate buffer.  The last workgroup of the apply
+// pass moves the front (tile_kernel.hpp).  No chain waits for another.  The
+// host reads the chains' states after every chunk of iterations, picks each
+// chain's window length and rebuilds the work items when a length changes or
+// chains have finished; the iterations in between are replayed from a
+// captured hipGraph (an iteration pair, since the iterate buffers alternate).
 namespace pmcmc {
 
-struct BInit {
-    const float* x0;  // window start state (user x0 or the previous window's last column)
-    float* S0;        // iterate buffer 0
-    int64_t D, n;     // n = D * L
+// One chain's window placement: its buffers and the addresses of its entries
+// in the device descriptor tables.
+struct WinDesc {
+    int64_t D, T, L;
+    int TC, kind;  // kind: 0 logistic, 1 dense Gaussian
+    float* buf[2];
+    const float* x0;
+    const float* Xi;
+    const float* U;
+    uint8_t* acc_out;
+    int8_t* prev_dec;  // T entries (absolute columns)
+    SlideState* win;
     SyncState* sy;
-    int* rcnt;         // refinement counters and gate flag (4), cleared
-    int8_t* prev_dec;  // L previous decisions, set to -1 (none)
-    int64_t L;
+    const int* span;
+    TileArgs<float>* ta[2];
+    GlmPassArgs* gp[3];
+    GlmReduceArgs* gr[3];
+    GaussPassArgs* qp[3];
+    RefineArgs* ra[2];
+    RefineWork* rw;
+};
+
+struct SlideInit {
+    SlideState* win;
+    SyncState* sy;
+    int* rcnt;  // refinement counters and gate flag (4), cleared
+    int8_t* prev_dec;
+    int64_t T;
 };
 
 struct BCopy {
@@ -1846,8 +1874,8 @@ struct BCopy {
     const SyncState* sy;
 };
 
-__global__ __launch_bounds__(256) void batch_init_kernel(const BInit* t) {
-    const BInit& a = t[blockIdx.y];
+__global__ __launch_bounds__(256) void batch_slide_init_kernel(const SlideInit* t) {
+    const SlideInit& a = t[blockIdx.y];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         SyncState* s = a.sy;
         s->done = 0u;
@@ -1861,14 +1889,93 @@ __global__ __launch_bounds__(256) void batch_init_kernel(const BInit* t) {
         s->live_from = INT_MAX;
         s->skip_cols = 0;
         s->smax_prefix = 0.0;
+        SlideState* w = a.win;
+        w->front = 0;
+        w->hi = 0;
+        w->T = a.T;
+        w->hw = 0;
+        w->pend_lo = w->pend_hi = 0;
+        w->seg_start = 0;
+        w->accepts = 0;
+        w->last_adv_iter = -1;
     }
-    if (blockIdx.x == 0 && threadIdx.x < 4 && a.rcnt) a.rcnt[threadIdx.x] = 0;
-    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * 256) a.S0[i] = a.x0[i % a.D];
-    if (a.prev_dec)
-        for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.L; i += (int64_t)gridDim.x * 256) a.prev_dec[i] = -1;
+    if (blockIdx.x == 0 && threadIdx.x < 4) a.rcnt[threadIdx.x] = 0;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < a.T; i += (int64_t)gridDim.x * 256) a.prev_dec[i] = -1;
 }
 
-// the converged iterate of iteration k sits in buffer (k + 1) & 1
+// Places the windows of iteration parity `par` (its input buffer is buf[par]).
+// Reads the chain's SlideState without changing it: every workgroup of the
+// chain sees the same state, and the apply pass's last workgroup, which owns
+// the updates, runs after this kernel.
+__global__ __launch_bounds__(256) void batch_window_kernel(const WinDesc* wds, const int* ids, int par) {
+    const WinDesc& w = wds[ids[blockIdx.y]];
+    if (*(volatile const int*)&w.sy->converged) return;
+    const SlideState st = *w.win;
+    const int64_t D = w.D;
+    int64_t span = *w.span;
+    if (span < 1) span = 1;
+    if (span > w.L) span = w.L;
+    const int64_t front = st.front;
+    const int64_t hi = front + span < w.T ? front + span : w.T;
+    float* sin = w.buf[par];
+    float* sout = w.buf[par ^ 1];
+    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x, stride = (int64_t)gridDim.x * 256;
+    // columns the last iteration accepted: into the other buffer too, so that
+    // both hold them (the next iteration reads its state just before the front)
+    if (st.pend_hi > st.pend_lo) {
+        const int64_t lo = st.pend_lo * D, n = (st.pend_hi - st.pend_lo) * D;
+        for (int64_t i = gid; i < n; i += stride) sout[lo + i] = sin[lo + i];
+    }
+    // columns entering the window for the first time start from the state
+    // just before them
+    if (hi > st.hw) {
+        const float* src = st.hw == 0 ? w.x0 : sin + (st.hw - 1) * D;
+        const int64_t lo = st.hw * D, n = (hi - st.hw) * D;
+        for (int64_t i = gid; i < n; i += stride) sin[lo + i] = src[i % D];
+    }
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int64_t n = hi - front;
+    const int64_t off = front * D;
+    const float* s0 = front == 0 ? w.x0 : sin + off - D;
+    TileArgs<float>& a = *w.ta[par];
+    a.s0 = s0;
+    a.T_len = n;
+    a.numTiles = (int)((n + w.TC - 1) / w.TC);
+    a.S_in = sin + off;
+    a.S = sout + off;
+    a.Xi = w.Xi + off;
+    a.U = w.U + front;
+    a.acc_out = w.acc_out ? w.acc_out + front : nullptr;
+    a.prev_dec = w.prev_dec + front;
+    a.zoff = off;
+    const int vs[2] = {par, 2};
+    for (int q = 0; q < 2; ++q) {
+        const int v = vs[q];
+        ColSrc<float>* src = w.kind == 0 ? &w.gp[v]->src : &w.qp[v]->src;
+        src->Tn = n;
+        src->zoff = off;
+        if (v < 2) {
+            src->V = sin + off;
+            src->x0 = s0;
+        }
+        if (w.kind == 0) {
+            w.gr[v]->src = *src;
+            w.gr[v]->Xi = w.Xi + off;
+            w.gr[v]->llstride = n;  // the pass lays its log-likelihood partials out per split over Tn columns
+        } else {
+            w.qp[v]->Xi = w.Xi + off;
+        }
+    }
+    RefineArgs& r = *w.ra[par];
+    r.x0 = s0;
+    r.S_in = sin + off;
+    r.Xi = w.Xi + off;
+    r.U = w.U + front;
+    w.rw->T = n;
+    w.win->hi = hi;
+}
+
+// the final iterate of a chain that finished at iteration k sits in buffer (k + 1) & 1
 __global__ __launch_bounds__(256) void batch_copy_kernel(const BCopy* t) {
     const BCopy& a = t[blockIdx.y];
     const SyncState* s = a.sy;
@@ -1915,7 +2022,7 @@ __global__ __launch_bounds__(256) void batch_agg_scan_kernel(const TileArgs<floa
 
 struct BatchChain {
     const pmcmc_target* tgt = nullptr;
-    int64_t D = 0, T = 0, Dp = 0;
+    int64_t D = 0, T = 0, Dp = 0, L = 0;
     double eps = 0;
     const float* x0 = nullptr;
     const float* Xi = nullptr;
@@ -1923,22 +2030,27 @@ struct BatchChain {
     float* S_out = nullptr;
     uint8_t* acc_out = nullptr;
     // device state (carved from the arena)
-    float* buf[2] = {nullptr, nullptr};
-    float *Y = nullptr, *gx = nullptr, *gy = nullptr, *W = nullptr, *hw = nullptr;
+    float* buf[2] = {nullptr, nullptr};  // D x T each: the whole trajectory
+    float *Y = nullptr, *gx = nullptr, *gy = nullptr, *W = nullptr, *hw = nullptr;  // D x L (window)
     double *lpx = nullptr, *lpy = nullptr;
     float* part = nullptr;  // logistic only
     float *agg_a = nullptr, *agg_b = nullptr, *prev = nullptr;
-    double* tile_smax = nullptr;  // per level-0 tile (final prefix)
+    double* tile_smax = nullptr;
+    int8_t* prev_dec = nullptr;  // T
     int ns = 1;
     int64_t rows_per_split = 0;
-    // level-0 tile of every window, fixed at creation from the longest window
-    // (the aggregates are sized for it; shorter windows clamp TC to their length)
-    int TC = 0, P = 0;
-    // FP64 decisions near the threshold (as in the single-chain solve)
+    int TC = 0, P = 0, nTmax = 0;
+    int nslab = 1;  // 64-row slabs of the Lambda pass (dense Gaussian), else 1
     RefineTarget rtg{};
     double band = 0;
-    RefineWork rw;              // buffers sized for the longest window
-    int8_t* prev_dec = nullptr;  // hysteresis: previous iteration's decisions
+    RefineWork rw;
+    // window length policy (host): the chain's narrow length, and what the
+    // previous synchronisation saw
+    int span = 0, narrow = 0;
+    int64_t prev_front = 0;
+    int prev_iter = -1;
+    double prev_metric = INFINITY;
+    bool stopped = false;
     // per-solve results
     int iters = 0;
     bool converged = true;
@@ -1955,52 +2067,46 @@ struct pmcmc_batch {
     bool use_graph = true;
     bool refine = true;  // FP64 decisions near the threshold (PMCMC_REFINE=0: off)
     void* arena = nullptr;
-    SyncState* syncs = nullptr;  // n, device
-    SyncState* hsync = nullptr;  // n, pinned host
-    void* tables = nullptr;      // device descriptor tables
+    SyncState* syncs = nullptr;    // n, device
+    SlideState* slides = nullptr;  // n, device
+    int* spans = nullptr;          // n, device (host-written)
+    SyncState* hsync = nullptr;    // n, pinned host
+    SlideState* hslide = nullptr;  // n, pinned host
+    int* hspans = nullptr;         // n, pinned host
+    void* tables = nullptr;        // device descriptor tables
     size_t tables_cap = 0;
-    void* items = nullptr;       // device work-item lists
+    void* items = nullptr;         // device work-item lists
     size_t items_cap = 0;
 };
 
 namespace {
 
-struct RoundPlan {
-    // host mirrors of the per-chain descriptor tables of one window round
-    std::vector<TileArgs<float>> targs[2];
-    std::vector<GlmPassArgs> gpass[3];  // pass 0 parity 0, pass 0 parity 1, pass 1
-    std::vector<GlmReduceArgs> gred[3];
-    std::vector<GaussPassArgs> qpass[3];
-    std::vector<BInit> inits;
-    std::vector<BCopy> copies;
-    std::vector<RefineTarget> rtgs;      // per active slot
-    std::vector<RefineArgs> rargs[2];    // per parity
-    std::vector<RefineWork> rws;
-    std::vector<int> active;  // chain index of each active slot
-    // per active slot: shape of its work
-    struct Shape {
-        int kind, nT, ncb, ns, dpt, gcb;
-        int64_t L;
-        size_t lds;
-    };
-    std::vector<Shape> shape;
-};
-
-// Work-item lists over the live (not yet converged) slots of a round.  Tile
-// items are split into LDS-size classes so that small tiles are not launched
-// with the largest chain's LDS footprint (and its occupancy).
+// Work-item lists over the live chains.  Tile items are split into LDS-size
+// classes so that small tiles are not launched with the largest chain's LDS
+// footprint (and its occupancy).
 static constexpr int NLDS = 3;
 static int lds_class(size_t lds) { return lds <= 16384 ? 0 : (lds <= 32768 ? 1 : 2); }
 struct Items {
-    std::vector<int2> red[NLDS], single[NLDS], glm_red;
-    std::vector<int> multi;
+    std::vector<int2> red[NLDS], glm_red;
+    std::vector<int> multi, wins;
     std::vector<int4> glm[17];
-    std::vector<int2> gauss[17];
-    size_t lds_multi[NLDS] = {0, 0, 0}, lds_single[NLDS] = {0, 0, 0};
-    // FP64 decision recompute: live slots (all, logistic, Gaussian)
+    std::vector<int4> gslab;  // (chain, column block, slab) of the Lambda passes
+    size_t lds_multi[NLDS] = {0, 0, 0};
+    // FP64 decision recompute: live chains (all, logistic, Gaussian)
     std::vector<int> r_all, r_glm, r_gauss;
     int64_t r_maxT = 0;
     int r_split_glm = 0, r_split_gauss = 0;
+};
+
+// Host staging of a set of device tables, each 256-byte aligned.
+struct Pack {
+    std::vector<char> host;
+    size_t put(const void* src, size_t bytes) {
+        const size_t o = align256(host.size());
+        host.resize(o + bytes);
+        if (bytes) memcpy(host.data() + o, src, bytes);
+        return o;
+    }
 };
 
 }  // namespace
@@ -2008,13 +2114,21 @@ struct Items {
 static void batch_free(pmcmc_batch* b) {
     hipFree(b->arena);
     hipFree(b->syncs);
+    hipFree(b->slides);
+    hipFree(b->spans);
     hipHostFree(b->hsync);
+    hipHostFree(b->hslide);
+    hipHostFree(b->hspans);
     hipFree(b->tables);
     hipFree(b->items);
     b->items = nullptr;
     b->arena = nullptr;
     b->syncs = nullptr;
+    b->slides = nullptr;
+    b->spans = nullptr;
     b->hsync = nullptr;
+    b->hslide = nullptr;
+    b->hspans = nullptr;
     b->tables = nullptr;
 }
 
@@ -2029,6 +2143,21 @@ extern "C" int pmcmc_batch_destroy(pmcmc_batch* b) {
     delete b;
     if (c) ctx_release(c);
     return PMCMC_OK;
+}
+
+// Window length of a chain whose front only moves about one column per
+// iteration (the causal progress of a Newton iteration whose Jacobian
+// estimate misses the dynamics): a few tiles past the front, since the
+// columns further ahead cannot settle before the front reaches them.
+static int narrow_span(int64_t L, int TC) {
+    static const int base = [] {
+        const char* e = getenv("PMCMC_SLIDE_NARROW");
+        const int v = e ? atoi(e) : 0;
+        return v > 0 ? v : 128;
+    }();
+    int64_t n = std::max<int64_t>(base, 2 * (int64_t)TC);
+    n = (n + TC - 1) / TC * TC;
+    return (int)std::min<int64_t>(n, L);
 }
 
 extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_desc* chains,
@@ -2075,22 +2204,30 @@ extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_des
         h.S_out = (float*)d.S_out;
         h.acc_out = d.accepted_out;
         h.Dp = d.target->kind == TGT_GAUSS ? d.target->gauss.Dp : d.target->glm.Dp;
+        h.nslab = d.target->kind == TGT_GAUSS ? (int)((h.Dp + 63) / 64) : 1;
         if (h.Dp > 256) {
             delete b;
             return fail(c, PMCMC_ERR_UNSUPPORTED, "batch: chain %d: D > 256", i);
         }
         const int64_t L = window < d.T ? window : d.T;
+        h.L = L;
         int TC, P, nT;
         batch_tile_shape(h.D, L, &TC, &P, &nT);
         h.TC = TC;
         h.P = P;
+        h.nTmax = nT;
+        h.span = (int)L;
+        h.narrow = narrow_span(L, TC);
         if (tile_lds_bytes<float>(h.D, TC, true) > 64512) {
             delete b;
             return fail(c, PMCMC_ERR_UNSUPPORTED, "batch: chain %d: D too large for the tile kernel", i);
         }
-        size_t mat = align256((size_t)h.D * L * sizeof(float));
-        size_t sz = 7 * mat + 2 * align256((size_t)L * sizeof(double)) + 3 * align256((size_t)h.D * nT * sizeof(float)) +
-                    align256((size_t)nT * sizeof(double));
+        const size_t traj = align256((size_t)h.D * h.T * sizeof(float));
+        const size_t mat = align256((size_t)h.D * L * sizeof(float));
+        // log-densities: up to four 64-row slab partials per column (dense Gaussian)
+        size_t sz = 2 * traj + 5 * mat + 2 * align256((size_t)4 * L * sizeof(double)) +
+                    3 * align256((size_t)h.D * nT * sizeof(float)) + align256((size_t)nT * sizeof(double)) +
+                    align256((size_t)h.T);
         h.rtg = RefineTarget{};
         h.rtg.D = h.D;
         if (d.target->kind == TGT_LOGISTIC) {
@@ -2106,7 +2243,7 @@ extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_des
             h.rtg.mu64 = d.target->gauss.mu64;
             h.band = 1e-3;
         }
-        sz += align256(refine_chain_bytes(h.rtg, L, h.band)) + align256((size_t)L);
+        sz += align256(refine_chain_bytes(h.rtg, L, h.band));
         if (d.target->kind == TGT_LOGISTIC) {
             glm_split_plan(&d.target->glm, L, &h.ns, &h.rows_per_split);
             sz += align256((size_t)h.ns * 2 * L * h.Dp * sizeof(float) + (size_t)h.ns * L * sizeof(double));
@@ -2117,7 +2254,11 @@ extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_des
     bool ok = hipMalloc(&b->arena, bytes > 0 ? bytes : 256) == hipSuccess &&
               hipMalloc(&b->syncs, n * sizeof(SyncState)) == hipSuccess &&
               hipMemset(b->syncs, 0, n * sizeof(SyncState)) == hipSuccess &&
-              hipHostMalloc(&b->hsync, n * sizeof(SyncState)) == hipSuccess;
+              hipMalloc(&b->slides, n * sizeof(SlideState)) == hipSuccess &&
+              hipMalloc(&b->spans, n * sizeof(int)) == hipSuccess &&
+              hipHostMalloc(&b->hsync, n * sizeof(SyncState)) == hipSuccess &&
+              hipHostMalloc(&b->hslide, n * sizeof(SlideState)) == hipSuccess &&
+              hipHostMalloc(&b->hspans, n * sizeof(int)) == hipSuccess;
     if (!ok) {
         batch_free(b);
         delete b;
@@ -2125,31 +2266,32 @@ extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_des
     }
     for (int i = 0; i < n; ++i) {
         BatchChain& h = b->ch[i];
-        const int64_t L = window < h.T ? window : h.T;
-        const int nT = (int)((L + h.TC - 1) / h.TC);
+        const int64_t L = h.L;
+        const int nT = h.nTmax;
         char* q = (char*)b->arena + off[i];
+        const size_t traj = align256((size_t)h.D * h.T * sizeof(float));
         const size_t mat = align256((size_t)h.D * L * sizeof(float));
         auto take = [&](size_t s) {
             char* r = q;
             q += s;
             return r;
         };
-        h.buf[0] = (float*)take(mat);
-        h.buf[1] = (float*)take(mat);
+        h.buf[0] = (float*)take(traj);
+        h.buf[1] = (float*)take(traj);
         h.Y = (float*)take(mat);
         h.gx = (float*)take(mat);
         h.gy = (float*)take(mat);
         h.W = (float*)take(mat);
         h.hw = (float*)take(mat);
-        h.lpx = (double*)take(align256((size_t)L * sizeof(double)));
-        h.lpy = (double*)take(align256((size_t)L * sizeof(double)));
+        h.lpx = (double*)take(align256((size_t)4 * L * sizeof(double)));
+        h.lpy = (double*)take(align256((size_t)4 * L * sizeof(double)));
         const size_t agg = align256((size_t)h.D * nT * sizeof(float));
         h.agg_a = (float*)take(agg);
         h.agg_b = (float*)take(agg);
         h.prev = (float*)take(agg);
         h.tile_smax = (double*)take(align256((size_t)nT * sizeof(double)));
+        h.prev_dec = (int8_t*)take(align256((size_t)h.T));
         refine_chain_carve(&h.rw, h.rtg, L, h.band, take(align256(refine_chain_bytes(h.rtg, L, h.band))));
-        h.prev_dec = (int8_t*)take(align256((size_t)L));
         if (h.tgt->kind == TGT_LOGISTIC) h.part = (float*)q;
     }
     {
@@ -2168,44 +2310,44 @@ extern "C" int pmcmc_batch_set_graph(pmcmc_batch* b, int32_t on) {
     return PMCMC_OK;
 }
 
-// Tables of window round r; returns false when no chain is active.
-static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
+namespace {
+// host mirrors of the per-chain descriptor tables (indexed by chain)
+struct BatchTables {
+    std::vector<TileArgs<float>> targs[2];
+    std::vector<GlmPassArgs> gpass[3];  // pass 0 parity 0, pass 0 parity 1, pass 1
+    std::vector<GlmReduceArgs> gred[3];
+    std::vector<GaussPassArgs> qpass[3];
+    std::vector<RefineTarget> rtgs;
+    std::vector<RefineArgs> rargs[2];
+    std::vector<RefineWork> rws;
+    std::vector<SlideInit> inits;
+    std::vector<BCopy> copies;
+    std::vector<WinDesc> wins;
+};
+}  // namespace
+
+// Window-independent fields of every descriptor; the window kernel fills in
+// the rest at the start of each iteration.
+static void batch_tables(pmcmc_batch* b, BatchTables* bt) {
     const pmcmc_deer_params& p = b->p;
     const int n = (int)b->ch.size();
     for (int i = 0; i < n; ++i) {
         BatchChain& h = b->ch[i];
-        const int64_t t0 = r * b->W;
-        if (t0 >= h.T) continue;
-        const int64_t L = (h.T - t0) < b->W ? (h.T - t0) : b->W;
-        rp->active.push_back(i);
-        const int64_t D = h.D;
-        const float* s0 = t0 == 0 ? h.x0 : h.S_out + (size_t)(t0 - 1) * D;
-        const float* Xi = h.Xi + (size_t)t0 * D;
-        const float* U = h.U + t0;
+        const int64_t D = h.D, L = h.L;
         SyncState* sy = b->syncs + i;
         const int* skip = &sy->converged;
-        // a fresh probe per chain and window, fixed over the window's iterations
-        const uint64_t seed = p.seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(i + 1)) ^ ((uint64_t)r << 40);
+        // one probe set per chain, attached to its absolute columns
+        const uint64_t seed = p.seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(i + 1));
         const uint32_t key = probe_key(seed, 0xFFFFFFFFu, 0u);
         const int TC = (int)std::min<int64_t>(h.TC, L);
         const int P = std::min(h.P, TC);
-        const int nT = (int)((L + TC - 1) / TC);
         for (int par = 0; par < 2; ++par) {
             TileArgs<float> a{};
-            a.s0 = s0;
             a.D = D;
-            a.T_len = L;
             a.TC = TC;
             a.P = P;
-            a.numTiles = nT;
             a.skip = skip;
             a.sync = sy;
-            a.S_in = h.buf[par];
-            a.S = h.buf[par ^ 1];
-            a.Xi = Xi;
-            a.U = U;
-            a.acc_out = h.acc_out ? h.acc_out + t0 : nullptr;
-            a.la_out = nullptr;
             a.eps = h.eps;
             a.damping = p.damping;
             a.tol_abs = p.tol_abs;
@@ -2220,32 +2362,32 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
             a.gy = h.gy;
             a.W = h.W;
             a.hw = h.hw;
-            a.a_keep = h.W;  // used by multi-tile windows only (reduce keeps A, B)
+            a.a_keep = h.W;  // the reduce pass keeps A, B over the probe products it has consumed
             a.b_keep = h.hw;
             a.lpx = h.lpx;
             a.lpy = h.lpy;
+            a.lp_nslab = h.nslab;
+            a.lp_stride = L;
             a.agg_a = h.agg_a;
             a.agg_b = h.agg_b;
             a.prev = h.prev;
-            a.final_cols = &sy->skip_cols;
+            a.final_cols = &sy->skip_cols;  // stays 0: the window starts at the front
             a.tile_smax = h.tile_smax;
-            a.prev_dec = h.prev_dec;
             a.hyst = 1e-6;
+            a.win = b->slides + i;
             if (b->refine) {
                 a.dec = h.rw.dec;
                 a.refine_on = h.rw.cnt + 2;
             }
-            rp->targs[par].push_back(a);
+            bt->targs[par].push_back(a);
             RefineArgs ra{};
-            ra.x0 = s0;
-            ra.S_in = h.buf[par];
-            ra.Xi = Xi;
-            ra.U = U;
             ra.Y = h.Y;
             ra.gx = h.gx;
             ra.gy = h.gy;
             ra.lpx = h.lpx;
             ra.lpy = h.lpy;
+            ra.lp_nslab = h.nslab;
+            ra.lp_stride = L;
             ra.eps = h.eps;
             ra.band = h.band;
             ra.skip = skip;
@@ -2256,36 +2398,16 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
             ra.iter_dev = &sy->last_iter;
             ra.hyst = 1e-6;
             ra.final_cols = &sy->skip_cols;
-            rp->rargs[par].push_back(ra);
+            bt->rargs[par].push_back(ra);
         }
-        {
-            RefineWork w = h.rw;
-            w.T = L;
-            rp->rws.push_back(w);
-            rp->rtgs.push_back(h.rtg);
-        }
-        RoundPlan::Shape sh{};
-        sh.kind = h.tgt->kind;
-        sh.nT = nT;
-        sh.L = L;
-        sh.lds = tile_lds_bytes<float>(D, TC, true);
-        sh.dpt = (int)(h.Dp / 16);
-        sh.ns = h.ns;
-        sh.ncb = (int)((L + GLM_CB - 1) / GLM_CB);
-        sh.gcb = (int)((L + GAUSS_CB - 1) / GAUSS_CB);
-        rp->shape.push_back(sh);
-        // products
+        bt->rws.push_back(h.rw);
+        bt->rtgs.push_back(h.rtg);
         ColSrc<float> s0src{};
-        s0src.V = nullptr;
-        s0src.x0 = s0;
-        s0src.Z = nullptr;
         s0src.keys[0] = key;
         s0src.K = 1;
         s0src.D = D;
-        s0src.Tn = L;
         ColSrc<float> s1src = s0src;
         s1src.V = h.Y;
-        s1src.x0 = nullptr;
         s1src.Z = h.W;
         if (h.tgt->kind == TGT_LOGISTIC) {
             const GlmData& g = h.tgt->glm;
@@ -2294,7 +2416,6 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
             for (int v = 0; v < 3; ++v) {
                 GlmPassArgs ga{};
                 ga.src = v < 2 ? s0src : s1src;
-                if (v < 2) ga.src.V = h.buf[v];
                 ga.Xrm = g.Xrm;
                 ga.y32 = g.y;
                 ga.N = g.N;
@@ -2302,7 +2423,7 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
                 ga.part = h.part;
                 ga.llpart = llpart;
                 ga.skip = skip;
-                rp->gpass[v].push_back(ga);
+                bt->gpass[v].push_back(ga);
                 GlmReduceArgs gr{};
                 gr.src = ga.src;
                 gr.part = h.part;
@@ -2313,119 +2434,104 @@ static bool batch_round_plan(pmcmc_batch* b, int64_t r, RoundPlan* rp) {
                 gr.pass = v < 2 ? 0 : 1;
                 gr.inv_s2 = inv_s2;
                 gr.eps = h.eps;
-                gr.Xi = Xi;
                 gr.gout = v < 2 ? h.gx : h.gy;
                 gr.hout = h.hw;
                 gr.lpout = v < 2 ? h.lpx : h.lpy;
                 gr.Yout = v < 2 ? h.Y : nullptr;
                 gr.Wout = v < 2 ? h.W : nullptr;
                 gr.skip = skip;
-                rp->gred[v].push_back(gr);
-                rp->qpass[v].push_back(GaussPassArgs{});
+                bt->gred[v].push_back(gr);
+                bt->qpass[v].push_back(GaussPassArgs{});
             }
         } else {
             const GaussData& g = h.tgt->gauss;
             for (int v = 0; v < 3; ++v) {
                 GaussPassArgs qa{};
                 qa.src = v < 2 ? s0src : s1src;
-                if (v < 2) qa.src.V = h.buf[v];
                 qa.L32 = g.L32;
                 qa.ldL = (int)g.Dp;
                 qa.mu = g.mu32;
                 qa.pass = v < 2 ? 0 : 1;
                 qa.eps = (float)h.eps;
-                qa.Xi = Xi;
                 qa.gout = v < 2 ? h.gx : h.gy;
                 qa.hout = h.hw;
                 qa.lpout = v < 2 ? h.lpx : h.lpy;
                 qa.Yout = v < 2 ? h.Y : nullptr;
                 qa.Wout = v < 2 ? h.W : nullptr;
                 qa.skip = skip;
-                qa.src.col_lo = &sy->skip_cols;  // columns below it are final
-                rp->qpass[v].push_back(qa);
-                rp->gpass[v].push_back(GlmPassArgs{});
-                rp->gred[v].push_back(GlmReduceArgs{});
+                qa.lp_stride = L;
+                bt->qpass[v].push_back(qa);
+                bt->gpass[v].push_back(GlmPassArgs{});
+                bt->gred[v].push_back(GlmReduceArgs{});
             }
         }
-        BInit in{};
-        in.x0 = s0;
-        in.S0 = h.buf[0];
-        in.D = D;
-        in.n = D * L;
+        SlideInit in{};
+        in.win = b->slides + i;
         in.sy = sy;
         in.rcnt = h.rw.cnt;
         in.prev_dec = h.prev_dec;
-        in.L = L;
-        rp->inits.push_back(in);
+        in.T = h.T;
+        bt->inits.push_back(in);
         BCopy cp{};
         cp.b0 = h.buf[0];
         cp.b1 = h.buf[1];
-        cp.dst = h.S_out + (size_t)t0 * D;
-        cp.n = D * L;
+        cp.dst = h.S_out;
+        cp.n = D * h.T;
         cp.sy = sy;
-        rp->copies.push_back(cp);
+        bt->copies.push_back(cp);
+        WinDesc wd{};
+        wd.D = D;
+        wd.T = h.T;
+        wd.L = L;
+        wd.TC = TC;
+        wd.kind = h.tgt->kind == TGT_LOGISTIC ? 0 : 1;
+        wd.buf[0] = h.buf[0];
+        wd.buf[1] = h.buf[1];
+        wd.x0 = h.x0;
+        wd.Xi = h.Xi;
+        wd.U = h.U;
+        wd.acc_out = h.acc_out;
+        wd.prev_dec = h.prev_dec;
+        wd.win = b->slides + i;
+        wd.sy = sy;
+        wd.span = b->spans + i;
+        bt->wins.push_back(wd);  // table addresses filled in once the tables have a device home
     }
-    return !rp->active.empty();
 }
 
-// Launch class of a Gaussian chain's Lambda pass.  With few live Gaussian
-// chains an iteration is bound by its launches, not its flops, so the widths
-// are bucketed into three classes (64, 128, 256 rows; the narrower chains pad
-// with zero tiles); with many, every width keeps its own class.
-static int gauss_class(int dpt, int n_live_gauss) {
-    if (n_live_gauss > 32) return dpt;
-    return dpt <= 4 ? 4 : (dpt <= 8 ? 8 : 16);
-}
-
-static void build_items(const RoundPlan& rp, const std::vector<char>& live, Items* it) {
-    int n_live_gauss = 0;
-    for (int k = 0; k < (int)rp.shape.size(); ++k)
-        if (live[k] && rp.shape[k].kind != TGT_LOGISTIC) ++n_live_gauss;
-    for (int k = 0; k < (int)rp.shape.size(); ++k) {
+static void build_items(const pmcmc_batch* b, const std::vector<char>& live, Items* it) {
+    const int n = (int)b->ch.size();
+    for (int k = 0; k < n; ++k) {
         if (!live[k]) continue;
-        const RoundPlan::Shape& sh = rp.shape[k];
-        const int lc = lds_class(sh.lds);
-        if (sh.nT > 1) {
-            for (int t = 0; t < sh.nT; ++t) it->red[lc].push_back(make_int2(k, t));
-            it->multi.push_back(k);
-            if (sh.lds > it->lds_multi[lc]) it->lds_multi[lc] = sh.lds;
-        } else {
-            it->single[lc].push_back(make_int2(k, 0));
-            if (sh.lds > it->lds_single[lc]) it->lds_single[lc] = sh.lds;
-        }
-        const RefineWork& rw = rp.rws[k];
+        const BatchChain& h = b->ch[k];
+        const int64_t span = std::min<int64_t>(h.span, h.L);
+        const int TC = (int)std::min<int64_t>(h.TC, h.L);
+        const int nT = (int)((span + TC - 1) / TC);
+        const size_t lds = tile_lds_bytes<float>(h.D, TC, true);
+        const int lc = lds_class(lds);
+        for (int t = 0; t < nT; ++t) it->red[lc].push_back(make_int2(k, t));
+        it->multi.push_back(k);
+        it->wins.push_back(k);
+        if (lds > it->lds_multi[lc]) it->lds_multi[lc] = lds;
         it->r_all.push_back(k);
-        it->r_maxT = std::max(it->r_maxT, rw.T);
-        if (rw.kind == 0) {
+        it->r_maxT = std::max(it->r_maxT, span);
+        const int dpt = (int)(h.Dp / 16);
+        if (h.tgt->kind == TGT_LOGISTIC) {
             it->r_glm.push_back(k);
-            it->r_split_glm = std::max(it->r_split_glm, rw.nsplit);
+            it->r_split_glm = std::max(it->r_split_glm, h.rw.nsplit);
+            const int ncb = (int)((span + GLM_CB - 1) / GLM_CB);
+            for (int cb = 0; cb < ncb; ++cb)
+                for (int sp = 0; sp < h.ns; ++sp) it->glm[dpt].push_back(make_int4(k, cb, sp, 0));
+            for (int64_t blk = 0; blk < (span + 3) / 4; ++blk) it->glm_red.push_back(make_int2(k, (int)blk));
         } else {
             it->r_gauss.push_back(k);
-            it->r_split_gauss = std::max(it->r_split_gauss, rw.nsplit);
-        }
-        if (sh.kind == TGT_LOGISTIC) {
-            for (int cb = 0; cb < sh.ncb; ++cb)
-                for (int sp = 0; sp < sh.ns; ++sp) it->glm[sh.dpt].push_back(make_int4(k, cb, sp, 0));
-            for (int64_t blk = 0; blk < (sh.L + 3) / 4; ++blk) it->glm_red.push_back(make_int2(k, (int)blk));
-        } else {
-            const int cls = gauss_class(sh.dpt, n_live_gauss);
-            for (int cb = 0; cb < sh.gcb; ++cb) it->gauss[cls].push_back(make_int2(k, cb));
+            it->r_split_gauss = std::max(it->r_split_gauss, h.rw.nsplit);
+            const int gcb = (int)((span + GAUSS_CB - 1) / GAUSS_CB);
+            for (int cb = 0; cb < gcb; ++cb)
+                for (int sl = 0; sl < h.nslab; ++sl) it->gslab.push_back(make_int4(k, cb, sl, 0));
         }
     }
 }
-
-namespace {
-// Host staging of a set of device tables, each 256-byte aligned.
-struct Pack {
-    std::vector<char> host;
-    size_t put(const void* src, size_t bytes) {
-        const size_t o = align256(host.size());
-        host.resize(o + bytes);
-        if (bytes) memcpy(host.data() + o, src, bytes);
-        return o;
-    }
-};
-}  // namespace
 
 static int upload(pmcmc_ctx* c, hipStream_t s, const Pack& pk, void** buf, size_t* cap) {
     if (pk.host.size() > *cap) {
@@ -2440,11 +2546,44 @@ static int upload(pmcmc_ctx* c, hipStream_t s, const Pack& pk, void** buf, size_
     return PMCMC_OK;
 }
 
+// The host's window-length policy at a synchronisation.  A chain starts with
+// the full window.  One that after 24 iterations of a window segment neither
+// reduced its metric 4x over the last chunk nor moved its front 4 columns per
+// iteration is converging causally (about one column per iteration, the
+// Jacobian estimate missing the coupling of its dynamics): its window shrinks
+// to a few tiles past the front, where all its progress happens.  A narrowed
+// chain whose front moves 4 or more columns per iteration widens 4x again.
+// Returns true when a length changed.
+static bool batch_policy(pmcmc_batch* b, int i) {
+    BatchChain& h = b->ch[i];
+    const SyncState& hs = b->hsync[i];
+    const SlideState& sl = b->hslide[i];
+    const int k = hs.last_iter;
+    const int dk = k - h.prev_iter;
+    const int64_t prog = sl.front - h.prev_front;
+    const int old = h.span;
+    if (h.span >= h.L && h.narrow < h.L) {
+        const int seg = k + 1 - sl.seg_start;
+        const bool falling = hs.last_metric < h.prev_metric * 0.25;  // false for NaN
+        if (seg >= 24 && dk >= 8 && !falling && prog < 4 * (int64_t)dk) h.span = h.narrow;
+    } else if (h.span < h.L && dk > 0 && prog >= 4 * (int64_t)dk) {
+        const int64_t TC = std::min<int64_t>(h.TC, h.L);
+        int64_t ns = std::min<int64_t>(4 * (int64_t)h.span, h.L);
+        ns = std::min<int64_t>((ns + TC - 1) / TC * TC, h.L);
+        h.span = (int)ns;
+    }
+    h.prev_front = sl.front;
+    h.prev_iter = k;
+    h.prev_metric = hs.last_metric;
+    return h.span != old;
+}
+
 extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
     if (!b) return PMCMC_ERR_ARGUMENT;
-    // PMCMC_BATCH_PROGRESS=1: one line per window round on stderr
+    // PMCMC_BATCH_PROGRESS=1: a line per rebuild of the work items on stderr
     const char* pe = getenv("PMCMC_BATCH_PROGRESS");
     const bool progress = pe && pe[0] == '1';
+    const auto t_start = std::chrono::steady_clock::now();
     pmcmc_ctx* c = b->ctx;
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
@@ -2453,194 +2592,242 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
         h.iters = 0;
         h.converged = true;
         h.metric = 0;
+        h.span = (int)h.L;
+        h.prev_front = 0;
+        h.prev_iter = -1;
+        h.prev_metric = INFINITY;
+        h.stopped = false;
     }
-    for (int64_t r = 0;; ++r) {
-        RoundPlan rp;
-        if (!batch_round_plan(b, r, &rp)) break;
-        const int na = (int)rp.active.size();
-        // per-chain descriptor tables: uploaded once per round (pageable staging
-        // stays alive until the round's last synchronisation)
-        Pack tp;
-        size_t o_t[2], o_gp[3], o_gr[3], o_qp[3];
-        for (int v = 0; v < 2; ++v) o_t[v] = tp.put(rp.targs[v].data(), rp.targs[v].size() * sizeof(TileArgs<float>));
-        for (int v = 0; v < 3; ++v) {
-            o_gp[v] = tp.put(rp.gpass[v].data(), rp.gpass[v].size() * sizeof(GlmPassArgs));
-            o_gr[v] = tp.put(rp.gred[v].data(), rp.gred[v].size() * sizeof(GlmReduceArgs));
-            o_qp[v] = tp.put(rp.qpass[v].data(), rp.qpass[v].size() * sizeof(GaussPassArgs));
+    // descriptor tables: uploaded once; the window kernel keeps their
+    // window-dependent fields current
+    BatchTables bt;
+    batch_tables(b, &bt);
+    Pack tp;
+    size_t o_t[2], o_gp[3], o_gr[3], o_qp[3], o_ra[2];
+    for (int v = 0; v < 2; ++v) o_t[v] = tp.put(bt.targs[v].data(), n * sizeof(TileArgs<float>));
+    for (int v = 0; v < 3; ++v) {
+        o_gp[v] = tp.put(bt.gpass[v].data(), n * sizeof(GlmPassArgs));
+        o_gr[v] = tp.put(bt.gred[v].data(), n * sizeof(GlmReduceArgs));
+        o_qp[v] = tp.put(bt.qpass[v].data(), n * sizeof(GaussPassArgs));
+    }
+    const size_t o_rt = tp.put(bt.rtgs.data(), n * sizeof(RefineTarget));
+    const size_t o_rw = tp.put(bt.rws.data(), n * sizeof(RefineWork));
+    for (int v = 0; v < 2; ++v) o_ra[v] = tp.put(bt.rargs[v].data(), n * sizeof(RefineArgs));
+    const size_t o_init = tp.put(bt.inits.data(), n * sizeof(SlideInit));
+    const size_t o_copy = tp.put(bt.copies.data(), n * sizeof(BCopy));
+    const size_t o_win = tp.put(bt.wins.data(), n * sizeof(WinDesc));
+    if (tp.host.size() > b->tables_cap) {
+        HIPCHK(c, hipStreamSynchronize(s));
+        if (b->tables) HIPCHK(c, hipFree(b->tables));
+        b->tables = nullptr;
+        b->tables_cap = 0;
+        HIPCHK(c, hipMalloc(&b->tables, tp.host.size()));
+        b->tables_cap = tp.host.size();
+    }
+    char* T0 = (char*)b->tables;
+    auto at = [&](size_t o) { return (void*)(T0 + o); };
+    {
+        WinDesc* wh = reinterpret_cast<WinDesc*>(tp.host.data() + o_win);
+        for (int i = 0; i < n; ++i) {
+            for (int v = 0; v < 2; ++v) {
+                wh[i].ta[v] = (TileArgs<float>*)at(o_t[v]) + i;
+                wh[i].ra[v] = (RefineArgs*)at(o_ra[v]) + i;
+            }
+            for (int v = 0; v < 3; ++v) {
+                wh[i].gp[v] = (GlmPassArgs*)at(o_gp[v]) + i;
+                wh[i].gr[v] = (GlmReduceArgs*)at(o_gr[v]) + i;
+                wh[i].qp[v] = (GaussPassArgs*)at(o_qp[v]) + i;
+            }
+            wh[i].rw = (RefineWork*)at(o_rw) + i;
         }
-        const size_t o_rt = tp.put(rp.rtgs.data(), rp.rtgs.size() * sizeof(RefineTarget));
-        const size_t o_rw = tp.put(rp.rws.data(), rp.rws.size() * sizeof(RefineWork));
-        size_t o_ra[2];
-        for (int v = 0; v < 2; ++v) o_ra[v] = tp.put(rp.rargs[v].data(), rp.rargs[v].size() * sizeof(RefineArgs));
-        const size_t o_init = tp.put(rp.inits.data(), rp.inits.size() * sizeof(BInit));
-        const size_t o_copy = tp.put(rp.copies.data(), rp.copies.size() * sizeof(BCopy));
-        int rc = upload(c, s, tp, &b->tables, &b->tables_cap);
-        if (rc != PMCMC_OK) return rc;
-        char* T0 = (char*)b->tables;
-        auto at = [&](size_t o) { return (void*)(T0 + o); };
-        const dim3 ig(64, (unsigned)na);
-        hipLaunchKernelGGL(batch_init_kernel, ig, dim3(256), 0, s, (const BInit*)at(o_init));
-        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipMemcpyAsync(b->tables, tp.host.data(), tp.host.size(), hipMemcpyHostToDevice, s));
+    for (int i = 0; i < n; ++i) b->hspans[i] = b->ch[i].span;
+    HIPCHK(c, hipMemcpyAsync(b->spans, b->hspans, n * sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(batch_slide_init_kernel, dim3(16, (unsigned)n), dim3(256), 0, s, (const SlideInit*)at(o_init));
+    HIPCHK(c, hipGetLastError());
 
-        // work items over the live slots; rebuilt (and the graph re-captured)
-        // whenever an eighth of them has converged
-        std::vector<char> live(na, 1);
-        int nlive = na, built_live = 0;
-        Items it;
-        Pack ip;
-        size_t o_red[NLDS] = {0, 0, 0}, o_single[NLDS] = {0, 0, 0}, o_multi = 0, o_glmred = 0, o_gi[17], o_qi[17];
-        size_t o_rall = 0, o_rglm = 0, o_rgauss = 0;
-        hipGraphExec_t exec = nullptr;
-        hipGraph_t graph = nullptr;
-        auto drop_graph = [&]() {
-            if (exec) hipGraphExecDestroy(exec);
-            if (graph) hipGraphDestroy(graph);
-            exec = nullptr;
-            graph = nullptr;
-        };
-        auto iteration = [&](int par) {
-            char* I0 = (char*)b->items;
-            auto ia = [&](size_t o) { return (void*)(I0 + o); };
-            for (int d = 1; d <= 16; ++d) {
-                launch_glm_pass_batched(d, b->jbf, (const GlmPassArgs*)at(o_gp[par]), (const int4*)ia(o_gi[d]),
-                                        (int)it.glm[d].size(), s);
-                launch_gauss_pass_batched(d, b->jbf, (const GaussPassArgs*)at(o_qp[par]), (const int2*)ia(o_qi[d]),
-                                          (int)it.gauss[d].size(), s);
-            }
-            launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[par]), (const int2*)ia(o_glmred),
-                                      (int)it.glm_red.size(), s);
-            for (int d = 1; d <= 16; ++d) {
-                launch_glm_pass_batched(d, b->jbf, (const GlmPassArgs*)at(o_gp[2]), (const int4*)ia(o_gi[d]),
-                                        (int)it.glm[d].size(), s);
-                launch_gauss_pass_batched(d, b->jbf, (const GaussPassArgs*)at(o_qp[2]), (const int2*)ia(o_qi[d]),
-                                          (int)it.gauss[d].size(), s);
-            }
-            launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[2]), (const int2*)ia(o_glmred),
-                                      (int)it.glm_red.size(), s);
-            if (b->refine) {
-                RefineBatch rb{};
-                rb.tgs = (const RefineTarget*)at(o_rt);
-                rb.ras = (const RefineArgs*)at(o_ra[par]);
-                rb.rws = (const RefineWork*)at(o_rw);
-                rb.all = (const int*)ia(o_rall);
-                rb.n_all = (int)it.r_all.size();
-                rb.glm = (const int*)ia(o_rglm);
-                rb.n_glm = (int)it.r_glm.size();
-                rb.gauss = (const int*)ia(o_rgauss);
-                rb.n_gauss = (int)it.r_gauss.size();
-                rb.max_T = it.r_maxT;
-                rb.max_split_glm = it.r_split_glm;
-                rb.max_split_gauss = it.r_split_gauss;
-                refine_decisions_batched(rb, par, s);
-            }
-            const TileArgs<float>* ta = (const TileArgs<float>*)at(o_t[par]);
-            for (int lc = 0; lc < NLDS; ++lc)
-                if (!it.single[lc].empty())
-                    hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_SINGLE>),
-                                       dim3((unsigned)it.single[lc].size()), dim3(PMCMC_NT), it.lds_single[lc], s, ta,
-                                       (const int2*)ia(o_single[lc]));
-            if (!it.multi.empty()) {
-                for (int lc = 0; lc < NLDS; ++lc)
-                    if (!it.red[lc].empty())
-                        hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_REDUCE>),
-                                           dim3((unsigned)it.red[lc].size()), dim3(PMCMC_NT), it.lds_multi[lc], s, ta,
-                                           (const int2*)ia(o_red[lc]));
-                hipLaunchKernelGGL(batch_agg_scan_kernel, dim3((unsigned)it.multi.size()), dim3(256), 0, s, ta,
-                                   (const int*)ia(o_multi));
-                for (int lc = 0; lc < NLDS; ++lc)
-                    if (!it.red[lc].empty())
-                        hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_APPLY>),
-                                           dim3((unsigned)it.red[lc].size()), dim3(PMCMC_NT), it.lds_multi[lc], s, ta,
-                                           (const int2*)ia(o_red[lc]));
-            }
-        };
-        auto rebuild = [&]() -> int {
+    std::vector<char> live(n, 1);
+    int nlive = n, built_live = 0;
+    Items it;
+    Pack ip;
+    size_t o_red[NLDS] = {0, 0, 0}, o_multi = 0, o_wins = 0, o_glmred = 0, o_gi[17], o_qs = 0;
+    size_t o_rall = 0, o_rglm = 0, o_rgauss = 0;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    auto drop_graph = [&]() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    };
+    auto iteration = [&](int par) {
+        char* I0 = (char*)b->items;
+        auto ia = [&](size_t o) { return (void*)(I0 + o); };
+        hipLaunchKernelGGL(batch_window_kernel, dim3(4, (unsigned)it.wins.size()), dim3(256), 0, s,
+                           (const WinDesc*)at(o_win), (const int*)ia(o_wins), par);
+        for (int d = 1; d <= 16; ++d)
+            launch_glm_pass_batched(d, b->jbf, (const GlmPassArgs*)at(o_gp[par]), (const int4*)ia(o_gi[d]),
+                                    (int)it.glm[d].size(), s);
+        launch_gauss_slab_batched(b->jbf, (const GaussPassArgs*)at(o_qp[par]), (const int4*)ia(o_qs),
+                                  (int)it.gslab.size(), s);
+        launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[par]), (const int2*)ia(o_glmred),
+                                  (int)it.glm_red.size(), s);
+        for (int d = 1; d <= 16; ++d)
+            launch_glm_pass_batched(d, b->jbf, (const GlmPassArgs*)at(o_gp[2]), (const int4*)ia(o_gi[d]),
+                                    (int)it.glm[d].size(), s);
+        launch_gauss_slab_batched(b->jbf, (const GaussPassArgs*)at(o_qp[2]), (const int4*)ia(o_qs),
+                                  (int)it.gslab.size(), s);
+        launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[2]), (const int2*)ia(o_glmred),
+                                  (int)it.glm_red.size(), s);
+        if (b->refine) {
+            RefineBatch rb{};
+            rb.tgs = (const RefineTarget*)at(o_rt);
+            rb.ras = (const RefineArgs*)at(o_ra[par]);
+            rb.rws = (const RefineWork*)at(o_rw);
+            rb.all = (const int*)ia(o_rall);
+            rb.n_all = (int)it.r_all.size();
+            rb.glm = (const int*)ia(o_rglm);
+            rb.n_glm = (int)it.r_glm.size();
+            rb.gauss = (const int*)ia(o_rgauss);
+            rb.n_gauss = (int)it.r_gauss.size();
+            rb.max_T = it.r_maxT;
+            rb.max_split_glm = it.r_split_glm;
+            rb.max_split_gauss = it.r_split_gauss;
+            refine_decisions_batched(rb, par, s);
+        }
+        const TileArgs<float>* ta = (const TileArgs<float>*)at(o_t[par]);
+        for (int lc = 0; lc < NLDS; ++lc)
+            if (!it.red[lc].empty())
+                hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_REDUCE>),
+                                   dim3((unsigned)it.red[lc].size()), dim3(PMCMC_NT), it.lds_multi[lc], s, ta,
+                                   (const int2*)ia(o_red[lc]));
+        hipLaunchKernelGGL(batch_agg_scan_kernel, dim3((unsigned)it.multi.size()), dim3(256), 0, s, ta,
+                           (const int*)ia(o_multi));
+        for (int lc = 0; lc < NLDS; ++lc)
+            if (!it.red[lc].empty())
+                hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_APPLY>),
+                                   dim3((unsigned)it.red[lc].size()), dim3(PMCMC_NT), it.lds_multi[lc], s, ta,
+                                   (const int2*)ia(o_red[lc]));
+    };
+    auto rebuild = [&]() -> int {
+        drop_graph();
+        it = Items();
+        build_items(b, live, &it);
+        ip = Pack();
+        for (int lc = 0; lc < NLDS; ++lc) o_red[lc] = ip.put(it.red[lc].data(), it.red[lc].size() * sizeof(int2));
+        o_multi = ip.put(it.multi.data(), it.multi.size() * sizeof(int));
+        o_wins = ip.put(it.wins.data(), it.wins.size() * sizeof(int));
+        o_rall = ip.put(it.r_all.data(), it.r_all.size() * sizeof(int));
+        o_rglm = ip.put(it.r_glm.data(), it.r_glm.size() * sizeof(int));
+        o_rgauss = ip.put(it.r_gauss.data(), it.r_gauss.size() * sizeof(int));
+        o_glmred = ip.put(it.glm_red.data(), it.glm_red.size() * sizeof(int2));
+        for (int d = 0; d < 17; ++d) o_gi[d] = ip.put(it.glm[d].data(), it.glm[d].size() * sizeof(int4));
+        o_qs = ip.put(it.gslab.data(), it.gslab.size() * sizeof(int4));
+        const int rc2 = upload(c, s, ip, &b->items, &b->items_cap);
+        if (rc2 != PMCMC_OK) return rc2;
+        built_live = nlive;
+        if (!b->use_graph) return PMCMC_OK;
+        HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        iteration(0);
+        iteration(1);
+        const hipError_t ec = hipStreamEndCapture(s, &graph);
+        if (ec != hipSuccess) return fail(c, PMCMC_ERR_HIP, "batch: graph capture failed: %s", hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
             drop_graph();
-            it = Items();
-            build_items(rp, live, &it);
-            ip = Pack();
-            for (int lc = 0; lc < NLDS; ++lc) {
-                o_red[lc] = ip.put(it.red[lc].data(), it.red[lc].size() * sizeof(int2));
-                o_single[lc] = ip.put(it.single[lc].data(), it.single[lc].size() * sizeof(int2));
-            }
-            o_multi = ip.put(it.multi.data(), it.multi.size() * sizeof(int));
-            o_rall = ip.put(it.r_all.data(), it.r_all.size() * sizeof(int));
-            o_rglm = ip.put(it.r_glm.data(), it.r_glm.size() * sizeof(int));
-            o_rgauss = ip.put(it.r_gauss.data(), it.r_gauss.size() * sizeof(int));
-            o_glmred = ip.put(it.glm_red.data(), it.glm_red.size() * sizeof(int2));
-            for (int d = 0; d < 17; ++d) {
-                o_gi[d] = ip.put(it.glm[d].data(), it.glm[d].size() * sizeof(int4));
-                o_qi[d] = ip.put(it.gauss[d].data(), it.gauss[d].size() * sizeof(int2));
-            }
-            const int rc2 = upload(c, s, ip, &b->items, &b->items_cap);
-            if (rc2 != PMCMC_OK) return rc2;
-            built_live = nlive;
-            if (!b->use_graph) return PMCMC_OK;
-            HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            iteration(0);
-            iteration(1);
-            const hipError_t ec = hipStreamEndCapture(s, &graph);
-            if (ec != hipSuccess) return fail(c, PMCMC_ERR_HIP, "batch: graph capture failed: %s", hipGetErrorString(ec));
-            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            if (ei != hipSuccess) {
-                drop_graph();
-                return fail(c, PMCMC_ERR_HIP, "batch: graph instantiate failed: %s", hipGetErrorString(ei));
-            }
-            return PMCMC_OK;
-        };
-        rc = rebuild();
-        if (rc != PMCMC_OK) return rc;
-        int k = 0, chunk = 2;
-        while (k < b->p.maxiter && nlive > 0) {
-            int m = chunk;  // even; only the last chunk before maxiter can be odd
-            if (m > b->p.maxiter - k) m = b->p.maxiter - k;
-            for (int j = 0; j < m / 2; ++j) {
-                if (b->use_graph) {
-                    const hipError_t el = hipGraphLaunch(exec, s);
-                    if (el != hipSuccess) {
-                        drop_graph();
-                        return fail(c, PMCMC_ERR_HIP, "batch: graph launch failed");
-                    }
-                } else {
-                    iteration(0);
-                    iteration(1);
-                }
-            }
-            if (m & 1) iteration(0);  // k is even here, so the buffers are at parity 0
-            k += m;
-            HIPCHK(c, hipGetLastError());
-            HIPCHK(c, hipMemcpyAsync(b->hsync, b->syncs, n * sizeof(SyncState), hipMemcpyDeviceToHost, s));
-            HIPCHK(c, hipStreamSynchronize(s));
-            nlive = 0;
-            for (int q = 0; q < na; ++q) {
-                live[q] = b->hsync[rp.active[q]].converged ? 0 : 1;
-                nlive += live[q];
-            }
-            if (chunk < 32) chunk <<= 1;
-            if (nlive > 0 && k < b->p.maxiter && 8 * nlive <= 7 * built_live) {
-                rc = rebuild();
-                if (rc != PMCMC_OK) {
+            return fail(c, PMCMC_ERR_HIP, "batch: graph instantiate failed: %s", hipGetErrorString(ei));
+        }
+        return PMCMC_OK;
+    };
+    int rc = rebuild();
+    if (rc != PMCMC_OK) return rc;
+    // iteration caps: maxiter per window's worth of columns, and maxiter
+    // iterations without the front moving
+    int k = 0, chunk = 2, rebuilds = 1;
+    while (nlive > 0) {
+        const int m = chunk;  // even
+        for (int j = 0; j < m / 2; ++j) {
+            if (b->use_graph) {
+                const hipError_t el = hipGraphLaunch(exec, s);
+                if (el != hipSuccess) {
                     drop_graph();
-                    return rc;
+                    return fail(c, PMCMC_ERR_HIP, "batch: graph launch failed");
                 }
+            } else {
+                iteration(0);
+                iteration(1);
             }
         }
-        drop_graph();
-        hipLaunchKernelGGL(batch_copy_kernel, ig, dim3(256), 0, s, (const BCopy*)at(o_copy));
+        k += m;
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipMemcpyAsync(b->hsync, b->syncs, n * sizeof(SyncState), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipMemcpyAsync(b->hslide, b->slides, n * sizeof(SlideState), hipMemcpyDeviceToHost, s));
         HIPCHK(c, hipStreamSynchronize(s));
-        for (int ci : rp.active) {
-            const SyncState& hs = b->hsync[ci];
-            BatchChain& h = b->ch[ci];
-            h.iters += hs.converged ? hs.conv_iter + 1 : hs.last_iter + 1;
-            h.converged = h.converged && hs.converged;
-            h.metric = hs.last_metric;
+        nlive = 0;
+        bool changed = false, stop_any = false;
+        for (int q = 0; q < n; ++q) {
+            if (!live[q]) continue;
+            SyncState& hs = b->hsync[q];
+            BatchChain& h = b->ch[q];
+            if (hs.converged) {
+                live[q] = 0;
+                continue;
+            }
+            const int64_t windows = (h.T + h.L - 1) / h.L;
+            const bool over = (int64_t)hs.last_iter + 1 >= (int64_t)b->p.maxiter * windows;
+            const bool stalled = hs.last_iter - std::max(b->hslide[q].last_adv_iter, b->hslide[q].seg_start - 1) >=
+                                 b->p.maxiter;
+            if (over || stalled) {
+                // give up on the chain: mark it done on the device (the host copy is
+                // current and no kernel runs now)
+                h.stopped = true;
+                hs.converged = 1;
+                hs.conv_iter = hs.last_iter;
+                HIPCHK(c, hipMemcpyAsync(b->syncs + q, &hs, sizeof(SyncState), hipMemcpyHostToDevice, s));
+                live[q] = 0;
+                stop_any = true;
+                continue;
+            }
+            ++nlive;
+            changed = batch_policy(b, q) || changed;
         }
-        if (progress) {
-            fprintf(stderr, "[pmcmc batch] window round %lld: %d chains, %d iterations\n", (long long)r, na, k);
-            fflush(stderr);
+        if (chunk < 32) chunk <<= 1;
+        if (nlive > 0 && (changed || stop_any || 8 * nlive <= 7 * built_live)) {
+            for (int q = 0; q < n; ++q) b->hspans[q] = b->ch[q].span;
+            HIPCHK(c, hipMemcpyAsync(b->spans, b->hspans, n * sizeof(int), hipMemcpyHostToDevice, s));
+            rc = rebuild();
+            if (rc != PMCMC_OK) {
+                drop_graph();
+                return rc;
+            }
+            ++rebuilds;
+            if (progress) {
+                const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+                size_t nt = 0;
+                for (int lc = 0; lc < NLDS; ++lc) nt += it.red[lc].size();
+                fprintf(stderr, "[pmcmc batch] iteration %d: %d live chains, %zu tiles, %.3f s\n", k, nlive, nt, el);
+                fflush(stderr);
+            }
         }
+    }
+    drop_graph();
+    hipLaunchKernelGGL(batch_copy_kernel, dim3(64, (unsigned)n), dim3(256), 0, s, (const BCopy*)at(o_copy));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(b->hsync, b->syncs, n * sizeof(SyncState), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(b->hslide, b->slides, n * sizeof(SlideState), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    for (int i = 0; i < n; ++i) {
+        const SyncState& hs = b->hsync[i];
+        BatchChain& h = b->ch[i];
+        h.iters = hs.last_iter + 1;
+        h.converged = hs.converged && !h.stopped;
+        h.metric = hs.last_metric;
+    }
+    if (progress) {
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        fprintf(stderr, "[pmcmc batch] done: %d iterations, %d rebuilds, %.3f s\n", k, rebuilds, el);
+        fflush(stderr);
     }
     if (infos)
         for (int i = 0; i < n; ++i) {

@@ -79,6 +79,13 @@ __device__ inline double state_at(const RefineArgs& a, int64_t t, int64_t d, int
 
 __device__ inline bool skipped(const RefineArgs& a) { return a.skip && *(volatile const int*)a.skip; }
 
+// log-density of column t from its slab partials, summed in slab order
+__device__ inline double lp_at(const RefineArgs& a, const double* lp, int64_t t) {
+    double v = lp[t];
+    for (int s = 1; s < a.lp_nslab; ++s) v += lp[(size_t)s * a.lp_stride + t];
+    return v;
+}
+
 __device__ inline int iter_of(const RefineArgs& a) { return a.iter_dev ? *a.iter_dev + 1 : a.iter; }
 
 // log alpha of column t from the FP32 products (one wave); lists t when it is
@@ -126,20 +133,24 @@ __device__ void refine_flag_column(const RefineArgs& a, int8_t* dec, int* cnt, i
     qxy = wave_sum_r(qxy);
     if (lane == 0) {
         const double c_norm = ((double)D / 2.0) * log(4.0 * 3.14159265358979323846 * eps);
-        const double la = (a.lpy[t] + (-0.5 * qxy / (2.0 * eps) - c_norm)) -
-                          (a.lpx[t] + (-0.5 * qyx / (2.0 * eps) - c_norm));
+        const double la = (lp_at(a, a.lpy, t) + (-0.5 * qxy / (2.0 * eps) - c_norm)) -
+                          (lp_at(a, a.lpx, t) + (-0.5 * qyx / (2.0 * eps) - c_norm));
         const double lu = log((double)a.U[t]);
         dec[t] = -1;
         if (fabs(la - lu) < a.band) {
-            const int j = atomicAdd(&cnt[par], 1);
-            if (j < lmax) list[j] = (int)t;
+            if (list) {
+                const int j = atomicAdd(&cnt[par], 1);
+                if (j < lmax) list[j] = (int)t;
+            } else {
+                dec[t] = -2;  // flagged for the caller's own FP64 decision
+            }
         }
     }
 }
 
 // Columns FLAG_COLS blk .. FLAG_COLS (blk + 1) - 1, one wave per column at a
 // time (a workgroup per handful of columns made the launch dispatch-bound).
-static constexpr int FLAG_COLS = 32;
+static constexpr int FLAG_COLS = 8;
 __device__ void refine_flag_body(const RefineArgs& a, int8_t* dec, int* cnt, int* list, int lmax, int par, int64_t D,
                                  int64_t T, int64_t blk) {
     if (skipped(a)) return;
@@ -380,6 +391,86 @@ __global__ __launch_bounds__(RNT) void refine_finish_batched(const RefineTarget*
     refine_finish_body(tgs[c], ras[c], rws[c], par, phase, blockIdx.x, gridDim.x);
 }
 
+// Dense-Gaussian chains of a batched solve: flag and FP64 decision in one
+// launch.  A workgroup owns DIRECT_COLS columns: it flags them from the FP32
+// products (one wave per column) and then decides each flagged column in
+// FP64 itself, the two D x D mat-vecs of Lambda (x and the FP64 proposal) one
+// row per thread, so no list, split partials or further launches are needed.
+// The same FP64 quantities as the split path, summed in another order.
+static constexpr int DIRECT_COLS = 8;
+__global__ __launch_bounds__(RNT) void refine_gauss_direct_batched(const RefineTarget* tgs, const RefineArgs* ras,
+                                                                  const RefineWork* rws, const int* ids, int par) {
+    const int c = ids[blockIdx.z];
+    const RefineArgs& a = ras[c];
+    const RefineWork& w = rws[c];
+    const RefineTarget& tg = tgs[c];
+    if (skipped(a)) return;
+    const int64_t T = w.T, D = w.D;
+    const int64_t t0 = (int64_t)blockIdx.x * DIRECT_COLS;
+    if (t0 >= T) return;
+    const int64_t t1 = t0 + DIRECT_COLS < T ? t0 + DIRECT_COLS : T;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // the gate (fixed during this launch), as in refine_flag_body
+    const bool on = w.cnt[2] != 0 || *a.last_metric <= a.gate || iter_of(a) >= a.gate_iters;
+    if (blockIdx.x == 0 && tid == 0 && on) w.cnt[2] = 1;
+    __shared__ double v[RMAXD], g0[RMAXD], yv[RMAXD], red[4];
+    __shared__ int flagged[DIRECT_COLS];
+    if (tid < DIRECT_COLS) flagged[tid] = 0;
+    __syncthreads();
+    const int64_t lo = a.final_cols ? (int64_t)*(volatile const int*)a.final_cols : 0;
+    for (int64_t t = t0 + wv; t < t1; t += RNT / 64) {
+        if (t < lo) continue;
+        if (!on) {
+            if (lane == 0) w.dec[t] = -1;
+            continue;
+        }
+        refine_flag_column(a, w.dec, nullptr, nullptr, 0, par, D, t, lane);
+        if (lane == 0) flagged[t - t0] = w.dec[t] == -2;  // the lane's own store
+    }
+    __syncthreads();
+    const double eps = a.eps;
+    const double sq2e = sqrt(2.0 * eps);
+    for (int64_t t = t0; t < t1; ++t) {
+        if (!flagged[t - t0]) continue;  // the same for every thread of the block
+        for (int64_t d = tid; d < D; d += RNT) v[d] = state_at(a, t, d, D) - (tg.mu64 ? tg.mu64[d] : 0.0);
+        __syncthreads();
+        double lpp = 0.0;
+        for (int64_t d = tid; d < D; d += RNT) {
+            double p = 0.0;
+            for (int64_t k = 0; k < D; ++k) p += tg.L64[d + k * D] * v[k];
+            const double g = -p;
+            g0[d] = g;
+            lpp += 0.5 * v[d] * g;
+            yv[d] = state_at(a, t, d, D) + eps * g + sq2e * (double)a.Xi[(size_t)t * D + d];
+        }
+        const double lp0 = block_sum(lpp, red);  // its barriers also order the reads of v before the rewrite
+        for (int64_t d = tid; d < D; d += RNT) v[d] = yv[d] - (tg.mu64 ? tg.mu64[d] : 0.0);
+        __syncthreads();
+        double lq = 0.0, qyx = 0.0, qxy = 0.0;
+        for (int64_t d = tid; d < D; d += RNT) {
+            double p = 0.0;
+            for (int64_t k = 0; k < D; ++k) p += tg.L64[d + k * D] * v[k];
+            const double g = -p;
+            lq += 0.5 * v[d] * g;
+            const double x = state_at(a, t, d, D), y = yv[d];
+            const double e1 = y - x - eps * g0[d];
+            const double e2 = x - y - eps * g;
+            qyx += e1 * e1;
+            qxy += e2 * e2;
+        }
+        const double lp1 = block_sum(lq, red);
+        qyx = block_sum(qyx, red);
+        qxy = block_sum(qxy, red);
+        if (tid == 0) {
+            const double c_norm = ((double)D / 2.0) * log(4.0 * 3.14159265358979323846 * eps);
+            const double la = (lp1 + (-0.5 * qxy / (2.0 * eps) - c_norm)) - (lp0 + (-0.5 * qyx / (2.0 * eps) - c_norm));
+            const double lu = log((double)a.U[t]);
+            w.dec[t] = (int8_t)((lu < la ? 1 : 0) | (fabs(la - lu) < a.hyst ? 2 : 0));
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void refine_zero_kernel(int* cnt) {
     if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
 }
@@ -527,21 +618,24 @@ void refine_chain_carve(RefineWork* w, const RefineTarget& tg, int64_t L, double
 }
 
 void refine_decisions_batched(const RefineBatch& b, int par, hipStream_t s) {
-    if (b.n_all == 0) return;
     const unsigned slots = 16;
-    hipLaunchKernelGGL(refine_flag_batched, dim3((unsigned)((b.max_T + FLAG_COLS - 1) / FLAG_COLS), 1, (unsigned)b.n_all),
-                       dim3(RNT), 0,
-                       s, b.ras, b.rws, b.all, par);
-    for (int phase = 0; phase < 2; ++phase) {
-        if (b.n_glm > 0)
+    // logistic chains: flag pass, then the FP64 gradients over row splits of X
+    if (b.n_glm > 0) {
+        hipLaunchKernelGGL(refine_flag_batched,
+                           dim3((unsigned)((b.max_T + FLAG_COLS - 1) / FLAG_COLS), 1, (unsigned)b.n_glm), dim3(RNT), 0,
+                           s, b.ras, b.rws, b.glm, par);
+        for (int phase = 0; phase < 2; ++phase) {
             hipLaunchKernelGGL(refine_pass_glm_batched, dim3((unsigned)b.max_split_glm, slots, (unsigned)b.n_glm),
                                dim3(RNT), 0, s, b.tgs, b.ras, b.rws, b.glm, par, phase);
-        if (b.n_gauss > 0)
-            hipLaunchKernelGGL(refine_pass_gauss_batched, dim3((unsigned)b.max_split_gauss, slots, (unsigned)b.n_gauss),
-                               dim3(RNT), 0, s, b.tgs, b.ras, b.rws, b.gauss, par, phase);
-        hipLaunchKernelGGL(refine_finish_batched, dim3(slots, 1, (unsigned)b.n_all), dim3(RNT), 0, s, b.tgs, b.ras,
-                           b.rws, b.all, par, phase);
+            hipLaunchKernelGGL(refine_finish_batched, dim3(slots, 1, (unsigned)b.n_glm), dim3(RNT), 0, s, b.tgs,
+                               b.ras, b.rws, b.glm, par, phase);
+        }
     }
+    // dense-Gaussian chains: flag and FP64 decision in one launch
+    if (b.n_gauss > 0)
+        hipLaunchKernelGGL(refine_gauss_direct_batched,
+                           dim3((unsigned)((b.max_T + DIRECT_COLS - 1) / DIRECT_COLS), 1, (unsigned)b.n_gauss),
+                           dim3(RNT), 0, s, b.tgs, b.ras, b.rws, b.gauss, par);
 }
 
 }  // namespace pmcmc

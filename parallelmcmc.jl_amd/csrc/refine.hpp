@@ -34,6 +34,10 @@ struct RefineArgs {
     const float* gy;
     const double* lpx;
     const double* lpy;
+    // log-densities in lp_nslab partials (lp[s * lp_stride + t], summed in
+    // slab order); 0 or 1: one value per column
+    int lp_nslab;
+    int64_t lp_stride;
     double eps;
     double band;
     const int* skip;

@@ -122,6 +122,10 @@ struct TileArgs {
     const T* hw;  // H(y) w_k
     const double* lpx;
     const double* lpy;
+    // log-densities in lp_nslab partials (lp[s * lp_stride + t], summed in
+    // slab order); 0 or 1: a single value per column
+    int lp_nslab;
+    int64_t lp_stride;
     // MODE_GEMM: FP64 decision override per column (-1 none; else bit 0 the
     // decision, bit 1 set when |log alpha - log u| < hyst), or null
     const int8_t* dec;
@@ -153,6 +157,13 @@ struct TileArgs {
     // nonzero residual, once the FP64 decision override (refine_on) is active.
     const int* final_cols;
     double* tile_smax;
+    // Sliding window of a batched chain (null otherwise): the last workgroup
+    // moves the window's front past its final prefix, or past the whole
+    // window when the metric test passes, and flags the chain done at T.
+    SlideState* win;
+    // element offset of column 0 in the probe stream (window of a longer
+    // trajectory: probes stay attached to absolute columns)
+    int64_t zoff;
 };
 
 // LDS layout (elements of T):
@@ -418,11 +429,44 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                 sy->last_metric = metric;
                 sy->last_iter = it;
                 const bool dec_final = !a.refine_on || *(volatile const int*)a.refine_on;
-                if (metric <= 1.0 && dec_final) {
+                if (!a.win && metric <= 1.0 && dec_final) {
                     sy->conv_iter = it;
                     sy->converged = 1;
                 }
-                if (a.final_cols) {
+                if (a.win) {
+                    // sliding window: the front moves past the final prefix (tiles
+                    // before the first one with a nonzero residual), or past the
+                    // whole window when it meets the tolerance; the window stays
+                    // where it is while decisions may still change with the gate
+                    SlideState* w = a.win;
+                    const int lf = __hip_atomic_load(&sy->live_from, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    sy->live_from = INT_MAX;
+                    // this iteration's window kernel initialised the window's new
+                    // columns and copied the previous acceptance: both are done
+                    if (w->front + a.T_len > w->hw) w->hw = w->front + a.T_len;
+                    w->pend_lo = w->pend_hi = 0;
+                    int64_t adv = 0;
+                    if (dec_final) {
+                        if (metric <= 1.0 || lf == INT_MAX)
+                            adv = a.T_len;
+                        else
+                            adv = lf;
+                    }
+                    if (adv > 0) {
+                        w->pend_lo = w->front;
+                        w->pend_hi = w->front + adv;
+                        w->front += adv;
+                        w->last_adv_iter = it;
+                        if (metric <= 1.0) {
+                            w->accepts += 1;
+                            w->seg_start = it + 1;
+                        }
+                    }
+                    if (w->front >= w->T) {
+                        sy->conv_iter = it;
+                        sy->converged = 1;
+                    }
+                } else if (a.final_cols) {
                     // tiles before the first one with a nonzero residual are final,
                     // once decisions no longer change with the override's gate
                     const int lf = __hip_atomic_load(&sy->live_from, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -518,6 +562,12 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         // per-column outputs: by the pass that writes the tile, or by the
         // reduce pass when it keeps A and B
         const bool side = keep ? PASS == PASS_REDUCE : OUT;
+        // a column's log-density: the slab partials summed in slab order
+        auto lp_of = [&](const double* lp, int64_t t) -> double {
+            double v = lp[t];
+            for (int sl = 1; sl < a.lp_nslab; ++sl) v += lp[(size_t)sl * a.lp_stride + t];
+            return v;
+        };
 
         // Accept decision of column c from its (group-summed) sums; returns it
         // and sets gg = g (1 - g) of the acceptance probability g and dg[k],
@@ -572,7 +622,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                         vhw[i] = a.hw[gi];
                     }
                 }
-                const double s_lpx = a.lpx[t], s_lpy = a.lpy[t], u = (double)a.U[t];
+                const double s_lpx = lp_of(a.lpx, t), s_lpy = lp_of(a.lpy, t), u = (double)a.U[t];
                 const int8_t o = a.dec ? a.dec[t] : (int8_t)-1;
                 const int8_t pv = a.prev_dec ? a.prev_dec[t] : (int8_t)-1;
                 const T* xcol = xs + (size_t)c * D;
@@ -587,7 +637,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                         const double x = (double)xcol[d];
                         const size_t gi = (size_t)t * D + d;
                         const double gx = (double)vgx[i], y = (double)vy[i], gy = (double)vgy[i];
-                        const double z = (double)rademacher(keys[0], gi);
+                        const double z = (double)rademacher(keys[0], gi + (size_t)a.zoff);
                         const double w = (double)vw[i], hy = (double)vhw[i];
                         const double r = x - y - eps * gy;
                         const double dr = z - w - eps * hy;
@@ -614,7 +664,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                     if (d < D) {
                         const T x = xcol[d];
                         const T y = vy[i];
-                        const T z = (T)rademacher(keys[0], (size_t)t * D + d);
+                        const T z = (T)rademacher(keys[0], (size_t)t * D + d + (size_t)a.zoff);
                         const T jt = (acc ? vw[i] : z) + (y - x) * dgt;
                         acol[d] = z * jt;
                         bcol[d] = (acc ? y : x) - corr_base(xcol[D + d]);  // FT_t - S_t
@@ -646,7 +696,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
 #pragma unroll
                         for (int k = 0; k < PMCMC_MAXP; ++k) {
                             if (k >= K) break;
-                            const double z = (double)rademacher(keys[k], gi);
+                            const double z = (double)rademacher(keys[k], gi + (size_t)a.zoff);
                             const double w = z + eps * (-p * z);
                             const double hy = -p * w;
                             const double r = x - y - eps * gy;
@@ -662,7 +712,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
 #pragma unroll
                         for (int k = 0; k < PMCMC_MAXP; ++k) {
                             if (k >= K) break;
-                            const double z = (double)rademacher(keys[k], gi);
+                            const double z = (double)rademacher(keys[k], gi + (size_t)a.zoff);
                             const double w = (double)a.W[k * DT + gi];
                             const double hy = (double)a.hw[k * DT + gi];
                             const double r = x - y - eps * gy;
@@ -683,8 +733,8 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                     s_lpx = group_sum(s_lpx, G);
                     s_lpy = group_sum(s_lpy, G);
                 } else {
-                    s_lpx = a.lpx[t];
-                    s_lpy = a.lpy[t];
+                    s_lpx = lp_of(a.lpx, t);
+                    s_lpy = lp_of(a.lpy, t);
                 }
 #pragma unroll
                 for (int k = 0; k < PMCMC_MAXP; ++k) {
@@ -727,7 +777,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
 #pragma unroll
                         for (int k = 0; k < PMCMC_MAXP; ++k) {
                             if (k >= K) break;
-                            const double z = (double)rademacher(keys[k], gi);
+                            const double z = (double)rademacher(keys[k], gi + (size_t)a.zoff);
                             double w;
                             if (MODE == MODE_ELEM)
                                 w = z + eps * (-p * z);
@@ -851,6 +901,7 @@ __global__ __launch_bounds__(PMCMC_NT, sizeof(T) == 4 ? 3 : 2) void tile_kernel(
 template <typename T, int MODE, int PASS>
 __global__ __launch_bounds__(PMCMC_NT, sizeof(T) == 4 ? 3 : 2) void tile_kernel_batched(const TileArgs<T>* args, const int2* items) {
     const int2 w = items[blockIdx.x];
+    if (w.y >= args[w.x].numTiles) return;  // beyond the chain's current window
     tile_body<T, MODE, PASS>(args[w.x], w.y);
 }
 
