@@ -310,113 +310,139 @@ __global__ __launch_bounds__(256) void gauss_pass_mfma_batched(const GaussPassAr
 // ---------------------------------------------------------------------------
 template <bool JBF>
 __device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const int64_t cb, const int slab) {
-    constexpr int NC = 2 * GCB;
-    constexpr int CPB = GCB;
-    constexpr int SL = GKC + 4;
-    constexpr int SVc = GKC + 4;
-    constexpr int NF = JBF ? 2 : 4;  // column tiles on FP32 MFMA
-    __shared__ float Ls[64 * SL];
-    __shared__ float Vc[NC * SVc];
-    __shared__ double lpred[4][NC];
+    // Lambda's slab rows stay in registers for the whole K: lane (li, lk) of
+    // wave w holds row r0 + 16 w + li at k = 16 j + 4 lk + e (j < 16, e < 4),
+    // loaded as 16 float4 all in flight at once.  The 16x16x4 MFMA step (j, e)
+    // takes that k from every lane, and the column values come from LDS in
+    // the same order.  The 32 states, then the 32 directions, are staged in
+    // LDS (k contiguous per column) and multiplied against the same registers;
+    // the directions' values are loaded during the states' MFMAs.
+    constexpr int CPB = GCB;        // 32 states + 32 directions per block
+    constexpr int KMAX = 256;
+    constexpr int SV = KMAX + 4;    // column stride in LDS
+    constexpr int NVL = CPB * KMAX / 256;  // column values per thread per half
+    __shared__ float Vc[CPB * SV];
+    __shared__ double lpred[4][CPB];
 
     const ColSrc<float>& src = a.src;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int64_t D = src.D, Tn = src.Tn;
-    const int ldL = a.ldL;
+    const int ldL = a.ldL;   // a multiple of 32, <= 256
+    const int nj = ldL / 16;  // float4 columns of A per lane
     const int r0 = 64 * slab;
+    const int row = r0 + 16 * wave + li;
     const float* mu = a.mu;
 
-    v4f acc[4];
+    float4 A[16];
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct) acc[ct] = (v4f){0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 16; ++j)
+        A[j] = (j < nj && row < ldL) ? *reinterpret_cast<const float4*>(a.L32 + (size_t)row * ldL + 16 * j + 4 * lk)
+                                     : make_float4(0.f, 0.f, 0.f, 0.f);
 
-    // k-slices software-pipelined as in the whole-column pass: 64 rows x 32 k
-    // of Lambda (two float4 per thread) and 64 columns x 32 k of values
-    constexpr int NCV = NC * GKC / 256;
-    float4 lv[2];
-    float cv[NCV];
-    float muv = 0.f;
-    auto fetch = [&](int k0) {
+    // column values of one half (set 0: centred states, set 1: directions):
+    // thread (m, k = tid) of column m, k contiguous in HBM and in LDS
+    float cv[NVL];
+    auto fetch = [&](int set) {
+        const int64_t k = tid;
+        const float muk = (set == 0 && mu && k < D) ? mu[k] : 0.f;
 #pragma unroll
-        for (int m = 0; m < 2; ++m) {
-            const int i = tid + 256 * m;
-            const int d = r0 + i / (GKC / 4), q = i % (GKC / 4);
-            lv[m] = d < ldL ? *reinterpret_cast<const float4*>(a.L32 + (size_t)d * ldL + k0 + 4 * q)
-                            : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        const int64_t kk = k0 + (tid % GKC);
-#pragma unroll
-        for (int m = 0; m < NCV; ++m) {
-            const int j = (tid + 256 * m) / GKC;
-            const int set = j / CPB;
-            const int64_t t = cb * CPB + (j % CPB);
+        for (int m = 0; m < NVL; ++m) {
+            const int64_t t = cb * CPB + m;
             float v = 0.f;
-            if (t < Tn && kk < D) {
+            if (t < Tn && k < D) {
                 if (set == 0) {
-                    v = src.x0 ? (t == 0 ? src.x0[kk] : src.V[(size_t)(t - 1) * D + kk]) : src.V[(size_t)t * D + kk];
+                    v = (src.x0 ? (t == 0 ? src.x0[k] : src.V[(size_t)(t - 1) * D + k]) : src.V[(size_t)t * D + k]) -
+                        muk;
                 } else {
-                    const size_t gi = (size_t)t * D + kk;
+                    const size_t gi = (size_t)t * D + k;
                     v = src.Z ? src.Z[gi] : rademacher(src.keys[0], gi + (size_t)src.zoff);
                 }
             }
             cv[m] = v;
         }
-        muv = (mu && kk < D) ? mu[kk] : 0.f;
     };
+    auto stash = [&]() {
+        if (tid < ldL)
+#pragma unroll
+            for (int m = 0; m < NVL; ++m) Vc[m * SV + tid] = cv[m];
+    };
+
+    v4f acc[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) acc[ct] = (v4f){0.f, 0.f, 0.f, 0.f};
+
     fetch(0);
-    for (int k0 = 0; k0 < ldL; k0 += GKC) {
-        __syncthreads();
+    stash();
+    __syncthreads();
+    fetch(1);  // in flight during the states' MFMAs
+    // states: column tiles 0, 1 (16 columns each) on FP32 MFMA (the loops over
+    // A are unrolled, so that A stays in registers)
 #pragma unroll
-        for (int m = 0; m < 2; ++m) {
-            const int i = tid + 256 * m;
-            const int d = i / (GKC / 4), q = i % (GKC / 4);
-            float* dst = &Ls[d * SL + 4 * q];
-            dst[0] = lv[m].x;
-            dst[1] = lv[m].y;
-            dst[2] = lv[m].z;
-            dst[3] = lv[m].w;
-        }
+    for (int j = 0; j < 16; ++j) {
+        if (j < nj) {
+            const float av[4] = {A[j].x, A[j].y, A[j].z, A[j].w};
 #pragma unroll
-        for (int m = 0; m < NCV; ++m) {
-            const int i = tid + 256 * m;
-            const int j = i / GKC, k = i % GKC;
-            const bool cen = j / CPB == 0 && cb * CPB + (j % CPB) < Tn;
-            Vc[j * SVc + k] = cen ? cv[m] - muv : cv[m];
-        }
-        __syncthreads();
-        if (k0 + GKC < ldL) fetch(k0 + GKC);
+            for (int e = 0; e < 4; ++e) {
+                const int k = 16 * j + 4 * lk + e;
 #pragma unroll
-        for (int ks = 0; ks < GKC / 4; ++ks) {
-            const float av = Ls[(16 * wave + li) * SL + 4 * ks + lk];
-#pragma unroll
-            for (int ct = 0; ct < NF; ++ct)
-                acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, Vc[(16 * ct + li) * SVc + 4 * ks + lk], acc[ct],
-                                                               0, 0, 0);
-        }
-        if (JBF) {
-            v8bf av, b2, b3;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                av[e] = (__bf16)Ls[(16 * wave + li) * SL + 8 * lk + e];
-                b2[e] = (__bf16)Vc[(32 + li) * SVc + 8 * lk + e];
-                b3[e] = (__bf16)Vc[(48 + li) * SVc + 8 * lk + e];
+                for (int ct = 0; ct < 2; ++ct)
+                    acc[ct] =
+                        __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], Vc[(16 * ct + li) * SV + k], acc[ct], 0, 0, 0);
             }
-            acc[2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b2, acc[2], 0, 0, 0);
-            acc[3] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b3, acc[3], 0, 0, 0);
+        }
+    }
+    __syncthreads();  // the states' reads of Vc are done
+    stash();
+    __syncthreads();
+    // directions: column tiles 2, 3 (BF16 MFMA, 32 k per step: the lane's
+    // eight values of two float4 columns, in the same k order for A and B)
+    if (JBF) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (2 * q >= nj) continue;
+            v8bf av, b0, b1;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const float4 f = A[2 * q + h];
+                const float fv[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int k = 16 * (2 * q + h) + 4 * lk + e;
+                    av[4 * h + e] = (__bf16)fv[e];
+                    b0[4 * h + e] = (__bf16)Vc[li * SV + k];
+                    b1[4 * h + e] = (__bf16)Vc[(16 + li) * SV + k];
+                }
+            }
+            acc[2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b0, acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b1, acc[3], 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (j >= nj) continue;
+            const float av[4] = {A[j].x, A[j].y, A[j].z, A[j].w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int k = 16 * j + 4 * lk + e;
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct)
+                    acc[2 + ct] =
+                        __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], Vc[(16 * ct + li) * SV + k], acc[2 + ct], 0, 0, 0);
+            }
         }
     }
 
     // epilogue: lane holds P[d = r0 + 16 wave + 4 lk + r][column 16 ct + li]
+    // (column tiles 0-1: the states t = cb*32 + 16 ct + li; 2-3: their directions)
     const double eps = (double)a.eps;
     const double sq2e = sqrt(2.0 * eps);
-    double lpc[4] = {0.0, 0.0, 0.0, 0.0};
+    double lpc[2] = {0.0, 0.0};
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) {
-        const int j = 16 * ct + li;
-        const int set = j / CPB;
-        const int64_t t = cb * CPB + (j % CPB);
+        const int set = ct >> 1;
+        const int64_t t = cb * CPB + 16 * (ct & 1) + li;
         if (t >= Tn) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -439,7 +465,7 @@ __device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const in
         }
     }
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct) {
+    for (int ct = 0; ct < 2; ++ct) {
         lpc[ct] += __shfl_xor(lpc[ct], 16, 64);
         lpc[ct] += __shfl_xor(lpc[ct], 32, 64);
         if (lk == 0) lpred[wave][16 * ct + li] = lpc[ct];

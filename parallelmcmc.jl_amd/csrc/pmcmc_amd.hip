@@ -2376,8 +2376,17 @@ static void batch_tables(pmcmc_batch* b, BatchTables* bt) {
             a.hyst = 1e-6;
             a.win = b->slides + i;
             if (b->refine) {
-                a.dec = h.rw.dec;
                 a.refine_on = h.rw.cnt + 2;
+                if (h.tgt->kind == TGT_LOGISTIC) {
+                    a.dec = h.rw.dec;  // decided by the refinement kernels
+                } else {
+                    // decided inline by the tile reduce (its target table entry is
+                    // filled in once the tables have a device home)
+                    a.rflag = h.rw.cnt + 2;
+                    a.band = h.band;
+                    a.gate = 100.0;
+                    a.gate_iters = 16;
+                }
             }
             bt->targs[par].push_back(a);
             RefineArgs ra{};
@@ -2547,13 +2556,23 @@ static int upload(pmcmc_ctx* c, hipStream_t s, const Pack& pk, void** buf, size_
 }
 
 // The host's window-length policy at a synchronisation.  A chain starts with
-// the full window.  One that after 24 iterations of a window segment neither
-// reduced its metric 4x over the last chunk nor moved its front 4 columns per
-// iteration is converging causally (about one column per iteration, the
-// Jacobian estimate missing the coupling of its dynamics): its window shrinks
-// to a few tiles past the front, where all its progress happens.  A narrowed
-// chain whose front moves 4 or more columns per iteration widens 4x again.
+// a short window (PMCMC_SLIDE_START=wide: the full window), a few tiles past
+// its front.  A chain whose front moves 4 or more columns per iteration over
+// the last chunk converges faster than causally: its window widens 4x (up to
+// the full window).  A wide chain that after 24 iterations of a window
+// segment neither reduced its metric 4x over the last chunk nor moved its
+// front 2 columns per iteration is converging causally (about one column per
+// iteration, the Jacobian estimate missing the coupling of its dynamics): its
+// window shrinks back to the short one, where all its progress happens.
 // Returns true when a length changed.
+static bool slide_start_wide() {
+    static const bool wide = [] {
+        const char* e = getenv("PMCMC_SLIDE_START");
+        return e && !strcmp(e, "wide");
+    }();
+    return wide;
+}
+
 static bool batch_policy(pmcmc_batch* b, int i) {
     BatchChain& h = b->ch[i];
     const SyncState& hs = b->hsync[i];
@@ -2562,10 +2581,10 @@ static bool batch_policy(pmcmc_batch* b, int i) {
     const int dk = k - h.prev_iter;
     const int64_t prog = sl.front - h.prev_front;
     const int old = h.span;
-    if (h.span >= h.L && h.narrow < h.L) {
+    if (h.span > h.narrow && dk > 0 && prog < 2 * (int64_t)dk) {
         const int seg = k + 1 - sl.seg_start;
         const bool falling = hs.last_metric < h.prev_metric * 0.25;  // false for NaN
-        if (seg >= 24 && dk >= 8 && !falling && prog < 4 * (int64_t)dk) h.span = h.narrow;
+        if (seg >= 24 && dk >= 8 && !falling) h.span = h.narrow;
     } else if (h.span < h.L && dk > 0 && prog >= 4 * (int64_t)dk) {
         const int64_t TC = std::min<int64_t>(h.TC, h.L);
         int64_t ns = std::min<int64_t>(4 * (int64_t)h.span, h.L);
@@ -2592,7 +2611,7 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
         h.iters = 0;
         h.converged = true;
         h.metric = 0;
-        h.span = (int)h.L;
+        h.span = slide_start_wide() ? (int)h.L : h.narrow;
         h.prev_front = 0;
         h.prev_iter = -1;
         h.prev_metric = INFINITY;
@@ -2639,6 +2658,11 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
                 wh[i].qp[v] = (GaussPassArgs*)at(o_qp[v]) + i;
             }
             wh[i].rw = (RefineWork*)at(o_rw) + i;
+            if (b->refine && b->ch[i].tgt->kind != TGT_LOGISTIC)
+                for (int v = 0; v < 2; ++v) {
+                    TileArgs<float>* th = reinterpret_cast<TileArgs<float>*>(tp.host.data() + o_t[v]) + i;
+                    th->rtg = (const RefineTarget*)at(o_rt) + i;
+                }
         }
     }
     HIPCHK(c, hipMemcpyAsync(b->tables, tp.host.data(), tp.host.size(), hipMemcpyHostToDevice, s));
@@ -2690,7 +2714,7 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
             rb.glm = (const int*)ia(o_rglm);
             rb.n_glm = (int)it.r_glm.size();
             rb.gauss = (const int*)ia(o_rgauss);
-            rb.n_gauss = (int)it.r_gauss.size();
+            rb.n_gauss = 0;  // dense-Gaussian chains decide inline in the tile reduce
             rb.max_T = it.r_maxT;
             rb.max_split_glm = it.r_split_glm;
             rb.max_split_gauss = it.r_split_gauss;
@@ -2710,7 +2734,14 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
                                    dim3((unsigned)it.red[lc].size()), dim3(PMCMC_NT), it.lds_multi[lc], s, ta,
                                    (const int2*)ia(o_red[lc]));
     };
+    double rebuild_s = 0.0;
     auto rebuild = [&]() -> int {
+        const auto r0 = std::chrono::steady_clock::now();
+        struct Acc {
+            double* s;
+            std::chrono::steady_clock::time_point t;
+            ~Acc() { *s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count(); }
+        } acc_{&rebuild_s, r0};
         drop_graph();
         it = Items();
         build_items(b, live, &it);
@@ -2826,7 +2857,8 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
     }
     if (progress) {
         const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-        fprintf(stderr, "[pmcmc batch] done: %d iterations, %d rebuilds, %.3f s\n", k, rebuilds, el);
+        fprintf(stderr, "[pmcmc batch] done: %d iterations, %d rebuilds (%.3f s), %.3f s\n", k, rebuilds, rebuild_s,
+                el);
         fflush(stderr);
     }
     if (infos)

@@ -150,7 +150,7 @@ __device__ void refine_flag_column(const RefineArgs& a, int8_t* dec, int* cnt, i
 
 // Columns FLAG_COLS blk .. FLAG_COLS (blk + 1) - 1, one wave per column at a
 // time (a workgroup per handful of columns made the launch dispatch-bound).
-static constexpr int FLAG_COLS = 8;
+static constexpr int FLAG_COLS = 32;
 __device__ void refine_flag_body(const RefineArgs& a, int8_t* dec, int* cnt, int* list, int lmax, int par, int64_t D,
                                  int64_t T, int64_t blk) {
     if (skipped(a)) return;
@@ -618,7 +618,9 @@ void refine_chain_carve(RefineWork* w, const RefineTarget& tg, int64_t L, double
 }
 
 void refine_decisions_batched(const RefineBatch& b, int par, hipStream_t s) {
-    const unsigned slots = 16;
+    // a handful of columns is listed per chain and iteration: two slots per
+    // (chain, split), so that empty workgroups do not dominate the launches
+    const unsigned slots = 2;
     // logistic chains: flag pass, then the FP64 gradients over row splits of X
     if (b.n_glm > 0) {
         hipLaunchKernelGGL(refine_flag_batched,

@@ -46,6 +46,7 @@
 #include <type_traits>
 
 #include "common.hpp"
+#include "refine.hpp"
 
 namespace pmcmc {
 
@@ -164,7 +165,68 @@ struct TileArgs {
     // element offset of column 0 in the probe stream (window of a longer
     // trajectory: probes stay attached to absolute columns)
     int64_t zoff;
+    // Inline FP64 decisions (dense-Gaussian chains of a batched solve; null
+    // otherwise): the pass that forms the decisions re-decides, in FP64, every
+    // column whose log alpha is within `band` of log u, once the gate is open
+    // (the previous update within `gate` tolerance units, or iteration
+    // `gate_iters`; then sticky in *rflag), and rebuilds the column's A and B
+    // if the decision changed.  The same FP64 quantities as the separate
+    // refinement kernels (the state is the FP32 iterate, exact in double).
+    const RefineTarget* rtg;
+    int* rflag;
+    double band, gate;
+    int gate_iters;
 };
+
+// FP64 log alpha of one column for a dense Gaussian target, by the whole
+// workgroup: the state x (FP32 in LDS, exact in double) and tape column xi;
+// v: D doubles of LDS scratch, red: 4 doubles.  One row per thread in the two
+// D x D mat-vecs of Lambda (column-major, coalesced over the rows).
+template <typename T>
+__device__ double gauss_logalpha_fp64(const RefineTarget& tg, const T* x, const T* xi, double eps, double* v,
+                                      double* red) {
+    const int tid = threadIdx.x;
+    const int64_t D = tg.D;
+    auto bsum = [&](double q) {
+        for (int m = 32; m >= 1; m >>= 1) q += __shfl_xor(q, m, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = q;
+        __syncthreads();
+        const double r = (red[0] + red[1]) + (red[2] + red[3]);
+        __syncthreads();
+        return r;
+    };
+    const double sq2e = sqrt(2.0 * eps);
+    for (int64_t d = tid; d < D; d += PMCMC_NT) v[d] = (double)x[d] - (tg.mu64 ? tg.mu64[d] : 0.0);
+    __syncthreads();
+    double g0 = 0.0, yv = 0.0, lpp = 0.0;  // thread tid owns row tid (D <= 256)
+    if (tid < D) {
+        double p = 0.0;
+        for (int64_t k = 0; k < D; ++k) p += tg.L64[tid + k * D] * v[k];
+        g0 = -p;
+        lpp = 0.5 * v[tid] * g0;
+        yv = (double)x[tid] + eps * g0 + sq2e * (double)xi[tid];
+    }
+    const double lp0 = bsum(lpp);  // its barriers order the reads of v before the rewrite
+    if (tid < D) v[tid] = yv - (tg.mu64 ? tg.mu64[tid] : 0.0);
+    __syncthreads();
+    double lq = 0.0, qyx = 0.0, qxy = 0.0;
+    if (tid < D) {
+        double p = 0.0;
+        for (int64_t k = 0; k < D; ++k) p += tg.L64[tid + k * D] * v[k];
+        const double g = -p;
+        lq = 0.5 * v[tid] * g;
+        const double xx = (double)x[tid];
+        const double e1 = yv - xx - eps * g0;
+        const double e2 = xx - yv - eps * g;
+        qyx = e1 * e1;
+        qxy = e2 * e2;
+    }
+    const double lp1 = bsum(lq);
+    qyx = bsum(qyx);
+    qxy = bsum(qxy);
+    const double c_norm = ((double)D / 2.0) * log(4.0 * 3.14159265358979323846 * eps);
+    return (lp1 + (-0.5 * qxy / (2.0 * eps) - c_norm)) - (lp0 + (-0.5 * qyx / (2.0 * eps) - c_norm));
+}
 
 // LDS layout (elements of T):
 //   xs  : (TC+1)*D  iterate columns c0-1 .. c0+TC-1 (Xbar and S_old); DEER modes only
@@ -562,6 +624,25 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         // per-column outputs: by the pass that writes the tile, or by the
         // reduce pass when it keeps A and B
         const bool side = keep ? PASS == PASS_REDUCE : OUT;
+        // inline FP64 decisions: the gate (fixed during this launch), and the
+        // columns within the band (their FP32-built A and B are fixed up below)
+        constexpr int FLMAX = 64;
+        __shared__ int fl_n;
+        __shared__ int fl_c[FLMAX];
+        __shared__ int8_t fl_pv[FLMAX];
+        __shared__ double fl_dg[FLMAX];
+        __shared__ double fl_v[256];
+        const bool inl = MODE == MODE_GEMM && a.rtg != nullptr && side;
+        bool inl_on = false;
+        if (inl) {
+            inl_on = *(volatile const int*)a.rflag != 0 || a.sync->last_metric <= a.gate ||
+                     a.sync->last_iter + 1 >= a.gate_iters;
+            if (tid == 0) {
+                fl_n = 0;
+                if (inl_on && tile == 0) *a.rflag = 1;
+            }
+            __syncthreads();
+        }
         // a column's log-density: the slab partials summed in slab order
         auto lp_of = [&](const double* lp, int64_t t) -> double {
             double v = lp[t];
@@ -596,8 +677,20 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
             for (int k = 0; k < PMCMC_MAXP; ++k) dg[k] = gg * (d1[k] - d2[k] - d3[k] / (2.0 * eps));
             if (side && lane == 0) {
                 accl[c] = acc ? 1 : 0;
-                if (a.prev_dec) a.prev_dec[t] = acc ? 1 : 0;
-                if (a.acc_out) a.acc_out[t] = acc ? 1 : 0;
+                bool listed = false;
+                if (inl_on && fabs(la - lu) < a.band) {
+                    const int q = atomicAdd(&fl_n, 1);
+                    if (q < FLMAX) {  // beyond the list: the FP32 decision stands
+                        fl_c[q] = c;
+                        fl_pv[q] = pv;
+                        fl_dg[q] = dg[0];
+                        listed = true;
+                    }
+                }
+                if (!listed) {
+                    if (a.prev_dec) a.prev_dec[t] = acc ? 1 : 0;
+                    if (a.acc_out) a.acc_out[t] = acc ? 1 : 0;
+                }
                 if (a.la_out) a.la_out[t] = la;
             }
             return acc;
@@ -794,6 +887,39 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
             }
         }
         __syncthreads();
+        if (inl) {
+            // FP64 decisions of the listed columns (whole workgroup, one column
+            // at a time); a changed decision rebuilds the column's A and B
+            const int nfl = fl_n < FLMAX ? fl_n : FLMAX;
+            for (int q = 0; q < nfl; ++q) {
+                const int c = fl_c[q];
+                const int64_t t = c0 + c;
+                const T* xcol = xs + (size_t)c * D;
+                const double la64 = gauss_logalpha_fp64<T>(*a.rtg, xcol, a.Xi + (size_t)t * D, eps, fl_v, s_red[0]);
+                const double lu = log((double)a.U[t]);
+                bool acc = lu < la64;
+                const int8_t pv = fl_pv[q];
+                if (a.prev_dec && fabs(la64 - lu) < a.hyst && pv >= 0) acc = pv != 0;
+                if (acc != (accl[c] != 0)) {
+                    const T dgt = (T)fl_dg[q];
+                    for (int64_t d = tid; d < D; d += PMCMC_NT) {
+                        const size_t gi = (size_t)t * D + d;
+                        const T x = xcol[d];
+                        const T y = a.Y[gi];
+                        const T z = (T)rademacher(keys[0], gi + (size_t)a.zoff);
+                        const T jt = (acc ? a.W[gi] : z) + (y - x) * dgt;
+                        al[(size_t)c * D + d] = z * jt;
+                        bl[(size_t)c * D + d] = (acc ? y : x) - corr_base(xcol[D + d]);
+                    }
+                }
+                if (tid == 0) {
+                    accl[c] = acc ? 1 : 0;
+                    if (a.prev_dec) a.prev_dec[t] = acc ? 1 : 0;
+                    if (a.acc_out) a.acc_out[t] = acc ? 1 : 0;
+                }
+                __syncthreads();
+            }
+        }
         if (PASS == PASS_REDUCE && keep) {
             for (int64_t i = tid; i < nel; i += PMCMC_NT) {
                 a.a_keep[gbase + i] = al[i];
