@@ -1,27 +1,25 @@
 #!/usr/bin/env python3
 """MALA samples/s of DEER on one MI355X.
 
-Workload (the logistic-regression configuration): Bayesian logistic
-regression, X is 10000 x 100 with entries N(0,1)/sqrt(100), theta* ~ N(0,I),
-y ~ Bernoulli(sigmoid(X theta*)), prior N(0,I); eps = 5e-3, T = 4096,
-jacobian = :stoch_diag (1 probe), the ParallelMALASampler tolerances
-tol_abs = 1e-6, tol_rel = 1e-5, maxiter = 200, and DEER.solve's default
-damping 1.0 (--damping 0.5 gives the sampler's default).
+Default workload (the batched uneven-chain configuration): 256 chains with
+T ~ round(exp U(ln 1024, ln 65536)) and D ~ U{2..256}; even chains dense
+Gaussians (precision Q diag(10^U(0,3)) Q^T), odd chains Bayesian logistic
+regressions (N = 2000, X ~ N(0,1)/sqrt(D)); BF16 :stoch_diag Jacobians (one
+probe set per chain), sliding windows of at most 4096 columns, tol 1e-6 /
+1e-5, damping 1.0.  A step is one batched solve of every chain's whole
+trajectory, with tapes, targets and work buffers resident in HBM; `value` is
+samples/s over all chains.  The plain FP32 DEER baseline (the single-chain
+solve, one chain at a time, window by window, tuned tile shapes, no graphs)
+runs on a stated subset (every eighth pair of chains) beside the batched BF16
+and FP32 solves of that subset; the ratio is in `config.subset`.  The CPU leg
+times the reference CPU DEER restatement and the MALASampler restatement
+(oracle) on a bounded sample of the same workload.
 
-A step is one DEER solve of one T-block (the Newton loop to convergence),
-blocks chained like the sampler's block loop (x0 of block k+1 is the last
-state of block k), every block with its own synthetic tape.  Tapes, data and
-workspaces are resident in HBM before the timed region.  Both variants run:
-plain FP32 DEER and BF16-Jacobian DEER (probe GEMMs on BF16 MFMA, residual
-path FP32); `value` is the BF16-Jacobian samples/s, the FP32 figure and the
-ratio are in `config`.  The CPU leg checks block 0 of each variant against
-the FP64 sequential taped MALA run (trace error, accept decisions).
-
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--precision both|f32|bf16_jac]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+    python bench.py --workload logistic   # N=10000 D=100 T=4096 logistic regression, FP32 and BF16-Jacobian
     python bench.py --workload scan --scan-D 64 --scan-T 1048576
     python bench.py --workload scanfull   # dense-Jacobian scan D=64 T=16384 (materialised J)
     python bench.py --workload full       # correlated Gaussian D=64 T=16384, dense Jacobian
-    python bench.py --workload c3         # 256 uneven chains, batched BF16 vs per-chain FP32
     python bench.py --workload adjoint    # gradient of mean |x_t|^2, logistic T=32768
 """
 import argparse
@@ -344,12 +342,98 @@ def _c3_tune(cl, W):
     return tune_chain_widths(cl, W, _lib.PREC_BF16_JAC, 0)
 
 
+def _cpu_info():
+    """CPU model string, BLAS libraries and thread counts of this host."""
+    from threadpoolctl import threadpool_info
+
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    blas = [{"api": i.get("internal_api"), "version": i.get("version"), "threads": i.get("num_threads")}
+            for i in threadpool_info() if i.get("user_api") == "blas"]
+    return model, blas
+
+
+def c3_cpu_baseline(chain, L, gpu_iters, args, budget_s):
+    """CPU leg of the batched workload, on this host, on a bounded sample: the
+    numpy restatement of the reference CPU DEER (src/DEER/DEER.jl:375-595,
+    batched callbacks, redrawn probes, FP64) run to convergence on the first
+    L-step window of one logistic chain of the instance, with OpenBLAS on
+    every core and, for s/iteration, on one thread; and the MALASampler
+    restatement (src/interface.jl:163-194) on the same target, one thread."""
+    import numpy as np
+    from threadpoolctl import threadpool_info, threadpool_limits
+
+    from oracle import deer_oracle as O
+
+    tg = chain.model.target
+    ot = O.LogReg(tg.X, tg.y, tg.prior_scale)
+    Xi = chain.Xi[:, :L].astype(np.float64)
+    U = chain.U[:L].astype(np.float64)
+    x0 = chain.x0.astype(np.float64)
+    threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    fwd, jdiag = O.mala_rec(ot, chain.eps, Xi, U)
+    S = np.repeat(x0[:, None], L, axis=1)
+    t0 = time.perf_counter()
+    conv, it = False, 0
+    while it < L + 1 and time.perf_counter() - t0 < budget_s:
+        S_new, _, _ = O.deer_update_rec(fwd, x0, S, "stoch_diag", 1.0, 1, 1, it, None, jdiag, None)
+        dmax = float(np.max(np.abs(S_new - S)))
+        smax = float(np.max(np.abs(S_new)))
+        S = S_new
+        it += 1
+        if dmax / (args.tol_abs + args.tol_rel * smax) <= 1:
+            conv = True
+            break
+    t_deer = time.perf_counter() - t0
+    with threadpool_limits(limits=1):
+        t1 = time.perf_counter()
+        for k in range(2):
+            O.deer_update_rec(fwd, x0, S, "stoch_diag", 1.0, 1, 1, k, None, jdiag, None)
+        s_iter_1t = (time.perf_counter() - t1) / 2
+        rng = np.random.default_rng(17)
+        t2 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t2 < min(5.0, budget_s / 4):
+            O.mala_sampler_run(ot, x0, chain.eps, 200, rng)
+            n += 200
+        mala_sps = n / (time.perf_counter() - t2)
+    model, blas = _cpu_info()
+    return {
+        "value": L / t_deer if conv else 0.0,
+        "unit": "samples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"numpy FP64 restatement of the reference CPU DEER (redrawn Rademacher probe, damping 1) run "
+                   f"to {'convergence' if conv else 'its time budget (not converged)'} on the first {L}-step "
+                   f"window of a logistic chain of the instance (N={ot.Xd.shape[0]}, D={ot.dim}): {it} Newton "
+                   f"iterations in {t_deer:.1f} s with OpenBLAS on {threads} threads; the GPU's plain FP32 "
+                   f"DEER took {gpu_iters} iterations on that window"),
+        "reference_deer_iterations": it,
+        "reference_deer_converged": conv,
+        "gpu_plain_f32_iterations_same_window": gpu_iters,
+        "reference_deer_s_per_iteration_1_thread": s_iter_1t,
+        "mala_sampler_samples_per_s_1_thread": mala_sps,
+        "cpu_model": model,
+        "blas": blas,
+    }
+
+
 def bench_c3(args):
-    """Batched BF16-Jacobian DEER over uneven chains (one set of batched
-    launches per iteration, graph replay) against plain FP32 DEER one chain at
-    a time (single-chain solves per window, no graphs).  Both keep one probe
-    set per window: with probes redrawn every iteration :stoch_diag DEER
-    overflows FP32 on the condition-1e3 Gaussian chains."""
+    """Batched uneven chains: 256 chains, T ~ logU[1024, 65536], D ~ U{2..256},
+    half dense Gaussians (precision condition 1e3), half logistic regressions
+    (N = 2000); BF16 :stoch_diag Jacobians, sliding windows of at most 4096
+    columns, graph-replayed batched iterations.  A step is one batched solve
+    of every chain's whole trajectory (tapes resident in HBM).  The plain FP32
+    DEER baseline (the single-chain solve one chain at a time, window by
+    window, tuned tile shapes, no graphs) runs on a stated subset, with the
+    batched solve timed again on that same subset."""
     import numpy as np
     import torch
 
@@ -357,90 +441,162 @@ def bench_c3(args):
     from ParallelMCMC.Batched import BatchPlan
 
     W = args.window
+    torch.cuda.set_device(0)
+    t_gen = time.perf_counter()
     chains = make_c3(args.c3_chains)
-    log(f"c3: {len(chains)} chains generated")
     cl = [c for _, c in chains]
     total = sum(c.Xi.shape[1] for c in cl)
+    log(f"c3: {len(cl)} chains, {total} samples generated in {time.perf_counter() - t_gen:.1f} s")
     kw = dict(tol_abs=args.tol_abs, tol_rel=args.tol_rel, maxiter=W + 1)
-    out = {}
-    tuned, tune_s = {}, 0.0
-    if not args.no_tune:
+    plan = BatchPlan(cl, window=W, precision="bf16_jac", damping=args.damping, **kw)
+    for k in range(args.warmup):
         t0 = time.perf_counter()
-        tuned = _c3_tune(cl, W)
-        tune_s = time.perf_counter() - t0
-        log(f"c3: tuned {len(tuned)} width buckets in {tune_s:.1f} s")
-    for prec in ("bf16_jac", "f32"):
-        plan = BatchPlan(cl, window=W, precision=prec, **kw)
+        plan.solve()
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
+        log(f"c3 warmup {k + 1}/{args.warmup}: {time.perf_counter() - t0:.2f} s")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = None
+    for k in range(args.steps):
         res = plan.solve()
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        log(f"c3 batched {prec}: {el:.2f} s")
-        its = {k: [r[2].iters for r, (kk, _) in zip(res, chains) if kk == k] for k in ("gauss", "logreg")}
-        out[prec] = {"samples_per_s": total / el, "seconds": el, "converged": all(r[2].converged for r in res),
-                     "newton_iters_mean": {k: float(np.mean(v)) for k, v in its.items()},
-                     "newton_iters_max": {k: int(np.max(v)) for k, v in its.items()}}
-        del plan
-    # per-chain FP32 baseline on a systematic subset (every k-th pair), the
-    # batched BF16 run on that same subset, and both against FP64 sequential MALA
+        log(f"c3 step {k + 1}/{args.steps} queued at {time.perf_counter() - t0:.2f} s")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    sps = args.steps * total / elapsed
+    log(f"c3 batched bf16: {sps:.0f} samples/s ({elapsed / args.steps:.2f} s per solve)")
+    its = {kk: [r[2].iters for r, (k2, _) in zip(res, chains) if k2 == kk] for kk in ("gauss", "logreg")}
+    all_conv = all(r[2].converged for r in res)
+    del plan, res
+
+    # subset: every k-th pair (one Gaussian, one logistic chain)
     pick = [j for j in range(len(cl)) if (j // 2) % args.c3_baseline_every == 0]
     sub = [cl[j] for j in pick]
     sub_total = sum(c.Xi.shape[1] for c in sub)
+    plan_b = BatchPlan(sub, window=W, precision="bf16_jac", damping=args.damping, **kw)
+    plan_b.solve()  # warm (graphs, caches)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    base = []
+    res_b = plan_b.solve()
+    torch.cuda.synchronize()
+    sub_b = time.perf_counter() - t0
+    # roofline of the dominant kernel (the Lambda passes): one more, profiled,
+    # solve of the subset with events around every pass launch
+    plan_b.set_profiling(True)
+    plan_b.solve()
+    torch.cuda.synchronize()
+    n_pass, ms_pass, fl_pass = plan_b.pass_stats()
+    plan_b.set_profiling(False)
+    S_b = [r[0].cpu().numpy() for r in res_b]
+    A_b = [r[1].cpu().numpy().astype(bool) for r in res_b]
+    del plan_b
+    plan_f = BatchPlan(sub, window=W, precision="f32", damping=args.damping, **kw)
+    plan_f.solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan_f.solve()
+    torch.cuda.synchronize()
+    sub_f = time.perf_counter() - t0
+    del plan_f
+    log(f"c3 subset ({len(sub)} chains): batched bf16 {sub_b:.2f} s, batched f32 {sub_f:.2f} s; plain f32 next")
+    tuned, tune_s = {}, 0.0
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tuned = _c3_tune(sub, W)
+        tune_s = time.perf_counter() - t0
+        log(f"c3: tuned {len(tuned)} width buckets for the per-chain solves in {tune_s:.1f} s")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    base, base_infos = [], []
     for j, c in enumerate(sub):
-        S, _, infos = solve_windowed(c.model, c.eps, c.Xi, c.U, c.x0, W, "f32", 0, damping=1.0, fixed_probes=True,
-                                     **kw)
+        S, _, infos = solve_windowed(c.model, c.eps, c.Xi, c.U, c.x0, W, "f32", 0, damping=args.damping,
+                                     fixed_probes=True, **kw)
         base.append(S)
-        log(f"c3 plain f32 chain {j + 1}/{len(sub)} (T={c.Xi.shape[1]}, D={c.Xi.shape[0]}): "
-            f"{time.perf_counter() - t0:.1f} s")
+        base_infos.append(infos)
+        if (j + 1) % 8 == 0:
+            log(f"c3 plain f32: {j + 1}/{len(sub)} chains, {time.perf_counter() - t0:.1f} s")
     torch.cuda.synchronize()
     base_s = time.perf_counter() - t0
-    plan = BatchPlan(sub, window=W, precision="bf16_jac", **kw)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    res_sub = plan.solve()
-    torch.cuda.synchronize()
-    sub_s = time.perf_counter() - t0
-    log(f"c3 subset: plain {base_s:.1f} s, batched {sub_s:.1f} s; FP64 sequential checks next")
+    log(f"c3 subset: plain f32 {base_s:.1f} s, batched bf16 {sub_b:.2f} s ({base_s / sub_b:.2f}x)")
+
+    # accuracy against FP64 sequential MALA on a stated part of the subset
     from oracle import deer_oracle as O
 
     acc_rows = []
-    for j, c, r, bS in zip(pick, sub, res_sub, base):
+    checked = [q for q in range(len(sub)) if sub[q].Xi.shape[1] <= args.c3_check_tmax]
+    for q in checked:
+        c, j = sub[q], pick[q]
         tg = c.model.target
-        if hasattr(tg, "Lam"):
-            ot = O.Gaussian(tg.Lam, np.zeros(tg.dim))
-        else:
-            ot = O.LogReg(tg.X, tg.y, tg.prior_scale)
+        ot = O.Gaussian(tg.Lam, np.zeros(tg.dim)) if hasattr(tg, "Lam") else O.LogReg(tg.X, tg.y, tg.prior_scale)
         ref, acc_ref, _ = O.run_mala_sequential_taped(ot, c.x0.astype(np.float64), c.eps, c.Xi.astype(np.float64),
                                                       c.U.astype(np.float64))
-        Sb, Sp = r[0].cpu().numpy().astype(np.float64), bS.cpu().numpy().astype(np.float64)
-        mism = np.nonzero(r[1].cpu().numpy().astype(bool) != acc_ref)[0]
+        Sp = base[q].cpu().numpy().astype(np.float64)
         acc_rows.append({"chain": j, "kind": chains[j][0], "D": c.Xi.shape[0], "T": c.Xi.shape[1],
-                         "trace_err_batched": O.trace_error(Sb, ref), "trace_err_plain": O.trace_error(Sp, ref),
-                         "accept_mismatches_batched": int(len(mism)),
-                         "first_mismatch_batched": int(mism[0]) if len(mism) else None})
-    print(json.dumps({
-        "metric": "MALA samples/s, batched uneven chains: BF16-Jac batched vs plain FP32 DEER", "value":
-        out["bf16_jac"]["samples_per_s"], "unit": "samples/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-        "ms_per_step": 1e3 * out["bf16_jac"]["seconds"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "bf16+f32", "data": "synthetic (seeded targets and tapes)",
-        "config": {"workload": f"{len(cl)} chains, T~logU[1024,65536], D~U{{2..256}}, half dense Gaussian "
-                               f"(cond 1e3) half logistic N=2000, :stoch_diag, window {W}, tol "
-                               f"{args.tol_abs:g}/{args.tol_rel:g}", "samples": total, "batched": out,
-                   "autotuner": {"width_buckets": len(tuned), "seconds": tune_s,
-                                 "note": "single-chain tile shapes tuned once before the timed solves and used by "
-                                         "the per-chain runs; the batched solve keeps the fixed rule, which measured "
-                                         "faster in batches"},
-                   "subset": {"every": args.c3_baseline_every, "chains": len(sub), "samples": sub_total,
-                              "plain_f32_per_chain_samples_per_s": sub_total / base_s,
-                              "batched_bf16_samples_per_s": sub_total / sub_s,
-                              "batched_bf16_over_plain_f32": base_s / sub_s,
-                              "vs_fp64_sequential": acc_rows}},
-        "roofline": None,
-    }))
+                         "trace_err_batched_bf16": O.trace_error(S_b[q].astype(np.float64), ref),
+                         "trace_err_plain_f32": O.trace_error(Sp, ref),
+                         "accept_mismatches_batched_bf16": int(np.sum(A_b[q] != acc_ref))})
+    log(f"c3: FP64 sequential checks of {len(checked)} chains done")
+
+    # the dominant kernel's roofline: states on FP32 MFMA, directions on BF16 MFMA
+    peak_eff = 2.0 / (1.0 / PEAK_TFLOPS["f32"] + 1.0 / PEAK_BF16_TFLOPS)
+    achieved = fl_pass / (ms_pass / 1e3) / 1e12 if ms_pass > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": sps,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16+f32",
+        "data": "synthetic (seeded targets and tapes; nothing loaded)",
+        "config": {
+            "workload": (f"{len(cl)} uneven chains, T~logU[1024,65536], D~U{{2..256}}, half dense Gaussian "
+                         f"(precision condition 1e3, eps 0.5/lambda_max) half logistic N=2000 "
+                         f"(eps 0.15/lambda_max), BF16 :stoch_diag, sliding window {W}, graph-replayed "
+                         f"batched iterations, tol {args.tol_abs:g}/{args.tol_rel:g}, damping {args.damping}"),
+            "chains": len(cl),
+            "samples": total,
+            "precision": "bf16_jac",
+            "all_converged": all_conv,
+            "newton_iters_mean": {k: float(np.mean(v)) for k, v in its.items()},
+            "newton_iters_max": {k: int(np.max(v)) for k, v in its.items()},
+            "subset": {
+                "chains": len(sub), "every": args.c3_baseline_every, "samples": sub_total,
+                "plain_f32_per_chain_samples_per_s": sub_total / base_s,
+                "batched_bf16_samples_per_s": sub_total / sub_b,
+                "batched_f32_samples_per_s": sub_total / sub_f,
+                "batched_bf16_over_plain_f32": base_s / sub_b,
+                "batched_bf16_over_batched_f32": sub_f / sub_b,
+                "plain_f32_newton_iters": [int(sum(i.iters for i in inf)) for inf in base_infos],
+                "autotuner": {"width_buckets": len(tuned), "seconds": tune_s},
+                "vs_fp64_sequential": acc_rows,
+            },
+        },
+        "roofline": {
+            "kernel": "gauss_slab_batched (dense-Gaussian Lambda pass over 64-row slabs), bf16_jac",
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": peak_eff,
+            "unit": "TFLOP/s",
+            "frac": achieved / peak_eff,
+            "traffic": None,
+            "launches": n_pass,
+            "avg_launch_us": 1e3 * ms_pass / max(n_pass, 1),
+            "note": ("measured on the subset solve with HIP events around every pass launch (direct launches); "
+                     "algorithmic flops 2 x rows x D x columns (states and directions) counted on the device; "
+                     "peak: states on FP32 MFMA (157.3 TFLOP/s), directions on BF16 MFMA (2500 TFLOP/s)"),
+        },
+    }
+    if not args.no_cpu_baseline:
+        # the logistic chain of the subset whose first window the plain solve finished fastest
+        lq = [q for q in range(len(sub)) if chains[pick[q]][0] == "logreg"]
+        q = min(lq, key=lambda q: base_infos[q][0].iters * sub[q].Xi.shape[0] * min(W, sub[q].Xi.shape[1]))
+        L = min(W, sub[q].Xi.shape[1])
+        out["cpu_baseline"] = c3_cpu_baseline(sub[q], L, base_infos[q][0].iters, args, args.cpu_budget)
+    print(json.dumps(out))
 
 
 def bench_adjoint(args):
@@ -514,8 +670,11 @@ def main():
     ap.add_argument("--tol-rel", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--c3-check-tmax", type=int, default=12000,
+                    help="FP64 sequential checks of the subset chains with T up to this")
     ap.add_argument("--damping", type=float, default=1.0)
-    ap.add_argument("--workload", default="deer", choices=["deer", "scan", "scanfull", "full", "c3", "adjoint"])
+    ap.add_argument("--workload", default="c3",
+                    choices=["c3", "logistic", "deer", "scan", "scanfull", "full", "adjoint"])
     ap.add_argument("--c3-chains", type=int, default=256)
     ap.add_argument("--c3-baseline-every", type=int, default=8,
                     help="per-chain FP32 baseline on every k-th pair of chains (one Gaussian, one logistic; 1 = all)")

@@ -322,8 +322,17 @@ int pmcmc_batch_create(pmcmc_ctx* ctx, int32_t n, const pmcmc_chain_desc* chains
                        const pmcmc_deer_params* params, int64_t window, pmcmc_batch** out);
 /* 0: launch the batched kernels directly instead of replaying graphs */
 int pmcmc_batch_set_graph(pmcmc_batch* b, int32_t on);
-/* infos: n HOST records or NULL (iters summed over windows).  Synchronises. */
+/* infos: n HOST records or NULL (iters: Newton iterations of the chain's
+ * whole trajectory).  Synchronises. */
 int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos);
+/* Measurement of the dense-Gaussian Lambda passes (no reference counterpart):
+ * with profiling on, solves launch directly (no graphs), record HIP events
+ * around every Lambda-pass launch and count the passes' algorithmic flops
+ * (2 x rows x D x columns, states and directions) on the device;
+ * pmcmc_batch_pass_stats returns the last profiled solve's launches, their
+ * summed duration and the flops. */
+int pmcmc_batch_set_profiling(pmcmc_batch* b, int32_t on);
+int pmcmc_batch_pass_stats(const pmcmc_batch* b, int64_t* launches, double* total_ms, double* flops);
 int pmcmc_batch_destroy(pmcmc_batch* b);
 
 #ifdef __cplusplus

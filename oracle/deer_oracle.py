@@ -250,6 +250,38 @@ def run_mala_sequential_taped(target, x0, eps, Xi, U, check_u=True):
     return S, acc, la
 
 
+def mala_sampler_run(target, x0, eps, n, rng):
+    """``n`` steps of the sequential MALASampler (the CPU baseline).
+
+    src/interface.jl:163-194: per step a fresh xi ~ N(0, I) and u ~ U(0,1)
+    from the sampler's RNG (:174-175), then mala_step_with_logalpha!
+    (src/MALA/MALA.jl:377-411: proposal with grad(x), logp(x), logp(y),
+    grad(y), the two log-q terms, accept iff log(u) < logalpha), and
+    logdensity(x_next) re-evaluated when the step is accepted (:190).
+    Returns the D x n states, the accept mask and the logp of every state."""
+    x = np.asarray(x0, dtype=np.float64).copy()
+    D = len(x)
+    S = np.empty((D, n))
+    acc = np.zeros(n, dtype=bool)
+    lps = np.empty(n)
+    lp = target.logp(x)
+    for t in range(n):
+        xi = rng.standard_normal(D)
+        u = rng.random()
+        y, gx = mala_proposal(target, x, eps, xi)
+        lpx = target.logp(x)
+        lpy = target.logp(y)
+        gy = target.grad(y)
+        la = (lpy + logq_mala(x, y, gy, eps)) - (lpx + logq_mala(y, x, gx, eps))
+        if math.log(u) < la:
+            x = y
+            acc[t] = True
+            lp = target.logp(x)
+        S[:, t] = x
+        lps[t] = lp
+    return S, acc, lps
+
+
 def mala_step_taped_and_jvp(target, x, eps, xi, u, v):
     """Fused exact step + JVP of the sigmoid surrogate.  src/MALA/MALA.jl:732-797
     (the JVP alone is :671-725 and returns the same jvp vector)."""
@@ -811,6 +843,8 @@ def make_tape(rng, D, T, dtype=np.float64):
 # How each oracle function is pinned (tests/test_oracle_*.py).
 PINNED = {
     "run_mala_frozen": "equals run_mala_sequential_taped when fed that run's own decisions",
+    "mala_sampler_run": "equals run_mala_sequential_taped on the tape drawn from the same RNG stream in the "
+                        "same order (xi then u per step); logp of every state equals logp_batch of the states",
     "mala_adjoint": "central differences of L(run_mala_frozen) in eps, prior_scale and s0; "
                     "hard-decision differences where no decision flips",
     "logq_mala": "closed-form Gaussian log-density (scipy.stats.multivariate_normal) at 1e-12",

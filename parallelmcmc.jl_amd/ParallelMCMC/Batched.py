@@ -1,11 +1,13 @@
 """Many chains of uneven D and T in one batched DEER solve.
 
 Each chain is a taped MALA recursion (a DensityModel with a logistic or
-dense-Gaussian GPU target, its own eps, x0 and tape).  The chains are solved
-window by window; within a window round every Newton iteration of every
-chain runs as one set of batched kernel launches over per-chain descriptor
-tables, replayed from a captured hipGraph, and each chain stops at its own
-convergence (include/pmcmc_amd.h, pmcmc_batch_*).  The reference solves one
+dense-Gaussian GPU target, its own eps, x0 and tape).  Each chain's Newton
+iterations run over its own sliding window of at most ``window`` columns,
+which moves on as its prefix becomes final or the whole window meets the
+tolerance; every iteration of every live chain runs as one set of batched
+kernel launches over per-chain descriptor tables, replayed from a captured
+hipGraph, and each chain stops at the end of its own trajectory
+(include/pmcmc_amd.h, pmcmc_batch_*).  The reference solves one
 chain per call (src/interface.jl:475-515) and has no windowing.
 """
 from __future__ import annotations
@@ -123,6 +125,22 @@ class BatchPlan:
                  Info(bool(infos[i].converged), int(infos[i].iters), float(infos[i].metric), "stoch_diag",
                       self.damping, 1, self.acc[i]))
                 for i in range(self.n)]
+
+    def set_profiling(self, on=True):
+        """Measure the dense-Gaussian Lambda passes of the next solves: HIP
+        events around every launch (direct launches, no graph replay) and
+        their algorithmic flops counted on the device (``pass_stats``)."""
+        _lib.check(_lib.load().pmcmc_batch_set_profiling(self.h, 1 if on else 0), self.ctx.h,
+                   "pmcmc_batch_set_profiling")
+
+    def pass_stats(self):
+        """(launches, total_ms, flops) of the last profiled solve's Lambda passes."""
+        n = C.c_int64()
+        ms = C.c_double()
+        fl = C.c_double()
+        _lib.check(_lib.load().pmcmc_batch_pass_stats(self.h, C.byref(n), C.byref(ms), C.byref(fl)), self.ctx.h,
+                   "pmcmc_batch_pass_stats")
+        return int(n.value), float(ms.value), float(fl.value)
 
     def __del__(self):
         try:

@@ -138,6 +138,8 @@ _SIGS = {
     "pmcmc_batch_create": ([_P, C.c_int32, C.POINTER(ChainDesc), C.POINTER(DeerParams), _I64, C.POINTER(_P)],
                            C.c_int),
     "pmcmc_batch_set_graph": ([_P, C.c_int32], C.c_int),
+    "pmcmc_batch_set_profiling": ([_P, C.c_int32], C.c_int),
+    "pmcmc_batch_pass_stats": ([_P, _P, _P, _P], C.c_int),
     "pmcmc_batch_solve": ([_P, C.POINTER(DeerInfo)], C.c_int),
     "pmcmc_batch_destroy": ([_P], C.c_int),
     "pmcmc_deer_adjoint": ([_P, _P, _P, C.POINTER(DeerParams), _P, _P, _P, _P, _P, C.POINTER(C.c_double), _P, _P,

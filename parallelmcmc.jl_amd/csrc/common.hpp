@@ -158,6 +158,24 @@ struct Prof {
     }
 };
 
+// Pointers that kernels read from descriptor tables in device memory are
+// generic (flat) pointers to the compiler: flat accesses also count against
+// the LDS counter, and guarded flat loads get a full wait in front of each
+// one, which serialises a workgroup's loads.  Hot loops access memory through
+// these global-address-space views instead.
+#define PMCMC_GLOBAL __attribute__((address_space(1)))
+typedef float pmcmc_f4v __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ const PMCMC_GLOBAL T* gp(const T* p) {
+    return (const PMCMC_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ PMCMC_GLOBAL T* gp(T* p) {
+    return (PMCMC_GLOBAL T*)p;
+}
+// 16-byte load of four floats (p 16-byte aligned)
+__device__ __forceinline__ pmcmc_f4v gld4(const float* p) { return *(const PMCMC_GLOBAL pmcmc_f4v*)p; }
+
 // max that propagates NaN from either operand (fmax drops it): a NaN state
 // must fail the convergence test, as maximum() does in src/DEER/DEER.jl:567-571
 __host__ __device__ inline double nan_max(double a, double b) { return (a > b || a != a) ? a : b; }

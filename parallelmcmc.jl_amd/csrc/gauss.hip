@@ -308,94 +308,157 @@ __global__ __launch_bounds__(256) void gauss_pass_mfma_batched(const GaussPassAr
 // log-density of a column is written per slab (lpout[slab * lp_stride + t])
 // and summed over the slabs, in slab order, by its readers.
 // ---------------------------------------------------------------------------
+// One contiguous span of ncol columns (column-major, D floats each, from
+// src) into LDS columns dst[col * SV + k] (col < ncol, k < D), minus mu[k]
+// when given: 16-byte loads, every one in flight before the first LDS write.
+template <int SV>
+__device__ inline void span_to_lds(float* dst, const float* src, int D, int ncol, int tid) {
+    const int n = D * ncol;  // at most 32 x 256
+    if (n <= 0) return;
+    const uintptr_t addr = (uintptr_t)src;
+    int head = (int)(((16 - (addr & 15)) & 15) / 4);
+    if (head > n) head = n;
+    const int nv = (n - head) / 4;
+    const float* sv = src + head;
+    // element i of the span -> column i / D, row i % D (32-bit, once per vector)
+    auto put4 = [&](int i0, const float* v, int cnt) {
+        int col = (int)((unsigned)i0 / (unsigned)D);
+        int k = i0 - col * D;
+        for (int e = 0; e < cnt; ++e) {
+            dst[col * SV + k] = v[e];
+            if (++k == D) {
+                k = 0;
+                ++col;
+            }
+        }
+    };
+    for (int base = 0; base < nv || base == 0; base += 8 * 256) {
+        // every load is issued (clamped to a valid address), then the writes
+        pmcmc_f4v r[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            int idx = base + tid + 256 * q;
+            idx = idx < nv ? idx : (nv > 0 ? nv - 1 : 0);
+            r[q] = nv > 0 ? gld4(sv + 4 * idx) : pmcmc_f4v{0.f, 0.f, 0.f, 0.f};
+        }
+        const int ti = head + nv * 4 + tid;
+        float hv = 0.f, tv = 0.f;
+        if (base == 0) {
+            hv = gp(src)[tid < head ? tid : 0];
+            tv = gp(src)[ti < n ? ti : 0];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int idx = base + tid + 256 * q;
+            if (idx < nv) {
+                const float v[4] = {r[q].x, r[q].y, r[q].z, r[q].w};
+                put4(head + 4 * idx, v, 4);
+            }
+        }
+        if (base == 0) {
+            if (tid < head) put4(tid, &hv, 1);
+            if (ti < n) put4(ti, &tv, 1);
+        }
+    }
+}
+
 template <bool JBF>
 __device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const int64_t cb, const int slab) {
     // Lambda's slab rows stay in registers for the whole K: lane (li, lk) of
     // wave w holds row r0 + 16 w + li at k = 16 j + 4 lk + e (j < 16, e < 4),
-    // loaded as 16 float4 all in flight at once.  The 16x16x4 MFMA step (j, e)
-    // takes that k from every lane, and the column values come from LDS in
-    // the same order.  The 32 states, then the 32 directions, are staged in
-    // LDS (k contiguous per column) and multiplied against the same registers;
-    // the directions' values are loaded during the states' MFMAs.
-    constexpr int CPB = GCB;        // 32 states + 32 directions per block
-    constexpr int KMAX = 256;
-    constexpr int SV = KMAX + 4;    // column stride in LDS
-    constexpr int NVL = CPB * KMAX / 256;  // column values per thread per half
-    __shared__ float Vc[CPB * SV];
-    __shared__ double lpred[4][CPB];
+    // loaded as 16 float4 all in flight at once.  The block's 32 states
+    // (minus the mean) and 32 directions sit in LDS, k contiguous per column
+    // (stride SV = 4 mod 64 words, so the 16-byte B reads of a 16-lane group
+    // hit distinct banks); both are loaded as contiguous float4 spans in the
+    // same round trip as A.  The 16x16x4 MFMA step (j, e) takes k = 16 j +
+    // 4 lk + e from every lane; even and odd j accumulate separately (four
+    // independent chains for the states).  The epilogue goes through LDS so
+    // that every output and operand access is a coalesced run of rows.
+    constexpr int CPB = GCB;  // 32 states + 32 directions per block
+    constexpr int SV = 260;
+    constexpr int SP = 65;    // stride of the staged products [column][row]
+    __shared__ __align__(16) float Vs[CPB * SV];
+    __shared__ __align__(16) float Vd[CPB * SV];
+    __shared__ double lpw[CPB];
 
     const ColSrc<float>& src = a.src;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int64_t D = src.D, Tn = src.Tn;
-    const int ldL = a.ldL;   // a multiple of 32, <= 256
+    const int ldL = a.ldL;    // a multiple of 32, <= 256
     const int nj = ldL / 16;  // float4 columns of A per lane
     const int r0 = 64 * slab;
     const int row = r0 + 16 * wave + li;
     const float* mu = a.mu;
+    const int64_t t0 = cb * CPB;
+    const int nc = (int)(Tn - t0 < CPB ? Tn - t0 : CPB);  // >= 1 (items beyond Tn exit before)
 
-    float4 A[16];
+    pmcmc_f4v A[16];
+    {
+        // every load in flight at once (clamped addresses, zeros selected after)
+        const int rowc = row < ldL ? row : 0;
 #pragma unroll
-    for (int j = 0; j < 16; ++j)
-        A[j] = (j < nj && row < ldL) ? *reinterpret_cast<const float4*>(a.L32 + (size_t)row * ldL + 16 * j + 4 * lk)
-                                     : make_float4(0.f, 0.f, 0.f, 0.f);
-
-    // column values of one half (set 0: centred states, set 1: directions):
-    // thread (m, k = tid) of column m, k contiguous in HBM and in LDS
-    float cv[NVL];
-    auto fetch = [&](int set) {
-        const int64_t k = tid;
-        const float muk = (set == 0 && mu && k < D) ? mu[k] : 0.f;
-#pragma unroll
-        for (int m = 0; m < NVL; ++m) {
-            const int64_t t = cb * CPB + m;
-            float v = 0.f;
-            if (t < Tn && k < D) {
-                if (set == 0) {
-                    v = (src.x0 ? (t == 0 ? src.x0[k] : src.V[(size_t)(t - 1) * D + k]) : src.V[(size_t)t * D + k]) -
-                        muk;
-                } else {
-                    const size_t gi = (size_t)t * D + k;
-                    v = src.Z ? src.Z[gi] : rademacher(src.keys[0], gi + (size_t)src.zoff);
-                }
-            }
-            cv[m] = v;
+        for (int j = 0; j < 16; ++j) {
+            const int jc = j < nj ? j : 0;
+            A[j] = gld4(a.L32 + (size_t)rowc * ldL + 16 * jc + 4 * lk);
         }
-    };
-    auto stash = [&]() {
-        if (tid < ldL)
 #pragma unroll
-            for (int m = 0; m < NVL; ++m) Vc[m * SV + tid] = cv[m];
-    };
-
-    v4f acc[4];
-#pragma unroll
-    for (int ct = 0; ct < 4; ++ct) acc[ct] = (v4f){0.f, 0.f, 0.f, 0.f};
-
-    fetch(0);
-    stash();
+        for (int j = 0; j < 16; ++j)
+            if (!(j < nj && row < ldL)) A[j] = pmcmc_f4v{0.f, 0.f, 0.f, 0.f};
+    }
+    // states: column t is V[:, t] (or Xbar: x0, then V[:, t - 1])
+    const int Di = (int)D;
+    if (src.x0) {
+        if (t0 == 0) {
+            if (tid < Di) Vs[tid] = gp(src.x0)[tid];
+            span_to_lds<SV>(Vs + SV, src.V, Di, nc - 1, tid);
+        } else {
+            span_to_lds<SV>(Vs, src.V + (size_t)(t0 - 1) * D, Di, nc, tid);
+        }
+    } else {
+        span_to_lds<SV>(Vs, src.V + (size_t)t0 * D, Di, nc, tid);
+    }
+    // directions: the given matrix, or the probe of the block's absolute columns
+    if (src.Z) {
+        span_to_lds<SV>(Vd, src.Z + (size_t)t0 * D, Di, nc, tid);
+    } else if (tid < Di) {
+        for (int col = 0; col < nc; ++col)
+            Vd[col * SV + tid] = rademacher(src.keys[0], (size_t)(t0 + col) * D + tid + (size_t)src.zoff);
+    }
+    const float muk = (mu && tid < Di) ? gp(mu)[tid] : 0.f;
+    // zeros where the MFMAs read beyond the data: rows k in [D, ldL), columns >= nc
+    if (tid < ldL)
+        for (int col = 0; col < CPB; ++col)
+            if (tid >= Di || col >= nc) {
+                Vs[col * SV + tid] = 0.f;
+                Vd[col * SV + tid] = 0.f;
+            }
     __syncthreads();
-    fetch(1);  // in flight during the states' MFMAs
-    // states: column tiles 0, 1 (16 columns each) on FP32 MFMA (the loops over
-    // A are unrolled, so that A stays in registers)
+    if (mu && tid < Di) {  // the states, centred (each thread its own row k = tid)
+        for (int col = 0; col < nc; ++col) Vs[col * SV + tid] -= muk;
+        __syncthreads();
+    }
+
+    v4f acc[4][2];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) acc[ct][0] = acc[ct][1] = (v4f){0.f, 0.f, 0.f, 0.f};
+    // states: column tiles 0, 1 on FP32 MFMA (the loops over A are unrolled,
+    // so that A stays in registers)
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         if (j < nj) {
             const float av[4] = {A[j].x, A[j].y, A[j].z, A[j].w};
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int k = 16 * j + 4 * lk + e;
+            for (int ct = 0; ct < 2; ++ct) {
+                const float4 b = *reinterpret_cast<const float4*>(&Vs[(16 * ct + li) * SV + 16 * j + 4 * lk]);
+                const float bv[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
-                for (int ct = 0; ct < 2; ++ct)
-                    acc[ct] =
-                        __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], Vc[(16 * ct + li) * SV + k], acc[ct], 0, 0, 0);
+                for (int e = 0; e < 4; ++e)
+                    acc[ct][j & 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], bv[e], acc[ct][j & 1], 0, 0, 0);
             }
         }
     }
-    __syncthreads();  // the states' reads of Vc are done
-    stash();
-    __syncthreads();
     // directions: column tiles 2, 3 (BF16 MFMA, 32 k per step: the lane's
     // eight values of two float4 columns, in the same k order for A and B)
     if (JBF) {
@@ -405,18 +468,21 @@ __device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const in
             v8bf av, b0, b1;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const float4 f = A[2 * q + h];
+                const pmcmc_f4v f = A[2 * q + h];
+                const float4 c0 = *reinterpret_cast<const float4*>(&Vd[li * SV + 16 * (2 * q + h) + 4 * lk]);
+                const float4 c1 = *reinterpret_cast<const float4*>(&Vd[(16 + li) * SV + 16 * (2 * q + h) + 4 * lk]);
                 const float fv[4] = {f.x, f.y, f.z, f.w};
+                const float v0[4] = {c0.x, c0.y, c0.z, c0.w};
+                const float v1[4] = {c1.x, c1.y, c1.z, c1.w};
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int k = 16 * (2 * q + h) + 4 * lk + e;
                     av[4 * h + e] = (__bf16)fv[e];
-                    b0[4 * h + e] = (__bf16)Vc[li * SV + k];
-                    b1[4 * h + e] = (__bf16)Vc[(16 + li) * SV + k];
+                    b0[4 * h + e] = (__bf16)v0[e];
+                    b1[4 * h + e] = (__bf16)v1[e];
                 }
             }
-            acc[2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b0, acc[2], 0, 0, 0);
-            acc[3] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b1, acc[3], 0, 0, 0);
+            acc[2][q & 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b0, acc[2][q & 1], 0, 0, 0);
+            acc[3][q & 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b1, acc[3][q & 1], 0, 0, 0);
         }
     } else {
 #pragma unroll
@@ -424,58 +490,65 @@ __device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const in
             if (j >= nj) continue;
             const float av[4] = {A[j].x, A[j].y, A[j].z, A[j].w};
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int k = 16 * j + 4 * lk + e;
+            for (int ct = 0; ct < 2; ++ct) {
+                const float4 b = *reinterpret_cast<const float4*>(&Vd[(16 * ct + li) * SV + 16 * j + 4 * lk]);
+                const float bv[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
-                for (int ct = 0; ct < 2; ++ct)
-                    acc[2 + ct] =
-                        __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], Vc[(16 * ct + li) * SV + k], acc[2 + ct], 0, 0, 0);
+                for (int e = 0; e < 4; ++e)
+                    acc[2 + ct][j & 1] =
+                        __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], bv[e], acc[2 + ct][j & 1], 0, 0, 0);
             }
         }
     }
-
-    // epilogue: lane holds P[d = r0 + 16 wave + 4 lk + r][column 16 ct + li]
-    // (column tiles 0-1: the states t = cb*32 + 16 ct + li; 2-3: their directions)
-    const double eps = (double)a.eps;
-    const double sq2e = sqrt(2.0 * eps);
-    double lpc[2] = {0.0, 0.0};
+    __syncthreads();  // the MFMAs' reads of Vs are done: Vs now stages the products
+    // lane holds P[row 16 wave + 4 lk + r][column 16 ct + li] (columns 0-31:
+    // the states, 32-63: their directions); staged as Pt[column][row]
+    float* Pt = Vs;
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) {
-        const int set = ct >> 1;
-        const int64_t t = cb * CPB + 16 * (ct & 1) + li;
-        if (t >= Tn) continue;
+        const v4f p = acc[ct][0] + acc[ct][1];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int64_t d = r0 + 16 * wave + 4 * lk + r;
-            if (d >= D) continue;
-            const size_t gi = (size_t)t * D + d;
-            const float g = -acc[ct][r];
-            if (set == 0) {
-                const double x = src.get(0, t, d);
-                const double xm = x - (mu ? (double)mu[d] : 0.0);
-                lpc[ct] += 0.5 * xm * (double)g;
-                a.gout[gi] = g;
-                if (a.pass == 0) a.Yout[gi] = (float)(x + eps * g + sq2e * (double)a.Xi[gi]);
-            } else {
-                if (a.pass == 0)
-                    a.Wout[gi] = (float)(src.get(1, t, d) + eps * g);
-                else
-                    a.hout[gi] = g;
-            }
-        }
-    }
-#pragma unroll
-    for (int ct = 0; ct < 2; ++ct) {
-        lpc[ct] += __shfl_xor(lpc[ct], 16, 64);
-        lpc[ct] += __shfl_xor(lpc[ct], 32, 64);
-        if (lk == 0) lpred[wave][16 * ct + li] = lpc[ct];
+        for (int r = 0; r < 4; ++r) Pt[(16 * ct + li) * SP + 16 * wave + 4 * lk + r] = p[r];
     }
     __syncthreads();
-    if (tid < CPB) {
-        const int64_t t = cb * CPB + tid;
-        if (t < Tn)
-            a.lpout[(size_t)slab * a.lp_stride + t] =
-                (lpred[0][tid] + lpred[1][tid]) + (lpred[2][tid] + lpred[3][tid]);
+
+    // outputs, one column per wave and round, lanes over the slab's 64 rows
+    const double eps = (double)a.eps;
+    const double sq2e = sqrt(2.0 * eps);
+    const int64_t d = r0 + lane;
+    const bool drow = d < D;
+#pragma unroll 2
+    for (int q = 0; q < 16; ++q) {
+        const int j = wave + 4 * q;  // column of the block: 0-31 states, 32-63 directions
+        const int jc = j & (CPB - 1);
+        const int64_t t = t0 + jc;
+        const bool live = jc < nc;
+        const float g = -Pt[j * SP + lane];
+        const size_t gi = (size_t)t * D + d;
+        if (j < CPB) {
+            double lpp = 0.0;
+            if (live && drow) {
+                const float* xp = src.x0 ? (t == 0 ? src.x0 + d : src.V + gi - D) : src.V + gi;
+                const double x = (double)*gp(xp);
+                const double xm = x - (mu ? (double)gp(mu)[d] : 0.0);
+                lpp = 0.5 * xm * (double)g;
+                gp(a.gout)[gi] = g;
+                if (a.pass == 0) gp(a.Yout)[gi] = (float)(x + eps * g + sq2e * (double)gp(a.Xi)[gi]);
+            }
+            for (int m = 32; m >= 1; m >>= 1) lpp += __shfl_xor(lpp, m, 64);
+            if (lane == 0) lpw[jc] = lpp;
+        } else if (live && drow) {
+            if (a.pass == 0)
+                gp(a.Wout)[gi] = (float)((double)Vd[jc * SV + d] + eps * g);
+            else
+                gp(a.hout)[gi] = g;
+        }
+    }
+    __syncthreads();
+    if (tid < nc) gp(a.lpout)[(size_t)slab * a.lp_stride + t0 + tid] = lpw[tid];
+    if (a.flops && tid == 0) {
+        const int64_t rows = D - r0 < 64 ? D - r0 : 64;
+        if (rows > 0) atomicAdd(a.flops, (unsigned long long)(2 * rows * D * 2 * nc));
     }
 }
 

@@ -39,6 +39,9 @@ struct GaussPassArgs {
     int ldL;  // Lambda's padded width (row stride of L32), <= 16 * dpt of the launch
     // slab passes: the log-density partial of slab s goes to lpout[s * lp_stride + t]
     int64_t lp_stride;
+    // non-null: every workgroup adds its algorithmic flops (2 x rows x D x
+    // columns, states and directions) here
+    unsigned long long* flops;
 };
 
 // Batched FP32 Lambda passes over 64-row slabs (one probe set), any widths in

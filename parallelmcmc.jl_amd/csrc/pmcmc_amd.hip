@@ -2077,6 +2077,13 @@ struct pmcmc_batch {
     size_t tables_cap = 0;
     void* items = nullptr;         // device work-item lists
     size_t items_cap = 0;
+    // profiling of the Lambda passes (pmcmc_batch_set_profiling): event pairs
+    // around every launch (direct launches, no graph) and the passes'
+    // algorithmic flops counted on the device
+    bool prof_on = false;
+    Prof prof;
+    unsigned long long* flop_ctr = nullptr;  // device
+    double prof_flops = 0;
 };
 
 namespace {
@@ -2121,6 +2128,8 @@ static void batch_free(pmcmc_batch* b) {
     hipHostFree(b->hspans);
     hipFree(b->tables);
     hipFree(b->items);
+    hipFree(b->flop_ctr);
+    b->flop_ctr = nullptr;
     b->items = nullptr;
     b->arena = nullptr;
     b->syncs = nullptr;
@@ -2310,6 +2319,25 @@ extern "C" int pmcmc_batch_set_graph(pmcmc_batch* b, int32_t on) {
     return PMCMC_OK;
 }
 
+extern "C" int pmcmc_batch_set_profiling(pmcmc_batch* b, int32_t on) {
+    if (!b) return PMCMC_ERR_ARGUMENT;
+    pmcmc_ctx* c = b->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (on && !b->flop_ctr) HIPCHK(c, hipMalloc(&b->flop_ctr, sizeof(unsigned long long)));
+    b->prof_on = on != 0;
+    b->prof.on = b->prof_on;
+    b->prof.kinds = 1u << KT_GLM_PASS;
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_batch_pass_stats(const pmcmc_batch* b, int64_t* launches, double* total_ms, double* flops) {
+    if (!b) return PMCMC_ERR_ARGUMENT;
+    if (launches) *launches = b->prof.n[KT_GLM_PASS];
+    if (total_ms) *total_ms = b->prof.ms[KT_GLM_PASS];
+    if (flops) *flops = b->prof_flops;
+    return PMCMC_OK;
+}
+
 namespace {
 // host mirrors of the per-chain descriptor tables (indexed by chain)
 struct BatchTables {
@@ -2469,6 +2497,7 @@ static void batch_tables(pmcmc_batch* b, BatchTables* bt) {
                 qa.Wout = v < 2 ? h.W : nullptr;
                 qa.skip = skip;
                 qa.lp_stride = L;
+                qa.flops = b->prof_on ? b->flop_ctr : nullptr;
                 bt->qpass[v].push_back(qa);
                 bt->gpass[v].push_back(GlmPassArgs{});
                 bt->gred[v].push_back(GlmReduceArgs{});
@@ -2607,6 +2636,12 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const int n = (int)b->ch.size();
+    const bool graphs = b->use_graph && !b->prof_on;  // profiled solves launch directly
+    if (b->prof_on) {
+        b->prof.reset();
+        b->prof_flops = 0;
+        HIPCHK(c, hipMemsetAsync(b->flop_ctr, 0, sizeof(unsigned long long), s));
+    }
     for (auto& h : b->ch) {
         h.iters = 0;
         h.converged = true;
@@ -2693,15 +2728,23 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
         for (int d = 1; d <= 16; ++d)
             launch_glm_pass_batched(d, b->jbf, (const GlmPassArgs*)at(o_gp[par]), (const int4*)ia(o_gi[d]),
                                     (int)it.glm[d].size(), s);
-        launch_gauss_slab_batched(b->jbf, (const GaussPassArgs*)at(o_qp[par]), (const int4*)ia(o_qs),
-                                  (int)it.gslab.size(), s);
+        {
+            const int pe = it.gslab.empty() ? -1 : b->prof.begin(KT_GLM_PASS, 0, s);
+            launch_gauss_slab_batched(b->jbf, (const GaussPassArgs*)at(o_qp[par]), (const int4*)ia(o_qs),
+                                      (int)it.gslab.size(), s);
+            b->prof.end(pe, s);
+        }
         launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[par]), (const int2*)ia(o_glmred),
                                   (int)it.glm_red.size(), s);
         for (int d = 1; d <= 16; ++d)
             launch_glm_pass_batched(d, b->jbf, (const GlmPassArgs*)at(o_gp[2]), (const int4*)ia(o_gi[d]),
                                     (int)it.glm[d].size(), s);
-        launch_gauss_slab_batched(b->jbf, (const GaussPassArgs*)at(o_qp[2]), (const int4*)ia(o_qs),
-                                  (int)it.gslab.size(), s);
+        {
+            const int pe = it.gslab.empty() ? -1 : b->prof.begin(KT_GLM_PASS, 0, s);
+            launch_gauss_slab_batched(b->jbf, (const GaussPassArgs*)at(o_qp[2]), (const int4*)ia(o_qs),
+                                      (int)it.gslab.size(), s);
+            b->prof.end(pe, s);
+        }
         launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[2]), (const int2*)ia(o_glmred),
                                   (int)it.glm_red.size(), s);
         if (b->refine) {
@@ -2758,7 +2801,7 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
         const int rc2 = upload(c, s, ip, &b->items, &b->items_cap);
         if (rc2 != PMCMC_OK) return rc2;
         built_live = nlive;
-        if (!b->use_graph) return PMCMC_OK;
+        if (!graphs) return PMCMC_OK;
         HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         iteration(0);
         iteration(1);
@@ -2779,7 +2822,7 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
     while (nlive > 0) {
         const int m = chunk;  // even
         for (int j = 0; j < m / 2; ++j) {
-            if (b->use_graph) {
+            if (graphs) {
                 const hipError_t el = hipGraphLaunch(exec, s);
                 if (el != hipSuccess) {
                     drop_graph();
@@ -2795,6 +2838,7 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
         HIPCHK(c, hipMemcpyAsync(b->hsync, b->syncs, n * sizeof(SyncState), hipMemcpyDeviceToHost, s));
         HIPCHK(c, hipMemcpyAsync(b->hslide, b->slides, n * sizeof(SlideState), hipMemcpyDeviceToHost, s));
         HIPCHK(c, hipStreamSynchronize(s));
+        if (b->prof_on) b->prof.flush(INT_MAX);
         nlive = 0;
         bool changed = false, stop_any = false;
         for (int q = 0; q < n; ++q) {
@@ -2848,6 +2892,11 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
     HIPCHK(c, hipMemcpyAsync(b->hsync, b->syncs, n * sizeof(SyncState), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipMemcpyAsync(b->hslide, b->slides, n * sizeof(SlideState), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
+    if (b->prof_on) {
+        unsigned long long fl = 0;
+        HIPCHK(c, hipMemcpy(&fl, b->flop_ctr, sizeof fl, hipMemcpyDeviceToHost));
+        b->prof_flops = (double)fl;
+    }
     for (int i = 0; i < n; ++i) {
         const SyncState& hs = b->hsync[i];
         BatchChain& h = b->ch[i];

@@ -57,6 +57,12 @@ enum { PASS_REDUCE = 0, PASS_APPLY = 1, PASS_SINGLE = 2, PASS_CHAINED = 3 };
 #define PMCMC_LB_SPIN_CAP (1u << 21)
 enum { JAC_STOCH = 0, JAC_DIAG = 1 };
 #define PMCMC_MAXP 4
+#ifndef PMCMC_FAST_RM
+#define PMCMC_FAST_RM 16
+#endif
+#ifndef PMCMC_BATCHED_WAVES
+#define PMCMC_BATCHED_WAVES 2
+#endif
 
 // Base of the Newton correction of one entry: the iterate, or 0 where the
 // iterate is not finite, so that a non-finite region of S_in shrinks by (at
@@ -311,6 +317,64 @@ __device__ inline void stage_lds(T* dst, const T* src, int64_t n, int tid) {
     for (int64_t i = head + nv * VEC + tid; i < n; i += PMCMC_NT) dst[i] = src[i];
 }
 
+// Copy up to NR regions from global memory into LDS in one round trip: every
+// region's 16-byte loads (up to eight per thread per region and round) are in
+// flight before the first LDS write.  Regions need no common alignment.
+template <typename T, int NR>
+__device__ inline void stage_lds_multi(T* const (&dst)[NR], const T* const (&src)[NR], const int64_t (&n)[NR],
+                                       int tid) {
+    typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type V;
+    constexpr int VEC = 16 / sizeof(T);
+    int64_t head[NR], nv[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const uintptr_t addr = (uintptr_t)src[r];
+        head[r] = (int64_t)(((16 - (addr & 15)) & 15) / sizeof(T));
+        if (head[r] > n[r]) head[r] = n[r];
+        nv[r] = (n[r] - head[r]) / VEC;
+    }
+    int64_t nmax = 0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) nmax = nv[r] > nmax ? nv[r] : nmax;
+    for (int64_t base = 0; base < nmax || base == 0; base += 8 * PMCMC_NT) {
+        V rv[NR][8];
+        T hs[NR], ts[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const V* sv = reinterpret_cast<const V*>(src[r] + head[r]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int64_t idx = base + tid + (int64_t)k * PMCMC_NT;
+                if (idx < nv[r]) rv[r][k] = sv[idx];
+            }
+            // the unaligned head and the tail: at most one element per thread each
+            if (base == 0) {
+                if (tid < head[r]) hs[r] = src[r][tid];
+                const int64_t ti = head[r] + nv[r] * VEC + tid;
+                if (ti < n[r]) ts[r] = src[r][ti];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            T* dh = dst[r] + head[r];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int64_t idx = base + tid + (int64_t)k * PMCMC_NT;
+                if (idx < nv[r]) {
+                    const T* e = reinterpret_cast<const T*>(&rv[r][k]);
+#pragma unroll
+                    for (int j = 0; j < VEC; ++j) dh[idx * VEC + j] = e[j];
+                }
+            }
+            if (base == 0) {
+                if (tid < head[r]) dst[r][tid] = hs[r];
+                const int64_t ti = head[r] + nv[r] * VEC + tid;
+                if (ti < n[r]) dst[r][ti] = ts[r];
+            }
+        }
+    }
+}
+
 // Publication between workgroups of one launch: values are written and read
 // with agent-scope relaxed atomics (coherent across the XCDs' L2s without an
 // L2 write-back per publication).  Before the status word is stored, each
@@ -442,8 +506,10 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
     __shared__ double s_red[2][PMCMC_NT / 64];
 
     const int tid = threadIdx.x;
-    // Newton loop already converged: later queued iterations are no-ops.
-    if (a.skip && *(volatile const int*)a.skip) return;
+    // Newton loop already converged: later queued iterations are no-ops.  The
+    // flag is loaded here and tested once this tile's staging loads are in
+    // flight too (one memory round trip for both).
+    const int skip_now = a.skip ? *(volatile const int*)a.skip : 0;
     const int64_t D = a.D;
     const int TC = a.TC;
     const int64_t c0 = (int64_t)tile * TC;
@@ -541,19 +607,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
             }
         }
     };
-    if constexpr (DEERM && (PASS == PASS_REDUCE || PASS == PASS_APPLY)) {
-        if (a.final_cols && c0 + TC <= (int64_t)*(volatile const int*)a.final_cols) {
-            if constexpr (PASS == PASS_REDUCE) {
-                for (int64_t r = tid; r < D; r += PMCMC_NT) {
-                    a.agg_a[(size_t)tile * D + r] = (T)1;
-                    a.agg_b[(size_t)tile * D + r] = (T)0;
-                }
-            } else {
-                finish(0.0, tid == 0 ? a.tile_smax[tile] : 0.0, false, true);
-            }
-            return;
-        }
-    }
+    const int final_now = (DEERM && a.final_cols) ? *(volatile const int*)a.final_cols : 0;
 
     T* xs = reinterpret_cast<T*>(smem);
     T* bl = xs + (DEERM ? (size_t)(TC + 1) * D : (size_t)0);
@@ -568,27 +622,49 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
     const int64_t nel = (int64_t)tc * D;
     const int64_t gbase = c0 * D;
 
-    // ---- phase 0: stage inputs (coalesced) -------------------------------
+    // ---- phase 0: stage inputs (coalesced, every region in one round) -------
     // A and B kept by the reduce pass of this iteration (MODE_GEMM multi-tile)
     const bool keep = DEERM && (PASS == PASS_REDUCE || PASS == PASS_APPLY) && a.a_keep != nullptr;
     const bool build = DEERM && !(PASS == PASS_APPLY && keep);
     if (!DEERM) {
-        stage_lds<T>(al, a.A + gbase, nel, tid);
-        stage_lds<T>(bl, a.B + gbase, nel, tid);
+        T* const dst[2] = {al, bl};
+        const T* const src[2] = {a.A + gbase, a.B + gbase};
+        const int64_t n[2] = {nel, nel};
+        stage_lds_multi<T, 2>(dst, src, n, tid);
     } else {
-        if (c0 == 0) {
+        // the iterate's columns c0-1 .. c0+tc-1 (column c0-1 is s0 for the first tile)
+        T* xd = c0 == 0 ? xs + D : xs;
+        const T* xsrc = c0 == 0 ? a.S_in : a.S_in + gbase - D;
+        const int64_t xn = c0 == 0 ? nel : nel + D;
+        if (c0 == 0)
             for (int64_t i = tid; i < D; i += PMCMC_NT) xs[i] = a.s0[i];
-            stage_lds<T>(xs + D, a.S_in, nel, tid);
-        } else {
-            stage_lds<T>(xs, a.S_in + gbase - D, nel + D, tid);
-        }
         if (build) {
-            stage_lds<T>(bl, a.Xi + gbase, nel, tid);
+            T* const dst[2] = {xd, bl};
+            const T* const src[2] = {xsrc, a.Xi + gbase};
+            const int64_t n[2] = {xn, nel};
+            stage_lds_multi<T, 2>(dst, src, n, tid);
         } else {
-            stage_lds<T>(al, a.a_keep + gbase, nel, tid);
-            stage_lds<T>(bl, a.b_keep + gbase, nel, tid);
+            T* const dst[3] = {xd, al, bl};
+            const T* const src[3] = {xsrc, a.a_keep + gbase, a.b_keep + gbase};
+            const int64_t n[3] = {xn, nel, nel};
+            stage_lds_multi<T, 3>(dst, src, n, tid);
         }
     }
+    if (skip_now) return;
+    if constexpr (DEERM && (PASS == PASS_REDUCE || PASS == PASS_APPLY)) {
+        if (a.final_cols && c0 + TC <= (int64_t)final_now) {
+            if constexpr (PASS == PASS_REDUCE) {
+                for (int64_t r = tid; r < D; r += PMCMC_NT) {
+                    a.agg_a[(size_t)tile * D + r] = (T)1;
+                    a.agg_b[(size_t)tile * D + r] = (T)0;
+                }
+            } else {
+                finish(0.0, tid == 0 ? a.tile_smax[tile] : 0.0, false, true);
+            }
+            return;
+        }
+    }
+
     // incoming state of the tile (needed only by the output passes)
     if (OUT && PASS != PASS_CHAINED) {
         for (int64_t r = tid; r < D; r += PMCMC_NT)
@@ -605,10 +681,11 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         // of two), so each group has one or two columns of the tile and the
         // column's reductions take log2(G) <= 6 shuffle steps.  Otherwise one
         // row per lane (G = D rounded up, at most 64).
-        const bool fast = MODE == MODE_GEMM && K == 1 && D <= 8 * 64;
+        constexpr int RM = PMCMC_FAST_RM;  // rows per lane of the fast path
+        const bool fast = MODE == MODE_GEMM && K == 1 && D <= RM * 64;
         int G = 1;
         if (fast)
-            while (G * 8 < D) G <<= 1;
+            while (G * RM < D) G <<= 1;
         else
             while (G < D && G < 64) G <<= 1;
         const int lane = tid & (G - 1);
@@ -699,7 +776,6 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         if (fast) {
             // the column's operands are loaded once, all in flight together,
             // and kept in registers for both sweeps
-            constexpr int RM = 8;
             for (int c = grp; c < tc; c += ngrp) {
                 const int64_t t = c0 + c;
                 T vgx[RM], vy[RM], vgy[RM], vw[RM], vhw[RM];
@@ -1025,7 +1101,7 @@ __global__ __launch_bounds__(PMCMC_NT, sizeof(T) == 4 ? 3 : 2) void tile_kernel(
 // Batched launch over many problems: work item blockIdx.x is (problem, tile) =
 // items[blockIdx.x]; args[problem] describes the problem.
 template <typename T, int MODE, int PASS>
-__global__ __launch_bounds__(PMCMC_NT, sizeof(T) == 4 ? 3 : 2) void tile_kernel_batched(const TileArgs<T>* args, const int2* items) {
+__global__ __launch_bounds__(PMCMC_NT, PMCMC_BATCHED_WAVES) void tile_kernel_batched(const TileArgs<T>* args, const int2* items) {
     const int2 w = items[blockIdx.x];
     if (w.y >= args[w.x].numTiles) return;  // beyond the chain's current window
     tile_body<T, MODE, PASS>(args[w.x], w.y);

@@ -295,3 +295,28 @@ def test_gaussian_structured_jacobian_form():
         u = g * (1 - g) * (y - x)
         Js = np.eye(D) - (1.0 if acc[t] else 0.0) * eps * Lam + np.outer(u, c)
         assert np.allclose(Js, J[t], rtol=1e-11, atol=1e-12)
+
+
+def test_mala_sampler_run_is_the_taped_chain_of_its_own_draws():
+    """The MALASampler restatement (src/interface.jl:163-194) draws xi then u
+    per step; fed the same draws as a tape, sequential taped MALA gives the
+    same states and decisions, and the cached logp equals logp of the state."""
+    import numpy as np
+
+    from oracle import deer_oracle as O
+
+    rng = np.random.default_rng(5)
+    Xd, y, _ = O.make_logreg_data(rng, 300, 6)
+    tgt = O.LogReg(Xd, y)
+    x0 = 0.1 * rng.standard_normal(6)
+    S, acc, lps = O.mala_sampler_run(tgt, x0, 0.05, 200, np.random.default_rng(11))
+    r = np.random.default_rng(11)
+    Xi = np.empty((6, 200))
+    U = np.empty(200)
+    for t in range(200):
+        Xi[:, t] = r.standard_normal(6)
+        U[t] = r.random()
+    S_ref, acc_ref, _ = O.run_mala_sequential_taped(tgt, x0, 0.05, Xi, U)
+    assert np.array_equal(acc, acc_ref) and 0 < acc.sum() < 200
+    assert np.allclose(S, S_ref, rtol=0, atol=1e-14)
+    assert np.allclose(lps, tgt.logp_batch(S), rtol=1e-12, atol=1e-12)
