@@ -173,8 +173,26 @@ template <typename T>
 __device__ __forceinline__ PMCMC_GLOBAL T* gp(T* p) {
     return (PMCMC_GLOBAL T*)p;
 }
+typedef double pmcmc_d2v __attribute__((ext_vector_type(2)));
 // 16-byte load of four floats (p 16-byte aligned)
 __device__ __forceinline__ pmcmc_f4v gld4(const float* p) { return *(const PMCMC_GLOBAL pmcmc_f4v*)p; }
+// 16-byte vector types and loads by element type
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef pmcmc_f4v type;
+    static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+    typedef pmcmc_d2v type;
+    static constexpr int n = 2;
+};
+template <typename T>
+__device__ __forceinline__ typename Vec16<T>::type gld16(const T* p) {
+    return *(const PMCMC_GLOBAL typename Vec16<T>::type*)p;
+}
 
 // max that propagates NaN from either operand (fmax drops it): a NaN state
 // must fail the convergence test, as maximum() does in src/DEER/DEER.jl:567-571

@@ -72,6 +72,14 @@ __device__ inline T corr_base(T v) {
     return isfinite(v) ? v : (T)0;
 }
 
+// Operands of one column in the fast GEMM build: RM rows per lane.
+template <typename T, int RM>
+struct FastOps {
+    T gx[RM], y[RM], gy[RM], w[RM], hw[RM];
+    double lpx, lpy, u;
+    int8_t o, pv;
+};
+
 template <typename T>
 struct TileArgs {
     // scan operands (MODE_SCAN)
@@ -323,35 +331,34 @@ __device__ inline void stage_lds(T* dst, const T* src, int64_t n, int tid) {
 template <typename T, int NR>
 __device__ inline void stage_lds_multi(T* const (&dst)[NR], const T* const (&src)[NR], const int64_t (&n)[NR],
                                        int tid) {
-    typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type V;
-    constexpr int VEC = 16 / sizeof(T);
+    typedef typename Vec16<T>::type V;
+    constexpr int VEC = Vec16<T>::n;
     int64_t head[NR], nv[NR];
+    const T* s0[NR];  // 16-byte aligned start of each region's vectors
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         const uintptr_t addr = (uintptr_t)src[r];
         head[r] = (int64_t)(((16 - (addr & 15)) & 15) / sizeof(T));
         if (head[r] > n[r]) head[r] = n[r];
         nv[r] = (n[r] - head[r]) / VEC;
+        // with no full vector, loads go to the aligned block holding src[0]
+        // (same page), and their values are not used
+        s0[r] = nv[r] > 0 ? src[r] + head[r] : (const T*)(addr & ~(uintptr_t)15);
     }
     int64_t nmax = 0;
 #pragma unroll
     for (int r = 0; r < NR; ++r) nmax = nv[r] > nmax ? nv[r] : nmax;
-    for (int64_t base = 0; base < nmax || base == 0; base += 8 * PMCMC_NT) {
+    // one round: every region's loads issued (global address space, indices
+    // clamped to valid vectors, no branches), then the LDS writes
+    auto round = [&](int64_t base) {
         V rv[NR][8];
-        T hs[NR], ts[NR];
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            const V* sv = reinterpret_cast<const V*>(src[r] + head[r]);
+            const int64_t last = nv[r] > 0 ? nv[r] - 1 : 0;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const int64_t idx = base + tid + (int64_t)k * PMCMC_NT;
-                if (idx < nv[r]) rv[r][k] = sv[idx];
-            }
-            // the unaligned head and the tail: at most one element per thread each
-            if (base == 0) {
-                if (tid < head[r]) hs[r] = src[r][tid];
-                const int64_t ti = head[r] + nv[r] * VEC + tid;
-                if (ti < n[r]) ts[r] = src[r][ti];
+                rv[r][k] = gld16<T>(s0[r] + (idx < last ? idx : last) * VEC);
             }
         }
 #pragma unroll
@@ -361,17 +368,27 @@ __device__ inline void stage_lds_multi(T* const (&dst)[NR], const T* const (&src
             for (int k = 0; k < 8; ++k) {
                 const int64_t idx = base + tid + (int64_t)k * PMCMC_NT;
                 if (idx < nv[r]) {
-                    const T* e = reinterpret_cast<const T*>(&rv[r][k]);
 #pragma unroll
-                    for (int j = 0; j < VEC; ++j) dh[idx * VEC + j] = e[j];
+                    for (int j = 0; j < VEC; ++j) dh[idx * VEC + j] = rv[r][k][j];
                 }
             }
-            if (base == 0) {
-                if (tid < head[r]) dst[r][tid] = hs[r];
-                const int64_t ti = head[r] + nv[r] * VEC + tid;
-                if (ti < n[r]) dst[r][ti] = ts[r];
-            }
         }
+    };
+    // the unaligned heads and tails: at most one element per thread each
+    T hs[NR], ts[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int64_t ti = head[r] + nv[r] * VEC + tid;
+        hs[r] = gp(src[r])[tid < head[r] ? tid : 0];
+        ts[r] = gp(src[r])[ti < n[r] ? ti : 0];
+    }
+    round(0);  // straight-line: the common case
+    for (int64_t base = 8 * PMCMC_NT; base < nmax; base += 8 * PMCMC_NT) round(base);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        if (tid < head[r]) dst[r][tid] = hs[r];
+        const int64_t ti = head[r] + nv[r] * VEC + tid;
+        if (ti < n[r]) dst[r][ti] = ts[r];
     }
 }
 
@@ -637,7 +654,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         const T* xsrc = c0 == 0 ? a.S_in : a.S_in + gbase - D;
         const int64_t xn = c0 == 0 ? nel : nel + D;
         if (c0 == 0)
-            for (int64_t i = tid; i < D; i += PMCMC_NT) xs[i] = a.s0[i];
+            for (int64_t i = tid; i < D; i += PMCMC_NT) xs[i] = gp(a.s0)[i];
         if (build) {
             T* const dst[2] = {xd, bl};
             const T* const src[2] = {xsrc, a.Xi + gbase};
@@ -668,8 +685,8 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
     // incoming state of the tile (needed only by the output passes)
     if (OUT && PASS != PASS_CHAINED) {
         for (int64_t r = tid; r < D; r += PMCMC_NT)
-            sin_[r] = (PASS == PASS_SINGLE || tile == 0) ? ((DEERM || !a.s0) ? (T)0 : a.s0[r])
-                                                         : a.prev[(size_t)(tile - 1) * D + r];
+            sin_[r] = (PASS == PASS_SINGLE || tile == 0) ? ((DEERM || !a.s0) ? (T)0 : gp(a.s0)[r])
+                                                         : gp(a.prev)[(size_t)(tile - 1) * D + r];
     }
     __syncthreads();
 
@@ -709,23 +726,46 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         __shared__ int8_t fl_pv[FLMAX];
         __shared__ double fl_dg[FLMAX];
         __shared__ double fl_v[256];
+        // a column's log-density: the slab partials summed in slab order
+        auto lp_of = [&](const double* lp, int64_t t) -> double {
+            double v = gp(lp)[t];
+            for (int sl = 1; sl < a.lp_nslab; ++sl) v += gp(lp)[(size_t)sl * a.lp_stride + t];
+            return v;
+        };
+        auto load_ops = [&](int c, FastOps<T, RM>& f) {
+            const int64_t t = c0 + (c < tc ? c : 0);
+            // every operand load in flight at once: global address space,
+            // rows beyond D clamped to a valid row (their values unused)
+#pragma unroll
+            for (int i = 0; i < RM; ++i) {
+                const int64_t d = lane + (int64_t)i * G;
+                const size_t gi = (size_t)t * D + (d < D ? d : D - 1);
+                f.gx[i] = gp(a.gx)[gi];
+                f.y[i] = gp(a.Y)[gi];
+                f.gy[i] = gp(a.gy)[gi];
+                f.w[i] = gp(a.W)[gi];
+                f.hw[i] = gp(a.hw)[gi];
+            }
+            f.lpx = lp_of(a.lpx, t);
+            f.lpy = lp_of(a.lpy, t);
+            f.u = (double)gp(a.U)[t];
+            f.o = a.dec ? gp(a.dec)[t] : (int8_t)-1;
+            f.pv = a.prev_dec ? gp(a.prev_dec)[t] : (int8_t)-1;
+        };
+        // the first round's operands, in flight across the gate's barrier
+        FastOps<T, RM> ops0;
+        if (fast) load_ops(grp, ops0);
         const bool inl = MODE == MODE_GEMM && a.rtg != nullptr && side;
         bool inl_on = false;
         if (inl) {
-            inl_on = *(volatile const int*)a.rflag != 0 || a.sync->last_metric <= a.gate ||
-                     a.sync->last_iter + 1 >= a.gate_iters;
+            inl_on = *gp(a.rflag) != 0 || *gp(&a.sync->last_metric) <= a.gate ||
+                     *gp(&a.sync->last_iter) + 1 >= a.gate_iters;
             if (tid == 0) {
                 fl_n = 0;
                 if (inl_on && tile == 0) *a.rflag = 1;
             }
             __syncthreads();
         }
-        // a column's log-density: the slab partials summed in slab order
-        auto lp_of = [&](const double* lp, int64_t t) -> double {
-            double v = lp[t];
-            for (int sl = 1; sl < a.lp_nslab; ++sl) v += lp[(size_t)sl * a.lp_stride + t];
-            return v;
-        };
 
         // Accept decision of column c from its (group-summed) sums; returns it
         // and sets gg = g (1 - g) of the acceptance probability g and dg[k],
@@ -765,35 +805,21 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                     }
                 }
                 if (!listed) {
-                    if (a.prev_dec) a.prev_dec[t] = acc ? 1 : 0;
-                    if (a.acc_out) a.acc_out[t] = acc ? 1 : 0;
+                    if (a.prev_dec) gp(a.prev_dec)[t] = acc ? 1 : 0;
+                    if (a.acc_out) gp(a.acc_out)[t] = acc ? 1 : 0;
                 }
-                if (a.la_out) a.la_out[t] = la;
+                if (a.la_out) gp(a.la_out)[t] = la;
             }
             return acc;
         };
 
         if (fast) {
             // the column's operands are loaded once, all in flight together,
-            // and kept in registers for both sweeps
-            for (int c = grp; c < tc; c += ngrp) {
+            // and kept in registers for both sweeps; the first round (every
+            // column of a tile at D > 64, where ngrp >= TC) is straight-line
+            // code, its loads issued before the inline-decision gate's barrier
+            auto column = [&](int c, const FastOps<T, RM>& f) {
                 const int64_t t = c0 + c;
-                T vgx[RM], vy[RM], vgy[RM], vw[RM], vhw[RM];
-#pragma unroll
-                for (int i = 0; i < RM; ++i) {
-                    const int64_t d = lane + (int64_t)i * G;
-                    if (d < D) {
-                        const size_t gi = (size_t)t * D + d;
-                        vgx[i] = a.gx[gi];
-                        vy[i] = a.Y[gi];
-                        vgy[i] = a.gy[gi];
-                        vw[i] = a.W[gi];
-                        vhw[i] = a.hw[gi];
-                    }
-                }
-                const double s_lpx = lp_of(a.lpx, t), s_lpy = lp_of(a.lpy, t), u = (double)a.U[t];
-                const int8_t o = a.dec ? a.dec[t] : (int8_t)-1;
-                const int8_t pv = a.prev_dec ? a.prev_dec[t] : (int8_t)-1;
                 const T* xcol = xs + (size_t)c * D;
                 T* bcol = bl + (size_t)c * D;
                 T* acol = al + (size_t)c * D;
@@ -805,9 +831,9 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                     if (d < D) {
                         const double x = (double)xcol[d];
                         const size_t gi = (size_t)t * D + d;
-                        const double gx = (double)vgx[i], y = (double)vy[i], gy = (double)vgy[i];
+                        const double gx = (double)f.gx[i], y = (double)f.y[i], gy = (double)f.gy[i];
                         const double z = (double)rademacher(keys[0], gi + (size_t)a.zoff);
-                        const double w = (double)vw[i], hy = (double)vhw[i];
+                        const double w = (double)f.w[i], hy = (double)f.hw[i];
                         const double r = x - y - eps * gy;
                         const double dr = z - w - eps * hy;
                         d1[0] += gy * w;
@@ -824,7 +850,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                 d2[0] = group_sum(d2[0], G);
                 d3[0] = group_sum(d3[0], G);
                 double dg[PMCMC_MAXP], gg;
-                const bool acc = decide(c, t, s_lpx, s_lpy, s_qyx, s_qxy, d1, d2, d3, u, o, pv, dg, gg);
+                const bool acc = decide(c, t, f.lpx, f.lpy, s_qyx, s_qxy, d1, d2, d3, f.u, f.o, f.pv, dg, gg);
                 // A and B in the iterate's precision (they are stored in it)
                 const T dgt = (T)dg[0];
 #pragma unroll
@@ -832,13 +858,19 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                     const int64_t d = lane + (int64_t)i * G;
                     if (d < D) {
                         const T x = xcol[d];
-                        const T y = vy[i];
+                        const T y = f.y[i];
                         const T z = (T)rademacher(keys[0], (size_t)t * D + d + (size_t)a.zoff);
-                        const T jt = (acc ? vw[i] : z) + (y - x) * dgt;
+                        const T jt = (acc ? f.w[i] : z) + (y - x) * dgt;
                         acol[d] = z * jt;
                         bcol[d] = (acc ? y : x) - corr_base(xcol[D + d]);  // FT_t - S_t
                     }
                 }
+            };
+            if (grp < tc) column(grp, ops0);
+            for (int c = grp + ngrp; c < tc; c += ngrp) {
+                FastOps<T, RM> f;
+                load_ops(c, f);
+                column(c, f);
             }
         } else {
             for (int c = grp; c < tc; c += ngrp) {
@@ -998,8 +1030,8 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         }
         if (PASS == PASS_REDUCE && keep) {
             for (int64_t i = tid; i < nel; i += PMCMC_NT) {
-                a.a_keep[gbase + i] = al[i];
-                a.b_keep[gbase + i] = bl[i];
+                gp(a.a_keep)[gbase + i] = al[i];
+                gp(a.b_keep)[gbase + i] = bl[i];
             }
         }
     }
@@ -1048,8 +1080,8 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
     if (PASS == PASS_REDUCE) {
         // tile aggregate of row r: (pa[(P-1)D+r], pb[(P-1)D+r]), column `tile`
         for (int64_t r = tid; r < D; r += PMCMC_NT) {
-            a.agg_a[(size_t)tile * D + r] = pa[(P - 1) * D + r];
-            a.agg_b[(size_t)tile * D + r] = pb[(P - 1) * D + r];
+            gp(a.agg_a)[(size_t)tile * D + r] = pa[(P - 1) * D + r];
+            gp(a.agg_b)[(size_t)tile * D + r] = pb[(P - 1) * D + r];
         }
         return;
     }
@@ -1088,7 +1120,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
     __syncthreads();
 
     // ---- phase 5: store the tile (coalesced) ------------------------------
-    for (int64_t i = tid; i < nel; i += PMCMC_NT) a.S[gbase + i] = al[i];
+    for (int64_t i = tid; i < nel; i += PMCMC_NT) gp(a.S)[gbase + i] = al[i];
 
     if (DEERM) finish(dmax, smax, nz, false);
 }
