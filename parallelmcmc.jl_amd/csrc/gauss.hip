@@ -289,7 +289,7 @@ __global__ __launch_bounds__(256) void gauss_pass_mfma_kernel(ColSrc<float> src,
 // Batched over chains of padded width up to 16 DPT (each chain's own width
 // in its args): work item blockIdx.x is (chain, column block) = items[blockIdx.x].
 template <int DPT, bool JBF>
-__global__ __launch_bounds__(256) void gauss_pass_mfma_batched(const GaussPassArgs* args, const int2* items) {
+__global__ __launch_bounds__(256) void gauss_pass_mfma_batched(const GaussPassArgs* __restrict__ args, const int2* __restrict__ items) {
     const int2 w = items[blockIdx.x];
     const GaussPassArgs& a = args[w.x];
     if ((int64_t)w.y * GCB >= a.src.Tn) return;  // beyond the chain's current window
@@ -435,8 +435,9 @@ __device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const in
                 Vd[col * SV + tid] = 0.f;
             }
     __syncthreads();
-    if (mu && tid < Di) {  // the states, centred (each thread its own row k = tid)
-        for (int col = 0; col < nc; ++col) Vs[col * SV + tid] -= muk;
+    if (mu) {  // the states, centred (each thread its own row k = tid)
+        if (tid < Di)
+            for (int col = 0; col < nc; ++col) Vs[col * SV + tid] -= muk;
         __syncthreads();
     }
 
@@ -512,33 +513,48 @@ __device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const in
     }
     __syncthreads();
 
-    // outputs, one column per wave and round, lanes over the slab's 64 rows
+    // outputs, one column per wave and round, lanes over the slab's 64 rows:
+    // the wave's columns j = wave + 4 q (q < 8: states, q >= 8: their
+    // directions).  The states' x and tape values are loaded first, all in
+    // flight at once (clamped addresses; values of dead lanes unused).
     const double eps = (double)a.eps;
     const double sq2e = sqrt(2.0 * eps);
-    const int64_t d = r0 + lane;
-    const bool drow = d < D;
-#pragma unroll 2
+    const int d = r0 + lane;
+    const bool drow = d < Di;
+    const int dc = drow ? d : 0;
+    const bool p0 = a.pass == 0;
+    float xv[8], xiv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int jc = wave + 4 * q;
+        const int t = (int)t0 + (jc < nc ? jc : 0);
+        const float* xp = src.x0 ? (t == 0 ? src.x0 + dc : src.V + ((size_t)(t - 1) * Di + dc))
+                                 : src.V + ((size_t)t * Di + dc);
+        xv[q] = *gp(xp);
+        xiv[q] = gp(p0 ? a.Xi : src.V)[(size_t)t * Di + dc];
+    }
+    const double mud = (mu && drow) ? (double)gp(mu)[dc] : 0.0;
+#pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int j = wave + 4 * q;  // column of the block: 0-31 states, 32-63 directions
         const int jc = j & (CPB - 1);
-        const int64_t t = t0 + jc;
+        const int t = (int)t0 + jc;
         const bool live = jc < nc;
         const float g = -Pt[j * SP + lane];
-        const size_t gi = (size_t)t * D + d;
-        if (j < CPB) {
+        const size_t gi = (size_t)t * Di + d;
+        if (q < 8) {
             double lpp = 0.0;
             if (live && drow) {
-                const float* xp = src.x0 ? (t == 0 ? src.x0 + d : src.V + gi - D) : src.V + gi;
-                const double x = (double)*gp(xp);
-                const double xm = x - (mu ? (double)gp(mu)[d] : 0.0);
-                lpp = 0.5 * xm * (double)g;
+                const double x = (double)xv[q];
+                lpp = 0.5 * (x - mud) * (double)g;
                 gp(a.gout)[gi] = g;
-                if (a.pass == 0) gp(a.Yout)[gi] = (float)(x + eps * g + sq2e * (double)gp(a.Xi)[gi]);
+                if (p0) gp(a.Yout)[gi] = (float)(x + eps * g + sq2e * (double)xiv[q]);
             }
+#pragma unroll
             for (int m = 32; m >= 1; m >>= 1) lpp += __shfl_xor(lpp, m, 64);
             if (lane == 0) lpw[jc] = lpp;
         } else if (live && drow) {
-            if (a.pass == 0)
+            if (p0)
                 gp(a.Wout)[gi] = (float)((double)Vd[jc * SV + d] + eps * g);
             else
                 gp(a.hout)[gi] = g;
@@ -554,7 +570,7 @@ __device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const in
 
 // work item blockIdx.x is (chain, column block, slab) = items[blockIdx.x]
 template <bool JBF>
-__global__ __launch_bounds__(256) void gauss_slab_batched(const GaussPassArgs* args, const int4* items) {
+__global__ __launch_bounds__(256) void gauss_slab_batched(const GaussPassArgs* __restrict__ args, const int4* __restrict__ items) {
     const int4 w = items[blockIdx.x];
     const GaussPassArgs& a = args[w.x];
     if (a.skip && *(volatile const int*)a.skip) return;

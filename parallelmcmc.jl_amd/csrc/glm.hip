@@ -460,7 +460,7 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, c
 // Batched over chains of one padded width: work item blockIdx.x is
 // (chain, column block, N split) = items[blockIdx.x].
 template <int DPT, int K, bool JBF>
-__global__ __launch_bounds__(256) void glm_pass_mfma_batched(const GlmPassArgs* args, const int4* items) {
+__global__ __launch_bounds__(256) void glm_pass_mfma_batched(const GlmPassArgs* __restrict__ args, const int4* __restrict__ items) {
     const int4 w = items[blockIdx.x];
     const GlmPassArgs& a = args[w.x];
     if ((int64_t)w.y * CB >= a.src.Tn) return;  // beyond the chain's current window
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(256) void glm_reduce_kernel(ColSrc<T> src, const P*
 }
 
 // Batched: work item blockIdx.x is (chain, block of 4 columns) = items[blockIdx.x].
-__global__ __launch_bounds__(256) void glm_reduce_batched(const GlmReduceArgs* args, const int2* items) {
+__global__ __launch_bounds__(256) void glm_reduce_batched(const GlmReduceArgs* __restrict__ args, const int2* __restrict__ items) {
     const int2 w = items[blockIdx.x];
     const GlmReduceArgs& a = args[w.x];
     glm_reduce_body<float, float>(a.src, a.part, a.llpart, a.ns, a.Dp, a.llstride, a.pass, a.inv_s2, a.eps, a.Xi,

@@ -1907,12 +1907,21 @@ __global__ __launch_bounds__(256) void batch_slide_init_kernel(const SlideInit* 
 // Reads the chain's SlideState without changing it: every workgroup of the
 // chain sees the same state, and the apply pass's last workgroup, which owns
 // the updates, runs after this kernel.
-__global__ __launch_bounds__(256) void batch_window_kernel(const WinDesc* wds, const int* ids, int par) {
+__global__ __launch_bounds__(256) void batch_window_kernel(const WinDesc* __restrict__ wds, const int* __restrict__ ids, int par) {
     const WinDesc& w = wds[ids[blockIdx.y]];
-    if (*(volatile const int*)&w.sy->converged) return;
-    const SlideState st = *w.win;
+    // the flag, the slide state and the span come from earlier launches:
+    // plain loads, all in flight together
+    const int conv = *gp(&w.sy->converged);
+    struct {
+        long long front, hw, pend_lo, pend_hi;
+    } st;
+    st.front = *gp(&w.win->front);
+    st.hw = *gp(&w.win->hw);
+    st.pend_lo = *gp(&w.win->pend_lo);
+    st.pend_hi = *gp(&w.win->pend_hi);
+    int64_t span = *gp(w.span);
+    if (conv) return;
     const int64_t D = w.D;
-    int64_t span = *w.span;
     if (span < 1) span = 1;
     if (span > w.L) span = w.L;
     const int64_t front = st.front;
@@ -1930,8 +1939,9 @@ __global__ __launch_bounds__(256) void batch_window_kernel(const WinDesc* wds, c
     // just before them
     if (hi > st.hw) {
         const float* src = st.hw == 0 ? w.x0 : sin + (st.hw - 1) * D;
-        const int64_t lo = st.hw * D, n = (hi - st.hw) * D;
-        for (int64_t i = gid; i < n; i += stride) sin[lo + i] = src[i % D];
+        const int64_t lo = st.hw * D;
+        const int n = (int)((hi - st.hw) * D), Di = (int)D;  // < 2^31: one window
+        for (int i = (int)gid; i < n; i += (int)stride) sin[lo + i] = src[i % Di];
     }
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     const int64_t n = hi - front;
@@ -1941,6 +1951,7 @@ __global__ __launch_bounds__(256) void batch_window_kernel(const WinDesc* wds, c
     a.s0 = s0;
     a.T_len = n;
     a.numTiles = (int)((n + w.TC - 1) / w.TC);
+    a.agg_inline = a.numTiles <= PMCMC_AGG_INLINE && D <= PMCMC_NT;
     a.S_in = sin + off;
     a.S = sout + off;
     a.Xi = w.Xi + off;
@@ -1987,9 +1998,10 @@ __global__ __launch_bounds__(256) void batch_copy_kernel(const BCopy* t) {
 // Scan of the level-0 tile aggregates of each chain (one workgroup per chain,
 // rows over threads): prev[:, k] = Newton correction after tile k, from zero
 // (the window's start state is fixed).
-__global__ __launch_bounds__(256) void batch_agg_scan_kernel(const TileArgs<float>* args, const int* chains) {
+__global__ __launch_bounds__(256) void batch_agg_scan_kernel(const TileArgs<float>* __restrict__ args, const int* __restrict__ chains) {
     const TileArgs<float>& a = args[chains[blockIdx.x]];
-    if (a.skip && *(volatile const int*)a.skip) return;
+    if (a.agg_inline) return;  // the apply pass folds this chain's few aggregates itself
+    if (a.skip && *gp(a.skip)) return;
     const int64_t D = a.D;
     const int nT = a.numTiles;
     float* prev = const_cast<float*>(a.prev);
@@ -2548,7 +2560,9 @@ static void build_items(const pmcmc_batch* b, const std::vector<char>& live, Ite
         const size_t lds = tile_lds_bytes<float>(h.D, TC, true);
         const int lc = lds_class(lds);
         for (int t = 0; t < nT; ++t) it->red[lc].push_back(make_int2(k, t));
-        it->multi.push_back(k);
+        // chains whose windows never exceed PMCMC_AGG_INLINE tiles fold their
+        // aggregates in the apply pass (agg_inline, set by the window kernel)
+        if (nT > PMCMC_AGG_INLINE || h.D > PMCMC_NT) it->multi.push_back(k);
         it->wins.push_back(k);
         if (lds > it->lds_multi[lc]) it->lds_multi[lc] = lds;
         it->r_all.push_back(k);
@@ -2769,8 +2783,9 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
                 hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_REDUCE>),
                                    dim3((unsigned)it.red[lc].size()), dim3(PMCMC_NT), it.lds_multi[lc], s, ta,
                                    (const int2*)ia(o_red[lc]));
-        hipLaunchKernelGGL(batch_agg_scan_kernel, dim3((unsigned)it.multi.size()), dim3(256), 0, s, ta,
-                           (const int*)ia(o_multi));
+        if (!it.multi.empty())
+            hipLaunchKernelGGL(batch_agg_scan_kernel, dim3((unsigned)it.multi.size()), dim3(256), 0, s, ta,
+                               (const int*)ia(o_multi));
         for (int lc = 0; lc < NLDS; ++lc)
             if (!it.red[lc].empty())
                 hipLaunchKernelGGL((tile_kernel_batched<float, MODE_GEMM, PASS_APPLY>),

@@ -63,7 +63,6 @@ enum { JAC_STOCH = 0, JAC_DIAG = 1 };
 #ifndef PMCMC_BATCHED_WAVES
 #define PMCMC_BATCHED_WAVES 2
 #endif
-
 // Base of the Newton correction of one entry: the iterate, or 0 where the
 // iterate is not finite, so that a non-finite region of S_in shrinks by (at
 // least) one column per iteration, as it does in the state form.
@@ -72,11 +71,17 @@ __device__ inline T corr_base(T v) {
     return isfinite(v) ? v : (T)0;
 }
 
+// most log-density partials per column (64-row slabs of a D <= 256 target)
+#define PMCMC_LP_MAXSLAB 4
+// most tiles whose incoming states an apply pass folds in place (agg_inline)
+#define PMCMC_AGG_INLINE 16
+
 // Operands of one column in the fast GEMM build: RM rows per lane.
 template <typename T, int RM>
 struct FastOps {
     T gx[RM], y[RM], gy[RM], w[RM], hw[RM];
-    double lpx, lpy, u;
+    double lpx, lpy;
+    float u;
     int8_t o, pv;
 };
 
@@ -138,7 +143,7 @@ struct TileArgs {
     const double* lpx;
     const double* lpy;
     // log-densities in lp_nslab partials (lp[s * lp_stride + t], summed in
-    // slab order); 0 or 1: a single value per column
+    // slab order; at most PMCMC_LP_MAXSLAB); 0 or 1: a single value per column
     int lp_nslab;
     int64_t lp_stride;
     // MODE_GEMM: FP64 decision override per column (-1 none; else bit 0 the
@@ -190,6 +195,11 @@ struct TileArgs {
     int* rflag;
     double band, gate;
     int gate_iters;
+    // PASS_APPLY with at most PMCMC_AGG_INLINE tiles and D <= PMCMC_NT
+    // (nonzero): a tile's incoming state is folded from the aggregates of the
+    // tiles before it (agg_a, agg_b) in the tile itself, in the order the
+    // aggregate scan would fold them, instead of being read from `prev`.
+    int agg_inline;
 };
 
 // FP64 log alpha of one column for a dense Gaussian target, by the whole
@@ -525,8 +535,11 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
     const int tid = threadIdx.x;
     // Newton loop already converged: later queued iterations are no-ops.  The
     // flag is loaded here and tested once this tile's staging loads are in
-    // flight too (one memory round trip for both).
-    const int skip_now = a.skip ? *(volatile const int*)a.skip : 0;
+    // flight too (one memory round trip for both).  It and the final-prefix
+    // count below are written by earlier launches only (within a launch, the
+    // last workgroup to arrive writes them, after every other one has read
+    // them), so plain loads see their current values.
+    const int skip_now = a.skip ? *gp(a.skip) : 0;
     const int64_t D = a.D;
     const int TC = a.TC;
     const int64_t c0 = (int64_t)tile * TC;
@@ -624,7 +637,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
             }
         }
     };
-    const int final_now = (DEERM && a.final_cols) ? *(volatile const int*)a.final_cols : 0;
+    const int final_now = (DEERM && a.final_cols) ? *gp(a.final_cols) : 0;
 
     T* xs = reinterpret_cast<T*>(smem);
     T* bl = xs + (DEERM ? (size_t)(TC + 1) * D : (size_t)0);
@@ -638,6 +651,25 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
 
     const int64_t nel = (int64_t)tc * D;
     const int64_t gbase = c0 * D;
+
+    // incoming state of an output pass's later tile: its first PMCMC_NT
+    // entries are loaded ahead of the staging loads
+    constexpr bool SIN = OUT && PASS != PASS_CHAINED && PASS != PASS_SINGLE;
+    T sinv = (T)0;
+    T fa[PMCMC_AGG_INLINE], fb[PMCMC_AGG_INLINE];
+    const int rr = tid < D ? tid : 0;
+    if (SIN && tile > 0) {
+        if (PASS == PASS_APPLY && a.agg_inline) {
+#pragma unroll
+            for (int q = 0; q < PMCMC_AGG_INLINE; ++q) {
+                const size_t k = q < tile ? (size_t)q : 0;
+                fa[q] = gp(a.agg_a)[k * D + rr];
+                fb[q] = gp(a.agg_b)[k * D + rr];
+            }
+        } else {
+            sinv = gp(a.prev)[(size_t)(tile - 1) * D + rr];
+        }
+    }
 
     // ---- phase 0: stage inputs (coalesced, every region in one round) -------
     // A and B kept by the reduce pass of this iteration (MODE_GEMM multi-tile)
@@ -653,8 +685,10 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         T* xd = c0 == 0 ? xs + D : xs;
         const T* xsrc = c0 == 0 ? a.S_in : a.S_in + gbase - D;
         const int64_t xn = c0 == 0 ? nel : nel + D;
-        if (c0 == 0)
-            for (int64_t i = tid; i < D; i += PMCMC_NT) xs[i] = gp(a.s0)[i];
+        // s0 (first tile): its first PMCMC_NT entries are loaded ahead of
+        // the staging loads and written after them
+        T s0v = (T)0;
+        if (c0 == 0) s0v = gp(a.s0)[tid < D ? tid : 0];
         if (build) {
             T* const dst[2] = {xd, bl};
             const T* const src[2] = {xsrc, a.Xi + gbase};
@@ -665,6 +699,10 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
             const T* const src[3] = {xsrc, a.a_keep + gbase, a.b_keep + gbase};
             const int64_t n[3] = {xn, nel, nel};
             stage_lds_multi<T, 3>(dst, src, n, tid);
+        }
+        if (c0 == 0) {
+            if (tid < D) xs[tid] = s0v;
+            for (int64_t i = tid + PMCMC_NT; i < D; i += PMCMC_NT) xs[i] = gp(a.s0)[i];
         }
     }
     if (skip_now) return;
@@ -682,11 +720,39 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         }
     }
 
+    if (SIN && PASS == PASS_APPLY && tile > 0 && a.agg_inline) {
+        T f = (T)0;
+#pragma unroll
+        for (int q = 0; q < PMCMC_AGG_INLINE; ++q)
+            if (q < tile) f = fa[q] * f + fb[q];
+        sinv = f;
+    }
     // incoming state of the tile (needed only by the output passes)
     if (OUT && PASS != PASS_CHAINED) {
         for (int64_t r = tid; r < D; r += PMCMC_NT)
             sin_[r] = (PASS == PASS_SINGLE || tile == 0) ? ((DEERM || !a.s0) ? (T)0 : gp(a.s0)[r])
+                      : (SIN && r == tid)                ? sinv
                                                          : gp(a.prev)[(size_t)(tile - 1) * D + r];
+    }
+    // per-column outputs: by the pass that writes the tile, or by the reduce
+    // pass when it keeps A and B
+    const bool side = keep ? PASS == PASS_REDUCE : OUT;
+    // inline FP64 decisions (GEMM targets with a refinement target): the gate
+    // is read once, fixed for this launch, and the list of near-threshold
+    // columns emptied; the barrier below publishes both
+    const bool inl = MODE == MODE_GEMM && build && a.rtg != nullptr && side;
+    bool inl_on = false;
+    __shared__ int fl_n;
+    if (inl) {
+        // the three loads are independent: none waits for another
+        const int rf = *gp(a.rflag);
+        const double lm = *gp(&a.sync->last_metric);
+        const int li = *gp(&a.sync->last_iter);
+        inl_on = (rf != 0) | (lm <= a.gate) | (li + 1 >= a.gate_iters);
+        if (tid == 0) {
+            fl_n = 0;
+            if (inl_on && tile == 0) *a.rflag = 1;
+        }
     }
     __syncthreads();
 
@@ -715,22 +781,34 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
 #pragma unroll
         for (int k = 0; k < PMCMC_MAXP; ++k) keys[k] = probe_key(a.seed, a.probe_iter, (uint32_t)k);
         const size_t DT = (size_t)D * (size_t)a.T_len;
-        // per-column outputs: by the pass that writes the tile, or by the
-        // reduce pass when it keeps A and B
-        const bool side = keep ? PASS == PASS_REDUCE : OUT;
         // inline FP64 decisions: the gate (fixed during this launch), and the
         // columns within the band (their FP32-built A and B are fixed up below)
         constexpr int FLMAX = 64;
-        __shared__ int fl_n;
         __shared__ int fl_c[FLMAX];
         __shared__ int8_t fl_pv[FLMAX];
         __shared__ double fl_dg[FLMAX];
         __shared__ double fl_v[256];
         // a column's log-density: the slab partials summed in slab order
-        auto lp_of = [&](const double* lp, int64_t t) -> double {
-            double v = gp(lp)[t];
-            for (int sl = 1; sl < a.lp_nslab; ++sl) v += gp(lp)[(size_t)sl * a.lp_stride + t];
+        // (lp_nslab <= PMCMC_LP_MAXSLAB: the partials are loaded together,
+        // clamped to slab 0; a loop with a runtime trip count would wait for
+        // each load in turn)
+        auto lp_load = [&](const double* lp, int64_t t, double* part) {
+            const PMCMC_GLOBAL double* p = gp(lp) + t;
+            const int ns = a.lp_nslab;
+            const size_t st = (size_t)a.lp_stride;
+#pragma unroll
+            for (int sl = 0; sl < PMCMC_LP_MAXSLAB; ++sl) part[sl] = p[(sl < ns ? (size_t)sl : 0) * st];
+        };
+        auto lp_sum = [&](const double* part) -> double {
+            double v = part[0];
+#pragma unroll
+            for (int sl = 1; sl < PMCMC_LP_MAXSLAB; ++sl) v = sl < a.lp_nslab ? v + part[sl] : v;
             return v;
+        };
+        auto lp_of = [&](const double* lp, int64_t t) -> double {
+            double part[PMCMC_LP_MAXSLAB];
+            lp_load(lp, t, part);
+            return lp_sum(part);
         };
         auto load_ops = [&](int c, FastOps<T, RM>& f) {
             const int64_t t = c0 + (c < tc ? c : 0);
@@ -746,26 +824,19 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                 f.w[i] = gp(a.W)[gi];
                 f.hw[i] = gp(a.hw)[gi];
             }
-            f.lpx = lp_of(a.lpx, t);
-            f.lpy = lp_of(a.lpy, t);
-            f.u = (double)gp(a.U)[t];
-            f.o = a.dec ? gp(a.dec)[t] : (int8_t)-1;
-            f.pv = a.prev_dec ? gp(a.prev_dec)[t] : (int8_t)-1;
+            double px[PMCMC_LP_MAXSLAB], py[PMCMC_LP_MAXSLAB];
+            lp_load(a.lpx, t, px);
+            lp_load(a.lpy, t, py);
+            f.u = gp(a.U)[t];
+            // byte loads without a branch: a missing array reads the tape's U
+            // (4 T bytes) instead and the value is replaced
+            const int8_t o = gp(a.dec ? a.dec : (const int8_t*)a.U)[t];
+            const int8_t pv = gp(a.prev_dec ? (const int8_t*)a.prev_dec : (const int8_t*)a.U)[t];
+            f.o = a.dec ? o : (int8_t)-1;
+            f.pv = a.prev_dec ? pv : (int8_t)-1;
+            f.lpx = lp_sum(px);
+            f.lpy = lp_sum(py);
         };
-        // the first round's operands, in flight across the gate's barrier
-        FastOps<T, RM> ops0;
-        if (fast) load_ops(grp, ops0);
-        const bool inl = MODE == MODE_GEMM && a.rtg != nullptr && side;
-        bool inl_on = false;
-        if (inl) {
-            inl_on = *gp(a.rflag) != 0 || *gp(&a.sync->last_metric) <= a.gate ||
-                     *gp(&a.sync->last_iter) + 1 >= a.gate_iters;
-            if (tid == 0) {
-                fl_n = 0;
-                if (inl_on && tile == 0) *a.rflag = 1;
-            }
-            __syncthreads();
-        }
 
         // Accept decision of column c from its (group-summed) sums; returns it
         // and sets gg = g (1 - g) of the acceptance probability g and dg[k],
@@ -815,9 +886,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
 
         if (fast) {
             // the column's operands are loaded once, all in flight together,
-            // and kept in registers for both sweeps; the first round (every
-            // column of a tile at D > 64, where ngrp >= TC) is straight-line
-            // code, its loads issued before the inline-decision gate's barrier
+            // and kept in registers for both sweeps
             auto column = [&](int c, const FastOps<T, RM>& f) {
                 const int64_t t = c0 + c;
                 const T* xcol = xs + (size_t)c * D;
@@ -850,7 +919,8 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                 d2[0] = group_sum(d2[0], G);
                 d3[0] = group_sum(d3[0], G);
                 double dg[PMCMC_MAXP], gg;
-                const bool acc = decide(c, t, f.lpx, f.lpy, s_qyx, s_qxy, d1, d2, d3, f.u, f.o, f.pv, dg, gg);
+                const bool acc = decide(c, t, f.lpx, f.lpy, s_qyx, s_qxy, d1, d2, d3, (double)f.u, f.o, f.pv,
+                                        dg, gg);
                 // A and B in the iterate's precision (they are stored in it)
                 const T dgt = (T)dg[0];
 #pragma unroll
@@ -866,8 +936,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                     }
                 }
             };
-            if (grp < tc) column(grp, ops0);
-            for (int c = grp + ngrp; c < tc; c += ngrp) {
+            for (int c = grp; c < tc; c += ngrp) {
                 FastOps<T, RM> f;
                 load_ops(c, f);
                 column(c, f);
@@ -1133,7 +1202,7 @@ __global__ __launch_bounds__(PMCMC_NT, sizeof(T) == 4 ? 3 : 2) void tile_kernel(
 // Batched launch over many problems: work item blockIdx.x is (problem, tile) =
 // items[blockIdx.x]; args[problem] describes the problem.
 template <typename T, int MODE, int PASS>
-__global__ __launch_bounds__(PMCMC_NT, PMCMC_BATCHED_WAVES) void tile_kernel_batched(const TileArgs<T>* args, const int2* items) {
+__global__ __launch_bounds__(PMCMC_NT, PMCMC_BATCHED_WAVES) void tile_kernel_batched(const TileArgs<T>* __restrict__ args, const int2* __restrict__ items) {
     const int2 w = items[blockIdx.x];
     if (w.y >= args[w.x].numTiles) return;  // beyond the chain's current window
     tile_body<T, MODE, PASS>(args[w.x], w.y);
