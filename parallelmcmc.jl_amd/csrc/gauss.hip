@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256) void gauss_pass_mfma_batched(const GaussPassAr
 // src) into LDS columns dst[col * SV + k] (col < ncol, k < D), minus mu[k]
 // when given: 16-byte loads, every one in flight before the first LDS write.
 template <int SV>
-__device__ inline void span_to_lds(float* dst, const float* src, int D, int ncol, int tid) {
+__device__ __forceinline__ void span_to_lds(float* dst, const float* src, int D, int ncol, int tid) {
     const int n = D * ncol;  // at most 32 x 256
     if (n <= 0) return;
     const uintptr_t addr = (uintptr_t)src;

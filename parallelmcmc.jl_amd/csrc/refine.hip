@@ -182,7 +182,7 @@ __device__ void refine_flag_body(const RefineArgs& a, int8_t* dec, int* cnt, int
 // both products, and reduces its X^T r accumulators across lanes once at the
 // end.  Q >= ceil(D / 4).
 template <int Q>
-__device__ void refine_pass_glm_body(const RefineTarget& tg, const RefineArgs& a, const RefineWork& w, int par,
+__device__ __forceinline__ void refine_pass_glm_body(const RefineTarget& tg, const RefineArgs& a, const RefineWork& w, int par,
                                      int phase, int64_t split, int slot, int nslots) {
     if (skipped(a) || split >= w.nsplit) return;
     __shared__ double v[RMAXD], zpart[RNT / 64][64], red[4];
