@@ -1361,13 +1361,18 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
         k += n;
         HIPCHK(c, hipMemcpyAsync(c->host_sync, w->sync, sizeof(SyncState), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        // a bounded device-side wait that gave up leaves stale end states in
+        // the iterate: stop at once rather than iterate on them
+        if (c->host_sync->err)
+            return fail(c, PMCMC_ERR_DEVICE, "DEER solve: a device-side wait hit its spin limit (iteration %d)",
+                        c->host_sync->last_iter);
         done = c->host_sync->converged != 0;
         if (chunk < 16) chunk <<= 1;
         // full Jacobian: tile only the columns that are not final yet
         if (full && !done) full_retile(&w->fullw, w->sync, c->host_sync->skip_cols, c->stream);
     }
     const SyncState hs = *c->host_sync;
-    if (hs.err) return fail(c, PMCMC_ERR_HIP, "DEER solve: a device-side wait hit its spin limit");
+    if (hs.err) return fail(c, PMCMC_ERR_DEVICE, "DEER solve: a device-side wait hit its spin limit");
     const int iters = hs.converged ? hs.conv_iter + 1 : (hs.last_iter + 1);
     T* result = bufs[iters & 1];
     if ((void*)result != S_out)
