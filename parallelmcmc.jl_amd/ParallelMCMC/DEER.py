@@ -107,7 +107,14 @@ def solve(rec: TapedRecursion, s0, init=None, tol_abs=1e-4, tol_rel=1e-3, maxite
     for the whole solve instead of drawing new ones every iteration (the
     reference's choice, src/DEER/DEER.jl:407): the Newton iteration is then a
     stationary linear iteration, which on ill-conditioned targets does not
-    show the transient growth that can overflow FP32 with redrawn probes."""
+    show the transient growth that can overflow FP32 with redrawn probes.
+
+    ``jacobian``: ``:stoch_diag`` (Hutchinson, ``probes`` Rademacher
+    probes), ``:diag`` (the exact diagonal: closed form for diagonal-Hessian
+    targets; for the logistic and dense Gaussian targets, D <= 256, two
+    Hessian-vector GEMM passes and diag(H) per iteration instead of the
+    reference's D basis JVPs, src/DEER/DEER.jl:327-342), ``:full`` /
+    ``:full_sigmoid`` (dense Gaussian, D <= 64)."""
     if maxiter < 1:
         raise ArgumentError("maxiter must be >= 1")
     if tol_abs < 0:

@@ -778,6 +778,27 @@ template int gauss_dir_products<double>(GlmWork*, const GaussData*, hipStream_t,
                                         const double*, const double*, uint32_t, const double*, int64_t, int64_t,
                                         const int*, Prof*, int, std::string*);
 
+// As glm_hvp_products: hout = H(V) Z = -Lambda Z; gradient and log-density
+// of the states to gscr / lpscr.
+template <typename T>
+int gauss_hvp_products(const GaussData* g, hipStream_t s, const T* V, const T* x0, const T* Z, T* gscr,
+                       double* lpscr, T* hout, int64_t D, int64_t Tn, const int* skip, Prof* prof, int iter, bool jbf,
+                       std::string* msg) {
+    ColSrc<T> a{};
+    a.V = V;
+    a.x0 = x0;
+    a.Z = Z;
+    a.K = 1;
+    a.D = D;
+    a.Tn = Tn;
+    return gauss_pass<T>(g, s, a, 1, 0.0, nullptr, gscr, hout, lpscr, nullptr, nullptr, skip, prof, iter, jbf, msg);
+}
+template int gauss_hvp_products<float>(const GaussData*, hipStream_t, const float*, const float*, const float*, float*,
+                                       double*, float*, int64_t, int64_t, const int*, Prof*, int, bool, std::string*);
+template int gauss_hvp_products<double>(const GaussData*, hipStream_t, const double*, const double*, const double*,
+                                        double*, double*, double*, int64_t, int64_t, const int*, Prof*, int, bool,
+                                        std::string*);
+
 template <typename T>
 static int gauss_fwd_jvp_t(const GaussData* g, hipStream_t s, double eps, const T* X, const T* Xi, const T* U,
                            const T* Z, T* FT, T* Jt, uint8_t* acc, double* la, int64_t D, int64_t Tn, bool jbf,

@@ -72,6 +72,12 @@ int gauss_dir_products(GlmWork* w, const GaussData* g, hipStream_t s, double eps
                        const T* Z, uint32_t key, const T* Xi, int64_t D, int64_t Tn, const int* skip, Prof* prof,
                        int iter, std::string* msg);
 
+// As glm_hvp_products: hout = H(V) Z = -Lambda Z.
+template <typename T>
+int gauss_hvp_products(const GaussData* g, hipStream_t s, const T* V, const T* x0, const T* Z, T* gscr,
+                       double* lpscr, T* hout, int64_t D, int64_t Tn, const int* skip, Prof* prof, int iter, bool jbf,
+                       std::string* msg);
+
 int gauss_fwd_and_jvp(const GaussData* g, hipStream_t s, int prec, double eps, const void* X,
                       const void* Xi, const void* U, const void* Z, void* FT, void* Jt,
                       uint8_t* accepted, double* logalpha, int64_t D, int64_t T, std::string* msg);

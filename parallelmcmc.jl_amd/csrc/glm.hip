@@ -758,6 +758,31 @@ template int glm_dir_products<double>(GlmWork*, const GlmData*, hipStream_t, dou
                                       const double*, const double*, uint32_t, const double*, int64_t, int64_t,
                                       const int*, Prof*, int, std::string*);
 
+// One pass at the states V (shifted by x0 when given) with one direction set
+// Z: hout = H(V) Z, the Hessian-vector products of the log-density; the
+// states' gradient and log-density go to gscr / lpscr (not needed by the
+// caller, written by the pass).
+template <typename T>
+int glm_hvp_products(GlmWork* w, const GlmData* g, hipStream_t s, double prior_scale, const T* V, const T* x0,
+                     const T* Z, T* gscr, double* lpscr, T* hout, int64_t D, int64_t Tn, const int* skip, Prof* prof,
+                     int iter, bool jbf, std::string* msg) {
+    ColSrc<T> a{};
+    a.V = V;
+    a.x0 = x0;
+    a.Z = Z;
+    a.K = 1;
+    a.D = D;
+    a.Tn = Tn;
+    return glm_pass<T>(g, w, s, a, 1, 0.0, prior_scale, nullptr, gscr, hout, lpscr, nullptr, nullptr, skip, prof,
+                       iter, jbf, msg);
+}
+template int glm_hvp_products<float>(GlmWork*, const GlmData*, hipStream_t, double, const float*, const float*,
+                                     const float*, float*, double*, float*, int64_t, int64_t, const int*, Prof*, int,
+                                     bool, std::string*);
+template int glm_hvp_products<double>(GlmWork*, const GlmData*, hipStream_t, double, const double*, const double*,
+                                      const double*, double*, double*, double*, int64_t, int64_t, const int*, Prof*,
+                                      int, bool, std::string*);
+
 // ---------------------------------------------------------------------------
 // standalone batched MALA step + JVP, and log-densities
 // ---------------------------------------------------------------------------

@@ -137,6 +137,14 @@ int glm_dir_products(GlmWork* w, const GlmData* g, hipStream_t s, double eps, do
                      const T* x0, const T* Z, uint32_t key, const T* Xi, int64_t D, int64_t Tn, const int* skip,
                      Prof* prof, int iter, std::string* msg);
 
+// One pass at the states V (shifted by x0 when given) along the directions Z:
+// hout = H(V) Z (Hessian of the log-density); the states' gradient and
+// log-density are written to gscr / lpscr.
+template <typename T>
+int glm_hvp_products(GlmWork* w, const GlmData* g, hipStream_t s, double prior_scale, const T* V, const T* x0,
+                     const T* Z, T* gscr, double* lpscr, T* hout, int64_t D, int64_t Tn, const int* skip, Prof* prof,
+                     int iter, bool jbf, std::string* msg);
+
 int glm_fwd_and_jvp(const GlmData* g, hipStream_t s, int prec, double eps, double prior_scale,
                     const void* X, const void* Xi, const void* U, const void* Z, void* FT, void* Jt,
                     uint8_t* accepted, double* logalpha, int64_t D, int64_t T, std::string* msg);

@@ -24,6 +24,7 @@
 #include "../../include/pmcmc_amd.h"
 #include "adjoint.hpp"
 #include "common.hpp"
+#include "diagjac.hpp"
 #include "fullscan.hpp"
 #include "linalg.hpp"
 #include "refine.hpp"
@@ -135,6 +136,7 @@ struct pmcmc_deer_ws {
     GlmWork glmw;
     FullWork fullw;  // jacobian = :full
     RefineWork refw;  // FP64 decision overrides (GEMM targets, FP32)
+    DiagWork diagw;   // jacobian = :diag on a GEMM target
     int8_t* prev_dec = nullptr;  // previous iteration's decisions (FP32 hysteresis)
     double* tile_smax = nullptr;  // max |S| per level-0 tile when it last ran (final prefix)
     int tile_smax_n = 0;
@@ -1130,6 +1132,7 @@ extern "C" int pmcmc_deer_ws_destroy(pmcmc_deer_ws* w) {
     glm_work_destroy(&w->glmw);
     full_work_destroy(&w->fullw);
     refine_work_destroy(&w->refw);
+    diag_work_destroy(&w->diagw);
     hipFree(w->prev_dec);
     hipFree(w->tile_smax);
     hipFree(w->adj_lam);
@@ -1197,15 +1200,19 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
     const bool full = p->jacobian == PMCMC_JAC_FULL || p->jacobian == PMCMC_JAC_FULL_SIGMOID;
     if (full && tgt->kind != TGT_GAUSS)
         return fail(c, PMCMC_ERR_UNSUPPORTED, "jacobian=:full needs a dense Gaussian target");
+    // :diag on a GEMM target: the products of one probe set, then c and
+    // diag H_x in its W / hw slots (gemm_diag_products)
+    const bool gdiag = mode == MODE_GEMM && p->jacobian == PMCMC_JAC_DIAG;
+    const int gprobes = gdiag ? 1 : p->probes;
+    if (gdiag && D > 256) return fail(c, PMCMC_ERR_UNSUPPORTED, "jacobian=:diag on a GEMM target needs D <= 256");
     if (mode == MODE_GEMM) {
-        if (!full && p->jacobian != PMCMC_JAC_STOCH_DIAG)
-            return fail(c, PMCMC_ERR_UNSUPPORTED, "GEMM targets support jacobian=:stoch_diag and :full");
         std::string msg;
         GlmData gdummy{};
         gdummy.N = 1;
         gdummy.Dp = tgt->gauss.Dp;
         const GlmData* gd = tgt->kind == TGT_GAUSS ? &gdummy : &tgt->glm;
-        int rc = glm_work_ensure(&w->glmw, gd, c->stream, D, Tn, full ? 0 : p->probes, sizeof(T) == 8, &msg);
+        int rc = glm_work_ensure(&w->glmw, gd, c->stream, D, Tn, full ? 0 : gprobes, sizeof(T) == 8, &msg);
+        if (rc == PMCMC_OK && gdiag) rc = diag_work_ensure(&w->diagw, D, Tn, sizeof(T), &msg);
         if (rc == PMCMC_OK && full) rc = full_work_ensure(&w->fullw, D, Tn, sizeof(T) == 8, &msg);
         if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
     }
@@ -1273,7 +1280,7 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
     a.damping = p->damping;
     a.tol_abs = p->tol_abs;
     a.tol_rel = p->tol_rel;
-    a.probes = p->probes;
+    a.probes = gprobes;
     a.jacobian = p->jacobian;
     a.seed = p->seed;
     a.prev_dec = hold ? w->prev_dec : nullptr;
@@ -1310,12 +1317,17 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
                 int rc = tgt->kind == TGT_GAUSS
                              ? gauss_newton_products(&w->glmw, &tgt->gauss, c->stream, w->sync, p->precision,
                                                      p->eps, (const T*)x0, a.S_in, (const T*)Xi, p->seed, it,
-                                                     a.probe_iter, p->probes, D, Tn, &c->prof, &msg,
+                                                     a.probe_iter, gprobes, D, Tn, &c->prof, &msg,
                                                      &w->sync->skip_cols)
                              : glm_newton_products(&w->glmw, &tgt->glm, c->stream, w->sync, p->precision,
                                                    p->eps, tgt->prior_scale, (const T*)x0, a.S_in,
-                                                   (const T*)Xi, p->seed, it, a.probe_iter, p->probes, D, Tn,
+                                                   (const T*)Xi, p->seed, it, a.probe_iter, gprobes, D, Tn,
                                                    &c->prof, &msg);
+                if (rc == PMCMC_OK && gdiag)
+                    rc = gemm_diag_products<T>(&w->diagw, &w->glmw, tgt->kind == TGT_LOGISTIC, &tgt->glm,
+                                               &tgt->gauss, tgt->prior_scale, c->stream, p->eps, (const T*)x0,
+                                               a.S_in, D, Tn, &w->sync->converged, &c->prof, it,
+                                               p->precision == PMCMC_PREC_BF16_JAC, &msg);
                 if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
                 a.Y = (const T*)w->glmw.Y;
                 a.gx = (const T*)w->glmw.gx;

@@ -81,7 +81,7 @@ template <typename T, int RM>
 struct FastOps {
     T gx[RM], y[RM], gy[RM], w[RM], hw[RM];
     double lpx, lpy;
-    float u;
+    T u;  // the tape's uniform, in the iterate's precision
     int8_t o, pv;
 };
 
@@ -929,9 +929,16 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                     if (d < D) {
                         const T x = xcol[d];
                         const T y = f.y[i];
-                        const T z = (T)rademacher(keys[0], (size_t)t * D + d + (size_t)a.zoff);
-                        const T jt = (acc ? f.w[i] : z) + (y - x) * dgt;
-                        acol[d] = z * jt;
+                        if (MODE == MODE_GEMM && a.jacobian == JAC_DIAG) {
+                            // exact diagonal: w holds c (the surrogate's log-alpha
+                            // gradient), hw holds diag H_x
+                            acol[d] = (T)((acc ? 1.0 + eps * (double)f.hw[i] : 1.0) +
+                                          gg * ((double)y - (double)x) * (double)f.w[i]);
+                        } else {
+                            const T z = (T)rademacher(keys[0], (size_t)t * D + d + (size_t)a.zoff);
+                            const T jt = (acc ? f.w[i] : z) + (y - x) * dgt;
+                            acol[d] = z * jt;
+                        }
                         bcol[d] = (acc ? y : x) - corr_base(xcol[D + d]);  // FT_t - S_t
                     }
                 }
@@ -1043,6 +1050,9 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                         const double cc = (1.0 + eps * hx) * gy - gx -
                                           (r - (1.0 + eps * hx) * (1.0 + eps * hy) * r) / (2.0 * eps);
                         Av = (acc ? (1.0 + eps * hx) : 1.0) + gg * (y - x) * cc;
+                    } else if (a.jacobian == JAC_DIAG) {
+                        // GEMM target: c and diag H_x precomputed (W, hw)
+                        Av = (acc ? 1.0 + eps * (double)a.hw[gi] : 1.0) + gg * (y - x) * (double)a.W[gi];
                     } else {
 #pragma unroll
                         for (int k = 0; k < PMCMC_MAXP; ++k) {
