@@ -231,6 +231,18 @@ int pmcmc_deer_solve(pmcmc_ctx* ctx, pmcmc_deer_ws* ws, const pmcmc_target* tgt,
                      const void* U, const void* init, void* S_out, uint8_t* accepted_out,
                      pmcmc_deer_info* info);
 
+/* The trajectory and its log-densities from the final Newton iteration of
+ * the workspace's last pmcmc_deer_solve on a logistic or dense Gaussian
+ * target (replaces the separate evaluation of _trajectory_logps,
+ * src/interface.jl:517-524, on the block's trajectory): S_out (D x T, the
+ * workspace precision) = FT(S_k), the taped step applied to that iteration's
+ * iterate S_k (Y_t where step t accepted, else the state before t; within the
+ * solve's tolerance of the returned iterate and exactly the next Picard
+ * step), logp_out (T doubles, may be NULL) = log p(S_out[:, t]) as that
+ * iteration's passes evaluated it.  PMCMC_ERR_ARGUMENT when the last solve
+ * was on another target.  Synchronises the context stream.                 */
+int pmcmc_deer_last_trajectory(pmcmc_deer_ws* ws, void* S_out, double* logp_out);
+
 /* Trajectory log-densities logp(S[:,t]) (src/interface.jl:517-524),
  * written as double.                                                        */
 int pmcmc_target_logdensity_batch(pmcmc_ctx* ctx, const pmcmc_target* tgt, int prec,

@@ -166,6 +166,23 @@ def solve(rec: TapedRecursion, s0, init=None, tol_abs=1e-4, tol_rel=1e-3, maxite
     return S, Info(bool(info.converged), int(info.iters), float(info.metric), jname, float(damping), int(probes), acc)
 
 
+def last_trajectory(workspace):
+    """The trajectory and its log-densities from the final Newton iteration
+    of the workspace's last solve on a logistic or dense Gaussian target,
+    assembled on the device: S = FT(S_k), the taped step applied to that
+    iteration's iterate (within the solve's tolerance of the returned
+    iterate), and logp(S[:, t]) as that iteration's GEMM passes evaluated it
+    -- no separate log-density pass (src/interface.jl:517-524).  Returns
+    (S, logps): a new D x T device tensor and T float64 device tensor."""
+    ws = workspace
+    dt = _device.torch_dtype(ws.precision)
+    S = _device.colmajor_empty(ws.D, ws.T, dt, ws.device)
+    lp = torch.empty(ws.T, dtype=torch.float64, device=ws.device)
+    rc = _lib.load().pmcmc_deer_last_trajectory(ws.h, _device.ptr(S), _device.ptr(lp))
+    _lib.check(rc, ws.ctx.h, "pmcmc_deer_last_trajectory")
+    return S, lp
+
+
 def adjoint(rec: TapedRecursion, s0, S, accepted, G, tol_abs=0.0, tol_rel=None, maxiter=500, seed=0,
             workspace=None, return_lambda=False):
     """Gradient of a statistic L(S) of the converged trajectory S of ``rec``

@@ -172,6 +172,22 @@ def _trajectory_logps(model, S, precision):
     return MALA.logdensity_batch(model, S, precision=precision).cpu().numpy()
 
 
+def _block_trajectory(model, S, ws, precision):
+    """The block's trajectory and log-densities (src/interface.jl:517-524).
+    GEMM targets (logistic, dense Gaussian): both from the final Newton
+    iteration, assembled on the device (DEER.last_trajectory: FT of the last
+    iterate, within the solve's tolerance of S, with the log-densities its
+    passes already evaluated).  Diagonal targets: S and one elementwise
+    log-density kernel."""
+    from . import DEER
+    from .targets import GaussianTarget, LogisticTarget
+
+    if isinstance(model.target, (LogisticTarget, GaussianTarget)):
+        S_ft, lp = DEER.last_trajectory(ws)
+        return S_ft, lp.cpu().numpy()
+    return S, _trajectory_logps(model, S, precision)
+
+
 def sample(rng, model, sampler, N, initial_params=None, progress=False, return_infos=False):
     """Draw N samples in blocks of sampler.T (src/interface.jl:661-711)."""
     if N < 1:
@@ -186,8 +202,9 @@ def sample(rng, model, sampler, N, initial_params=None, progress=False, return_i
     while nsteps < N:
         S, tape, ws, info = _solve_block(rng, model, sampler, x0, ws)
         nkeep = min(sampler.T, N - nsteps)
+        S, logps = _block_trajectory(model, S, ws, sampler.precision)
         S_keep = S[:, :nkeep].cpu().numpy().astype(np.float64, copy=True)
-        logps = _trajectory_logps(model, S, sampler.precision)[:nkeep]
+        logps = logps[:nkeep]
         blocks.append(S_keep)
         logp_blocks.append(logps)
         infos.append(info)
@@ -211,8 +228,8 @@ def step(rng, model, sampler, state=None, initial_params=None):
             x0 = state.trajectory[:, sampler.T - 1].cpu().numpy().astype(np.float64)
             ws = state.workspace
         S, tape, ws, _ = _solve_block(rng, model, sampler, x0, ws)
+        S, logps = _block_trajectory(model, S, ws, sampler.precision)
         S = S.t().clone().t()
-        logps = _trajectory_logps(model, S, sampler.precision)
         x = S[:, 0].cpu().numpy().astype(np.float64)
         st = ParallelMALAState(x, float(logps[0]), S, logps, ws, tape, 0)
         return ParallelMALATransition(x, float(logps[0])), st

@@ -137,6 +137,14 @@ struct pmcmc_deer_ws {
     FullWork fullw;  // jacobian = :full
     RefineWork refw;  // FP64 decision overrides (GEMM targets, FP32)
     DiagWork diagw;   // jacobian = :diag on a GEMM target
+    // the last solve's final Newton iteration on a GEMM target: its iterate
+    // S_k (state before each step), decisions and the x0 copy, from which
+    // pmcmc_deer_last_trajectory assembles FT(S_k) and its log-densities
+    bool last_gemm = false;
+    const void* last_prev = nullptr;
+    uint8_t* last_acc = nullptr;
+    uint8_t* acc_ws = nullptr;  // decisions when the caller passes none
+    void* x0_ws = nullptr;
     int8_t* prev_dec = nullptr;  // previous iteration's decisions (FP32 hysteresis)
     double* tile_smax = nullptr;  // max |S| per level-0 tile when it last ran (final prefix)
     int tile_smax_n = 0;
@@ -1133,6 +1141,8 @@ extern "C" int pmcmc_deer_ws_destroy(pmcmc_deer_ws* w) {
     full_work_destroy(&w->fullw);
     refine_work_destroy(&w->refw);
     diag_work_destroy(&w->diagw);
+    hipFree(w->acc_ws);
+    hipFree(w->x0_ws);
     hipFree(w->prev_dec);
     hipFree(w->tile_smax);
     hipFree(w->adj_lam);
@@ -1261,6 +1271,17 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
         w->tile_smax_n = w->plan.lev[0].nT;
     }
     T* bufs[2] = {(T*)w->buf[0], (T*)w->buf[1]};
+    // GEMM targets keep the final iteration's decisions and x0 for the
+    // trajectory assembly (pmcmc_deer_last_trajectory)
+    w->last_gemm = false;
+    if (mode == MODE_GEMM) {
+        if (!acc_out) {
+            if (!w->acc_ws) HIPCHK(c, hipMalloc(&w->acc_ws, (size_t)Tn));
+            acc_out = w->acc_ws;
+        }
+        if (!w->x0_ws) HIPCHK(c, hipMalloc(&w->x0_ws, (size_t)D * sizeof(double)));
+        HIPCHK(c, hipMemcpyAsync(w->x0_ws, x0, (size_t)D * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    }
     hipLaunchKernelGGL(reset_sync_kernel, dim3(1), dim3(64), 0, c->stream, w->sync);
     if (full) full_geometry_reset(&w->fullw);
     hipLaunchKernelGGL((init_iterate_kernel<T>), dim3(grid_for(D * Tn)), dim3(256), 0, c->stream,
@@ -1391,12 +1412,32 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
         HIPCHK(c, hipMemcpyAsync(S_out, result, (size_t)D * Tn * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->prof.flush(iters - 1);
+    if (mode == MODE_GEMM) {
+        w->last_gemm = true;
+        w->last_prev = bufs[(iters - 1) & 1];
+        w->last_acc = acc_out;
+    }
     if (info) {
         info->converged = hs.converged;
         info->iters = iters;
         info->metric = hs.last_metric;
     }
     return PMCMC_OK;
+}
+
+// FT_t of the final Newton iteration (Y_t where step t accepted, else the
+// state before t) and its log-density (lpy or lpx of that iteration).
+template <typename T>
+__global__ __launch_bounds__(256) void last_trajectory_kernel(const T* x0, const T* S_prev, const T* Y,
+                                                              const uint8_t* acc, const double* lpx,
+                                                              const double* lpy, int64_t D, int64_t n, T* S_out,
+                                                              double* lp_out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = i / D, d = i - t * D;
+        const bool a = acc[t] != 0;
+        S_out[i] = a ? Y[i] : (t == 0 ? x0[d] : S_prev[i - D]);
+        if (d == 0 && lp_out) lp_out[t] = a ? lpy[t] : lpx[t];
+    }
 }
 
 extern "C" int pmcmc_deer_solve(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
@@ -1422,6 +1463,27 @@ extern "C" int pmcmc_deer_solve(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_targ
     if (w->prec == PMCMC_PREC_F64)
         return deer_solve_t<double>(c, w, tgt, p, x0, Xi, U, init, S_out, acc_out, info);
     return deer_solve_t<float>(c, w, tgt, p, x0, Xi, U, init, S_out, acc_out, info);
+}
+
+extern "C" int pmcmc_deer_last_trajectory(pmcmc_deer_ws* w, void* S_out, double* logp_out) {
+    if (!w || !w->ctx) return PMCMC_ERR_ARGUMENT;
+    pmcmc_ctx* c = w->ctx;
+    if (!w->last_gemm || !w->last_prev || !w->last_acc)
+        return fail(c, PMCMC_ERR_ARGUMENT, "last trajectory: the workspace's last solve was not a GEMM-target solve");
+    if (!S_out) return fail(c, PMCMC_ERR_ARGUMENT, "last trajectory: null S_out");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = w->D * w->T;
+    if (w->prec == PMCMC_PREC_F64)
+        hipLaunchKernelGGL((last_trajectory_kernel<double>), dim3(grid_for(n)), dim3(256), 0, c->stream,
+                           (const double*)w->x0_ws, (const double*)w->last_prev, (const double*)w->glmw.Y,
+                           w->last_acc, w->glmw.lpx, w->glmw.lpy, w->D, n, (double*)S_out, logp_out);
+    else
+        hipLaunchKernelGGL((last_trajectory_kernel<float>), dim3(grid_for(n)), dim3(256), 0, c->stream,
+                           (const float*)w->x0_ws, (const float*)w->last_prev, (const float*)w->glmw.Y,
+                           w->last_acc, w->glmw.lpx, w->glmw.lpy, w->D, n, (float*)S_out, logp_out);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PMCMC_OK;
 }
 
 // ---------------------------------------------------------------------------
