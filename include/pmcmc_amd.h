@@ -231,6 +231,15 @@ int pmcmc_deer_solve(pmcmc_ctx* ctx, pmcmc_deer_ws* ws, const pmcmc_target* tgt,
                      const void* U, const void* init, void* S_out, uint8_t* accepted_out,
                      pmcmc_deer_info* info);
 
+/* One block's MALA tape drawn on the device (opt-in; replaces the host draw
+ * and upload of _make_mala_tape_block, src/interface.jl:334-358, with another
+ * stream): Xi (D x T column-major) ~ N(0, I) and U (T) ~ U(0, 1) in the
+ * precision's state type (FP64 for PMCMC_PREC_F64, else FP32), a pure
+ * function of `seed`: Philox4x64-10 with key (seed, 0) for the noise (Box-
+ * Muller pairs) and (seed, 1) for the uniforms, counter = call index.
+ * Synchronises the context stream.                                          */
+int pmcmc_tape_generate(pmcmc_ctx* ctx, uint64_t seed, int precision, void* Xi, void* U, int64_t D, int64_t T);
+
 /* The trajectory and its log-densities from the final Newton iteration of
  * the workspace's last pmcmc_deer_solve on a logistic or dense Gaussian
  * target (replaces the separate evaluation of _trajectory_logps,

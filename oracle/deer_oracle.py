@@ -840,8 +840,58 @@ def make_tape(rng, D, T, dtype=np.float64):
     return np.asfortranarray(Xi), U
 
 
+def philox4x64_10(ctr, key):
+    """Philox4x64-10 (Salmon et al., SC 2011) on Python integers: ten rounds
+    of two 64x64->128 multiplies with the published constants and Weyl key
+    schedule.  No reference counterpart (the reference draws its tape from
+    Julia's RNG); pinned against numpy.random.Philox."""
+    M0, M1 = 0xD2E7470EE14C6C93, 0xCA5A826395121157
+    W0, W1 = 0x9E3779B97F4A7C15, 0xBB67AE8584CAA73B
+    mask = (1 << 64) - 1
+    x = [int(v) & mask for v in ctr]
+    k0, k1 = int(key[0]) & mask, int(key[1]) & mask
+    for _ in range(10):
+        p0, p1 = M0 * x[0], M1 * x[2]
+        x = [(p1 >> 64) ^ x[1] ^ k0, p1 & mask, (p0 >> 64) ^ x[3] ^ k1, p0 & mask]
+        k0, k1 = (k0 + W0) & mask, (k1 + W1) & mask
+    return x
+
+
+def philox_words(seed, stream, ncalls):
+    """Words of calls 0..ncalls-1 of Philox4x64-10 with key (seed, stream) and
+    counter (j, 0, 0, 0), in call order: numpy's Philox started one step
+    before counter 0 (it advances the counter before each call)."""
+    m = np.iinfo(np.uint64).max
+    bg = np.random.Philox(key=np.array([seed, stream], dtype=np.uint64),
+                          counter=np.array([m, m, m, m], dtype=np.uint64))
+    return bg.random_raw(4 * ncalls).astype(np.uint64)
+
+
+def device_tape(seed, D, T, dtype=np.float64):
+    """The opt-in on-device tape (no reference counterpart; it replaces the
+    host draw of src/interface.jl:334-358 with another stream): U from
+    stream 1 as ((w >> 11) + 1/2) 2^-53; Xi (column-major element i = tD + d)
+    by Box-Muller on stream 0's word pairs (cosine for even i, sine for odd)."""
+    n = D * T
+    w = philox_words(seed, 1, (T + 3) // 4)[:T]
+    U = ((w >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0**-53
+    npair = (n + 1) // 2
+    z = philox_words(seed, 0, (n + 3) // 4)[:2 * npair].reshape(npair, 2)
+    u1 = ((z[:, 0] >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0**-53
+    u2 = (z[:, 1] >> np.uint64(11)).astype(np.float64) * 2.0**-53
+    r = np.sqrt(-2.0 * np.log(u1))
+    th = 6.283185307179586 * u2
+    flat = np.stack([r * np.cos(th), r * np.sin(th)], axis=1).reshape(-1)[:n]
+    Xi = flat.reshape(T, D).T
+    if dtype == np.float32:
+        return np.asfortranarray(Xi.astype(np.float32)), np.minimum(U, 1.0 - 2.0**-24).astype(np.float32)
+    return np.asfortranarray(Xi), U
+
+
 # How each oracle function is pinned (tests/test_oracle_*.py).
 PINNED = {
+    "philox4x64_10": "equals numpy.random.Philox word for word (several keys and counters, including carries)",
+    "device_tape": "Philox words from numpy's generator; U moments and Xi moments / normality (KS) on 2^18 draws",
     "run_mala_frozen": "equals run_mala_sequential_taped when fed that run's own decisions",
     "mala_sampler_run": "equals run_mala_sequential_taped on the tape drawn from the same RNG stream in the "
                         "same order (xi then u per step); logp of every state equals logp_batch of the states",

@@ -136,6 +136,7 @@ _SIGS = {
     "pmcmc_deer_solve": ([_P, _P, _P, C.POINTER(DeerParams), _P, _P, _P, _P, _P, _P, C.POINTER(DeerInfo)], C.c_int),
     "pmcmc_target_logdensity_batch": ([_P, _P, C.c_int, _P, _P, _I64], C.c_int),
     "pmcmc_deer_last_trajectory": ([_P, _P, _P], C.c_int),
+    "pmcmc_tape_generate": ([_P, C.c_uint64, C.c_int, _P, _P, _I64, _I64], C.c_int),
     "pmcmc_batch_create": ([_P, C.c_int32, C.POINTER(ChainDesc), C.POINTER(DeerParams), _I64, C.POINTER(_P)],
                            C.c_int),
     "pmcmc_batch_set_graph": ([_P, C.c_int32], C.c_int),

@@ -14,6 +14,10 @@ running the solve inside HIP kernels:
   src/interface.jl:486, 510).
 * ``backend`` (the AD backend for the HVP fallback) is accepted and ignored:
   HVPs are analytic in the kernels.
+* ``tape_rng="device"`` (opt-in, no reference counterpart) draws each
+  block's tape on the GPU (``make_mala_tape_block_device``: Philox4x64-10
+  from one seed drawn from the Generator) instead of on the host and
+  uploading it; the default ``"host"`` keeps the numpy stream.
 """
 from __future__ import annotations
 
@@ -93,8 +97,11 @@ class ParallelMALASampler:
     cholM: object = None
     backend: object = None
     precision: object = None
+    tape_rng: str = "host"
 
     def __post_init__(self):
+        if self.tape_rng not in ("host", "device"):
+            raise ArgumentError("tape_rng must be \"host\" or \"device\"")
         if not self.epsilon > 0:
             raise ArgumentError(f"epsilon must be > 0, got {self.epsilon}")
         j = str(self.jacobian).lstrip(":")
@@ -151,11 +158,28 @@ def make_mala_tape_block(rng, D, T, precision):
     return Xi, U
 
 
+def make_mala_tape_block_device(seed, D, T, precision, device=None):
+    """One block's tape drawn on the device (opt-in; the reference draws it
+    on the host, src/interface.jl:334-358): Xi (D x T) ~ N(0, I) and U (T) ~
+    U(0, 1) as device tensors, a pure function of ``seed``
+    (pmcmc_tape_generate: Philox4x64-10, Box-Muller)."""
+    dev = torch.device("cuda", device if device is not None else torch.cuda.current_device())
+    dt = _device.torch_dtype(precision)
+    Xi = _device.colmajor_empty(D, T, dt, dev)
+    U = torch.empty(T, dtype=dt, device=dev)
+    _device.context(dev.index).call("pmcmc_tape_generate", int(seed) & 0xFFFFFFFFFFFFFFFF, int(precision),
+                                    _device.ptr(Xi), _device.ptr(U), int(D), int(T))
+    return Xi, U
+
+
 def _solve_block(rng, model, sampler, x0, ws):
     from . import DEER
 
     D, T = model.dim, sampler.T
-    Xi, U = make_mala_tape_block(rng, D, T, sampler.precision)
+    if sampler.tape_rng == "device":
+        Xi, U = make_mala_tape_block_device(int(rng.integers(0, 2**63 - 1)), D, T, sampler.precision)
+    else:
+        Xi, U = make_mala_tape_block(rng, D, T, sampler.precision)
     seed = int(rng.integers(0, 2**63 - 1))
     rec = DEER.TapedRecursion(model, sampler.epsilon, Xi, U, sampler.precision)
     if ws is None:

@@ -30,6 +30,7 @@
 #include "refine.hpp"
 #include "scanfull.hpp"
 #include "scanreg.hpp"
+#include "tape.hpp"
 #include "gauss.hpp"
 #include "glm.hpp"
 #include "tile_kernel.hpp"
@@ -1463,6 +1464,20 @@ extern "C" int pmcmc_deer_solve(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_targ
     if (w->prec == PMCMC_PREC_F64)
         return deer_solve_t<double>(c, w, tgt, p, x0, Xi, U, init, S_out, acc_out, info);
     return deer_solve_t<float>(c, w, tgt, p, x0, Xi, U, init, S_out, acc_out, info);
+}
+
+extern "C" int pmcmc_tape_generate(pmcmc_ctx* c, uint64_t seed, int precision, void* Xi, void* U, int64_t D,
+                                   int64_t T) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    if (!Xi || !U) return fail(c, PMCMC_ERR_ARGUMENT, "tape: null buffer");
+    if (D < 1 || T < 1) return fail(c, PMCMC_ERR_DIMENSION, "tape: D and T must be >= 1");
+    if (precision != PMCMC_PREC_F64 && precision != PMCMC_PREC_F32 && precision != PMCMC_PREC_BF16_JAC)
+        return fail(c, PMCMC_ERR_ARGUMENT, "tape: bad precision %d", precision);
+    HIPCHK(c, hipSetDevice(c->device));
+    if (tape_generate(c->stream, seed, precision == PMCMC_PREC_F64, Xi, U, D, T) != 0)
+        return fail(c, PMCMC_ERR_HIP, "tape: kernel launch failed");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PMCMC_OK;
 }
 
 extern "C" int pmcmc_deer_last_trajectory(pmcmc_deer_ws* w, void* S_out, double* logp_out) {
