@@ -105,6 +105,29 @@ int pmcmc_scan_diag_f32(pmcmc_ctx* ctx, const float* A, const float* B, const fl
 int pmcmc_scan_diag_f64(pmcmc_ctx* ctx, const double* A, const double* B, const double* s0,
                         double* S, int64_t D, int64_t T);
 
+/* ---- validation helpers of the diagonal scan -----------------------------
+ * DEERScan.solve_affine_seq! (src/DEER/DEERScan.jl:51-68): the sequential
+ * recurrence, one device thread per row, product and sum rounded separately
+ * as the reference's broadcast does.  DEERScan.affine_scan_residual
+ * (:203-224): *out = max over (d, t) of |S - (A .* S_prev + B)| (S_prev = s0
+ * for t = 1; NaN when any entry is NaN).  The comparison of check_affine_scan
+ * (:240-258): out[0] = max |X - Y|, out[1] = max |X - Y| / (atol + rtol |X|)
+ * (NaN where both are 0, as the reference's ratio), out[2] = 1 when every
+ * |X - Y| <= atol + rtol |X| (its `ok`), else 0; n entries.  out: HOST
+ * pointers.  Synchronise the context stream.                                */
+int pmcmc_affine_seq_f32(pmcmc_ctx* ctx, const float* A, const float* B, const float* s0, float* S,
+                         int64_t D, int64_t T);
+int pmcmc_affine_seq_f64(pmcmc_ctx* ctx, const double* A, const double* B, const double* s0, double* S,
+                         int64_t D, int64_t T);
+int pmcmc_affine_residual_f32(pmcmc_ctx* ctx, const float* S, const float* A, const float* B,
+                              const float* s0, int64_t D, int64_t T, double* out);
+int pmcmc_affine_residual_f64(pmcmc_ctx* ctx, const double* S, const double* A, const double* B,
+                              const double* s0, int64_t D, int64_t T, double* out);
+int pmcmc_affine_compare_f32(pmcmc_ctx* ctx, const float* X, const float* Y, int64_t n, double atol,
+                             double rtol, double* out);
+int pmcmc_affine_compare_f64(pmcmc_ctx* ctx, const double* X, const double* Y, int64_t n, double atol,
+                             double rtol, double* out);
+
 /* ---- allocation-free diagonal scan ---------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

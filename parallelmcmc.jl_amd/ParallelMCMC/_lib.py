@@ -112,6 +112,12 @@ _SIGS = {
     "pmcmc_scan_diag_f32": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
     "pmcmc_scan_diag_f64": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
     "pmcmc_scan_diag_f32_noalloc": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
+    "pmcmc_affine_seq_f32": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
+    "pmcmc_affine_seq_f64": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
+    "pmcmc_affine_residual_f32": ([_P, _P, _P, _P, _P, _I64, _I64, C.POINTER(C.c_double)], C.c_int),
+    "pmcmc_affine_residual_f64": ([_P, _P, _P, _P, _P, _I64, _I64, C.POINTER(C.c_double)], C.c_int),
+    "pmcmc_affine_compare_f32": ([_P, _P, _P, _I64, C.c_double, C.c_double, C.POINTER(C.c_double)], C.c_int),
+    "pmcmc_affine_compare_f64": ([_P, _P, _P, _I64, C.c_double, C.c_double, C.POINTER(C.c_double)], C.c_int),
     "pmcmc_scan_full_f32": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
     "pmcmc_scan_full_f64": ([_P, _P, _P, _P, _P, _I64, _I64], C.c_int),
     "pmcmc_scan_tune": ([_P, C.c_int, _I64, _I64, C.c_int32, C.POINTER(TileShape)], C.c_int),

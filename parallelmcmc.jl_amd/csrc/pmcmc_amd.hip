@@ -23,6 +23,7 @@
 
 #include "../../include/pmcmc_amd.h"
 #include "adjoint.hpp"
+#include "affine.hpp"
 #include "common.hpp"
 #include "diagjac.hpp"
 #include "fullscan.hpp"
@@ -64,6 +65,7 @@ struct pmcmc_ctx {
     size_t scan_blob_cap = 0;
     SyncState* host_sync = nullptr;  // pinned
     double* red_scratch = nullptr;   // partial sums of pmcmc_dot / pmcmc_dotsum
+    unsigned long long* affine_red = nullptr;  // maxima and flag of the affine-scan checks
     Prof prof;
     // the creator's reference plus one per live target, workspace and batch
     // created on this context: pmcmc_ctx_destroy drops the creator's, the
@@ -516,6 +518,7 @@ static void ctx_release(pmcmc_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->scan_blob) hipFree(c->scan_blob);
     if (c->red_scratch) hipFree(c->red_scratch);
+    if (c->affine_red) hipFree(c->affine_red);
     if (c->host_sync) hipHostFree(c->host_sync);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -795,6 +798,98 @@ extern "C" int pmcmc_scan_diag_f32_noalloc(pmcmc_ctx* c, const float* A, const f
 extern "C" int pmcmc_scan_diag_f64(pmcmc_ctx* c, const double* A, const double* B,
                                    const double* s0, double* S, int64_t D, int64_t T) {
     return scan_diag<double>(c, A, B, s0, S, D, T);
+}
+
+// sequential solve, residual and comparison of diagonal affine scans
+// (src/DEER/DEERScan.jl:51-68, 203-258)
+static int affine_args(pmcmc_ctx* c, const char* what, int64_t D, int64_t Tn) {
+    if (D < 1 || Tn < 0)
+        return fail(c, PMCMC_ERR_DIMENSION, "%s: need D >= 1 and T >= 0 (D=%lld T=%lld)", what, (long long)D,
+                    (long long)Tn);
+    return PMCMC_OK;
+}
+
+static int affine_red_get(pmcmc_ctx* c, const unsigned long long* red, double* out, int n) {
+    unsigned long long h[3] = {0ull, 0ull, 0ull};
+    HIPCHK(c, hipMemcpyAsync(h, red, (size_t)n * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n && i < 2; ++i) memcpy(&out[i], &h[i], sizeof(double));
+    if (n > 2) out[2] = h[2] ? 0.0 : 1.0;  // 1: every entry within atol + rtol |X|
+    return PMCMC_OK;
+}
+
+static int affine_seq_entry(pmcmc_ctx* c, bool f64, const void* A, const void* B, const void* s0, void* S,
+                            int64_t D, int64_t Tn) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    int rc = affine_args(c, "sequential affine solve", D, Tn);
+    if (rc != PMCMC_OK) return rc;
+    if (Tn == 0) return PMCMC_OK;
+    if (!A || !B || !s0 || !S) return fail(c, PMCMC_ERR_ARGUMENT, "sequential affine solve: null buffer");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (affine_seq(c->stream, f64, A, B, s0, S, D, Tn) != 0)
+        return fail(c, PMCMC_ERR_HIP, "sequential affine solve: launch failed");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PMCMC_OK;
+}
+extern "C" int pmcmc_affine_seq_f32(pmcmc_ctx* c, const float* A, const float* B, const float* s0, float* S,
+                                    int64_t D, int64_t T) {
+    return affine_seq_entry(c, false, A, B, s0, S, D, T);
+}
+extern "C" int pmcmc_affine_seq_f64(pmcmc_ctx* c, const double* A, const double* B, const double* s0, double* S,
+                                    int64_t D, int64_t T) {
+    return affine_seq_entry(c, true, A, B, s0, S, D, T);
+}
+
+static int affine_residual_entry(pmcmc_ctx* c, bool f64, const void* S, const void* A, const void* B,
+                                 const void* s0, int64_t D, int64_t Tn, double* out) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    int rc = affine_args(c, "affine scan residual", D, Tn);
+    if (rc != PMCMC_OK) return rc;
+    if (!out) return fail(c, PMCMC_ERR_ARGUMENT, "affine scan residual: null output");
+    if (Tn == 0) {
+        *out = 0.0;
+        return PMCMC_OK;
+    }
+    if (!A || !B || !s0 || !S) return fail(c, PMCMC_ERR_ARGUMENT, "affine scan residual: null buffer");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->affine_red) HIPCHK(c, hipMalloc(&c->affine_red, 3 * sizeof(unsigned long long)));
+    if (affine_residual(c->stream, f64, S, A, B, s0, D, Tn, c->affine_red) != 0)
+        return fail(c, PMCMC_ERR_HIP, "affine scan residual: launch failed");
+    return affine_red_get(c, c->affine_red, out, 1);
+}
+extern "C" int pmcmc_affine_residual_f32(pmcmc_ctx* c, const float* S, const float* A, const float* B,
+                                         const float* s0, int64_t D, int64_t T, double* out) {
+    return affine_residual_entry(c, false, S, A, B, s0, D, T, out);
+}
+extern "C" int pmcmc_affine_residual_f64(pmcmc_ctx* c, const double* S, const double* A, const double* B,
+                                         const double* s0, int64_t D, int64_t T, double* out) {
+    return affine_residual_entry(c, true, S, A, B, s0, D, T, out);
+}
+
+static int affine_compare_entry(pmcmc_ctx* c, bool f64, const void* X, const void* Y, int64_t n, double atol,
+                                double rtol, double* out) {
+    if (!c) return PMCMC_ERR_ARGUMENT;
+    if (n < 0) return fail(c, PMCMC_ERR_DIMENSION, "affine scan compare: negative length");
+    if (!out) return fail(c, PMCMC_ERR_ARGUMENT, "affine scan compare: null output");
+    if (n == 0) {
+        out[0] = out[1] = 0.0;
+        out[2] = 1.0;
+        return PMCMC_OK;
+    }
+    if (!X || !Y) return fail(c, PMCMC_ERR_ARGUMENT, "affine scan compare: null buffer");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->affine_red) HIPCHK(c, hipMalloc(&c->affine_red, 3 * sizeof(unsigned long long)));
+    if (affine_compare(c->stream, f64, X, Y, n, atol, rtol, c->affine_red) != 0)
+        return fail(c, PMCMC_ERR_HIP, "affine scan compare: launch failed");
+    return affine_red_get(c, c->affine_red, out, 3);
+}
+extern "C" int pmcmc_affine_compare_f32(pmcmc_ctx* c, const float* X, const float* Y, int64_t n, double atol,
+                                        double rtol, double* out) {
+    return affine_compare_entry(c, false, X, Y, n, atol, rtol, out);
+}
+extern "C" int pmcmc_affine_compare_f64(pmcmc_ctx* c, const double* X, const double* Y, int64_t n, double atol,
+                                        double rtol, double* out) {
+    return affine_compare_entry(c, true, X, Y, n, atol, rtol, out);
 }
 
 // dense-Jacobian scan with materialised Jacobians (no reference counterpart)
