@@ -192,6 +192,14 @@ int pmcmc_target_diag_gaussian(pmcmc_ctx* ctx, int64_t D, int dtype_prec, const 
  * benchmarks/ParallelMCMCBenchmarks/src/models/bayes_logreg.jl:91-206.      */
 int pmcmc_target_logistic(pmcmc_ctx* ctx, int64_t N, int64_t D, int dtype_prec, const void* Xd,
                           const void* y, double prior_scale, pmcmc_target** out);
+/* Replaces a logistic target's isotropic prior by N(0, P^-1) with the dense
+ * precision P (D x D, symmetric), for every kernel that evaluates it
+ * (gradient, Hessian-vector products, log-density, the FP64 decisions).  No
+ * reference counterpart as a separate call: it carries a mass-matrix
+ * preconditioner (ParallelMALASampler cholM, src/MALA/MALA.jl:29-47) into
+ * the reparametrized target x = L theta, whose prior precision is
+ * L' L / prior_scale^2.                                                     */
+int pmcmc_target_logistic_set_prior(pmcmc_target* tgt, const void* P, int dtype_prec);
 /* Dense Gaussian N(mean, Lambda^-1); Lambda is D x D symmetric (column-major),
  * mean may be NULL (zero).  The precision-matrix form of the correlated
  * Gaussian models of the reference's GPU tests (grad = -Lambda (x - mean),

@@ -250,6 +250,46 @@ def run_mala_sequential_taped(target, x0, eps, Xi, U, check_u=True):
     return S, acc, la
 
 
+def logq_mala_mass(y, x, gx, eps, L):
+    """log q(y|x) with the mass matrix M = L L': proposal N(x + eps M g,
+    2 eps M).  src/MALA/MALA.jl:172-193 (cholM branch, _quad_Minv! :41-44,
+    _logdet_M :47)."""
+    d = len(x)
+    M = L @ L.T
+    r = y - (x + eps * (M @ gx))
+    w = np.linalg.solve(L, r)
+    logdet = 2.0 * np.sum(np.log(np.diag(L)))
+    return -0.5 * np.dot(w, w) / (2 * eps) - (d / 2) * math.log(4 * math.pi * eps) - 0.5 * logdet
+
+
+def mala_step_taped_mass(target, x, eps, xi, u, L):
+    """One taped MALA step with the mass matrix M = L L':
+    y = x + eps M grad(x) + sqrt(2 eps) L xi (src/MALA/MALA.jl:207-229,
+    _apply_M! :31-35, _apply_L! :38), log q as logq_mala_mass, accept iff
+    log u < log alpha (:276-305).  Returns (x_next, accepted, logalpha)."""
+    M = L @ L.T
+    gx = target.grad(x)
+    y = x + eps * (M @ gx) + math.sqrt(2 * eps) * (L @ xi)
+    gy = target.grad(y)
+    la = (target.logp(y) + logq_mala_mass(x, y, gy, eps, L)) - (target.logp(x) + logq_mala_mass(y, x, gx, eps, L))
+    acc = math.log(u) < la if u > 0 else True
+    return (y.copy() if acc else x.copy()), acc, la
+
+
+def run_mala_sequential_taped_mass(target, x0, eps, Xi, U, L):
+    """Sequential taped MALA with the mass matrix M = L L' (src/MALA/MALA.jl:
+    322-346 with cholM).  Returns S (D x T), accept mask, logalpha."""
+    D, T = Xi.shape
+    S = np.empty((D, T))
+    acc = np.zeros(T, dtype=bool)
+    la = np.zeros(T)
+    x = np.asarray(x0, dtype=np.float64).copy()
+    for t in range(T):
+        x, acc[t], la[t] = mala_step_taped_mass(target, x, eps, Xi[:, t].astype(np.float64), float(U[t]), L)
+        S[:, t] = x
+    return S, acc, la
+
+
 def mala_sampler_run(target, x0, eps, n, rng):
     """``n`` steps of the sequential MALASampler (the CPU baseline).
 
@@ -890,6 +930,10 @@ def device_tape(seed, D, T, dtype=np.float64):
 
 # How each oracle function is pinned (tests/test_oracle_*.py).
 PINNED = {
+    "logq_mala_mass": "equals scipy.stats.multivariate_normal(x + eps M g, 2 eps M).logpdf(y) at 1e-12",
+    "mala_step_taped_mass": "L = I reduces to mala_step_taped; forced accept/reject via u; the proposal's "
+                            "mean and covariance (eps M g, 2 eps M) from 2e5 draws",
+    "run_mala_sequential_taped_mass": "composition of mala_step_taped_mass; L = I equals run_mala_sequential_taped",
     "philox4x64_10": "equals numpy.random.Philox word for word (several keys and counters, including carries)",
     "device_tape": "Philox words from numpy's generator; U moments and Xi moments / normality (KS) on 2^18 draws",
     "run_mala_frozen": "equals run_mala_sequential_taped when fed that run's own decisions",

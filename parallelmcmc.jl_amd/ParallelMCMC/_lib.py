@@ -134,6 +134,7 @@ _SIGS = {
     "pmcmc_target_diag_gaussian": ([_P, _I64, C.c_int, _P, _P, C.POINTER(_P)], C.c_int),
     "pmcmc_target_logistic": ([_P, _I64, _I64, C.c_int, _P, _P, C.c_double, C.POINTER(_P)], C.c_int),
     "pmcmc_target_gaussian": ([_P, _I64, C.c_int, _P, _P, C.POINTER(_P)], C.c_int),
+    "pmcmc_target_logistic_set_prior": ([_P, _P, C.c_int], C.c_int),
     "pmcmc_target_destroy": ([_P], C.c_int),
     "pmcmc_target_dim": ([_P], _I64),
     "pmcmc_mala_fwd_and_jvp": ([_P, _P, C.c_int, C.c_double, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64], C.c_int),

@@ -14,6 +14,14 @@ running the solve inside HIP kernels:
   src/interface.jl:486, 510).
 * ``backend`` (the AD backend for the HVP fallback) is accepted and ignored:
   HVPs are analytic in the kernels.
+* ``cholM`` (the mass matrix M = L L') is applied by reparametrization: each
+  block is solved for theta = L^-1 x on the target p(L theta) (a dense
+  Gaussian, or the logistic target with design X L and prior precision
+  L' L / s^2), where standard MALA driven by the same tape is the
+  preconditioned MALA of src/MALA/MALA.jl:29-47, 430-438 on x; the
+  trajectory is mapped back by x = L theta on the device.  The fixed point
+  (the trajectory) is the reference's; the Newton iterates differ (the
+  Jacobian estimators act in theta coordinates).
 * ``tape_rng="device"`` (opt-in, no reference counterpart) draws each
   block's tape on the GPU (``make_mala_tape_block_device``: Philox4x64-10
   from one seed drawn from the Generator) instead of on the host and
@@ -109,7 +117,15 @@ class ParallelMALASampler:
             raise ArgumentError("jacobian must be :stoch_diag, :diag, :full or :full_sigmoid")
         self.jacobian = j
         if self.cholM is not None:
-            raise NotImplementedError("mass-matrix preconditioning (cholM) is not implemented on the GPU path")
+            # a Cholesky factor object (with .L, as Julia's Cholesky) or its
+            # lower-triangular factor L of the mass matrix M = L L'
+            L = np.asarray(getattr(self.cholM, "L", self.cholM), dtype=np.float64)
+            if L.ndim != 2 or L.shape[0] != L.shape[1]:
+                raise ArgumentError("cholM must be a D x D lower-triangular Cholesky factor")
+            if np.any(np.triu(L, 1) != 0) or not np.all(np.diag(L) > 0):
+                raise ArgumentError("cholM must be lower-triangular with a positive diagonal")
+            self._L = L
+            self._theta_models = {}
         self.precision = _precision_of(self.epsilon, self.precision)
 
 
@@ -172,10 +188,29 @@ def make_mala_tape_block_device(seed, D, T, precision, device=None):
     return Xi, U
 
 
+def _theta_model(model, sampler):
+    """With a mass matrix M = L L' (cholM) the block is solved for
+    theta = L^-1 x on the reparametrized target (targets.reparametrized):
+    standard MALA there, driven by the same tape, is the preconditioned MALA
+    of src/MALA/MALA.jl:29-47, 430-438, 467-471 on x.  Cached per model."""
+    if sampler.cholM is None:
+        return model
+    key = id(model)
+    if key not in sampler._theta_models:
+        from .targets import reparametrized
+
+        tgt = reparametrized(_target_of(model), sampler._L)
+        sampler._theta_models[key] = (model, DensityModel(dim=model.dim, target=tgt))
+    return sampler._theta_models[key][1]
+
+
 def _solve_block(rng, model, sampler, x0, ws):
     from . import DEER
 
     D, T = model.dim, sampler.T
+    if sampler.cholM is not None:
+        x0 = np.linalg.solve(sampler._L, np.asarray(x0, dtype=np.float64))
+        model = _theta_model(model, sampler)
     if sampler.tape_rng == "device":
         Xi, U = make_mala_tape_block_device(int(rng.integers(0, 2**63 - 1)), D, T, sampler.precision)
     else:
@@ -196,7 +231,7 @@ def _trajectory_logps(model, S, precision):
     return MALA.logdensity_batch(model, S, precision=precision).cpu().numpy()
 
 
-def _block_trajectory(model, S, ws, precision):
+def _block_trajectory(model, S, ws, sampler):
     """The block's trajectory and log-densities (src/interface.jl:517-524).
     GEMM targets (logistic, dense Gaussian): both from the final Newton
     iteration, assembled on the device (DEER.last_trajectory: FT of the last
@@ -204,12 +239,20 @@ def _block_trajectory(model, S, ws, precision):
     passes already evaluated).  Diagonal targets: S and one elementwise
     log-density kernel."""
     from . import DEER
+    from .owned import pmcmc_matmul
     from .targets import GaussianTarget, LogisticTarget
 
-    if isinstance(model.target, (LogisticTarget, GaussianTarget)):
-        S_ft, lp = DEER.last_trajectory(ws)
-        return S_ft, lp.cpu().numpy()
-    return S, _trajectory_logps(model, S, precision)
+    m = _theta_model(model, sampler)
+    if isinstance(m.target, (LogisticTarget, GaussianTarget)):
+        S, lp = DEER.last_trajectory(ws)
+        lp = lp.cpu().numpy()
+    else:
+        lp = _trajectory_logps(m, S, sampler.precision)
+    if sampler.cholM is not None:
+        # x = L theta (device GEMM); the log-densities are those of x already
+        L = torch.as_tensor(sampler._L, dtype=S.dtype, device=S.device)
+        S = pmcmc_matmul(L, S)
+    return S, lp
 
 
 def sample(rng, model, sampler, N, initial_params=None, progress=False, return_infos=False):
@@ -226,7 +269,7 @@ def sample(rng, model, sampler, N, initial_params=None, progress=False, return_i
     while nsteps < N:
         S, tape, ws, info = _solve_block(rng, model, sampler, x0, ws)
         nkeep = min(sampler.T, N - nsteps)
-        S, logps = _block_trajectory(model, S, ws, sampler.precision)
+        S, logps = _block_trajectory(model, S, ws, sampler)
         S_keep = S[:, :nkeep].cpu().numpy().astype(np.float64, copy=True)
         logps = logps[:nkeep]
         blocks.append(S_keep)
@@ -252,7 +295,7 @@ def step(rng, model, sampler, state=None, initial_params=None):
             x0 = state.trajectory[:, sampler.T - 1].cpu().numpy().astype(np.float64)
             ws = state.workspace
         S, tape, ws, _ = _solve_block(rng, model, sampler, x0, ws)
-        S, logps = _block_trajectory(model, S, ws, sampler.precision)
+        S, logps = _block_trajectory(model, S, ws, sampler)
         S = S.t().clone().t()
         x = S[:, 0].cpu().numpy().astype(np.float64)
         st = ParallelMALAState(x, float(logps[0]), S, logps, ws, tape, 0)

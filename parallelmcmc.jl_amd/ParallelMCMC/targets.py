@@ -69,10 +69,12 @@ class StdNormalTarget(DiagGaussianTarget):
 
 
 class LogisticTarget(GPUTarget):
-    """Bayesian logistic regression, prior N(0, prior_scale^2 I).
+    """Bayesian logistic regression, prior N(0, prior_scale^2 I), or
+    N(0, P^-1) with a dense ``prior_precision`` P (the reparametrized target
+    of a mass-matrix preconditioner).
     benchmarks/ParallelMCMCBenchmarks/src/models/bayes_logreg.jl:91-206."""
 
-    def __init__(self, X, y, prior_scale=1.0):
+    def __init__(self, X, y, prior_scale=1.0, prior_precision=None):
         super().__init__()
         self.X = np.asarray(X, dtype=np.float64)
         self.y = np.asarray(y, dtype=np.float64)
@@ -80,6 +82,12 @@ class LogisticTarget(GPUTarget):
             raise _lib.DimensionMismatch("X must be N x D and y must have N entries")
         self.N, self.dim = self.X.shape
         self.prior_scale = float(prior_scale)
+        self.prior_precision = None
+        if prior_precision is not None:
+            P = np.asarray(prior_precision, dtype=np.float64)
+            if P.shape != (self.dim, self.dim):
+                raise _lib.DimensionMismatch("prior_precision must be D x D")
+            self.prior_precision = 0.5 * (P + P.T)
 
     def _create(self, ctx):
         dev = torch.device("cuda", ctx.device)
@@ -88,6 +96,11 @@ class LogisticTarget(GPUTarget):
         h = C.c_void_p()
         ctx.call("pmcmc_target_logistic", self.N, self.dim, _lib.PREC_F64, _device.ptr(Xd), _device.ptr(y),
                  self.prior_scale, C.byref(h))
+        if self.prior_precision is not None:
+            P = torch.from_numpy(np.ascontiguousarray(self.prior_precision)).to(dev)
+            torch.cuda.current_stream(dev).synchronize()
+            rc = _lib.load().pmcmc_target_logistic_set_prior(h, _device.ptr(P), _lib.PREC_F64)
+            _lib.check(rc, ctx.h, "pmcmc_target_logistic_set_prior")
         return h
 
 
@@ -110,3 +123,33 @@ class GaussianTarget(GPUTarget):
         h = C.c_void_p()
         ctx.call("pmcmc_target_gaussian", self.dim, _lib.PREC_F64, _device.ptr(L), _device.ptr(m), C.byref(h))
         return h
+
+
+def reparametrized(target, L):
+    """The target of theta = L^-1 x for a lower-triangular L with L L' = M:
+    p_theta(theta) = p(L theta) (the same log-density values).  Standard MALA
+    on theta with the tape (xi, u) maps through x = L theta onto MALA with the
+    mass matrix M on x with the same tape (src/MALA/MALA.jl:29-47, 430-438):
+    the proposal x + eps M g + sqrt(2 eps) L xi and the acceptance ratio are
+    identical.  Gaussian targets stay dense Gaussians (precision L' Lam L,
+    mean L^-1 mu); the logistic target gets the design X L and the prior
+    precision L' P L."""
+    L = np.asarray(L, dtype=np.float64)
+    D = target.dim
+    if L.shape != (D, D):
+        raise _lib.DimensionMismatch("cholM must be D x D")
+
+    def solve_l(v):
+        return np.linalg.solve(L, np.asarray(v, dtype=np.float64))
+
+    if isinstance(target, LogisticTarget):
+        P = target.prior_precision if target.prior_precision is not None else np.eye(D) / target.prior_scale**2
+        return LogisticTarget(target.X @ L, target.y, target.prior_scale, prior_precision=L.T @ P @ L)
+    if isinstance(target, GaussianTarget):
+        mu = None if target.mean is None else solve_l(target.mean)
+        return GaussianTarget(L.T @ target.Lam @ L, mu)
+    if isinstance(target, DiagGaussianTarget):
+        prec = np.asarray(target.prec, dtype=np.float64)
+        mu = None if target.mean is None else solve_l(target.mean)
+        return GaussianTarget(L.T @ (prec[:, None] * L), mu)
+    raise _lib.ArgumentError(f"no mass-matrix form for {type(target).__name__}")

@@ -69,7 +69,8 @@ __global__ __launch_bounds__(256) void diag_c_kernel(const T* U, const T* gx, co
     }
 }
 
-// Logistic diag(H_x)[d, t] = -sum_n s_n (1 - s_n) X_nd^2 - 1/prior^2,
+// Logistic diag(H_x)[d, t] = -sum_n s_n (1 - s_n) X_nd^2 - 1/prior^2 (or
+// - P_dd with a dense prior precision P),
 // s_n = sigmoid(X_n . x_t), x_t the shifted iterate.  A workgroup owns 8
 // columns; the design streams through LDS in chunks of 32 rows: first the
 // 32 x 8 linear predictors (one per thread, FP64 sums), then each thread
@@ -77,7 +78,8 @@ __global__ __launch_bounds__(256) void diag_c_kernel(const T* U, const T* gx, co
 template <typename T>
 __global__ __launch_bounds__(256) void glm_hess_diag_kernel(const float* Xrm, const double* Xd64, int64_t N,
                                                             int64_t D, int64_t Dp, const T* x0, const T* S_in,
-                                                            int64_t Tn, double inv_s2, T* Hd, const int* skip) {
+                                                            int64_t Tn, double inv_s2, const double* P64, T* Hd,
+                                                            const int* skip) {
     constexpr int CT = 8, NR = 32, XS = 257;
     __shared__ T xs[CT][256];
     __shared__ T xc[NR][XS];
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(256) void glm_hess_diag_kernel(const float* Xrm, co
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
             const int64_t t = t0 + c;
-            if (t < Tn) Hd[t * D + tid] = (T)(-acc[c] - inv_s2);
+            if (t < Tn) Hd[t * D + tid] = (T)(-acc[c] - (P64 ? P64[(size_t)tid * D + tid] : inv_s2));
         }
 }
 
@@ -180,7 +182,8 @@ int gemm_diag_products(DiagWork* dw, GlmWork* gw, bool kind_logistic, const GlmD
                        kind_logistic ? nullptr : gauss->L64, W, H, skip);
     if (kind_logistic)
         hipLaunchKernelGGL((glm_hess_diag_kernel<T>), dim3((unsigned)((Tn + 7) / 8)), dim3(256), 0, s, glm->Xrm,
-                           glm->Xd64, glm->N, D, glm->Dp, x0, S_in, Tn, 1.0 / (prior_scale * prior_scale), H, skip);
+                           glm->Xd64, glm->N, D, glm->Dp, x0, S_in, Tn, 1.0 / (prior_scale * prior_scale), glm->P64, H,
+                           skip);
     if (hipGetLastError() != hipSuccess) {
         *msg = ":diag products: kernel launch failed";
         return PMCMC_ERR_HIP;

@@ -108,7 +108,33 @@ void glm_data_destroy(GlmData* g) {
     (void)hipFree(g->Xd64);
     (void)hipFree(g->y);
     (void)hipFree(g->y64);
+    (void)hipFree(g->P64);
     *g = GlmData{};
+}
+
+template <typename S>
+__global__ void glm_prior_kernel(const S* P, int64_t n, double* out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (double)P[i];
+}
+
+int glm_data_set_prior(GlmData* g, hipStream_t s, bool src_f64, const void* P, std::string* msg) {
+    const int64_t n = g->D * g->D;
+    if (!g->P64 && hipMalloc(&g->P64, (size_t)n * sizeof(double)) != hipSuccess) {
+        *msg = "logistic target: prior allocation failed";
+        return PMCMC_ERR_ALLOC;
+    }
+    if (src_f64)
+        hipLaunchKernelGGL((glm_prior_kernel<double>), dim3(grid_cap(n)), dim3(256), 0, s, (const double*)P, n,
+                           g->P64);
+    else
+        hipLaunchKernelGGL((glm_prior_kernel<float>), dim3(grid_cap(n)), dim3(256), 0, s, (const float*)P, n,
+                           g->P64);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        *msg = "logistic target: prior copy failed";
+        return PMCMC_ERR_HIP;
+    }
+    return PMCMC_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -476,8 +502,8 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_batched(const GlmPassArgs* 
 template <typename T, typename P>
 __device__ __forceinline__ void glm_reduce_body(const ColSrc<T>& src, const P* part, const double* llpart, int ns,
                                                 int64_t Dp, int64_t llstride, int pass, double inv_s2, double eps,
-                                                const T* Xi, T* gout, T* hout, double* lpout, T* Yout, T* Wout,
-                                                const int* skip, const int64_t blk) {
+                                                const double* Pm, const T* Xi, T* gout, T* hout, double* lpout,
+                                                T* Yout, T* Wout, const int* skip, const int64_t blk) {
     if (skip && *(volatile const int*)skip) return;
     const int lane = threadIdx.x & 63;
     const int64_t t = blk * 4 + (threadIdx.x >> 6);
@@ -486,14 +512,22 @@ __device__ __forceinline__ void glm_reduce_body(const ColSrc<T>& src, const P* p
     if (t >= Tn) return;
     const double sq2e = sqrt(2.0 * eps);
     double nrm = 0.0;
+    // prior term of the gradient / HVP: P v (dense prior) or v / s^2
+    auto prior = [&](int set, int64_t d, double v) -> double {
+        if (!Pm) return v * inv_s2;
+        double a = 0.0;
+        for (int64_t k = 0; k < D; ++k) a += Pm[(size_t)d * D + k] * src.get(set, t, k);
+        return a;
+    };
     for (int64_t d = lane; d < D; d += 64) {
         const double x = src.get(0, t, d);
         double G = 0.0;
         for (int s = 0; s < ns; ++s) G += (double)part[(((size_t)s * (1 + K) + 0) * Tn + t) * Dp + d];
-        const double g = G - x * inv_s2;
+        const double px = prior(0, d, x);
+        const double g = G - px;
         const size_t gi = (size_t)t * D + d;
         gout[gi] = (T)g;
-        nrm += x * x;
+        nrm += x * px;
         double y = 0.0;
         if (pass == 0) {
             y = x + eps * g + sq2e * (double)Xi[gi];
@@ -504,7 +538,7 @@ __device__ __forceinline__ void glm_reduce_body(const ColSrc<T>& src, const P* p
             for (int s = 0; s < ns; ++s)
                 H += (double)part[(((size_t)s * (1 + K) + 1 + k) * Tn + t) * Dp + d];
             const double z = src.get(1 + k, t, d);
-            const double h = -H - z * inv_s2;
+            const double h = -H - prior(1 + k, d, z);
             hout[(size_t)k * D * Tn + gi] = (T)h;
             if (pass == 0) Wout[(size_t)k * D * Tn + gi] = (T)(z + eps * h);
         }
@@ -513,17 +547,17 @@ __device__ __forceinline__ void glm_reduce_body(const ColSrc<T>& src, const P* p
     if (lane == 0) {
         double ll = 0.0;
         for (int s = 0; s < ns; ++s) ll += llpart[(size_t)s * llstride + t];
-        lpout[t] = ll - 0.5 * nrm * inv_s2;
+        lpout[t] = ll - 0.5 * nrm;  // nrm = x' P x (x' x / s^2 for the isotropic prior)
     }
 }
 
 template <typename T, typename P>
 __global__ __launch_bounds__(256) void glm_reduce_kernel(ColSrc<T> src, const P* part, const double* llpart,
                                                          int ns, int64_t Dp, int64_t llstride, int pass,
-                                                         double inv_s2, double eps, const T* Xi, T* gout,
-                                                         T* hout, double* lpout, T* Yout, T* Wout,
-                                                         const int* skip) {
-    glm_reduce_body<T, P>(src, part, llpart, ns, Dp, llstride, pass, inv_s2, eps, Xi, gout, hout, lpout, Yout,
+                                                         double inv_s2, double eps, const double* Pm,
+                                                         const T* Xi, T* gout, T* hout, double* lpout, T* Yout,
+                                                         T* Wout, const int* skip) {
+    glm_reduce_body<T, P>(src, part, llpart, ns, Dp, llstride, pass, inv_s2, eps, Pm, Xi, gout, hout, lpout, Yout,
                           Wout, skip, blockIdx.x);
 }
 
@@ -531,8 +565,8 @@ __global__ __launch_bounds__(256) void glm_reduce_kernel(ColSrc<T> src, const P*
 __global__ __launch_bounds__(256) void glm_reduce_batched(const GlmReduceArgs* __restrict__ args, const int2* __restrict__ items) {
     const int2 w = items[blockIdx.x];
     const GlmReduceArgs& a = args[w.x];
-    glm_reduce_body<float, float>(a.src, a.part, a.llpart, a.ns, a.Dp, a.llstride, a.pass, a.inv_s2, a.eps, a.Xi,
-                                  a.gout, a.hout, a.lpout, a.Yout, a.Wout, a.skip, w.y);
+    glm_reduce_body<float, float>(a.src, a.part, a.llpart, a.ns, a.Dp, a.llstride, a.pass, a.inv_s2, a.eps, a.P,
+                                  a.Xi, a.gout, a.hout, a.lpout, a.Yout, a.Wout, a.skip, w.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -633,7 +667,7 @@ static int glm_pass(const GlmData* g, GlmWork* w, hipStream_t s, const ColSrc<T>
         if (prof) prof->end(pe, s);
         const int pr = prof ? prof->begin(KT_GLM_REDUCE, iter, s) : -1;
         hipLaunchKernelGGL((glm_reduce_kernel<T, float>), dim3(rgrid), dim3(256), 0, s, src, part, llpart, ns,
-                           g->Dp, Tn, pass, inv_s2, eps, Xi, gout, hout, lpout, Yout, Wout, skip);
+                           g->Dp, Tn, pass, inv_s2, eps, g->P64, Xi, gout, hout, lpout, Yout, Wout, skip);
         if (prof) prof->end(pr, s);
     } else {
         // plain path: residual matrices in the part buffer
@@ -660,7 +694,7 @@ static int glm_pass(const GlmData* g, GlmWork* w, hipStream_t s, const ColSrc<T>
         hipLaunchKernelGGL((glm_naive_bwd_kernel<T>), dim3(grid_cap((1 + K) * Tn * src.D)), dim3(256), 0, s, K,
                            src.D, g->Dp, Tn, g->Xd64, g->N, R, llnt, sums, llsum, skip);
         hipLaunchKernelGGL((glm_reduce_kernel<T, double>), dim3(rgrid), dim3(256), 0, s, src, sums, llsum, 1,
-                           g->Dp, Tn, pass, inv_s2, eps, Xi, gout, hout, lpout, Yout, Wout, skip);
+                           g->Dp, Tn, pass, inv_s2, eps, g->P64, Xi, gout, hout, lpout, Yout, Wout, skip);
     }
     if (hipGetLastError() != hipSuccess) {
         *msg = "logistic pass: kernel launch failed";
@@ -900,7 +934,8 @@ int glm_fwd_and_jvp(const GlmData* g, hipStream_t s, int prec, double eps, doubl
 // logp of every column: one workgroup per column, threads over n, in double.
 template <typename T>
 __global__ __launch_bounds__(256) void glm_logp_kernel(const T* S, const double* X64, const double* y64,
-                                                       int64_t N, int64_t D, double inv_s2, double* out) {
+                                                       int64_t N, int64_t D, double inv_s2, const double* Pm,
+                                                       double* out) {
     __shared__ double red[4];
     const int64_t t = blockIdx.x;
     const T* x = S + (size_t)t * D;
@@ -912,8 +947,15 @@ __global__ __launch_bounds__(256) void glm_logp_kernel(const T* S, const double*
         acc += -yv * softplusd(-l) - (1.0 - yv) * softplusd(l);
     }
     double nrm = 0.0;
-    for (int64_t d = threadIdx.x; d < D; d += 256) nrm += (double)x[d] * (double)x[d];
-    acc -= 0.5 * nrm * inv_s2;
+    for (int64_t d = threadIdx.x; d < D; d += 256) {
+        double px = (double)x[d] * inv_s2;
+        if (Pm) {
+            px = 0.0;
+            for (int64_t k = 0; k < D; ++k) px += Pm[(size_t)d * D + k] * (double)x[k];
+        }
+        nrm += (double)x[d] * px;
+    }
+    acc -= 0.5 * nrm;
     for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
@@ -925,10 +967,10 @@ int glm_logdensity(const GlmData* g, hipStream_t s, int prec, double prior_scale
     const double inv_s2 = 1.0 / (prior_scale * prior_scale);
     if (prec == PMCMC_PREC_F64)
         hipLaunchKernelGGL((glm_logp_kernel<double>), dim3((unsigned)T), dim3(256), 0, s, (const double*)S,
-                           g->Xd64, g->y64, g->N, g->D, inv_s2, out);
+                           g->Xd64, g->y64, g->N, g->D, inv_s2, g->P64, out);
     else
         hipLaunchKernelGGL((glm_logp_kernel<float>), dim3((unsigned)T), dim3(256), 0, s, (const float*)S,
-                           g->Xd64, g->y64, g->N, g->D, inv_s2, out);
+                           g->Xd64, g->y64, g->N, g->D, inv_s2, g->P64, out);
     if (hipGetLastError() != hipSuccess) {
         *msg = "logistic log-density kernel failed";
         return PMCMC_ERR_HIP;

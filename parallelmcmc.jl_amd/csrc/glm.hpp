@@ -21,6 +21,9 @@ struct GlmData {
     double* Xd64 = nullptr; // N x D column-major copy for the FP64 path
     float* y = nullptr;
     double* y64 = nullptr;
+    // Gaussian prior N(0, P^-1) with a dense precision P (D x D, symmetric,
+    // FP64), or null: the isotropic prior N(0, prior_scale^2 I)
+    double* P64 = nullptr;
 };
 
 // Column sources of a GEMM pass.
@@ -87,6 +90,7 @@ struct GlmReduceArgs {
     int64_t Dp, llstride;
     int pass;
     double inv_s2, eps;
+    const double* P;  // dense prior precision or null (isotropic inv_s2)
     const float* Xi;
     float* gout;
     float* hout;
@@ -108,6 +112,8 @@ void launch_glm_reduce_batched(const GlmReduceArgs* args, const int2* items, int
 int glm_data_create(GlmData* g, hipStream_t s, int64_t N, int64_t D, bool src_f64, const void* Xd,
                     const void* y, std::string* msg);
 void glm_data_destroy(GlmData* g);
+// Dense prior precision P (D x D, symmetric) in place of the isotropic prior.
+int glm_data_set_prior(GlmData* g, hipStream_t s, bool src_f64, const void* P, std::string* msg);
 int glm_work_ensure(GlmWork* w, const GlmData* g, hipStream_t s, int64_t D, int64_t T, int probes,
                     bool f64, std::string* msg);
 void glm_work_destroy(GlmWork* w);

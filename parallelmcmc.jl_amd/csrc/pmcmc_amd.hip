@@ -1016,6 +1016,19 @@ extern "C" int pmcmc_target_logistic(pmcmc_ctx* c, int64_t N, int64_t D, int dty
     return PMCMC_OK;
 }
 
+extern "C" int pmcmc_target_logistic_set_prior(pmcmc_target* t, const void* P, int dtype_prec) {
+    if (!t || !t->ctx) return PMCMC_ERR_ARGUMENT;
+    pmcmc_ctx* c = t->ctx;
+    if (t->kind != TGT_LOGISTIC) return fail(c, PMCMC_ERR_ARGUMENT, "prior precision: not a logistic target");
+    if (!P) return fail(c, PMCMC_ERR_ARGUMENT, "prior precision: null matrix");
+    if (!valid_prec(dtype_prec)) return fail(c, PMCMC_ERR_ARGUMENT, "bad precision %d", dtype_prec);
+    HIPCHK(c, hipSetDevice(c->device));
+    std::string msg;
+    const int rc = glm_data_set_prior(&t->glm, c->stream, dtype_prec == PMCMC_PREC_F64, P, &msg);
+    if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+    return PMCMC_OK;
+}
+
 extern "C" int pmcmc_target_gaussian(pmcmc_ctx* c, int64_t D, int dtype_prec, const void* Lambda,
                                      const void* mean, pmcmc_target** out) {
     if (!c || !out) return PMCMC_ERR_ARGUMENT;
@@ -1342,6 +1355,7 @@ static int deer_solve_t(pmcmc_ctx* c, pmcmc_deer_ws* w, const pmcmc_target* tgt,
             rtg.X64 = tgt->glm.Xd64;
             rtg.y64 = tgt->glm.y64;
             rtg.inv_s2 = 1.0 / (tgt->prior_scale * tgt->prior_scale);
+            rtg.P64 = tgt->glm.P64;
         } else {
             rtg.kind = 1;
             rtg.L64 = tgt->gauss.L64;
@@ -2446,6 +2460,7 @@ extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_des
             h.rtg.X64 = d.target->glm.Xd64;
             h.rtg.y64 = d.target->glm.y64;
             h.rtg.inv_s2 = 1.0 / (d.target->prior_scale * d.target->prior_scale);
+            h.rtg.P64 = d.target->glm.P64;
             h.band = std::max(1e-4, 2.5e-8 * (double)d.target->glm.N);
         } else {
             h.rtg.kind = 1;
@@ -2671,6 +2686,7 @@ static void batch_tables(pmcmc_batch* b, BatchTables* bt) {
                 gr.llstride = L;
                 gr.pass = v < 2 ? 0 : 1;
                 gr.inv_s2 = inv_s2;
+                gr.P = h.tgt->glm.P64;
                 gr.eps = h.eps;
                 gr.gout = v < 2 ? h.gx : h.gy;
                 gr.hout = h.hw;

@@ -312,16 +312,27 @@ __device__ void refine_finish_body(const RefineTarget& tg, const RefineArgs& a, 
             const size_t jd = (size_t)j * D + d;
             if (phase == 0) {
                 if (tg.kind == 0) {
-                    g -= x * tg.inv_s2;
-                    nrm += x * x;
+                    // prior: P x (dense precision) or x / s^2
+                    double px = x * tg.inv_s2;
+                    if (tg.P64) {
+                        px = 0.0;
+                        for (int64_t k = 0; k < D; ++k) px += tg.P64[(size_t)d * D + k] * state_at(a, t, k, D);
+                    }
+                    g -= px;
+                    nrm += x * px;
                 }
                 w.g0[jd] = g;
                 w.y[jd] = x + eps * g + sq2e * (double)a.Xi[(size_t)t * D + d];
             } else {
                 const double y = w.y[jd];
                 if (tg.kind == 0) {
-                    g -= y * tg.inv_s2;
-                    nrm += y * y;
+                    double py = y * tg.inv_s2;
+                    if (tg.P64) {
+                        py = 0.0;
+                        for (int64_t k = 0; k < D; ++k) py += tg.P64[(size_t)d * D + k] * w.y[(size_t)j * D + k];
+                    }
+                    g -= py;
+                    nrm += y * py;
                 }
                 const double e1 = y - x - eps * w.g0[jd];
                 const double e2 = x - y - eps * g;
@@ -329,7 +340,7 @@ __device__ void refine_finish_body(const RefineTarget& tg, const RefineArgs& a, 
                 qxy += e2 * e2;
             }
         }
-        lp -= 0.5 * tg.inv_s2 * block_sum(nrm, red);
+        lp -= 0.5 * block_sum(nrm, red);  // x' P x (x' x / s^2 isotropic); 0 for the Gaussian
         if (phase == 0) {
             if (tid == 0) w.lp0[j] = lp;
         } else {

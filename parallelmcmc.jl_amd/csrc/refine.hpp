@@ -16,6 +16,7 @@ struct RefineTarget {
     const double* X64;
     const double* y64;
     double inv_s2;
+    const double* P64;  // dense prior precision (D x D) or null
     const double* L64;
     const double* mu64;
 };

@@ -38,3 +38,19 @@ def test_dot_and_dotsum_shapes():
         pmcmc_dot(torch.zeros(3, 1, dtype=torch.float64), torch.zeros(3, 1, dtype=torch.float64))
     with pytest.raises(DimensionMismatch):
         pmcmc_dotsum(torch.zeros(3, 2, dtype=torch.float64), torch.zeros(2, 3, dtype=torch.float64))
+
+
+def test_sampler_rejects_bad_mass_factor():
+    """cholM must be a square lower-triangular factor with a positive diagonal."""
+    import numpy as np
+    import pytest
+
+    from ParallelMCMC import ArgumentError, ParallelMALASampler
+
+    with pytest.raises(ArgumentError):
+        ParallelMALASampler(0.1, cholM=np.ones((3, 3)))
+    with pytest.raises(ArgumentError):
+        ParallelMALASampler(0.1, cholM=-np.eye(3))
+    with pytest.raises(ArgumentError):
+        ParallelMALASampler(0.1, cholM=np.ones((3, 2)))
+    ParallelMALASampler(0.1, cholM=np.eye(3))
