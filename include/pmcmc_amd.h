@@ -220,6 +220,18 @@ int pmcmc_mala_fwd_and_jvp(pmcmc_ctx* ctx, const pmcmc_target* tgt, int prec, do
                            const void* X, const void* Xi, const void* U, const void* Z, void* FT,
                            void* Jt, uint8_t* accepted, double* logalpha, int64_t D, int64_t T);
 
+/* Sequential MALA on the device for N chains of one target, K steps: step k
+ * is mala_step_batched! (src/MALA/MALA.jl:487-528) on the N columns of the
+ * previous step's states (X0 first, D x N): proposal, both log-densities and
+ * gradients, accept iff log u < log alpha.  Xi and S_out are D x (N K)
+ * column-major with step k's N columns in block k, U and accepted have N K
+ * entries likewise.  The GPU comparator (and warm-up path) of DEER; the
+ * multi-step driver has no reference counterpart (the reference loops over
+ * the one-step call).  Synchronises the context stream.                    */
+int pmcmc_mala_run_batched(pmcmc_ctx* ctx, const pmcmc_target* tgt, int prec, double eps, const void* X0,
+                           const void* Xi, const void* U, void* S_out, uint8_t* accepted, int64_t D, int64_t N,
+                           int64_t K);
+
 /* ---- DEER Newton solve ----------------------------------------------------
  * Replaces DEER.solve (src/DEER/DEER.jl:506-595) on the taped MALA
  * recursion built by _build_mala_deer_rec (src/interface.jl:360-473):

@@ -70,3 +70,48 @@ def logdensity_batch(model, S, precision=None, device=None):
     ctx = _device.context(dev.index)
     ctx.call("pmcmc_target_logdensity_batch", tgt.handle(ctx), prec, _device.ptr(S_), _device.ptr(out), T)
     return out
+
+
+def mala_run_batched(model, X0, eps, Xi, U, precision=None, device=None):
+    """Sequential MALA on the device for N chains of one model, K steps
+    (each step is mala_step_batched!, src/MALA/MALA.jl:487-528, on the N
+    columns).  X0: (D, N); Xi: (D, N, K); U: (N, K).  Returns (S, accepted):
+    S (D, N, K) device tensor, S[:, j, k] the state of chain j after step k;
+    accepted (N, K) uint8."""
+    tgt = _target_of(model)
+    prec = _precision_of(eps, precision)
+    X0 = np.asarray(X0) if not isinstance(X0, torch.Tensor) else X0
+    if len(X0.shape) != 2:
+        raise DimensionMismatch("X0 must be D x N")
+    D, N = X0.shape
+    if D != tgt.dim:
+        raise DimensionMismatch("X0 rows must equal the model dimension")
+    if len(Xi.shape) != 3 or tuple(Xi.shape[:2]) != (D, N):
+        raise DimensionMismatch("Xi must be D x N x K")
+    K = Xi.shape[2]
+    if tuple(U.shape) != (N, K):
+        raise DimensionMismatch("U must be N x K")
+    dev = torch.device("cuda", device if device is not None else torch.cuda.current_device())
+    dt = _device.torch_dtype(prec)
+    X0_ = _device.to_colmajor(X0, dt, dev)
+    # step k's N columns contiguous: (D, N, K) column-major is exactly that
+    Xi_ = torch.as_tensor(np.asarray(Xi) if not isinstance(Xi, torch.Tensor) else Xi).to(dev, dt)
+    Xi_ = Xi_.permute(2, 1, 0).contiguous()  # (K, N, D): element (d, j, k) at (k N + j) D + d
+    U_ = torch.as_tensor(np.asarray(U) if not isinstance(U, torch.Tensor) else U).to(dev, dt)
+    U_ = U_.t().contiguous()  # (K, N)
+    S = torch.empty((K, N, D), dtype=dt, device=dev)
+    acc = torch.empty((K, N), dtype=torch.uint8, device=dev)
+    ctx = _device.context(dev.index)
+    ctx.call("pmcmc_mala_run_batched", tgt.handle(ctx), prec, float(eps), _device.ptr(X0_), _device.ptr(Xi_),
+             _device.ptr(U_), _device.ptr(S), _device.ptr(acc), D, N, K)
+    return S.permute(2, 1, 0), acc.t()
+
+
+def mala_step_batched(model, X, eps, Xi, U, precision=None, device=None):
+    """One step of mala_step_batched! (src/MALA/MALA.jl:487-528) for the N
+    columns of X: returns (X_next, accepted)."""
+    D, N = X.shape
+    S, acc = mala_run_batched(model, X, eps, np.asarray(Xi).reshape(D, N, 1) if not isinstance(Xi, torch.Tensor)
+                              else Xi.reshape(D, N, 1), np.asarray(U).reshape(N, 1) if not isinstance(U, torch.Tensor)
+                              else U.reshape(N, 1), precision, device)
+    return S[:, :, 0], acc[:, 0]

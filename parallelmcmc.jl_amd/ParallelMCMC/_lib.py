@@ -138,6 +138,7 @@ _SIGS = {
     "pmcmc_target_destroy": ([_P], C.c_int),
     "pmcmc_target_dim": ([_P], _I64),
     "pmcmc_mala_fwd_and_jvp": ([_P, _P, C.c_int, C.c_double, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64], C.c_int),
+    "pmcmc_mala_run_batched": ([_P, _P, C.c_int, C.c_double, _P, _P, _P, _P, _P, _I64, _I64, _I64], C.c_int),
     "pmcmc_deer_ws_create": ([_P, _I64, _I64, C.c_int, C.POINTER(_P)], C.c_int),
     "pmcmc_deer_ws_destroy": ([_P], C.c_int),
     "pmcmc_deer_solve": ([_P, _P, _P, C.POINTER(DeerParams), _P, _P, _P, _P, _P, _P, C.POINTER(DeerInfo)], C.c_int),

@@ -800,23 +800,17 @@ template int gauss_hvp_products<double>(const GaussData*, hipStream_t, const dou
                                         std::string*);
 
 template <typename T>
-static int gauss_fwd_jvp_t(const GaussData* g, hipStream_t s, double eps, const T* X, const T* Xi, const T* U,
-                           const T* Z, T* FT, T* Jt, uint8_t* acc, double* la, int64_t D, int64_t Tn, bool jbf,
-                           std::string* msg) {
-    GlmWork w{};
-    GlmData dummy{};
-    dummy.N = 1;
-    dummy.Dp = g->Dp;
-    int rc = glm_work_ensure(&w, &dummy, s, D, Tn, 1, sizeof(T) == 8, msg);
-    if (rc != PMCMC_OK) return rc;
+static int gauss_fwd_jvp_w(const GaussData* g, GlmWork& w, hipStream_t s, double eps, const T* X, const T* Xi,
+                           const T* U, const T* Z, T* FT, T* Jt, uint8_t* acc, double* la, int64_t D, int64_t Tn,
+                           bool jbf, std::string* msg) {
     ColSrc<T> a{};
     a.V = X;
     a.Z = Z;
     a.K = 1;
     a.D = D;
     a.Tn = Tn;
-    rc = gauss_pass<T>(g, s, a, 0, eps, Xi, (T*)w.gx, (T*)w.hw, w.lpx, (T*)w.Y, (T*)w.W, nullptr, nullptr, 0,
-                       jbf, msg);
+    int rc = gauss_pass<T>(g, s, a, 0, eps, Xi, (T*)w.gx, (T*)w.hw, w.lpx, (T*)w.Y, (T*)w.W, nullptr, nullptr, 0,
+                           jbf, msg);
     if (rc == PMCMC_OK) {
         ColSrc<T> b = a;
         b.V = (const T*)w.Y;
@@ -831,9 +825,54 @@ static int gauss_fwd_jvp_t(const GaussData* g, hipStream_t s, double eps, const 
             rc = PMCMC_ERR_HIP;
         }
     }
+    return rc;
+}
+
+static int gauss_work(GlmWork* w, const GaussData* g, hipStream_t s, int64_t D, int64_t Tn, bool f64,
+                      std::string* msg) {
+    GlmData dummy{};
+    dummy.N = 1;
+    dummy.Dp = g->Dp;
+    return glm_work_ensure(w, &dummy, s, D, Tn, 1, f64, msg);
+}
+
+template <typename T>
+static int gauss_fwd_jvp_t(const GaussData* g, hipStream_t s, double eps, const T* X, const T* Xi, const T* U,
+                           const T* Z, T* FT, T* Jt, uint8_t* acc, double* la, int64_t D, int64_t Tn, bool jbf,
+                           std::string* msg) {
+    GlmWork w{};
+    int rc = gauss_work(&w, g, s, D, Tn, sizeof(T) == 8, msg);
+    if (rc == PMCMC_OK) rc = gauss_fwd_jvp_w<T>(g, w, s, eps, X, Xi, U, Z, FT, Jt, acc, la, D, Tn, jbf, msg);
     (void)hipStreamSynchronize(s);
     glm_work_destroy(&w);
     return rc;
+}
+
+// K sequential MALA steps of N chains (as glm_mala_run)
+template <typename T>
+static int gauss_mala_run_t(const GaussData* g, hipStream_t s, double eps, const T* X0, const T* Xi, const T* U,
+                            int64_t K, int64_t D, int64_t N, T* S_out, uint8_t* acc, const T* Z, T* Jw, bool jbf,
+                            std::string* msg) {
+    GlmWork w{};
+    int rc = gauss_work(&w, g, s, D, N, sizeof(T) == 8, msg);
+    for (int64_t k = 0; k < K && rc == PMCMC_OK; ++k) {
+        const T* X = k == 0 ? X0 : S_out + (size_t)(k - 1) * D * N;
+        rc = gauss_fwd_jvp_w<T>(g, w, s, eps, X, Xi + (size_t)k * D * N, U + (size_t)k * N, Z,
+                                S_out + (size_t)k * D * N, Jw, acc + (size_t)k * N, nullptr, D, N, jbf, msg);
+    }
+    (void)hipStreamSynchronize(s);
+    glm_work_destroy(&w);
+    return rc;
+}
+
+int gauss_mala_run(const GaussData* g, hipStream_t s, int prec, double eps, const void* X0, const void* Xi,
+                   const void* U, int64_t K, int64_t D, int64_t N, void* S_out, uint8_t* acc, const void* Z, void* Jw,
+                   std::string* msg) {
+    if (prec == PMCMC_PREC_F64)
+        return gauss_mala_run_t<double>(g, s, eps, (const double*)X0, (const double*)Xi, (const double*)U, K, D, N,
+                                        (double*)S_out, acc, (const double*)Z, (double*)Jw, false, msg);
+    return gauss_mala_run_t<float>(g, s, eps, (const float*)X0, (const float*)Xi, (const float*)U, K, D, N,
+                                   (float*)S_out, acc, (const float*)Z, (float*)Jw, prec == PMCMC_PREC_BF16_JAC, msg);
 }
 
 int gauss_fwd_and_jvp(const GaussData* g, hipStream_t s, int prec, double eps, const void* X,

@@ -78,6 +78,11 @@ int gauss_hvp_products(const GaussData* g, hipStream_t s, const T* V, const T* x
                        double* lpscr, T* hout, int64_t D, int64_t Tn, const int* skip, Prof* prof, int iter, bool jbf,
                        std::string* msg);
 
+// K sequential MALA steps of N chains (as glm_mala_run).
+int gauss_mala_run(const GaussData* g, hipStream_t s, int prec, double eps, const void* X0, const void* Xi,
+                   const void* U, int64_t K, int64_t D, int64_t N, void* S_out, uint8_t* acc, const void* Z, void* Jw,
+                   std::string* msg);
+
 int gauss_fwd_and_jvp(const GaussData* g, hipStream_t s, int prec, double eps, const void* X,
                       const void* Xi, const void* U, const void* Z, void* FT, void* Jt,
                       uint8_t* accepted, double* logalpha, int64_t D, int64_t T, std::string* msg);

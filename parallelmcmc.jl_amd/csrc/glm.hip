@@ -883,12 +883,10 @@ template void launch_fwd_jvp_finish<double>(hipStream_t, const double*, const do
                                             uint8_t*, double*);
 
 template <typename T>
-static int fwd_jvp_t(const GlmData* g, hipStream_t s, double eps, double prior_scale, const T* X,
-                     const T* Xi, const T* U, const T* Z, T* FT, T* Jt, uint8_t* acc, double* la,
-                     int64_t D, int64_t Tn, bool jbf, std::string* msg) {
-    GlmWork w{};
-    int rc = glm_work_ensure(&w, g, s, D, Tn, 1, sizeof(T) == 8, msg);
-    if (rc != PMCMC_OK) return rc;
+static int fwd_jvp_w(const GlmData* g, GlmWork& w, hipStream_t s, double eps, double prior_scale, const T* X,
+                     const T* Xi, const T* U, const T* Z, T* FT, T* Jt, uint8_t* acc, double* la, int64_t D,
+                     int64_t Tn, bool jbf, std::string* msg) {
+    int rc = PMCMC_OK;
     ColSrc<T> a{};
     a.V = X;
     a.x0 = nullptr;
@@ -914,9 +912,49 @@ static int fwd_jvp_t(const GlmData* g, hipStream_t s, double eps, double prior_s
             rc = PMCMC_ERR_HIP;
         }
     }
+    return rc;
+}
+
+template <typename T>
+static int fwd_jvp_t(const GlmData* g, hipStream_t s, double eps, double prior_scale, const T* X,
+                     const T* Xi, const T* U, const T* Z, T* FT, T* Jt, uint8_t* acc, double* la,
+                     int64_t D, int64_t Tn, bool jbf, std::string* msg) {
+    GlmWork w{};
+    int rc = glm_work_ensure(&w, g, s, D, Tn, 1, sizeof(T) == 8, msg);
+    if (rc == PMCMC_OK) rc = fwd_jvp_w<T>(g, w, s, eps, prior_scale, X, Xi, U, Z, FT, Jt, acc, la, D, Tn, jbf, msg);
     hipStreamSynchronize(s);
     glm_work_destroy(&w);
     return rc;
+}
+
+// K sequential MALA steps of N chains (columns): step k reads the states of
+// step k-1 (X0 first) and writes column block k of S_out (D x N each); one
+// work allocation for all steps, no synchronisation between them.  Z and Jw:
+// D x N scratch (Z zero: the JVP half of the passes is not used).
+template <typename T>
+static int mala_run_t(const GlmData* g, hipStream_t s, double eps, double prior_scale, const T* X0, const T* Xi,
+                      const T* U, int64_t K, int64_t D, int64_t N, T* S_out, uint8_t* acc, const T* Z, T* Jw,
+                      bool jbf, std::string* msg) {
+    GlmWork w{};
+    int rc = glm_work_ensure(&w, g, s, D, N, 1, sizeof(T) == 8, msg);
+    for (int64_t k = 0; k < K && rc == PMCMC_OK; ++k) {
+        const T* X = k == 0 ? X0 : S_out + (size_t)(k - 1) * D * N;
+        rc = fwd_jvp_w<T>(g, w, s, eps, prior_scale, X, Xi + (size_t)k * D * N, U + (size_t)k * N, Z,
+                          S_out + (size_t)k * D * N, Jw, acc + (size_t)k * N, nullptr, D, N, jbf, msg);
+    }
+    hipStreamSynchronize(s);
+    glm_work_destroy(&w);
+    return rc;
+}
+
+int glm_mala_run(const GlmData* g, hipStream_t s, int prec, double eps, double prior_scale, const void* X0,
+                 const void* Xi, const void* U, int64_t K, int64_t D, int64_t N, void* S_out, uint8_t* acc,
+                 const void* Z, void* Jw, std::string* msg) {
+    if (prec == PMCMC_PREC_F64)
+        return mala_run_t<double>(g, s, eps, prior_scale, (const double*)X0, (const double*)Xi, (const double*)U, K,
+                                  D, N, (double*)S_out, acc, (const double*)Z, (double*)Jw, false, msg);
+    return mala_run_t<float>(g, s, eps, prior_scale, (const float*)X0, (const float*)Xi, (const float*)U, K, D, N,
+                             (float*)S_out, acc, (const float*)Z, (float*)Jw, prec == PMCMC_PREC_BF16_JAC, msg);
 }
 
 int glm_fwd_and_jvp(const GlmData* g, hipStream_t s, int prec, double eps, double prior_scale,

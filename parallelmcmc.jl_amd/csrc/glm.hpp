@@ -114,6 +114,12 @@ int glm_data_create(GlmData* g, hipStream_t s, int64_t N, int64_t D, bool src_f6
 void glm_data_destroy(GlmData* g);
 // Dense prior precision P (D x D, symmetric) in place of the isotropic prior.
 int glm_data_set_prior(GlmData* g, hipStream_t s, bool src_f64, const void* P, std::string* msg);
+// K sequential MALA steps of N chains (src/MALA/MALA.jl:487-528 per step):
+// Xi, S_out D x (N K), U and acc N K, step k in column block k; Z, Jw D x N
+// scratch with Z zero.  Synchronises the stream.
+int glm_mala_run(const GlmData* g, hipStream_t s, int prec, double eps, double prior_scale, const void* X0,
+                 const void* Xi, const void* U, int64_t K, int64_t D, int64_t N, void* S_out, uint8_t* acc,
+                 const void* Z, void* Jw, std::string* msg);
 int glm_work_ensure(GlmWork* w, const GlmData* g, hipStream_t s, int64_t D, int64_t T, int probes,
                     bool f64, std::string* msg);
 void glm_work_destroy(GlmWork* w);

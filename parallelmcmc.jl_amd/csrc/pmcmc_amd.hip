@@ -1117,6 +1117,65 @@ extern "C" int pmcmc_mala_fwd_and_jvp(pmcmc_ctx* c, const pmcmc_target* tgt, int
     return PMCMC_OK;
 }
 
+extern "C" int pmcmc_mala_run_batched(pmcmc_ctx* c, const pmcmc_target* tgt, int prec, double eps,
+                                      const void* X0, const void* Xi, const void* U, void* S_out,
+                                      uint8_t* accepted, int64_t D, int64_t N, int64_t K) {
+    if (!c || !tgt) return fail(c, PMCMC_ERR_ARGUMENT, "null context/target");
+    if (D != tgt->D) return fail(c, PMCMC_ERR_DIMENSION, "X0 has %lld rows but the target has dim %lld", (long long)D, (long long)tgt->D);
+    if (N < 0 || K < 0) return fail(c, PMCMC_ERR_DIMENSION, "N and K must be >= 0");
+    if (!valid_prec(prec)) return fail(c, PMCMC_ERR_ARGUMENT, "bad precision %d", prec);
+    if (!(eps > 0)) return fail(c, PMCMC_ERR_ARGUMENT, "eps must be > 0");
+    if (N == 0 || K == 0) return PMCMC_OK;
+    if (!X0 || !Xi || !U || !S_out || !accepted) return fail(c, PMCMC_ERR_ARGUMENT, "null buffer");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t esz = prec == PMCMC_PREC_F64 ? 8 : 4;
+    void* Z = nullptr;
+    void* Jw = nullptr;
+    if (hipMalloc(&Z, (size_t)D * N * esz) != hipSuccess || hipMalloc(&Jw, (size_t)D * N * esz) != hipSuccess) {
+        (void)hipFree(Z);
+        return fail(c, PMCMC_ERR_ALLOC, "MALA run: scratch allocation failed");
+    }
+    int rc = PMCMC_OK;
+    std::string msg;
+    if (hipMemsetAsync(Z, 0, (size_t)D * N * esz, c->stream) != hipSuccess) {
+        rc = PMCMC_ERR_HIP;
+        msg = "MALA run: zero fill failed";
+    } else if (tgt->kind == TGT_DIAG) {
+        const int G = group_size(D);
+        const int grid = grid_for(N * G);
+        for (int64_t k = 0; k < K; ++k) {
+            const size_t o = (size_t)k * D * N;
+            if (prec == PMCMC_PREC_F64) {
+                const double* X = k == 0 ? (const double*)X0 : (const double*)S_out + o - (size_t)D * N;
+                hipLaunchKernelGGL((fwdjvp_diag_kernel<double>), dim3(grid), dim3(256), 0, c->stream, X,
+                                   (const double*)Xi + o, (const double*)U + (size_t)k * N, (const double*)Z,
+                                   tgt->prec_d, tgt->mean_d, (double*)S_out + o, (double*)Jw, accepted + (size_t)k * N,
+                                   nullptr, eps, D, N, G);
+            } else {
+                const float* X = k == 0 ? (const float*)X0 : (const float*)S_out + o - (size_t)D * N;
+                hipLaunchKernelGGL((fwdjvp_diag_kernel<float>), dim3(grid), dim3(256), 0, c->stream, X,
+                                   (const float*)Xi + o, (const float*)U + (size_t)k * N, (const float*)Z,
+                                   tgt->prec_f, tgt->mean_f, (float*)S_out + o, (float*)Jw, accepted + (size_t)k * N,
+                                   nullptr, eps, D, N, G);
+            }
+        }
+        if (hipGetLastError() != hipSuccess) {
+            rc = PMCMC_ERR_HIP;
+            msg = "MALA run: kernel launch failed";
+        }
+    } else if (tgt->kind == TGT_GAUSS) {
+        rc = gauss_mala_run(&tgt->gauss, c->stream, prec, eps, X0, Xi, U, K, D, N, S_out, accepted, Z, Jw, &msg);
+    } else {
+        rc = glm_mala_run(&tgt->glm, c->stream, prec, eps, tgt->prior_scale, X0, Xi, U, K, D, N, S_out, accepted, Z,
+                          Jw, &msg);
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(Z);
+    (void)hipFree(Jw);
+    if (rc != PMCMC_OK) return fail(c, rc, "%s", msg.c_str());
+    return PMCMC_OK;
+}
+
 // ---------------------------------------------------------------------------
 // owned matmul and reductions (src/ParallelMCMC.jl:23-25)
 // ---------------------------------------------------------------------------
