@@ -74,13 +74,32 @@ __global__ void glm_prep_kernel(const Tin* Xcm, const Tin* y, int64_t N, int64_t
     }
 }
 
+// BF16 copies of the FP32 design (row-major Np x Dq and transposed Dp x Np),
+// rounded to nearest even from the FP32 values, zero padded.
+__global__ void glm_prep_bf16_kernel(const float* Xrm, int64_t Np, int64_t Dp, int64_t Dq, uint16_t* Xb,
+                                     uint16_t* Xtb) {
+    const int64_t n_all = Np * Dq;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_all;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t n = i / Dq, d = i % Dq;
+        const __bf16 v = (__bf16)(d < Dp ? Xrm[n * Dp + d] : 0.f);
+        const uint16_t b = __builtin_bit_cast(uint16_t, v);
+        Xb[i] = b;
+        if (d < Dp) Xtb[d * Np + n] = b;
+    }
+}
+
 int glm_data_create(GlmData* g, hipStream_t s, int64_t N, int64_t D, bool src_f64, const void* Xd,
                     const void* y, std::string* msg) {
     g->N = N;
     g->D = D;
     g->Dp = round_up(D, 16);
+    g->Dq = round_up(D, 32);
     const int64_t Np = round_up(N, 32);
+    g->Np = Np;
     bool ok = hipMalloc(&g->Xrm, Np * g->Dp * sizeof(float)) == hipSuccess &&
+              hipMalloc(&g->Xb, Np * g->Dq * sizeof(uint16_t)) == hipSuccess &&
+              hipMalloc(&g->Xtb, g->Dp * Np * sizeof(uint16_t)) == hipSuccess &&
               hipMalloc(&g->y, Np * sizeof(float)) == hipSuccess &&
               hipMalloc(&g->Xd64, N * D * sizeof(double)) == hipSuccess &&
               hipMalloc(&g->y64, N * sizeof(double)) == hipSuccess;
@@ -95,6 +114,8 @@ int glm_data_create(GlmData* g, hipStream_t s, int64_t N, int64_t D, bool src_f6
     else
         hipLaunchKernelGGL((glm_prep_kernel<float>), dim3(grid), dim3(256), 0, s, (const float*)Xd,
                            (const float*)y, N, D, Np, g->Dp, g->Xrm, g->y, g->Xd64, g->y64);
+    hipLaunchKernelGGL(glm_prep_bf16_kernel, dim3(grid_cap(Np * g->Dq)), dim3(256), 0, s, g->Xrm, Np, g->Dp,
+                       g->Dq, g->Xb, g->Xtb);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
         *msg = "logistic target: preparation kernel failed";
         return PMCMC_ERR_HIP;
@@ -104,6 +125,8 @@ int glm_data_create(GlmData* g, hipStream_t s, int64_t N, int64_t D, bool src_f6
 
 void glm_data_destroy(GlmData* g) {
     (void)hipFree(g->Xrm);
+    (void)hipFree(g->Xb);
+    (void)hipFree(g->Xtb);
     (void)hipFree(g->Xcm);
     (void)hipFree(g->Xd64);
     (void)hipFree(g->y);
@@ -213,18 +236,19 @@ void glm_work_destroy(GlmWork* w) {
 // FP32 MFMA fused pass
 // ---------------------------------------------------------------------------
 // grid: (ceil(T/CB), NS); block 256 = 4 waves.
-// LDS: Vs[NC][Dp+2] column block, Xs[NB][Dp+2] design chunk, Rs[NB][NC+4]
+// LDS: Vs[NC][Dp+4] column block, Xs[NB][Dp+4] design chunk, Rs[NB][NC+4]
 // (53 KB at D = 100: three workgroups per CU).  The next design chunk is
 // fetched into registers (float4) while the current one is multiplied, then
 // stored over it once both GEMMs are done with it: three barriers per chunk,
-// global latency off the critical path.
-template <int DPT, int K, bool JBF>
+// global latency off the critical path.  Wave w owns column tile w (and
+// w + 4, ...) for every d tile.
+template <int DPT, int K>
 __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const float* Xrm, const float* y32,
                                               int64_t N, int64_t rows_per_split, float* part, double* llpart,
                                               const int* skip, const int64_t cb, const int split) {
     constexpr int Dp = DPT * 16;
     constexpr int NC = CB * (1 + K);
-    constexpr int SV = Dp + 4;  // multiples of 4 floats: 16-byte rows for the BF16 operand reads
+    constexpr int SV = Dp + 4;
     constexpr int SX = Dp + 4;
     constexpr int SR = NC + 4;
     constexpr int NCT = NC / 16;             // column tiles
@@ -286,23 +310,11 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
     if (nchunk > 0) stash();
     __syncthreads();
 
-    // FP32 mode: wave w owns column tile w for every d tile.
-    // BF16-Jacobian mode: wave w owns d tiles w, w+4, ... for every column
-    // tile (x tiles on FP32 MFMA, probe tiles on BF16 MFMA).
-    constexpr int DPW = (DPT + 3) / 4;
-    v4f acc[JBF ? 1 : CT_PER_WAVE][JBF ? 1 : DPT];
-    v4f accx[JBF ? DPW : 1][2], accp[JBF ? DPW : 1][2];
-    if constexpr (!JBF) {
+    v4f acc[CT_PER_WAVE][DPT];
 #pragma unroll
-        for (int c = 0; c < CT_PER_WAVE; ++c)
+    for (int c = 0; c < CT_PER_WAVE; ++c)
 #pragma unroll
-            for (int dt = 0; dt < DPT; ++dt) acc[c][dt] = (v4f){0.f, 0.f, 0.f, 0.f};
-    } else {
-#pragma unroll
-        for (int m = 0; m < DPW; ++m)
-#pragma unroll
-            for (int c = 0; c < 2; ++c) accx[m][c] = accp[m][c] = (v4f){0.f, 0.f, 0.f, 0.f};
-    }
+        for (int dt = 0; dt < DPT; ++dt) acc[c][dt] = (v4f){0.f, 0.f, 0.f, 0.f};
     double ll = 0.0;
 
     const int rt = wave >> 1;  // GEMM-A row tile
@@ -311,44 +323,18 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
     for (int c = 0; c < nchunk; ++c) {
         const int64_t n0 = n_begin + (int64_t)c * NB;
         if (c + 1 < nchunk) fetch(c + 1);
-        const float* X_ = Xs;
         // GEMM-A: logits of the x tile and the matching probe tiles
         v4f cl[1 + K];
 #pragma unroll
         for (int s = 0; s <= K; ++s) cl[s] = (v4f){0.f, 0.f, 0.f, 0.f};
-        constexpr int SA = JBF ? 0 : K;  // last set on FP32 MFMA
 #pragma unroll 7
         for (int ks = 0; ks < Dp / 4; ++ks) {
-            const float av = X_[(16 * rt + li) * SX + 4 * ks + lk];
+            const float av = Xs[(16 * rt + li) * SX + 4 * ks + lk];
 #pragma unroll
-            for (int s = 0; s <= SA; ++s) {
+            for (int s = 0; s <= K; ++s) {
                 const int col = s * CB + 16 * cx + li;
                 const float bv = Vs[col * SV + 4 * ks + lk];
                 cl[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, cl[s], 0, 0, 0);
-            }
-        }
-        if constexpr (JBF) {
-            // probe logits q = Xd z on BF16 MFMA (K = 32 per step), operands
-            // rounded from the FP32 tiles; lane (li, lk) holds k = 8 lk .. 8 lk + 7
-#pragma unroll
-            for (int ks = 0; ks < (Dp + 31) / 32; ++ks) {
-                v8bf av, bv;
-                const int col = CB + 16 * cx + li;
-                const int d0 = 32 * ks + 8 * lk;  // Dp % 8 == 0: the 8 entries are all in or all out
-                float4 xa0 = make_float4(0.f, 0.f, 0.f, 0.f), xa1 = xa0, vb0 = xa0, vb1 = xa0;
-                if (d0 < Dp) {
-                    const float4* xp = reinterpret_cast<const float4*>(&X_[(16 * rt + li) * SX + d0]);
-                    const float4* vp = reinterpret_cast<const float4*>(&Vs[col * SV + d0]);
-                    xa0 = xp[0];
-                    xa1 = xp[1];
-                    vb0 = vp[0];
-                    vb1 = vp[1];
-                }
-                av[0] = (__bf16)xa0.x; av[1] = (__bf16)xa0.y; av[2] = (__bf16)xa0.z; av[3] = (__bf16)xa0.w;
-                av[4] = (__bf16)xa1.x; av[5] = (__bf16)xa1.y; av[6] = (__bf16)xa1.z; av[7] = (__bf16)xa1.w;
-                bv[0] = (__bf16)vb0.x; bv[1] = (__bf16)vb0.y; bv[2] = (__bf16)vb0.z; bv[3] = (__bf16)vb0.w;
-                bv[4] = (__bf16)vb1.x; bv[5] = (__bf16)vb1.y; bv[6] = (__bf16)vb1.z; bv[7] = (__bf16)vb1.w;
-                cl[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, cl[1], 0, 0, 0);
             }
         }
         // elementwise: residuals and sigmoid'-weighted directions into LDS.
@@ -372,52 +358,20 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
         }
         __syncthreads();
         // GEMM-B: accumulate Xd^T R
-        if constexpr (!JBF) {
 #pragma unroll
-            for (int ks = 0; ks < NB / 4; ++ks) {
-                float xa[DPT];
+        for (int ks = 0; ks < NB / 4; ++ks) {
+            float xa[DPT];
 #pragma unroll
-                for (int dt = 0; dt < DPT; ++dt) xa[dt] = X_[(4 * ks + lk) * SX + 16 * dt + li];
+            for (int dt = 0; dt < DPT; ++dt) xa[dt] = Xs[(4 * ks + lk) * SX + 16 * dt + li];
 #pragma unroll
-                for (int cc = 0; cc < CT_PER_WAVE; ++cc) {
-                    // NCT is a multiple of 4: every wave owns CT_PER_WAVE tiles (no branch,
-                    // so the accumulators stay in place across k steps)
-                    const int ct = wave + 4 * cc;
-                    const float bv = Rs[(4 * ks + lk) * SR + 16 * ct + li];
+            for (int cc = 0; cc < CT_PER_WAVE; ++cc) {
+                // NCT is a multiple of 4: every wave owns CT_PER_WAVE tiles (no branch,
+                // so the accumulators stay in place across k steps)
+                const int ct = wave + 4 * cc;
+                const float bv = Rs[(4 * ks + lk) * SR + 16 * ct + li];
 #pragma unroll
-                    for (int dt = 0; dt < DPT; ++dt)
-                        acc[cc][dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[dt], bv, acc[cc][dt], 0, 0, 0);
-                }
-            }
-        } else {
-#pragma unroll
-            for (int m = 0; m < DPW; ++m) {
-                // d tile of this wave, clamped so every wave runs the same
-                // branch-free sequence (a clamped duplicate is not stored)
-                const int dt = min(wave + 4 * m, DPT - 1);
-                {
-                    // residual columns: FP32 MFMA over the 32 rows
-#pragma unroll
-                    for (int ks = 0; ks < NB / 4; ++ks) {
-                        const float xa = X_[(4 * ks + lk) * SX + 16 * dt + li];
-#pragma unroll
-                        for (int c = 0; c < 2; ++c) {
-                            const float bv = Rs[(4 * ks + lk) * SR + 16 * c + li];
-                            accx[m][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, bv, accx[m][c], 0, 0, 0);
-                        }
-                    }
-                    // probe columns: one BF16 MFMA over the 32 rows
-                    v8bf xa;
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) xa[e] = (__bf16)X_[(8 * lk + e) * SX + 16 * dt + li];
-#pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-                        v8bf bv;
-#pragma unroll
-                        for (int e = 0; e < 8; ++e) bv[e] = (__bf16)Rs[(8 * lk + e) * SR + CB + 16 * c + li];
-                        accp[m][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa, bv, accp[m][c], 0, 0, 0);
-                    }
-                }
+                for (int dt = 0; dt < DPT; ++dt)
+                    acc[cc][dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[dt], bv, acc[cc][dt], 0, 0, 0);
             }
         }
         if (c + 1 < nchunk) {
@@ -428,38 +382,19 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
     }
 
     // partial sums out: part[split][set][t][Dp]
-    if constexpr (!JBF) {
 #pragma unroll
-        for (int cc = 0; cc < CT_PER_WAVE; ++cc) {
-            const int ct = wave + 4 * cc;
-            if (ct >= NCT) continue;
-            const int j = 16 * ct + li;
-            const int set = j / CB;
-            const int64_t t = cb * CB + (j % CB);
-            if (t >= T) continue;
-            float* dst = part + (((size_t)split * (1 + K) + set) * T + t) * Dp;
+    for (int cc = 0; cc < CT_PER_WAVE; ++cc) {
+        const int ct = wave + 4 * cc;
+        if (ct >= NCT) continue;
+        const int j = 16 * ct + li;
+        const int set = j / CB;
+        const int64_t t = cb * CB + (j % CB);
+        if (t >= T) continue;
+        float* dst = part + (((size_t)split * (1 + K) + set) * T + t) * Dp;
 #pragma unroll
-            for (int dt = 0; dt < DPT; ++dt)
+        for (int dt = 0; dt < DPT; ++dt)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) dst[16 * dt + 4 * lk + r] = acc[cc][dt][r];
-        }
-    } else {
-#pragma unroll
-        for (int m = 0; m < DPW; ++m) {
-            const int dt = wave + 4 * m;
-            if (dt >= DPT) continue;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int j = 16 * c + li;
-                const int set = j / CB;
-                const int64_t t = cb * CB + (j % CB);
-                if (t >= T) continue;
-                float* dst = part + (((size_t)split * (1 + K) + set) * T + t) * Dp;
-                const v4f v = c < 2 ? accx[m][c & 1] : accp[m][c & 1];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) dst[16 * dt + 4 * lk + r] = v[r];
-            }
-        }
+            for (int r = 0; r < 4; ++r) dst[16 * dt + 4 * lk + r] = acc[cc][dt][r];
     }
     // log-likelihood of column 16*cx + li: lanes li, li+16, li+32, li+48 and waves cx, cx+2
     ll += __shfl_xor(ll, 16, 64);
@@ -474,23 +409,264 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
     }
 }
 
-template <int DPT, int K, bool JBF>
-__global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, const float* Xrm,
-                                                            const float* y32, int64_t N,
-                                                            int64_t rows_per_split,
-                                                            float* part, double* llpart,
-                                                            const int* skip) {
-    glm_pass_body<DPT, K, JBF>(src, Xrm, y32, N, rows_per_split, part, llpart, skip, blockIdx.x, blockIdx.y);
+// ---------------------------------------------------------------------------
+// BF16-Jacobian pass: the states' products (logits, residual, gradient) on
+// the FP32 MFMA as in the FP32 pass; the directions' products on the
+// 16x16x32 BF16 MFMA with operands that are BF16 already:
+//   * the design's BF16 copies (row-major Xb for GEMM-A, transposed Xtb for
+//     GEMM-B), 16-byte loads straight from global memory (L2): GEMM-A's at
+//     the start of its chunk, ahead of the FP32 logit MFMAs that hide their
+//     latency, GEMM-B's a chunk ahead;
+//   * the block's 32 direction columns, rounded once into registers (each
+//     wave's GEMM-A B operand for every k step);
+//   * the directions' sigmoid'-weighted logits u, written to LDS as BF16 in
+//     [column][row] order, so that a lane's eight k values are one 16-byte
+//     read.
+// The FP32 design chunk and the 32 state columns go through LDS as in the
+// FP32 pass.  The values and their summation order are those of rounding
+// the FP32 operands to BF16 inside the products.
+// Waves: GEMM-A, wave (rt, cx) owns row tile rt and column tile cx of the
+// states and of the directions; GEMM-B, wave w owns d tiles w, w + 4, ...
+// for all four column tiles (two of states, two of directions).
+// ---------------------------------------------------------------------------
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+typedef short v4s __attribute__((ext_vector_type(4)));
+
+template <int DPT>
+__device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, const float* Xrm,
+                                                   const uint16_t* Xb16, const uint16_t* Xtb16, int64_t Np,
+                                                   const float* y32, int64_t N, int64_t rows_per_split,
+                                                   float* part, double* llpart, const int* skip, const int64_t cb,
+                                                   const int split) {
+    constexpr int Dp = DPT * 16;
+    constexpr int Dq = (Dp + 31) / 32 * 32;  // K of the BF16 GEMM-A, in steps of 32
+    constexpr int KQ = Dq / 32;
+    constexpr int SV = Dp + 4;
+    constexpr int SX = Dp + 4;
+    constexpr int SR = CB + 4;  // states' residual, FP32 [row][column]
+    constexpr int SU = NB + 8;  // directions' u, BF16 [column][row]: 80-byte rows, so the
+                                // 16-byte reads of 8 consecutive lanes cover distinct banks
+    constexpr int F4 = NB * Dp / 4;
+    constexpr int PF = (F4 + 255) / 256;
+    constexpr int DPW = (DPT + 3) / 4;
+    __shared__ __align__(16) float Vs[CB * SV];
+    __shared__ __align__(16) float Xs[NB * SX];
+    __shared__ float Rs[NB * SR];
+    __shared__ __align__(16) __bf16 Ub[CB * SU];
+    __shared__ double llred[4][16];
+
+    if (skip && *(volatile const int*)skip) return;
+    const __bf16* Xb = reinterpret_cast<const __bf16*>(Xb16);
+    const __bf16* Xtb = reinterpret_cast<const __bf16*>(Xtb16);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int li = lane & 15;
+    const int lk = lane >> 4;
+    const int64_t D = src.D, T = src.Tn;
+    const int64_t n_begin = (int64_t)split * rows_per_split;
+    int64_t n_end = n_begin + rows_per_split;
+    if (n_end > N) n_end = N;
+    const int nchunk = n_end > n_begin ? (int)((n_end - n_begin + NB - 1) / NB) : 0;
+    const int rt = wave >> 1;  // GEMM-A row tile
+    const int cx = wave & 1;   // GEMM-A column tile
+
+    float4 pf[PF];
+    auto fetch = [&](int c) {
+        const float4* src4 = reinterpret_cast<const float4*>(Xrm + (size_t)(n_begin + (int64_t)c * NB) * Dp);
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int idx = tid + 256 * i;
+            if (idx < F4) pf[i] = src4[idx];
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int idx = tid + 256 * i;
+            if (idx < F4) {
+                const int e = 4 * idx;
+                const int r = e / Dp, d = e % Dp;
+                float2* dst = reinterpret_cast<float2*>(&Xs[r * SX + d]);
+                dst[0] = make_float2(pf[i].x, pf[i].y);
+                dst[1] = make_float2(pf[i].z, pf[i].w);
+            }
+        }
+    };
+    // BF16 design operands of chunk c: GEMM-A rows 16 rt + li, k = 32 q + 8 lk
+    // + e (Xb); GEMM-B d tile rows 16 dt + li, k = rows 8 lk + e of the chunk
+    // (Xtb; d tiles clamped, a clamped duplicate is not stored)
+    v8bf xa[KQ], xt[DPW];
+    auto fetch_xa = [&](int c) {
+        const int64_t n0 = n_begin + (int64_t)c * NB;
+        const __bf16* ra = Xb + (size_t)(n0 + 16 * rt + li) * Dq + 8 * lk;
+#pragma unroll
+        for (int q = 0; q < KQ; ++q) xa[q] = *reinterpret_cast<const v8bf*>(ra + 32 * q);
+    };
+    auto fetch_xt = [&](int c) {
+        const int64_t n0 = n_begin + (int64_t)c * NB;
+#pragma unroll
+        for (int m = 0; m < DPW; ++m) {
+            const int dt = min(wave + 4 * m, DPT - 1);
+            xt[m] = *reinterpret_cast<const v8bf*>(Xtb + (size_t)(16 * dt + li) * Np + n0 + 8 * lk);
+        }
+    };
+    if (nchunk > 0) {
+        fetch(0);
+        fetch_xt(0);
+    }
+    // states of the block into LDS (zero padded)
+    for (int i = tid; i < CB * Dp; i += 256) {
+        const int j = i / Dp, d = i % Dp;
+        const int64_t t = cb * CB + j;
+        float v = 0.f;
+        if (t < T && d < D) v = (float)src.get(0, t, d);
+        Vs[j * SV + d] = v;
+    }
+    // directions of column 16 cx + li, rounded once: the GEMM-A B operand
+    v8bf zb[KQ];
+    {
+        const int64_t t = cb * CB + 16 * cx + li;
+#pragma unroll
+        for (int q = 0; q < KQ; ++q)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int64_t d = 32 * q + 8 * lk + e;
+                zb[q][e] = (__bf16)((t < T && d < D) ? (float)src.get(1, t, d) : 0.f);
+            }
+    }
+    if (nchunk > 0) stash();
+    __syncthreads();
+
+    v4f accx[DPW][2], accp[DPW][2];
+#pragma unroll
+    for (int m = 0; m < DPW; ++m)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) accx[m][c] = accp[m][c] = (v4f){0.f, 0.f, 0.f, 0.f};
+    double ll = 0.0;
+
+    for (int c = 0; c < nchunk; ++c) {
+        const int64_t n0 = n_begin + (int64_t)c * NB;
+        const bool more = c + 1 < nchunk;
+        if (more) fetch(c + 1);
+        fetch_xa(c);
+        // GEMM-A: state logits (FP32) and direction logits (BF16)
+        v4f cl = (v4f){0.f, 0.f, 0.f, 0.f}, cq = cl;
+#pragma unroll 7
+        for (int ks = 0; ks < Dp / 4; ++ks) {
+            const float av = Xs[(16 * rt + li) * SX + 4 * ks + lk];
+            const float bv = Vs[(16 * cx + li) * SV + 4 * ks + lk];
+            cl = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, cl, 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < KQ; ++q) cq = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa[q], zb[q], cq, 0, 0, 0);
+        // elementwise: the states' residual (FP32) and the directions' u (BF16)
+        v4bf u4;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * rt + 4 * lk + r;
+            const int64_t n = n0 + row;
+            const bool valid = n < N;
+            const float l = cl[r];
+            const float e = __expf(-fabsf(l));
+            const float inv = 1.f / (1.f + e);
+            const float sg = l >= 0.f ? inv : e * inv;
+            const float yv = y32[n];
+            if (valid) ll += (double)(yv * l - (fmaxf(l, 0.f) + __logf(1.f + e)));
+            Rs[row * SR + 16 * cx + li] = valid ? (yv - sg) : 0.f;
+            u4[r] = (__bf16)(valid ? sg * (1.f - sg) * cq[r] : 0.f);
+        }
+        *reinterpret_cast<v4bf*>(&Ub[(16 * cx + li) * SU + 16 * rt + 4 * lk]) = u4;
+        __syncthreads();
+        // GEMM-B: Xd^T [r | u] over the chunk's 32 rows
+#pragma unroll
+        for (int m = 0; m < DPW; ++m) {
+            const int dt = min(wave + 4 * m, DPT - 1);
+#pragma unroll
+            for (int ks = 0; ks < NB / 4; ++ks) {
+                const float xv = Xs[(4 * ks + lk) * SX + 16 * dt + li];
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) {
+                    const float bv = Rs[(4 * ks + lk) * SR + 16 * cc + li];
+                    accx[m][cc] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv, bv, accx[m][cc], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                const v8bf bv = *reinterpret_cast<const v8bf*>(&Ub[(16 * cc + li) * SU + 8 * lk]);
+                accp[m][cc] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xt[m], bv, accp[m][cc], 0, 0, 0);
+            }
+        }
+        if (more) fetch_xt(c + 1);
+        if (more) {
+            __syncthreads();  // both GEMMs are done with this chunk
+            stash();
+        }
+        __syncthreads();
+    }
+
+    // partial sums out: part[split][set][t][Dp]
+#pragma unroll
+    for (int m = 0; m < DPW; ++m) {
+        const int dt = wave + 4 * m;
+        if (dt >= DPT) continue;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int set = c >> 1;
+            const int64_t t = cb * CB + 16 * (c & 1) + li;
+            if (t >= T) continue;
+            float* dst = part + (((size_t)split * 2 + set) * T + t) * Dp;
+            const v4f v = c < 2 ? accx[m][c & 1] : accp[m][c & 1];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) dst[16 * dt + 4 * lk + r] = v[r];
+        }
+    }
+    ll += __shfl_xor(ll, 16, 64);
+    ll += __shfl_xor(ll, 32, 64);
+    if (lk == 0) llred[wave][li] = ll;
+    __syncthreads();
+    if (tid < CB) {
+        const int cxx = tid / 16, l2 = tid % 16;
+        const double v = llred[cxx][l2] + llred[cxx + 2][l2];
+        const int64_t t = cb * CB + tid;
+        if (t < T) llpart[(size_t)split * T + t] = v;
+    }
+}
+
+// The BF16 form is held to two waves per SIMD: at the widest designs its
+// registers would otherwise allow only one, with no second wave to run while
+// the first waits at a barrier.
+template <int DPT, int K>
+__global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, const float* Xrm, const float* y32,
+                                                            int64_t N, int64_t rows_per_split, float* part,
+                                                            double* llpart, const int* skip) {
+    glm_pass_body<DPT, K>(src, Xrm, y32, N, rows_per_split, part, llpart, skip, blockIdx.x, blockIdx.y);
+}
+
+template <int DPT>
+__global__ __launch_bounds__(256, 2) void glm_pass_bf16_kernel(ColSrc<float> src, const float* Xrm,
+                                                               const uint16_t* Xb, const uint16_t* Xtb, int64_t Np,
+                                                               const float* y32, int64_t N, int64_t rows_per_split,
+                                                               float* part, double* llpart, const int* skip) {
+    glm_pass_bf16_body<DPT>(src, Xrm, Xb, Xtb, Np, y32, N, rows_per_split, part, llpart, skip, blockIdx.x, blockIdx.y);
 }
 
 // Batched over chains of one padded width: work item blockIdx.x is
 // (chain, column block, N split) = items[blockIdx.x].
-template <int DPT, int K, bool JBF>
+template <int DPT, int K>
 __global__ __launch_bounds__(256) void glm_pass_mfma_batched(const GlmPassArgs* __restrict__ args, const int4* __restrict__ items) {
     const int4 w = items[blockIdx.x];
     const GlmPassArgs& a = args[w.x];
     if ((int64_t)w.y * CB >= a.src.Tn) return;  // beyond the chain's current window
-    glm_pass_body<DPT, K, JBF>(a.src, a.Xrm, a.y32, a.N, a.rows_per_split, a.part, a.llpart, a.skip, w.y, w.z);
+    glm_pass_body<DPT, K>(a.src, a.Xrm, a.y32, a.N, a.rows_per_split, a.part, a.llpart, a.skip, w.y, w.z);
+}
+
+template <int DPT>
+__global__ __launch_bounds__(256, 2) void glm_pass_bf16_batched(const GlmPassArgs* __restrict__ args, const int4* __restrict__ items) {
+    const int4 w = items[blockIdx.x];
+    const GlmPassArgs& a = args[w.x];
+    if ((int64_t)w.y * CB >= a.src.Tn) return;  // beyond the chain's current window
+    glm_pass_bf16_body<DPT>(a.src, a.Xrm, a.Xb, a.Xtb, a.Np, a.y32, a.N, a.rows_per_split, a.part, a.llpart, a.skip, w.y, w.z);
 }
 
 // Sum the splits, add the prior terms, and form the next pass's columns.
@@ -628,10 +804,10 @@ static void launch_mfma(const ColSrc<float>& src, const GlmData* g, int ns, int6
                         float* part, double* llpart, const int* skip, bool jbf, hipStream_t s) {
     dim3 grid((unsigned)((src.Tn + CB - 1) / CB), (unsigned)ns);
     if (jbf)
-        hipLaunchKernelGGL((glm_pass_mfma_kernel<DPT, 1, true>), grid, dim3(256), 0, s, src, g->Xrm, g->y, g->N,
-                           rows_per_split, part, llpart, skip);
+        hipLaunchKernelGGL((glm_pass_bf16_kernel<DPT>), grid, dim3(256), 0, s, src, g->Xrm, g->Xb, g->Xtb, g->Np,
+                           g->y, g->N, rows_per_split, part, llpart, skip);
     else
-        hipLaunchKernelGGL((glm_pass_mfma_kernel<DPT, 1, false>), grid, dim3(256), 0, s, src, g->Xrm, g->y, g->N,
+        hipLaunchKernelGGL((glm_pass_mfma_kernel<DPT, 1>), grid, dim3(256), 0, s, src, g->Xrm, g->y, g->N,
                            rows_per_split, part, llpart, skip);
 }
 
@@ -743,11 +919,9 @@ template <int DPT>
 static void launch_glm_batched_dpt(bool jbf, const GlmPassArgs* args, const int4* items, int nitems,
                                    hipStream_t s) {
     if (jbf)
-        hipLaunchKernelGGL((glm_pass_mfma_batched<DPT, 1, true>), dim3((unsigned)nitems), dim3(256), 0, s, args,
-                           items);
+        hipLaunchKernelGGL((glm_pass_bf16_batched<DPT>), dim3((unsigned)nitems), dim3(256), 0, s, args, items);
     else
-        hipLaunchKernelGGL((glm_pass_mfma_batched<DPT, 1, false>), dim3((unsigned)nitems), dim3(256), 0, s, args,
-                           items);
+        hipLaunchKernelGGL((glm_pass_mfma_batched<DPT, 1>), dim3((unsigned)nitems), dim3(256), 0, s, args, items);
 }
 
 void launch_glm_pass_batched(int dpt, bool jbf, const GlmPassArgs* args, const int4* items, int nitems,

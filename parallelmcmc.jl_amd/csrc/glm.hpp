@@ -19,6 +19,13 @@ struct GlmData {
     float* Xrm = nullptr;   // N x Dp row-major (row n contiguous), zero padded
     float* Xcm = nullptr;   // Dp x N: column d of the design contiguous over n
     double* Xd64 = nullptr; // N x D column-major copy for the FP64 path
+    // BF16 copies of the design for the directions' GEMMs of the BF16-Jacobian
+    // pass (the FP32 values rounded to nearest even, zero padded):
+    // Xb row-major Np x Dq (Dq = D rounded up to 32, one 16x16x32 MFMA K step
+    // per 32 columns), Xtb transposed Dp x Np (row d contiguous over n)
+    uint16_t* Xb = nullptr;
+    uint16_t* Xtb = nullptr;
+    int64_t Np = 0, Dq = 0;
     float* y = nullptr;
     double* y64 = nullptr;
     // Gaussian prior N(0, P^-1) with a dense precision P (D x D, symmetric,
@@ -73,6 +80,9 @@ struct GlmWork {
 struct GlmPassArgs {
     ColSrc<float> src;
     const float* Xrm;
+    const uint16_t* Xb;   // BF16 copies (GlmData)
+    const uint16_t* Xtb;
+    int64_t Np;
     const float* y32;
     int64_t N;
     int64_t rows_per_split;
