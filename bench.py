@@ -466,7 +466,21 @@ def bench_c3(args):
     log(f"c3 batched bf16: {sps:.0f} samples/s ({elapsed / args.steps:.2f} s per solve)")
     its = {kk: [r[2].iters for r, (k2, _) in zip(res, chains) if k2 == kk] for kk in ("gauss", "logreg")}
     all_conv = all(r[2].converged for r in res)
-    del plan, res
+    del res
+    # roofline of the dominant kernel: one more, profiled, solve of every chain
+    # (direct launches, HIP events around every GEMM-pass and Lambda-pass
+    # launch, algorithmic flops counted on the device)
+    plan.set_profiling(True)
+    t0 = time.perf_counter()
+    plan.solve()
+    torch.cuda.synchronize()
+    prof_s = time.perf_counter() - t0
+    n_glm, ms_glm, fl_glm = plan.glm_pass_stats()
+    n_lam, ms_lam, fl_lam = plan.pass_stats()
+    plan.set_profiling(False)
+    del plan
+    log(f"c3 profiled solve {prof_s:.2f} s: GEMM passes {n_glm} launches {ms_glm / 1e3:.2f} s, "
+        f"Lambda passes {n_lam} launches {ms_lam / 1e3:.2f} s")
 
     # subset: every k-th pair (one Gaussian, one logistic chain)
     pick = [j for j in range(len(cl)) if (j // 2) % args.c3_baseline_every == 0]
@@ -479,13 +493,6 @@ def bench_c3(args):
     res_b = plan_b.solve()
     torch.cuda.synchronize()
     sub_b = time.perf_counter() - t0
-    # roofline of the dominant kernel (the Lambda passes): one more, profiled,
-    # solve of the subset with events around every pass launch
-    plan_b.set_profiling(True)
-    plan_b.solve()
-    torch.cuda.synchronize()
-    n_pass, ms_pass, fl_pass = plan_b.pass_stats()
-    plan_b.set_profiling(False)
     S_b = [r[0].cpu().numpy() for r in res_b]
     A_b = [r[1].cpu().numpy().astype(bool) for r in res_b]
     del plan_b
@@ -497,7 +504,18 @@ def bench_c3(args):
     torch.cuda.synchronize()
     sub_f = time.perf_counter() - t0
     del plan_f
-    log(f"c3 subset ({len(sub)} chains): batched bf16 {sub_b:.2f} s, batched f32 {sub_f:.2f} s; plain f32 next")
+    # the sampler's default damping (ParallelMALASampler: 0.5) on the same
+    # chains: one solve (it needs about 7x the Newton iterations of damping 1)
+    plan_h = BatchPlan(sub, window=W, precision="bf16_jac", damping=0.5, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res_h = plan_h.solve()
+    torch.cuda.synchronize()
+    sub_h = time.perf_counter() - t0
+    its_h = int(sum(r[2].iters for r in res_h))
+    del plan_h, res_h
+    log(f"c3 subset ({len(sub)} chains): batched bf16 {sub_b:.2f} s, batched f32 {sub_f:.2f} s, "
+        f"batched bf16 damping 0.5 {sub_h:.2f} s; plain f32 next")
     tuned, tune_s = {}, 0.0
     if not args.no_tune:
         t0 = time.perf_counter()
@@ -538,7 +556,8 @@ def bench_c3(args):
 
     # the dominant kernel's roofline: states on FP32 MFMA, directions on BF16 MFMA
     peak_eff = 2.0 / (1.0 / PEAK_TFLOPS["f32"] + 1.0 / PEAK_BF16_TFLOPS)
-    achieved = fl_pass / (ms_pass / 1e3) / 1e12 if ms_pass > 0 else 0.0
+    achieved = fl_glm / (ms_glm / 1e3) / 1e12 if ms_glm > 0 else 0.0
+    achieved_lam = fl_lam / (ms_lam / 1e3) / 1e12 if ms_lam > 0 else 0.0
     out = {
         "metric": METRIC,
         "value": sps,
@@ -570,24 +589,37 @@ def bench_c3(args):
                 "batched_f32_samples_per_s": sub_total / sub_f,
                 "batched_bf16_over_plain_f32": base_s / sub_b,
                 "batched_bf16_over_batched_f32": sub_f / sub_b,
+                "batched_bf16_damping_0.5_samples_per_s": sub_total / sub_h,
+                "batched_bf16_damping_0.5_newton_iters": its_h,
                 "plain_f32_newton_iters": [int(sum(i.iters for i in inf)) for inf in base_infos],
                 "autotuner": {"width_buckets": len(tuned), "seconds": tune_s},
                 "vs_fp64_sequential": acc_rows,
             },
         },
         "roofline": {
-            "kernel": "gauss_slab_batched (dense-Gaussian Lambda pass over 64-row slabs), bf16_jac",
+            "kernel": "glm_pass_bf16_batched (logistic GEMM -> sigmoid -> GEMM^T pass, every padded width)",
             "bound": "mfma",
             "achieved": achieved,
             "peak": peak_eff,
             "unit": "TFLOP/s",
             "frac": achieved / peak_eff,
             "traffic": None,
-            "launches": n_pass,
-            "avg_launch_us": 1e3 * ms_pass / max(n_pass, 1),
-            "note": ("measured on the subset solve with HIP events around every pass launch (direct launches); "
-                     "algorithmic flops 2 x rows x D x columns (states and directions) counted on the device; "
-                     "peak: states on FP32 MFMA (157.3 TFLOP/s), directions on BF16 MFMA (2500 TFLOP/s)"),
+            "launches": n_glm,
+            "avg_launch_us": 1e3 * ms_glm / max(n_glm, 1),
+            "share_of_profiled_solve": ms_glm / 1e3 / prof_s,
+            "note": ("one profiled solve of every chain (direct launches, HIP events around every pass launch); "
+                     "algorithmic flops 8 x rows x D x columns per pass (two GEMMs of the states and of the "
+                     "directions) counted on the device; peak: states on FP32 MFMA (157.3 TFLOP/s), directions "
+                     "on BF16 MFMA (2500 TFLOP/s), time-weighted"),
+            "lambda_pass": {
+                "kernel": "gauss_slab_batched (dense-Gaussian Lambda pass over 64-row slabs)",
+                "achieved_tflops": achieved_lam,
+                "frac": achieved_lam / peak_eff,
+                "launches": n_lam,
+                "avg_launch_us": 1e3 * ms_lam / max(n_lam, 1),
+                "share_of_profiled_solve": ms_lam / 1e3 / prof_s,
+                "note": "latency-bound: in the causal phase a launch covers two tiles of a few chains",
+            },
         },
     }
     if not args.no_cpu_baseline:

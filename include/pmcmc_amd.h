@@ -79,6 +79,7 @@ const char* pmcmc_version(void);
 #define PMCMC_KT_GLM_REDUCE 3  /* split-sum + prior terms of a GEMM pass */
 #define PMCMC_KT_SCAN_SPINE 4  /* scan of the tile aggregates (all levels) */
 #define PMCMC_KT_ITERATION 5   /* one whole Newton iteration */
+#define PMCMC_KT_LAMBDA_PASS 6 /* batched dense-Gaussian Lambda pass (pmcmc_batch_* only) */
 int pmcmc_ctx_set_profiling(pmcmc_ctx* ctx, int on); /* on: every kind */
 /* Profile only the kinds in `kinds` (bit k = 1u << PMCMC_KT_k; 0 turns
  * profiling off).  Each event pair adds a marker to the stream, so timing a
@@ -391,12 +392,17 @@ int pmcmc_batch_set_graph(pmcmc_batch* b, int32_t on);
 int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos);
 /* Measurement of the dense-Gaussian Lambda passes (no reference counterpart):
  * with profiling on, solves launch directly (no graphs), record HIP events
- * around every Lambda-pass launch and count the passes' algorithmic flops
+ * around every Lambda-pass (and GEMM-pass) launch and count the passes' algorithmic flops
  * (2 x rows x D x columns, states and directions) on the device;
  * pmcmc_batch_pass_stats returns the last profiled solve's launches, their
  * summed duration and the flops. */
 int pmcmc_batch_set_profiling(pmcmc_batch* b, int32_t on);
 int pmcmc_batch_pass_stats(const pmcmc_batch* b, int64_t* launches, double* total_ms, double* flops);
+/* The same for the logistic chains' GEMM passes (no reference counterpart;
+ * they replace bayes_logreg.jl's gradient / HVP GEMMs): launches, summed
+ * duration and algorithmic flops (8 x rows x D x columns: two GEMMs of the
+ * states and of the directions) of the last profiled solve. */
+int pmcmc_batch_glm_pass_stats(const pmcmc_batch* b, int64_t* launches, double* total_ms, double* flops);
 int pmcmc_batch_destroy(pmcmc_batch* b);
 
 #ifdef __cplusplus

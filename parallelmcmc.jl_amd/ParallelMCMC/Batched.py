@@ -127,9 +127,10 @@ class BatchPlan:
                 for i in range(self.n)]
 
     def set_profiling(self, on=True):
-        """Measure the dense-Gaussian Lambda passes of the next solves: HIP
-        events around every launch (direct launches, no graph replay) and
-        their algorithmic flops counted on the device (``pass_stats``)."""
+        """Measure the dense-Gaussian Lambda passes and the logistic GEMM passes
+        of the next solves: HIP events around every launch (direct launches,
+        no graph replay) and their algorithmic flops counted on the device
+        (``pass_stats``, ``glm_pass_stats``)."""
         _lib.check(_lib.load().pmcmc_batch_set_profiling(self.h, 1 if on else 0), self.ctx.h,
                    "pmcmc_batch_set_profiling")
 
@@ -140,6 +141,16 @@ class BatchPlan:
         fl = C.c_double()
         _lib.check(_lib.load().pmcmc_batch_pass_stats(self.h, C.byref(n), C.byref(ms), C.byref(fl)), self.ctx.h,
                    "pmcmc_batch_pass_stats")
+        return int(n.value), float(ms.value), float(fl.value)
+
+    def glm_pass_stats(self):
+        """(launches, total_ms, flops) of the last profiled solve's logistic GEMM
+        passes (one launch per padded width class and pass)."""
+        n = C.c_int64()
+        ms = C.c_double()
+        fl = C.c_double()
+        _lib.check(_lib.load().pmcmc_batch_glm_pass_stats(self.h, C.byref(n), C.byref(ms), C.byref(fl)),
+                   self.ctx.h, "pmcmc_batch_glm_pass_stats")
         return int(n.value), float(ms.value), float(fl.value)
 
     def __del__(self):

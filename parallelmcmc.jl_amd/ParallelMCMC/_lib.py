@@ -30,7 +30,8 @@ KT_TILE_APPLY = 2
 KT_GLM_REDUCE = 3
 KT_SCAN_SPINE = 4
 KT_ITERATION = 5
-KT_NAMES = ("glm_pass", "tile_reduce", "tile_apply", "glm_reduce", "scan_spine", "iteration")
+KT_LAMBDA_PASS = 6
+KT_NAMES = ("glm_pass", "tile_reduce", "tile_apply", "glm_reduce", "scan_spine", "iteration", "lambda_pass")
 
 JAC_STOCH_DIAG = 0
 JAC_DIAG = 1
@@ -150,6 +151,7 @@ _SIGS = {
     "pmcmc_batch_set_graph": ([_P, C.c_int32], C.c_int),
     "pmcmc_batch_set_profiling": ([_P, C.c_int32], C.c_int),
     "pmcmc_batch_pass_stats": ([_P, _P, _P, _P], C.c_int),
+    "pmcmc_batch_glm_pass_stats": ([_P, _P, _P, _P], C.c_int),
     "pmcmc_batch_solve": ([_P, C.POINTER(DeerInfo)], C.c_int),
     "pmcmc_batch_destroy": ([_P], C.c_int),
     "pmcmc_deer_adjoint": ([_P, _P, _P, C.POINTER(DeerParams), _P, _P, _P, _P, _P, C.POINTER(C.c_double), _P, _P,

@@ -93,7 +93,7 @@ struct SlideState {
 // convergence (which exit at once) are not counted.
 // ---------------------------------------------------------------------------
 enum { KT_GLM_PASS = 0, KT_TILE_REDUCE = 1, KT_TILE_APPLY = 2, KT_GLM_REDUCE = 3, KT_SCAN_SPINE = 4,
-       KT_ITERATION = 5, KT_COUNT = 6 };
+       KT_ITERATION = 5, KT_LAMBDA_PASS = 6, KT_COUNT = 7 };
 
 struct Prof {
     bool on = false;

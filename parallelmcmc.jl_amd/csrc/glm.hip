@@ -245,7 +245,8 @@ void glm_work_destroy(GlmWork* w) {
 template <int DPT, int K>
 __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const float* Xrm, const float* y32,
                                               int64_t N, int64_t rows_per_split, float* part, double* llpart,
-                                              const int* skip, const int64_t cb, const int split) {
+                                              const int* skip, unsigned long long* flops, const int64_t cb,
+                                              const int split) {
     constexpr int Dp = DPT * 16;
     constexpr int NC = CB * (1 + K);
     constexpr int SV = Dp + 4;
@@ -407,6 +408,10 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
         const int64_t t = cb * CB + tid;
         if (t < T) llpart[(size_t)split * T + t] = v;
     }
+    if (flops && tid == 0 && n_end > n_begin && cb * CB < T) {
+        const int64_t nc = T - cb * CB < CB ? T - cb * CB : CB;
+        atomicAdd(flops, (unsigned long long)(8 * (n_end - n_begin) * D * nc));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -436,8 +441,8 @@ template <int DPT>
 __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, const float* Xrm,
                                                    const uint16_t* Xb16, const uint16_t* Xtb16, int64_t Np,
                                                    const float* y32, int64_t N, int64_t rows_per_split,
-                                                   float* part, double* llpart, const int* skip, const int64_t cb,
-                                                   const int split) {
+                                                   float* part, double* llpart, const int* skip,
+                                                   unsigned long long* flops, const int64_t cb, const int split) {
     constexpr int Dp = DPT * 16;
     constexpr int Dq = (Dp + 31) / 32 * 32;  // K of the BF16 GEMM-A, in steps of 32
     constexpr int KQ = Dq / 32;
@@ -631,6 +636,10 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
         const int64_t t = cb * CB + tid;
         if (t < T) llpart[(size_t)split * T + t] = v;
     }
+    if (flops && tid == 0 && n_end > n_begin && cb * CB < T) {
+        const int64_t nc = T - cb * CB < CB ? T - cb * CB : CB;
+        atomicAdd(flops, (unsigned long long)(8 * (n_end - n_begin) * D * nc));
+    }
 }
 
 // The BF16 form is held to two waves per SIMD: at the widest designs its
@@ -640,7 +649,7 @@ template <int DPT, int K>
 __global__ __launch_bounds__(256) void glm_pass_mfma_kernel(ColSrc<float> src, const float* Xrm, const float* y32,
                                                             int64_t N, int64_t rows_per_split, float* part,
                                                             double* llpart, const int* skip) {
-    glm_pass_body<DPT, K>(src, Xrm, y32, N, rows_per_split, part, llpart, skip, blockIdx.x, blockIdx.y);
+    glm_pass_body<DPT, K>(src, Xrm, y32, N, rows_per_split, part, llpart, skip, nullptr, blockIdx.x, blockIdx.y);
 }
 
 template <int DPT>
@@ -648,7 +657,8 @@ __global__ __launch_bounds__(256, 2) void glm_pass_bf16_kernel(ColSrc<float> src
                                                                const uint16_t* Xb, const uint16_t* Xtb, int64_t Np,
                                                                const float* y32, int64_t N, int64_t rows_per_split,
                                                                float* part, double* llpart, const int* skip) {
-    glm_pass_bf16_body<DPT>(src, Xrm, Xb, Xtb, Np, y32, N, rows_per_split, part, llpart, skip, blockIdx.x, blockIdx.y);
+    glm_pass_bf16_body<DPT>(src, Xrm, Xb, Xtb, Np, y32, N, rows_per_split, part, llpart, skip, nullptr, blockIdx.x,
+                            blockIdx.y);
 }
 
 // Batched over chains of one padded width: work item blockIdx.x is
@@ -658,7 +668,7 @@ __global__ __launch_bounds__(256) void glm_pass_mfma_batched(const GlmPassArgs* 
     const int4 w = items[blockIdx.x];
     const GlmPassArgs& a = args[w.x];
     if ((int64_t)w.y * CB >= a.src.Tn) return;  // beyond the chain's current window
-    glm_pass_body<DPT, K>(a.src, a.Xrm, a.y32, a.N, a.rows_per_split, a.part, a.llpart, a.skip, w.y, w.z);
+    glm_pass_body<DPT, K>(a.src, a.Xrm, a.y32, a.N, a.rows_per_split, a.part, a.llpart, a.skip, a.flops, w.y, w.z);
 }
 
 template <int DPT>
@@ -666,7 +676,8 @@ __global__ __launch_bounds__(256, 2) void glm_pass_bf16_batched(const GlmPassArg
     const int4 w = items[blockIdx.x];
     const GlmPassArgs& a = args[w.x];
     if ((int64_t)w.y * CB >= a.src.Tn) return;  // beyond the chain's current window
-    glm_pass_bf16_body<DPT>(a.src, a.Xrm, a.Xb, a.Xtb, a.Np, a.y32, a.N, a.rows_per_split, a.part, a.llpart, a.skip, w.y, w.z);
+    glm_pass_bf16_body<DPT>(a.src, a.Xrm, a.Xb, a.Xtb, a.Np, a.y32, a.N, a.rows_per_split, a.part, a.llpart, a.skip,
+                            a.flops, w.y, w.z);
 }
 
 // Sum the splits, add the prior terms, and form the next pass's columns.

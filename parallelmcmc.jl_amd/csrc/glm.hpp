@@ -89,6 +89,9 @@ struct GlmPassArgs {
     float* part;
     double* llpart;
     const int* skip;
+    // non-null: every workgroup adds its algorithmic flops (8 x rows x D x
+    // columns: two GEMMs of the states and of the directions)
+    unsigned long long* flops;
 };
 
 // One chain's operands of a batched split-sum / epilogue (glm_reduce_batched).

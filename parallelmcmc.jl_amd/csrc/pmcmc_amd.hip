@@ -2356,8 +2356,8 @@ struct pmcmc_batch {
     // algorithmic flops counted on the device
     bool prof_on = false;
     Prof prof;
-    unsigned long long* flop_ctr = nullptr;  // device
-    double prof_flops = 0;
+    unsigned long long* flop_ctr = nullptr;  // device: [0] Lambda passes, [1] GLM passes
+    double prof_flops[2] = {0, 0};
 };
 
 namespace {
@@ -2601,18 +2601,27 @@ extern "C" int pmcmc_batch_set_profiling(pmcmc_batch* b, int32_t on) {
     if (!b) return PMCMC_ERR_ARGUMENT;
     pmcmc_ctx* c = b->ctx;
     HIPCHK(c, hipSetDevice(c->device));
-    if (on && !b->flop_ctr) HIPCHK(c, hipMalloc(&b->flop_ctr, sizeof(unsigned long long)));
+    if (on && !b->flop_ctr) HIPCHK(c, hipMalloc(&b->flop_ctr, 2 * sizeof(unsigned long long)));
     b->prof_on = on != 0;
     b->prof.on = b->prof_on;
-    b->prof.kinds = 1u << KT_GLM_PASS;
+    b->prof.kinds = (1u << KT_LAMBDA_PASS) | (1u << KT_GLM_PASS);
     return PMCMC_OK;
 }
 
 extern "C" int pmcmc_batch_pass_stats(const pmcmc_batch* b, int64_t* launches, double* total_ms, double* flops) {
     if (!b) return PMCMC_ERR_ARGUMENT;
+    if (launches) *launches = b->prof.n[KT_LAMBDA_PASS];
+    if (total_ms) *total_ms = b->prof.ms[KT_LAMBDA_PASS];
+    if (flops) *flops = b->prof_flops[0];
+    return PMCMC_OK;
+}
+
+extern "C" int pmcmc_batch_glm_pass_stats(const pmcmc_batch* b, int64_t* launches, double* total_ms,
+                                          double* flops) {
+    if (!b) return PMCMC_ERR_ARGUMENT;
     if (launches) *launches = b->prof.n[KT_GLM_PASS];
     if (total_ms) *total_ms = b->prof.ms[KT_GLM_PASS];
-    if (flops) *flops = b->prof_flops;
+    if (flops) *flops = b->prof_flops[1];
     return PMCMC_OK;
 }
 
@@ -2741,6 +2750,7 @@ static void batch_tables(pmcmc_batch* b, BatchTables* bt) {
                 ga.part = h.part;
                 ga.llpart = llpart;
                 ga.skip = skip;
+                ga.flops = b->prof_on ? b->flop_ctr + 1 : nullptr;
                 bt->gpass[v].push_back(ga);
                 GlmReduceArgs gr{};
                 gr.src = ga.src;
@@ -2923,8 +2933,8 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
     const bool graphs = b->use_graph && !b->prof_on;  // profiled solves launch directly
     if (b->prof_on) {
         b->prof.reset();
-        b->prof_flops = 0;
-        HIPCHK(c, hipMemsetAsync(b->flop_ctr, 0, sizeof(unsigned long long), s));
+        b->prof_flops[0] = b->prof_flops[1] = 0;
+        HIPCHK(c, hipMemsetAsync(b->flop_ctr, 0, 2 * sizeof(unsigned long long), s));
     }
     for (auto& h : b->ch) {
         h.iters = 0;
@@ -3009,22 +3019,28 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
         auto ia = [&](size_t o) { return (void*)(I0 + o); };
         hipLaunchKernelGGL(batch_window_kernel, dim3(4, (unsigned)it.wins.size()), dim3(256), 0, s,
                            (const WinDesc*)at(o_win), (const int*)ia(o_wins), par);
-        for (int d = 1; d <= 16; ++d)
+        for (int d = 1; d <= 16; ++d) {
+            const int pg = it.glm[d].empty() ? -1 : b->prof.begin(KT_GLM_PASS, 0, s);
             launch_glm_pass_batched(d, b->jbf, (const GlmPassArgs*)at(o_gp[par]), (const int4*)ia(o_gi[d]),
                                     (int)it.glm[d].size(), s);
+            b->prof.end(pg, s);
+        }
         {
-            const int pe = it.gslab.empty() ? -1 : b->prof.begin(KT_GLM_PASS, 0, s);
+            const int pe = it.gslab.empty() ? -1 : b->prof.begin(KT_LAMBDA_PASS, 0, s);
             launch_gauss_slab_batched(b->jbf, (const GaussPassArgs*)at(o_qp[par]), (const int4*)ia(o_qs),
                                       (int)it.gslab.size(), s);
             b->prof.end(pe, s);
         }
         launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[par]), (const int2*)ia(o_glmred),
                                   (int)it.glm_red.size(), s);
-        for (int d = 1; d <= 16; ++d)
+        for (int d = 1; d <= 16; ++d) {
+            const int pg = it.glm[d].empty() ? -1 : b->prof.begin(KT_GLM_PASS, 0, s);
             launch_glm_pass_batched(d, b->jbf, (const GlmPassArgs*)at(o_gp[2]), (const int4*)ia(o_gi[d]),
                                     (int)it.glm[d].size(), s);
+            b->prof.end(pg, s);
+        }
         {
-            const int pe = it.gslab.empty() ? -1 : b->prof.begin(KT_GLM_PASS, 0, s);
+            const int pe = it.gslab.empty() ? -1 : b->prof.begin(KT_LAMBDA_PASS, 0, s);
             launch_gauss_slab_batched(b->jbf, (const GaussPassArgs*)at(o_qp[2]), (const int4*)ia(o_qs),
                                       (int)it.gslab.size(), s);
             b->prof.end(pe, s);
@@ -3178,9 +3194,10 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
     HIPCHK(c, hipMemcpyAsync(b->hslide, b->slides, n * sizeof(SlideState), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     if (b->prof_on) {
-        unsigned long long fl = 0;
-        HIPCHK(c, hipMemcpy(&fl, b->flop_ctr, sizeof fl, hipMemcpyDeviceToHost));
-        b->prof_flops = (double)fl;
+        unsigned long long fl[2] = {0, 0};
+        HIPCHK(c, hipMemcpy(fl, b->flop_ctr, sizeof fl, hipMemcpyDeviceToHost));
+        b->prof_flops[0] = (double)fl[0];
+        b->prof_flops[1] = (double)fl[1];
     }
     for (int i = 0; i < n; ++i) {
         const SyncState& hs = b->hsync[i];

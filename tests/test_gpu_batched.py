@@ -77,6 +77,41 @@ def test_batched_graph_and_direct_launches_identical():
         assert it1 == i2.iters and np.array_equal(S1.cpu().numpy(), S2.cpu().numpy())
 
 
+def test_batched_profiled_solve_counts_pass_flops():
+    """A profiled solve (direct launches, events around the GEMM and Lambda
+    passes) returns the graph-replayed solve's results bit for bit; with one
+    logistic chain whose trajectory fits one window, every live GEMM pass
+    covers all N rows and between 1 and T columns, so the counted flops lie
+    between 2 x iters x 8 N D and 2 x iters x 8 N D T (launches queued after
+    convergence are timed but count no flops)."""
+    from ParallelMCMC.Batched import BatchPlan
+
+    pairs = _chains(9)
+    chains = [c for _, c in pairs]
+    plan = BatchPlan(chains, window=300, precision="bf16_jac", tol_abs=1e-6, tol_rel=1e-6, maxiter=300)
+    ref = [(S.clone(), a.clone(), i.iters) for S, a, i in plan.solve()]
+    plan.set_profiling(True)
+    got = plan.solve()
+    n_glm, ms_glm, fl_glm = plan.glm_pass_stats()
+    n_lam, ms_lam, fl_lam = plan.pass_stats()
+    plan.set_profiling(False)
+    for (S1, a1, it1), (S2, a2, i2) in zip(ref, got):
+        assert it1 == i2.iters
+        assert np.array_equal(S1.cpu().numpy(), S2.cpu().numpy())
+        assert np.array_equal(a1.cpu().numpy(), a2.cpu().numpy())
+    assert n_glm > 0 and ms_glm > 0 and fl_glm > 0
+    assert n_lam > 0 and ms_lam > 0 and fl_lam > 0
+    # a single logistic chain (D=3, T=300, N=500) in a window of 300 columns
+    (tgt, ch), = [pairs[0]]
+    one = BatchPlan([ch], window=300, precision="bf16_jac", tol_abs=1e-6, tol_rel=1e-6, maxiter=300)
+    one.set_profiling(True)
+    (_, _, info), = one.solve()
+    n1, _, f1 = one.glm_pass_stats()
+    D, T, N = tgt.dim, ch.Xi.shape[1], tgt.Xd.shape[0]
+    assert n1 >= 2 * info.iters
+    assert 2 * info.iters * 8 * N * D <= f1 <= 2 * info.iters * 8 * N * D * T
+
+
 def _near_threshold_chain(seed, N=2000, D=20, T=768, eps=1e-2):
     """A logistic chain whose tape puts every twelfth column 4e-6 from its
     accept threshold along the FP64 chain (inside the FP32 products' error)."""
