@@ -360,13 +360,21 @@ int pmcmc_tune_entry(int64_t i, int* kind, int* prec, int64_t* D, int32_t* log2T
                      pmcmc_tile_shape* out);
 
 /* ---- many chains in one solve ---------------------------------------------
- * Chains of uneven D and T (logistic or dense-Gaussian targets, D <= 256),
- * each solved window by window (`window` steps; window k starts from the
- * last state of window k-1), all chains' Newton iterations running as one
- * set of batched launches (work items over per-chain descriptor tables;
- * each chain stops at its own convergence), an iteration pair captured as
- * a hipGraph per window round.  params: jacobian = :stoch_diag with one
- * probe, precision F32 or BF16_JAC, maxiter per window; eps per chain.
+ * Chains of uneven D and T (logistic or dense-Gaussian targets, D <= 256).
+ * Each chain iterates over its own sliding window of at most `window`
+ * columns: a window kernel at the start of every iteration places it from
+ * the chain's front (its first column that is not final), and the apply
+ * pass's last workgroup moves the front past the final prefix (tiles whose
+ * residual FT - S is exactly 0), or past the whole window once it meets the
+ * tolerance.  All chains' Newton iterations run as one set of batched
+ * launches (work items over per-chain descriptor tables; each chain stops
+ * at the end of its trajectory).  Between chunks of iterations the host
+ * picks each chain's window length (two tiles while its front moves about
+ * one column per iteration, wider when it moves faster) and rebuilds the
+ * work items; the iterations in between are replayed from a captured
+ * hipGraph (an iteration pair).  params: jacobian = :stoch_diag with one
+ * probe, precision F32 or BF16_JAC, maxiter per window's worth of columns;
+ * eps per chain.
  * The batched counterpart of running src/interface.jl:475-515 once per
  * chain and block; the reference has no batched or windowed solve.
  * All pointers in a descriptor are device pointers (FP32); S_out: D x T. */

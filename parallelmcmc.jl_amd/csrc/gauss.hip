@@ -407,8 +407,26 @@ __device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const in
         for (int j = 0; j < 16; ++j)
             if (!(j < nj && row < ldL)) A[j] = pmcmc_f4v{0.f, 0.f, 0.f, 0.f};
     }
-    // states: column t is V[:, t] (or Xbar: x0, then V[:, t - 1])
+    // the epilogue's operands: the states' x and tape values of the wave's
+    // columns (lane = row of the slab), in flight with Lambda (clamped
+    // addresses; values of dead lanes unused)
     const int Di = (int)D;
+    const int d = r0 + lane;
+    const bool drow = d < Di;
+    const int dc = drow ? d : 0;
+    const bool p0 = a.pass == 0;
+    float xv[8], xiv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int jc = wave + 4 * q;
+        const int t = (int)t0 + (jc < nc ? jc : 0);
+        const float* xp = src.x0 ? (t == 0 ? src.x0 + dc : src.V + ((size_t)(t - 1) * Di + dc))
+                                 : src.V + ((size_t)t * Di + dc);
+        xv[q] = *gp(xp);
+        xiv[q] = gp(p0 ? a.Xi : src.V)[(size_t)t * Di + dc];
+    }
+    const double mud = (mu && drow) ? (double)gp(mu)[dc] : 0.0;
+    // states: column t is V[:, t] (or Xbar: x0, then V[:, t - 1])
     if (src.x0) {
         if (t0 == 0) {
             if (tid < Di) Vs[tid] = gp(src.x0)[tid];
@@ -515,25 +533,9 @@ __device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const in
 
     // outputs, one column per wave and round, lanes over the slab's 64 rows:
     // the wave's columns j = wave + 4 q (q < 8: states, q >= 8: their
-    // directions).  The states' x and tape values are loaded first, all in
-    // flight at once (clamped addresses; values of dead lanes unused).
+    // directions); the states' x and tape values were loaded with Lambda
     const double eps = (double)a.eps;
     const double sq2e = sqrt(2.0 * eps);
-    const int d = r0 + lane;
-    const bool drow = d < Di;
-    const int dc = drow ? d : 0;
-    const bool p0 = a.pass == 0;
-    float xv[8], xiv[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int jc = wave + 4 * q;
-        const int t = (int)t0 + (jc < nc ? jc : 0);
-        const float* xp = src.x0 ? (t == 0 ? src.x0 + dc : src.V + ((size_t)(t - 1) * Di + dc))
-                                 : src.V + ((size_t)t * Di + dc);
-        xv[q] = *gp(xp);
-        xiv[q] = gp(p0 ? a.Xi : src.V)[(size_t)t * Di + dc];
-    }
-    const double mud = (mu && drow) ? (double)gp(mu)[dc] : 0.0;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int j = wave + 4 * q;  // column of the block: 0-31 states, 32-63 directions

@@ -540,6 +540,18 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
     // last workgroup to arrive writes them, after every other one has read
     // them), so plain loads see their current values.
     const int skip_now = a.skip ? *gp(a.skip) : 0;
+    // the gate of the inline FP64 decisions (below), loaded here with the
+    // skip flag so that it arrives with the staging loads instead of one
+    // memory round trip after them; every input is written by earlier
+    // launches, except the sticky flag that tile 0 of this launch may set,
+    // whose other inputs then open the gate for every tile anyway
+    int g_rf = 0, g_li = 0;
+    double g_lm = 0.0;
+    if (MODE == MODE_GEMM && a.rtg != nullptr) {
+        g_rf = *gp(a.rflag);
+        g_lm = *gp(&a.sync->last_metric);
+        g_li = *gp(&a.sync->last_iter);
+    }
     const int64_t D = a.D;
     const int TC = a.TC;
     const int64_t c0 = (int64_t)tile * TC;
@@ -744,11 +756,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
     bool inl_on = false;
     __shared__ int fl_n;
     if (inl) {
-        // the three loads are independent: none waits for another
-        const int rf = *gp(a.rflag);
-        const double lm = *gp(&a.sync->last_metric);
-        const int li = *gp(&a.sync->last_iter);
-        inl_on = (rf != 0) | (lm <= a.gate) | (li + 1 >= a.gate_iters);
+        inl_on = (g_rf != 0) | (g_lm <= a.gate) | (g_li + 1 >= a.gate_iters);
         if (tid == 0) {
             fl_n = 0;
             if (inl_on && tile == 0) *a.rflag = 1;
