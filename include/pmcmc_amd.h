@@ -324,10 +324,11 @@ int pmcmc_deer_adjoint(pmcmc_ctx* ctx, pmcmc_deer_ws* ws, const pmcmc_target* tg
  * form) are looked up in a process-wide table keyed by (kind, precision, D,
  * ceil(log2 T)); without an entry (or with one that does not fit) a fixed
  * rule applies.  single_pass: 0 two-pass reduce/apply, 1 LDS-tiled
- * decoupled look-back, 2 register-resident tiles with look-back (D = 64;
- * tc = 64, 128 or 256 columns, 256 FP32 only; parts unused), 3 the same
- * with persistent workgroups that load the next tile during a tile's
- * look-back (D = 64; tc = 64 or 128).
+ * decoupled look-back, 2 register-resident tiles with look-back
+ * (16 <= D <= 256, as row blocks of at most 64 rows; tc = 64, 128 or 256
+ * columns, 256 FP32 only; parts unused), 3 the same with persistent
+ * workgroups that load the next tile during a tile's look-back (tc = 64 or
+ * 128).
  * pmcmc_scan_tune times candidate shapes of the standalone
  * diagonal scan on synthetic operands; pmcmc_deer_tune times the
  * shape-dependent kernels (tile reduce, spine, tile apply) of `iters`

@@ -5,8 +5,8 @@ reference's scan has no tile shapes (src/DEER/DEERScan.jl:102-173).
 The level-0 tile of the diagonal scan and of the fused DEER tile kernels
 (``tc`` columns per tile, ``parts`` parts per row of the in-tile scan, and
 for the standalone scan the single-pass forms: 1 LDS-tiled look-back, 2
-register-resident tiles with look-back for D = 64, 3 the same with persistent
-workgroups) is looked up in a
+register-resident tiles with look-back for 16 <= D <= 256, 3 the same with
+persistent workgroups) is looked up in a
 process-wide table keyed by (kind, precision, D, ceil(log2 T)); without an
 entry a fixed rule applies.  ``tune_scan`` measures candidate shapes of the
 standalone diagonal scan on synthetic operands, ``tune_solve`` the
@@ -85,8 +85,10 @@ def lookup(kind: str, dtype, D: int, T: int):
 
 def set_entry(kind: str, dtype, D: int, T: int, tc: int, parts: int, single_pass: int = 0, us: float = 0.0):
     """Put a shape in the table (raises if the tile kernels cannot run it).
-    ``single_pass``: 0 two-pass, 1 LDS-tiled look-back, 2 register tiles (D = 64),
-    3 persistent register tiles (D = 64)."""
+    ``single_pass``: 0 two-pass, 1 LDS-tiled look-back, 2 register tiles,
+    3 persistent register tiles (both for 16 <= D <= 256, run as row blocks of
+    at most 64 rows; ``tc`` = 64 or 128 columns, or 256 in FP32 one tile per
+    workgroup)."""
     s = _lib.TileShape(int(tc), int(parts), int(single_pass), 0, float(us))
     rc = _lib.load().pmcmc_tune_set(KINDS[kind], _prec_code(dtype), int(D), int(T), C.byref(s))
     if rc != 0:

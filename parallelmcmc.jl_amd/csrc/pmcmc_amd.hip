@@ -377,11 +377,27 @@ static void tune_put(int kind, int esz, int64_t D, int64_t T, const TileChoice& 
 // tile per workgroup (chained == 2): CPW in {16, 32, 64} (FP32) or {16, 32}
 // (FP64); persistent workgroups with the next tile in registers too
 // (chained == 3): CPW in {16, 32}
+// Row blocks of the register-tile scans for D rows (0: not offered): D is
+// split into ceil(D / 64) blocks of at most 64 rows, each scanned as its own
+// chain of tiles of tc columns, one row per lane.
+static int reg_blocks(int64_t D) { return D >= 16 && D <= 256 ? (int)((D + 63) / 64) : 0; }
+
 template <typename T>
 static bool reg_tile_fits(int64_t D, const TileChoice& c) {
-    if (D != 64) return false;
+    if (!reg_blocks(D)) return false;
     if (c.chained == 3) return c.TC == 64 || c.TC == 128;
     return c.TC == 64 || c.TC == 128 || (c.TC == 256 && sizeof(T) == 4);
+}
+
+template <typename T>
+using RegScanKernel = void (*)(TileArgs<T>);
+
+template <typename T>
+static RegScanKernel<T> reg_scan_kernel(int tc, bool persist) {
+    if (persist) return tc == 64 ? scan_reg_persist_kernel<T, 16> : scan_reg_persist_kernel<T, 32>;
+    if (tc == 64) return scan_reg_kernel<T, 16>;
+    if (tc == 128) return scan_reg_kernel<T, 32>;
+    return scan_reg_kernel<T, sizeof(T) == 4 ? 64 : 32>;
 }
 
 template <typename T>
@@ -667,11 +683,13 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
     }
     if (have && ch.chained >= 2 && reg_tile_fits<T>(D, ch) && Tn > ch.TC) {
         const int TC = ch.TC;
-        const int nT = (int)((Tn + TC - 1) / TC);
+        const int nb = reg_blocks(D);                 // row blocks, each its own chain of tiles
+        const int nT = (int)((Tn + TC - 1) / TC);     // tiles per block
         const int nS = (nT + SCAN_SUPER - 1) / SCAN_SUPER;
-        const size_t lbm = align256((size_t)D * nT * sizeof(T));
-        const size_t sbm = align256((size_t)D * nS * sizeof(T));
-        const size_t nint = (size_t)nT + nS + 64;
+        const int64_t ntk = (int64_t)nb * nT;         // tickets
+        const size_t lbm = align256((size_t)64 * ntk * sizeof(T));
+        const size_t sbm = align256((size_t)64 * nb * nS * sizeof(T));
+        const size_t nint = (size_t)ntk + (size_t)nb * nS + 64;
         const size_t need = 2 * lbm + 3 * sbm + nint * sizeof(int);
         char* blob = nullptr;
         const int rb = scan_scratch(c, need, noalloc, &blob);
@@ -692,28 +710,23 @@ static int scan_diag(pmcmc_ctx* c, const T* A, const T* B, const T* s0, T* S, in
         a.lb_sagg_b = (T*)(blob + 2 * lbm + 2 * sbm);
         int* ints = (int*)(blob + 2 * lbm + 3 * sbm);
         a.lb_status = ints;
-        a.lb_sstatus = ints + nT;
-        a.lb_ticket = (unsigned*)(ints + nT + nS);
-        a.lb_err = (unsigned*)(ints + nT + nS + 16);
+        a.lb_sstatus = ints + ntk;
+        a.lb_ticket = (unsigned*)(ints + ntk + (size_t)nb * nS);
+        a.lb_err = (unsigned*)(ints + ntk + (size_t)nb * nS + 16);
         const int e = noalloc ? -1 : c->prof.begin(KT_ITERATION, 0, c->stream);
         hipLaunchKernelGGL(zero_ints_kernel, dim3(grid_for((int64_t)nint)), dim3(256), 0, c->stream, ints,
                            (int64_t)nint);
+        const auto kern = reg_scan_kernel<T>(TC, ch.chained == 3);
         if (ch.chained == 3) {
             // persistent: no more workgroups than are resident at once
-            auto kern = TC == 64 ? scan_reg64_persist_kernel<T, 16> : scan_reg64_persist_kernel<T, 32>;
             int per_cu = 0, n_cu = 0;
             HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0));
             HIPCHK(c, hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c->device));
             const int64_t resident = (int64_t)std::max(per_cu, 1) * std::max(n_cu, 1);
-            const unsigned grid = (unsigned)std::min<int64_t>(nT, resident);
+            const unsigned grid = (unsigned)std::min<int64_t>(ntk, resident);
             hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, c->stream, a);
-        } else if (TC == 64) {
-            hipLaunchKernelGGL((scan_reg64_kernel<T, 16>), dim3((unsigned)nT), dim3(256), 0, c->stream, a);
-        } else if (TC == 128) {
-            hipLaunchKernelGGL((scan_reg64_kernel<T, 32>), dim3((unsigned)nT), dim3(256), 0, c->stream, a);
         } else {
-            hipLaunchKernelGGL((scan_reg64_kernel<T, sizeof(T) == 4 ? 64 : 32>), dim3((unsigned)nT), dim3(256), 0,
-                               c->stream, a);
+            hipLaunchKernelGGL(kern, dim3((unsigned)ntk), dim3(256), 0, c->stream, a);
         }
         c->prof.end(e, c->stream);
         HIPCHK(c, hipGetLastError());
@@ -1724,7 +1737,7 @@ static std::vector<TileChoice> tile_candidates(int64_t D, int64_t Tn, bool with_
             if (pp == 1) break;
         }
     }
-    if (chained_too && D == 64) {
+    if (chained_too && reg_blocks(D)) {
         for (int form : {2, 3})
             for (int tc : {64, 128, 256}) {
                 TileChoice c;

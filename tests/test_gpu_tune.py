@@ -26,7 +26,19 @@ def _scan_case(rng, D, T):
     return A, B, s0, O.solve_affine_seq(A, B, s0)
 
 
-@pytest.mark.parametrize("D,T", [(64, 20000), (5, 30001), (100, 3000)])
+def _reg_shapes(D, fp64=True):
+    """Register-tile shapes (tc, parts, form) the library offers at D (16 <= D
+    <= 256, split into row blocks of at most 64 rows): tc = 64 or 128 (and 256
+    in FP32, one tile per workgroup)."""
+    if not 16 <= D <= 256:
+        return set()
+    out = {(64, 1, 2), (128, 1, 2), (64, 1, 3), (128, 1, 3)}
+    if not fp64:
+        out.add((256, 1, 2))
+    return out
+
+
+@pytest.mark.parametrize("D,T", [(64, 20000), (5, 30001), (100, 3000), (40, 7001), (256, 4099), (131, 5003)])
 def test_every_candidate_scan_shape_matches_sequential(D, T):
     import torch
 
@@ -42,9 +54,10 @@ def test_every_candidate_scan_shape_matches_sequential(D, T):
             if p <= tc:
                 shapes.add((tc, p, False))
         shapes.add((tc, min(pmax, tc), 1))
-    if D == 64:  # register-resident tiles (FP64: 64 or 128 columns), one per workgroup or persistent
-        shapes.update({(64, 1, 2), (128, 1, 2), (64, 1, 3), (128, 1, 3)})
+    reg = _reg_shapes(D)  # register-resident tiles, one per workgroup or persistent
+    shapes.update(reg)
     checked = 0
+    reg_checked = 0
     for tc, p, sp in sorted(shapes):
         try:
             Tune.set_entry("scan_diag", torch.float64, D, T, tc=tc, parts=p, single_pass=sp)
@@ -53,7 +66,9 @@ def test_every_candidate_scan_shape_matches_sequential(D, T):
         S = DEERScan.solve_affine_scan_diag(A, B, s0).cpu().numpy()
         assert np.max(np.abs(S - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref))), (tc, p, sp)
         checked += 1
+        reg_checked += (tc, p, sp) in reg
     assert checked >= 6
+    assert reg_checked == len(reg)  # every register shape is accepted and checked
 
 
 def test_scan_tune_records_and_applies_its_winner():
@@ -146,20 +161,23 @@ def test_register_tile_scan_long_carry_fp32(tc, form):
     assert np.max(np.abs(S - ref)) <= 1e-4 * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("tc,T,form", [(128, 5000, 2), (256, 64 * 256 * 3 + 17, 2), (64, 1_200_000, 2),
-                                       (128, 5000, 3), (64, 1_200_000, 3), (128, 64 * 128 * 5 + 3, 3)])
-def test_register_tile_scan_exact_running_sum(tc, T, form):
+@pytest.mark.parametrize("D,tc,T,form", [(64, 128, 5000, 2), (64, 256, 64 * 256 * 3 + 17, 2), (64, 64, 1_200_000, 2),
+                                         (64, 128, 5000, 3), (64, 64, 1_200_000, 3), (64, 128, 64 * 128 * 5 + 3, 3),
+                                         (72, 64, 1_100_000, 2), (128, 64, 64 * 64 * 5 + 3, 3),
+                                         (200, 128, 300_001, 3), (256, 256, 20_000, 2), (48, 64, 9_999, 3)])
+def test_register_tile_scan_exact_running_sum(D, tc, T, form):
     """Register-tile scans (form 2: one tile per workgroup; 3: persistent
     workgroups) with A = 1 and integer B (|S| < 2^24, so FP32 is exact): S
     must equal s0 + cumsum(B) exactly.  Covers one partial group of
     tiles (T = 5000, 40 tiles), whole groups, and more than 256 groups of 64
     tiles (T = 1.2e6 at 64 columns per tile: 293 groups, a multi-window
-    look-back)."""
+    look-back); D above 64 runs as row blocks (D = 72: two blocks of 36 rows,
+    more than 256 groups each; 128, 200, 256: two to four blocks), D = 48 has
+    fewer rows than lanes."""
     import torch
 
     from ParallelMCMC import DEERScan, Tune
 
-    D = 64
     rng = np.random.default_rng(T)
     A = np.ones((D, T), dtype=np.float32)
     B = rng.integers(-1, 2, (D, T)).astype(np.float32)

@@ -35,15 +35,20 @@ def test_shapes_that_cannot_run_are_rejected():
         Tune.set_entry("deer", "f32", 4, 4096, tc=2, parts=4)
     with pytest.raises(ArgumentError):  # beyond the LDS budget
         Tune.set_entry("deer", "f32", 100, 1 << 20, tc=4096, parts=1)
-    with pytest.raises(ArgumentError):  # register tiles: D = 64 only
-        Tune.set_entry("scan_diag", "f32", 32, 4096, tc=128, parts=1, single_pass=2)
+    with pytest.raises(ArgumentError):  # register tiles: 16 <= D <= 256
+        Tune.set_entry("scan_diag", "f32", 8, 4096, tc=128, parts=1, single_pass=2)
+    with pytest.raises(ArgumentError):
+        Tune.set_entry("scan_diag", "f32", 300, 4096, tc=64, parts=1, single_pass=2)
     with pytest.raises(ArgumentError):  # 256-column register tiles are FP32 only
         Tune.set_entry("scan_diag", "f64", 64, 4096, tc=256, parts=1, single_pass=2)
     Tune.set_entry("scan_diag", "f64", 64, 4096, tc=128, parts=1, single_pass=2)
     with pytest.raises(ArgumentError):  # persistent register tiles: 64 or 128 columns
         Tune.set_entry("scan_diag", "f32", 64, 4096, tc=256, parts=1, single_pass=3)
-    with pytest.raises(ArgumentError):  # persistent register tiles: D = 64 only
-        Tune.set_entry("scan_diag", "f32", 128, 4096, tc=64, parts=1, single_pass=3)
+    with pytest.raises(ArgumentError):  # register tiles: 64, 128 or 256 columns
+        Tune.set_entry("scan_diag", "f32", 128, 4096, tc=32, parts=1, single_pass=2)
+    Tune.set_entry("scan_diag", "f32", 128, 4096, tc=64, parts=1, single_pass=3)
+    Tune.set_entry("scan_diag", "f32", 100, 4096, tc=128, parts=1, single_pass=2)
+    Tune.set_entry("scan_diag", "f64", 256, 4096, tc=128, parts=1, single_pass=3)
     Tune.set_entry("scan_diag", "f32", 64, 4096, tc=128, parts=1, single_pass=3)
     assert Tune.lookup("scan_diag", "f32", 64, 4096)["single_pass"] == 3
     Tune.clear()
