@@ -260,6 +260,7 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
     __shared__ __align__(16) float Vs[NC * SV];
     __shared__ __align__(16) float Xs[NB * SX];
     __shared__ float Rs[NB * SR];
+    __shared__ float Ys[NB];  // the chunk's labels
     __shared__ double llred[4][16];
 
     if (skip && *(volatile const int*)skip) return;
@@ -276,13 +277,16 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
 
     // design chunk c -> registers (rows beyond N are zero in Xrm, padded to 32)
     float4 pf[PF];
+    float pfy = 0.f;
     auto fetch = [&](int c) {
-        const float4* src4 = reinterpret_cast<const float4*>(Xrm + (size_t)(n_begin + (int64_t)c * NB) * Dp);
+        const int64_t n0 = n_begin + (int64_t)c * NB;
+        const float4* src4 = reinterpret_cast<const float4*>(Xrm + (size_t)n0 * Dp);
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
             const int idx = tid + 256 * i;
             if (idx < F4) pf[i] = src4[idx];
         }
+        if (tid < NB) pfy = n0 + tid < N ? y32[n0 + tid] : 0.f;
     };
     auto stash = [&]() {
 #pragma unroll
@@ -296,6 +300,7 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
                 dst[1] = make_float2(pf[i].z, pf[i].w);
             }
         }
+        if (tid < NB) Ys[tid] = pfy;
     };
 
     if (nchunk > 0) fetch(0);
@@ -350,7 +355,7 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
             const float e = __expf(-fabsf(l));
             const float inv = 1.f / (1.f + e);
             const float sg = l >= 0.f ? inv : e * inv;
-            const float yv = y32[n];
+            const float yv = Ys[row];
             if (valid) ll += (double)(yv * l - (fmaxf(l, 0.f) + __logf(1.f + e)));
             Rs[row * SR + 16 * cx + li] = valid ? (yv - sg) : 0.f;
             const float wgt = sg * (1.f - sg);
@@ -458,6 +463,7 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
     __shared__ __align__(16) float Xs[NB * SX];
     __shared__ float Rs[NB * SR];
     __shared__ __align__(16) __bf16 Ub[CB * SU];
+    __shared__ float Ys[NB];  // the chunk's labels
     __shared__ double llred[4][16];
 
     if (skip && *(volatile const int*)skip) return;
@@ -477,13 +483,16 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
     const int cx = wave & 1;   // GEMM-A column tile
 
     float4 pf[PF];
+    float pfy = 0.f;
     auto fetch = [&](int c) {
-        const float4* src4 = reinterpret_cast<const float4*>(Xrm + (size_t)(n_begin + (int64_t)c * NB) * Dp);
+        const int64_t n0 = n_begin + (int64_t)c * NB;
+        const float4* src4 = reinterpret_cast<const float4*>(Xrm + (size_t)n0 * Dp);
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
             const int idx = tid + 256 * i;
             if (idx < F4) pf[i] = src4[idx];
         }
+        if (tid < NB) pfy = n0 + tid < N ? y32[n0 + tid] : 0.f;
     };
     auto stash = [&]() {
 #pragma unroll
@@ -497,6 +506,7 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
                 dst[1] = make_float2(pf[i].z, pf[i].w);
             }
         }
+        if (tid < NB) Ys[tid] = pfy;
     };
     // BF16 design operands of chunk c: GEMM-A rows 16 rt + li, k = 32 q + 8 lk
     // + e (Xb); GEMM-B d tile rows 16 dt + li, k = rows 8 lk + e of the chunk
@@ -542,6 +552,14 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
     }
     if (nchunk > 0) stash();
     __syncthreads();
+    // GEMM-A B operand: the block's state columns, the same for every chunk
+    constexpr int KA = Dp / 4;
+    constexpr bool VREG = DPT <= 8;
+    float vreg[VREG ? KA : 1];
+    if constexpr (VREG) {
+#pragma unroll
+        for (int ks = 0; ks < KA; ++ks) vreg[ks] = Vs[(16 * cx + li) * SV + 4 * ks + lk];
+    }
 
     v4f accx[DPW][2], accp[DPW][2];
 #pragma unroll
@@ -557,11 +575,21 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
         fetch_xa(c);
         // GEMM-A: state logits (FP32) and direction logits (BF16)
         v4f cl = (v4f){0.f, 0.f, 0.f, 0.f}, cq = cl;
-#pragma unroll 7
-        for (int ks = 0; ks < Dp / 4; ++ks) {
-            const float av = Xs[(16 * rt + li) * SX + 4 * ks + lk];
-            const float bv = Vs[(16 * cx + li) * SV + 4 * ks + lk];
-            cl = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, cl, 0, 0, 0);
+        {
+            // every operand read issued before the products: one LDS latency per
+            // chunk instead of one per pair of MFMAs
+            float ar[KA], br[VREG ? 1 : KA];
+#pragma unroll
+            for (int ks = 0; ks < KA; ++ks) {
+                ar[ks] = Xs[(16 * rt + li) * SX + 4 * ks + lk];
+                if constexpr (!VREG) br[ks] = Vs[(16 * cx + li) * SV + 4 * ks + lk];
+            }
+#pragma unroll
+            for (int ks = 0; ks < KA; ++ks) {
+                float bv;
+                if constexpr (VREG) bv = vreg[ks]; else bv = br[ks];
+                cl = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[ks], bv, cl, 0, 0, 0);
+            }
         }
 #pragma unroll
         for (int q = 0; q < KQ; ++q) cq = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa[q], zb[q], cq, 0, 0, 0);
@@ -576,26 +604,32 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
             const float e = __expf(-fabsf(l));
             const float inv = 1.f / (1.f + e);
             const float sg = l >= 0.f ? inv : e * inv;
-            const float yv = y32[n];
+            const float yv = Ys[row];
             if (valid) ll += (double)(yv * l - (fmaxf(l, 0.f) + __logf(1.f + e)));
             Rs[row * SR + 16 * cx + li] = valid ? (yv - sg) : 0.f;
             u4[r] = (__bf16)(valid ? sg * (1.f - sg) * cq[r] : 0.f);
         }
         *reinterpret_cast<v4bf*>(&Ub[(16 * cx + li) * SU + 16 * rt + 4 * lk]) = u4;
         __syncthreads();
-        // GEMM-B: Xd^T [r | u] over the chunk's 32 rows
+        // GEMM-B: Xd^T [r | u] over the chunk's 32 rows; the residual operands
+        // (the same for every d tile) and each d tile's design values are read
+        // before its products
+        float rr[NB / 4][2];
+#pragma unroll
+        for (int ks = 0; ks < NB / 4; ++ks)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) rr[ks][cc] = Rs[(4 * ks + lk) * SR + 16 * cc + li];
 #pragma unroll
         for (int m = 0; m < DPW; ++m) {
             const int dt = min(wave + 4 * m, DPT - 1);
+            float xr[NB / 4];
 #pragma unroll
-            for (int ks = 0; ks < NB / 4; ++ks) {
-                const float xv = Xs[(4 * ks + lk) * SX + 16 * dt + li];
+            for (int ks = 0; ks < NB / 4; ++ks) xr[ks] = Xs[(4 * ks + lk) * SX + 16 * dt + li];
 #pragma unroll
-                for (int cc = 0; cc < 2; ++cc) {
-                    const float bv = Rs[(4 * ks + lk) * SR + 16 * cc + li];
-                    accx[m][cc] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv, bv, accx[m][cc], 0, 0, 0);
-                }
-            }
+            for (int ks = 0; ks < NB / 4; ++ks)
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc)
+                    accx[m][cc] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[ks], rr[ks][cc], accx[m][cc], 0, 0, 0);
 #pragma unroll
             for (int cc = 0; cc < 2; ++cc) {
                 const v8bf bv = *reinterpret_cast<const v8bf*>(&Ub[(16 * cc + li) * SU + 8 * lk]);
