@@ -236,12 +236,16 @@ void glm_work_destroy(GlmWork* w) {
 // FP32 MFMA fused pass
 // ---------------------------------------------------------------------------
 // grid: (ceil(T/CB), NS); block 256 = 4 waves.
-// LDS: Vs[NC][Dp+4] column block, Xs[NB][Dp+4] design chunk, Rs[NB][NC+4]
-// (53 KB at D = 100: three workgroups per CU).  The next design chunk is
-// fetched into registers (float4) while the current one is multiplied, then
-// stored over it once both GEMMs are done with it: three barriers per chunk,
-// global latency off the critical path.  Wave w owns column tile w (and
-// w + 4, ...) for every d tile.
+// LDS: Vs[NC][Dp+4] column block, Xs[NB][Dp+4] design chunk, Rs[NC][NB+4]
+// residual tile (column-major), Ys[NB] the chunk's labels (54 KB at D = 100:
+// three workgroups per CU).  The next design chunk and its labels are
+// fetched into registers while the current one is multiplied, then stored
+// over it once both GEMMs are done with it: three barriers per chunk, global
+// latency off the critical path.  Both GEMMs read 16-byte runs along k: MFMA
+// step 4 j + i of a 16-wide k block takes k = 16 j + 4 lk + i from lane group
+// lk (A and B rows of the GEMM-A from Xs / Vs, the GEMM-B residuals from Rs),
+// so each k block is four MFMAs per 16-byte read.  Wave w owns column tile w
+// (and w + 4, ...) for every d tile.
 template <int DPT, int K>
 __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const float* Xrm, const float* y32,
                                               int64_t N, int64_t rows_per_split, float* part, double* llpart,
@@ -251,7 +255,7 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
     constexpr int NC = CB * (1 + K);
     constexpr int SV = Dp + 4;
     constexpr int SX = Dp + 4;
-    constexpr int SR = NC + 4;
+    constexpr int SR = NB + 4;  // residual tile, [column][row]: 16-byte row reads hit distinct banks
     constexpr int NCT = NC / 16;             // column tiles
     static_assert(NCT % 4 == 0, "column tiles must divide over the 4 waves");
     constexpr int CT_PER_WAVE = NCT / 4;
@@ -259,7 +263,7 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
     constexpr int PF = (F4 + 255) / 256;     // float4 per thread per chunk
     __shared__ __align__(16) float Vs[NC * SV];
     __shared__ __align__(16) float Xs[NB * SX];
-    __shared__ float Rs[NB * SR];
+    __shared__ __align__(16) float Rs[NC * SR];
     __shared__ float Ys[NB];  // the chunk's labels
     __shared__ double llred[4][16];
 
@@ -333,19 +337,24 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
         v4f cl[1 + K];
 #pragma unroll
         for (int s = 0; s <= K; ++s) cl[s] = (v4f){0.f, 0.f, 0.f, 0.f};
-#pragma unroll 7
-        for (int ks = 0; ks < Dp / 4; ++ks) {
-            const float av = Xs[(16 * rt + li) * SX + 4 * ks + lk];
+        // step 4 j + i takes k = 16 j + 4 lk + i from lane group lk (16-byte reads)
+#pragma unroll
+        for (int j = 0; j < Dp / 16; ++j) {
+            const float4 a4 = *reinterpret_cast<const float4*>(&Xs[(16 * rt + li) * SX + 16 * j + 4 * lk]);
+            const float av[4] = {a4.x, a4.y, a4.z, a4.w};
 #pragma unroll
             for (int s = 0; s <= K; ++s) {
                 const int col = s * CB + 16 * cx + li;
-                const float bv = Vs[col * SV + 4 * ks + lk];
-                cl[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, cl[s], 0, 0, 0);
+                const float4 b4 = *reinterpret_cast<const float4*>(&Vs[col * SV + 16 * j + 4 * lk]);
+                const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cl[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[i], cl[s], 0, 0, 0);
             }
         }
         // elementwise: residuals and sigmoid'-weighted directions into LDS.
         // With e = exp(-|l|): softplus(l) = max(l,0) + log(1+e),
         // sigmoid(l) = (l >= 0 ? 1 : e) / (1+e), log-lik = y l - softplus(l).
+        v4f rv[1 + K];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * rt + 4 * lk + r;
@@ -357,27 +366,37 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
             const float sg = l >= 0.f ? inv : e * inv;
             const float yv = Ys[row];
             if (valid) ll += (double)(yv * l - (fmaxf(l, 0.f) + __logf(1.f + e)));
-            Rs[row * SR + 16 * cx + li] = valid ? (yv - sg) : 0.f;
+            rv[0][r] = valid ? (yv - sg) : 0.f;
             const float wgt = sg * (1.f - sg);
 #pragma unroll
-            for (int s = 1; s <= K; ++s) Rs[row * SR + s * CB + 16 * cx + li] = valid ? wgt * cl[s][r] : 0.f;
+            for (int s = 1; s <= K; ++s) rv[s][r] = valid ? wgt * cl[s][r] : 0.f;
         }
+        // rows 16 rt + 4 lk .. + 3 of column 16 cx + li of each set: one 16-byte store
+#pragma unroll
+        for (int s = 0; s <= K; ++s) *reinterpret_cast<v4f*>(&Rs[(s * CB + 16 * cx + li) * SR + 16 * rt + 4 * lk]) = rv[s];
         __syncthreads();
-        // GEMM-B: accumulate Xd^T R
+        // GEMM-B: accumulate Xd^T R; step 4 j + i takes chunk row k = 16 j + 4 lk + i
+        // from lane group lk (the residual tile as 16-byte reads)
 #pragma unroll
-        for (int ks = 0; ks < NB / 4; ++ks) {
-            float xa[DPT];
-#pragma unroll
-            for (int dt = 0; dt < DPT; ++dt) xa[dt] = Xs[(4 * ks + lk) * SX + 16 * dt + li];
+        for (int j = 0; j < NB / 16; ++j) {
+            v4f rb[CT_PER_WAVE];
 #pragma unroll
             for (int cc = 0; cc < CT_PER_WAVE; ++cc) {
                 // NCT is a multiple of 4: every wave owns CT_PER_WAVE tiles (no branch,
                 // so the accumulators stay in place across k steps)
                 const int ct = wave + 4 * cc;
-                const float bv = Rs[(4 * ks + lk) * SR + 16 * ct + li];
+                rb[cc] = *reinterpret_cast<const v4f*>(&Rs[(16 * ct + li) * SR + 16 * j + 4 * lk]);
+            }
 #pragma unroll
-                for (int dt = 0; dt < DPT; ++dt)
-                    acc[cc][dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[dt], bv, acc[cc][dt], 0, 0, 0);
+            for (int i = 0; i < 4; ++i) {
+                float xa[DPT];
+#pragma unroll
+                for (int dt = 0; dt < DPT; ++dt) xa[dt] = Xs[(16 * j + 4 * lk + i) * SX + 16 * dt + li];
+#pragma unroll
+                for (int cc = 0; cc < CT_PER_WAVE; ++cc)
+#pragma unroll
+                    for (int dt = 0; dt < DPT; ++dt)
+                        acc[cc][dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[dt], rb[cc][i], acc[cc][dt], 0, 0, 0);
             }
         }
         if (c + 1 < nchunk) {
@@ -432,9 +451,12 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
 //   * the directions' sigmoid'-weighted logits u, written to LDS as BF16 in
 //     [column][row] order, so that a lane's eight k values are one 16-byte
 //     read.
-// The FP32 design chunk and the 32 state columns go through LDS as in the
-// FP32 pass.  The values and their summation order are those of rounding
-// the FP32 operands to BF16 inside the products.
+// The FP32 design chunk, its labels and the 32 state columns go through LDS
+// as in the FP32 pass, with the same 16-byte k runs and the same summation
+// order, so the states' products equal the FP32 pass's bit for bit; each
+// wave keeps its GEMM-A state column (the same for every chunk) in registers
+// up to D = 128.  The directions' values and summation order are those of
+// rounding the FP32 operands to BF16 inside the products.
 // Waves: GEMM-A, wave (rt, cx) owns row tile rt and column tile cx of the
 // states and of the directions; GEMM-B, wave w owns d tiles w, w + 4, ...
 // for all four column tiles (two of states, two of directions).
@@ -453,7 +475,7 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
     constexpr int KQ = Dq / 32;
     constexpr int SV = Dp + 4;
     constexpr int SX = Dp + 4;
-    constexpr int SR = CB + 4;  // states' residual, FP32 [row][column]
+    constexpr int SR = NB + 4;  // states' residual, FP32 [column][row]: 16-byte row reads hit distinct banks
     constexpr int SU = NB + 8;  // directions' u, BF16 [column][row]: 80-byte rows, so the
                                 // 16-byte reads of 8 consecutive lanes cover distinct banks
     constexpr int F4 = NB * Dp / 4;
@@ -461,7 +483,7 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
     constexpr int DPW = (DPT + 3) / 4;
     __shared__ __align__(16) float Vs[CB * SV];
     __shared__ __align__(16) float Xs[NB * SX];
-    __shared__ float Rs[NB * SR];
+    __shared__ __align__(16) float Rs[CB * SR];
     __shared__ __align__(16) __bf16 Ub[CB * SU];
     __shared__ float Ys[NB];  // the chunk's labels
     __shared__ double llred[4][16];
@@ -558,7 +580,13 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
     float vreg[VREG ? KA : 1];
     if constexpr (VREG) {
 #pragma unroll
-        for (int ks = 0; ks < KA; ++ks) vreg[ks] = Vs[(16 * cx + li) * SV + 4 * ks + lk];
+        for (int j = 0; j < KA / 4; ++j) {
+            const float4 v4 = *reinterpret_cast<const float4*>(&Vs[(16 * cx + li) * SV + 16 * j + 4 * lk]);
+            vreg[4 * j] = v4.x;
+            vreg[4 * j + 1] = v4.y;
+            vreg[4 * j + 2] = v4.z;
+            vreg[4 * j + 3] = v4.w;
+        }
     }
 
     v4f accx[DPW][2], accp[DPW][2];
@@ -578,11 +606,22 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
         {
             // every operand read issued before the products: one LDS latency per
             // chunk instead of one per pair of MFMAs
+            // step 4 j + i takes k = 16 j + 4 lk + i from lane group lk (16-byte reads)
             float ar[KA], br[VREG ? 1 : KA];
 #pragma unroll
-            for (int ks = 0; ks < KA; ++ks) {
-                ar[ks] = Xs[(16 * rt + li) * SX + 4 * ks + lk];
-                if constexpr (!VREG) br[ks] = Vs[(16 * cx + li) * SV + 4 * ks + lk];
+            for (int j = 0; j < KA / 4; ++j) {
+                const float4 a4 = *reinterpret_cast<const float4*>(&Xs[(16 * rt + li) * SX + 16 * j + 4 * lk]);
+                ar[4 * j] = a4.x;
+                ar[4 * j + 1] = a4.y;
+                ar[4 * j + 2] = a4.z;
+                ar[4 * j + 3] = a4.w;
+                if constexpr (!VREG) {
+                    const float4 b4 = *reinterpret_cast<const float4*>(&Vs[(16 * cx + li) * SV + 16 * j + 4 * lk]);
+                    br[4 * j] = b4.x;
+                    br[4 * j + 1] = b4.y;
+                    br[4 * j + 2] = b4.z;
+                    br[4 * j + 3] = b4.w;
+                }
             }
 #pragma unroll
             for (int ks = 0; ks < KA; ++ks) {
@@ -595,6 +634,7 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
         for (int q = 0; q < KQ; ++q) cq = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa[q], zb[q], cq, 0, 0, 0);
         // elementwise: the states' residual (FP32) and the directions' u (BF16)
         v4bf u4;
+        v4f r4;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * rt + 4 * lk + r;
@@ -606,30 +646,33 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
             const float sg = l >= 0.f ? inv : e * inv;
             const float yv = Ys[row];
             if (valid) ll += (double)(yv * l - (fmaxf(l, 0.f) + __logf(1.f + e)));
-            Rs[row * SR + 16 * cx + li] = valid ? (yv - sg) : 0.f;
+            r4[r] = valid ? (yv - sg) : 0.f;
             u4[r] = (__bf16)(valid ? sg * (1.f - sg) * cq[r] : 0.f);
         }
+        *reinterpret_cast<v4f*>(&Rs[(16 * cx + li) * SR + 16 * rt + 4 * lk]) = r4;
         *reinterpret_cast<v4bf*>(&Ub[(16 * cx + li) * SU + 16 * rt + 4 * lk]) = u4;
         __syncthreads();
         // GEMM-B: Xd^T [r | u] over the chunk's 32 rows; the residual operands
         // (the same for every d tile) and each d tile's design values are read
         // before its products
-        float rr[NB / 4][2];
+        // (step 4 j + i takes chunk row k = 16 j + 4 lk + i from lane group lk)
+        v4f rr[NB / 16][2];
 #pragma unroll
-        for (int ks = 0; ks < NB / 4; ++ks)
+        for (int j = 0; j < NB / 16; ++j)
 #pragma unroll
-            for (int cc = 0; cc < 2; ++cc) rr[ks][cc] = Rs[(4 * ks + lk) * SR + 16 * cc + li];
+            for (int cc = 0; cc < 2; ++cc)
+                rr[j][cc] = *reinterpret_cast<const v4f*>(&Rs[(16 * cc + li) * SR + 16 * j + 4 * lk]);
 #pragma unroll
         for (int m = 0; m < DPW; ++m) {
             const int dt = min(wave + 4 * m, DPT - 1);
             float xr[NB / 4];
 #pragma unroll
-            for (int ks = 0; ks < NB / 4; ++ks) xr[ks] = Xs[(4 * ks + lk) * SX + 16 * dt + li];
+            for (int ks = 0; ks < NB / 4; ++ks) xr[ks] = Xs[(16 * (ks >> 2) + 4 * lk + (ks & 3)) * SX + 16 * dt + li];
 #pragma unroll
             for (int ks = 0; ks < NB / 4; ++ks)
 #pragma unroll
                 for (int cc = 0; cc < 2; ++cc)
-                    accx[m][cc] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[ks], rr[ks][cc], accx[m][cc], 0, 0, 0);
+                    accx[m][cc] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[ks], rr[ks >> 2][cc][ks & 3], accx[m][cc], 0, 0, 0);
 #pragma unroll
             for (int cc = 0; cc < 2; ++cc) {
                 const v8bf bv = *reinterpret_cast<const v8bf*>(&Ub[(16 * cc + li) * SU + 8 * lk]);
