@@ -236,16 +236,17 @@ void glm_work_destroy(GlmWork* w) {
 // FP32 MFMA fused pass
 // ---------------------------------------------------------------------------
 // grid: (ceil(T/CB), NS); block 256 = 4 waves.
-// LDS: Vs[NC][Dp+4] column block, Xs[NB][Dp+4] design chunk, Rs[NC][NB+4]
-// residual tile (column-major), Ys[NB] the chunk's labels (54 KB at D = 100:
-// three workgroups per CU).  The next design chunk and its labels are
-// fetched into registers while the current one is multiplied, then stored
-// over it once both GEMMs are done with it: three barriers per chunk, global
-// latency off the critical path.  Both GEMMs read 16-byte runs along k: MFMA
-// step 4 j + i of a 16-wide k block takes k = 16 j + 4 lk + i from lane group
-// lk (A and B rows of the GEMM-A from Xs / Vs, the GEMM-B residuals from Rs),
-// so each k block is four MFMAs per 16-byte read.  Wave w owns column tile w
-// (and w + 4, ...) for every d tile.
+// LDS: Vs[NC][Dp+4] column block, Xs[NB][Dp+4] design chunk, Rs[NB][NC+4]
+// residual tile, Ys[NB] the chunk's labels (53.4 KB at D = 100: three
+// workgroups per CU).  The next design chunk and its labels are fetched into
+// registers while the current one is multiplied, then stored over it once
+// both GEMMs are done with it: three barriers per chunk, global latency off
+// the critical path.  MFMA step 4 j + i of a 16-wide k block takes
+// k = 16 j + 4 lk + i from lane group lk: the GEMM-A reads each lane's four
+// k values of Xs and Vs as one 16-byte read, and the GEMM-B's lane groups
+// read rows 4 apart of Xs and Rs, which fall in distinct banks.  The BF16
+// pass sums its states' products in the same order.  Wave w owns column tile
+// w (and w + 4, ...) for every d tile.
 template <int DPT, int K>
 __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const float* Xrm, const float* y32,
                                               int64_t N, int64_t rows_per_split, float* part, double* llpart,
@@ -255,7 +256,7 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
     constexpr int NC = CB * (1 + K);
     constexpr int SV = Dp + 4;
     constexpr int SX = Dp + 4;
-    constexpr int SR = NB + 4;  // residual tile, [column][row]: 16-byte row reads hit distinct banks
+    constexpr int SR = NC + 4;  // residual tile, [row][column]
     constexpr int NCT = NC / 16;             // column tiles
     static_assert(NCT % 4 == 0, "column tiles must divide over the 4 waves");
     constexpr int CT_PER_WAVE = NCT / 4;
@@ -263,9 +264,11 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
     constexpr int PF = (F4 + 255) / 256;     // float4 per thread per chunk
     __shared__ __align__(16) float Vs[NC * SV];
     __shared__ __align__(16) float Xs[NB * SX];
-    __shared__ __align__(16) float Rs[NC * SR];
+    __shared__ __align__(16) float Rs[NB * SR];
     __shared__ float Ys[NB];  // the chunk's labels
-    __shared__ double llred[4][16];
+    // the log-likelihood reduction reuses Rs after the chunk loop (whose last
+    // barrier ends every read of it)
+    double (*llred)[16] = reinterpret_cast<double (*)[16]>(Rs);
 
     if (skip && *(volatile const int*)skip) return;
     const int tid = threadIdx.x;
@@ -290,7 +293,9 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
             const int idx = tid + 256 * i;
             if (idx < F4) pf[i] = src4[idx];
         }
-        if (tid < NB) pfy = n0 + tid < N ? y32[n0 + tid] : 0.f;
+        // rows past N read row N - 1 (their products are masked): no write to
+        // pfy before the load, which would wait for the loads just issued
+        if (tid < NB) pfy = y32[n0 + tid < N ? n0 + tid : N - 1];
     };
     auto stash = [&]() {
 #pragma unroll
@@ -371,32 +376,28 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
 #pragma unroll
             for (int s = 1; s <= K; ++s) rv[s][r] = valid ? wgt * cl[s][r] : 0.f;
         }
-        // rows 16 rt + 4 lk .. + 3 of column 16 cx + li of each set: one 16-byte store
 #pragma unroll
-        for (int s = 0; s <= K; ++s) *reinterpret_cast<v4f*>(&Rs[(s * CB + 16 * cx + li) * SR + 16 * rt + 4 * lk]) = rv[s];
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int s = 0; s <= K; ++s) Rs[(16 * rt + 4 * lk + r) * SR + s * CB + 16 * cx + li] = rv[s][r];
         __syncthreads();
         // GEMM-B: accumulate Xd^T R; step 4 j + i takes chunk row k = 16 j + 4 lk + i
         // from lane group lk (the residual tile as 16-byte reads)
 #pragma unroll
-        for (int j = 0; j < NB / 16; ++j) {
-            v4f rb[CT_PER_WAVE];
+        for (int ks = 0; ks < NB / 4; ++ks) {
+            const int kr = 16 * (ks >> 2) + 4 * lk + (ks & 3);  // rows 4 apart: distinct banks
+            float xa[DPT];
+#pragma unroll
+            for (int dt = 0; dt < DPT; ++dt) xa[dt] = Xs[kr * SX + 16 * dt + li];
 #pragma unroll
             for (int cc = 0; cc < CT_PER_WAVE; ++cc) {
                 // NCT is a multiple of 4: every wave owns CT_PER_WAVE tiles (no branch,
                 // so the accumulators stay in place across k steps)
                 const int ct = wave + 4 * cc;
-                rb[cc] = *reinterpret_cast<const v4f*>(&Rs[(16 * ct + li) * SR + 16 * j + 4 * lk]);
-            }
+                const float bv = Rs[kr * SR + 16 * ct + li];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                float xa[DPT];
-#pragma unroll
-                for (int dt = 0; dt < DPT; ++dt) xa[dt] = Xs[(16 * j + 4 * lk + i) * SX + 16 * dt + li];
-#pragma unroll
-                for (int cc = 0; cc < CT_PER_WAVE; ++cc)
-#pragma unroll
-                    for (int dt = 0; dt < DPT; ++dt)
-                        acc[cc][dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[dt], rb[cc][i], acc[cc][dt], 0, 0, 0);
+                for (int dt = 0; dt < DPT; ++dt)
+                    acc[cc][dt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[dt], bv, acc[cc][dt], 0, 0, 0);
             }
         }
         if (c + 1 < nchunk) {
@@ -514,7 +515,9 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
             const int idx = tid + 256 * i;
             if (idx < F4) pf[i] = src4[idx];
         }
-        if (tid < NB) pfy = n0 + tid < N ? y32[n0 + tid] : 0.f;
+        // rows past N read row N - 1 (their products are masked): no write to
+        // pfy before the load, which would wait for the loads just issued
+        if (tid < NB) pfy = y32[n0 + tid < N ? n0 + tid : N - 1];
     };
     auto stash = [&]() {
 #pragma unroll
