@@ -418,13 +418,22 @@ static bool deer_tuned(int esz, int64_t D, int64_t T, TileChoice* ch) {
     return D16 != D && tune_get(PMCMC_TUNE_DEER, esz, D16, T, ch);
 }
 
-// level-0 tile of one window (L columns) of a batched chain.  The rule, not
-// the single-chain tuner's entries: the narrower tiles that win for one chain
-// lose in a batch (measured on 32 uneven chains: 31.7 s against 28.9 s), where
-// the chains already fill the GPU and more tiles lengthen each chain's
-// aggregate scan.
+// level-0 tile of one window (L columns) of a batched chain: about 2048
+// elements (half the single-chain rule's), not the single-chain tuner's
+// entries.  Once only a few causally converging chains are left, each
+// iteration is a launch over two or four tiles per chain, and a tile's
+// reduce is one workgroup's chain of dependent FP64 work over its columns:
+// half the elements per tile halve each thread's share of it.  On 256
+// uneven chains: 6.48 s per solve at 2048 elements, 7.35 s at 4096, 6.74 s at
+// 3072, 6.99 s at 1536; 1024 (tiles of 4 columns at D = 256) took 15.8 s.
 static void batch_tile_shape(int64_t D, int64_t L, int* TC, int* P, int* nT) {
     tile_shape<float>(D, L, true, TC, P, nT);
+    const int64_t tc = std::max<int64_t>(1, std::min<int64_t>(*TC, 2048 / D));
+    *TC = (int)tc;
+    *P = D >= PMCMC_NT ? 1 : (int)(PMCMC_NT / D);
+    if (*P > *TC) *P = *TC;
+    if (*P < 1) *P = 1;
+    *nT = (int)((L + tc - 1) / tc);
 }
 
 template <typename T>
@@ -2443,11 +2452,13 @@ extern "C" int pmcmc_batch_destroy(pmcmc_batch* b) {
 
 // Window length of a chain whose front only moves about one column per
 // iteration (the causal progress of a Newton iteration whose Jacobian
-// estimate misses the dynamics): two tiles past the front, since the
-// columns further ahead cannot settle before the front reaches them.  On
-// 256 uneven chains (128 of them causal) a 32-column floor measured 8.08 s
-// per solve, against 8.35 s at 64, 10.6 s at 128 and 12.5 s at 256: a wider
-// window only adds work to every iteration.
+// estimate misses the dynamics): at least two tiles and 32 columns past the
+// front, since the columns further ahead cannot settle before the front
+// reaches them.  On 256 uneven chains (128 of them causal), with 4096-element
+// tiles a 32-column floor measured 8.08 s per solve, against 8.35 s at 64,
+// 10.6 s at 128 and 12.5 s at 256: a wider window only adds work to every
+// iteration; with 2048-element tiles, 6.48 s at 32 against 6.83 s at 48,
+// and 12.9 s and 14.6 s at 16 and 24.
 static int narrow_span(int64_t L, int TC) {
     static const int base = [] {
         const char* e = getenv("PMCMC_SLIDE_NARROW");
