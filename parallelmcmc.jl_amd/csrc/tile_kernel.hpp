@@ -775,10 +775,14 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         constexpr int RM = PMCMC_FAST_RM;  // rows per lane of the fast path
         const bool fast = MODE == MODE_GEMM && K == 1 && D <= RM * 64;
         int G = 1;
-        if (fast)
+        if (fast) {
             while (G * RM < D) G <<= 1;
-        else
+            // wider groups while the groups still cover the tile's columns in
+            // one round: fewer rows per lane on the chain of dependent FP64 work
+            while (G < 64 && PMCMC_NT / (2 * G) >= tc) G <<= 1;
+        } else {
             while (G < D && G < 64) G <<= 1;
+        }
         const int lane = tid & (G - 1);
         const int grp = tid / G;
         const int ngrp = PMCMC_NT / G;
@@ -824,6 +828,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
             // rows beyond D clamped to a valid row (their values unused)
 #pragma unroll
             for (int i = 0; i < RM; ++i) {
+                if ((int64_t)i * G >= D) break;  // the same for every lane: rows past D unused
                 const int64_t d = lane + (int64_t)i * G;
                 const size_t gi = (size_t)t * D + (d < D ? d : D - 1);
                 f.gx[i] = gp(a.gx)[gi];
