@@ -907,6 +907,8 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                 T* acol = al + (size_t)c * D;
                 double s_qyx = 0, s_qxy = 0;
                 double d1[PMCMC_MAXP] = {0, 0, 0, 0}, d2[PMCMC_MAXP] = {0, 0, 0, 0}, d3[PMCMC_MAXP] = {0, 0, 0, 0};
+                static_assert(RM <= 32, "one sign bit per row of the lane");
+                uint32_t zneg = 0;  // probe signs of the lane's rows (bit i: row i is -1), hashed once
 #pragma unroll
                 for (int i = 0; i < RM; ++i) {
                     const int64_t d = lane + (int64_t)i * G;
@@ -914,7 +916,9 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                         const double x = (double)xcol[d];
                         const size_t gi = (size_t)t * D + d;
                         const double gx = (double)f.gx[i], y = (double)f.y[i], gy = (double)f.gy[i];
-                        const double z = (double)rademacher(keys[0], gi + (size_t)a.zoff);
+                        const float zf = rademacher(keys[0], gi + (size_t)a.zoff);
+                        zneg |= (zf < 0.f ? 1u : 0u) << i;
+                        const double z = (double)zf;
                         const double w = (double)f.w[i], hy = (double)f.hw[i];
                         const double r = x - y - eps * gy;
                         const double dr = z - w - eps * hy;
@@ -948,7 +952,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                             acol[d] = (T)((acc ? 1.0 + eps * (double)f.hw[i] : 1.0) +
                                           gg * ((double)y - (double)x) * (double)f.w[i]);
                         } else {
-                            const T z = (T)rademacher(keys[0], (size_t)t * D + d + (size_t)a.zoff);
+                            const T z = (zneg >> i) & 1u ? (T)-1 : (T)1;
                             const T jt = (acc ? f.w[i] : z) + (y - x) * dgt;
                             acol[d] = z * jt;
                         }
