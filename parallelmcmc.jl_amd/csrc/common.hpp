@@ -209,4 +209,59 @@ struct DevTraits<double> {
     static constexpr int tile_elems = 2048;
 };
 
+// One contiguous span of ncol columns (column-major, D floats each, from
+// src) into LDS columns dst[col * SV + k] (col < ncol, k < D): 16-byte
+// loads, every one in flight before the first LDS write.
+template <int SV>
+__device__ __forceinline__ void span_to_lds(float* dst, const float* src, int D, int ncol, int tid) {
+    const int n = D * ncol;  // at most 32 x 256
+    if (n <= 0) return;
+    const uintptr_t addr = (uintptr_t)src;
+    int head = (int)(((16 - (addr & 15)) & 15) / 4);
+    if (head > n) head = n;
+    const int nv = (n - head) / 4;
+    const float* sv = src + head;
+    // element i of the span -> column i / D, row i % D (32-bit, once per vector)
+    auto put4 = [&](int i0, const float* v, int cnt) {
+        int col = (int)((unsigned)i0 / (unsigned)D);
+        int k = i0 - col * D;
+        for (int e = 0; e < cnt; ++e) {
+            dst[col * SV + k] = v[e];
+            if (++k == D) {
+                k = 0;
+                ++col;
+            }
+        }
+    };
+    for (int base = 0; base < nv || base == 0; base += 8 * 256) {
+        // every load is issued (clamped to a valid address), then the writes
+        pmcmc_f4v r[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            int idx = base + tid + 256 * q;
+            idx = idx < nv ? idx : (nv > 0 ? nv - 1 : 0);
+            r[q] = nv > 0 ? gld4(sv + 4 * idx) : pmcmc_f4v{0.f, 0.f, 0.f, 0.f};
+        }
+        const int ti = head + nv * 4 + tid;
+        float hv = 0.f, tv = 0.f;
+        if (base == 0) {
+            hv = gp(src)[tid < head ? tid : 0];
+            tv = gp(src)[ti < n ? ti : 0];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int idx = base + tid + 256 * q;
+            if (idx < nv) {
+                const float v[4] = {r[q].x, r[q].y, r[q].z, r[q].w};
+                put4(head + 4 * idx, v, 4);
+            }
+        }
+        if (base == 0) {
+            if (tid < head) put4(tid, &hv, 1);
+            if (ti < n) put4(ti, &tv, 1);
+        }
+    }
+}
+
+
 }  // namespace pmcmc

@@ -232,6 +232,37 @@ void glm_work_destroy(GlmWork* w) {
     *w = GlmWork{};
 }
 
+// The block's state columns t0 .. t0 + CB - 1 of a column source (set 0:
+// V[:, t], or with x0 the iterate shifted by one step) into LDS columns
+// Vs[j * SV + d], zero beyond T and for d >= D up to Dp: the columns are one
+// contiguous span of V, read with 16-byte loads all in flight at once (one
+// load round trip, where a load per element would cost one per element a
+// thread stages).  Needs a barrier before Vs is read.
+template <int SV>
+__device__ __forceinline__ void stage_state_cols(float* Vs, const ColSrc<float>& src, int64_t t0, int CB, int Dp,
+                                                 int tid) {
+    const int D = (int)src.D;
+    const int64_t T = src.Tn;
+    const int nc = t0 < T ? (int)(T - t0 < CB ? T - t0 : CB) : 0;
+    if (nc > 0) {
+        if (src.x0) {
+            if (t0 == 0) {
+                for (int d = tid; d < D; d += 256) Vs[d] = src.x0[d];
+                span_to_lds<SV>(Vs + SV, src.V, D, nc - 1, tid);
+            } else {
+                span_to_lds<SV>(Vs, src.V + (size_t)(t0 - 1) * D, D, nc, tid);
+            }
+        } else {
+            span_to_lds<SV>(Vs, src.V + (size_t)t0 * D, D, nc, tid);
+        }
+    }
+    // zeros: rows D .. Dp - 1 of every column, every row of columns past T
+    for (int i = tid; i < CB * Dp; i += 256) {
+        const int j = i / Dp, d = i % Dp;
+        if (d >= D || j >= nc) Vs[j * SV + d] = 0.f;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // FP32 MFMA fused pass
 // ---------------------------------------------------------------------------
@@ -313,9 +344,11 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
     };
 
     if (nchunk > 0) fetch(0);
-    // 1. column block into LDS (zero padded)
-    for (int i = tid; i < NC * Dp; i += 256) {
-        const int j = i / Dp, d = i % Dp;
+    // 1. column block into LDS (zero padded): the states as one span, the
+    // direction sets element by element
+    stage_state_cols<SV>(Vs, src, cb * CB, CB, Dp, tid);
+    for (int i = tid; i < (NC - CB) * Dp; i += 256) {
+        const int j = CB + i / Dp, d = i % Dp;
         const int set = j / CB;
         const int64_t t = cb * CB + (j % CB);
         float v = 0.f;
@@ -556,13 +589,7 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
         fetch_xt(0);
     }
     // states of the block into LDS (zero padded)
-    for (int i = tid; i < CB * Dp; i += 256) {
-        const int j = i / Dp, d = i % Dp;
-        const int64_t t = cb * CB + j;
-        float v = 0.f;
-        if (t < T && d < D) v = (float)src.get(0, t, d);
-        Vs[j * SV + d] = v;
-    }
+    stage_state_cols<SV>(Vs, src, cb * CB, CB, Dp, tid);
     // directions of column 16 cx + li, rounded once: the GEMM-A B operand
     v8bf zb[KQ];
     {
