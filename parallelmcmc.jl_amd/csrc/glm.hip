@@ -352,7 +352,9 @@ __device__ __forceinline__ void glm_pass_body(const ColSrc<float>& src, const fl
         const int set = j / CB;
         const int64_t t = cb * CB + (j % CB);
         float v = 0.f;
-        if (t < T && d < D) v = (float)src.get(set, t, d);
+        if (t < T && d < D)
+            v = src.Z ? (float)src.get(set, t, d)
+                      : rademacher(src.keys[set - 1], (uint64_t)t * (uint64_t)D + (uint64_t)d + (uint64_t)src.zoff);
         Vs[j * SV + d] = v;
     }
     if (nchunk > 0) stash();
@@ -594,13 +596,25 @@ __device__ __forceinline__ void glm_pass_bf16_body(const ColSrc<float>& src, con
     v8bf zb[KQ];
     {
         const int64_t t = cb * CB + 16 * cx + li;
+        if (!src.Z) {
+            // the probe: +-1, exact in BF16; hashed straight from its element index
+            const uint64_t base = (uint64_t)t * (uint64_t)D + (uint64_t)src.zoff;
 #pragma unroll
-        for (int q = 0; q < KQ; ++q)
+            for (int q = 0; q < KQ; ++q)
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int64_t d = 32 * q + 8 * lk + e;
-                zb[q][e] = (__bf16)((t < T && d < D) ? (float)src.get(1, t, d) : 0.f);
-            }
+                for (int e = 0; e < 8; ++e) {
+                    const int d = 32 * q + 8 * lk + e;
+                    zb[q][e] = (__bf16)((t < T && d < D) ? rademacher(src.keys[0], base + (uint64_t)d) : 0.f);
+                }
+        } else {
+#pragma unroll
+            for (int q = 0; q < KQ; ++q)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int64_t d = 32 * q + 8 * lk + e;
+                    zb[q][e] = (__bf16)((t < T && d < D) ? (float)src.get(1, t, d) : 0.f);
+                }
+        }
     }
     if (nchunk > 0) stash();
     __syncthreads();
