@@ -211,16 +211,48 @@ struct DevTraits<double> {
 
 // One contiguous span of ncol columns (column-major, D floats each, from
 // src) into LDS columns dst[col * SV + k] (col < ncol, k < D): 16-byte
-// loads, every one in flight before the first LDS write.
-template <int SV>
-__device__ __forceinline__ void span_to_lds(float* dst, const float* src, int D, int ncol, int tid) {
-    const int n = D * ncol;  // at most 32 x 256
-    if (n <= 0) return;
+// loads, every one of a round (up to 8 x 256 vectors) in flight before the
+// first LDS write.  span_issue / span_store split a round in two, so that a
+// kernel can put other loads (and VALU work) between them; span_to_lds is
+// both, round by round.
+struct SpanRegs {
+    pmcmc_f4v r[8];
+    float hv, tv;
+    int head, nv, n;
+};
+
+__device__ __forceinline__ void span_issue(SpanRegs& s, const float* src, int D, int ncol, int tid, int base = 0) {
+    s.n = D * ncol;
+    if (s.n <= 0) {
+        s.head = s.nv = 0;
+        return;
+    }
     const uintptr_t addr = (uintptr_t)src;
     int head = (int)(((16 - (addr & 15)) & 15) / 4);
-    if (head > n) head = n;
-    const int nv = (n - head) / 4;
+    if (head > s.n) head = s.n;
+    s.head = head;
+    s.nv = (s.n - head) / 4;
     const float* sv = src + head;
+    // only the span's own vectors are read (a wave past the span issues
+    // nothing): clamped duplicates would still pass through the CU's address
+    // and data path, which a short kernel's loads saturate
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int idx = base + tid + 256 * q;
+        s.r[q] = pmcmc_f4v{0.f, 0.f, 0.f, 0.f};
+        if (idx < s.nv) s.r[q] = gld4(sv + 4 * idx);
+    }
+    s.hv = s.tv = 0.f;
+    if (base == 0) {
+        const int ti = head + s.nv * 4 + tid;
+        if (tid < head) s.hv = gp(src)[tid];
+        if (ti < s.n) s.tv = gp(src)[ti];
+    }
+}
+
+template <int SV>
+__device__ __forceinline__ void span_store(float* dst, const SpanRegs& s, int D, int tid, int base = 0) {
+    if (s.n <= 0) return;
     // element i of the span -> column i / D, row i % D (32-bit, once per vector)
     auto put4 = [&](int i0, const float* v, int cnt) {
         int col = (int)((unsigned)i0 / (unsigned)D);
@@ -233,33 +265,29 @@ __device__ __forceinline__ void span_to_lds(float* dst, const float* src, int D,
             }
         }
     };
-    for (int base = 0; base < nv || base == 0; base += 8 * 256) {
-        // every load is issued (clamped to a valid address), then the writes
-        pmcmc_f4v r[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            int idx = base + tid + 256 * q;
-            idx = idx < nv ? idx : (nv > 0 ? nv - 1 : 0);
-            r[q] = nv > 0 ? gld4(sv + 4 * idx) : pmcmc_f4v{0.f, 0.f, 0.f, 0.f};
+    for (int q = 0; q < 8; ++q) {
+        const int idx = base + tid + 256 * q;
+        if (idx < s.nv) {
+            const float v[4] = {s.r[q].x, s.r[q].y, s.r[q].z, s.r[q].w};
+            put4(s.head + 4 * idx, v, 4);
         }
-        const int ti = head + nv * 4 + tid;
-        float hv = 0.f, tv = 0.f;
-        if (base == 0) {
-            hv = gp(src)[tid < head ? tid : 0];
-            tv = gp(src)[ti < n ? ti : 0];
-        }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int idx = base + tid + 256 * q;
-            if (idx < nv) {
-                const float v[4] = {r[q].x, r[q].y, r[q].z, r[q].w};
-                put4(head + 4 * idx, v, 4);
-            }
-        }
-        if (base == 0) {
-            if (tid < head) put4(tid, &hv, 1);
-            if (ti < n) put4(ti, &tv, 1);
-        }
+    }
+    if (base == 0) {
+        const int ti = s.head + s.nv * 4 + tid;
+        if (tid < s.head) put4(tid, &s.hv, 1);
+        if (ti < s.n) put4(ti, &s.tv, 1);
+    }
+}
+
+template <int SV>
+__device__ __forceinline__ void span_to_lds(float* dst, const float* src, int D, int ncol, int tid) {
+    SpanRegs s;
+    span_issue(s, src, D, ncol, tid);
+    span_store<SV>(dst, s, D, tid);
+    for (int base = 8 * 256; base < s.nv; base += 8 * 256) {
+        span_issue(s, src, D, ncol, tid, base);
+        span_store<SV>(dst, s, D, tid, base);
     }
 }
 

@@ -342,16 +342,14 @@ __device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const in
 
     pmcmc_f4v A[16];
     {
-        // every load in flight at once (clamped addresses, zeros selected after)
+        // every load in flight at once: the K steps j < nj only (the products
+        // read no others; nj is even, so the BF16 pairs stay inside).  Rows
+        // past ldL read row 0: they only make product rows past D, which are
+        // never stored.
         const int rowc = row < ldL ? row : 0;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int jc = j < nj ? j : 0;
-            A[j] = gld4(a.L32 + (size_t)rowc * ldL + 16 * jc + 4 * lk);
-        }
-#pragma unroll
         for (int j = 0; j < 16; ++j)
-            if (!(j < nj && row < ldL)) A[j] = pmcmc_f4v{0.f, 0.f, 0.f, 0.f};
+            if (j < nj) A[j] = gld4(a.L32 + (size_t)rowc * ldL + 16 * j + 4 * lk);
     }
     // the epilogue's operands: the states' x and tape values of the wave's
     // columns (lane = row of the slab), in flight with Lambda (clamped
@@ -371,26 +369,35 @@ __device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const in
         xv[q] = *gp(xp);
         xiv[q] = gp(p0 ? a.Xi : src.V)[(size_t)t * Di + dc];
     }
-    const double mud = (mu && drow) ? (double)gp(mu)[dc] : 0.0;
-    // states: column t is V[:, t] (or Xbar: x0, then V[:, t - 1])
-    if (src.x0) {
-        if (t0 == 0) {
-            if (tid < Di) Vs[tid] = gp(src.x0)[tid];
-            span_to_lds<SV>(Vs + SV, src.V, Di, nc - 1, tid);
-        } else {
-            span_to_lds<SV>(Vs, src.V + (size_t)(t0 - 1) * D, Di, nc, tid);
-        }
+    // loaded unconditionally (any valid row when there is no mean) and
+    // selected and converted in the epilogue: doing either here would wait
+    // for every load issued so far
+    const float muf = gp(mu ? mu : src.V)[dc];
+    // states: column t is V[:, t] (or Xbar: x0, then V[:, t - 1]); given
+    // directions: Z's columns.  Both spans are issued with Lambda's loads and
+    // written to LDS after the probe hashing, one memory round trip in all.
+    SpanRegs sps, spz;
+    const bool xfirst = src.x0 && t0 == 0;
+    float x0v = 0.f;
+    if (xfirst) {
+        x0v = gp(src.x0)[tid < Di ? tid : 0];
+        span_issue(sps, src.V, Di, nc - 1, tid);
     } else {
-        span_to_lds<SV>(Vs, src.V + (size_t)t0 * D, Di, nc, tid);
+        span_issue(sps, src.V + (size_t)(src.x0 ? t0 - 1 : t0) * D, Di, nc, tid);
     }
-    // directions: the given matrix, or the probe of the block's absolute columns
-    if (src.Z) {
-        span_to_lds<SV>(Vd, src.Z + (size_t)t0 * D, Di, nc, tid);
-    } else if (tid < Di) {
+    if (src.Z) span_issue(spz, src.Z + (size_t)t0 * D, Di, nc, tid);
+    const float muk = (mu && tid < Di) ? gp(mu)[tid] : 0.f;
+    // directions without a matrix: the probe of the block's absolute columns
+    if (!src.Z && tid < Di)
         for (int col = 0; col < nc; ++col)
             Vd[col * SV + tid] = rademacher(src.keys[0], (size_t)(t0 + col) * D + tid + (size_t)src.zoff);
+    if (xfirst) {
+        if (tid < Di) Vs[tid] = x0v;
+        span_store<SV>(Vs + SV, sps, Di, tid);
+    } else {
+        span_store<SV>(Vs, sps, Di, tid);
     }
-    const float muk = (mu && tid < Di) ? gp(mu)[tid] : 0.f;
+    if (src.Z) span_store<SV>(Vd, spz, Di, tid);
     // zeros where the MFMAs read beyond the data: rows k in [D, ldL), columns >= nc
     if (tid < ldL)
         for (int col = 0; col < CPB; ++col)
@@ -482,13 +489,26 @@ __device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const in
     // directions); the states' x and tape values were loaded with Lambda
     const double eps = (double)a.eps;
     const double sq2e = sqrt(2.0 * eps);
+    const double mud = mu ? (double)muf : 0.0;
+    // the eight state columns' log-density sums over the slab's rows run as
+    // one interleaved butterfly after the loop (each column's own order of
+    // additions unchanged), not as eight chains of lane exchanges in turn
+    double lpq[8];
+    // every LDS operand of the loop read first (one round trip; the empty asm
+    // keeps the compiler from sinking each read into its column's branch)
+    float gq[16], wq[8];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) gq[q] = -Pt[(wave + 4 * q) * SP + lane];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) wq[q] = Vd[(wave + 4 * q) * SV + dc];
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int j = wave + 4 * q;  // column of the block: 0-31 states, 32-63 directions
         const int jc = j & (CPB - 1);
         const int t = (int)t0 + jc;
         const bool live = jc < nc;
-        const float g = -Pt[j * SP + lane];
+        const float g = gq[q];
         const size_t gi = (size_t)t * Di + d;
         if (q < 8) {
             double lpp = 0.0;
@@ -498,16 +518,21 @@ __device__ __forceinline__ void gauss_slab_body(const GaussPassArgs& a, const in
                 gp(a.gout)[gi] = g;
                 if (p0) gp(a.Yout)[gi] = (float)(x + eps * g + sq2e * (double)xiv[q]);
             }
-#pragma unroll
-            for (int m = 32; m >= 1; m >>= 1) lpp += __shfl_xor(lpp, m, 64);
-            if (lane == 0) lpw[jc] = lpp;
+            lpq[q] = lpp;
         } else if (live && drow) {
             if (p0)
-                gp(a.Wout)[gi] = (float)((double)Vd[jc * SV + d] + eps * g);
+                gp(a.Wout)[gi] = (float)((double)wq[q - 8] + eps * g);
             else
                 gp(a.hout)[gi] = g;
         }
     }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) lpq[q] += __shfl_xor(lpq[q], m, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) lpw[wave + 4 * q] = lpq[q];
     __syncthreads();
     if (tid < nc) gp(a.lpout)[(size_t)slab * a.lp_stride + t0 + tid] = lpw[tid];
     if (a.flops && tid == 0) {
