@@ -113,12 +113,8 @@ __global__ __launch_bounds__(256) void adj_update_kernel(const T* Dr, T* lam, in
             m0 = nan_max(m0, red[0][w]);
             m1 = nan_max(m1, red[1][w]);
         }
-        atomicMax(&sy->dmax_bits, (unsigned long long)__double_as_longlong(m0));
-        atomicMax(&sy->smax_bits, (unsigned long long)__double_as_longlong(m1));
-        const unsigned prev = __hip_atomic_fetch_add(&sy->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == gridDim.x - 1u) {
-            const double dm = __longlong_as_double((long long)atomicAdd(&sy->dmax_bits, 0ull));
-            const double sm = __longlong_as_double((long long)atomicAdd(&sy->smax_bits, 0ull));
+        double dm, sm;
+        if (sync_arrive(sy, gridDim.x, m0, m1, -1, &dm, &sm)) {
             const double metric = dm / (tol_abs + tol_rel * sm);
             sy->last_metric = metric;
             sy->last_iter = iter;
@@ -126,9 +122,6 @@ __global__ __launch_bounds__(256) void adj_update_kernel(const T* Dr, T* lam, in
                 sy->conv_iter = iter;
                 sy->converged = 1;
             }
-            sy->dmax_bits = 0ull;
-            sy->smax_bits = 0ull;
-            sy->done = 0u;
         }
     }
 }

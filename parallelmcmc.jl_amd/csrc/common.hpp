@@ -63,6 +63,45 @@ struct SyncState {
     double smax_prefix;  // max |S| over final columns that the current tiling no longer covers
 };
 
+// A workgroup's arrival at the end of an iteration's output pass (one thread
+// per workgroup).  It hands its maxima of |S_new - S| and |S_new| (and, with
+// live_col >= 0, the start of its tile, which still has a nonzero residual)
+// to the last workgroup to arrive; only that one returns true, with the
+// iteration's maxima read and reset.  Everything handed over is an
+// agent-scope atomic, coherent across the XCDs' L2s, so the arrival needs no
+// L2 write-back or invalidation (an acq_rel arrival costs both, in every
+// workgroup): the atomics return their old values, so once the wave has
+// waited for them (vmcnt(0)) they have been performed, and only then does the
+// arrival count.  Anything else the last workgroup reads must have been
+// written by an earlier launch or by its own thread.
+__device__ inline bool sync_arrive(SyncState* sy, unsigned nblocks, double dmax, double smax, int live_col,
+                                   double* dm, double* sm) {
+    int o2 = 0;
+    if (live_col >= 0) o2 = __hip_atomic_fetch_min(&sy->live_from, live_col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // non-negative doubles order like their bit patterns; NaN sorts above +Inf
+    const unsigned long long o0 = __hip_atomic_fetch_max(&sy->dmax_bits, (unsigned long long)__double_as_longlong(dmax),
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long o1 = __hip_atomic_fetch_max(&sy->smax_bits, (unsigned long long)__double_as_longlong(smax),
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    asm volatile("" ::"v"(o0), "v"(o1), "v"(o2));  // keeps the returning forms
+    __builtin_amdgcn_s_waitcnt(0x0F70);            // vmcnt(0); expcnt and lgkmcnt not waited on
+    const unsigned prev = __hip_atomic_fetch_add(&sy->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev != nblocks - 1u) return false;
+    // read and reset at once: the next launch's first atomics see zero
+    *dm = __longlong_as_double(
+        (long long)__hip_atomic_exchange(&sy->dmax_bits, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    *sm = __longlong_as_double(
+        (long long)__hip_atomic_exchange(&sy->smax_bits, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    __hip_atomic_store(&sy->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// The last workgroup's read and reset of live_from (0x7fffffff: no tile of the
+// iteration had a nonzero residual).
+__device__ inline int sync_take_live_from(SyncState* sy) {
+    return __hip_atomic_exchange(&sy->live_from, 0x7fffffff, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---------------------------------------------------------------------------
 // Sliding window of one chain of a batched solve.  The Newton iteration runs
 // over the columns [front, hi) of the chain's whole trajectory, hi =

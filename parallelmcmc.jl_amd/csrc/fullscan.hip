@@ -524,17 +524,15 @@ __device__ inline void full_apply_finish(const FullArgs<T>& p, double dmax, doub
     const bool any_nz = __any(nz ? 1 : 0) != 0;
     if (lane == 0) {
         SyncState* sy = p.sy;
+        int live_col = -1;
         if (!final_tile) {
             p.tile_smax[blockIdx.x] = smax;
-            if (any_nz) atomicMin(&sy->live_from, (int)(p.col0 + (int64_t)blockIdx.x * p.TC));
+            if (any_nz) live_col = (int)(p.col0 + (int64_t)blockIdx.x * p.TC);
         }
-        atomicMax(&sy->dmax_bits, (unsigned long long)__double_as_longlong(dmax));
-        atomicMax(&sy->smax_bits, (unsigned long long)__double_as_longlong(smax));
-        const unsigned prev = __hip_atomic_fetch_add(&sy->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == (unsigned)p.nT - 1u) {
-            const double dm = __longlong_as_double((long long)atomicAdd(&sy->dmax_bits, 0ull));
-            const double sm = nan_max(__longlong_as_double((long long)atomicAdd(&sy->smax_bits, 0ull)),
-                                      sy->smax_prefix);
+        double dm, smt;
+        if (sync_arrive(sy, (unsigned)p.nT, dmax, smax, live_col, &dm, &smt)) {
+            // smax_prefix: written by an earlier launch
+            const double sm = nan_max(smt, sy->smax_prefix);
             // src/DEER/DEER.jl:567-577
             const double metric = dm / (p.tol_abs + p.tol_rel * sm);
             sy->last_metric = metric;
@@ -544,12 +542,8 @@ __device__ inline void full_apply_finish(const FullArgs<T>& p, double dmax, doub
                 sy->converged = 1;
             }
             // tiles before the first one with a nonzero residual are final
-            const int lf = __hip_atomic_load(&sy->live_from, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int lf = sync_take_live_from(sy);
             sy->skip_cols = lf == INT_MAX ? (int)(p.col0 + (int64_t)p.nT * p.TC) : lf;
-            sy->live_from = INT_MAX;
-            sy->dmax_bits = 0ull;
-            sy->smax_bits = 0ull;
-            sy->done = 0u;
         }
     }
 }

@@ -580,25 +580,20 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                 m0 = nan_max(m0, s_red[0][w]);
                 m1 = nan_max(m1, s_red[1][w]);
             }
+            int live_col = -1;
             if (a.final_cols && !final_tile) {
                 a.tile_smax[tile] = m1;
-                if (any_nz) atomicMin(&sy->live_from, (int)c0);
+                if (any_nz) live_col = (int)c0;
             }
-            // non-negative doubles order like their bit patterns; NaN sorts above +Inf
-            atomicMax(&sy->dmax_bits, (unsigned long long)__double_as_longlong(m0));
-            atomicMax(&sy->smax_bits, (unsigned long long)__double_as_longlong(m1));
-            // the last workgroup closes the iteration: metric and convergence flag.
-            // acq_rel: this block's maxima are visible before its arrival counts,
-            // and the last arrival sees every block's maxima.
-            const unsigned prev = __hip_atomic_fetch_add(&sy->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            if (prev == (unsigned)a.numTiles - 1u) {
-                const double dm = __longlong_as_double((long long)atomicAdd(&sy->dmax_bits, 0ull));
-                const double sm = __longlong_as_double((long long)atomicAdd(&sy->smax_bits, 0ull));
+            // the last workgroup closes the iteration: metric and convergence flag
+            double dm, sm;
+            if (sync_arrive(sy, (unsigned)a.numTiles, m0, m1, live_col, &dm, &sm)) {
                 const double metric = dm / (a.tol_abs + a.tol_rel * sm);
                 const int it = a.iter >= 0 ? a.iter : sy->last_iter + 1;
                 sy->last_metric = metric;
                 sy->last_iter = it;
-                const bool dec_final = !a.refine_on || *(volatile const int*)a.refine_on;
+                const bool dec_final =
+                    !a.refine_on || __hip_atomic_load(a.refine_on, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (!a.win && metric <= 1.0 && dec_final) {
                     sy->conv_iter = it;
                     sy->converged = 1;
@@ -609,8 +604,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                     // whole window when it meets the tolerance; the window stays
                     // where it is while decisions may still change with the gate
                     SlideState* w = a.win;
-                    const int lf = __hip_atomic_load(&sy->live_from, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    sy->live_from = INT_MAX;
+                    const int lf = sync_take_live_from(sy);
                     // this iteration's window kernel initialised the window's new
                     // columns and copied the previous acceptance: both are done
                     if (w->front + a.T_len > w->hw) w->hw = w->front + a.T_len;
@@ -639,13 +633,9 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
                 } else if (a.final_cols) {
                     // tiles before the first one with a nonzero residual are final,
                     // once decisions no longer change with the override's gate
-                    const int lf = __hip_atomic_load(&sy->live_from, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const int lf = sync_take_live_from(sy);
                     if (dec_final) sy->skip_cols = lf == INT_MAX ? (int)((int64_t)a.numTiles * TC) : lf;
-                    sy->live_from = INT_MAX;
                 }
-                sy->dmax_bits = 0ull;
-                sy->smax_bits = 0ull;
-                sy->done = 0u;
             }
         }
     };
@@ -759,7 +749,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         inl_on = (g_rf != 0) | (g_lm <= a.gate) | (g_li + 1 >= a.gate_iters);
         if (tid == 0) {
             fl_n = 0;
-            if (inl_on && tile == 0) *a.rflag = 1;
+            if (inl_on && tile == 0) __hip_atomic_store(a.rflag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     __syncthreads();
