@@ -166,11 +166,11 @@ int glm_data_set_prior(GlmData* g, hipStream_t s, bool src_f64, const void* P, s
 static constexpr int CB = GLM_CB;  // t-columns per column block of the MFMA pass
 static constexpr int NB = 32;   // design rows per LDS chunk
 
-static int glm_splits(int64_t N, int64_t T) {
-    // enough workgroups to fill 256 CUs three times (three fit per CU), chunks
-    // of >= 8 row blocks each
+static int glm_splits(int64_t N, int64_t T, int64_t target = 768) {
+    // enough workgroups to fill 256 CUs three times (three fit per CU; a
+    // batch's pass sets a smaller target), chunks of >= 8 row blocks each
     const int64_t nblk = (T + CB - 1) / CB;
-    int64_t ns = (768 + nblk - 1) / nblk;
+    int64_t ns = (target + nblk - 1) / nblk;
     const int64_t nchunks = (N + NB - 1) / NB;
     if (ns > nchunks / 8) ns = nchunks / 8;
     if (ns < 1) ns = 1;
@@ -178,15 +178,15 @@ static int glm_splits(int64_t N, int64_t T) {
     return (int)ns;
 }
 
-void glm_split_plan(const GlmData* g, int64_t Tn, int* ns, int64_t* rows_per_split) {
-    *ns = glm_splits(g->N, Tn);
+void glm_split_plan(const GlmData* g, int64_t Tn, int* ns, int64_t* rows_per_split, int target_wgs) {
+    *ns = glm_splits(g->N, Tn, target_wgs);
     const int64_t nch = (g->N + NB - 1) / NB;
     *rows_per_split = ((nch + *ns - 1) / *ns) * NB;
 }
 
 void launch_glm_pass_batched(int dpt, bool jbf, const GlmPassArgs* args, const int4* items, int nitems,
                              hipStream_t s);
-void launch_glm_reduce_batched(const GlmReduceArgs* args, const int2* items, int nitems, hipStream_t s);
+void launch_glm_reduce_batched(const GlmReduceArgs* args, const int4* items, int nitems, hipStream_t s);
 
 int glm_work_ensure(GlmWork* w, const GlmData* g, hipStream_t, int64_t D, int64_t T, int probes,
                     bool f64, std::string* msg) {
@@ -782,14 +782,21 @@ __global__ __launch_bounds__(256, 2) void glm_pass_bf16_kernel(ColSrc<float> src
                             blockIdx.y);
 }
 
-// Batched over chains of one padded width: work item blockIdx.x is
-// (chain, column block, N split) = items[blockIdx.x].
+// Batched over chains of one padded width: work item blockIdx.x is (chain,
+// column block, N split, split count) = items[blockIdx.x]; the splits cover
+// the chain's row chunks evenly (the split count is chosen per item list).
+__device__ inline int64_t batched_split_rows(int64_t N, int ns) {
+    const int64_t nch = (N + NB - 1) / NB;
+    return (nch + ns - 1) / ns * NB;
+}
+
 template <int DPT, int K>
 __global__ __launch_bounds__(256) void glm_pass_mfma_batched(const GlmPassArgs* __restrict__ args, const int4* __restrict__ items) {
     const int4 w = items[blockIdx.x];
     const GlmPassArgs& a = args[w.x];
     if ((int64_t)w.y * CB >= a.src.Tn) return;  // beyond the chain's current window
-    glm_pass_body<DPT, K>(a.src, a.Xrm, a.y32, a.N, a.rows_per_split, a.part, a.llpart, a.skip, a.flops, w.y, w.z);
+    glm_pass_body<DPT, K>(a.src, a.Xrm, a.y32, a.N, batched_split_rows(a.N, w.w), a.part, a.llpart, a.skip, a.flops,
+                          w.y, w.z);
 }
 
 template <int DPT>
@@ -797,8 +804,8 @@ __global__ __launch_bounds__(256, 2) void glm_pass_bf16_batched(const GlmPassArg
     const int4 w = items[blockIdx.x];
     const GlmPassArgs& a = args[w.x];
     if ((int64_t)w.y * CB >= a.src.Tn) return;  // beyond the chain's current window
-    glm_pass_bf16_body<DPT>(a.src, a.Xrm, a.Xb, a.Xtb, a.Np, a.y32, a.N, a.rows_per_split, a.part, a.llpart, a.skip,
-                            a.flops, w.y, w.z);
+    glm_pass_bf16_body<DPT>(a.src, a.Xrm, a.Xb, a.Xtb, a.Np, a.y32, a.N, batched_split_rows(a.N, w.w), a.part,
+                            a.llpart, a.skip, a.flops, w.y, w.z);
 }
 
 // Sum the splits, add the prior terms, and form the next pass's columns.
@@ -869,11 +876,12 @@ __global__ __launch_bounds__(256) void glm_reduce_kernel(ColSrc<T> src, const P*
                           Wout, skip, blockIdx.x);
 }
 
-// Batched: work item blockIdx.x is (chain, block of 4 columns) = items[blockIdx.x].
-__global__ __launch_bounds__(256) void glm_reduce_batched(const GlmReduceArgs* __restrict__ args, const int2* __restrict__ items) {
-    const int2 w = items[blockIdx.x];
+// Batched: work item blockIdx.x is (chain, block of 4 columns, the pass's
+// split count) = items[blockIdx.x].
+__global__ __launch_bounds__(256) void glm_reduce_batched(const GlmReduceArgs* __restrict__ args, const int4* __restrict__ items) {
+    const int4 w = items[blockIdx.x];
     const GlmReduceArgs& a = args[w.x];
-    glm_reduce_body<float, float>(a.src, a.part, a.llpart, a.ns, a.Dp, a.llstride, a.pass, a.inv_s2, a.eps, a.P,
+    glm_reduce_body<float, float>(a.src, a.part, a.llpart, w.z, a.Dp, a.llstride, a.pass, a.inv_s2, a.eps, a.P,
                                   a.Xi, a.gout, a.hout, a.lpout, a.Yout, a.Wout, a.skip, w.y);
 }
 
@@ -1070,7 +1078,7 @@ void launch_glm_pass_batched(int dpt, bool jbf, const GlmPassArgs* args, const i
     }
 }
 
-void launch_glm_reduce_batched(const GlmReduceArgs* args, const int2* items, int nitems, hipStream_t s) {
+void launch_glm_reduce_batched(const GlmReduceArgs* args, const int4* items, int nitems, hipStream_t s) {
     if (nitems > 0) hipLaunchKernelGGL(glm_reduce_batched, dim3((unsigned)nitems), dim3(256), 0, s, args, items);
 }
 

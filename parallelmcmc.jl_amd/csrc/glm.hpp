@@ -85,8 +85,7 @@ struct GlmPassArgs {
     int64_t Np;
     const float* y32;
     int64_t N;
-    int64_t rows_per_split;
-    float* part;
+    float* part;  // split partial sums (the split count comes with each work item)
     double* llpart;
     const int* skip;
     // non-null: every workgroup adds its algorithmic flops (8 x rows x D x
@@ -99,7 +98,6 @@ struct GlmReduceArgs {
     ColSrc<float> src;
     const float* part;
     const double* llpart;
-    int ns;
     int64_t Dp, llstride;
     int pass;
     double inv_s2, eps;
@@ -114,13 +112,14 @@ struct GlmReduceArgs {
 };
 
 // N-split plan of the FP32 GEMM pass over Tn columns (ns splits of
-// rows_per_split rows); the pass's partial sums take ns * 2 * Tn * Dp floats
-// followed by ns * Tn doubles of log-likelihood partials.
-void glm_split_plan(const GlmData* g, int64_t Tn, int* ns, int64_t* rows_per_split);
+// rows_per_split rows, enough for about target_wgs workgroups); the pass's
+// partial sums take ns * 2 * Tn * Dp floats followed by ns * Tn doubles of
+// log-likelihood partials.
+void glm_split_plan(const GlmData* g, int64_t Tn, int* ns, int64_t* rows_per_split, int target_wgs = 768);
 // Batched FP32 passes (one probe set) over chains of padded width 16 * dpt.
 void launch_glm_pass_batched(int dpt, bool jbf, const GlmPassArgs* args, const int4* items, int nitems,
                              hipStream_t s);
-void launch_glm_reduce_batched(const GlmReduceArgs* args, const int2* items, int nitems, hipStream_t s);
+void launch_glm_reduce_batched(const GlmReduceArgs* args, const int4* items, int nitems, hipStream_t s);
 
 int glm_data_create(GlmData* g, hipStream_t s, int64_t N, int64_t D, bool src_f64, const void* Xd,
                     const void* y, std::string* msg);

@@ -2333,8 +2333,7 @@ struct BatchChain {
     float *agg_a = nullptr, *agg_b = nullptr, *prev = nullptr;
     double* tile_smax = nullptr;
     int8_t* prev_dec = nullptr;  // T
-    int ns = 1;
-    int64_t rows_per_split = 0;
+    int ns = 1;  // the most N splits of the GEMM passes (sizes the partial sums)
     int TC = 0, P = 0, nTmax = 0;
     int nslab = 1;  // 64-row slabs of the Lambda pass (dense Gaussian), else 1
     RefineTarget rtg{};
@@ -2390,7 +2389,8 @@ namespace {
 static constexpr int NLDS = 3;
 static int lds_class(size_t lds) { return lds <= 16384 ? 0 : (lds <= 32768 ? 1 : 2); }
 struct Items {
-    std::vector<int2> red[NLDS], glm_red;
+    std::vector<int2> red[NLDS];
+    std::vector<int4> glm_red;
     std::vector<int> multi, wins;
     std::vector<int4> glm[17];
     std::vector<int4> gslab;  // (chain, column block, slab) of the Lambda passes
@@ -2556,7 +2556,8 @@ extern "C" int pmcmc_batch_create(pmcmc_ctx* c, int32_t n, const pmcmc_chain_des
         }
         sz += align256(refine_chain_bytes(h.rtg, L, h.band));
         if (d.target->kind == TGT_LOGISTIC) {
-            glm_split_plan(&d.target->glm, L, &h.ns, &h.rows_per_split);
+            int64_t rps;
+            glm_split_plan(&d.target->glm, L, &h.ns, &rps);
             sz += align256((size_t)h.ns * 2 * L * h.Dp * sizeof(float) + (size_t)h.ns * L * sizeof(double));
         }
         off[i] = bytes;
@@ -2770,7 +2771,6 @@ static void batch_tables(pmcmc_batch* b, BatchTables* bt) {
                 ga.Np = g.Np;
                 ga.y32 = g.y;
                 ga.N = g.N;
-                ga.rows_per_split = h.rows_per_split;
                 ga.part = h.part;
                 ga.llpart = llpart;
                 ga.skip = skip;
@@ -2780,7 +2780,6 @@ static void batch_tables(pmcmc_batch* b, BatchTables* bt) {
                 gr.src = ga.src;
                 gr.part = h.part;
                 gr.llpart = llpart;
-                gr.ns = h.ns;
                 gr.Dp = h.Dp;
                 gr.llstride = L;
                 gr.pass = v < 2 ? 0 : 1;
@@ -2855,6 +2854,18 @@ static void batch_tables(pmcmc_batch* b, BatchTables* bt) {
 
 static void build_items(const pmcmc_batch* b, const std::vector<char>& live, Items* it) {
     const int n = (int)b->ch.size();
+    // N splits of the GEMM passes: sized to fill the GPU from each chain (768
+    // workgroups) while few GEMM column blocks are live, to 512 per chain while
+    // at least 512 are, since every split adds a partial sum of the window
+    // that the pass writes and the reduce reads (the plan's split count, the
+    // most, sets the buffers).  256 uneven chains: 5.91 s at 768 throughout,
+    // 5.69 s at 512 throughout; their 32-chain subset at damping 0.5, where
+    // a few logistic chains stay live for long, 23.5 s and 28.5 s.
+    int64_t gemm_blocks = 0;
+    for (int k = 0; k < n; ++k)
+        if (live[k] && b->ch[k].tgt->kind == TGT_LOGISTIC)
+            gemm_blocks += (std::min<int64_t>(b->ch[k].span, b->ch[k].L) + GLM_CB - 1) / GLM_CB;
+    const int split_target = gemm_blocks >= 512 ? 512 : 768;
     for (int k = 0; k < n; ++k) {
         if (!live[k]) continue;
         const BatchChain& h = b->ch[k];
@@ -2876,9 +2887,13 @@ static void build_items(const pmcmc_batch* b, const std::vector<char>& live, Ite
             it->r_glm.push_back(k);
             it->r_split_glm = std::max(it->r_split_glm, h.rw.nsplit);
             const int ncb = (int)((span + GLM_CB - 1) / GLM_CB);
+            int nsk;
+            int64_t rps;
+            glm_split_plan(&h.tgt->glm, span, &nsk, &rps, split_target);
+            nsk = std::min(nsk, h.ns);
             for (int cb = 0; cb < ncb; ++cb)
-                for (int sp = 0; sp < h.ns; ++sp) it->glm[dpt].push_back(make_int4(k, cb, sp, 0));
-            for (int64_t blk = 0; blk < (span + 3) / 4; ++blk) it->glm_red.push_back(make_int2(k, (int)blk));
+                for (int sp = 0; sp < nsk; ++sp) it->glm[dpt].push_back(make_int4(k, cb, sp, nsk));
+            for (int64_t blk = 0; blk < (span + 3) / 4; ++blk) it->glm_red.push_back(make_int4(k, (int)blk, nsk, 0));
         } else {
             it->r_gauss.push_back(k);
             it->r_split_gauss = std::max(it->r_split_gauss, h.rw.nsplit);
@@ -3055,7 +3070,7 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
                                       (int)it.gslab.size(), s);
             b->prof.end(pe, s);
         }
-        launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[par]), (const int2*)ia(o_glmred),
+        launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[par]), (const int4*)ia(o_glmred),
                                   (int)it.glm_red.size(), s);
         for (int d = 1; d <= 16; ++d) {
             const int pg = it.glm[d].empty() ? -1 : b->prof.begin(KT_GLM_PASS, 0, s);
@@ -3069,7 +3084,7 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
                                       (int)it.gslab.size(), s);
             b->prof.end(pe, s);
         }
-        launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[2]), (const int2*)ia(o_glmred),
+        launch_glm_reduce_batched((const GlmReduceArgs*)at(o_gr[2]), (const int4*)ia(o_glmred),
                                   (int)it.glm_red.size(), s);
         if (b->refine) {
             RefineBatch rb{};
@@ -3120,7 +3135,7 @@ extern "C" int pmcmc_batch_solve(pmcmc_batch* b, pmcmc_deer_info* infos) {
         o_rall = ip.put(it.r_all.data(), it.r_all.size() * sizeof(int));
         o_rglm = ip.put(it.r_glm.data(), it.r_glm.size() * sizeof(int));
         o_rgauss = ip.put(it.r_gauss.data(), it.r_gauss.size() * sizeof(int));
-        o_glmred = ip.put(it.glm_red.data(), it.glm_red.size() * sizeof(int2));
+        o_glmred = ip.put(it.glm_red.data(), it.glm_red.size() * sizeof(int4));
         for (int d = 0; d < 17; ++d) o_gi[d] = ip.put(it.glm[d].data(), it.glm[d].size() * sizeof(int4));
         o_qs = ip.put(it.gslab.data(), it.gslab.size() * sizeof(int4));
         const int rc2 = upload(c, s, ip, &b->items, &b->items_cap);
