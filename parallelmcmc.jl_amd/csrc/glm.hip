@@ -834,10 +834,35 @@ __device__ __forceinline__ void glm_reduce_body(const ColSrc<T>& src, const P* p
         for (int64_t k = 0; k < D; ++k) a += Pm[(size_t)d * D + k] * src.get(set, t, k);
         return a;
     };
+    // The splits' partial sums of a set, added in split order.  Loaded in
+    // groups of 8 splits, all in flight together, with the first direction
+    // set's alongside the states' (a loop with a runtime trip count would
+    // wait for each load in turn); clamped indices, so no load is branched.
+    constexpr int SG = 8;
+    auto split_sums = [&](int64_t d, double* G0, double* G1) {
+        double a0 = 0.0, a1 = 0.0;
+        for (int s0 = 0; s0 < ns; s0 += SG) {
+            P v0[SG], v1[SG];
+#pragma unroll
+            for (int j = 0; j < SG; ++j) {
+                const size_t sj = (size_t)(s0 + j < ns ? s0 + j : ns - 1);
+                v0[j] = part[((sj * (1 + K) + 0) * Tn + t) * Dp + d];
+                if (K > 0) v1[j] = part[((sj * (1 + K) + 1) * Tn + t) * Dp + d];
+            }
+#pragma unroll
+            for (int j = 0; j < SG; ++j)
+                if (s0 + j < ns) {
+                    a0 += (double)v0[j];
+                    if (K > 0) a1 += (double)v1[j];
+                }
+        }
+        *G0 = a0;
+        *G1 = a1;
+    };
     for (int64_t d = lane; d < D; d += 64) {
         const double x = src.get(0, t, d);
-        double G = 0.0;
-        for (int s = 0; s < ns; ++s) G += (double)part[(((size_t)s * (1 + K) + 0) * Tn + t) * Dp + d];
+        double G, H1;
+        split_sums(d, &G, &H1);
         const double px = prior(0, d, x);
         const double g = G - px;
         const size_t gi = (size_t)t * D + d;
@@ -849,9 +874,12 @@ __device__ __forceinline__ void glm_reduce_body(const ColSrc<T>& src, const P* p
             Yout[gi] = (T)y;
         }
         for (int k = 0; k < K; ++k) {
-            double H = 0.0;
-            for (int s = 0; s < ns; ++s)
-                H += (double)part[(((size_t)s * (1 + K) + 1 + k) * Tn + t) * Dp + d];
+            double H = H1;
+            if (k > 0) {
+                H = 0.0;
+                for (int s = 0; s < ns; ++s)
+                    H += (double)part[(((size_t)s * (1 + K) + 1 + k) * Tn + t) * Dp + d];
+            }
             const double z = src.get(1 + k, t, d);
             const double h = -H - prior(1 + k, d, z);
             hout[(size_t)k * D * Tn + gi] = (T)h;
@@ -861,7 +889,14 @@ __device__ __forceinline__ void glm_reduce_body(const ColSrc<T>& src, const P* p
     for (int m = 32; m >= 1; m >>= 1) nrm += __shfl_xor(nrm, m, 64);
     if (lane == 0) {
         double ll = 0.0;
-        for (int s = 0; s < ns; ++s) ll += llpart[(size_t)s * llstride + t];
+        for (int s0 = 0; s0 < ns; s0 += SG) {
+            double v[SG];
+#pragma unroll
+            for (int j = 0; j < SG; ++j) v[j] = llpart[(size_t)(s0 + j < ns ? s0 + j : ns - 1) * llstride + t];
+#pragma unroll
+            for (int j = 0; j < SG; ++j)
+                if (s0 + j < ns) ll += v[j];
+        }
         lpout[t] = ll - 0.5 * nrm;  // nrm = x' P x (x' x / s^2 for the isotropic prior)
     }
 }
