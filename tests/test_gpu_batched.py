@@ -57,6 +57,43 @@ def test_batched_uneven_chains_match_sequential(prec):
     _check(pairs, plan.solve())
 
 
+def _wide_logistic_chains(seed):
+    """Four logistic chains at N=2000 with 4096-column windows: once their
+    windows have widened to full width they hold 512 live GEMM column blocks,
+    and the GEMM passes take the split count sized for 512 workgroups per
+    chain (4 splits instead of the plan's 6)."""
+    from ParallelMCMC import DensityModel, LogisticTarget
+    from ParallelMCMC.Batched import Chain
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for D, T in [(6, 4200), (12, 4100), (20, 4500), (30, 4096)]:
+        Xd, y, _ = O.make_logreg_data(rng, 2000, D)
+        Xd = Xd.astype(np.float32).astype(np.float64)
+        tgt = O.LogReg(Xd, y, 1.0)
+        ev = np.linalg.eigvalsh(-np.stack([tgt.hvp(np.zeros(D), e) for e in np.eye(D)], axis=1))
+        eps = 0.15 / ev.max()
+        x0 = (0.2 * rng.standard_normal(D)).astype(np.float32)
+        Xi, U = O.make_tape(rng, D, T, np.float32)
+        out.append((tgt, Chain(DensityModel(dim=D, target=LogisticTarget(Xd, y, 1.0)), eps, x0, Xi, U)))
+    return out
+
+
+def test_batched_many_gemm_blocks_match_sequential():
+    from ParallelMCMC.Batched import BatchPlan
+
+    pairs = _wide_logistic_chains(11)
+    chains = [c for _, c in pairs]
+    a = BatchPlan(chains, window=4096, precision="bf16_jac", tol_abs=1e-6, tol_rel=1e-5, maxiter=600).solve()
+    _check(pairs, a)
+    b = BatchPlan(chains, window=4096, precision="bf16_jac", tol_abs=1e-6, tol_rel=1e-5, maxiter=600,
+                  use_graph=False).solve()
+    for (Sa, acca, ia), (Sb, accb, ib) in zip(a, b):
+        assert ia.iters == ib.iters
+        assert np.array_equal(Sa.cpu().numpy(), Sb.cpu().numpy())
+        assert np.array_equal(acca.cpu().numpy(), accb.cpu().numpy())
+
+
 def test_batched_graph_and_direct_launches_identical():
     from ParallelMCMC.Batched import BatchPlan
 
