@@ -691,7 +691,14 @@ __device__ __forceinline__ void tile_body(const TileArgs<T>& a, const int tile) 
         // the staging loads and written after them
         T s0v = (T)0;
         if (c0 == 0) s0v = gp(a.s0)[tid < D ? tid : 0];
-        if (build) {
+        if (build && MODE == MODE_GEMM) {
+            // GEMM targets read their proposals Y from the pass: the tape's
+            // noise is not needed (bl is written before it is read)
+            T* const dst[1] = {xd};
+            const T* const src[1] = {xsrc};
+            const int64_t n[1] = {xn};
+            stage_lds_multi<T, 1>(dst, src, n, tid);
+        } else if (build) {
             T* const dst[2] = {xd, bl};
             const T* const src[2] = {xsrc, a.Xi + gbase};
             const int64_t n[2] = {xn, nel};
